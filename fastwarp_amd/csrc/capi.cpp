@@ -1,0 +1,357 @@
+// extern "C" entry points of include/fastwarp_hip.h over HipBackend. Each function catches every
+// exception and turns it into a status code plus the context's last-error text, as the trait
+// methods return Result<_, RegError> (crates/fastwarp-reg/src/error.rs).
+#include "../../include/fastwarp_hip.h"
+
+#include <cstring>
+#include <string>
+
+#include "hip/backend.hpp"
+
+using fw::Grid;
+using fw::hip::HField;
+using fw::hip::HImage;
+using fw::hip::HipBackend;
+
+struct fw_ctx {
+  HipBackend backend;
+  std::string error;
+  explicit fw_ctx(int device) : backend(device) {}
+};
+struct fw_image {
+  HImage h;
+};
+struct fw_field {
+  HField h;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+Grid grid_from(const fw_grid* g) {
+  if (!g) fw::invalid("grid", "null grid");
+  for (int a = 0; a < 3; ++a)
+    if (g->size[a] <= 0) fw::invalid("grid", "zero size");
+  fw::Mat3 dir;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) dir[r][c] = g->direction[3 * r + c];
+  return Grid({size_t(g->size[0]), size_t(g->size[1]), size_t(g->size[2])}, {g->spacing[0], g->spacing[1], g->spacing[2]},
+              {g->origin[0], g->origin[1], g->origin[2]}, dir);
+}
+
+void grid_to(const Grid& g, fw_grid* out) {
+  for (int a = 0; a < 3; ++a) {
+    out->size[a] = int64_t(g.size[a]);
+    out->spacing[a] = g.spacing[a];
+    out->origin[a] = g.origin[a];
+    for (int c = 0; c < 3; ++c) out->direction[3 * a + c] = g.direction[a][c];
+  }
+}
+
+fw::AffineTransform affine_from(const fw_affine* a) {
+  fw::AffineTransform t;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) t.matrix[r][c] = a->matrix[3 * r + c];
+    t.translation[r] = a->translation[r];
+    t.center[r] = a->center[r];
+  }
+  return t;
+}
+
+template <typename F>
+int guard(fw_ctx* ctx, F&& f) {
+  if (!ctx) return FW_ERR_INVALID;
+  try {
+    f();
+    ctx->error.clear();
+    return FW_OK;
+  } catch (const fw::Cancelled& e) {
+    ctx->error = e.what();
+    return FW_ERR_CANCELLED;
+  } catch (const fw::Error& e) {
+    ctx->error = e.what();
+    return e.what_ == "hip" ? FW_ERR_DEVICE : FW_ERR_INVALID;
+  } catch (const std::exception& e) {
+    ctx->error = e.what();
+    return FW_ERR_INVALID;
+  }
+}
+
+void need(const void* p, const char* what) {
+  if (!p) fw::invalid("fastwarp", std::string(what) + " is null");
+}
+
+}  // namespace
+
+extern "C" {
+
+int fw_ctx_create(int device, fw_ctx** out) {
+  if (!out) return FW_ERR_INVALID;
+  try {
+    *out = new fw_ctx(device);
+    return FW_OK;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    *out = nullptr;
+    return FW_ERR_DEVICE;
+  }
+}
+
+void fw_ctx_destroy(fw_ctx* ctx) { delete ctx; }
+
+const char* fw_last_error(const fw_ctx* ctx) { return ctx ? ctx->error.c_str() : g_create_error.c_str(); }
+
+int fw_release_idle(fw_ctx* ctx) {
+  return guard(ctx, [&] { ctx->backend.release_idle_buffers(); });
+}
+
+int fw_memory_stats(fw_ctx* ctx, uint64_t* held, uint64_t* peak) {
+  return guard(ctx, [&] {
+    if (held) *held = ctx->backend.ctx().held();
+    if (peak) *peak = ctx->backend.ctx().peak();
+  });
+}
+
+int fw_image_upload(fw_ctx* ctx, const fw_grid* grid, const double* data, fw_image** out) {
+  return guard(ctx, [&] {
+    need(data, "image data");
+    need(out, "output");
+    const Grid g = grid_from(grid);
+    fw::Image v(g, std::vector<double>(data, data + g.num_voxels()));
+    *out = new fw_image{ctx->backend.upload_image(v)};
+  });
+}
+
+int fw_image_upload_f32(fw_ctx* ctx, const fw_grid* grid, const float* data, fw_image** out) {
+  return guard(ctx, [&] {
+    need(data, "image data");
+    need(out, "output");
+    *out = new fw_image{ctx->backend.upload_image_f32(grid_from(grid), data)};
+  });
+}
+
+int fw_image_download(fw_ctx* ctx, const fw_image* image, double* out) {
+  return guard(ctx, [&] {
+    need(image, "image");
+    need(out, "output");
+    fw::Image v = ctx->backend.download_image(image->h);
+    std::memcpy(out, v.data.data(), v.data.size() * sizeof(double));
+  });
+}
+
+int fw_image_grid(const fw_image* image, fw_grid* out) {
+  if (!image || !out) return FW_ERR_INVALID;
+  grid_to(image->h.grid, out);
+  return FW_OK;
+}
+
+void fw_image_free(fw_ctx*, fw_image* image) { delete image; }
+
+int fw_field_zero(fw_ctx* ctx, const fw_grid* grid, fw_field** out) {
+  return guard(ctx, [&] {
+    need(out, "output");
+    *out = new fw_field{ctx->backend.zero_field(grid_from(grid))};
+  });
+}
+
+int fw_field_upload(fw_ctx* ctx, const fw_grid* grid, const double* aos, fw_field** out) {
+  return guard(ctx, [&] {
+    need(aos, "field data");
+    need(out, "output");
+    const Grid g = grid_from(grid);
+    std::vector<fw::Vec3> d(g.num_voxels());
+    for (size_t i = 0; i < d.size(); ++i) d[i] = {aos[3 * i], aos[3 * i + 1], aos[3 * i + 2]};
+    *out = new fw_field{ctx->backend.upload_field(fw::VectorField(g, std::move(d)))};
+  });
+}
+
+int fw_field_download(fw_ctx* ctx, const fw_field* field, double* aos) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    need(aos, "output");
+    fw::VectorField v = ctx->backend.download_field(field->h);
+    for (size_t i = 0; i < v.data.size(); ++i)
+      for (int c = 0; c < 3; ++c) aos[3 * i + c] = v.data[i][c];
+  });
+}
+
+int fw_field_grid(const fw_field* field, fw_grid* out) {
+  if (!field || !out) return FW_ERR_INVALID;
+  grid_to(field->h.grid, out);
+  return FW_OK;
+}
+
+int fw_field_copy(fw_ctx* ctx, const fw_field* field, fw_field** out) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    need(out, "output");
+    *out = new fw_field{ctx->backend.copy_field(field->h)};
+  });
+}
+
+void fw_field_free(fw_ctx*, fw_field* field) { delete field; }
+
+int fw_warp(fw_ctx* ctx, const fw_image* src, const fw_field* field, const fw_field* pre, const fw_affine* affine,
+            fw_image** out) {
+  return guard(ctx, [&] {
+    need(src, "source");
+    need(field, "field");
+    need(out, "output");
+    fw::AffineTransform a;
+    if (affine) a = affine_from(affine);
+    *out = new fw_image{ctx->backend.warp_through_field(src->h, field->h, pre ? &pre->h : nullptr, affine ? &a : nullptr)};
+  });
+}
+
+int fw_warp_mask(fw_ctx* ctx, const fw_image* mask, const fw_field* field, const fw_field* pre, const fw_affine* affine,
+                 fw_image* product, fw_image** out) {
+  return guard(ctx, [&] {
+    need(mask, "mask");
+    need(field, "field");
+    need(out, "output");
+    fw::AffineTransform a;
+    if (affine) a = affine_from(affine);
+    HImage* prod = product ? &product->h : nullptr;
+    fw_image* res = new fw_image{ctx->backend.warp_mask_through_field(mask->h, field->h, pre ? &pre->h : nullptr,
+                                                                      affine ? &a : nullptr, prod)};
+    delete product;
+    *out = res;
+  });
+}
+
+int fw_cc_pair(fw_ctx* ctx, const fw_image* fixed, const fw_image* moving, const fw_image* valid, uint32_t radius,
+               double values[2], double norms[2], fw_field** d_fixed, fw_field** d_moving) {
+  return guard(ctx, [&] {
+    need(fixed, "fixed");
+    need(moving, "moving");
+    need(values, "values");
+    need(d_fixed, "output");
+    need(d_moving, "output");
+    auto r = ctx->backend.cc_pair(fixed->h, moving->h, valid ? &valid->h : nullptr, radius, norms != nullptr);
+    values[0] = r.values[0];
+    values[1] = r.values[1];
+    if (norms) {
+      norms[0] = r.norms[0];
+      norms[1] = r.norms[1];
+    }
+    *d_fixed = new fw_field{std::move(r.fields[0])};
+    *d_moving = new fw_field{std::move(r.fields[1])};
+  });
+}
+
+int fw_smooth_field(fw_ctx* ctx, fw_field* field, double variance, fw_field** out) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    need(out, "output");
+    HField f = std::move(field->h);
+    delete field;
+    *out = new fw_field{ctx->backend.smooth_field(std::move(f), variance)};
+  });
+}
+
+int fw_max_scaled_norm_pair(fw_ctx* ctx, const fw_field* a, const fw_field* b, double out[2]) {
+  return guard(ctx, [&] {
+    need(a, "field");
+    need(out, "output");
+    auto r = ctx->backend.max_scaled_norm_pair(a->h, b ? &b->h : nullptr);
+    out[0] = r.first;
+    if (b) out[1] = r.second;
+  });
+}
+
+int fw_l2_norm(fw_ctx* ctx, const fw_field* field, double* out) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    need(out, "output");
+    *out = ctx->backend.l2_norm(field->h);
+  });
+}
+
+int fw_scale(fw_ctx* ctx, fw_field* field, double s) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    ctx->backend.scale(field->h, s);
+  });
+}
+
+int fw_axpy(fw_ctx* ctx, fw_field* acc, double a, const fw_field* x) {
+  return guard(ctx, [&] {
+    need(acc, "accumulator");
+    need(x, "field");
+    ctx->backend.axpy(acc->h, a, x->h);
+  });
+}
+
+int fw_compose(fw_ctx* ctx, const fw_field* displacement, const fw_field* warping, fw_field** out) {
+  return guard(ctx, [&] {
+    need(displacement, "displacement");
+    need(warping, "warping");
+    need(out, "output");
+    *out = new fw_field{ctx->backend.compose(displacement->h, warping->h)};
+  });
+}
+
+static fw::hip::InvertParams params_from(const fw_invert_params* p) {
+  fw::hip::InvertParams q;
+  if (p) {
+    q.max_iterations = p->max_iterations;
+    q.mean_error_tolerance = p->mean_error_tolerance;
+    q.max_error_tolerance = p->max_error_tolerance;
+    q.enforce_boundary_condition = p->enforce_boundary_condition != 0;
+  }
+  return q;
+}
+
+static void stats_to(const fw::hip::InvertStats& s, fw_invert_stats* out) {
+  if (!out) return;
+  out->iterations = s.iterations;
+  out->mean_error_norm = s.mean_error_norm;
+  out->max_error_norm = s.max_error_norm;
+}
+
+int fw_invert(fw_ctx* ctx, const fw_field* field, fw_field* initial, const fw_invert_params* params, fw_field** out,
+              fw_invert_stats* stats) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    need(initial, "initial");
+    need(out, "output");
+    HField init = std::move(initial->h);
+    delete initial;
+    auto r = ctx->backend.invert(field->h, std::move(init), params_from(params));
+    stats_to(r.second, stats);
+    *out = new fw_field{std::move(r.first)};
+  });
+}
+
+int fw_invert_pair(fw_ctx* ctx, fw_field* field, fw_field* inverse, const fw_invert_params* params,
+                   fw_field** new_inverse, fw_invert_stats* inverse_stats, fw_field** new_field,
+                   fw_invert_stats* field_stats) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    need(inverse, "inverse");
+    need(new_inverse, "output");
+    need(new_field, "output");
+    HField f = std::move(field->h);
+    HField inv = std::move(inverse->h);
+    delete field;
+    delete inverse;
+    const auto p = params_from(params);
+    auto a = ctx->backend.invert(f, std::move(inv), p);
+    auto b = ctx->backend.invert(a.first, std::move(f), p);
+    stats_to(a.second, inverse_stats);
+    stats_to(b.second, field_stats);
+    *new_inverse = new fw_field{std::move(a.first)};
+    *new_field = new fw_field{std::move(b.first)};
+  });
+}
+
+int fw_resample_field(fw_ctx* ctx, const fw_field* field, const fw_grid* grid, fw_field** out) {
+  return guard(ctx, [&] {
+    need(field, "field");
+    need(out, "output");
+    *out = new fw_field{ctx->backend.resample_field(field->h, grid_from(grid))};
+  });
+}
+
+}  // extern "C"

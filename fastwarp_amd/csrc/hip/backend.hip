@@ -1,0 +1,509 @@
+// HipBackend implementation: buffer pool, host<->device conversions and the mapping of every
+// trait op (crates/fastwarp-reg/src/backend.rs) onto the kernels of kernels.hip. Geometry is
+// composed in f64 on the host into continuous-index maps and narrowed to f32 once per launch.
+#include "backend.hpp"
+
+#include <cmath>
+
+#include "../host/gauss_taps.hpp"
+
+namespace fw::hip {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error("hip", std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DeviceContext::DeviceContext(int device) : device_(device) {
+  check(hipSetDevice(device), "hipSetDevice");
+  check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+DeviceContext::~DeviceContext() {
+  (void)hipSetDevice(device_);
+  (void)hipStreamSynchronize(stream_);
+  release_idle();
+  for (auto& s : scratch_)
+    if (s.p) hipFree(s.p);
+  if (pinned_) hipHostFree(pinned_);
+  for (auto& kv : taps_) hipFree(kv.second.first);
+  (void)hipStreamDestroy(stream_);
+}
+
+static size_t round_bytes(size_t b) { return (b + 255) & ~size_t(255); }
+
+void* DeviceContext::acquire(size_t bytes) {
+  bytes = round_bytes(std::max<size_t>(bytes, 256));
+  auto it = free_.find(bytes);
+  if (it != free_.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    return p;
+  }
+  (void)hipSetDevice(device_);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    hipStreamSynchronize(stream_);
+    release_idle();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error("hip", "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+  }
+  held_ += bytes;
+  peak_ = std::max(peak_, held_);
+  return p;
+}
+
+void DeviceContext::release(void* p, size_t bytes) {
+  if (!p) return;
+  free_[round_bytes(std::max<size_t>(bytes, 256))].push_back(p);
+}
+
+void DeviceContext::release_idle() {
+  (void)hipStreamSynchronize(stream_);
+  for (auto& kv : free_)
+    for (void* p : kv.second) {
+      hipFree(p);
+      held_ -= kv.first;
+    }
+  free_.clear();
+}
+
+void* DeviceContext::scratch(int slot, size_t bytes) {
+  Scratch& s = scratch_[slot];
+  if (s.bytes < bytes) {
+    if (s.p) {
+      hipStreamSynchronize(stream_);
+      hipFree(s.p);
+    }
+    s.bytes = round_bytes(bytes * 2);
+    check(hipMalloc(&s.p, s.bytes), "hipMalloc scratch");
+  }
+  return s.p;
+}
+
+void* DeviceContext::pinned(size_t bytes) {
+  if (pinned_bytes_ < bytes) {
+    if (pinned_) hipHostFree(pinned_);
+    pinned_bytes_ = round_bytes(bytes * 2);
+    check(hipHostMalloc(&pinned_, pinned_bytes_, hipHostMallocDefault), "hipHostMalloc");
+  }
+  return pinned_;
+}
+
+void DeviceContext::sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+const float* DeviceContext::taps(double variance, size_t length, int& radius) {
+  auto key = std::make_pair(variance, length);
+  auto it = taps_.find(key);
+  if (it == taps_.end()) {
+    const std::vector<double> t = gaussian_operator_taps(variance, 0.001, length);
+    std::vector<float> f(t.begin(), t.end());
+    void* p = nullptr;
+    check(hipMalloc(&p, f.size() * sizeof(float)), "hipMalloc taps");
+    check(hipMemcpy(p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy taps");
+    it = taps_.emplace(key, std::make_pair(p, int(t.size() / 2))).first;
+  }
+  radius = it->second.second;
+  return static_cast<const float*>(it->second.first);
+}
+
+DevBuf::DevBuf(DeviceContext* ctx, size_t floats) : ctx_(ctx), bytes_(floats * sizeof(float)) {
+  p_ = static_cast<float*>(ctx->acquire(bytes_));
+}
+
+void DevBuf::reset() {
+  if (p_ && ctx_) ctx_->release(p_, bytes_);
+  p_ = nullptr;
+}
+
+// ---------------------------------------------------------------------------- geometry ---
+
+namespace {
+
+Dims dims_of(const Grid& g) { return {int(g.size[0]), int(g.size[1]), int(g.size[2])}; }
+
+Mat3 scaled_direction(const Grid& g) {
+  Mat3 m = g.direction;
+  for (auto& row : m)
+    for (int c = 0; c < 3; ++c) row[c] *= g.spacing[c];
+  return m;
+}
+
+void put(float* dst, const Mat3& m) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) dst[r * 3 + c] = float(m[r][c]);
+}
+
+void put(float* dst, const Vec3& v) {
+  for (int r = 0; r < 3; ++r) dst[r] = float(v[r]);
+}
+
+float3 inv_spacing(const Grid& g) {
+  return make_float3(float(1.0 / g.spacing[0]), float(1.0 / g.spacing[1]), float(1.0 / g.spacing[2]));
+}
+
+// The central-difference probe offsets of a grid in index units: +-half a spacing along each
+// physical axis d maps to index_from_offset * (h_d e_d) (interp.rs:102-121).
+Probes probes_of(const Grid& g) {
+  Probes p{};
+  bool diagonal = true;
+  for (int d = 0; d < 3; ++d) {
+    const double h = 0.5 * g.spacing[d];
+    int nonzero = 0;
+    for (int r = 0; r < 3; ++r) {
+      double v = g.index_from_offset[r][d] * h;
+      if (std::fabs(v) < 1e-12) v = 0.0;
+      if (v != 0.0) {
+        ++nonzero;
+        if (std::fabs(std::fabs(v) - 0.5) < 1e-9) v = v > 0 ? 0.5 : -0.5;
+        else diagonal = false;
+      }
+      p.o[d][r] = float(v);
+    }
+    if (nonzero != 1) diagonal = false;
+    p.inv_spacing[d] = float(1.0 / g.spacing[d]);
+  }
+  p.diagonal = diagonal ? 1 : 0;
+  return p;
+}
+
+float fixed_point_scale(float absmax) {
+  if (!(absmax > 0.0f) || !std::isfinite(absmax)) return float(1 << 22);
+  int e = 0;
+  std::frexp(double(absmax), &e);  // absmax < 2^e
+  return float(std::ldexp(1.0, 22 - e));
+}
+
+}  // namespace
+
+HipBackend::HipBackend(int device) : ctx_(std::make_shared<DeviceContext>(device)) {}
+
+// --------------------------------------------------------------------- transfers (a1, a2) ---
+
+HImage HipBackend::upload_image(const Image& v) {
+  const size_t n = v.grid.num_voxels();
+  std::vector<float> f(n);
+  parallel_for(n, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) f[i] = float(v.data[i]);
+  });
+  return upload_image_f32(v.grid, f.data());
+}
+
+HImage HipBackend::upload_image_f32(const Grid& g, const float* data) {
+  const size_t n = g.num_voxels();
+  float m = 0.0f;
+  for (size_t i = 0; i < n; ++i)
+    if (std::isfinite(data[i])) m = std::max(m, std::fabs(data[i]));
+  HImage im{g, DevBuf(ctx_.get(), n), m};
+  check(hipMemcpyAsync(im.buf.get(), data, n * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()), "upload image");
+  ctx_->sync();
+  return im;
+}
+
+Image HipBackend::download_image(const HImage& im) {
+  const size_t n = im.grid.num_voxels();
+  std::vector<float> f(n);
+  check(hipMemcpyAsync(f.data(), im.buf.get(), n * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "download image");
+  ctx_->sync();
+  return Image(im.grid, std::vector<double>(f.begin(), f.end()));
+}
+
+HField HipBackend::zero_field(const Grid& g) {
+  HField f{g, DevBuf(ctx_.get(), 3 * g.num_voxels())};
+  check(hipMemsetAsync(f.buf.get(), 0, f.buf.bytes(), ctx_->stream()), "zero field");
+  return f;
+}
+
+HField HipBackend::upload_field(const VectorField& v) {
+  const size_t n = v.grid.num_voxels();
+  std::vector<float> soa(3 * n);
+  parallel_for(n, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i)
+      for (int c = 0; c < 3; ++c) soa[c * n + i] = float(v.data[i][c]);
+  });
+  HField f{v.grid, DevBuf(ctx_.get(), 3 * n)};
+  check(hipMemcpyAsync(f.buf.get(), soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()),
+        "upload field");
+  ctx_->sync();
+  return f;
+}
+
+VectorField HipBackend::download_field(const HField& f) {
+  const size_t n = f.grid.num_voxels();
+  std::vector<float> soa(3 * n);
+  check(hipMemcpyAsync(soa.data(), f.buf.get(), soa.size() * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()),
+        "download field");
+  ctx_->sync();
+  std::vector<Vec3> aos(n);
+  parallel_for(n, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) aos[i] = {double(soa[i]), double(soa[n + i]), double(soa[2 * n + i])};
+  });
+  return VectorField(f.grid, std::move(aos));
+}
+
+HField HipBackend::copy_field(const HField& f) {
+  HField out{f.grid, DevBuf(ctx_.get(), 3 * f.grid.num_voxels())};
+  check(hipMemcpyAsync(out.buf.get(), f.buf.get(), f.buf.bytes(), hipMemcpyDeviceToDevice, ctx_->stream()), "copy field");
+  return out;
+}
+
+// ------------------------------------------------------------------------ warps (a3, a4) ---
+
+static WarpArgs warp_args(const Grid& src, const HField& field, const HField* pre, const AffineTransform* affine) {
+  WarpArgs a{};
+  const Grid& og = field.grid;
+  const Mat3 M = affine ? affine->matrix : kIdentity3;
+  const Vec3 off = affine ? affine->offset() : Vec3{0, 0, 0};
+  const Mat3 L = mat3_mul(src.index_from_offset, M);
+  // c = IFO_s (M (x + u + d) + off - o_s), x = o_f + D_f S_f idx
+  a.to_src = {};
+  put(a.to_src.m, mat3_mul(L, scaled_direction(og)));
+  put(a.to_src.c0, mat3_mul_vec(src.index_from_offset, vsub(vadd(mat3_mul_vec(M, og.origin), off), src.origin)));
+  put(a.to_src.l, L);
+  put(a.pre_l, L);
+  a.sd = dims_of(src);
+  a.od = dims_of(og);
+  a.field = field.buf.get();
+  if (pre) {
+    const Grid& pg = pre->grid;
+    put(a.to_pre.m, mat3_mul(pg.index_from_offset, scaled_direction(og)));
+    put(a.to_pre.c0, mat3_mul_vec(pg.index_from_offset, vsub(og.origin, pg.origin)));
+    put(a.to_pre.l, pg.index_from_offset);
+    a.pre = pre->buf.get();
+    a.pd = dims_of(pg);
+  }
+  return a;
+}
+
+HImage HipBackend::warp_through_field(const HImage& src, const HField& field, const HField* pre, const AffineTransform* affine) {
+  WarpArgs a = warp_args(src.grid, field, pre, affine);
+  HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), src.absmax};
+  a.src = src.buf.get();
+  a.out = out.buf.get();
+  launch_warp(a, ctx_->stream());
+  return out;
+}
+
+HImage HipBackend::warp_mask_through_field(const HImage& mask, const HField& field, const HField* pre,
+                                           const AffineTransform* affine, HImage* product) {
+  if (product && product->grid != field.grid) invalid("warp mask", "the product image is not on the grid of the field");
+  WarpArgs a = warp_args(mask.grid, field, pre, affine);
+  a.src = mask.buf.get();
+  a.nearest = 1;
+  if (product) {
+    HImage out = std::move(*product);
+    a.product = out.buf.get();
+    a.out = out.buf.get();
+    out.absmax = 1.0f;
+    launch_warp(a, ctx_->stream());
+    return out;
+  }
+  HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), 1.0f};
+  a.out = out.buf.get();
+  launch_warp(a, ctx_->stream());
+  return out;
+}
+
+// ----------------------------------------------------------------------------- CC (a5, a6) ---
+
+MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving, const HImage* valid, size_t radius,
+                                       bool want_norms) {
+  if (fixed.grid != moving.grid) invalid("hip backend", "cc: the images are not on one grid");
+  if (valid && valid->grid != fixed.grid) invalid("hip backend", "cc mask: the images are not on one grid");
+  if (radius == 0 || radius > 8) invalid("hip backend", "cc radius " + std::to_string(radius) + " outside 1..=8");
+  const Grid& g = fixed.grid;
+  const size_t n = g.num_voxels();
+  MetricPair<HField> out;
+  out.fields[0] = HField{g, DevBuf(ctx_.get(), 3 * n)};
+  out.fields[1] = HField{g, DevBuf(ctx_.get(), 3 * n)};
+  CcArgs a{};
+  a.fixed = fixed.buf.get();
+  a.moving = moving.buf.get();
+  a.valid = valid ? valid->buf.get() : nullptr;
+  a.d = dims_of(g);
+  a.radius = int(radius);
+  a.tile_z = std::max(1, cc_tile_z);
+  a.qf = fixed_point_scale(fixed.absmax);
+  a.qm = fixed_point_scale(moving.absmax);
+  a.inv_qf = 1.0f / a.qf;
+  a.inv_qm = 1.0f / a.qm;
+  a.probes = probes_of(g);
+  a.d_fixed = out.fields[0].buf.get();
+  a.d_moving = out.fields[1].buf.get();
+  const dim3 gr = cc_grid(a.d, a.tile_z);
+  const size_t blocks = size_t(gr.x) * gr.y * gr.z;
+  a.partial_cc = static_cast<float*>(ctx_->scratch(0, blocks * sizeof(float)));
+  a.partial_count = static_cast<int*>(ctx_->scratch(1, blocks * sizeof(int)));
+  launch_cc_pair(a, ctx_->stream());
+  // the launched grid may be smaller than `blocks` (wider x tiles): count the real one
+  const int tx = radius <= 5 ? 64 : 32;
+  const size_t real = size_t((a.d.nx + tx - 1) / tx) * ((a.d.ny + 3) / 4) * ((a.d.nz + a.tile_z - 1) / a.tile_z);
+  int nb = 0;
+  float* pm = nullptr;
+  float* ps = nullptr;
+  if (want_norms) {
+    nb = norm_blocks(long(n));
+    pm = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
+    ps = static_cast<float*>(ctx_->scratch(3, 2 * nb * sizeof(float)));
+    launch_field_norms(out.fields[0].buf.get(), out.fields[1].buf.get(), long(n), inv_spacing(g), pm, ps, nb, ctx_->stream());
+  }
+  const size_t bytes = real * (sizeof(float) + sizeof(int)) + (want_norms ? 2 * nb * sizeof(float) : 0);
+  char* host = static_cast<char*>(ctx_->pinned(bytes));
+  check(hipMemcpyAsync(host, a.partial_cc, real * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "cc partials");
+  check(hipMemcpyAsync(host + real * sizeof(float), a.partial_count, real * sizeof(int), hipMemcpyDeviceToHost, ctx_->stream()),
+        "cc counts");
+  if (want_norms)
+    check(hipMemcpyAsync(host + real * (sizeof(float) + sizeof(int)), ps, 2 * nb * sizeof(float), hipMemcpyDeviceToHost,
+                         ctx_->stream()),
+          "cc norms");
+  ctx_->sync();
+  const float* pcc = reinterpret_cast<const float*>(host);
+  const int* pcount = reinterpret_cast<const int*>(host + real * sizeof(float));
+  double total = 0.0;
+  long long count = 0;
+  for (size_t b = 0; b < real; ++b) {
+    total += double(pcc[b]);
+    count += pcount[b];
+  }
+  if (count == 0)
+    invalid("cc metric", valid ? "no voxel of the registration level lies inside both the fixed and the moving mask"
+                               : "no virtual voxel maps inside both image buffers");
+  out.values[0] = out.values[1] = -total / double(count);
+  if (want_norms) {
+    const float* sq = reinterpret_cast<const float*>(host + real * (sizeof(float) + sizeof(int)));
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      s0 += double(sq[b]);
+      s1 += double(sq[nb + b]);
+    }
+    out.norms[0] = std::sqrt(s0);
+    out.norms[1] = std::sqrt(s1);
+  }
+  return out;
+}
+
+// ------------------------------------------------------------------------- field ops (a8-a13) ---
+
+HField HipBackend::smooth_field(HField field, double variance) {
+  if (variance <= 0.0) return field;
+  const Grid& g = field.grid;
+  const Dims d = dims_of(g);
+  const size_t n = g.num_voxels();
+  DevBuf t1(ctx_.get(), 3 * n), t2(ctx_.get(), 3 * n);
+  const double w1 = variance < 0.5 ? 1.0 - variance / 0.5 : 1.0;
+  const double w2 = 1.0 - w1;
+  int r = 0;
+  const float* tx = ctx_->taps(variance, g.size[0], r);
+  launch_gauss_axis(field.buf.get(), t1.get(), d, 0, tx, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
+  const float* ty = ctx_->taps(variance, g.size[1], r);
+  launch_gauss_axis(t1.get(), t2.get(), d, 1, ty, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
+  const float* tz = ctx_->taps(variance, g.size[2], r);
+  launch_gauss_axis(t2.get(), t1.get(), d, 2, tz, r, field.buf.get(), float(w1), float(w2), 1, ctx_->stream());
+  return HField{g, std::move(t1)};
+}
+
+std::pair<double, double> HipBackend::max_scaled_norm_pair(const HField& a, const HField* b) {
+  if (b && b->grid.num_voxels() != a.grid.num_voxels()) invalid("hip backend", "norm pair: grids differ");
+  const long long n = long(a.grid.num_voxels());
+  const int nb = norm_blocks(n);
+  float* pm = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
+  float* ps = static_cast<float*>(ctx_->scratch(3, 2 * nb * sizeof(float)));
+  double result[2] = {0.0, 0.0};
+  // the two fields may live on different grids only in spacing-free size; run each on its own
+  const HField* fields[2] = {&a, b};
+  for (int q = 0; q < (b ? 2 : 1); ++q) {
+    launch_field_norms(fields[q]->buf.get(), nullptr, n, inv_spacing(fields[q]->grid), pm, ps, nb, ctx_->stream());
+    float* host = static_cast<float*>(ctx_->pinned(nb * sizeof(float)));
+    check(hipMemcpyAsync(host, pm, nb * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "norm partials");
+    ctx_->sync();
+    double m = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const double v = host[i];
+      if (std::isnan(v) || std::isnan(m)) m = NAN;
+      else m = std::max(m, v);
+    }
+    result[q] = m;
+  }
+  return {result[0], result[1]};
+}
+
+double HipBackend::l2_norm(const HField& f) {
+  const long long n = long(f.grid.num_voxels());
+  const int nb = norm_blocks(n);
+  float* pm = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
+  float* ps = static_cast<float*>(ctx_->scratch(3, 2 * nb * sizeof(float)));
+  launch_field_norms(f.buf.get(), nullptr, n, inv_spacing(f.grid), pm, ps, nb, ctx_->stream());
+  float* host = static_cast<float*>(ctx_->pinned(nb * sizeof(float)));
+  check(hipMemcpyAsync(host, ps, nb * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "l2 partials");
+  ctx_->sync();
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) s += double(host[i]);
+  return std::sqrt(s);
+}
+
+void HipBackend::scale(HField& f, double s) { launch_scale(f.buf.get(), long(3 * f.grid.num_voxels()), float(s), ctx_->stream()); }
+
+void HipBackend::axpy(HField& acc, double a, const HField& x) {
+  if (acc.grid != x.grid) invalid("field axpy", "grids differ");
+  launch_axpy(acc.buf.get(), x.buf.get(), long(3 * acc.grid.num_voxels()), float(a), ctx_->stream());
+}
+
+HField HipBackend::compose(const HField& displacement, const HField& warping) {
+  if (displacement.grid != warping.grid) invalid("compose", "displacement and warping grids differ");
+  const Grid& g = warping.grid;
+  HField out{g, DevBuf(ctx_.get(), 3 * g.num_voxels())};
+  float l[9];
+  put(l, g.index_from_offset);
+  launch_compose(displacement.buf.get(), warping.buf.get(), dims_of(g), l, out.buf.get(), ctx_->stream());
+  return out;
+}
+
+std::pair<HField, InvertStats> HipBackend::invert(const HField& field, HField initial, const InvertParams& p) {
+  if (initial.grid != field.grid) invalid("invert", "initial estimate grid differs");
+  const Grid& g = field.grid;
+  const Dims d = dims_of(g);
+  float l[9];
+  put(l, g.index_from_offset);
+  const float3 isp = inv_spacing(g);
+  const int nb = inv_blocks(d);
+  auto* state = static_cast<InvertState*>(ctx_->scratch(4, sizeof(InvertState)));
+  auto* psum = static_cast<float*>(ctx_->scratch(5, nb * sizeof(float)));
+  auto* pmax = static_cast<float*>(ctx_->scratch(6, nb * sizeof(float)));
+  hipStream_t s = ctx_->stream();
+  launch_inv_init(state, int(p.max_iterations), s);
+  for (uint32_t step = 0; step < p.max_iterations; ++step) {
+    launch_inv_residual(field.buf.get(), initial.buf.get(), d, l, isp, state, psum, pmax, nb, s);
+    launch_inv_reduce(psum, pmax, nb, d.n(), state, int(step), int(p.max_iterations), p.max_error_tolerance,
+                      p.mean_error_tolerance, s);
+    launch_inv_update(field.buf.get(), initial.buf.get(), d, l, isp, state, int(step), p.enforce_boundary_condition ? 1 : 0, s);
+  }
+  auto* host = static_cast<InvertState*>(ctx_->pinned(sizeof(InvertState)));
+  check(hipMemcpyAsync(host, state, sizeof(InvertState), hipMemcpyDeviceToHost, s), "invert state");
+  ctx_->sync();
+  InvertStats st;
+  st.iterations = uint32_t(host->executed);
+  st.mean_error_norm = host->executed ? double(host->mean_error) : 0.0;
+  st.max_error_norm = host->executed ? double(host->max_error) : 0.0;
+  return {std::move(initial), st};
+}
+
+HField HipBackend::resample_field(const HField& field, const Grid& grid) {
+  if (field.grid == grid) return copy_field(field);
+  const Grid& sg = field.grid;
+  ResampleArgs a{};
+  a.src = field.buf.get();
+  a.sd = dims_of(sg);
+  a.od = dims_of(grid);
+  put(a.to_src.m, mat3_mul(sg.index_from_offset, scaled_direction(grid)));
+  put(a.to_src.c0, mat3_mul_vec(sg.index_from_offset, vsub(grid.origin, sg.origin)));
+  HField out{grid, DevBuf(ctx_.get(), 3 * grid.num_voxels())};
+  a.out = out.buf.get();
+  launch_resample(a, ctx_->stream());
+  return out;
+}
+
+}  // namespace fw::hip

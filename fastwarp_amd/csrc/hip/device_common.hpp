@@ -1,0 +1,162 @@
+// Device helpers shared by the kernel files: ITK interpolation and buffer rules, central
+// differences, wavefront and block reductions, window counts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace fw::hip {
+
+namespace {
+
+constexpr float kEps64 = 2.220446049250313e-16f;  // f64 machine epsilon, the reference's guard
+
+__device__ __forceinline__ long long lin(const Dims& d, int i, int j, int k) {
+  return (long long)i + (long long)d.nx * ((long long)j + (long long)d.ny * k);
+}
+
+// ITK IsInsideBuffer on a continuous index: every coordinate in [-0.5, n - 0.5).
+__device__ __forceinline__ bool inside(const Dims& d, float cx, float cy, float cz) {
+  return cx >= -0.5f && cx < (float)d.nx - 0.5f && cy >= -0.5f && cy < (float)d.ny - 0.5f && cz >= -0.5f &&
+         cz < (float)d.nz - 0.5f;
+}
+
+// ITK LinearInterpolateImageFunction::EvaluateOptimized (crates/fastwarp-reg/src/interp.rs:43-87):
+// base clamped to the buffer, no interpolation along an axis whose fraction is not positive or
+// whose upper neighbour is past the last voxel.
+__device__ __forceinline__ void axis_weight(float c, int n, int& b, float& w) {
+  const float f = floorf(c);
+  int bb = f < 0.0f ? 0 : (int)f;
+  bb = bb > n - 1 ? n - 1 : bb;
+  const float dd = c - (float)bb;
+  b = bb;
+  w = (dd <= 0.0f || bb + 1 > n - 1) ? 0.0f : dd;
+}
+
+__device__ __forceinline__ float tri(const float* __restrict__ img, const Dims& d, float cx, float cy, float cz) {
+  int bx, by, bz;
+  float wx, wy, wz;
+  axis_weight(cx, d.nx, bx, wx);
+  axis_weight(cy, d.ny, by, wy);
+  axis_weight(cz, d.nz, bz, wz);
+  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+  const float* p = img + lin(d, bx, by, bz);
+  auto row = [&](const float* q) {
+    const float v0 = q[0];
+    return wx > 0.0f ? v0 + (q[1] - v0) * wx : v0;
+  };
+  auto plane = [&](const float* q) {
+    const float v0 = row(q);
+    return wy > 0.0f ? v0 + (row(q + sy) - v0) * wy : v0;
+  };
+  const float v0 = plane(p);
+  return wz > 0.0f ? v0 + (plane(p + sz) - v0) * wz : v0;
+}
+
+// Nearest voxel with rounding half up (crates/fastwarp-reg/src/mask.rs:99-111); -1 outside.
+__device__ __forceinline__ long long nearest_index(const Dims& d, float cx, float cy, float cz) {
+  const float rx = floorf(cx + 0.5f), ry = floorf(cy + 0.5f), rz = floorf(cz + 0.5f);
+  if (!(rx >= 0.0f && rx < (float)d.nx && ry >= 0.0f && ry < (float)d.ny && rz >= 0.0f && rz < (float)d.nz)) return -1;
+  return lin(d, (int)rx, (int)ry, (int)rz);
+}
+
+// One component of ITK's CentralDifferenceImageFunction at voxel (i, j, k) (interp.rs:102-121):
+// probes at +-half a spacing along physical axis `a`, zero when either probe leaves the buffer.
+__device__ __forceinline__ float central_difference(const float* __restrict__ img, const Dims& d, const Probes& pr,
+                                                    int a, int i, int j, int k) {
+  if (pr.diagonal) {
+    // The probes lie on grid axis `ax` at -+0.5: interpolation reduces to its two neighbours.
+    int ax = 0;
+    float sgn = pr.o[a][0];
+    if (pr.o[a][1] != 0.0f) {
+      ax = 1;
+      sgn = pr.o[a][1];
+    } else if (pr.o[a][2] != 0.0f) {
+      ax = 2;
+      sgn = pr.o[a][2];
+    }
+    const int pos = ax == 0 ? i : ax == 1 ? j : k;
+    const int n = ax == 0 ? d.nx : ax == 1 ? d.ny : d.nz;
+    if (pos >= n - 1) return 0.0f;  // the upper probe is at n - 0.5, outside the buffer
+    const long long stride = ax == 0 ? 1 : ax == 1 ? (long long)d.nx : (long long)d.nx * d.ny;
+    const float* p = img + lin(d, i, j, k);
+    const float c0 = p[0];
+    const float up = c0 + (p[stride] - c0) * 0.5f;
+    float down = c0;
+    if (pos >= 1) {
+      const float m1 = p[-stride];
+      down = m1 + (c0 - m1) * 0.5f;
+    }
+    const float diff = sgn > 0.0f ? up - down : down - up;
+    return diff * pr.inv_spacing[a];
+  }
+  const float ci = (float)i, cj = (float)j, ck = (float)k;
+  const float px = ci + pr.o[a][0], py = cj + pr.o[a][1], pz = ck + pr.o[a][2];
+  const float mx = ci - pr.o[a][0], my = cj - pr.o[a][1], mz = ck - pr.o[a][2];
+  if (!inside(d, px, py, pz) || !inside(d, mx, my, mz)) return 0.0f;
+  return (tri(img, d, px, py, pz) - tri(img, d, mx, my, mz)) * pr.inv_spacing[a];
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ float nan_max(float a, float b) { return (b > a || b != b) ? b : a; }
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// Block sum (blockDim.x a multiple of 64, at most 1024) in a fixed tree; result in thread 0.
+__device__ float block_sum(float v) {
+  __shared__ float red[16];
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
+  return t;
+}
+
+__device__ float block_max(float v) {
+  __shared__ float red[16];
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t = nan_max(t, red[q]);
+  return t;
+}
+
+__device__ __forceinline__ int window_count(int pos, int n, int r) {
+  const int lo = pos > r ? pos - r : 0;
+  const int hi = pos + r < n - 1 ? pos + r : n - 1;
+  return hi - lo + 1;
+}
+
+// Floor division of a window sum by its voxel count: s = q * n + r with 0 <= r < n.
+__device__ __forceinline__ void floor_divmod(long long s, long long n, float inv_n, long long& q, long long& r) {
+  q = (long long)floorf((float)s * inv_n);
+  r = s - q * n;
+  while (r < 0) {
+    q -= 1;
+    r += n;
+  }
+  while (r >= n) {
+    q += 1;
+    r -= n;
+  }
+}
+
+}  // namespace
+
+}  // namespace fw::hip

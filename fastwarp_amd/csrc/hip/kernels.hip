@@ -1,0 +1,641 @@
+// gfx950 kernels of the SyN hot path: fused CC pair, warp/resample/compose gathers, separable
+// Gaussian smoothing, the fixed-point field inversion, and the reductions the SyN loop reads back.
+//
+// Layout: images are f32[n], fields three f32[n] planes (x, y, z), x-fastest. Every kernel is a
+// memory-bound stencil or gather; launches use 64-wide x rows so each wavefront reads and writes
+// 256 contiguous bytes per plane. Reductions are fixed trees per block followed by a fixed-order
+// combination, so results do not depend on scheduling.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "device_common.hpp"
+
+namespace fw::hip {
+
+// ------------------------------------------------------------------------------------------
+// CC pair (metric/cc.rs:170-321 and wgpu_backend.rs:1354-1520 semantics, fused).
+//
+// Both images are quantised to fixed point (|value| * q <= 2^22, one q per image) and every
+// window sum is an exact 64-bit integer, so the sums carry no rounding at all and do not depend
+// on tiling. A block owns a TX x TY column of outputs over tile_z planes: each input plane is
+// staged in LDS with an r-voxel halo, summed along x then y, and the z window slides through a
+// ring of 2r+1 plane sums held in registers (the loop is unrolled by 2r+1 so every ring index is
+// static). Centred moments come from the integer sums by a floor division (s = q n + r), which
+// keeps the cancellation inside exact integer arithmetic. Each output voxel then writes both
+// derivative directions; localCC is symmetric, so one value partial serves both.
+// ------------------------------------------------------------------------------------------
+template <int R, int TX, int TY, bool MASKED>
+__global__ __launch_bounds__(TX* TY) void k_cc_pair(CcArgs a) {
+  constexpr int W = 2 * R + 1;
+  constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
+  constexpr int NT = TX * TY;
+  __shared__ int qa[LY][LX];
+  __shared__ int qb[LY][LX];
+  __shared__ int qv[MASKED ? LY : 1][MASKED ? LX : 1];
+  // x-window sums: first moments fit 32 bits (|q| <= 2^22, at most 17 terms), second moments not.
+  __shared__ int xa[LY][TX], xb[LY][TX];
+  __shared__ long long xaa[LY][TX], xbb[LY][TX], xab[LY][TX];
+  __shared__ int xv[MASKED ? LY : 1][MASKED ? TX : 1];
+
+  const Dims d = a.d;
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.tile_z;
+  const int x = x0 + tx, y = y0 + ty;
+  const int zcount = min(a.tile_z, d.nz - z0);
+  const int steps = zcount + 2 * R;
+  const long long sz = (long long)d.nx * d.ny;
+  const long long n3 = d.n();
+
+  // Shift register of the last W xy-plane sums (slot 0 leaves next); xy sums of the first
+  // moments fit 32 bits (at most 17 x 17 terms of |q| <= 2^22).
+  int ra[W], rb[W], rv[MASKED ? W : 1];
+  long long raa[W], rbb[W], rab[W];
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    ra[q] = rb[q] = 0;
+    raa[q] = rbb[q] = rab[q] = 0;
+  }
+#pragma unroll
+  for (int q = 0; q < (MASKED ? W : 1); ++q) rv[q] = 0;
+  long long sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+  int sv = 0;
+  float cc_sum = 0.0f, cc_comp = 0.0f;
+  int counted = 0;
+
+  for (int step = 0; step < steps; ++step) {
+    const int zin = z0 - R + step;
+    int pa = 0, pb = 0, pv = 0;
+    long long paa = 0, pbb = 0, pab = 0;
+    if (zin >= 0 && zin < d.nz) {
+      __syncthreads();
+      for (int t = tid; t < LX * LY; t += NT) {
+        const int lx = t % LX, ly = t / LX;
+        const int gx = x0 - R + lx, gy = y0 - R + ly;
+        int va = 0, vb = 0, vv = 0;
+        if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) {
+          const long long g = (long long)gx + (long long)d.nx * gy + sz * zin;
+          const bool ok = !MASKED || a.valid[g] != 0.0f;
+          if (ok) {
+            va = __float2int_rn(a.fixed[g] * a.qf);
+            vb = __float2int_rn(a.moving[g] * a.qm);
+            vv = 1;
+          }
+        }
+        qa[ly][lx] = va;
+        qb[ly][lx] = vb;
+        if constexpr (MASKED) qv[ly][lx] = vv;
+      }
+      __syncthreads();
+      for (int t = tid; t < LY * TX; t += NT) {
+        const int ly = t / TX, lx = t % TX;
+        int s1 = 0, s2 = 0, s3 = 0;
+        long long s11 = 0, s22 = 0, s12 = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+          const int u = qa[ly][lx + k], v = qb[ly][lx + k];
+          s1 += u;
+          s2 += v;
+          s11 += (long long)u * u;
+          s22 += (long long)v * v;
+          s12 += (long long)u * v;
+          if constexpr (MASKED) s3 += qv[ly][lx + k];
+        }
+        xa[ly][lx] = s1;
+        xb[ly][lx] = s2;
+        xaa[ly][lx] = s11;
+        xbb[ly][lx] = s22;
+        xab[ly][lx] = s12;
+        if constexpr (MASKED) xv[ly][lx] = s3;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        pa += xa[ty + k][tx];
+        pb += xb[ty + k][tx];
+        paa += xaa[ty + k][tx];
+        pbb += xbb[ty + k][tx];
+        pab += xab[ty + k][tx];
+        if constexpr (MASKED) pv += xv[ty + k][tx];
+      }
+    }
+    sa += (long long)(pa - ra[0]);
+    sb += (long long)(pb - rb[0]);
+    saa += paa - raa[0];
+    sbb += pbb - rbb[0];
+    sab += pab - rab[0];
+    if constexpr (MASKED) sv += pv - rv[0];
+#pragma unroll
+    for (int q = 0; q < W - 1; ++q) {
+      ra[q] = ra[q + 1];
+      rb[q] = rb[q + 1];
+      raa[q] = raa[q + 1];
+      rbb[q] = rbb[q + 1];
+      rab[q] = rab[q + 1];
+      if constexpr (MASKED) rv[q] = rv[q + 1];
+    }
+    ra[W - 1] = pa;
+    rb[W - 1] = pb;
+    raa[W - 1] = paa;
+    rbb[W - 1] = pbb;
+    rab[W - 1] = pab;
+    if constexpr (MASKED) rv[W - 1] = pv;
+
+    if (step >= 2 * R && x < d.nx && y < d.ny) {
+      const int z = zin - R;
+      const long long g = (long long)x + (long long)d.nx * y + sz * z;
+      float dfx = 0.0f, dfy = 0.0f, dfz = 0.0f, dmx = 0.0f, dmy = 0.0f, dmz = 0.0f;
+      const bool centre_valid = !MASKED || a.valid[g] != 0.0f;
+      if (centre_valid) {
+        const long long n = MASKED ? (long long)sv
+                                   : (long long)window_count(x, d.nx, R) * window_count(y, d.ny, R) *
+                                         window_count(z, d.nz, R);
+        const float inv_n = 1.0f / (float)n;
+        long long qa_, ra_, qb_, rb_;
+        floor_divmod(sa, n, inv_n, qa_, ra_);
+        floor_divmod(sb, n, inv_n, qb_, rb_);
+        // Sums of squared deviations from the integer part of the mean, exact in wrapping
+        // 64-bit arithmetic (the true values are small), then the fractional correction.
+        const long long eaa = saa - n * qa_ * qa_ - 2 * qa_ * ra_;
+        const long long ebb = sbb - n * qb_ * qb_ - 2 * qb_ * rb_;
+        const long long eab = sab - n * qa_ * qb_ - qb_ * ra_ - qa_ * rb_;
+        const float s_ff = ((float)eaa - (float)(ra_ * ra_) * inv_n) * (a.inv_qf * a.inv_qf);
+        const float s_mm = ((float)ebb - (float)(rb_ * rb_) * inv_n) * (a.inv_qm * a.inv_qm);
+        const float s_fm = ((float)eab - (float)(ra_ * rb_) * inv_n) * (a.inv_qf * a.inv_qm);
+        const float fmean = ((float)qa_ + (float)ra_ * inv_n) * a.inv_qf;
+        const float mmean = ((float)qb_ + (float)rb_ * inv_n) * a.inv_qm;
+        const float product = s_ff * s_mm;
+        float cc = 1.0f;
+        if (fabsf(product) > kEps64) cc = s_fm * s_fm / product;
+        const float yk = cc - cc_comp;
+        const float tk = cc_sum + yk;
+        cc_comp = (tk - cc_sum) - yk;
+        cc_sum = tk;
+        counted += 1;
+        if (s_ff > kEps64 && s_mm > kEps64) {
+          const float fi = a.fixed[g] - fmean;
+          const float mi = a.moving[g] - mmean;
+          const float k1 = 2.0f * s_fm / product;
+          const float cf = k1 * (fi - s_fm / s_mm * mi);
+          const float cm = k1 * (mi - s_fm / s_ff * fi);
+          dfx = cf * central_difference(a.moving, d, a.probes, 0, x, y, z);
+          dfy = cf * central_difference(a.moving, d, a.probes, 1, x, y, z);
+          dfz = cf * central_difference(a.moving, d, a.probes, 2, x, y, z);
+          dmx = cm * central_difference(a.fixed, d, a.probes, 0, x, y, z);
+          dmy = cm * central_difference(a.fixed, d, a.probes, 1, x, y, z);
+          dmz = cm * central_difference(a.fixed, d, a.probes, 2, x, y, z);
+        }
+      }
+      a.d_fixed[g] = dfx;
+      a.d_fixed[g + n3] = dfy;
+      a.d_fixed[g + 2 * n3] = dfz;
+      a.d_moving[g] = dmx;
+      a.d_moving[g + n3] = dmy;
+      a.d_moving[g + 2 * n3] = dmz;
+    }
+  }
+  const float total = block_sum(cc_sum);
+  const float cnt = block_sum((float)counted);
+  if (tid == 0) {
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    a.partial_cc[b] = total;
+    a.partial_count[b] = (int)cnt;
+  }
+}
+
+template <int R, bool M>
+static void launch_cc_r(const CcArgs& a, hipStream_t s) {
+  constexpr int TX = R <= 5 ? 64 : 32;
+  constexpr int TY = 4;
+  const dim3 grid((a.d.nx + TX - 1) / TX, (a.d.ny + TY - 1) / TY, (a.d.nz + a.tile_z - 1) / a.tile_z);
+  hipLaunchKernelGGL((k_cc_pair<R, TX, TY, M>), grid, dim3(TX * TY), 0, s, a);
+}
+
+dim3 cc_grid(const Dims& d, int tile_z) {
+  // The radius decides TX; callers size the partial buffers with the largest grid (TX = 32).
+  return dim3((d.nx + 31) / 32, (d.ny + 3) / 4, (d.nz + tile_z - 1) / tile_z);
+}
+
+void launch_cc_pair(const CcArgs& a, hipStream_t s) {
+  const bool m = a.valid != nullptr;
+  switch (a.radius) {
+#define FW_CC_CASE(R)                         \
+  case R:                                     \
+    if (m) launch_cc_r<R, true>(a, s);       \
+    else launch_cc_r<R, false>(a, s);        \
+    break;
+    FW_CC_CASE(1)
+    FW_CC_CASE(2)
+    FW_CC_CASE(3)
+    FW_CC_CASE(4)
+    FW_CC_CASE(5)
+    FW_CC_CASE(6)
+    FW_CC_CASE(7)
+    FW_CC_CASE(8)
+#undef FW_CC_CASE
+    default:
+      break;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Field norms (field_ops.rs:191-213): per block max |u/spacing| and sum of |u|^2 of up to two
+// fields; the host combines the block partials in block order.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_field_norms(const float* __restrict__ a, const float* __restrict__ b, long long n,
+                                                     float3 inv_sp, float* partial_max, float* partial_sq) {
+  float ma = 0.0f, mb = 0.0f, sa = 0.0f, ca = 0.0f, sb = 0.0f, cb = 0.0f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    {
+      const float x = a[i], y = a[i + n], z = a[i + 2 * n];
+      const float px = x * inv_sp.x, py = y * inv_sp.y, pz = z * inv_sp.z;
+      ma = nan_max(ma, sqrtf(px * px + py * py + pz * pz));
+      const float q = x * x + y * y + z * z - ca;
+      const float t = sa + q;
+      ca = (t - sa) - q;
+      sa = t;
+    }
+    if (b) {
+      const float x = b[i], y = b[i + n], z = b[i + 2 * n];
+      const float px = x * inv_sp.x, py = y * inv_sp.y, pz = z * inv_sp.z;
+      mb = nan_max(mb, sqrtf(px * px + py * py + pz * pz));
+      const float q = x * x + y * y + z * z - cb;
+      const float t = sb + q;
+      cb = (t - sb) - q;
+      sb = t;
+    }
+  }
+  const float rma = block_max(ma);
+  const float rsa = block_sum(sa);
+  const float rmb = block_max(mb);
+  const float rsb = block_sum(sb);
+  if (threadIdx.x == 0) {
+    partial_max[blockIdx.x] = rma;
+    partial_sq[blockIdx.x] = rsa;
+    partial_max[gridDim.x + blockIdx.x] = rmb;
+    partial_sq[gridDim.x + blockIdx.x] = rsb;
+  }
+}
+
+int norm_blocks(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+void launch_field_norms(const float* a, const float* b, long long n, float3 inv_spacing, float* partial_max,
+                        float* partial_sq, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_field_norms, dim3(blocks), dim3(256), 0, s, a, b, n, inv_spacing, partial_max, partial_sq);
+}
+
+__global__ __launch_bounds__(256) void k_abs_max(const float* __restrict__ x, long long n, float* partial) {
+  float m = 0.0f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    m = nan_max(m, fabsf(x[i]));
+  const float r = block_max(m);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+void launch_abs_max(const float* x, long long n, float* partial, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_abs_max, dim3(blocks), dim3(256), 0, s, x, n, partial);
+}
+
+// ------------------------------------------------------------------------------------------
+// Elementwise scale and axpy over the three planes (field_ops.rs:216-244).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scale(float* __restrict__ x, long long count, float s) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x)
+    x[i] *= s;
+}
+
+__global__ __launch_bounds__(256) void k_axpy(float* __restrict__ acc, const float* __restrict__ x, long long count, float a) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x)
+    acc[i] += a * x[i];
+}
+
+static int stream_blocks(long long count) {
+  long long b = (count + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+void launch_scale(float* x, long long count, float s, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale, dim3(stream_blocks(count)), dim3(256), 0, st, x, count, s);
+}
+
+void launch_axpy(float* acc, const float* x, long long count, float a, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy, dim3(stream_blocks(count)), dim3(256), 0, st, acc, x, count, a);
+}
+
+// ------------------------------------------------------------------------------------------
+// Warp (cpu/mod.rs:60-132, wgpu_backend.rs:2023-2130): output voxel x maps to the source
+// continuous index to_src.m * x + to_src.c0 + to_src.l * (u(x) [+ pre(q)]), trilinear inside the
+// buffer and zero outside; in mask mode the nearest voxel decides {0, 1}.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_warp(WarpArgs a) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i >= a.od.nx || j >= a.od.ny) return;
+  const long long g = lin(a.od, i, j, k);
+  const long long n = a.od.n();
+  const float ux = a.field[g], uy = a.field[g + n], uz = a.field[g + 2 * n];
+  const float fi = (float)i, fj = (float)j, fk = (float)k;
+  const IndexMap& m = a.to_src;
+  float cx = fmaf(m.m[0], fi, fmaf(m.m[1], fj, fmaf(m.m[2], fk, m.c0[0])));
+  float cy = fmaf(m.m[3], fi, fmaf(m.m[4], fj, fmaf(m.m[5], fk, m.c0[1])));
+  float cz = fmaf(m.m[6], fi, fmaf(m.m[7], fj, fmaf(m.m[8], fk, m.c0[2])));
+  cx += m.l[0] * ux + m.l[1] * uy + m.l[2] * uz;
+  cy += m.l[3] * ux + m.l[4] * uy + m.l[5] * uz;
+  cz += m.l[6] * ux + m.l[7] * uy + m.l[8] * uz;
+  if (a.pre) {
+    const IndexMap& p = a.to_pre;
+    float px = fmaf(p.m[0], fi, fmaf(p.m[1], fj, fmaf(p.m[2], fk, p.c0[0])));
+    float py = fmaf(p.m[3], fi, fmaf(p.m[4], fj, fmaf(p.m[5], fk, p.c0[1])));
+    float pz = fmaf(p.m[6], fi, fmaf(p.m[7], fj, fmaf(p.m[8], fk, p.c0[2])));
+    px += p.l[0] * ux + p.l[1] * uy + p.l[2] * uz;
+    py += p.l[3] * ux + p.l[4] * uy + p.l[5] * uz;
+    pz += p.l[6] * ux + p.l[7] * uy + p.l[8] * uz;
+    if (inside(a.pd, px, py, pz)) {
+      const long long pn = a.pd.n();
+      const float dx = tri(a.pre, a.pd, px, py, pz);
+      const float dy = tri(a.pre + pn, a.pd, px, py, pz);
+      const float dz = tri(a.pre + 2 * pn, a.pd, px, py, pz);
+      cx += a.pre_l[0] * dx + a.pre_l[1] * dy + a.pre_l[2] * dz;
+      cy += a.pre_l[3] * dx + a.pre_l[4] * dy + a.pre_l[5] * dz;
+      cz += a.pre_l[6] * dx + a.pre_l[7] * dy + a.pre_l[8] * dz;
+    }
+  }
+  float v = 0.0f;
+  if (a.nearest) {
+    const long long q = nearest_index(a.sd, cx, cy, cz);
+    v = (q >= 0 && a.src[q] != 0.0f) ? 1.0f : 0.0f;
+    if (a.product) v *= a.product[g];
+  } else if (inside(a.sd, cx, cy, cz)) {
+    v = tri(a.src, a.sd, cx, cy, cz);
+  }
+  a.out[g] = v;
+}
+
+void launch_warp(const WarpArgs& a, hipStream_t s) {
+  const dim3 grid((a.od.nx + 63) / 64, (a.od.ny + 3) / 4, a.od.nz);
+  hipLaunchKernelGGL(k_warp, grid, dim3(256), 0, s, a);
+}
+
+// Field resampling onto another grid (field_ops.rs:580-605): linear, zero outside the old buffer.
+__global__ __launch_bounds__(256) void k_resample(ResampleArgs a) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i >= a.od.nx || j >= a.od.ny) return;
+  const long long g = lin(a.od, i, j, k);
+  const long long n = a.od.n(), sn = a.sd.n();
+  const float fi = (float)i, fj = (float)j, fk = (float)k;
+  const IndexMap& m = a.to_src;
+  const float cx = fmaf(m.m[0], fi, fmaf(m.m[1], fj, fmaf(m.m[2], fk, m.c0[0])));
+  const float cy = fmaf(m.m[3], fi, fmaf(m.m[4], fj, fmaf(m.m[5], fk, m.c0[1])));
+  const float cz = fmaf(m.m[6], fi, fmaf(m.m[7], fj, fmaf(m.m[8], fk, m.c0[2])));
+  float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+  if (inside(a.sd, cx, cy, cz)) {
+    vx = tri(a.src, a.sd, cx, cy, cz);
+    vy = tri(a.src + sn, a.sd, cx, cy, cz);
+    vz = tri(a.src + 2 * sn, a.sd, cx, cy, cz);
+  }
+  a.out[g] = vx;
+  a.out[g + n] = vy;
+  a.out[g + 2 * n] = vz;
+}
+
+void launch_resample(const ResampleArgs& a, hipStream_t s) {
+  const dim3 grid((a.od.nx + 63) / 64, (a.od.ny + 3) / 4, a.od.nz);
+  hipLaunchKernelGGL(k_resample, grid, dim3(256), 0, s, a);
+}
+
+// ------------------------------------------------------------------------------------------
+// Compose (field_ops.rs:251-272, ITK ComposeDisplacementFieldsImageFilter): out = w + u(x + w),
+// u interpolated on the shared grid at the continuous index (i, j, k) + l * w, zero outside.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void compose_at(const float* __restrict__ u, const Dims& d, const float* l, int i, int j,
+                                           int k, float wx, float wy, float wz, float& ox, float& oy, float& oz) {
+  const float cx = (float)i + (l[0] * wx + l[1] * wy + l[2] * wz);
+  const float cy = (float)j + (l[3] * wx + l[4] * wy + l[5] * wz);
+  const float cz = (float)k + (l[6] * wx + l[7] * wy + l[8] * wz);
+  ox = wx;
+  oy = wy;
+  oz = wz;
+  if (inside(d, cx, cy, cz)) {
+    const long long n = d.n();
+    ox += tri(u, d, cx, cy, cz);
+    oy += tri(u + n, d, cx, cy, cz);
+    oz += tri(u + 2 * n, d, cx, cy, cz);
+  }
+}
+
+struct L9 {
+  float v[9];
+};
+
+__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u, const float* __restrict__ w, Dims d, L9 l,
+                                                 float* __restrict__ out) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const long long g = lin(d, i, j, k), n = d.n();
+  float ox, oy, oz;
+  compose_at(u, d, l.v, i, j, k, w[g], w[g + n], w[g + 2 * n], ox, oy, oz);
+  out[g] = ox;
+  out[g + n] = oy;
+  out[g + 2 * n] = oz;
+}
+
+void launch_compose(const float* displacement, const float* warping, Dims d, const float l[9], float* out, hipStream_t s) {
+  L9 lv;
+  for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
+  const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
+  hipLaunchKernelGGL(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out);
+}
+
+// ------------------------------------------------------------------------------------------
+// Separable Gaussian (field_ops.rs:508-574): taps clamped at the edges; the final pass zeroes
+// every face voxel and blends w1 * smooth + w2 * original.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gauss_axis(const float* __restrict__ in, float* __restrict__ out, Dims d, int axis,
+                                                    const float* __restrict__ taps, int radius,
+                                                    const float* __restrict__ orig, float w1, float w2, int final_pass) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const long long g = lin(d, i, j, k), n = d.n();
+  if (final_pass && (i == 0 || j == 0 || k == 0 || i == d.nx - 1 || j == d.ny - 1 || k == d.nz - 1)) {
+    out[g] = 0.0f;
+    out[g + n] = 0.0f;
+    out[g + 2 * n] = 0.0f;
+    return;
+  }
+  const int pos = axis == 0 ? i : axis == 1 ? j : k;
+  const int len = axis == 0 ? d.nx : axis == 1 ? d.ny : d.nz;
+  const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)d.nx : (long long)d.nx * d.ny;
+  const long long rowbase = g - (long long)pos * stride;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int t = 0; t <= 2 * radius; ++t) {
+    int q = pos + t - radius;
+    q = q < 0 ? 0 : (q > len - 1 ? len - 1 : q);
+    const long long src = rowbase + (long long)q * stride;
+    const float wt = taps[t];
+    ax = fmaf(wt, in[src], ax);
+    ay = fmaf(wt, in[src + n], ay);
+    az = fmaf(wt, in[src + 2 * n], az);
+  }
+  if (final_pass) {
+    ax = ax * w1 + orig[g] * w2;
+    ay = ay * w1 + orig[g + n] * w2;
+    az = az * w1 + orig[g + 2 * n] * w2;
+  }
+  out[g] = ax;
+  out[g + n] = ay;
+  out[g + 2 * n] = az;
+}
+
+void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const float* taps_dev, int radius,
+                       const float* orig, float w1, float w2, int final_pass, hipStream_t s) {
+  const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
+  hipLaunchKernelGGL(k_gauss_axis, grid, dim3(256), 0, s, in, out, d, axis, taps_dev, radius, orig, w1, w2, final_pass);
+}
+
+// ------------------------------------------------------------------------------------------
+// Inversion (field_ops.rs:315-392, ITK InvertDisplacementFieldImageFilter) as device-controlled
+// steps: residual partials -> one-block reduction that decides whether the step runs -> update.
+// The residual e = psi + field(x + psi) is recomputed by the update instead of being stored.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_inv_residual(const float* __restrict__ field, const float* __restrict__ psi,
+                                                      Dims d, L9 l, float3 inv_sp, const InvertState* state,
+                                                      float* partial_sum, float* partial_max) {
+  if (state->stop) return;
+  const long long n = d.n();
+  float s = 0.0f, c = 0.0f, m = 0.0f;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(g % d.nx);
+    const long long r = g / d.nx;
+    const int j = (int)(r % d.ny), k = (int)(r / d.ny);
+    float ex, ey, ez;
+    compose_at(field, d, l.v, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez);
+    ex *= inv_sp.x;
+    ey *= inv_sp.y;
+    ez *= inv_sp.z;
+    const float nrm = sqrtf(ex * ex + ey * ey + ez * ez);
+    m = nan_max(m, nrm);
+    const float y = nrm - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  const float bs = block_sum(s);
+  const float bm = block_max(m);
+  if (threadIdx.x == 0) {
+    partial_sum[blockIdx.x] = bs;
+    partial_max[blockIdx.x] = bm;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_inv_reduce(const float* __restrict__ partial_sum, const float* __restrict__ partial_max,
+                                                    int blocks, long long n, InvertState* state, int step, int max_iterations,
+                                                    float max_tol, float mean_tol) {
+  if (state->stop) return;
+  float s = 0.0f, c = 0.0f, m = 0.0f;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    const float y = partial_sum[b] - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+    m = nan_max(m, partial_max[b]);
+  }
+  const float total = block_sum(s);
+  const float mx = block_max(m);
+  if (threadIdx.x == 0) {
+    const float mean = total / (float)n;
+    state->mean_error = mean;
+    state->max_error = mx;
+    state->executed = step + 1;
+    const bool go_on = (step + 1) < max_iterations && mx > max_tol && mean > mean_tol;
+    state->stop = go_on ? 0 : 1;
+    state->active_step = step;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_inv_update(const float* __restrict__ field, float* __restrict__ psi, Dims d, L9 l,
+                                                    float3 inv_sp, const InvertState* state, int step, int enforce) {
+  if (state->active_step != step) return;
+  const float eps = step == 0 ? 0.75f : 0.5f;
+  const float limit = eps * state->max_error;
+  const long long n = d.n();
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(g % d.nx);
+    const long long r = g / d.nx;
+    const int j = (int)(r % d.ny), k = (int)(r / d.ny);
+    const float wx = psi[g], wy = psi[g + n], wz = psi[g + 2 * n];
+    float ex, ey, ez;
+    compose_at(field, d, l.v, i, j, k, wx, wy, wz, ex, ey, ez);
+    const float px = ex * inv_sp.x, py = ey * inv_sp.y, pz = ez * inv_sp.z;
+    const float nrm = sqrtf(px * px + py * py + pz * pz);
+    float ux = -ex, uy = -ey, uz = -ez;
+    if (nrm > limit) {
+      const float f = limit / nrm;
+      ux *= f;
+      uy *= f;
+      uz *= f;
+    }
+    float nx = wx + ux * eps, ny = wy + uy * eps, nz = wz + uz * eps;
+    if (enforce && (i == 0 || j == 0 || k == 0 || i == d.nx - 1 || j == d.ny - 1 || k == d.nz - 1)) nx = ny = nz = 0.0f;
+    psi[g] = nx;
+    psi[g + n] = ny;
+    psi[g + 2 * n] = nz;
+  }
+}
+
+__global__ void k_inv_init(InvertState* state, int max_iterations) {
+  if (threadIdx.x == 0) {
+    state->mean_error = 0.0f;
+    state->max_error = 0.0f;
+    state->executed = 0;
+    state->stop = max_iterations > 0 ? 0 : 1;
+    state->active_step = -1;
+    state->pad = 0;
+  }
+}
+
+void launch_inv_init(InvertState* state, int max_iterations, hipStream_t s) {
+  hipLaunchKernelGGL(k_inv_init, dim3(1), dim3(64), 0, s, state, max_iterations);
+}
+
+int inv_blocks(const Dims& d) {
+  long long b = (d.n() + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+void launch_inv_residual(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
+                         const InvertState* state, float* partial_sum, float* partial_max, int blocks, hipStream_t s) {
+  L9 lv;
+  for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
+  hipLaunchKernelGGL(k_inv_residual, dim3(blocks), dim3(256), 0, s, field, psi, d, lv, inv_spacing, state, partial_sum,
+                     partial_max);
+}
+
+void launch_inv_reduce(const float* partial_sum, const float* partial_max, int blocks, long long n, InvertState* state,
+                       int step, int max_iterations, double max_tol, double mean_tol, hipStream_t s) {
+  hipLaunchKernelGGL(k_inv_reduce, dim3(1), dim3(256), 0, s, partial_sum, partial_max, blocks, n, state, step,
+                     max_iterations, (float)max_tol, (float)mean_tol);
+}
+
+void launch_inv_update(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing,
+                       const InvertState* state, int step, int enforce_boundary, hipStream_t s) {
+  L9 lv;
+  for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
+  hipLaunchKernelGGL(k_inv_update, dim3(inv_blocks(d)), dim3(256), 0, s, field, psi, d, lv, inv_spacing, state, step,
+                     enforce_boundary);
+}
+
+}  // namespace fw::hip

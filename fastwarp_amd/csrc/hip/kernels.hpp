@@ -1,0 +1,147 @@
+// Launch interface of the gfx950 kernels. Host code (the backend) fills these plain structs in f64
+// and narrows to f32; the kernels see only raw device pointers and sizes.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace fw::hip {
+
+// A continuous-index map c = m * (i, j, k) + c0 (+ l * u for a displacement u), f32 row-major.
+struct IndexMap {
+  float m[9];
+  float c0[3];
+  float l[9];
+};
+
+struct Dims {
+  int nx, ny, nz;
+  __host__ __device__ long long n() const { return (long long)nx * ny * nz; }
+};
+
+// Offsets (continuous index units) of the half-spacing probes of a central difference along each
+// physical axis, and the spacing that divides their difference.
+struct Probes {
+  float o[3][3];  // o[d] = index offset of +h_d e_d
+  float inv_spacing[3];
+  int diagonal;   // every offset lies on its own grid axis with magnitude 0.5
+};
+
+// --- CC pair (fused window sums, both directions) ---
+struct CcArgs {
+  const float* fixed;
+  const float* moving;
+  const float* valid;  // nullptr: every voxel valid
+  Dims d;
+  int radius;
+  int tile_z;
+  float qf, qm;          // fixed-point scales of the two images
+  float inv_qf, inv_qm;
+  Probes probes;
+  float* d_fixed;        // 3 planes: coefficient * grad(moving)
+  float* d_moving;       // 3 planes: coefficient' * grad(fixed)
+  float* partial_cc;     // per block: sum of localCC over its valid voxels
+  int* partial_count;    // per block: valid voxels
+};
+void launch_cc_pair(const CcArgs& a, hipStream_t s);
+dim3 cc_grid(const Dims& d, int tile_z);
+
+// --- per-block partial reductions ---
+// Largest |u/spacing| of two fields (b may be null) and sums of squares.
+void launch_field_norms(const float* a, const float* b, long long n, float3 inv_spacing, float* partial_max,
+                        float* partial_sq, int blocks, hipStream_t s);
+int norm_blocks(long long n);
+// In-place scaled max of image magnitudes (for fixed-point scales): out[block] = max |x|.
+void launch_abs_max(const float* x, long long n, float* partial, int blocks, hipStream_t s);
+
+// --- elementwise ---
+void launch_scale(float* x, long long count, float s, hipStream_t st);
+void launch_axpy(float* acc, const float* x, long long count, float a, hipStream_t st);
+
+// --- warp / resample ---
+struct WarpArgs {
+  const float* src;
+  Dims sd;               // source dims
+  const float* field;    // 3 planes on the output grid
+  Dims od;               // output dims
+  IndexMap to_src;       // output index (+ displacement) -> source continuous index
+  const float* pre;      // optional prior field on its own grid
+  Dims pd;
+  IndexMap to_pre;       // output index (+ displacement) -> prior continuous index
+  float pre_l[9];        // prior displacement -> source continuous index
+  int nearest;           // 1: mask mode (nearest voxel, {0,1})
+  const float* product;  // mask mode: multiply into
+  float* out;
+};
+void launch_warp(const WarpArgs& a, hipStream_t s);
+
+struct ResampleArgs {
+  const float* src;  // 3 planes
+  Dims sd;
+  Dims od;
+  IndexMap to_src;
+  float* out;        // 3 planes
+};
+void launch_resample(const ResampleArgs& a, hipStream_t s);
+
+// --- compose: out = w + u(x + w) on one grid; l maps a displacement to index offsets ---
+void launch_compose(const float* displacement, const float* warping, Dims d, const float l[9], float* out, hipStream_t s);
+
+// --- Gaussian smoothing, separable, clamped edges; final pass zeroes faces and blends ---
+void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const float* taps_dev, int radius,
+                       const float* orig, float w1, float w2, int final_pass, hipStream_t s);
+
+// --- inversion (ITK InvertDisplacementFieldImageFilter) ---
+struct InvertState {
+  // written by the reduction kernel, read by the update kernel and the host
+  float mean_error;
+  float max_error;
+  int32_t executed;
+  int32_t stop;         // 1: no further step runs
+  int32_t active_step;  // the step whose update runs
+  int32_t pad;
+};
+void launch_inv_init(InvertState* state, int max_iterations, hipStream_t s);
+void launch_inv_residual(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
+                         const InvertState* state, float* partial_sum, float* partial_max, int blocks, hipStream_t s);
+void launch_inv_reduce(const float* partial_sum, const float* partial_max, int blocks, long long n, InvertState* state,
+                       int step, int max_iterations, double max_tol, double mean_tol, hipStream_t s);
+void launch_inv_update(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing,
+                       const InvertState* state, int step, int enforce_boundary, hipStream_t s);
+int inv_blocks(const Dims& d);
+
+// --- linear-stage Mattes MI ---
+struct MiLinearArgs {
+  const float4* samples;  // (x, y, z) virtual continuous coordinates relative to the anchor, w = fixed bin bits
+  long long count;
+  const float* moving;
+  Dims md;
+  float a[12];            // moving continuous index = A[0:9] * s + A[9:12]
+  float moving_min, moving_max, inv_bin_size, normalized_min;
+  int bins;
+  int chunk;              // samples per block
+  unsigned long long* hist_partial;  // per block: bins*bins joint + bins fixed marginal, 2^-32 fixed point
+  unsigned int* valid_partial;       // per block: contributing samples
+};
+void launch_mi_hist(const MiLinearArgs& a, int blocks, hipStream_t s);
+void launch_mi_merge(const unsigned long long* partial, const unsigned int* valid_partial, int blocks, int cells,
+                     unsigned long long* out, unsigned long long* valid_out, hipStream_t s);
+
+struct MiDerivArgs {
+  MiLinearArgs base;
+  const float* ratio;    // bins*bins f32 weights
+  float probe_o[3][3];   // central-difference probe offsets (index units)
+  float probe_inv_spacing[3];
+  float jac_outer[9];    // outer matrix (columns left-multiplied)
+  float center[3];       // optimised transform centre relative to the anchor (physical)
+  float euler[6];        // cx sx cy sy cz sz for Euler3D
+  float to_phys[12];     // sample continuous coords -> physical point relative to the anchor
+  int kind;              // 0 Euler3D, 1 affine
+  int params;
+  double* partial;       // per block: params doubles
+};
+void launch_mi_deriv(const MiDerivArgs& a, int blocks, hipStream_t s);
+
+}  // namespace fw::hip

@@ -1,0 +1,65 @@
+// Launch interface of the Mattes mutual-information kernels (linear-stage sampled MI and the dense
+// SyN-stage pair). Histograms are exact integers: Parzen weights in 2^-32 fixed point summed with
+// 64-bit LDS atomics, so the result does not depend on the order of the additions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace fw::hip {
+
+constexpr int kMiMaxBins = 64;
+constexpr double kMiFixedPoint = 4294967296.0;  // 2^32
+
+// One linear-stage evaluation (crates/fastwarp-reg/src/metric/mi.rs:364-437 `accumulate`).
+struct MiSampledArgs {
+  const float4* samples;  // (x, y, z) = physical point minus `anchor`, w = fixed bin (as int bits)
+  int count;
+  const float* moving;
+  Dims md;
+  float a[9];             // moving continuous index = a * s + b
+  float b[3];
+  float inv_bin_size;     // 1 / moving_bin_size
+  float moving_min, moving_max;
+  int bins;
+  unsigned long long* hist;  // bins*bins joint, then bins fixed marginal (counts), then the valid count
+  // derivative pass only
+  const float* ratio;     // bins*bins weights
+  Probes probes;          // central-difference probes of the moving grid
+  float outer[9];         // outer matrix that left-multiplies the Jacobian columns
+  float rel_center[3];    // anchor - optimised centre (physical)
+  float trig[6];          // cx sx cy sy cz sz of the Euler angles
+  int kind;               // 0 Euler3D (Rz Rx Ry), 1 affine
+  double* partial;        // per block: 12 doubles
+};
+void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s);
+void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
+int mi_sampled_blocks(int count);
+
+// Dense pair on one grid (mi.rs:578-676 in both directions, wgpu_backend/mi/dense.rs:241-487).
+struct MiDenseArgs {
+  const float* f;
+  const float* m;
+  const float* fmask;  // may be null
+  const float* mmask;  // may be null
+  Dims d;
+  int bins;
+  // direction 0: fixed = f, moving = m; direction 1: fixed = m, moving = f
+  float fixed_thr[2][kMiMaxBins];  // f32 thresholds of the fixed-axis bins 3..bins-3
+  float moving_min[2], moving_max[2], inv_bin_size[2];
+  unsigned long long* hist;        // 2 x (bins*bins + bins + 1)
+  const float* ratio;              // 2 x bins*bins
+  Probes probes;
+  float* d0;
+  float* d1;
+  float* range_partial;            // per block: fmin fmax mmin mmax
+};
+void launch_mi_dense_range(const MiDenseArgs& a, int blocks, hipStream_t s);
+void launch_mi_dense_hist(const MiDenseArgs& a, int blocks, hipStream_t s);
+void launch_mi_dense_deriv(const MiDenseArgs& a, hipStream_t s);
+int mi_dense_blocks(long long n);
+
+}  // namespace fw::hip

@@ -1,0 +1,240 @@
+/* fastwarp_hip.h — C ABI of the MI355X (gfx950) registration engine.
+ *
+ * This is the drop-in boundary of fastwarp's GPU registration hot path. The reference drives its
+ * device through three Rust traits:
+ *   - fastwarp_reg::Backend          crates/fastwarp-reg/src/backend.rs:39-305 (the SyN loop's ops)
+ *   - fastwarp_reg::LinearEngine     crates/fastwarp-reg/src/linear.rs:288-320
+ *   - fastwarp_reg::LinearMetric     crates/fastwarp-reg/src/optimizer.rs:60-65
+ * and implements them with wgpu in crates/fastwarp-gpu/src/wgpu_backend.rs (`WgpuBackend`) and
+ * crates/fastwarp-gpu/src/wgpu_backend/mi.rs (`GpuMiLevel`). Every entry point below replaces one
+ * trait method (cited per function), with opaque handles for the trait's associated
+ * `Image`/`Field` types. Plain pointers and sizes only; no C++ or torch types cross this line.
+ *
+ * Conventions
+ *   - One fw_ctx is one HIP device, one stream and one buffer pool. Calls on one context are
+ *     serialised by the caller (the trait's ops are issued from one thread, backend.rs:36-37);
+ *     different contexts may be used from different threads.
+ *   - Every function returning int returns FW_OK (0) or a negative status; fw_last_error()
+ *     gives the message ("what: message", as RegError::Invalid prints).
+ *   - Images are f32 on the device, x-fastest (wgpu_backend.rs:120-135). Fields are three f32
+ *     planes (x, y, z) of LPS millimetre displacements (wgpu_backend.rs:159-170). Host-side
+ *     arrays crossing the ABI are f64: images as n values, fields as 3n interleaved (dx,dy,dz),
+ *     as Volume<f64> and VectorField are on the host side of the reference.
+ *   - Ownership follows the trait: functions documented as consuming a handle take it over
+ *     (it must not be used or freed afterwards), exactly where the trait takes `Self::Field` by
+ *     value (smooth_field, invert, invert_pair, warp_mask_through_field's `product`).
+ */
+#ifndef FASTWARP_HIP_H
+#define FASTWARP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FW_OK 0
+#define FW_ERR_INVALID (-1)   /* invalid argument or state (RegError::Invalid) */
+#define FW_ERR_DEVICE (-2)    /* HIP runtime error or device allocation failure */
+#define FW_ERR_CANCELLED (-3) /* the cancel flag was set (RegError::Cancelled) */
+
+typedef struct fw_ctx fw_ctx;
+typedef struct fw_image fw_image;
+typedef struct fw_field fw_field;
+typedef struct fw_mi_level fw_mi_level;
+
+/* An ITK image grid in LPS mm: p = origin + direction * diag(spacing) * index; direction holds
+ * the axis directions as columns, row-major direction[3*row + axis] (grid.rs:26-44). */
+typedef struct {
+  int64_t size[3];
+  double spacing[3];
+  double origin[3];
+  double direction[9];
+} fw_grid;
+
+/* y = M (x - c) + c + t (transform/affine.rs:34-45), matrix row-major. */
+typedef struct {
+  double matrix[9];
+  double translation[3];
+  double center[3];
+} fw_affine;
+
+/* InvertParams (field_ops.rs:276-295); SyN uses 20, 0.001, 0.1, 1. */
+typedef struct {
+  uint32_t max_iterations;
+  double mean_error_tolerance;
+  double max_error_tolerance;
+  int32_t enforce_boundary_condition;
+} fw_invert_params;
+
+/* InvertStats (field_ops.rs:298-307). */
+typedef struct {
+  uint32_t iterations;
+  double mean_error_norm;
+  double max_error_norm;
+} fw_invert_stats;
+
+/* MiBinning (metric/mi.rs:121-143) of one linear-stage level. */
+typedef struct {
+  int64_t bins;
+  double fixed_min, fixed_max, moving_min, moving_max;
+  double fixed_bin_size, moving_bin_size, fixed_normalized_min, moving_normalized_min;
+} fw_mi_binning;
+
+/* LinearMovingTransform (transform/param.rs:306-315): x -> outer(optimised(x)). kind 0 is
+ * Euler3D (6 parameters, matrix Rz Rx Ry), kind 1 is affine (12: matrix row-major, translation). */
+typedef struct {
+  fw_affine outer;
+  int32_t kind;
+  double parameters[12];
+  double center[3];
+} fw_linear_transform;
+
+/* ---- context (WgpuBackend::new, release_idle_buffers backend.rs:50) ---- */
+int fw_ctx_create(int device, fw_ctx** out);
+void fw_ctx_destroy(fw_ctx* ctx);
+const char* fw_last_error(const fw_ctx* ctx);
+/* Backend::release_idle_buffers (backend.rs:48-50). */
+int fw_release_idle(fw_ctx* ctx);
+/* Device bytes the context's pool holds (in use plus idle) and its peak since creation. */
+int fw_memory_stats(fw_ctx* ctx, uint64_t* held, uint64_t* peak);
+
+/* ---- images (Backend::upload_image backend.rs:53-57) ---- */
+int fw_image_upload(fw_ctx* ctx, const fw_grid* grid, const double* data, fw_image** out);
+int fw_image_upload_f32(fw_ctx* ctx, const fw_grid* grid, const float* data, fw_image** out);
+int fw_image_download(fw_ctx* ctx, const fw_image* image, double* out);
+int fw_image_grid(const fw_image* image, fw_grid* out);
+void fw_image_free(fw_ctx* ctx, fw_image* image);
+
+/* ---- fields (Backend::zero_field/upload_field/download_field/field_grid/copy_field
+ *      backend.rs:59-72) ---- */
+int fw_field_zero(fw_ctx* ctx, const fw_grid* grid, fw_field** out);
+int fw_field_upload(fw_ctx* ctx, const fw_grid* grid, const double* aos, fw_field** out);
+int fw_field_download(fw_ctx* ctx, const fw_field* field, double* aos);
+int fw_field_grid(const fw_field* field, fw_grid* out);
+int fw_field_copy(fw_ctx* ctx, const fw_field* field, fw_field** out);
+void fw_field_free(fw_ctx* ctx, fw_field* field);
+
+/* Backend::warp_through_field (backend.rs:80-86): out(x) = src(affine(q + pre(q))), q = x + field(x);
+ * pre and affine may be NULL. */
+int fw_warp(fw_ctx* ctx, const fw_image* src, const fw_field* field, const fw_field* pre, const fw_affine* affine,
+            fw_image** out);
+/* Backend::warp_mask_through_field (backend.rs:95-102): nearest neighbour, {0,1}; `product`
+ * (consumed, may be NULL) is multiplied in and its buffer returned. */
+int fw_warp_mask(fw_ctx* ctx, const fw_image* mask, const fw_field* field, const fw_field* pre, const fw_affine* affine,
+                 fw_image* product, fw_image** out);
+
+/* Backend::cc_value_derivative_pair(_masked)(_with_norms) (backend.rs:121-177): both directions of
+ * ANTs neighbourhood CC from one set of window sums. `valid` may be NULL (no masks). values[0] is
+ * (fixed, moving), values[1] (moving, fixed); d_fixed / d_moving are new fields. norms (may be
+ * NULL) receives the l2 norms of both derivatives. */
+int fw_cc_pair(fw_ctx* ctx, const fw_image* fixed, const fw_image* moving, const fw_image* valid, uint32_t radius,
+               double values[2], double norms[2], fw_field** d_fixed, fw_field** d_moving);
+
+/* Backend::mi_value_derivative_pair(_masked)(_with_norms) (backend.rs:185-243): dense Mattes MI in
+ * both directions; masks may be NULL. */
+int fw_mi_dense_pair(fw_ctx* ctx, const fw_image* fixed, const fw_image* moving, const fw_image* fixed_mask,
+                     const fw_image* moving_mask, uint32_t bins, double values[2], double norms[2], fw_field** d_fixed,
+                     fw_field** d_moving);
+
+/* Backend::smooth_field (backend.rs:248): consumes `field`; variance <= 0 returns it unchanged. */
+int fw_smooth_field(fw_ctx* ctx, fw_field* field, double variance, fw_field** out);
+/* Backend::max_scaled_norm_pair (backend.rs:251-260); b may be NULL (out[1] untouched). */
+int fw_max_scaled_norm_pair(fw_ctx* ctx, const fw_field* a, const fw_field* b, double out[2]);
+/* Backend::l2_norm (backend.rs:263). */
+int fw_l2_norm(fw_ctx* ctx, const fw_field* field, double* out);
+/* Backend::scale (backend.rs:266). */
+int fw_scale(fw_ctx* ctx, fw_field* field, double s);
+/* Backend::axpy (backend.rs:269). */
+int fw_axpy(fw_ctx* ctx, fw_field* acc, double a, const fw_field* x);
+/* Backend::compose (backend.rs:273-277): out(x) = warping(x) + displacement(x + warping(x)). */
+int fw_compose(fw_ctx* ctx, const fw_field* displacement, const fw_field* warping, fw_field** out);
+/* Backend::invert (backend.rs:281-286): consumes `initial`. */
+int fw_invert(fw_ctx* ctx, const fw_field* field, fw_field* initial, const fw_invert_params* params, fw_field** out,
+              fw_invert_stats* stats);
+/* Backend::invert_pair (backend.rs:290-301): consumes both; new_inverse = invert(field, inverse),
+ * then new_field = invert(new_inverse, field). */
+int fw_invert_pair(fw_ctx* ctx, fw_field* field, fw_field* inverse, const fw_invert_params* params,
+                   fw_field** new_inverse, fw_invert_stats* inverse_stats, fw_field** new_field,
+                   fw_invert_stats* field_stats);
+/* Backend::resample_field (backend.rs:304). */
+int fw_resample_field(fw_ctx* ctx, const fw_field* field, const fw_grid* grid, fw_field** out);
+
+/* ---- linear-stage Mattes MI (LinearEngine::prepare_level linear.rs:298-319, LinearMetric
+ *      optimizer.rs:60-65, GpuMiLevel wgpu_backend/mi.rs:144-524) ---- */
+/* points: 3m physical LPS coordinates, fixed_bin: m fixed-axis bins (PreparedSamples,
+ * linear.rs:240-279). The moving image handle must outlive the level. */
+int fw_mi_level_create(fw_ctx* ctx, const fw_image* moving, const double* points, const uint32_t* fixed_bin, int64_t m,
+                       const fw_mi_binning* binning, fw_mi_level** out);
+int fw_mi_level_value(fw_mi_level* level, const fw_linear_transform* t, double* value);
+/* derivative receives 6 or 12 values. */
+int fw_mi_level_value_and_derivative(fw_mi_level* level, const fw_linear_transform* t, double* value,
+                                     double* derivative);
+void fw_mi_level_free(fw_mi_level* level);
+
+/* ---- whole registrations (stages.rs:286 register_stages_async with the HIP backend) ---- */
+
+/* The stage plan of one full normalisation: centre of mass, Rigid, Affine (Mattes MI), SyN (CC).
+ * Per-level vectors hold `levels` entries. */
+typedef struct {
+  int32_t linear_levels;
+  uint32_t linear_iterations[8];
+  uint32_t linear_shrink[8];
+  double linear_sigmas[8];
+  double linear_gradient_step;
+  int32_t mi_bins;
+  double sampling_percentage;
+  uint32_t seed;
+  double linear_convergence_threshold;
+  int32_t linear_convergence_window;
+
+  int32_t syn_levels;
+  uint32_t syn_iterations[8];
+  uint32_t syn_shrink[8];
+  double syn_sigmas[8];
+  double syn_gradient_step;
+  double update_variance;
+  double total_variance;
+  int32_t cc_radius;
+  int32_t channels;
+  double channel_weights[8];
+  double syn_convergence_threshold;
+  int32_t syn_convergence_window;
+
+  double winsorize_lower, winsorize_upper;
+  int32_t run_linear; /* 0: identity affine start, SyN only */
+  int32_t run_syn;    /* 0: linear stages only */
+} fw_plan;
+
+/* Summary of one registration: the collapsed affine, per-phase seconds and per-level counts. */
+typedef struct {
+  fw_affine affine;
+  double seconds_total, seconds_preprocess, seconds_linear, seconds_syn;
+  int32_t syn_levels;
+  int32_t syn_iterations[8];
+  double syn_last_energy[8];
+  int32_t linear_iterations[2][8];
+  double linear_final_value[2];
+  double rigid_parameters[6];
+  double affine_parameters[12];
+} fw_result;
+
+/* Fills `plan` with the cohort recipe: Rigid/Affine MI 32 bins, 25 % regular sampling,
+ * 1000x500x250x100, shrink 8/4/2/1, sigmas 3/2/1/0 vox; SyN[0.1,3,0], CC radius 4,
+ * 100x70x50x20, shrink 8/4/2/1, sigmas 3/2/1/0 vox (linear.rs:112-128, recipe.rs:93-112). */
+void fw_plan_cohort(fw_plan* plan, int32_t channels);
+
+/* One full registration of `channels` fixed/moving pairs (each fixed image on `fixed_grid`, each
+ * moving on `moving_grid`, f64 host arrays) on the context's device. forward / inverse (3n each
+ * on the fixed grid, may be NULL) receive the SyN fields (1Warp / 1InverseWarp). `cancel` may be
+ * NULL; it is polled per level and per iteration. */
+int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,
+                const fw_grid* moving_grid, const double* const* moving, double* forward, double* inverse,
+                fw_result* result, const volatile int32_t* cancel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTWARP_HIP_H */

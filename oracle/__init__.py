@@ -1,0 +1,178 @@
+"""ORACLE — test infrastructure only.
+
+ctypes bindings of the f64 CPU restatement of the reference's CPU engine (oracle/oracle.hpp). Only
+the tests, the smoke check and the CPU-baseline leg of the benchmark import this package, and only
+as the checker; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from fastwarp_amd._abi import Affine, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, aos, dptr, fptr, u32ptr
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfastwarp_oracle.so"
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            from . import build as _b
+
+            _b.build()
+        h = C.CDLL(str(LIB_PATH))
+        h.fwo_last_error.restype = C.c_char_p
+        h.fwo_bessel.restype = C.c_double
+        h.fwo_bessel.argtypes = [C.c_int, C.c_double]
+        h.fwo_cubic_bspline.restype = C.c_double
+        h.fwo_cubic_bspline.argtypes = [C.c_double, C.c_int]
+        _lib = h
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(st: int):
+    if st != 0:
+        raise OracleError(lib().fwo_last_error().decode())
+
+
+def warp(src_grid: Grid, src, field_grid: Grid, field, pre_grid: Grid | None = None, pre=None, affine: Affine | None = None,
+         nearest: bool = False, product=None) -> np.ndarray:
+    s = np.ascontiguousarray(src, dtype=np.float64)
+    f = aos(field)
+    p = aos(pre) if pre is not None else None
+    pr = np.ascontiguousarray(product, dtype=np.float64) if product is not None else None
+    out = np.empty(field_grid.n)
+    _check(
+        lib().fwo_warp(
+            C.byref(src_grid), dptr(s), C.byref(field_grid), dptr(f), C.byref(pre_grid) if pre_grid else None,
+            dptr(p) if p is not None else None, C.byref(affine) if affine else None, int(nearest),
+            dptr(pr) if pr is not None else None, dptr(out),
+        )
+    )
+    return out
+
+
+def cc(grid: Grid, fixed, moving, radius: int, *, moving_grid: Grid | None = None, moving_affine: Affine | None = None,
+       mask=None, derivative: bool = True) -> dict:
+    """CcMetric::evaluate_masked (metric/cc.rs:170-321)."""
+    mg = moving_grid or grid
+    f = np.ascontiguousarray(fixed, dtype=np.float64)
+    m = np.ascontiguousarray(moving, dtype=np.float64)
+    mk = np.ascontiguousarray(mask, dtype=np.float64) if mask is not None else None
+    n = grid.n
+    value = C.c_double()
+    nv = C.c_int64()
+    local = np.zeros(n)
+    coeff = np.zeros(n)
+    deriv = np.zeros((n, 3))
+    valid = np.zeros(n, dtype=np.uint8)
+    _check(
+        lib().fwo_cc(
+            C.byref(grid), dptr(f), C.byref(mg), dptr(m), C.byref(moving_affine) if moving_affine else None,
+            dptr(mk) if mk is not None else None, int(radius), int(derivative), C.byref(value), C.byref(nv),
+            dptr(local), dptr(coeff), dptr(deriv), valid.ctypes.data_as(C.POINTER(C.c_uint8)),
+        )
+    )
+    return {"value": value.value, "num_valid": nv.value, "local_cc": local, "coefficient": coeff, "derivative": deriv,
+            "valid": valid.astype(bool)}
+
+
+def mi_dense(fixed_grid: Grid, fixed, moving_grid: Grid, moving, bins: int, *, field_grid: Grid | None = None, field=None,
+             fixed_mask=None, moving_mask=None, derivative: bool = True) -> dict:
+    """MattesMi::evaluate_dense (metric/mi.rs:578-676)."""
+    f = np.ascontiguousarray(fixed, dtype=np.float64)
+    m = np.ascontiguousarray(moving, dtype=np.float64)
+    fl = np.ascontiguousarray(field, dtype=np.float32).reshape(-1) if field is not None else None
+    fm = np.ascontiguousarray(fixed_mask, dtype=np.float64) if fixed_mask is not None else None
+    mm = np.ascontiguousarray(moving_mask, dtype=np.float64) if moving_mask is not None else None
+    value = C.c_double()
+    nv = C.c_int64()
+    deriv = np.zeros((fixed_grid.n, 3))
+    _check(
+        lib().fwo_mi_dense(
+            C.byref(fixed_grid), dptr(f), C.byref(moving_grid), dptr(m), C.byref(field_grid) if field_grid else None,
+            fptr(fl) if fl is not None else None, dptr(fm) if fm is not None else None, dptr(mm) if mm is not None else None,
+            int(bins), int(derivative), C.byref(value), C.byref(nv), dptr(deriv),
+        )
+    )
+    return {"value": value.value, "num_valid": nv.value, "derivative": deriv}
+
+
+def mi_sampled(binning: MiBinning, moving_grid: Grid, moving, points, fixed_bin, t: LinearTransform, derivative=True) -> dict:
+    """MattesMi::evaluate with prepared fixed bins (metric/mi.rs:443-511)."""
+    m = np.ascontiguousarray(moving, dtype=np.float64)
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    b = np.ascontiguousarray(fixed_bin, dtype=np.uint32)
+    value = C.c_double()
+    nv = C.c_int64()
+    n = 6 if t.kind == 0 else 12
+    d = np.zeros(n)
+    pdf = np.zeros(binning.bins * binning.bins)
+    _check(
+        lib().fwo_mi_sampled(
+            C.byref(binning), C.byref(moving_grid), dptr(m), dptr(p), u32ptr(b), p.shape[0], C.byref(t), int(derivative),
+            C.byref(value), dptr(d), C.byref(nv), dptr(pdf),
+        )
+    )
+    return {"value": value.value, "derivative": d, "num_valid": nv.value, "joint_pdf": pdf}
+
+
+def gaussian_taps(variance: float, max_error: float = 0.001, max_width: int = 1000) -> np.ndarray:
+    out = np.zeros(4096)
+    n = C.c_int64()
+    _check(lib().fwo_gaussian_taps(C.c_double(variance), C.c_double(max_error), C.c_int64(max_width), dptr(out), 4096, C.byref(n)))
+    return out[: n.value].copy()
+
+
+def bessel(n: int, y: float) -> float:
+    return lib().fwo_bessel(n, y)
+
+
+def cubic_bspline(u: float, derivative: bool = False) -> float:
+    return lib().fwo_cubic_bspline(u, int(derivative))
+
+
+def smooth(grid: Grid, field, variance: float) -> np.ndarray:
+    f = aos(field)
+    out = np.empty_like(f)
+    _check(lib().fwo_smooth(C.byref(grid), dptr(f), C.c_double(variance), dptr(out)))
+    return out
+
+
+def compose(grid: Grid, displacement, warping) -> np.ndarray:
+    u, w = aos(displacement), aos(warping)
+    out = np.empty_like(u)
+    _check(lib().fwo_compose(C.byref(grid), dptr(u), dptr(w), dptr(out)))
+    return out
+
+
+def invert(grid: Grid, field, initial=None, params: InvertParams | None = None):
+    params = params or InvertParams.syn()
+    f = aos(field)
+    init = aos(initial) if initial is not None else None
+    out = np.empty_like(f)
+    st = InvertStats()
+    _check(lib().fwo_invert(C.byref(grid), dptr(f), dptr(init) if init is not None else None, C.byref(params), dptr(out), C.byref(st)))
+    return out, (int(st.iterations), st.mean_error_norm, st.max_error_norm)
+
+
+def resample(grid: Grid, field, target: Grid) -> np.ndarray:
+    f = aos(field)
+    out = np.empty((target.n, 3))
+    _check(lib().fwo_resample(C.byref(grid), dptr(f), C.byref(target), dptr(out)))
+    return out
+
+
+def norms(grid: Grid, field) -> tuple[float, float]:
+    f = aos(field)
+    a, b = C.c_double(), C.c_double()
+    _check(lib().fwo_norms(C.byref(grid), dptr(f), C.byref(a), C.byref(b)))
+    return a.value, b.value

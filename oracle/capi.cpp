@@ -1,0 +1,233 @@
+// ORACLE — test infrastructure only (see oracle.hpp). A flat C interface over the f64 restatement
+// so the Python tests can run the same inputs through the oracle and through the HIP library.
+// Grids use the fw_grid struct of include/fastwarp_hip.h; images are f64[n], fields f64[3n]
+// interleaved (dx, dy, dz).
+#include <cstring>
+#include <string>
+
+#include "../include/fastwarp_hip.h"
+#include "oracle.hpp"
+
+using namespace fw;
+using namespace fw::oracle;
+
+namespace {
+
+thread_local std::string g_error;
+
+Grid grid_from(const fw_grid* g) {
+  Mat3 dir;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) dir[r][c] = g->direction[3 * r + c];
+  return Grid({size_t(g->size[0]), size_t(g->size[1]), size_t(g->size[2])}, {g->spacing[0], g->spacing[1], g->spacing[2]},
+              {g->origin[0], g->origin[1], g->origin[2]}, dir);
+}
+
+Image image_from(const fw_grid* g, const double* data) {
+  Grid gr = grid_from(g);
+  return Image(gr, std::vector<double>(data, data + gr.num_voxels()));
+}
+
+VectorField field_from(const fw_grid* g, const double* aos) {
+  Grid gr = grid_from(g);
+  std::vector<Vec3> d(gr.num_voxels());
+  for (size_t i = 0; i < d.size(); ++i) d[i] = {aos[3 * i], aos[3 * i + 1], aos[3 * i + 2]};
+  return VectorField(gr, std::move(d));
+}
+
+void field_to(const VectorField& f, double* aos) {
+  for (size_t i = 0; i < f.data.size(); ++i)
+    for (int c = 0; c < 3; ++c) aos[3 * i + c] = f.data[i][c];
+}
+
+AffineTransform affine_from(const fw_affine* a) {
+  AffineTransform t;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) t.matrix[r][c] = a->matrix[3 * r + c];
+    t.translation[r] = a->translation[r];
+    t.center[r] = a->center[r];
+  }
+  return t;
+}
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fwo_last_error() { return g_error.c_str(); }
+
+int fwo_warp(const fw_grid* src_grid, const double* src, const fw_grid* field_grid, const double* field,
+             const fw_grid* pre_grid, const double* pre, const fw_affine* affine, int nearest, const double* product,
+             double* out) {
+  return guard([&] {
+    Image s = image_from(src_grid, src);
+    VectorField f = field_from(field_grid, field);
+    VectorField p;
+    if (pre) p = field_from(pre_grid, pre);
+    AffineTransform a;
+    if (affine) a = affine_from(affine);
+    Image r;
+    if (nearest) {
+      Image prod;
+      if (product) prod = image_from(field_grid, product);
+      r = warp_mask_through_field(s, f, pre ? &p : nullptr, affine ? &a : nullptr, product ? &prod : nullptr);
+    } else {
+      r = warp_through_field(s, f, pre ? &p : nullptr, affine ? &a : nullptr);
+    }
+    std::memcpy(out, r.data.data(), r.data.size() * sizeof(double));
+  });
+}
+
+// CC on a virtual grid; moving_affine (may be null) maps virtual points to moving points.
+int fwo_cc(const fw_grid* grid, const double* fixed, const fw_grid* moving_grid, const double* moving,
+           const fw_affine* moving_affine, const double* mask, int radius, int want_derivative, double* value,
+           int64_t* num_valid, double* local_cc, double* coefficient, double* derivative, uint8_t* valid) {
+  return guard([&] {
+    Grid g = grid_from(grid);
+    Image f = image_from(grid, fixed);
+    Image m = image_from(moving_grid, moving);
+    Image mk;
+    if (mask) mk = image_from(grid, mask);
+    AffineTransform a;
+    PointMap id, mm;
+    if (moving_affine) {
+      a = affine_from(moving_affine);
+      mm.affine = &a;
+    }
+    CcEvaluation e = cc_evaluate(g, f, m, id, mm, mask ? &mk : nullptr, size_t(radius), want_derivative != 0);
+    *value = e.value;
+    if (num_valid) *num_valid = int64_t(e.num_valid);
+    const size_t n = g.num_voxels();
+    if (local_cc) std::memcpy(local_cc, e.local_cc.data(), n * sizeof(double));
+    if (coefficient) std::memcpy(coefficient, e.coefficient.data(), n * sizeof(double));
+    if (valid) std::memcpy(valid, e.valid.data(), n);
+    if (derivative && want_derivative)
+      for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) derivative[3 * i + c] = e.derivative[i][c];
+  });
+}
+
+// Dense Mattes MI on the fixed grid as virtual grid; `field` (may be null) is an f32
+// displacement-field moving transform on `field_grid` (3 floats per voxel interleaved).
+int fwo_mi_dense(const fw_grid* fixed_grid, const double* fixed, const fw_grid* moving_grid, const double* moving,
+                 const fw_grid* field_grid, const float* field, const double* fixed_mask, const double* moving_mask,
+                 int bins, int want_derivative, double* value, int64_t* num_valid, double* derivative) {
+  return guard([&] {
+    Grid g = grid_from(fixed_grid);
+    Image f = image_from(fixed_grid, fixed);
+    Image m = image_from(moving_grid, moving);
+    Image fm, mm;
+    if (fixed_mask) fm = image_from(fixed_grid, fixed_mask);
+    if (moving_mask) mm = image_from(moving_grid, moving_mask);
+    DisplacementTransform dt;
+    PointMap id, mmap;
+    if (field) {
+      dt.grid = grid_from(field_grid);
+      dt.data.assign(field, field + 3 * dt.grid.num_voxels());
+      mmap.field = &dt;
+    }
+    MiDenseEvaluation e = mi_dense(g, f, m, id, mmap, fixed_mask ? &fm : nullptr, moving_mask ? &mm : nullptr, size_t(bins),
+                                   want_derivative != 0);
+    *value = e.value;
+    if (num_valid) *num_valid = int64_t(e.num_valid);
+    if (derivative && want_derivative)
+      for (size_t i = 0; i < e.derivative.size(); ++i)
+        for (int c = 0; c < 3; ++c) derivative[3 * i + c] = e.derivative[i][c];
+  });
+}
+
+// Sampled (linear-stage) Mattes MI with prepared fixed bins.
+int fwo_mi_sampled(const fw_mi_binning* b, const fw_grid* moving_grid, const double* moving, const double* points,
+                   const uint32_t* fixed_bin, int64_t m, const fw_linear_transform* t, int want_derivative, double* value,
+                   double* derivative, int64_t* num_valid, double* joint_pdf) {
+  return guard([&] {
+    Binning bn;
+    bn.bins = size_t(b->bins);
+    bn.fixed_min = b->fixed_min;
+    bn.fixed_max = b->fixed_max;
+    bn.moving_min = b->moving_min;
+    bn.moving_max = b->moving_max;
+    bn.fixed_bin_size = b->fixed_bin_size;
+    bn.moving_bin_size = b->moving_bin_size;
+    bn.fixed_normalized_min = b->fixed_normalized_min;
+    bn.moving_normalized_min = b->moving_normalized_min;
+    Image mv = image_from(moving_grid, moving);
+    std::vector<Vec3> pts(static_cast<size_t>(m));
+    for (int64_t i = 0; i < m; ++i) pts[size_t(i)] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    std::vector<uint32_t> fb(fixed_bin, fixed_bin + m);
+    LinearMovingTransform lt;
+    lt.outer = affine_from(&t->outer);
+    lt.optimised.kind = t->kind == 0 ? ParamKind::Euler3d : ParamKind::Affine;
+    for (int i = 0; i < 12; ++i) lt.optimised.params[i] = t->parameters[i];
+    lt.optimised.center = {t->center[0], t->center[1], t->center[2]};
+    MiSampledEvaluation e = mi_sampled(bn, mv, pts, fb, lt, want_derivative != 0);
+    *value = e.value;
+    if (num_valid) *num_valid = int64_t(e.num_valid);
+    if (derivative && want_derivative) std::memcpy(derivative, e.derivative.data(), e.derivative.size() * sizeof(double));
+    if (joint_pdf) std::memcpy(joint_pdf, e.hist.joint_pdf.data(), e.hist.joint_pdf.size() * sizeof(double));
+  });
+}
+
+int fwo_gaussian_taps(double variance, double max_error, int64_t max_width, double* out, int64_t capacity, int64_t* len) {
+  return guard([&] {
+    auto t = gaussian_taps(variance, max_error, size_t(max_width));
+    *len = int64_t(t.size());
+    if (int64_t(t.size()) <= capacity) std::memcpy(out, t.data(), t.size() * sizeof(double));
+  });
+}
+
+double fwo_bessel(int n, double y) { return n == 0 ? bessel_i0(y) : n == 1 ? bessel_i1(y) : bessel_i(n, y); }
+
+int fwo_smooth(const fw_grid* grid, const double* field, double variance, double* out) {
+  return guard([&] { field_to(smooth_field(field_from(grid, field), variance), out); });
+}
+
+int fwo_compose(const fw_grid* grid, const double* displacement, const double* warping, double* out) {
+  return guard([&] { field_to(compose(field_from(grid, displacement), field_from(grid, warping)), out); });
+}
+
+int fwo_invert(const fw_grid* grid, const double* field, const double* initial, const fw_invert_params* p, double* out,
+               fw_invert_stats* stats) {
+  return guard([&] {
+    VectorField f = field_from(grid, field);
+    VectorField init;
+    if (initial) init = field_from(grid, initial);
+    InvertParams q;
+    q.max_iterations = p->max_iterations;
+    q.mean_error_tolerance = p->mean_error_tolerance;
+    q.max_error_tolerance = p->max_error_tolerance;
+    q.enforce_boundary_condition = p->enforce_boundary_condition != 0;
+    auto r = invert(f, initial ? &init : nullptr, q);
+    field_to(r.first, out);
+    stats->iterations = r.second.iterations;
+    stats->mean_error_norm = r.second.mean_error_norm;
+    stats->max_error_norm = r.second.max_error_norm;
+  });
+}
+
+int fwo_resample(const fw_grid* grid, const double* field, const fw_grid* target, double* out) {
+  return guard([&] { field_to(resample_field(field_from(grid, field), grid_from(target)), out); });
+}
+
+int fwo_norms(const fw_grid* grid, const double* field, double* max_scaled, double* l2) {
+  return guard([&] {
+    VectorField f = field_from(grid, field);
+    *max_scaled = max_scaled_norm(f);
+    *l2 = l2_norm(f);
+  });
+}
+
+double fwo_cubic_bspline(double u, int derivative) { return derivative ? cubic_bspline_derivative(u) : cubic_bspline(u); }
+
+}  // extern "C"
