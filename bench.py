@@ -1,0 +1,160 @@
+"""Benchmark of the plain HIP registration path on one MI355X.
+
+A step is one full 1 mm normalisation (centre of mass, Rigid and Affine Mattes MI, SyN CC radius 4
+with the cohort schedule 8x4x2x1) of the seeded 182x218x182 phantom pair. W untimed warmup steps,
+then K timed steps; prints one JSON line with registrations/hour, per-step latency, the roofline of
+the dominant kernel (HIP events over the timed region) and the f64 CPU engine as a baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else "registrations/hour"
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def roofline(profile: dict) -> dict:
+    """The kernel with the largest total time: algorithmic bytes per launch over its mean duration."""
+    name, (launches, ms, nbytes) = max(profile.items(), key=lambda kv: kv[1][1])
+    per_launch_ms = ms / launches
+    achieved = (nbytes / launches) / (per_launch_ms * 1e-3) / 1e9
+    return {
+        "kernel": name,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "launches": launches,
+        "mean_launch_us": round(per_launch_ms * 1e3, 2),
+        "share_of_gpu_time": round(ms / sum(v[1] for v in profile.values()), 4),
+    }
+
+
+def cpu_baseline(pair, gpu_result) -> dict:
+    """The f64 CPU engine (the oracle) on a bounded sample of the same workload: one iteration per
+    level of every stage on the 1 mm pair, timed per level, extrapolated with the iteration and
+    evaluation counts of the GPU run (set-up time of a level counted once)."""
+    import oracle
+    from fastwarp_amd.register import cohort_plan
+
+    plan = cohort_plan(1)
+    plan.set_linear([1, 1, 1, 1], [8, 4, 2, 1], [3.0, 2.0, 1.0, 0.0])
+    plan.set_syn([1, 1, 1, 1], [8, 4, 2, 1], [3.0, 2.0, 1.0, 0.0])
+    t0 = time.perf_counter()
+    res, _, _ = oracle.register(plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=False)
+    wall = time.perf_counter() - t0
+    est = res.seconds_preprocess
+    for s in range(2):
+        for lv in range(4):
+            evals = max(res.linear_evaluations[s][lv], 1)
+            per_eval = res.linear_iteration_seconds[s][lv] / evals
+            setup = res.linear_level_seconds[s][lv] - res.linear_iteration_seconds[s][lv]
+            est += setup + per_eval * gpu_result.linear_evaluations[s][lv]
+    for lv in range(res.syn_levels):
+        per_it = res.syn_iteration_seconds[lv] / max(res.syn_iterations[lv], 1)
+        setup = res.syn_level_seconds[lv] - res.syn_iteration_seconds[lv]
+        est += setup + per_it * gpu_result.syn_iterations[lv]
+    threads = int(os.environ.get("FW_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(3600.0 / est, 3),
+        "unit": "registrations/hour",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"f64 CPU engine, 1 CG iteration and 1 SyN iteration per level on the 1 mm pair "
+        f"({wall:.1f} s measured), extrapolated to the GPU run's iteration/evaluation counts: {est:.1f} s per registration",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-engine baseline")
+    ap.add_argument("--config", default="1mm", choices=["1mm", "2mm"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box of this build has a single MI355X (asked for {args.gpus})", file=sys.stderr)
+        return 2
+
+    from fastwarp_amd.backend import HipBackend
+    from fastwarp_amd.phantom import CONFIGS, make_pair
+    from fastwarp_amd.register import cohort_plan, plan_2mm, register
+
+    cfg = CONFIGS[args.config]
+    t = time.perf_counter()
+    pair = make_pair(**cfg)
+    log(f"phantom {cfg['shape_xyz']} at {cfg['spacing']} mm ready in {time.perf_counter() - t:.1f} s")
+    plan = cohort_plan(1) if args.config == "1mm" else plan_2mm(1)
+    backend = HipBackend(0)
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        r = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=False)
+        log(f"warmup {w + 1}/{args.warmup}: {time.perf_counter() - t:.2f} s (linear {r.result.seconds_linear:.2f}, syn {r.result.seconds_syn:.2f})")
+    backend.set_profiling(True)
+    lat = []
+    last = None
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        last = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=False)
+        lat.append(time.perf_counter() - t)
+        log(f"step {k + 1}/{args.steps}: {lat[-1]:.2f} s (linear {last.result.seconds_linear:.2f}, syn {last.result.seconds_syn:.2f})")
+    total = time.perf_counter() - t_all
+    profile = backend.take_profile()
+    backend.set_profiling(False)
+    held, peak = backend.memory_stats()
+    line = {
+        "metric": METRIC,
+        "value": round(3600.0 * args.steps / total, 3),
+        "unit": "registrations/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * total / args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain full normalization {args.config}: centre of mass, Rigid+Affine Mattes MI (32 bins, 25% regular), "
+            f"SyN CC radius 4, grid {cfg['shape_xyz']} at {cfg['spacing']} mm, seeded phantom seed {cfg['seed']}",
+            "grid": list(cfg["shape_xyz"]),
+            "spacing_mm": cfg["spacing"],
+            "channels": 1,
+            "path": "plain one-at-a-time HIP",
+        },
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "phases_s": {"linear": round(last.result.seconds_linear, 3), "syn": round(last.result.seconds_syn, 3),
+                     "preprocess": round(last.result.seconds_preprocess, 3)},
+        "device_peak_bytes": int(peak),
+        "roofline": roofline(profile),
+        "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "gbs": round(v[2] / max(v[1], 1e-9) / 1e6, 1)}
+                    for k, v in sorted(profile.items(), key=lambda kv: -kv[1][1])},
+    }
+    if not args.no_cpu:
+        log("CPU-engine baseline (bounded sample)")
+        line["cpu_baseline"] = cpu_baseline(pair, last.result)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

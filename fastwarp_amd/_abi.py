@@ -157,3 +157,70 @@ def fptr(a: np.ndarray):
 
 def u32ptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class Plan(C.Structure):
+    """fw_plan: the stage plan of one full normalisation (include/fastwarp_hip.h)."""
+
+    _fields_ = [
+        ("linear_levels", C.c_int32),
+        ("linear_iterations", C.c_uint32 * 8),
+        ("linear_shrink", C.c_uint32 * 8),
+        ("linear_sigmas", C.c_double * 8),
+        ("linear_gradient_step", C.c_double),
+        ("mi_bins", C.c_int32),
+        ("sampling_percentage", C.c_double),
+        ("seed", C.c_uint32),
+        ("linear_convergence_threshold", C.c_double),
+        ("linear_convergence_window", C.c_int32),
+        ("syn_levels", C.c_int32),
+        ("syn_iterations", C.c_uint32 * 8),
+        ("syn_shrink", C.c_uint32 * 8),
+        ("syn_sigmas", C.c_double * 8),
+        ("syn_gradient_step", C.c_double),
+        ("update_variance", C.c_double),
+        ("total_variance", C.c_double),
+        ("cc_radius", C.c_int32),
+        ("channels", C.c_int32),
+        ("channel_weights", C.c_double * 8),
+        ("syn_convergence_threshold", C.c_double),
+        ("syn_convergence_window", C.c_int32),
+        ("winsorize_lower", C.c_double),
+        ("winsorize_upper", C.c_double),
+        ("run_linear", C.c_int32),
+        ("run_syn", C.c_int32),
+    ]
+
+    def set_linear(self, iterations, shrink, sigmas):
+        self.linear_levels = len(iterations)
+        for i, (it, sh, sg) in enumerate(zip(iterations, shrink, sigmas)):
+            self.linear_iterations[i], self.linear_shrink[i], self.linear_sigmas[i] = it, sh, sg
+
+    def set_syn(self, iterations, shrink, sigmas):
+        self.syn_levels = len(iterations)
+        for i, (it, sh, sg) in enumerate(zip(iterations, shrink, sigmas)):
+            self.syn_iterations[i], self.syn_shrink[i], self.syn_sigmas[i] = it, sh, sg
+
+
+class Result(C.Structure):
+    """fw_result: the collapsed affine, phase timings and per-level counts of one registration."""
+
+    _fields_ = [
+        ("affine", Affine),
+        ("seconds_total", C.c_double),
+        ("seconds_preprocess", C.c_double),
+        ("seconds_linear", C.c_double),
+        ("seconds_syn", C.c_double),
+        ("syn_levels", C.c_int32),
+        ("syn_iterations", C.c_int32 * 8),
+        ("syn_last_energy", C.c_double * 8),
+        ("linear_iterations", (C.c_int32 * 8) * 2),
+        ("linear_final_value", C.c_double * 2),
+        ("rigid_parameters", C.c_double * 6),
+        ("affine_parameters", C.c_double * 12),
+        ("syn_level_seconds", C.c_double * 8),
+        ("syn_iteration_seconds", C.c_double * 8),
+        ("linear_level_seconds", (C.c_double * 8) * 2),
+        ("linear_iteration_seconds", (C.c_double * 8) * 2),
+        ("linear_evaluations", (C.c_int64 * 8) * 2),
+    ]

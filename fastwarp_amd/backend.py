@@ -14,7 +14,7 @@ from pathlib import Path
 
 import numpy as np
 
-from ._abi import Affine, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, aos, dptr, u32ptr
+from ._abi import Affine, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, Plan, Result, aos, dptr, u32ptr
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfastwarp_hip.so"
 
@@ -76,6 +76,15 @@ _SIGNATURES = [
     ("fw_mi_level_value", C.c_int, [_P, C.POINTER(LinearTransform), _D]),
     ("fw_mi_level_value_and_derivative", C.c_int, [_P, C.POINTER(LinearTransform), _D, _D]),
     ("fw_mi_level_free", None, [_P]),
+    ("fw_set_profiling", C.c_int, [_P, C.c_int]),
+    ("fw_take_profile", C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    ("fw_plan_cohort", None, [C.POINTER(Plan), C.c_int32]),
+    (
+        "fw_register",
+        C.c_int,
+        [_P, C.POINTER(Plan), C.POINTER(Grid), C.POINTER(C.POINTER(C.c_double)), C.POINTER(Grid),
+         C.POINTER(C.POINTER(C.c_double)), _D, _D, C.POINTER(Result), C.POINTER(C.c_int32)],
+    ),
 ]
 
 
@@ -173,6 +182,19 @@ class HipBackend:
 
     def name(self) -> str:
         return "hip-gfx950"
+
+    def set_profiling(self, on: bool):
+        self._check(lib().fw_set_profiling(self.ctx, int(on)))
+
+    def take_profile(self) -> dict[str, tuple[int, float, float]]:
+        """Per kernel family: (launches, total milliseconds, algorithmic bytes)."""
+        buf = C.create_string_buffer(1 << 16)
+        self._check(lib().fw_take_profile(self.ctx, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, launches, ms, nbytes = line.split()
+            out[name] = (int(launches), float(ms), float(nbytes))
+        return out
 
     def _check(self, status: int):
         if status != FW_OK:

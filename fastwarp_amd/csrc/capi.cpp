@@ -7,6 +7,7 @@
 #include <string>
 
 #include "hip/backend.hpp"
+#include "plan.hpp"
 
 using fw::Grid;
 using fw::hip::HField;
@@ -23,6 +24,10 @@ struct fw_image {
 };
 struct fw_field {
   HField h;
+};
+struct fw_mi_level {
+  fw_ctx* ctx;
+  fw::hip::MiLevel level;
 };
 
 namespace {
@@ -240,6 +245,88 @@ int fw_cc_pair(fw_ctx* ctx, const fw_image* fixed, const fw_image* moving, const
   });
 }
 
+int fw_mi_dense_pair(fw_ctx* ctx, const fw_image* fixed, const fw_image* moving, const fw_image* fixed_mask,
+                     const fw_image* moving_mask, uint32_t bins, double values[2], double norms[2], fw_field** d_fixed,
+                     fw_field** d_moving) {
+  return guard(ctx, [&] {
+    need(fixed, "fixed");
+    need(moving, "moving");
+    need(values, "values");
+    need(d_fixed, "output");
+    need(d_moving, "output");
+    auto r = ctx->backend.mi_pair(fixed->h, moving->h, fixed_mask ? &fixed_mask->h : nullptr,
+                                  moving_mask ? &moving_mask->h : nullptr, bins, norms != nullptr);
+    values[0] = r.values[0];
+    values[1] = r.values[1];
+    if (norms) {
+      norms[0] = r.norms[0];
+      norms[1] = r.norms[1];
+    }
+    *d_fixed = new fw_field{std::move(r.fields[0])};
+    *d_moving = new fw_field{std::move(r.fields[1])};
+  });
+}
+
+int fw_mi_level_create(fw_ctx* ctx, const fw_image* moving, const double* points, const uint32_t* fixed_bin, int64_t m,
+                       const fw_mi_binning* binning, fw_mi_level** out) {
+  return guard(ctx, [&] {
+    need(moving, "moving");
+    need(binning, "binning");
+    need(out, "output");
+    if (m < 0) fw::invalid("mattes", "negative sample count");
+    if (m > 0) {
+      need(points, "points");
+      need(fixed_bin, "fixed bins");
+    }
+    std::vector<fw::Vec3> pts(static_cast<size_t>(m));
+    for (int64_t i = 0; i < m; ++i) pts[size_t(i)] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    std::vector<uint32_t> fb(fixed_bin, fixed_bin + m);
+    fw::MiBinning b;
+    b.bins = size_t(binning->bins);
+    b.fixed_min = binning->fixed_min;
+    b.fixed_max = binning->fixed_max;
+    b.moving_min = binning->moving_min;
+    b.moving_max = binning->moving_max;
+    b.fixed_bin_size = binning->fixed_bin_size;
+    b.moving_bin_size = binning->moving_bin_size;
+    b.fixed_normalized_min = binning->fixed_normalized_min;
+    b.moving_normalized_min = binning->moving_normalized_min;
+    *out = new fw_mi_level{ctx, fw::hip::MiLevel(ctx->backend, moving->h, pts, fb, b)};
+  });
+}
+
+static fw::LinearMovingTransform linear_from(const fw_linear_transform* t) {
+  if (!t) fw::invalid("mattes", "null transform");
+  fw::LinearMovingTransform lt;
+  lt.outer = affine_from(&t->outer);
+  lt.optimised.kind = t->kind == 0 ? fw::ParamKind::Euler3d : fw::ParamKind::Affine;
+  for (int i = 0; i < 12; ++i) lt.optimised.params[i] = t->parameters[i];
+  lt.optimised.center = {t->center[0], t->center[1], t->center[2]};
+  return lt;
+}
+
+int fw_mi_level_value(fw_mi_level* level, const fw_linear_transform* t, double* value) {
+  if (!level) return FW_ERR_INVALID;
+  return guard(level->ctx, [&] {
+    need(value, "output");
+    *value = level->level.value(linear_from(t));
+  });
+}
+
+int fw_mi_level_value_and_derivative(fw_mi_level* level, const fw_linear_transform* t, double* value,
+                                     double* derivative) {
+  if (!level) return FW_ERR_INVALID;
+  return guard(level->ctx, [&] {
+    need(value, "output");
+    need(derivative, "output");
+    auto r = level->level.value_and_derivative(linear_from(t));
+    *value = r.first;
+    std::memcpy(derivative, r.second.data(), r.second.size() * sizeof(double));
+  });
+}
+
+void fw_mi_level_free(fw_mi_level* level) { delete level; }
+
 int fw_smooth_field(fw_ctx* ctx, fw_field* field, double variance, fw_field** out) {
   return guard(ctx, [&] {
     need(field, "field");
@@ -351,6 +438,40 @@ int fw_resample_field(fw_ctx* ctx, const fw_field* field, const fw_grid* grid, f
     need(field, "field");
     need(out, "output");
     *out = new fw_field{ctx->backend.resample_field(field->h, grid_from(grid))};
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ whole registrations ---
+
+extern "C" {
+
+int fw_set_profiling(fw_ctx* ctx, int on) {
+  return guard(ctx, [&] { ctx->backend.ctx().set_profiling(on != 0); });
+}
+
+int fw_take_profile(fw_ctx* ctx, char* text, size_t capacity) {
+  return guard(ctx, [&] {
+    need(text, "output");
+    const std::string p = ctx->backend.ctx().take_profile();
+    if (capacity == 0) fw::invalid("profile", "zero capacity");
+    const size_t n = std::min(p.size(), capacity - 1);
+    std::memcpy(text, p.data(), n);
+    text[n] = '\0';
+  });
+}
+
+void fw_plan_cohort(fw_plan* plan, int32_t channels) {
+  if (plan) fw::cohort_plan(plan, channels);
+}
+
+int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,
+                const fw_grid* moving_grid, const double* const* moving, double* forward, double* inverse,
+                fw_result* result, const volatile int32_t* cancel) {
+  return guard(ctx, [&] {
+    fw::hip::HipLinearEngine engine(ctx->backend);
+    fw::run_register(ctx->backend, engine, plan, fixed_grid, fixed, moving_grid, moving, forward, inverse, result, cancel);
   });
 }
 

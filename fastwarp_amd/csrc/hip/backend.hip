@@ -6,6 +6,7 @@
 #include <cmath>
 
 #include "../host/gauss_taps.hpp"
+#include "geometry.hpp"
 
 namespace fw::hip {
 
@@ -26,6 +27,11 @@ DeviceContext::~DeviceContext() {
     if (s.p) hipFree(s.p);
   if (pinned_) hipHostFree(pinned_);
   for (auto& kv : taps_) hipFree(kv.second.first);
+  for (auto& s : spans_) {
+    (void)hipEventDestroy(s.a);
+    (void)hipEventDestroy(s.b);
+  }
+  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(stream_);
 }
 
@@ -111,6 +117,50 @@ const float* DeviceContext::taps(double variance, size_t length, int& radius) {
   return static_cast<const float*>(it->second.first);
 }
 
+void DeviceContext::set_profiling(bool on) {
+  if (!on && profiling_) (void)take_profile();
+  profiling_ = on;
+}
+
+hipEvent_t DeviceContext::event() {
+  if (!events_.empty()) {
+    hipEvent_t e = events_.back();
+    events_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  check(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
+void DeviceContext::span(const char* name, double bytes, hipEvent_t a, hipEvent_t b) { spans_.push_back({name, bytes, a, b}); }
+
+void DeviceContext::add_bytes(const char* name, double bytes) { totals_[name].bytes += bytes; }
+
+std::string DeviceContext::take_profile() {
+  sync();
+  for (Span& s : spans_) {
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, s.a, s.b), "hipEventElapsedTime");
+    Total& t = totals_[s.name];
+    t.launches += 1;
+    t.ms += double(ms);
+    t.bytes += s.bytes;
+    events_.push_back(s.a);
+    events_.push_back(s.b);
+  }
+  spans_.clear();
+  std::string out;
+  char line[256];
+  for (auto& kv : totals_) {
+    std::snprintf(line, sizeof line, "%s %llu %.6f %.0f\n", kv.first.c_str(), (unsigned long long)kv.second.launches, kv.second.ms,
+                  kv.second.bytes);
+    out += line;
+  }
+  totals_.clear();
+  return out;
+}
+
 DevBuf::DevBuf(DeviceContext* ctx, size_t floats) : ctx_(ctx), bytes_(floats * sizeof(float)) {
   p_ = static_cast<float*>(ctx->acquire(bytes_));
 }
@@ -122,63 +172,6 @@ void DevBuf::reset() {
 
 // ---------------------------------------------------------------------------- geometry ---
 
-namespace {
-
-Dims dims_of(const Grid& g) { return {int(g.size[0]), int(g.size[1]), int(g.size[2])}; }
-
-Mat3 scaled_direction(const Grid& g) {
-  Mat3 m = g.direction;
-  for (auto& row : m)
-    for (int c = 0; c < 3; ++c) row[c] *= g.spacing[c];
-  return m;
-}
-
-void put(float* dst, const Mat3& m) {
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) dst[r * 3 + c] = float(m[r][c]);
-}
-
-void put(float* dst, const Vec3& v) {
-  for (int r = 0; r < 3; ++r) dst[r] = float(v[r]);
-}
-
-float3 inv_spacing(const Grid& g) {
-  return make_float3(float(1.0 / g.spacing[0]), float(1.0 / g.spacing[1]), float(1.0 / g.spacing[2]));
-}
-
-// The central-difference probe offsets of a grid in index units: +-half a spacing along each
-// physical axis d maps to index_from_offset * (h_d e_d) (interp.rs:102-121).
-Probes probes_of(const Grid& g) {
-  Probes p{};
-  bool diagonal = true;
-  for (int d = 0; d < 3; ++d) {
-    const double h = 0.5 * g.spacing[d];
-    int nonzero = 0;
-    for (int r = 0; r < 3; ++r) {
-      double v = g.index_from_offset[r][d] * h;
-      if (std::fabs(v) < 1e-12) v = 0.0;
-      if (v != 0.0) {
-        ++nonzero;
-        if (std::fabs(std::fabs(v) - 0.5) < 1e-9) v = v > 0 ? 0.5 : -0.5;
-        else diagonal = false;
-      }
-      p.o[d][r] = float(v);
-    }
-    if (nonzero != 1) diagonal = false;
-    p.inv_spacing[d] = float(1.0 / g.spacing[d]);
-  }
-  p.diagonal = diagonal ? 1 : 0;
-  return p;
-}
-
-float fixed_point_scale(float absmax) {
-  if (!(absmax > 0.0f) || !std::isfinite(absmax)) return float(1 << 22);
-  int e = 0;
-  std::frexp(double(absmax), &e);  // absmax < 2^e
-  return float(std::ldexp(1.0, 22 - e));
-}
-
-}  // namespace
 
 HipBackend::HipBackend(int device) : ctx_(std::make_shared<DeviceContext>(device)) {}
 
@@ -284,6 +277,8 @@ HImage HipBackend::warp_through_field(const HImage& src, const HField& field, co
   HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), src.absmax};
   a.src = src.buf.get();
   a.out = out.buf.get();
+  const double no = double(field.grid.num_voxels());
+  Timed t(*ctx_, "k_warp", (16.0 + (pre ? 12.0 : 0.0)) * no + 4.0 * std::min(no, double(src.grid.num_voxels())));
   launch_warp(a, ctx_->stream());
   return out;
 }
@@ -299,11 +294,13 @@ HImage HipBackend::warp_mask_through_field(const HImage& mask, const HField& fie
     a.product = out.buf.get();
     a.out = out.buf.get();
     out.absmax = 1.0f;
+    Timed t(*ctx_, "k_warp_mask", 24.0 * double(field.grid.num_voxels()));
     launch_warp(a, ctx_->stream());
     return out;
   }
   HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), 1.0f};
   a.out = out.buf.get();
+  Timed t(*ctx_, "k_warp_mask", 20.0 * double(field.grid.num_voxels()));
   launch_warp(a, ctx_->stream());
   return out;
 }
@@ -338,7 +335,10 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
   const size_t blocks = size_t(gr.x) * gr.y * gr.z;
   a.partial_cc = static_cast<float*>(ctx_->scratch(0, blocks * sizeof(float)));
   a.partial_count = static_cast<int*>(ctx_->scratch(1, blocks * sizeof(int)));
-  launch_cc_pair(a, ctx_->stream());
+  {
+    Timed t(*ctx_, "k_cc_pair", (valid ? 60.0 : 56.0) * double(n));
+    launch_cc_pair(a, ctx_->stream());
+  }
   // the launched grid may be smaller than `blocks` (wider x tiles): count the real one
   const int tx = radius <= 5 ? 64 : 32;
   const size_t real = size_t((a.d.nx + tx - 1) / tx) * ((a.d.ny + 3) / 4) * ((a.d.nz + a.tile_z - 1) / a.tile_z);
@@ -349,6 +349,7 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
     nb = norm_blocks(long(n));
     pm = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
     ps = static_cast<float*>(ctx_->scratch(3, 2 * nb * sizeof(float)));
+    Timed t(*ctx_, "k_field_norms", 24.0 * double(n));
     launch_field_norms(out.fields[0].buf.get(), out.fields[1].buf.get(), long(n), inv_spacing(g), pm, ps, nb, ctx_->stream());
   }
   const size_t bytes = real * (sizeof(float) + sizeof(int)) + (want_norms ? 2 * nb * sizeof(float) : 0);
@@ -397,12 +398,22 @@ HField HipBackend::smooth_field(HField field, double variance) {
   const double w1 = variance < 0.5 ? 1.0 - variance / 0.5 : 1.0;
   const double w2 = 1.0 - w1;
   int r = 0;
+  const double nb = double(n);
   const float* tx = ctx_->taps(variance, g.size[0], r);
-  launch_gauss_axis(field.buf.get(), t1.get(), d, 0, tx, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
+  {
+    Timed t(*ctx_, "k_gauss_x", 24.0 * nb);
+    launch_gauss_axis(field.buf.get(), t1.get(), d, 0, tx, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
+  }
   const float* ty = ctx_->taps(variance, g.size[1], r);
-  launch_gauss_axis(t1.get(), t2.get(), d, 1, ty, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
+  {
+    Timed t(*ctx_, "k_gauss_y", 24.0 * nb);
+    launch_gauss_axis(t1.get(), t2.get(), d, 1, ty, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
+  }
   const float* tz = ctx_->taps(variance, g.size[2], r);
-  launch_gauss_axis(t2.get(), t1.get(), d, 2, tz, r, field.buf.get(), float(w1), float(w2), 1, ctx_->stream());
+  {
+    Timed t(*ctx_, "k_gauss_z", 36.0 * nb);
+    launch_gauss_axis(t2.get(), t1.get(), d, 2, tz, r, field.buf.get(), float(w1), float(w2), 1, ctx_->stream());
+  }
   return HField{g, std::move(t1)};
 }
 
@@ -416,7 +427,10 @@ std::pair<double, double> HipBackend::max_scaled_norm_pair(const HField& a, cons
   // the two fields may live on different grids only in spacing-free size; run each on its own
   const HField* fields[2] = {&a, b};
   for (int q = 0; q < (b ? 2 : 1); ++q) {
-    launch_field_norms(fields[q]->buf.get(), nullptr, n, inv_spacing(fields[q]->grid), pm, ps, nb, ctx_->stream());
+    {
+      Timed t(*ctx_, "k_field_norms", 12.0 * double(n));
+      launch_field_norms(fields[q]->buf.get(), nullptr, n, inv_spacing(fields[q]->grid), pm, ps, nb, ctx_->stream());
+    }
     float* host = static_cast<float*>(ctx_->pinned(nb * sizeof(float)));
     check(hipMemcpyAsync(host, pm, nb * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "norm partials");
     ctx_->sync();
@@ -436,7 +450,10 @@ double HipBackend::l2_norm(const HField& f) {
   const int nb = norm_blocks(n);
   float* pm = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
   float* ps = static_cast<float*>(ctx_->scratch(3, 2 * nb * sizeof(float)));
-  launch_field_norms(f.buf.get(), nullptr, n, inv_spacing(f.grid), pm, ps, nb, ctx_->stream());
+  {
+    Timed t(*ctx_, "k_field_norms", 12.0 * double(n));
+    launch_field_norms(f.buf.get(), nullptr, n, inv_spacing(f.grid), pm, ps, nb, ctx_->stream());
+  }
   float* host = static_cast<float*>(ctx_->pinned(nb * sizeof(float)));
   check(hipMemcpyAsync(host, ps, nb * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "l2 partials");
   ctx_->sync();
@@ -445,10 +462,14 @@ double HipBackend::l2_norm(const HField& f) {
   return std::sqrt(s);
 }
 
-void HipBackend::scale(HField& f, double s) { launch_scale(f.buf.get(), long(3 * f.grid.num_voxels()), float(s), ctx_->stream()); }
+void HipBackend::scale(HField& f, double s) {
+  Timed t(*ctx_, "k_scale", 24.0 * double(f.grid.num_voxels()));
+  launch_scale(f.buf.get(), long(3 * f.grid.num_voxels()), float(s), ctx_->stream());
+}
 
 void HipBackend::axpy(HField& acc, double a, const HField& x) {
   if (acc.grid != x.grid) invalid("field axpy", "grids differ");
+  Timed t(*ctx_, "k_axpy", 36.0 * double(acc.grid.num_voxels()));
   launch_axpy(acc.buf.get(), x.buf.get(), long(3 * acc.grid.num_voxels()), float(a), ctx_->stream());
 }
 
@@ -458,6 +479,7 @@ HField HipBackend::compose(const HField& displacement, const HField& warping) {
   HField out{g, DevBuf(ctx_.get(), 3 * g.num_voxels())};
   float l[9];
   put(l, g.index_from_offset);
+  Timed t(*ctx_, "k_compose", 36.0 * double(g.num_voxels()));
   launch_compose(displacement.buf.get(), warping.buf.get(), dims_of(g), l, out.buf.get(), ctx_->stream());
   return out;
 }
@@ -476,16 +498,31 @@ std::pair<HField, InvertStats> HipBackend::invert(const HField& field, HField in
   hipStream_t s = ctx_->stream();
   launch_inv_init(state, int(p.max_iterations), s);
   for (uint32_t step = 0; step < p.max_iterations; ++step) {
-    launch_inv_residual(field.buf.get(), initial.buf.get(), d, l, isp, state, psum, pmax, nb, s);
-    launch_inv_reduce(psum, pmax, nb, d.n(), state, int(step), int(p.max_iterations), p.max_error_tolerance,
-                      p.mean_error_tolerance, s);
-    launch_inv_update(field.buf.get(), initial.buf.get(), d, l, isp, state, int(step), p.enforce_boundary_condition ? 1 : 0, s);
+    {
+      Timed t(*ctx_, "k_inv_residual", 0.0);
+      launch_inv_residual(field.buf.get(), initial.buf.get(), d, l, isp, state, psum, pmax, nb, s);
+    }
+    {
+      Timed t(*ctx_, "k_inv_reduce", 0.0);
+      launch_inv_reduce(psum, pmax, nb, d.n(), state, int(step), int(p.max_iterations), p.max_error_tolerance,
+                        p.mean_error_tolerance, s);
+    }
+    {
+      Timed t(*ctx_, "k_inv_update", 0.0);
+      launch_inv_update(field.buf.get(), initial.buf.get(), d, l, isp, state, int(step), p.enforce_boundary_condition ? 1 : 0, s);
+    }
   }
   auto* host = static_cast<InvertState*>(ctx_->pinned(sizeof(InvertState)));
   check(hipMemcpyAsync(host, state, sizeof(InvertState), hipMemcpyDeviceToHost, s), "invert state");
   ctx_->sync();
   InvertStats st;
   st.iterations = uint32_t(host->executed);
+  if (ctx_->profiling()) {
+    // bytes only for the steps that ran: the residual reads psi and the field, the update reads
+    // both and writes psi
+    ctx_->add_bytes("k_inv_residual", 24.0 * double(d.n()) * host->executed);
+    ctx_->add_bytes("k_inv_update", 36.0 * double(d.n()) * host->executed);
+  }
   st.mean_error_norm = host->executed ? double(host->mean_error) : 0.0;
   st.max_error_norm = host->executed ? double(host->max_error) : 0.0;
   return {std::move(initial), st};
@@ -502,6 +539,7 @@ HField HipBackend::resample_field(const HField& field, const Grid& grid) {
   put(a.to_src.c0, mat3_mul_vec(sg.index_from_offset, vsub(grid.origin, sg.origin)));
   HField out{grid, DevBuf(ctx_.get(), 3 * grid.num_voxels())};
   a.out = out.buf.get();
+  Timed t(*ctx_, "k_resample", 12.0 * double(grid.num_voxels()) + 12.0 * std::min(double(grid.num_voxels()), double(sg.num_voxels())));
   launch_resample(a, ctx_->stream());
   return out;
 }

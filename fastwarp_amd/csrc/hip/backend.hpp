@@ -17,9 +17,12 @@
 #include <utility>
 #include <vector>
 
+#include "../host/backend_types.hpp"
 #include "../host/grid.hpp"
+#include "../host/mi_host.hpp"
 #include "../host/transform.hpp"
 #include "kernels.hpp"
+#include "mi_kernels.hpp"
 
 namespace fw::hip {
 
@@ -47,6 +50,15 @@ class DeviceContext {
   // Device copy of the Gaussian taps of (variance, axis length), cached.
   const float* taps(double variance, size_t length, int& radius);
 
+  // Per-kernel timing (the reference's profiler.rs:95-170): while on, launches are bracketed by
+  // events; take_profile() waits, adds the spans up per kernel family and clears them.
+  void set_profiling(bool on);
+  bool profiling() const { return profiling_; }
+  hipEvent_t event();
+  void span(const char* name, double bytes, hipEvent_t a, hipEvent_t b);
+  void add_bytes(const char* name, double bytes);
+  std::string take_profile();
+
  private:
   int device_;
   hipStream_t stream_{};
@@ -60,6 +72,45 @@ class DeviceContext {
   void* pinned_ = nullptr;
   size_t pinned_bytes_ = 0;
   std::map<std::pair<double, size_t>, std::pair<void*, int>> taps_;
+  bool profiling_ = false;
+  struct Span {
+    std::string name;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  std::vector<Span> spans_;
+  std::vector<hipEvent_t> events_;
+  struct Total {
+    uint64_t launches = 0;
+    double ms = 0.0, bytes = 0.0;
+  };
+  std::map<std::string, Total> totals_;
+};
+
+// Brackets the launches of its scope with events when the context is profiling.
+class Timed {
+ public:
+  Timed(DeviceContext& ctx, const char* name, double bytes) : ctx_(ctx), name_(name), bytes_(bytes) {
+    if (ctx_.profiling()) {
+      a_ = ctx_.event();
+      (void)hipEventRecord(a_, ctx_.stream());
+    }
+  }
+  ~Timed() {
+    if (a_) {
+      hipEvent_t b = ctx_.event();
+      (void)hipEventRecord(b, ctx_.stream());
+      ctx_.span(name_, bytes_, a_, b);
+    }
+  }
+  Timed(const Timed&) = delete;
+  Timed& operator=(const Timed&) = delete;
+
+ private:
+  DeviceContext& ctx_;
+  const char* name_;
+  double bytes_;
+  hipEvent_t a_ = nullptr;
 };
 
 // A pooled device buffer of f32 values (move-only; returns to the pool on destruction).
@@ -106,25 +157,9 @@ struct HField {
   DevBuf buf;
 };
 
-struct InvertParams {
-  uint32_t max_iterations = 20;
-  double mean_error_tolerance = 0.001;
-  double max_error_tolerance = 0.1;
-  bool enforce_boundary_condition = true;
-};
-
-struct InvertStats {
-  uint32_t iterations = 0;
-  double mean_error_norm = 0.0;
-  double max_error_norm = 0.0;
-};
-
-template <typename F>
-struct MetricPair {
-  double values[2] = {0, 0};
-  F fields[2];
-  double norms[2] = {0, 0};
-};
+using fw::InvertParams;
+using fw::InvertStats;
+using fw::MetricPair;
 
 class HipBackend {
  public:
@@ -153,6 +188,10 @@ class HipBackend {
   MetricPair<HField> cc_pair(const HImage& fixed, const HImage& moving, const HImage* valid, size_t radius,
                              bool want_norms);
 
+  // Dense Mattes MI in both directions (masks may be null; bins 5..48).
+  MetricPair<HField> mi_pair(const HImage& fixed, const HImage& moving, const HImage* fixed_mask, const HImage* moving_mask,
+                             size_t bins, bool want_norms);
+
   HField smooth_field(HField field, double variance);
   std::pair<double, double> max_scaled_norm_pair(const HField& a, const HField* b);
   double l2_norm(const HField& f);
@@ -167,6 +206,53 @@ class HipBackend {
 
  private:
   std::shared_ptr<DeviceContext> ctx_;
+};
+
+// The linear-stage metric of one level on the device (the reference's GpuMiLevel,
+// crates/fastwarp-gpu/src/wgpu_backend/mi.rs:144-524): sample records resident in HBM, one
+// histogram pass per value, plus one derivative pass with FP64 block reductions.
+class MiLevel {
+ public:
+  // `moving` must outlive the level. points: physical sample points; fixed_bin: their bins.
+  MiLevel(HipBackend& backend, const HImage& moving, const std::vector<Vec3>& points, const std::vector<uint32_t>& fixed_bin,
+          const MiBinning& binning);
+  double value(const LinearMovingTransform& t);
+  std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t);
+  size_t count() const { return count_; }
+
+ private:
+  double histogram(const LinearMovingTransform& t, MiSampledArgs& args, std::vector<double>* ratio);
+  DeviceContext* ctx_;
+  const HImage* moving_;
+  DevBuf samples_;
+  size_t count_ = 0;
+  Vec3 anchor_{0, 0, 0};
+  MiBinning binning_;
+};
+
+// A LinearEngine (linear.rs:288-320) whose levels own their uploaded moving image.
+class HipLinearEngine {
+ public:
+  explicit HipLinearEngine(HipBackend& b) : backend_(b) {}
+  struct Level {
+    std::unique_ptr<HImage> moving;
+    std::unique_ptr<MiLevel> metric;
+    double value(const LinearMovingTransform& t) { return metric->value(t); }
+    std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t) {
+      return metric->value_and_derivative(t);
+    }
+  };
+  Level prepare_level(const Image& moving, const std::vector<Vec3>& points, const std::vector<uint32_t>& fixed_bin,
+                      const MiBinning& binning) {
+    Level l;
+    l.moving = std::make_unique<HImage>(backend_.upload_image(moving));
+    l.metric = std::make_unique<MiLevel>(backend_, *l.moving, points, fixed_bin, binning);
+    return l;
+  }
+  std::string name() const { return "hip-gfx950"; }
+
+ private:
+  HipBackend& backend_;
 };
 
 }  // namespace fw::hip

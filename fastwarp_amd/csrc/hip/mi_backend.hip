@@ -1,0 +1,226 @@
+// Host side of the Mattes MI kernels: the linear-stage level (the reference's GpuMiLevel,
+// crates/fastwarp-gpu/src/wgpu_backend/mi.rs:144-524) and the dense SyN pair
+// (wgpu_backend/mi/dense.rs:241-487). The device returns exact integer histograms; the host
+// finishes them in f64 with the same code path for both (metric/mi.rs:702-769).
+#include <cstring>
+
+#include "backend.hpp"
+#include "geometry.hpp"
+
+namespace fw::hip {
+
+namespace {
+
+std::vector<double> joint_from(const unsigned long long* h, size_t bins) {
+  std::vector<double> j(bins * bins);
+  for (size_t i = 0; i < j.size(); ++i) j[i] = double(h[i]) / kMiFixedPoint;
+  return j;
+}
+
+std::vector<double> marginal_from(const unsigned long long* h, size_t bins) {
+  std::vector<double> m(bins);
+  for (size_t i = 0; i < bins; ++i) m[i] = double(h[bins * bins + i]);
+  return m;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------- linear stage level ---
+
+MiLevel::MiLevel(HipBackend& backend, const HImage& moving, const std::vector<Vec3>& points,
+                 const std::vector<uint32_t>& fixed_bin, const MiBinning& binning)
+    : ctx_(&backend.ctx()), moving_(&moving), count_(points.size()), binning_(binning) {
+  if (binning.bins < 5 || binning.bins > size_t(kMiMaxBins))
+    invalid("hip backend", "the linear Mattes metric supports 5 to 64 histogram bins, not " + std::to_string(binning.bins));
+  if (fixed_bin.size() != points.size()) invalid("hip backend", "one fixed bin per sample point is required");
+  if (points.size() > size_t(1) << 30) invalid("hip backend", "too many sample points");
+  Vec3 mean{0, 0, 0};
+  for (const Vec3& p : points)
+    for (int a = 0; a < 3; ++a) mean[a] += p[a];
+  if (!points.empty())
+    for (int a = 0; a < 3; ++a) mean[a] /= double(points.size());
+  anchor_ = mean;
+  std::vector<float> rec(4 * std::max<size_t>(points.size(), 1));
+  for (size_t i = 0; i < points.size(); ++i) {
+    for (int a = 0; a < 3; ++a) rec[4 * i + a] = float(points[i][a] - anchor_[a]);
+    if (fixed_bin[i] < 2 || fixed_bin[i] > binning.bins - 3) invalid("hip backend", "fixed bin outside [2, bins - 3]");
+    int32_t b = int32_t(fixed_bin[i]);
+    std::memcpy(&rec[4 * i + 3], &b, 4);
+  }
+  samples_ = DevBuf(ctx_, rec.size());
+  check(hipMemcpyAsync(samples_.get(), rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()),
+        "upload samples");
+  ctx_->sync();
+}
+
+double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std::vector<double>* ratio) {
+  const Grid& mg = moving_->grid;
+  const AffineTransform T = t.collapsed();
+  const Mat3 A = mat3_mul(mg.index_from_offset, T.matrix);
+  const Vec3 b = mat3_mul_vec(mg.index_from_offset, vsub(vadd(mat3_mul_vec(T.matrix, anchor_), T.offset()), mg.origin));
+  a = MiSampledArgs{};
+  a.samples = reinterpret_cast<const float4*>(samples_.get());
+  a.count = int(count_);
+  a.moving = moving_->buf.get();
+  a.md = dims_of(mg);
+  put(a.a, A);
+  put(a.b, b);
+  a.inv_bin_size = float(1.0 / binning_.moving_bin_size);
+  a.moving_min = float(binning_.moving_min);
+  a.moving_max = float(binning_.moving_max);
+  a.bins = int(binning_.bins);
+  const size_t bins = binning_.bins;
+  const size_t cells = bins * bins + bins + 1;
+  a.hist = static_cast<unsigned long long*>(ctx_->scratch(0, cells * sizeof(unsigned long long)));
+  hipStream_t s = ctx_->stream();
+  check(hipMemsetAsync(a.hist, 0, cells * sizeof(unsigned long long), s), "clear histogram");
+  if (count_ > 0) {
+    Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
+    launch_mi_sampled_hist(a, mi_sampled_blocks(int(count_)), s);
+  }
+  auto* host = static_cast<unsigned long long*>(ctx_->pinned(cells * sizeof(unsigned long long)));
+  check(hipMemcpyAsync(host, a.hist, cells * sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "read histogram");
+  ctx_->sync();
+  const uint64_t valid = host[cells - 1];
+  HistogramFinish fin =
+      finish_histogram(bins, binning_.moving_bin_size, valid, joint_from(host, bins), marginal_from(host, bins), ratio != nullptr);
+  if (ratio) *ratio = std::move(fin.ratio);
+  return fin.value;
+}
+
+double MiLevel::value(const LinearMovingTransform& t) {
+  MiSampledArgs a;
+  return histogram(t, a, nullptr);
+}
+
+std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const LinearMovingTransform& t) {
+  MiSampledArgs a;
+  std::vector<double> ratio;
+  const double value = histogram(t, a, &ratio);
+  const size_t bins = binning_.bins;
+  std::vector<float> rf(ratio.begin(), ratio.end());
+  auto* dev_ratio = static_cast<float*>(ctx_->scratch(1, rf.size() * sizeof(float)));
+  hipStream_t s = ctx_->stream();
+  check(hipMemcpyAsync(dev_ratio, rf.data(), rf.size() * sizeof(float), hipMemcpyHostToDevice, s), "upload ratio");
+  a.ratio = dev_ratio;
+  a.probes = probes_of(moving_->grid);
+  put(a.outer, t.outer.matrix);
+  put(a.rel_center, vsub(anchor_, t.optimised.center));
+  const double* p = t.optimised.params.data();
+  a.trig[0] = float(std::cos(p[0]));
+  a.trig[1] = float(std::sin(p[0]));
+  a.trig[2] = float(std::cos(p[1]));
+  a.trig[3] = float(std::sin(p[1]));
+  a.trig[4] = float(std::cos(p[2]));
+  a.trig[5] = float(std::sin(p[2]));
+  a.kind = t.optimised.kind == ParamKind::Affine ? 1 : 0;
+  if (t.optimised.kind == ParamKind::Euler3d && t.optimised.zyx)
+    invalid("hip backend", "the ZYX Euler order is not supported by the device derivative");
+  const int blocks = mi_sampled_blocks(int(count_));
+  a.partial = static_cast<double*>(ctx_->scratch(2, size_t(blocks) * 12 * sizeof(double)));
+  {
+    Timed tm(*ctx_, "k_mi_sampled_deriv", 20.0 * double(count_));
+    launch_mi_sampled_deriv(a, blocks, s);
+  }
+  auto* host = static_cast<double*>(ctx_->pinned(size_t(blocks) * 12 * sizeof(double)));
+  check(hipMemcpyAsync(host, a.partial, size_t(blocks) * 12 * sizeof(double), hipMemcpyDeviceToHost, s), "read derivative");
+  ctx_->sync();
+  const size_t np = t.optimised.count();
+  std::vector<double> d(np, 0.0);
+  for (int bl = 0; bl < blocks; ++bl)
+    for (size_t k = 0; k < np; ++k) d[k] += host[size_t(bl) * 12 + k];
+  (void)bins;
+  return {value, d};
+}
+
+// ---------------------------------------------------------------------------- dense pair ---
+
+MetricPair<HField> HipBackend::mi_pair(const HImage& fixed, const HImage& moving, const HImage* fixed_mask,
+                                       const HImage* moving_mask, size_t bins, bool want_norms) {
+  if (fixed.grid != moving.grid) invalid("mattes", "the images are not on one grid");
+  for (const HImage* m : {fixed_mask, moving_mask})
+    if (m && m->grid != fixed.grid) invalid("mattes", "a mask is not on the grid of the images");
+  if (bins < 5 || bins > 48) invalid("hip backend", "the dense Mattes metric supports 5 to 48 histogram bins, not " + std::to_string(bins));
+  const Grid& g = fixed.grid;
+  const long long n = (long long)g.num_voxels();
+  hipStream_t s = ctx_->stream();
+  MiDenseArgs a{};
+  a.f = fixed.buf.get();
+  a.m = moving.buf.get();
+  a.fmask = fixed_mask ? fixed_mask->buf.get() : nullptr;
+  a.mmask = moving_mask ? moving_mask->buf.get() : nullptr;
+  a.d = dims_of(g);
+  a.bins = int(bins);
+  const int rb = mi_dense_blocks(n);
+  a.range_partial = static_cast<float*>(ctx_->scratch(0, size_t(rb) * 4 * sizeof(float)));
+  {
+    Timed tm(*ctx_, "k_mi_dense_range", 8.0 * double(n));
+    launch_mi_dense_range(a, rb, s);
+  }
+  auto* rp = static_cast<float*>(ctx_->pinned(size_t(rb) * 4 * sizeof(float)));
+  check(hipMemcpyAsync(rp, a.range_partial, size_t(rb) * 4 * sizeof(float), hipMemcpyDeviceToHost, s), "read ranges");
+  ctx_->sync();
+  float fmin = INFINITY, fmax = -INFINITY, mmin = INFINITY, mmax = -INFINITY;
+  for (int b = 0; b < rb; ++b) {
+    fmin = std::min(fmin, rp[4 * b]);
+    fmax = std::max(fmax, rp[4 * b + 1]);
+    mmin = std::min(mmin, rp[4 * b + 2]);
+    mmax = std::max(mmax, rp[4 * b + 3]);
+  }
+  if (!(fmin <= fmax))
+    invalid("mattes", fixed_mask ? "no finite fixed-image voxel lies inside the fixed mask" : "the fixed image has no finite voxel");
+  if (!(mmin <= mmax))
+    invalid("mattes", moving_mask ? "no finite moving-image voxel lies inside the moving mask" : "the moving image has no finite voxel");
+  const MiBinning b0 = MiBinning::from_ranges(bins, fmin, fmax, mmin, mmax);
+  const MiBinning b1 = MiBinning::from_ranges(bins, mmin, mmax, fmin, fmax);
+  const MiBinning* bn[2] = {&b0, &b1};
+  for (int dir = 0; dir < 2; ++dir) {
+    const std::vector<float> thr = bn[dir]->fixed_thresholds();
+    for (size_t q = 0; q < thr.size(); ++q) a.fixed_thr[dir][q] = thr[q];
+    a.moving_min[dir] = float(bn[dir]->moving_min);
+    a.moving_max[dir] = float(bn[dir]->moving_max);
+    a.inv_bin_size[dir] = float(1.0 / bn[dir]->moving_bin_size);
+  }
+  const size_t stride = bins * bins + bins + 1;
+  a.hist = static_cast<unsigned long long*>(ctx_->scratch(1, 2 * stride * sizeof(unsigned long long)));
+  check(hipMemsetAsync(a.hist, 0, 2 * stride * sizeof(unsigned long long), s), "clear histograms");
+  {
+    Timed tm(*ctx_, "k_mi_dense_hist", 8.0 * double(n));
+    launch_mi_dense_hist(a, mi_dense_blocks(n), s);
+  }
+  auto* h = static_cast<unsigned long long*>(ctx_->pinned(2 * stride * sizeof(unsigned long long)));
+  check(hipMemcpyAsync(h, a.hist, 2 * stride * sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "read histograms");
+  ctx_->sync();
+  MetricPair<HField> out;
+  std::vector<float> ratio(2 * bins * bins);
+  for (int dir = 0; dir < 2; ++dir) {
+    const unsigned long long* hd = h + dir * stride;
+    const uint64_t valid = hd[stride - 1];
+    if (valid == 0 && (fixed_mask || moving_mask))
+      invalid("mattes", "no voxel of the registration level lies inside both the fixed and the moving mask");
+    HistogramFinish fin = finish_histogram(bins, bn[dir]->moving_bin_size, valid, joint_from(hd, bins), marginal_from(hd, bins), true);
+    out.values[dir] = fin.value;
+    for (size_t q = 0; q < bins * bins; ++q) ratio[dir * bins * bins + q] = float(fin.ratio[q]);
+  }
+  auto* dev_ratio = static_cast<float*>(ctx_->scratch(2, ratio.size() * sizeof(float)));
+  check(hipMemcpyAsync(dev_ratio, ratio.data(), ratio.size() * sizeof(float), hipMemcpyHostToDevice, s), "upload ratio");
+  a.ratio = dev_ratio;
+  a.probes = probes_of(g);
+  out.fields[0] = HField{g, DevBuf(ctx_.get(), 3 * g.num_voxels())};
+  out.fields[1] = HField{g, DevBuf(ctx_.get(), 3 * g.num_voxels())};
+  a.d0 = out.fields[0].buf.get();
+  a.d1 = out.fields[1].buf.get();
+  {
+    Timed tm(*ctx_, "k_mi_dense_deriv", 56.0 * double(n));
+    launch_mi_dense_deriv(a, s);
+  }
+  if (want_norms) {
+    out.norms[0] = l2_norm(out.fields[0]);
+    out.norms[1] = l2_norm(out.fields[1]);
+  } else {
+    ctx_->sync();  // the ratio upload reads host memory that goes out of scope
+  }
+  return out;
+}
+
+}  // namespace fw::hip

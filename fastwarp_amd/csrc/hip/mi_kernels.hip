@@ -1,0 +1,347 @@
+// Mattes mutual information on gfx950: the sampled metric of the linear stages and the dense pair
+// of a SyN MI channel (crates/fastwarp-reg/src/metric/mi.rs:364-676).
+//
+// Histograms: each cubic B-spline Parzen weight (f32) is converted exactly to 2^-32 fixed point and
+// added with a 64-bit LDS atomic into a block-private histogram, which is then added into the
+// global histogram with 64-bit atomics. Integer addition is associative, so the histogram is the
+// same bits whatever the block count or scheduling. The host finishes it in f64.
+// Parameter derivatives of the linear stage accumulate in FP64 per thread and reduce per block in
+// a fixed tree; the host adds the block partials in block order.
+#include "device_common.hpp"
+#include "mi_kernels.hpp"
+
+namespace fw::hip {
+
+namespace {
+
+// mi.rs:62-74 (ITK BSplineKernelFunction<3>)
+__device__ __forceinline__ float bspline3(float u) {
+  const float a = fabsf(u);
+  if (a < 1.0f) {
+    const float s = a * a;
+    return (4.0f - 6.0f * s + 3.0f * s * a) * (1.0f / 6.0f);
+  }
+  if (a < 2.0f) {
+    const float s = a * a;
+    return (8.0f - 12.0f * a + 6.0f * s - s * a) * (1.0f / 6.0f);
+  }
+  return 0.0f;
+}
+
+// mi.rs:77-88 (ITK BSplineDerivativeKernelFunction<3>)
+__device__ __forceinline__ float bspline3_d(float u) {
+  if (u >= 0.0f && u < 1.0f) return -2.0f * u + 1.5f * u * u;
+  if (u > -1.0f && u < 0.0f) return -2.0f * u - 1.5f * u * u;
+  if (u >= 1.0f && u < 2.0f) return -2.0f + 2.0f * u - 0.5f * u * u;
+  if (u > -2.0f && u <= -1.0f) return 2.0f + 2.0f * u + 0.5f * u * u;
+  return 0.0f;
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float w) {
+  return (unsigned long long)__float2ull_rn(w * 4294967296.0f);
+}
+
+__device__ __forceinline__ int clamp_bin(float term, int bins) {
+  int b = (int)term;  // truncation toward zero, as the reference's `as i64`
+  return b < 2 ? 2 : (b > bins - 3 ? bins - 3 : b);
+}
+
+// Central difference along physical axis `a` at a continuous index (interp.rs:102-121).
+__device__ __forceinline__ float central_difference_at(const float* __restrict__ img, const Dims& d, const Probes& pr, int a,
+                                                       float cx, float cy, float cz) {
+  const float px = cx + pr.o[a][0], py = cy + pr.o[a][1], pz = cz + pr.o[a][2];
+  const float mx = cx - pr.o[a][0], my = cy - pr.o[a][1], mz = cz - pr.o[a][2];
+  if (!inside(d, px, py, pz) || !inside(d, mx, my, mz)) return 0.0f;
+  return (tri(img, d, px, py, pz) - tri(img, d, mx, my, mz)) * pr.inv_spacing[a];
+}
+
+struct SampleEval {
+  bool ok;
+  int fbin, mbin;
+  float term, cx, cy, cz;
+};
+
+__device__ __forceinline__ SampleEval eval_sample(const MiSampledArgs& a, float4 s) {
+  SampleEval e;
+  e.ok = false;
+  e.fbin = __float_as_int(s.w);
+  e.cx = fmaf(a.a[0], s.x, fmaf(a.a[1], s.y, fmaf(a.a[2], s.z, a.b[0])));
+  e.cy = fmaf(a.a[3], s.x, fmaf(a.a[4], s.y, fmaf(a.a[5], s.z, a.b[1])));
+  e.cz = fmaf(a.a[6], s.x, fmaf(a.a[7], s.y, fmaf(a.a[8], s.z, a.b[2])));
+  if (!inside(a.md, e.cx, e.cy, e.cz)) return e;
+  const float mv = tri(a.moving, a.md, e.cx, e.cy, e.cz);
+  if (!(mv >= a.moving_min && mv <= a.moving_max)) return e;  // also rejects NaN
+  e.term = (mv - a.moving_min) * a.inv_bin_size + 2.0f;
+  e.mbin = clamp_bin(e.term, a.bins);
+  e.ok = true;
+  return e;
+}
+
+}  // namespace
+
+int mi_sampled_blocks(int count) {
+  int b = (count + 2047) / 2048;
+  return b < 1 ? 1 : (b > 512 ? 512 : b);
+}
+
+__global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
+  __shared__ unsigned long long h[kMiMaxBins * kMiMaxBins + kMiMaxBins];
+  __shared__ unsigned int valid;
+  const int bins = a.bins;
+  const int cells = bins * bins + bins;
+  for (int i = threadIdx.x; i < cells; i += blockDim.x) h[i] = 0ull;
+  if (threadIdx.x == 0) valid = 0u;
+  __syncthreads();
+  unsigned int mine = 0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < a.count; s += gridDim.x * blockDim.x) {
+    const SampleEval e = eval_sample(a, a.samples[s]);
+    if (!e.ok) continue;
+    ++mine;
+    atomicAdd(&h[bins * bins + e.fbin], 1ull);
+    unsigned long long* row = h + e.fbin * bins;
+#pragma unroll
+    for (int k = -1; k <= 2; ++k) {
+      const int m = e.mbin + k;
+      const float w = bspline3((float)m - e.term);
+      if (w > 0.0f) atomicAdd(&row[m], to_fixed(w));
+    }
+  }
+  if (mine) atomicAdd(&valid, mine);
+  __syncthreads();
+  for (int i = threadIdx.x; i < cells; i += blockDim.x)
+    if (h[i]) atomicAdd(&a.hist[i], h[i]);
+  if (threadIdx.x == 0 && valid) atomicAdd(&a.hist[cells], (unsigned long long)valid);
+}
+
+void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_mi_sampled_hist, dim3(blocks), dim3(256), 0, s, a);
+}
+
+// Per sample: sum_k B3'(k - term) * ratio[f, k] * (grad M . dT/dtheta_p) for every parameter
+// (mi.rs:405-433 with the ratio weights of :752-757), accumulated in FP64.
+__global__ __launch_bounds__(256) void k_mi_sampled_deriv(MiSampledArgs a) {
+  __shared__ float ratio[kMiMaxBins * kMiMaxBins];
+  __shared__ double red[256];
+  const int bins = a.bins;
+  for (int i = threadIdx.x; i < bins * bins; i += blockDim.x) ratio[i] = a.ratio[i];
+  __syncthreads();
+  double acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < a.count; s += gridDim.x * blockDim.x) {
+    const float4 smp = a.samples[s];
+    const SampleEval e = eval_sample(a, smp);
+    if (!e.ok) continue;
+    float w = 0.0f;
+#pragma unroll
+    for (int k = -1; k <= 2; ++k) {
+      const int m = e.mbin + k;
+      w += bspline3_d((float)m - e.term) * ratio[e.fbin * bins + m];
+    }
+    if (w == 0.0f) continue;
+    float g0 = central_difference_at(a.moving, a.md, a.probes, 0, e.cx, e.cy, e.cz);
+    float g1 = central_difference_at(a.moving, a.md, a.probes, 1, e.cx, e.cy, e.cz);
+    float g2 = central_difference_at(a.moving, a.md, a.probes, 2, e.cx, e.cy, e.cz);
+    if (!(isfinite(g0) && isfinite(g1) && isfinite(g2))) g0 = g1 = g2 = 0.0f;
+    // g' = outer^T grad: the Jacobian columns are left-multiplied by the outer matrix
+    const float q0 = a.outer[0] * g0 + a.outer[3] * g1 + a.outer[6] * g2;
+    const float q1 = a.outer[1] * g0 + a.outer[4] * g1 + a.outer[7] * g2;
+    const float q2 = a.outer[2] * g0 + a.outer[5] * g1 + a.outer[8] * g2;
+    const float px = smp.x + a.rel_center[0], py = smp.y + a.rel_center[1], pz = smp.z + a.rel_center[2];
+    if (a.kind == 1) {
+      // affine (param.rs:287-297): column r*3+c is e_r (p_c - c_c), column 9+r is e_r
+      const float q[3] = {q0, q1, q2};
+      const float p[3] = {px, py, pz};
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[r * 3 + c] += (double)(w * q[r] * p[c]);
+        acc[9 + r] += (double)(w * q[r]);
+      }
+    } else {
+      // Euler3D, matrix Rz Rx Ry (param.rs:254-281)
+      const float cx = a.trig[0], sx = a.trig[1], cy = a.trig[2], sy = a.trig[3], cz = a.trig[4], sz = a.trig[5];
+      const float j00 = (-sz * cx * sy) * px + (sz * sx) * py + (sz * cx * cy) * pz;
+      const float j01 = (cz * cx * sy) * px + (-cz * sx) * py + (-cz * cx * cy) * pz;
+      const float j02 = (sx * sy) * px + cx * py + (-sx * cy) * pz;
+      const float j10 = (-cz * sy - sz * sx * cy) * px + (cz * cy - sz * sx * sy) * pz;
+      const float j11 = (-sz * sy + cz * sx * cy) * px + (sz * cy + cz * sx * sy) * pz;
+      const float j12 = (-cx * cy) * px + (-cx * sy) * pz;
+      const float j20 = (-sz * cy - cz * sx * sy) * px + (-cz * cx) * py + (-sz * sy + cz * sx * cy) * pz;
+      const float j21 = (cz * cy - sz * sx * sy) * px + (-sz * cx) * py + (cz * sy + sz * sx * cy) * pz;
+      acc[0] += (double)(w * (q0 * j00 + q1 * j01 + q2 * j02));
+      acc[1] += (double)(w * (q0 * j10 + q1 * j11 + q2 * j12));
+      acc[2] += (double)(w * (q0 * j20 + q1 * j21));
+      acc[3] += (double)(w * q0);
+      acc[4] += (double)(w * q1);
+      acc[5] += (double)(w * q2);
+    }
+  }
+  const int np = a.kind == 1 ? 12 : 6;
+  for (int k = 0; k < np; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[blockIdx.x * 12 + k] = red[0];
+    __syncthreads();
+  }
+}
+
+void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_mi_sampled_deriv, dim3(blocks), dim3(256), 0, s, a);
+}
+
+// ------------------------------------------------------------------------------ dense pair ---
+
+int mi_dense_blocks(long long n) {
+  long long b = (n + 4095) / 4096;
+  return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+__device__ __forceinline__ bool dense_inside(const MiDenseArgs& a, long long g) {
+  return (!a.fmask || a.fmask[g] != 0.0f) && (!a.mmask || a.mmask[g] != 0.0f);
+}
+
+// mi.rs:171-184 / 290-304: min and max of each image over the voxels inside its own mask.
+__global__ __launch_bounds__(256) void k_mi_dense_range(MiDenseArgs a) {
+  const long long n = a.d.n();
+  float fmin = INFINITY, fmax = -INFINITY, mmin = INFINITY, mmax = -INFINITY;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
+    const float f = a.f[g], m = a.m[g];
+    if ((!a.fmask || a.fmask[g] != 0.0f) && isfinite(f)) {
+      fmin = fminf(fmin, f);
+      fmax = fmaxf(fmax, f);
+    }
+    if ((!a.mmask || a.mmask[g] != 0.0f) && isfinite(m)) {
+      mmin = fminf(mmin, m);
+      mmax = fmaxf(mmax, m);
+    }
+  }
+  __shared__ float r[4][256];
+  r[0][threadIdx.x] = fmin;
+  r[1][threadIdx.x] = fmax;
+  r[2][threadIdx.x] = mmin;
+  r[3][threadIdx.x] = mmax;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      r[0][threadIdx.x] = fminf(r[0][threadIdx.x], r[0][threadIdx.x + st]);
+      r[1][threadIdx.x] = fmaxf(r[1][threadIdx.x], r[1][threadIdx.x + st]);
+      r[2][threadIdx.x] = fminf(r[2][threadIdx.x], r[2][threadIdx.x + st]);
+      r[3][threadIdx.x] = fmaxf(r[3][threadIdx.x], r[3][threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) a.range_partial[blockIdx.x * 4 + threadIdx.x] = r[threadIdx.x][0];
+}
+
+void launch_mi_dense_range(const MiDenseArgs& a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_mi_dense_range, dim3(blocks), dim3(256), 0, s, a);
+}
+
+__device__ __forceinline__ int threshold_bin(const float* thr, int bins, float v) {
+  int b = 2;
+  for (int q = 0; q < bins - 5; ++q) b += (v >= thr[q]) ? 1 : 0;  // thresholds of bins 3 .. bins-3
+  return b;
+}
+
+// Both directions' joint histograms and fixed marginals in one pass.
+__global__ __launch_bounds__(256) void k_mi_dense_hist(MiDenseArgs a) {
+  extern __shared__ unsigned long long hs[];
+  const int bins = a.bins;
+  const int stride = bins * bins + bins;
+  for (int i = threadIdx.x; i < 2 * stride; i += blockDim.x) hs[i] = 0ull;
+  __shared__ unsigned int valid[2];
+  if (threadIdx.x < 2) valid[threadIdx.x] = 0u;
+  __syncthreads();
+  const long long n = a.d.n();
+  unsigned int mine[2] = {0u, 0u};
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
+    if (!dense_inside(a, g)) continue;
+    const float f = a.f[g], m = a.m[g];
+    if (!isfinite(f) || !isfinite(m)) continue;
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+      const float fixed = dir == 0 ? f : m;
+      const float moving = dir == 0 ? m : f;
+      if (!(moving >= a.moving_min[dir] && moving <= a.moving_max[dir])) continue;
+      const float term = (moving - a.moving_min[dir]) * a.inv_bin_size[dir] + 2.0f;
+      const int mbin = clamp_bin(term, bins);
+      const int fbin = threshold_bin(a.fixed_thr[dir], bins, fixed);
+      unsigned long long* h = hs + dir * stride;
+      ++mine[dir];
+      atomicAdd(&h[bins * bins + fbin], 1ull);
+#pragma unroll
+      for (int k = -1; k <= 2; ++k) {
+        const int mb = mbin + k;
+        const float w = bspline3((float)mb - term);
+        if (w > 0.0f) atomicAdd(&h[fbin * bins + mb], to_fixed(w));
+      }
+    }
+  }
+  if (mine[0]) atomicAdd(&valid[0], mine[0]);
+  if (mine[1]) atomicAdd(&valid[1], mine[1]);
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * stride; i += blockDim.x) {
+    const int dir = i / stride, cell = i % stride;
+    if (hs[i]) atomicAdd(&a.hist[dir * (stride + 1) + cell], hs[i]);
+  }
+  if (threadIdx.x < 2 && valid[threadIdx.x])
+    atomicAdd(&a.hist[threadIdx.x * (stride + 1) + stride], (unsigned long long)valid[threadIdx.x]);
+}
+
+void launch_mi_dense_hist(const MiDenseArgs& a, int blocks, hipStream_t s) {
+  const size_t shmem = size_t(2) * (a.bins * a.bins + a.bins) * sizeof(unsigned long long);
+  hipLaunchKernelGGL(k_mi_dense_hist, dim3(blocks), dim3(256), shmem, s, a);
+}
+
+// Per voxel sum_k B3'(m_k - term) ratio[f, m_k] * grad(moving) for both directions (mi.rs:650-666).
+__global__ __launch_bounds__(256) void k_mi_dense_deriv(MiDenseArgs a) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i >= a.d.nx || j >= a.d.ny) return;
+  const long long g = lin(a.d, i, j, k), n = a.d.n();
+  float out[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  const float f = a.f[g], m = a.m[g];
+  if (dense_inside(a, g) && isfinite(f) && isfinite(m)) {
+    const int bins = a.bins;
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+      const float fixed = dir == 0 ? f : m;
+      const float moving = dir == 0 ? m : f;
+      if (!(moving >= a.moving_min[dir] && moving <= a.moving_max[dir])) continue;
+      const float term = (moving - a.moving_min[dir]) * a.inv_bin_size[dir] + 2.0f;
+      const int mbin = clamp_bin(term, bins);
+      const int fbin = threshold_bin(a.fixed_thr[dir], bins, fixed);
+      const float* ratio = a.ratio + dir * bins * bins + fbin * bins;
+      float s = 0.0f;
+#pragma unroll
+      for (int q = -1; q <= 2; ++q) s += bspline3_d((float)(mbin + q) - term) * ratio[mbin + q];
+      const float* img = dir == 0 ? a.m : a.f;
+      float g0 = central_difference(img, a.d, a.probes, 0, i, j, k);
+      float g1 = central_difference(img, a.d, a.probes, 1, i, j, k);
+      float g2 = central_difference(img, a.d, a.probes, 2, i, j, k);
+      if (!(isfinite(g0) && isfinite(g1) && isfinite(g2))) g0 = g1 = g2 = 0.0f;
+      out[dir][0] = s * g0;
+      out[dir][1] = s * g1;
+      out[dir][2] = s * g2;
+    }
+  }
+  a.d0[g] = out[0][0];
+  a.d0[g + n] = out[0][1];
+  a.d0[g + 2 * n] = out[0][2];
+  a.d1[g] = out[1][0];
+  a.d1[g + n] = out[1][1];
+  a.d1[g + 2 * n] = out[1][2];
+}
+
+void launch_mi_dense_deriv(const MiDenseArgs& a, hipStream_t s) {
+  const dim3 grid((a.d.nx + 63) / 64, (a.d.ny + 3) / 4, a.d.nz);
+  hipLaunchKernelGGL(k_mi_dense_deriv, grid, dim3(256), 0, s, a);
+}
+
+}  // namespace fw::hip

@@ -1,0 +1,287 @@
+// The linear stages (crates/fastwarp-reg/src/linear.rs, init.rs): per level the fixed grid shrunk
+// by the level factors is the virtual domain, both full-resolution images are smoothed, one
+// REGULAR sample set is drawn with the next seed of the stage, Mattes MI is initialised on it and
+// ITK's conjugate gradient runs. Generic over a LinearEngine whose prepare_level(moving, points,
+// fixed_bin, binning) returns a LinearMetric.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <optional>
+
+#include "interp.hpp"
+#include "mi_host.hpp"
+#include "optimizer.hpp"
+#include "pyramid.hpp"
+#include "sampling.hpp"
+
+namespace fw {
+
+enum class LinearKind { Rigid, Affine };
+
+// linear.rs:75-145 `LinearRecipe`
+struct LinearRecipe {
+  LinearKind kind = LinearKind::Rigid;
+  double gradient_step = 0.1;
+  std::vector<uint32_t> iterations{1000, 500, 250, 100};
+  double convergence_threshold = 1e-6;
+  size_t convergence_window = 10;
+  std::vector<uint32_t> shrink_factors{8, 4, 2, 1};
+  std::vector<double> smoothing_sigmas{3.0, 2.0, 1.0, 0.0};
+  SigmaUnits sigma_units = SigmaUnits::Voxels;
+  size_t bins = 32;
+  float sampling_percentage = 0.25f;
+  uint32_t seed = 20260910u;
+
+  size_t levels() const { return iterations.size(); }
+  // linear.rs:191-239 `validate`
+  void validate() const {
+    const size_t n = iterations.size();
+    if (n == 0 || shrink_factors.size() != n || smoothing_sigmas.size() != n)
+      invalid("linear recipe", "iterations, shrink factors and smoothing sigmas must have one entry per level");
+    for (auto f : shrink_factors)
+      if (f == 0) invalid("linear recipe", "iterations, shrink factors and smoothing sigmas must have one entry per level");
+    if (convergence_window < 2 || bins <= 4) invalid("linear recipe", "window below 2 or at most 4 bins");
+    if (!(sampling_percentage > 0.0f && sampling_percentage <= 1.0f)) invalid("linear recipe", "sampling percentage outside (0, 1]");
+    if (!(std::isfinite(gradient_step) && gradient_step > 0.0)) invalid("linear recipe", "gradient step must be positive");
+    for (double s : smoothing_sigmas)
+      if (!(std::isfinite(s) && s >= 0.0)) invalid("linear recipe", "smoothing sigmas must be finite and non-negative");
+  }
+};
+
+// Smoothed full-resolution image pairs shared by the rigid, affine and SyN stages of one
+// registration (the reference's SmoothCache, linear.rs:569-754), keyed by sigma.
+struct SmoothCache {
+  std::map<double, std::pair<std::shared_ptr<Image>, std::shared_ptr<Image>>> pairs;
+  std::pair<std::shared_ptr<Image>, std::shared_ptr<Image>> get(const Image& fixed, const Image& moving, double sigma, SigmaUnits u) {
+    auto it = pairs.find(sigma);
+    if (it != pairs.end()) return it->second;
+    auto f = std::make_shared<Image>(smooth_for_level(fixed, sigma, u));
+    auto m = std::make_shared<Image>(smooth_for_level(moving, sigma, u));
+    pairs[sigma] = {f, m};
+    return {f, m};
+  }
+};
+
+// mi.rs:161-169 and 193-207: nearest-voxel value at a point, none outside.
+inline bool nearest_value(const Image& vol, const Vec3& p, double& out) {
+  size_t idx;
+  if (!nearest_voxel(vol.grid, p, idx)) return false;
+  out = vol.data[idx];
+  return true;
+}
+
+// mi.rs:193-207 `fixed_sample_range`: count and range of the finite nearest-voxel values.
+inline size_t fixed_sample_range(const Image& fixed, const std::vector<Vec3>& samples, double& lo, double& hi) {
+  size_t count = 0;
+  for (const Vec3& s : samples) {
+    double v;
+    if (!nearest_value(fixed, s, v) || !std::isfinite(v)) continue;
+    if (count == 0) lo = hi = v;
+    else {
+      lo = std::min(lo, v);
+      hi = std::max(hi, v);
+    }
+    ++count;
+  }
+  return count;
+}
+
+struct PreparedSamples {
+  std::vector<Vec3> points;
+  std::vector<uint32_t> fixed_bin;
+  MiBinning binning;
+};
+
+// linear.rs:251-279 `prepare_samples` with MiBinning::new (mi.rs:209-236).
+inline PreparedSamples prepare_samples(const Image& fixed, const Image& moving, const std::vector<Vec3>& samples, size_t bins) {
+  if (bins <= 4) invalid("mattes", "at least five histogram bins are required");
+  double flo = 0, fhi = 0;
+  if (fixed_sample_range(fixed, samples, flo, fhi) == 0) invalid("mattes", "no fixed-image samples were observed");
+  if (fhi - flo <= kEps) invalid("mattes", "the fixed image is constant over the samples");
+  double mlo = 0, mhi = 0;
+  bool any = false;
+  for (double v : moving.data) {
+    if (!std::isfinite(v)) continue;
+    if (!any) {
+      mlo = mhi = v;
+      any = true;
+    } else {
+      mlo = std::min(mlo, v);
+      mhi = std::max(mhi, v);
+    }
+  }
+  if (!any) invalid("mattes", "the moving image has no finite voxel");
+  PreparedSamples out;
+  out.binning = MiBinning::from_ranges(bins, flo, fhi, mlo, mhi);
+  std::vector<int64_t> bin(samples.size(), -1);
+  parallel_for(samples.size(), [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      double v;
+      if (linear_at_point(fixed, samples[i], v) && std::isfinite(v)) bin[i] = int64_t(out.binning.fixed_bin(v));
+    }
+  });
+  for (size_t i = 0; i < samples.size(); ++i)
+    if (bin[i] >= 0) {
+      out.points.push_back(samples[i]);
+      out.fixed_bin.push_back(uint32_t(bin[i]));
+    }
+  return out;
+}
+
+struct LinearLevelTrace {
+  size_t level = 0;
+  std::array<uint32_t, 3> shrink_factors{};
+  std::array<size_t, 3> grid_size{};
+  size_t samples = 0;
+  CgTrace cg;
+  double seconds = 0.0;
+  std::vector<double> parameters;
+  std::string skipped;
+};
+
+struct LinearStageResult {
+  ParametricTransform transform;
+  AffineTransform affine;
+  std::vector<LinearLevelTrace> levels;
+};
+
+// The engine levels of a stage, kept so that a chained stage with the same sampling reuses them:
+// the rigid and the affine stage of a chain draw identical sample sets per level (the same seed
+// sequence over the same smoothed images), as the reference's sample cache exploits.
+template <typename Engine>
+struct LevelCache {
+  std::map<size_t, std::shared_ptr<typename Engine::Level>> levels;
+  std::map<size_t, size_t> sample_counts;
+  std::map<size_t, std::string> skipped;
+};
+
+// linear.rs:873-1068 `run_linear_stage_cached_async`.
+template <typename Engine>
+LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, const Image& fixed, const Image& moving,
+                                   const AffineTransform& outer, const ParametricTransform& start, SmoothCache& cache,
+                                   LevelCache<Engine>* levels_cache, const volatile int32_t* cancel) {
+  recipe.validate();
+  if (start.count() != (recipe.kind == LinearKind::Rigid ? 6u : 12u))
+    invalid("linear stage", "start transform does not match the stage kind");
+  SeedSequence seeds(recipe.seed);
+  LinearMovingTransform current{outer, start};
+  LinearStageResult result;
+  for (size_t level = 0; level < recipe.levels(); ++level) {
+    check_cancel(cancel);
+    const auto t0 = std::chrono::steady_clock::now();
+    const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed.grid.spacing);
+    const Grid grid = level_virtual_grid(fixed.grid, factors);
+    const double sigma = recipe.smoothing_sigmas[level];
+    LinearLevelTrace tr;
+    tr.level = level;
+    tr.shrink_factors = factors;
+    tr.grid_size = grid.size;
+    std::shared_ptr<typename Engine::Level> metric;
+    if (levels_cache && levels_cache->skipped.count(level)) {
+      seeds.next_seed();
+      tr.skipped = levels_cache->skipped[level];
+    } else if (levels_cache && levels_cache->levels.count(level)) {
+      seeds.next_seed();
+      metric = levels_cache->levels[level];
+      tr.samples = levels_cache->sample_counts[level];
+    } else {
+      auto [fs, ms] = cache.get(fixed, moving, sigma, recipe.sigma_units);
+      const std::vector<Vec3> raw = metric_sample_points(grid, recipe.sampling_percentage, seeds);
+      double lo = 0, hi = 0;
+      const size_t observed = fixed_sample_range(*fs, raw, lo, hi);
+      if (observed == 0) tr.skipped = "no sample point lands on the fixed image";
+      else if (hi - lo <= kEps) tr.skipped = "the fixed image takes a single value at its sample points";
+      if (tr.skipped.empty()) {
+        PreparedSamples prep = prepare_samples(*fs, *ms, raw, recipe.bins);
+        if (prep.points.empty()) tr.skipped = "no sample point lies inside the fixed image";
+        else {
+          tr.samples = prep.points.size();
+          metric = std::make_shared<typename Engine::Level>(engine.prepare_level(*ms, prep.points, prep.fixed_bin, prep.binning));
+        }
+      }
+      if (levels_cache) {
+        if (metric) {
+          levels_cache->levels[level] = metric;
+          levels_cache->sample_counts[level] = tr.samples;
+        } else {
+          levels_cache->skipped[level] = tr.skipped;
+        }
+      }
+    }
+    if (metric) {
+      CgSettings st;
+      st.max_step_mm = recipe.gradient_step;
+      st.iterations = recipe.iterations[level];
+      st.convergence_threshold = recipe.convergence_threshold;
+      st.convergence_window = recipe.convergence_window;
+      auto [next, cg] = optimise_level(*metric, corner_points(grid), current, st, cancel);
+      current = next;
+      tr.cg = std::move(cg);
+    }
+    tr.parameters = current.optimised.parameters();
+    tr.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    result.levels.push_back(std::move(tr));
+  }
+  bool all_skipped = !result.levels.empty();
+  for (auto& l : result.levels) all_skipped = all_skipped && !l.skipped.empty();
+  if (all_skipped)
+    invalid("linear stage", "every level of the stage has too few usable samples for Mattes mutual information");
+  result.transform = current.optimised;
+  result.affine = current.optimised.to_affine();
+  return result;
+}
+
+// init.rs:47-86: intensity-weighted centre (ITK ImageMomentsCalculator) and the initial
+// translation moving centre - fixed centre about the fixed centre.
+inline Vec3 centre_of_gravity(const Image& vol) {
+  const Grid& g = vol.grid;
+  double mass = 0.0;
+  Vec3 cg{0, 0, 0};
+  for (size_t l = 0; l < vol.data.size(); ++l) {
+    const double v = vol.data[l];
+    if (v == 0.0) continue;
+    auto [i, j, k] = g.ijk(l);
+    const Vec3 p = g.index_to_point({double(i), double(j), double(k)});
+    mass += v;
+    for (int a = 0; a < 3; ++a) cg[a] += p[a] * v;
+  }
+  if (mass == 0.0) invalid("initial transform", "total mass of the image is zero");
+  return {cg[0] / mass, cg[1] / mass, cg[2] / mass};
+}
+
+inline AffineTransform initial_transform_com(const Image& fixed, const Image& moving) {
+  const Vec3 fc = centre_of_gravity(fixed), mc = centre_of_gravity(moving);
+  return AffineTransform{kIdentity3, vsub(mc, fc), fc};
+}
+
+struct LinearChainResult {
+  AffineTransform initial;
+  LinearStageResult rigid, affine;
+  AffineTransform total;
+};
+
+// linear.rs:1148-1212 `run_linear_chain_cached_async`: centre of mass, Rigid, Affine, collapsed
+// about the last linear transform's centre.
+template <typename Engine>
+LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine, const Image& fixed_raw,
+                                   const Image& moving_raw, const Image& fixed, const Image& moving, SmoothCache& cache,
+                                   const volatile int32_t* cancel) {
+  if (rigid.kind != LinearKind::Rigid || affine.kind != LinearKind::Affine)
+    invalid("linear chain", "expected a rigid then an affine recipe");
+  LinearChainResult out;
+  out.initial = initial_transform_com(fixed_raw, moving_raw);
+  const bool same_sampling = rigid.seed == affine.seed && rigid.shrink_factors == affine.shrink_factors &&
+                             rigid.smoothing_sigmas == affine.smoothing_sigmas && rigid.bins == affine.bins &&
+                             rigid.sampling_percentage == affine.sampling_percentage && rigid.sigma_units == affine.sigma_units;
+  LevelCache<Engine> levels;
+  out.rigid = run_linear_stage(engine, rigid, fixed, moving, out.initial, ParametricTransform::euler3d({0, 0, 0, 0, 0, 0}, out.initial.center),
+                               cache, same_sampling ? &levels : nullptr, cancel);
+  const AffineTransform after_rigid = AffineTransform::compose(out.initial, out.rigid.affine);
+  const ParametricTransform start = ParametricTransform::affine(AffineTransform{kIdentity3, {0, 0, 0}, out.rigid.affine.center});
+  out.affine = run_linear_stage(engine, affine, fixed, moving, after_rigid, start, cache, same_sampling ? &levels : nullptr, cancel);
+  out.total = with_center(AffineTransform::compose(after_rigid, out.affine.affine), out.affine.affine.center);
+  return out;
+}
+
+}  // namespace fw

@@ -40,7 +40,8 @@ struct MiBinning {
   }
   // mi.rs:307-316: the truncated term clamped to [2, bins - 3].
   size_t clamp_bin(double term) const {
-    long long idx = std::isnan(term) ? 0 : (long long)term;
+    // saturating truncation, as Rust's `as i64`
+    long long idx = std::isnan(term) ? 0 : (long long)std::max(-1e18, std::min(1e18, term));
     if (idx < 2) idx = 2;
     if (idx > (long long)bins - 3) idx = (long long)bins - 3;
     return size_t(idx);

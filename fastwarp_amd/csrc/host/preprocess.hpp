@@ -1,0 +1,109 @@
+// antsRegistration intensity preprocessing (crates/fastwarp-reg/src/preprocess.rs): winsorise by
+// the quantiles of a 256-bin histogram with ITK's marginal-scale margin, then window to [0, 1].
+#pragma once
+
+#include "grid.hpp"
+
+namespace fw {
+
+// preprocess.rs:40-150: the ImageToHistogramFilter histogram and its quantiles.
+class IntensityHistogram {
+ public:
+  explicit IntensityHistogram(const std::vector<double>& values) {
+    constexpr size_t kBins = 256;
+    double lo = std::numeric_limits<double>::max(), hi = std::numeric_limits<double>::lowest();
+    size_t finite = 0;
+    for (double v : values) {
+      if (!std::isfinite(v)) continue;
+      ++finite;
+      lo = std::min(lo, v);
+      hi = std::max(hi, v);
+    }
+    if (finite == 0) invalid("preprocess", "image has no finite values");
+    const double margin = ((hi - lo) / double(kBins)) / 10.0;
+    const double upper = hi + margin;
+    const float interval = (float(upper) - float(lo)) / float(kBins);
+    min_.assign(kBins, 0.0);
+    max_.assign(kBins, 0.0);
+    for (size_t j = 0; j + 1 < kBins; ++j) {
+      min_[j] = lo + double(float(j) * interval);
+      max_[j] = lo + double((float(j) + 1.0f) * interval);
+    }
+    min_[kBins - 1] = lo + double((float(kBins) - 1.0f) * interval);
+    max_[kBins - 1] = upper;
+    std::vector<uint64_t> counts(kBins, 0);
+    for (double v : values)
+      if (std::isfinite(v)) ++counts[index_of(v)];
+    freq_.assign(counts.begin(), counts.end());
+  }
+
+  size_t index_of(double v) const {
+    if (v < min_[0]) return 0;
+    const size_t n = min_.size();
+    if (v >= max_[n - 1]) return n - 1;
+    const size_t pp = size_t(std::upper_bound(min_.begin(), min_.end(), v) - min_.begin());
+    return pp == 0 ? 0 : pp - 1;
+  }
+
+  double quantile(double p) const {
+    const size_t size = freq_.size();
+    double total = 0.0;
+    for (double f : freq_) total += f;
+    double cumulated = 0.0;
+    if (p < 0.5) {
+      size_t n = 0;
+      double p_n = 0.0, p_prev = 0.0, f_n = 0.0;
+      for (;;) {
+        f_n = freq_[n];
+        cumulated += f_n;
+        p_prev = p_n;
+        p_n = cumulated / total;
+        ++n;
+        if (!(n < size && p_n < p)) break;
+      }
+      const double proportion = f_n / total;
+      const double mn = min_[n - 1];
+      return mn + ((p - p_prev) / proportion) * (max_[n - 1] - mn);
+    }
+    long long n = (long long)size - 1;
+    size_t m = 0;
+    double p_n = 1.0, p_prev = 1.0, f_n = 0.0;
+    for (;;) {
+      f_n = freq_[size_t(n)];
+      cumulated += f_n;
+      p_prev = p_n;
+      p_n = 1.0 - cumulated / total;
+      --n;
+      ++m;
+      if (!(m < size && p_n > p)) break;
+    }
+    const double proportion = f_n / total;
+    const size_t bin = size_t(n + 1);
+    const double mx = max_[bin];
+    return mx - ((p_prev - p) / proportion) * (mx - min_[bin]);
+  }
+
+ private:
+  std::vector<double> min_, max_, freq_;
+};
+
+// preprocess.rs:159-232: the quantile window rounded to f32, then the linear map of [lo, hi] onto
+// [out_min, out_max] with clamping; non-finite voxels pass unchanged.
+inline Image preprocess_volume(const Image& vol, float lower, float upper, double out_min = 0.0, double out_max = 1.0) {
+  IntensityHistogram h(vol.data);
+  const double lo = double(float(h.quantile(double(lower))));
+  const double hi = double(float(h.quantile(double(upper))));
+  if (!(hi > lo)) invalid("preprocess", "degenerate intensity window");
+  const double scale = (out_max - out_min) / (hi - lo);
+  const double shift = out_min - lo * scale;
+  std::vector<double> out(vol.data.size());
+  parallel_for(out.size(), [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      const double x = vol.data[i];
+      out[i] = x < lo ? out_min : (x > hi ? out_max : x * scale + shift);
+    }
+  });
+  return Image(vol.grid, std::move(out));
+}
+
+}  // namespace fw

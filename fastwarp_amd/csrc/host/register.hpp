@@ -1,0 +1,76 @@
+// One full normalisation as register_stages_async runs it (crates/fastwarp-reg/src/stages.rs:286-403
+// with register.rs:201-248): preprocess every channel (winsorise, window to [0, 1]), centre of mass
+// on the raw first channel, Rigid then Affine Mattes MI on the first channel, then SyN on all
+// channels from the collapsed affine. Generic over the SyN backend and the linear engine.
+#pragma once
+
+#include <chrono>
+
+#include "linear.hpp"
+#include "preprocess.hpp"
+#include "syn.hpp"
+
+namespace fw {
+
+struct RegisterPlan {
+  LinearRecipe rigid, affine;
+  SynRecipe syn;
+  float winsorize_lower = 0.005f, winsorize_upper = 0.995f;  // register.rs WINSORIZE_LOWER/UPPER
+  bool run_linear = true, run_syn = true;
+
+  RegisterPlan() {
+    rigid.kind = LinearKind::Rigid;
+    affine.kind = LinearKind::Affine;
+  }
+};
+
+struct RegisterOutput {
+  AffineTransform initial, total;
+  LinearChainResult linear;
+  SynFields syn;
+  double seconds_preprocess = 0.0, seconds_linear = 0.0, seconds_syn = 0.0, seconds_total = 0.0;
+};
+
+template <typename B, typename E>
+RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, const std::vector<Image>& fixed,
+                             const std::vector<Image>& moving, const volatile int32_t* cancel) {
+  using clock = std::chrono::steady_clock;
+  auto secs = [](clock::time_point a) { return std::chrono::duration<double>(clock::now() - a).count(); };
+  const auto t_all = clock::now();
+  if (fixed.empty() || fixed.size() != moving.size())
+    invalid("register", "fixed and moving must have the same, non-zero number of channels");
+  if (plan.run_syn && plan.syn.channel_weights.size() != fixed.size())
+    invalid("stage plan", "the number of channels differs from the number of channel weights");
+  for (const auto* set : {&fixed, &moving})
+    for (const Image& v : *set)
+      for (double x : v.data)
+        if (!std::isfinite(x)) invalid("register", "an input image has non-finite (NaN or infinite) voxels");
+  RegisterOutput out;
+  auto t = clock::now();
+  std::vector<SynChannel> channels;
+  for (size_t c = 0; c < fixed.size(); ++c)
+    channels.push_back({preprocess_volume(fixed[c], plan.winsorize_lower, plan.winsorize_upper),
+                        preprocess_volume(moving[c], plan.winsorize_lower, plan.winsorize_upper)});
+  out.seconds_preprocess = secs(t);
+  SmoothCache cache;
+  t = clock::now();
+  if (plan.run_linear) {
+    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, fixed[0], moving[0], channels[0].fixed, channels[0].moving,
+                                  cache, cancel);
+    out.initial = out.linear.initial;
+    out.total = out.linear.total;
+  } else {
+    out.initial = out.total = initial_transform_com(fixed[0], moving[0]);
+  }
+  out.seconds_linear = secs(t);
+  if (plan.run_syn) {
+    backend.release_idle_buffers();
+    t = clock::now();
+    out.syn = syn_fields(backend, plan.syn, channels, out.total, &cache, cancel);
+    out.seconds_syn = secs(t);
+  }
+  out.seconds_total = secs(t_all);
+  return out;
+}
+
+}  // namespace fw

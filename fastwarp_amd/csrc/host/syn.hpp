@@ -1,0 +1,239 @@
+// ITK's symmetric normalisation as antsRegistration drives it (crates/fastwarp-reg/src/syn.rs),
+// written once against a backend B with the stage interface of backend.rs:39-305:
+//   B::ImageH / B::FieldH, upload_image, zero_field, resample_field, warp_through_field,
+//   cc_pair, mi_pair, axpy, scale, smooth_field, max_scaled_norm_pair, compose, invert,
+//   download_field, release_idle_buffers.
+// Four fields live on the level grid (F, Finv, M, Minv); one iteration warps every channel's
+// smoothed full-resolution images into the middle, evaluates the metric both ways, smooths and
+// scales the two updates, composes them into the fields and re-inverts both fields.
+#pragma once
+
+#include <chrono>
+#include <optional>
+
+#include "backend_types.hpp"
+#include "convergence.hpp"
+#include "linear.hpp"
+#include "pyramid.hpp"
+
+namespace fw {
+
+enum class SynMetric { Cc, Mattes };
+
+// recipe.rs:52-127 `SynRecipe` (cohort: SyN[0.1,3,0], CC radius 4, 100x70x50x20, 8x4x2x1, 3x2x1x0vox).
+struct SynRecipe {
+  double gradient_step = 0.1;
+  double update_field_variance = 3.0;
+  double total_field_variance = 0.0;
+  size_t cc_radius = 4;
+  size_t mi_bins = 32;
+  std::vector<SynMetric> channel_metrics;  // empty: CC for every channel
+  std::vector<double> channel_weights{1.0};
+  std::vector<uint32_t> iterations{100, 70, 50, 20};
+  double convergence_threshold = 1e-6;
+  size_t convergence_window = 10;
+  std::vector<uint32_t> shrink_factors{8, 4, 2, 1};
+  std::vector<double> smoothing_sigmas{3.0, 2.0, 1.0, 0.0};
+  SigmaUnits sigma_units = SigmaUnits::Voxels;
+
+  size_t levels() const { return iterations.size(); }
+  SynMetric metric(size_t c) const { return channel_metrics.empty() ? SynMetric::Cc : channel_metrics[c]; }
+  void validate() const {
+    const size_t n = iterations.size();
+    if (n == 0 || shrink_factors.size() != n || smoothing_sigmas.size() != n)
+      invalid("syn recipe", "iterations, shrink factors and smoothing sigmas must have one entry per level");
+    if (convergence_window < 2) invalid("syn recipe", "convergence window below 2");
+    if (cc_radius == 0) invalid("syn recipe", "CC radius must be positive");
+    if (channel_weights.empty()) invalid("syn recipe", "no channel weights");
+    if (!channel_metrics.empty() && channel_metrics.size() != channel_weights.size())
+      invalid("syn recipe", "one metric per channel is required");
+  }
+};
+
+struct SynChannel {
+  Image fixed, moving;
+};
+
+// syn.rs:96-128
+struct SynIteration {
+  double energy = 0.0, energy_weighted = 0.0, convergence_value = 0.0;
+  double fixed_update_scale = 0.0, moving_update_scale = 0.0;
+  InvertStats inversions[4];
+  double seconds = 0.0;
+};
+
+struct SynLevelTrace {
+  size_t level = 0;
+  std::array<uint32_t, 3> shrink_factors{};
+  std::array<size_t, 3> grid_size{};
+  std::vector<SynIteration> iterations;
+  bool converged = false;
+  double seconds = 0.0;
+};
+
+struct SynFields {
+  VectorField warp, inverse_warp;
+  std::vector<SynLevelTrace> levels;
+};
+
+namespace detail {
+
+// syn.rs:203-297 `SideAccumulator` / `SmoothedSide` for one side of the update.
+template <typename B>
+struct Side {
+  std::vector<double> values;
+  std::optional<typename B::FieldH> gradient;
+  double total_magnitude = 0.0;
+
+  void add(B& b, double weight, double value, typename B::FieldH derivative, const double* magnitude) {
+    values.push_back(value);
+    if (!magnitude) {
+      gradient = std::move(derivative);
+      return;
+    }
+    if (!gradient) gradient = b.zero_field(b.field_grid(derivative));
+    if (!std::isfinite(*magnitude)) invalid("syn", "the metric gradient is not finite");
+    total_magnitude += *magnitude;
+    if (*magnitude > kEps) b.axpy(*gradient, weight / *magnitude, derivative);
+  }
+};
+
+}  // namespace detail
+
+// syn.rs:776-986 `syn_fields` from a linear start `initial` (maps fixed points to moving points).
+template <typename B>
+SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel>& channels, const AffineTransform& initial,
+                     SmoothCache* cache, const volatile int32_t* cancel) {
+  using FieldH = typename B::FieldH;
+  using ImageH = typename B::ImageH;
+  recipe.validate();
+  if (channels.empty()) invalid("syn", "no channels");
+  if (channels.size() != recipe.channel_weights.size())
+    invalid("syn", "the number of channels differs from the number of channel weights");
+  const Grid fixed_grid = channels[0].fixed.grid;
+  for (const auto& c : channels)
+    if (!c.fixed.grid.approx_eq(fixed_grid, 1e-9)) invalid("syn", "fixed images of the channels differ in grid");
+  double wsum = 0.0;
+  for (double w : recipe.channel_weights) wsum += w;
+  std::vector<double> normalised;
+  for (double w : recipe.channel_weights) normalised.push_back(w / wsum);
+  const size_t nc = channels.size();
+  const InvertParams inv_params;  // InvertParams::SYN
+  std::optional<FieldH> f2m, f2m_inv, m2m, m2m_inv;
+  SynFields out;
+  for (size_t level = 0; level < recipe.levels(); ++level) {
+    check_cancel(cancel);
+    const auto t0 = std::chrono::steady_clock::now();
+    const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing);
+    const Grid grid = level_virtual_grid(fixed_grid, factors);
+    const double sigma = recipe.smoothing_sigmas[level];
+    std::vector<ImageH> fixed_smooth, moving_smooth;
+    for (size_t c = 0; c < nc; ++c) {
+      if (c == 0 && cache) {
+        auto [f, m] = cache->get(channels[c].fixed, channels[c].moving, sigma, recipe.sigma_units);
+        fixed_smooth.push_back(b.upload_image(*f));
+        moving_smooth.push_back(b.upload_image(*m));
+      } else {
+        fixed_smooth.push_back(b.upload_image(smooth_for_level(channels[c].fixed, sigma, recipe.sigma_units)));
+        moving_smooth.push_back(b.upload_image(smooth_for_level(channels[c].moving, sigma, recipe.sigma_units)));
+      }
+    }
+    if (!f2m) {
+      f2m = b.zero_field(grid);
+      f2m_inv = b.zero_field(grid);
+      m2m = b.zero_field(grid);
+      m2m_inv = b.zero_field(grid);
+    } else {
+      // syn.rs:861-882: resample the four fields onto the next level grid
+      f2m = b.resample_field(*f2m, grid);
+      f2m_inv = b.resample_field(*f2m_inv, grid);
+      m2m = b.resample_field(*m2m, grid);
+      m2m_inv = b.resample_field(*m2m_inv, grid);
+    }
+    WindowConvergenceMonitor monitor(recipe.convergence_window);
+    SynLevelTrace trace;
+    trace.level = level;
+    trace.shrink_factors = factors;
+    trace.grid_size = grid.size;
+    uint32_t current = 0;
+    while (current < recipe.iterations[level] && !trace.converged) {
+      check_cancel(cancel);
+      ++current;
+      const auto t_it = std::chrono::steady_clock::now();
+      // 1. resample every channel into the middle (syn.rs:895-911)
+      detail::Side<B> fs, ms;
+      for (size_t c = 0; c < nc; ++c) {
+        ImageH fmid = b.warp_through_field(fixed_smooth[c], *f2m_inv, nullptr, nullptr);
+        ImageH mmid = b.warp_through_field(moving_smooth[c], *m2m_inv, nullptr, &initial);
+        // 2. the metric both ways (syn.rs:337-420 `side_updates`)
+        const bool single = nc == 1;
+        MetricPair<FieldH> pair = recipe.metric(c) == SynMetric::Cc
+                                      ? b.cc_pair(fmid, mmid, nullptr, recipe.cc_radius, !single)
+                                      : b.mi_pair(fmid, mmid, nullptr, nullptr, recipe.mi_bins, !single);
+        fs.add(b, normalised[c], pair.values[0], std::move(pair.fields[0]), single ? nullptr : &pair.norms[0]);
+        ms.add(b, normalised[c], pair.values[1], std::move(pair.fields[1]), single ? nullptr : &pair.norms[1]);
+      }
+      double weighted[2];
+      FieldH grads[2];
+      detail::Side<B>* sides[2] = {&fs, &ms};
+      for (int s = 0; s < 2; ++s) {
+        detail::Side<B>& side = *sides[s];
+        if (!side.gradient) invalid("syn", "no channel produced a gradient");
+        if (nc == 1) {
+          weighted[s] = side.values[0];
+        } else {
+          b.scale(*side.gradient, side.total_magnitude / double(nc));
+          weighted[s] = 0.0;
+          for (size_t c = 0; c < nc; ++c) weighted[s] += side.values[c] * normalised[c];
+        }
+        grads[s] = b.smooth_field(std::move(*side.gradient), recipe.update_field_variance);
+      }
+      // 3. scale each update so its largest voxel step is the gradient step
+      auto [fmax, mmax] = b.max_scaled_norm_pair(grads[0], &grads[1]);
+      if (!(std::isfinite(fmax) && std::isfinite(mmax))) invalid("syn", "the metric gradient is not finite");
+      double scales[2] = {recipe.gradient_step, recipe.gradient_step};
+      if (fmax > 0.0) scales[0] /= fmax;
+      if (mmax > 0.0) scales[1] /= mmax;
+      b.scale(grads[0], scales[0]);
+      b.scale(grads[1], scales[1]);
+      // 4-5. compose into the total fields and re-estimate both inverses (syn.rs:422-445)
+      SynIteration it;
+      std::optional<FieldH>* fields[2] = {&f2m, &m2m};
+      std::optional<FieldH>* inverses[2] = {&f2m_inv, &m2m_inv};
+      for (int s = 0; s < 2; ++s) {
+        FieldH composed = b.compose(grads[s], **fields[s]);
+        fields[s]->reset();
+        FieldH tmp = b.smooth_field(std::move(composed), recipe.total_field_variance);
+        auto inv = b.invert(tmp, std::move(**inverses[s]), inv_params);
+        auto fwd = b.invert(inv.first, std::move(tmp), inv_params);
+        for (const InvertStats& st : {inv.second, fwd.second})
+          if (!(std::isfinite(st.mean_error_norm) && std::isfinite(st.max_error_norm)))
+            invalid("syn", "the displacement field is not finite");
+        *inverses[s] = std::move(inv.first);
+        *fields[s] = std::move(fwd.first);
+        it.inversions[2 * s] = inv.second;
+        it.inversions[2 * s + 1] = fwd.second;
+      }
+      // 6. energy of the first metric into the monitor (syn.rs:941-948)
+      it.energy = 0.5 * (fs.values[0] + ms.values[0]);
+      it.energy_weighted = 0.5 * (weighted[0] + weighted[1]);
+      monitor.add_energy_value(it.energy);
+      it.convergence_value = monitor.convergence_value();
+      if (it.convergence_value < recipe.convergence_threshold) trace.converged = true;
+      it.fixed_update_scale = scales[0];
+      it.moving_update_scale = scales[1];
+      it.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
+      trace.iterations.push_back(it);
+    }
+    trace.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    out.levels.push_back(std::move(trace));
+  }
+  // syn.rs:969-980: SyN = Minv o F forward, Finv o M inverse
+  FieldH forward = b.compose(*m2m_inv, *f2m);
+  FieldH inverse = b.compose(*f2m_inv, *m2m);
+  out.warp = b.download_field(forward);
+  out.inverse_warp = b.download_field(inverse);
+  return out;
+}
+
+}  // namespace fw

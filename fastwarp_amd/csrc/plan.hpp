@@ -1,0 +1,153 @@
+// Conversion between the C ABI's fw_plan / fw_result and the host recipes of one registration.
+#pragma once
+
+#include "../../include/fastwarp_hip.h"
+#include "host/register.hpp"
+
+namespace fw {
+
+// The cohort recipes: Rigid/Affine `MI[...,1,32,Regular,0.25]`, `[1000x500x250x100,1e-6,10]`,
+// `8x4x2x1`, `3x2x1x0vox` (linear.rs:112-145); `SyN[0.1,3,0]`, `CC[...,1,4]`,
+// `[100x70x50x20,1e-6,10]`, `8x4x2x1`, `3x2x1x0vox` (recipe.rs:93-127).
+inline void cohort_plan(fw_plan* p, int32_t channels) {
+  *p = fw_plan{};
+  const uint32_t lin_it[4] = {1000, 500, 250, 100}, syn_it[4] = {100, 70, 50, 20}, shrink[4] = {8, 4, 2, 1};
+  const double sig[4] = {3.0, 2.0, 1.0, 0.0};
+  p->linear_levels = p->syn_levels = 4;
+  for (int l = 0; l < 4; ++l) {
+    p->linear_iterations[l] = lin_it[l];
+    p->linear_shrink[l] = p->syn_shrink[l] = shrink[l];
+    p->linear_sigmas[l] = p->syn_sigmas[l] = sig[l];
+    p->syn_iterations[l] = syn_it[l];
+  }
+  p->linear_gradient_step = 0.1;
+  p->mi_bins = 32;
+  p->sampling_percentage = 0.25;
+  p->seed = 20260910u;
+  p->linear_convergence_threshold = 1e-6;
+  p->linear_convergence_window = 10;
+  p->syn_gradient_step = 0.1;
+  p->update_variance = 3.0;
+  p->total_variance = 0.0;
+  p->cc_radius = 4;
+  p->channels = channels < 1 ? 1 : channels;
+  for (int c = 0; c < p->channels && c < 8; ++c) p->channel_weights[c] = 1.0 / double(p->channels);
+  p->syn_convergence_threshold = 1e-6;
+  p->syn_convergence_window = 10;
+  p->winsorize_lower = 0.005;
+  p->winsorize_upper = 0.995;
+  p->run_linear = 1;
+  p->run_syn = 1;
+}
+
+inline RegisterPlan plan_from(const fw_plan* p) {
+  if (!p) invalid("stage plan", "null plan");
+  if (p->linear_levels < 1 || p->linear_levels > 8 || p->syn_levels < 1 || p->syn_levels > 8)
+    invalid("stage plan", "between 1 and 8 levels per stage");
+  if (p->channels < 1 || p->channels > 8) invalid("stage plan", "between 1 and 8 channels");
+  RegisterPlan r;
+  for (LinearRecipe* lr : {&r.rigid, &r.affine}) {
+    lr->iterations.assign(p->linear_iterations, p->linear_iterations + p->linear_levels);
+    lr->shrink_factors.assign(p->linear_shrink, p->linear_shrink + p->linear_levels);
+    lr->smoothing_sigmas.assign(p->linear_sigmas, p->linear_sigmas + p->linear_levels);
+    lr->gradient_step = p->linear_gradient_step;
+    lr->bins = size_t(p->mi_bins);
+    lr->sampling_percentage = float(p->sampling_percentage);
+    lr->seed = p->seed;
+    lr->convergence_threshold = p->linear_convergence_threshold;
+    lr->convergence_window = size_t(p->linear_convergence_window);
+  }
+  r.rigid.kind = LinearKind::Rigid;
+  r.affine.kind = LinearKind::Affine;
+  SynRecipe& s = r.syn;
+  s.iterations.assign(p->syn_iterations, p->syn_iterations + p->syn_levels);
+  s.shrink_factors.assign(p->syn_shrink, p->syn_shrink + p->syn_levels);
+  s.smoothing_sigmas.assign(p->syn_sigmas, p->syn_sigmas + p->syn_levels);
+  s.gradient_step = p->syn_gradient_step;
+  s.update_field_variance = p->update_variance;
+  s.total_field_variance = p->total_variance;
+  s.cc_radius = size_t(p->cc_radius);
+  s.channel_weights.assign(p->channel_weights, p->channel_weights + p->channels);
+  s.convergence_threshold = p->syn_convergence_threshold;
+  s.convergence_window = size_t(p->syn_convergence_window);
+  r.winsorize_lower = float(p->winsorize_lower);
+  r.winsorize_upper = float(p->winsorize_upper);
+  r.run_linear = p->run_linear != 0;
+  r.run_syn = p->run_syn != 0;
+  return r;
+}
+
+inline void result_to(const RegisterOutput& o, fw_result* r) {
+  if (!r) return;
+  *r = fw_result{};
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) r->affine.matrix[3 * i + j] = o.total.matrix[i][j];
+    r->affine.translation[i] = o.total.translation[i];
+    r->affine.center[i] = o.total.center[i];
+  }
+  r->seconds_total = o.seconds_total;
+  r->seconds_preprocess = o.seconds_preprocess;
+  r->seconds_linear = o.seconds_linear;
+  r->seconds_syn = o.seconds_syn;
+  r->syn_levels = int32_t(std::min<size_t>(o.syn.levels.size(), 8));
+  for (int l = 0; l < r->syn_levels; ++l) {
+    r->syn_iterations[l] = int32_t(o.syn.levels[l].iterations.size());
+    r->syn_last_energy[l] = o.syn.levels[l].iterations.empty() ? 0.0 : o.syn.levels[l].iterations.back().energy;
+    r->syn_level_seconds[l] = o.syn.levels[l].seconds;
+    double it = 0.0;
+    for (const auto& i : o.syn.levels[l].iterations) it += i.seconds;
+    r->syn_iteration_seconds[l] = it;
+  }
+  const LinearStageResult* st[2] = {&o.linear.rigid, &o.linear.affine};
+  for (int s = 0; s < 2; ++s) {
+    for (size_t l = 0; l < st[s]->levels.size() && l < 8; ++l) {
+      const auto& lv = st[s]->levels[l];
+      r->linear_iterations[s][l] = int32_t(lv.cg.iterations);
+      r->linear_level_seconds[s][l] = lv.seconds;
+      r->linear_iteration_seconds[s][l] = lv.cg.iteration_seconds;
+      r->linear_evaluations[s][l] = int64_t(lv.cg.iterations + lv.cg.line_search_evaluations + (lv.cg.converged ? 1 : 0));
+    }
+    r->linear_final_value[s] = st[s]->levels.empty() || st[s]->levels.back().cg.values.empty() ? 0.0 : st[s]->levels.back().cg.values.back();
+  }
+  for (int i = 0; i < 6; ++i) r->rigid_parameters[i] = o.linear.rigid.transform.params[i];
+  for (int i = 0; i < 12; ++i) r->affine_parameters[i] = o.linear.affine.transform.params[i];
+}
+
+inline Grid grid_of(const fw_grid* g) {
+  if (!g) invalid("grid", "null grid");
+  Mat3 dir;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) dir[r][c] = g->direction[3 * r + c];
+  for (int a = 0; a < 3; ++a)
+    if (g->size[a] <= 0) invalid("grid", "zero size");
+  return Grid({size_t(g->size[0]), size_t(g->size[1]), size_t(g->size[2])}, {g->spacing[0], g->spacing[1], g->spacing[2]},
+              {g->origin[0], g->origin[1], g->origin[2]}, dir);
+}
+
+// Runs one registration and writes the fields (3n interleaved on the fixed grid) when asked.
+template <typename B, typename E>
+void run_register(B& backend, E& engine, const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,
+                  const fw_grid* moving_grid, const double* const* moving, double* forward, double* inverse, fw_result* result,
+                  const volatile int32_t* cancel) {
+  const RegisterPlan p = plan_from(plan);
+  const Grid fg = grid_of(fixed_grid), mg = grid_of(moving_grid);
+  std::vector<Image> f, m;
+  for (int32_t c = 0; c < plan->channels; ++c) {
+    if (!fixed || !moving || !fixed[c] || !moving[c]) invalid("register", "null image data");
+    f.emplace_back(fg, std::vector<double>(fixed[c], fixed[c] + fg.num_voxels()));
+    m.emplace_back(mg, std::vector<double>(moving[c], moving[c] + mg.num_voxels()));
+  }
+  const RegisterOutput o = register_full(backend, engine, p, f, m, cancel);
+  result_to(o, result);
+  auto write = [](const VectorField& v, double* dst) {
+    if (!dst) return;
+    for (size_t i = 0; i < v.data.size(); ++i)
+      for (int c = 0; c < 3; ++c) dst[3 * i + c] = v.data[i][c];
+  };
+  if (p.run_syn) {
+    write(o.syn.warp, forward);
+    write(o.syn.inverse_warp, inverse);
+  }
+}
+
+}  // namespace fw

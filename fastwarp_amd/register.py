@@ -1,0 +1,64 @@
+"""Whole registrations on the HIP engine: the front end of fastwarp's `register_syn`
+(python/fastwarp/_api.py:98 -> stages.rs:286 register_stages_async) over `fw_register`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._abi import Affine, Grid, Plan, Result, dptr
+from .backend import FW_OK, HipBackend, RegError, lib
+
+
+def cohort_plan(channels: int = 1) -> Plan:
+    """The cohort recipe (Rigid/Affine MI, SyN CC radius 4, 8x4x2x1, 3x2x1x0 vox)."""
+    p = Plan()
+    lib().fw_plan_cohort(C.byref(p), channels)
+    return p
+
+
+def plan_2mm(channels: int = 1) -> Plan:
+    """The 2 mm plain registration: Rigid/Affine MI at shrink 4/2/1 (100/50/25 iterations,
+    sigmas 2/1/0 vox), SyN CC radius 4, step 0.1, update variance 3 at shrink 4/2/1 (40/20/10)."""
+    p = cohort_plan(channels)
+    p.set_linear([100, 50, 25], [4, 2, 1], [2.0, 1.0, 0.0])
+    p.set_syn([40, 20, 10], [4, 2, 1], [2.0, 1.0, 0.0])
+    return p
+
+
+@dataclass
+class Registration:
+    affine: Affine
+    forward: np.ndarray | None  # (n, 3) on the fixed grid
+    inverse: np.ndarray | None
+    result: Result
+
+
+def _sig():
+    return lib()
+
+
+def channel_array(arrays):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in arrays]
+    ptrs = (C.POINTER(C.c_double) * len(arrs))(*[dptr(a) for a in arrs])
+    return arrs, ptrs
+
+
+def register(backend: HipBackend, plan: Plan, fixed_grid: Grid, fixed, moving_grid: Grid, moving,
+             fields: bool = True, cancel: C.c_int32 | None = None) -> Registration:
+    """One full normalisation on the device of `backend`."""
+    h = _sig()
+    fa, fp = channel_array(fixed)
+    ma, mp = channel_array(moving)
+    n = fixed_grid.n
+    fwd = np.empty((n, 3)) if fields and plan.run_syn else None
+    inv = np.empty((n, 3)) if fields and plan.run_syn else None
+    res = Result()
+    st = h.fw_register(backend.ctx, C.byref(plan), C.byref(fixed_grid), fp, C.byref(moving_grid), mp,
+                       dptr(fwd) if fwd is not None else None, dptr(inv) if inv is not None else None, C.byref(res),
+                       C.byref(cancel) if cancel is not None else None)
+    if st != FW_OK:
+        raise RegError(st, h.fw_last_error(backend.ctx).decode())
+    return Registration(res.affine, fwd, inv, res)

@@ -218,7 +218,22 @@ typedef struct {
   double linear_final_value[2];
   double rigid_parameters[6];
   double affine_parameters[12];
+  /* per level: wall seconds of the level and of its iterations alone (the rest is set-up:
+   * smoothing, sampling, uploads); linear index 0 is Rigid, 1 Affine */
+  double syn_level_seconds[8];
+  double syn_iteration_seconds[8];
+  double linear_level_seconds[2][8];
+  double linear_iteration_seconds[2][8];
+  int64_t linear_evaluations[2][8]; /* metric evaluations, line searches included */
 } fw_result;
+
+/* ---- profiling (the reference's WgpuBackend::set_profiling / take_profile,
+ *      crates/fastwarp-gpu/src/profiler.rs:95-170) ---- */
+/* While on, every kernel launch is bracketed by HIP events on the context's stream. */
+int fw_set_profiling(fw_ctx* ctx, int on);
+/* Writes one line per kernel family, "name launches total_ms algorithmic_bytes", into `text`
+ * (NUL-terminated, at most `capacity` bytes) and clears the totals. */
+int fw_take_profile(fw_ctx* ctx, char* text, size_t capacity);
 
 /* Fills `plan` with the cohort recipe: Rigid/Affine MI 32 bins, 25 % regular sampling,
  * 1000x500x250x100, shrink 8/4/2/1, sigmas 3/2/1/0 vox; SyN[0.1,3,0], CC radius 4,

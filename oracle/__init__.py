@@ -176,3 +176,23 @@ def norms(grid: Grid, field) -> tuple[float, float]:
     a, b = C.c_double(), C.c_double()
     _check(lib().fwo_norms(C.byref(grid), dptr(f), C.byref(a), C.byref(b)))
     return a.value, b.value
+
+
+def register(plan, fixed_grid: Grid, fixed, moving_grid: Grid, moving, fields: bool = True):
+    """The whole f64 CPU engine (engine="cpu") on the same plan as the HIP path: the generic
+    drivers over the oracle's ops. Returns (Result, forward, inverse)."""
+    from fastwarp_amd._abi import Plan, Result
+    from fastwarp_amd.register import channel_array
+
+    h = lib()
+    fa, fp = channel_array(fixed)
+    ma, mp = channel_array(moving)
+    n = fixed_grid.n
+    fwd = np.empty((n, 3)) if fields and plan.run_syn else None
+    inv = np.empty((n, 3)) if fields and plan.run_syn else None
+    res = Result()
+    _check(
+        h.fwo_register(C.byref(plan), C.byref(fixed_grid), fp, C.byref(moving_grid), mp,
+                       dptr(fwd) if fwd is not None else None, dptr(inv) if inv is not None else None, C.byref(res))
+    )
+    return res, fwd, inv

@@ -18,16 +18,14 @@ def build(verbose: bool = False) -> Path:
     cmd = [
         os.environ.get("CXX", "g++"),
         "-std=c++17",
-        "-O2",
+        "-O3",
         "-fPIC",
         "-shared",
         str(HERE / "capi.cpp"),
-        str(HERE / "engine.cpp"),
         "-o",
         str(tmp),
         "-lpthread",
     ]
-    cmd = [c for c in cmd if not c.endswith("engine.cpp") or Path(c).exists()]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     if verbose:

@@ -6,6 +6,8 @@
 #include <string>
 
 #include "../include/fastwarp_hip.h"
+#include "../fastwarp_amd/csrc/plan.hpp"
+#include "engine.hpp"
 #include "oracle.hpp"
 
 using namespace fw;
@@ -203,7 +205,7 @@ int fwo_invert(const fw_grid* grid, const double* field, const double* initial, 
     VectorField f = field_from(grid, field);
     VectorField init;
     if (initial) init = field_from(grid, initial);
-    InvertParams q;
+    oracle::InvertParams q;
     q.max_iterations = p->max_iterations;
     q.mean_error_tolerance = p->mean_error_tolerance;
     q.max_error_tolerance = p->max_error_tolerance;
@@ -231,3 +233,14 @@ int fwo_norms(const fw_grid* grid, const double* field, double* max_scaled, doub
 double fwo_cubic_bspline(double u, int derivative) { return derivative ? cubic_bspline_derivative(u) : cubic_bspline(u); }
 
 }  // extern "C"
+
+// The whole CPU engine: the generic drivers over the oracle's f64 ops (engine="cpu").
+extern "C" int fwo_register(const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,
+                            const fw_grid* moving_grid, const double* const* moving, double* forward, double* inverse,
+                            fw_result* result) {
+  return guard([&] {
+    CpuBackend backend;
+    CpuLinearEngine engine;
+    fw::run_register(backend, engine, plan, fixed_grid, fixed, moving_grid, moving, forward, inverse, result, nullptr);
+  });
+}

@@ -1,0 +1,95 @@
+// ORACLE — test infrastructure only. The f64 CPU engine: the oracle's ops behind the backend and
+// linear-engine interfaces that the generic drivers (SyN loop, linear stages) are written against,
+// as the reference's `CpuBackend` and `CpuLinearEngine` (crates/fastwarp-reg/src/cpu/mod.rs:386-597,
+// linear.rs:323-400) sit behind its `Backend` and `LinearEngine` traits.
+#pragma once
+
+#include "../fastwarp_amd/csrc/host/backend_types.hpp"
+#include "../fastwarp_amd/csrc/host/register.hpp"
+#include "oracle.hpp"
+
+namespace fw::oracle {
+
+class CpuBackend {
+ public:
+  using ImageH = Image;
+  using FieldH = VectorField;
+
+  void release_idle_buffers() {}
+  Image upload_image(const Image& v) { return v; }
+  VectorField zero_field(const Grid& g) { return VectorField::zeros(g); }
+  VectorField download_field(const VectorField& f) { return f; }
+  Grid field_grid(const VectorField& f) const { return f.grid; }
+  Image warp_through_field(const Image& src, const VectorField& f, const VectorField* pre, const AffineTransform* a) {
+    return oracle::warp_through_field(src, f, pre, a);
+  }
+  MetricPair<VectorField> cc_pair(const Image& f, const Image& m, const Image* valid, size_t radius, bool norms) {
+    MetricPair<VectorField> out;
+    PointMap id;
+    const Image* ab[2][2] = {{&f, &m}, {&m, &f}};
+    for (int d = 0; d < 2; ++d) {
+      CcEvaluation e = cc_evaluate(f.grid, *ab[d][0], *ab[d][1], id, id, valid, radius, true);
+      out.values[d] = e.value;
+      out.fields[d] = VectorField(f.grid, std::move(e.derivative));
+      if (norms) out.norms[d] = l2_norm(out.fields[d]);
+    }
+    return out;
+  }
+  MetricPair<VectorField> mi_pair(const Image& f, const Image& m, const Image* fm, const Image* mm, size_t bins, bool norms) {
+    MetricPair<VectorField> out;
+    PointMap id;
+    for (int d = 0; d < 2; ++d) {
+      MiDenseEvaluation e = d == 0 ? mi_dense(f.grid, f, m, id, id, fm, mm, bins, true) : mi_dense(f.grid, m, f, id, id, mm, fm, bins, true);
+      out.values[d] = e.value;
+      out.fields[d] = VectorField(f.grid, std::move(e.derivative));
+      if (norms) out.norms[d] = l2_norm(out.fields[d]);
+    }
+    return out;
+  }
+  void axpy(VectorField& acc, double a, const VectorField& x) {
+    if (acc.grid != x.grid) invalid("field axpy", "grids differ");
+    for (size_t i = 0; i < acc.data.size(); ++i)
+      for (int c = 0; c < 3; ++c) acc.data[i][c] += a * x.data[i][c];
+  }
+  void scale(VectorField& f, double s) {
+    for (auto& v : f.data)
+      for (int c = 0; c < 3; ++c) v[c] *= s;
+  }
+  VectorField smooth_field(VectorField f, double variance) { return oracle::smooth_field(f, variance); }
+  std::pair<double, double> max_scaled_norm_pair(const VectorField& a, const VectorField* b) {
+    return {max_scaled_norm(a), b ? max_scaled_norm(*b) : 0.0};
+  }
+  VectorField compose(const VectorField& u, const VectorField& w) { return oracle::compose(u, w); }
+  std::pair<VectorField, fw::InvertStats> invert(const VectorField& field, VectorField initial, const fw::InvertParams& p) {
+    oracle::InvertParams q{p.max_iterations, p.mean_error_tolerance, p.max_error_tolerance, p.enforce_boundary_condition};
+    auto r = oracle::invert(field, &initial, q);
+    return {std::move(r.first), fw::InvertStats{r.second.iterations, r.second.mean_error_norm, r.second.max_error_norm}};
+  }
+  VectorField resample_field(const VectorField& f, const Grid& g) { return oracle::resample_field(f, g); }
+};
+
+class CpuLinearEngine {
+ public:
+  struct Level {
+    Image moving;
+    std::vector<Vec3> points;
+    std::vector<uint32_t> fixed_bin;
+    Binning binning;
+    double value(const LinearMovingTransform& t) { return mi_sampled(binning, moving, points, fixed_bin, t, false).value; }
+    std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t) {
+      MiSampledEvaluation e = mi_sampled(binning, moving, points, fixed_bin, t, true);
+      return {e.value, e.derivative};
+    }
+  };
+  Level prepare_level(const Image& moving, const std::vector<Vec3>& points, const std::vector<uint32_t>& fixed_bin,
+                      const MiBinning& b) {
+    Level l;
+    l.moving = moving;
+    l.points = points;
+    l.fixed_bin = fixed_bin;
+    l.binning = Binning::from_ranges(b.bins, b.fixed_min, b.fixed_max, b.moving_min, b.moving_max);
+    return l;
+  }
+};
+
+}  // namespace fw::oracle

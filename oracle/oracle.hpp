@@ -377,8 +377,8 @@ struct Binning {
   }
   double moving_term(double v) const { return v / moving_bin_size - moving_normalized_min; }
   size_t clamp_bin(double term) const {
-    long long idx = (long long)term;  // truncation toward zero, as Rust `as i64`
-    if (std::isnan(term)) idx = 0;
+    // saturating truncation toward zero, as Rust's `as i64`
+    long long idx = std::isnan(term) ? 0 : (long long)std::max(-1e18, std::min(1e18, term));
     return size_t(std::clamp(idx, (long long)kPadding, (long long)bins - 3));
   }
   size_t fixed_bin(double v) const { return clamp_bin(v / fixed_bin_size - fixed_normalized_min); }
