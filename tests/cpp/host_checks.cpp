@@ -1,0 +1,252 @@
+// Checks of the product's host code (the C++ drivers' building blocks) against the reference's
+// published values and closed forms. Built and run by tests/test_host.py; prints "ok <name>" per
+// passing check and exits non-zero on the first failure.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+#include "../../fastwarp_amd/csrc/host/gauss_taps.hpp"
+#include "../../fastwarp_amd/csrc/host/linear.hpp"
+#include "../../fastwarp_amd/csrc/host/mi_host.hpp"
+#include "../../fastwarp_amd/csrc/host/preprocess.hpp"
+#include "../../oracle/oracle.hpp"
+
+using namespace fw;
+
+#define CHECK(cond, name)                                        \
+  do {                                                           \
+    if (!(cond)) {                                               \
+      std::printf("FAIL %s (%s:%d)\n", name, __FILE__, __LINE__); \
+      std::exit(1);                                              \
+    }                                                            \
+  } while (0)
+
+static void ok(const char* n) { std::printf("ok %s\n", n); }
+
+// sampling.rs:243-266: the published MT19937 outputs for seed 5489.
+static void mt19937() {
+  Mt19937 mt(5489);
+  const uint32_t first[10] = {3499211612u, 581869302u, 3890346734u, 3586334585u, 545404204u,
+                              4161255391u, 3922919429u, 949333985u,  2715962298u, 1323567403u};
+  for (uint32_t v : first) CHECK(mt.integer() == v, "mt19937 first outputs");
+  Mt19937 m2(5489);
+  uint32_t last = 0;
+  for (int i = 0; i < 10000; ++i) last = m2.integer();
+  CHECK(last == 4123659995u, "mt19937 10000th output");
+  ok("mt19937");
+}
+
+// sampling.rs:302-340: REGULAR sampling takes every 4th voxel, jittered with the generator's
+// Box-Muller draws in order (x first), and the seed advances once per call.
+static void regular_sampling() {
+  const Grid g({8, 4, 2}, {1, 1, 1}, {0, 0, 0}, kIdentity3);
+  SeedSequence seeds(20260910u);
+  auto a = metric_sample_points(g, 0.25f, seeds);
+  CHECK(a.size() == 16, "regular sample count");
+  CHECK(seeds.peek() == 20260911u, "seed advanced once");
+  Mt19937 rng(20260910u);
+  for (int q = 0; q < 2; ++q) {
+    const Vec3 base = g.index_to_point({double((q * 4) % 8), double((q * 4 / 8) % 4), 0.0});
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t u1 = rng.integer(), u2 = rng.integer();
+      const double want = base[d] + normal_from(u1, u2, 0.0, 1.0) / 3.0;
+      CHECK(std::fabs(a[q][d] - want) < 1e-15, "regular jitter");
+    }
+  }
+  ok("regular_sampling");
+}
+
+// pyramid.rs:478-494: the ITK shrink rule on the 1 mm MNI-like grid and the ANTs factors.
+static void shrink() {
+  const Grid g({197, 233, 189}, {1, 1, 1}, {-98, -126, -72}, kIdentity3);
+  const Grid s = shrink_grid(g, {8, 8, 8});
+  CHECK(s.size[0] == 24 && s.size[1] == 29 && s.size[2] == 23, "shrink sizes");
+  CHECK(std::fabs(s.origin[0] - (-98.0 + (98.0 - 11.5 * 8.0))) < 1e-12, "shrink origin x");
+  CHECK(std::fabs(s.origin[1] - (-126.0 + (116.0 - 14.0 * 8.0))) < 1e-12, "shrink origin y");
+  CHECK(std::fabs(s.origin[2] - (-72.0 + (94.0 - 11.0 * 8.0))) < 1e-12, "shrink origin z");
+  auto f = shrink_factors_per_dimension(4, {1.0, 1.0, 2.0});
+  CHECK(f[0] == 4 && f[1] == 4 && f[2] == 2, "factors 4 anisotropic");
+  f = shrink_factors_per_dimension(4, {0.5, 0.5, 3.0});
+  CHECK(f[0] == 4 && f[1] == 4 && f[2] == 1, "factors thick slices");
+  f = shrink_factors_per_dimension(8, {2.0, 1.0, 1.0});
+  CHECK(f[0] == 4 && f[1] == 8 && f[2] == 8, "factors smallest spacing");
+  const Grid one = shrink_grid(Grid({7, 6, 5}, {1, 2, 3}, {1, 2, 3}, kIdentity3), {100, 100, 100});
+  CHECK(one.size[0] == 1 && one.size[1] == 1 && one.size[2] == 1, "collapse to one voxel");
+  ok("shrink");
+}
+
+// pyramid.rs:544-584: unit DC gain, the requested variance, a preserved constant.
+static void deriche() {
+  const Grid g({81, 4, 4}, {2.0, 1.0, 1.0}, {0, 0, 0}, kIdentity3);
+  std::vector<double> d(g.num_voxels(), 0.0);
+  d[g.linear_index(40, 1, 1)] = 1.0;
+  std::vector<double> line = d;
+  deriche_axis(line, g.size, 0, Deriche::zero_order(6.0, 2.0));
+  double sum = 0, var = 0;
+  for (size_t i = 0; i < 81; ++i) {
+    const double w = line[g.linear_index(i, 1, 1)];
+    sum += w;
+    var += w * (double(i) - 40.0) * (double(i) - 40.0);
+  }
+  CHECK(std::fabs(sum - 1.0) < 1e-6, "deriche dc gain");
+  CHECK(std::fabs(var - 9.0) < 0.9, "deriche variance");
+  const Grid c({4, 4, 4}, {1, 1, 1}, {0, 0, 0}, kIdentity3);
+  Image k(c, std::vector<double>(64, 7.0));
+  Image s = smooth_recursive_gaussian(k, {1.5, 1.5, 1.5});
+  for (double v : s.data) CHECK(std::fabs(v - 7.0) < 1e-9, "deriche constant");
+  ok("deriche");
+}
+
+// convergence.rs:188-213: the closed form of the two-control-point fit.
+static void convergence() {
+  WindowConvergenceMonitor m(10);
+  const double vals[10] = {9.0, 8.5, 8.2, 8.0, 7.9, 7.85, 7.82, 7.80, 7.79, 7.785};
+  double total = 0;
+  for (double v : vals) {
+    m.add_energy_value(v);
+    total += v;
+  }
+  const double eps = 1e-4;
+  double d0 = 0, d1 = 0, o0 = 0, o1 = 0;
+  for (int n = 0; n < 10; ++n) {
+    double p = n / 9.0;
+    if (std::fabs(p - 1.0) <= eps) p = 1.0 - eps;
+    const double t0 = 1.0 - p, t1 = p, s = t0 * t0 + t1 * t1, y = vals[n] / total;
+    o0 += t0 * t0;
+    o1 += t1 * t1;
+    d0 += y * t0 * t0 * t0 / s;
+    d1 += y * t1 * t1 * t1 / s;
+  }
+  CHECK(std::fabs(m.convergence_value() - (d0 / o0 - d1 / o1)) < 1e-14, "monitor closed form");
+  WindowConvergenceMonitor early(10);
+  early.add_energy_value(1.0);
+  CHECK(early.convergence_value() == std::numeric_limits<double>::max(), "monitor window not full");
+  ok("convergence");
+}
+
+// optimizer.rs:526-548: the conjugate gradient with the golden section minimises a quadratic.
+struct Quadratic {
+  std::array<double, 6> target{0, 0, 0, 3, -2, 1};
+  double value(const LinearMovingTransform& t) const {
+    double s = 0;
+    for (int i = 0; i < 6; ++i) s += (t.optimised.params[i] - target[i]) * (t.optimised.params[i] - target[i]);
+    return s;
+  }
+  std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t) const {
+    std::vector<double> d(6);
+    for (int i = 0; i < 6; ++i) d[i] = -2.0 * (t.optimised.params[i] - target[i]);
+    return {value(t), d};
+  }
+};
+
+static void optimizer() {
+  Quadratic q;
+  std::array<Vec3, 8> corners{};
+  for (int i = 0; i < 8; ++i) corners[i] = {(i & 1) ? 20.0 : -20.0, (i & 2) ? 20.0 : -20.0, (i & 4) ? 20.0 : -20.0};
+  LinearMovingTransform t{AffineTransform::identity(), ParametricTransform::euler3d({0, 0, 0, 0, 0, 0}, {0, 0, 0})};
+  CgSettings st;
+  st.iterations = 200;
+  st.convergence_threshold = 1e-12;
+  auto [best, trace] = optimise_level(q, corners, t, st);
+  CHECK(std::fabs(best.optimised.params[3] - 3.0) < 0.05, "cg x");
+  CHECK(std::fabs(best.optimised.params[4] + 2.0) < 0.05, "cg y");
+  CHECK(std::fabs(best.optimised.params[5] - 1.0) < 0.05, "cg z");
+  CHECK(trace.best_value < 0.01 && trace.line_search_evaluations > 0, "cg trace");
+  // optimizer.rs:477-510: scales are squared corner shifts per unit parameter
+  std::array<Vec3, 8> cube{};
+  for (int i = 0; i < 8; ++i) cube[i] = {(i & 1) ? 50.0 : -50.0, (i & 2) ? 50.0 : -50.0, (i & 4) ? 50.0 : -50.0};
+  auto sc = estimate_scales(cube, t, 0.01);
+  for (int i = 3; i < 6; ++i) CHECK(std::fabs(sc[i] - 1.0) < 1e-9, "translation scales");
+  const double want = 2 * 50.0 * 50.0;
+  for (int i = 0; i < 3; ++i) CHECK(std::fabs(sc[i] - want) / want < 1e-3, "rotation scales");
+  CHECK(std::fabs(estimate_step_scale(cube, t, {0, 0, 0, 3, 4, 0}, 0.01) - 5.0) < 1e-9, "step scale");
+  ok("optimizer");
+}
+
+// transform/param.rs:398-430: the Euler and affine Jacobians against central differences.
+static void jacobians() {
+  for (int kind = 0; kind < 2; ++kind) {
+    ParametricTransform t = kind == 0 ? ParametricTransform::euler3d({0.1, -0.2, 0.3, 1, 2, 3}, {5, -4, 2})
+                                      : ParametricTransform::affine(AffineTransform{{{{1.02, 0.03, -0.01}, {-0.02, 0.97, 0.04}, {0.01, 0.02, 1.01}}}, {0.5, -0.3, 0.2}, {5, -4, 2}});
+    const Vec3 p{12.0, -7.0, 31.0};
+    std::array<Vec3, 12> cols;
+    const size_t n = t.jacobian(p, cols);
+    for (size_t k = 0; k < n; ++k) {
+      auto up = t, dn = t;
+      up.params[k] += 1e-6;
+      dn.params[k] -= 1e-6;
+      const Vec3 a = up.to_affine().apply(p), b = dn.to_affine().apply(p);
+      for (int d = 0; d < 3; ++d) CHECK(std::fabs((a[d] - b[d]) / 2e-6 - cols[k][d]) < 1e-6, "jacobian column");
+    }
+  }
+  ok("jacobians");
+}
+
+// The f32 thresholds reproduce the f64 fixed bin of every f32 intensity (wgpu_backend/mi/dense.rs
+// idea; checked exhaustively near each threshold and on random values).
+static void thresholds() {
+  const MiBinning b = MiBinning::from_ranges(32, 0.0312, 0.9471, -0.2, 1.3);
+  const std::vector<float> t = b.fixed_thresholds();
+  CHECK(t.size() == 27, "threshold count");
+  auto bin_of = [&](float v) {
+    int k = 2;
+    for (float th : t) k += v >= th ? 1 : 0;
+    return size_t(k);
+  };
+  std::mt19937 rng(3);
+  std::uniform_real_distribution<float> u(-0.2f, 1.2f);
+  for (int i = 0; i < 200000; ++i) {
+    const float v = u(rng);
+    CHECK(bin_of(v) == b.fixed_bin(double(v)), "random bins");
+  }
+  for (float th : t) {
+    CHECK(b.fixed_bin(double(th)) > b.fixed_bin(double(std::nextafter(th, -1.0f))), "threshold is a bin edge");
+    CHECK(bin_of(th) == b.fixed_bin(double(th)) && bin_of(std::nextafter(th, -1.0f)) == b.fixed_bin(double(std::nextafter(th, -1.0f))),
+          "edge bins");
+  }
+  ok("thresholds");
+}
+
+// The product's Gaussian taps equal the oracle's restatement.
+static void taps() {
+  for (double v : {0.25, 0.5, 1.0, 3.0, 8.0})
+    for (size_t w : {3u, 21u, 1000u}) {
+      const auto a = gaussian_operator_taps(v, 0.001, w);
+      const auto b = oracle::gaussian_taps(v, 0.001, w);
+      CHECK(a.size() == b.size(), "taps size");
+      for (size_t i = 0; i < a.size(); ++i) CHECK(a[i] == b[i], "taps value");
+    }
+  ok("taps");
+}
+
+// preprocess.rs: the window maps the quantile range onto [0, 1]; quantiles (0, 1) of a ramp give
+// its minimum and (within the histogram margin) its maximum.
+static void preprocess() {
+  const Grid g({100, 1, 1}, {1, 1, 1}, {0, 0, 0}, kIdentity3);
+  std::vector<double> d(100);
+  for (int i = 0; i < 100; ++i) d[i] = 3.0 + 0.5 * i;
+  const Image v(g, d);
+  const IntensityHistogram h(v.data);
+  CHECK(std::fabs(h.quantile(0.0) - 3.0) < 1e-9, "quantile 0 is the minimum");
+  CHECK(std::fabs(h.quantile(1.0) - (52.5 + 49.5 / 2560.0)) < 1e-4, "quantile 1 is the margined maximum");
+  const Image p = preprocess_volume(v, 0.0f, 1.0f);
+  CHECK(p.data[0] == 0.0 && p.data[99] <= 1.0 && p.data[99] > 0.99, "window to [0, 1]");
+  const Image w = preprocess_volume(v, 0.1f, 0.9f);
+  CHECK(w.data[0] == 0.0 && w.data[99] == 1.0, "winsorised ends clamp");
+  ok("preprocess");
+}
+
+int main() {
+  mt19937();
+  regular_sampling();
+  shrink();
+  deriche();
+  convergence();
+  optimizer();
+  jacobians();
+  thresholds();
+  taps();
+  preprocess();
+  std::printf("all host checks passed\n");
+  return 0;
+}
