@@ -20,7 +20,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else "registrations/hour"
+METRIC = "1mm SyN normalizations/hour batched+served vs plain; p50/p99 latency; HBM GB/s"
 
 
 def log(msg: str):
@@ -105,7 +105,7 @@ def main() -> int:
     backend = HipBackend(0)
     for w in range(args.warmup):
         t = time.perf_counter()
-        r = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=False)
+        r = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving)
         log(f"warmup {w + 1}/{args.warmup}: {time.perf_counter() - t:.2f} s (linear {r.result.seconds_linear:.2f}, syn {r.result.seconds_syn:.2f})")
     backend.set_profiling(True)
     lat = []
@@ -113,11 +113,13 @@ def main() -> int:
     t_all = time.perf_counter()
     for k in range(args.steps):
         t = time.perf_counter()
-        last = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=False)
+        last = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving)
         lat.append(time.perf_counter() - t)
         log(f"step {k + 1}/{args.steps}: {lat[-1]:.2f} s (linear {last.result.seconds_linear:.2f}, syn {last.result.seconds_syn:.2f})")
     total = time.perf_counter() - t_all
     profile = backend.take_profile()
+    host = {k[5:]: round(v[1] / 1e3 / args.steps, 4) for k, v in profile.items() if k.startswith("host:")}
+    profile = {k: v for k, v in profile.items() if not k.startswith("host:")}
     backend.set_profiling(False)
     held, peak = backend.memory_stats()
     line = {
@@ -144,6 +146,7 @@ def main() -> int:
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "phases_s": {"linear": round(last.result.seconds_linear, 3), "syn": round(last.result.seconds_syn, 3),
                      "preprocess": round(last.result.seconds_preprocess, 3)},
+        "host_s_per_step": dict(sorted(host.items(), key=lambda kv: -kv[1])),
         "device_peak_bytes": int(peak),
         "roofline": roofline(profile),
         "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "gbs": round(v[2] / max(v[1], 1e-9) / 1e6, 1)}

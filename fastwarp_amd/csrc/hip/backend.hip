@@ -20,6 +20,7 @@ DeviceContext::DeviceContext(int device) : device_(device) {
 }
 
 DeviceContext::~DeviceContext() {
+  if (host_span_sink() == this) host_span_sink() = nullptr;
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
   release_idle();
@@ -100,7 +101,10 @@ void* DeviceContext::pinned(size_t bytes) {
   return pinned_;
 }
 
-void DeviceContext::sync() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+void DeviceContext::sync() {
+  HostTimer t("sync_wait");
+  check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
 
 const float* DeviceContext::taps(double variance, size_t length, int& radius) {
   auto key = std::make_pair(variance, length);
@@ -120,6 +124,15 @@ const float* DeviceContext::taps(double variance, size_t length, int& radius) {
 void DeviceContext::set_profiling(bool on) {
   if (!on && profiling_) (void)take_profile();
   profiling_ = on;
+  HostSpanSink*& sink = host_span_sink();
+  if (on) sink = this;
+  else if (sink == this) sink = nullptr;
+}
+
+void DeviceContext::host_span(const char* name, double seconds) {
+  Total& t = totals_[std::string("host:") + name];
+  t.launches += 1;
+  t.ms += seconds * 1e3;
 }
 
 hipEvent_t DeviceContext::event() {
@@ -178,6 +191,7 @@ HipBackend::HipBackend(int device) : ctx_(std::make_shared<DeviceContext>(device
 // --------------------------------------------------------------------- transfers (a1, a2) ---
 
 HImage HipBackend::upload_image(const Image& v) {
+  HostTimer host_timer("upload_image");
   const size_t n = v.grid.num_voxels();
   std::vector<float> f(n);
   parallel_for(n, [&](size_t b, size_t e) {
@@ -212,6 +226,7 @@ HField HipBackend::zero_field(const Grid& g) {
 }
 
 HField HipBackend::upload_field(const VectorField& v) {
+  HostTimer host_timer("upload_field");
   const size_t n = v.grid.num_voxels();
   std::vector<float> soa(3 * n);
   parallel_for(n, [&](size_t b, size_t e) {
@@ -226,6 +241,7 @@ HField HipBackend::upload_field(const VectorField& v) {
 }
 
 VectorField HipBackend::download_field(const HField& f) {
+  HostTimer host_timer("download_field");
   const size_t n = f.grid.num_voxels();
   std::vector<float> soa(3 * n);
   check(hipMemcpyAsync(soa.data(), f.buf.get(), soa.size() * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()),
@@ -493,36 +509,25 @@ std::pair<HField, InvertStats> HipBackend::invert(const HField& field, HField in
   const float3 isp = inv_spacing(g);
   const int nb = inv_blocks(d);
   auto* state = static_cast<InvertState*>(ctx_->scratch(4, sizeof(InvertState)));
-  auto* psum = static_cast<float*>(ctx_->scratch(5, nb * sizeof(float)));
-  auto* pmax = static_cast<float*>(ctx_->scratch(6, nb * sizeof(float)));
+  auto* partials = static_cast<float*>(ctx_->scratch(5, 4 * nb * sizeof(float)));
   hipStream_t s = ctx_->stream();
-  launch_inv_init(state, int(p.max_iterations), s);
-  for (uint32_t step = 0; step < p.max_iterations; ++step) {
-    {
-      Timed t(*ctx_, "k_inv_residual", 0.0);
-      launch_inv_residual(field.buf.get(), initial.buf.get(), d, l, isp, state, psum, pmax, nb, s);
-    }
-    {
-      Timed t(*ctx_, "k_inv_reduce", 0.0);
-      launch_inv_reduce(psum, pmax, nb, d.n(), state, int(step), int(p.max_iterations), p.max_error_tolerance,
-                        p.mean_error_tolerance, s);
-    }
-    {
-      Timed t(*ctx_, "k_inv_update", 0.0);
-      launch_inv_update(field.buf.get(), initial.buf.get(), d, l, isp, state, int(step), p.enforce_boundary_condition ? 1 : 0, s);
-    }
+  const int max_it = int(p.max_iterations);
+  {
+    Timed t(*ctx_, "k_inv_begin", 24.0 * double(d.n()));
+    launch_inv_begin(field.buf.get(), initial.buf.get(), d, l, isp, state, partials, nb, max_it, s);
+  }
+  for (int step = 0; step < max_it; ++step) {
+    Timed t(*ctx_, "k_inv_iter", 0.0);
+    launch_inv_iter(field.buf.get(), initial.buf.get(), d, l, isp, state, partials, nb, step, max_it,
+                    p.max_error_tolerance, p.mean_error_tolerance, p.enforce_boundary_condition ? 1 : 0, s);
   }
   auto* host = static_cast<InvertState*>(ctx_->pinned(sizeof(InvertState)));
   check(hipMemcpyAsync(host, state, sizeof(InvertState), hipMemcpyDeviceToHost, s), "invert state");
   ctx_->sync();
   InvertStats st;
   st.iterations = uint32_t(host->executed);
-  if (ctx_->profiling()) {
-    // bytes only for the steps that ran: the residual reads psi and the field, the update reads
-    // both and writes psi
-    ctx_->add_bytes("k_inv_residual", 24.0 * double(d.n()) * host->executed);
-    ctx_->add_bytes("k_inv_update", 36.0 * double(d.n()) * host->executed);
-  }
+  // each executed iteration reads psi and the field and writes psi
+  if (ctx_->profiling()) ctx_->add_bytes("k_inv_iter", 36.0 * double(d.n()) * host->executed);
   st.mean_error_norm = host->executed ? double(host->mean_error) : 0.0;
   st.max_error_norm = host->executed ? double(host->max_error) : 0.0;
   return {std::move(initial), st};

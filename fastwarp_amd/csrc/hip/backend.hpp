@@ -28,7 +28,7 @@ namespace fw::hip {
 
 void check(hipError_t e, const char* what);
 
-class DeviceContext {
+class DeviceContext : public HostSpanSink {
  public:
   explicit DeviceContext(int device);
   ~DeviceContext();
@@ -57,6 +57,8 @@ class DeviceContext {
   hipEvent_t event();
   void span(const char* name, double bytes, hipEvent_t a, hipEvent_t b);
   void add_bytes(const char* name, double bytes);
+  // Host work of the drivers while profiling, listed as "host:<name>" (sync waits included).
+  void host_span(const char* name, double seconds) override;
   std::string take_profile();
 
  private:

@@ -507,80 +507,119 @@ void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const floa
 }
 
 // ------------------------------------------------------------------------------------------
-// Inversion (field_ops.rs:315-392, ITK InvertDisplacementFieldImageFilter) as device-controlled
-// steps: residual partials -> one-block reduction that decides whether the step runs -> update.
-// The residual e = psi + field(x + psi) is recomputed by the update instead of being stored.
+// Inversion (field_ops.rs:315-392, ITK InvertDisplacementFieldImageFilter): one launch per
+// iteration. Iteration i runs only if state->stop[i & 1] is clear. Every block first reduces the
+// residual partials of psi_i that the previous launch left (the same fixed-order reduction in
+// every block, so all blocks agree on mean_i and max_i without a grid barrier). It then applies
+// the update psi_{i+1} = psi_i + eps * clip(-e_i, eps * max_i), with e_i recomputed at each voxel.
+// If another iteration may follow, it also writes the partials of e_{i+1}. The residual at x
+// gathers only the forward field, never psi at other voxels, so psi is updated in place. Block 0
+// publishes the statistics and stop[(i + 1) & 1], a word no block of this launch reads.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_inv_residual(const float* __restrict__ field, const float* __restrict__ psi,
-                                                      Dims d, L9 l, float3 inv_sp, const InvertState* state,
-                                                      float* partial_sum, float* partial_max) {
-  if (state->stop) return;
+constexpr int kInvThreads = 256;
+
+__device__ __forceinline__ void inv_residual_at(const float* __restrict__ field, const Dims& d, const float* l,
+                                                float3 inv_sp, int i, int j, int k, float wx, float wy, float wz,
+                                                float& ex, float& ey, float& ez, float& nrm) {
+  compose_at(field, d, l, i, j, k, wx, wy, wz, ex, ey, ez);
+  const float px = ex * inv_sp.x, py = ey * inv_sp.y, pz = ez * inv_sp.z;
+  nrm = sqrtf(px * px + py * py + pz * pz);
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// Sum and max over the block, broadcast to every thread.
+__device__ __forceinline__ void block_sum_max_all(float s, float m, float& total, float& mx) {
+  __shared__ float sh[2];
+  const float bs = block_sum(s);
+  const float bm = block_max(m);
+  if (threadIdx.x == 0) {
+    sh[0] = bs;
+    sh[1] = bm;
+  }
+  __syncthreads();
+  total = sh[0];
+  mx = sh[1];
+}
+
+__device__ __forceinline__ void inv_write_partials(float s, float m, float* partials, int blocks) {
+  const float bs = block_sum(s);
+  const float bm = block_max(m);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = bs;
+    partials[blocks + blockIdx.x] = bm;
+  }
+}
+
+__global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restrict__ field, const float* __restrict__ psi,
+                                                           Dims d, L9 l, float3 inv_sp, InvertState* state,
+                                                           float* partials, int max_iterations) {
   const long long n = d.n();
   float s = 0.0f, c = 0.0f, m = 0.0f;
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(g % d.nx);
     const long long r = g / d.nx;
     const int j = (int)(r % d.ny), k = (int)(r / d.ny);
-    float ex, ey, ez;
-    compose_at(field, d, l.v, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez);
-    ex *= inv_sp.x;
-    ey *= inv_sp.y;
-    ez *= inv_sp.z;
-    const float nrm = sqrtf(ex * ex + ey * ey + ez * ez);
+    float ex, ey, ez, nrm;
+    inv_residual_at(field, d, l.v, inv_sp, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez, nrm);
     m = nan_max(m, nrm);
-    const float y = nrm - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
+    kahan_add(s, c, nrm);
   }
-  const float bs = block_sum(s);
-  const float bm = block_max(m);
-  if (threadIdx.x == 0) {
-    partial_sum[blockIdx.x] = bs;
-    partial_max[blockIdx.x] = bm;
+  inv_write_partials(s, m, partials, gridDim.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    state->mean_error = 0.0f;
+    state->max_error = 0.0f;
+    state->executed = 0;
+    state->stop[0] = max_iterations > 0 ? 0 : 1;
+    state->stop[1] = 0;
   }
 }
 
-__global__ __launch_bounds__(256) void k_inv_reduce(const float* __restrict__ partial_sum, const float* __restrict__ partial_max,
-                                                    int blocks, long long n, InvertState* state, int step, int max_iterations,
-                                                    float max_tol, float mean_tol) {
-  if (state->stop) return;
+__global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi, Dims d,
+                                                          L9 l, float3 inv_sp, InvertState* state, float* partials,
+                                                          int step, int max_iterations, float max_tol, float mean_tol,
+                                                          int enforce) {
+  if (state->stop[step & 1]) {
+    // once stopped, stay stopped: the next launch reads the other word
+    if (blockIdx.x == 0 && threadIdx.x == 0) state->stop[(step + 1) & 1] = 1;
+    return;
+  }
+  const int blocks = gridDim.x;
+  const float* in = partials + (step & 1) * 2 * blocks;
+  float* out = partials + ((step + 1) & 1) * 2 * blocks;
   float s = 0.0f, c = 0.0f, m = 0.0f;
   for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-    const float y = partial_sum[b] - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-    m = nan_max(m, partial_max[b]);
+    kahan_add(s, c, in[b]);
+    m = nan_max(m, in[blocks + b]);
   }
-  const float total = block_sum(s);
-  const float mx = block_max(m);
-  if (threadIdx.x == 0) {
-    const float mean = total / (float)n;
+  float total, mx;
+  block_sum_max_all(s, m, total, mx);
+  const long long n = d.n();
+  const float mean = total / (float)n;
+  const bool next = (step + 1) < max_iterations && mx > max_tol && mean > mean_tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     state->mean_error = mean;
     state->max_error = mx;
     state->executed = step + 1;
-    const bool go_on = (step + 1) < max_iterations && mx > max_tol && mean > mean_tol;
-    state->stop = go_on ? 0 : 1;
-    state->active_step = step;
+    state->stop[(step + 1) & 1] = next ? 0 : 1;
   }
-}
-
-__global__ __launch_bounds__(256) void k_inv_update(const float* __restrict__ field, float* __restrict__ psi, Dims d, L9 l,
-                                                    float3 inv_sp, const InvertState* state, int step, int enforce) {
-  if (state->active_step != step) return;
   const float eps = step == 0 ? 0.75f : 0.5f;
-  const float limit = eps * state->max_error;
-  const long long n = d.n();
+  const float limit = eps * mx;
+  s = 0.0f;
+  c = 0.0f;
+  m = 0.0f;
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(g % d.nx);
     const long long r = g / d.nx;
     const int j = (int)(r % d.ny), k = (int)(r / d.ny);
     const float wx = psi[g], wy = psi[g + n], wz = psi[g + 2 * n];
-    float ex, ey, ez;
-    compose_at(field, d, l.v, i, j, k, wx, wy, wz, ex, ey, ez);
-    const float px = ex * inv_sp.x, py = ey * inv_sp.y, pz = ez * inv_sp.z;
-    const float nrm = sqrtf(px * px + py * py + pz * pz);
+    float ex, ey, ez, nrm;
+    inv_residual_at(field, d, l.v, inv_sp, i, j, k, wx, wy, wz, ex, ey, ez, nrm);
     float ux = -ex, uy = -ey, uz = -ez;
     if (nrm > limit) {
       const float f = limit / nrm;
@@ -593,49 +632,36 @@ __global__ __launch_bounds__(256) void k_inv_update(const float* __restrict__ fi
     psi[g] = nx;
     psi[g + n] = ny;
     psi[g + 2 * n] = nz;
+    if (next) {
+      inv_residual_at(field, d, l.v, inv_sp, i, j, k, nx, ny, nz, ex, ey, ez, nrm);
+      m = nan_max(m, nrm);
+      kahan_add(s, c, nrm);
+    }
   }
-}
-
-__global__ void k_inv_init(InvertState* state, int max_iterations) {
-  if (threadIdx.x == 0) {
-    state->mean_error = 0.0f;
-    state->max_error = 0.0f;
-    state->executed = 0;
-    state->stop = max_iterations > 0 ? 0 : 1;
-    state->active_step = -1;
-    state->pad = 0;
-  }
-}
-
-void launch_inv_init(InvertState* state, int max_iterations, hipStream_t s) {
-  hipLaunchKernelGGL(k_inv_init, dim3(1), dim3(64), 0, s, state, max_iterations);
+  if (next) inv_write_partials(s, m, out, blocks);
 }
 
 int inv_blocks(const Dims& d) {
-  long long b = (d.n() + 255) / 256;
-  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+  long long b = (d.n() + kInvThreads - 1) / kInvThreads;
+  return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
 }
 
-void launch_inv_residual(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
-                         const InvertState* state, float* partial_sum, float* partial_max, int blocks, hipStream_t s) {
+void launch_inv_begin(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
+                      InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  hipLaunchKernelGGL(k_inv_residual, dim3(blocks), dim3(256), 0, s, field, psi, d, lv, inv_spacing, state, partial_sum,
-                     partial_max);
+  hipLaunchKernelGGL(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, d, lv, inv_spacing, state,
+                     partials, max_iterations);
 }
 
-void launch_inv_reduce(const float* partial_sum, const float* partial_max, int blocks, long long n, InvertState* state,
-                       int step, int max_iterations, double max_tol, double mean_tol, hipStream_t s) {
-  hipLaunchKernelGGL(k_inv_reduce, dim3(1), dim3(256), 0, s, partial_sum, partial_max, blocks, n, state, step,
-                     max_iterations, (float)max_tol, (float)mean_tol);
-}
-
-void launch_inv_update(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing,
-                       const InvertState* state, int step, int enforce_boundary, hipStream_t s) {
+void launch_inv_iter(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing, InvertState* state,
+                     float* partials, int blocks, int step, int max_iterations, double max_tol, double mean_tol,
+                     int enforce_boundary, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  hipLaunchKernelGGL(k_inv_update, dim3(inv_blocks(d)), dim3(256), 0, s, field, psi, d, lv, inv_spacing, state, step,
-                     enforce_boundary);
+  hipLaunchKernelGGL(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, d, lv, inv_spacing, state,
+                     partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary);
 }
+
 
 }  // namespace fw::hip

@@ -95,21 +95,20 @@ void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const floa
 
 // --- inversion (ITK InvertDisplacementFieldImageFilter) ---
 struct InvertState {
-  // written by the reduction kernel, read by the update kernel and the host
+  // written by block 0 of each iteration's launch; read by the host after the last launch
   float mean_error;
   float max_error;
   int32_t executed;
-  int32_t stop;         // 1: no further step runs
-  int32_t active_step;  // the step whose update runs
+  int32_t stop[2];  // stop[i & 1] set: iteration i does not run
   int32_t pad;
 };
-void launch_inv_init(InvertState* state, int max_iterations, hipStream_t s);
-void launch_inv_residual(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
-                         const InvertState* state, float* partial_sum, float* partial_max, int blocks, hipStream_t s);
-void launch_inv_reduce(const float* partial_sum, const float* partial_max, int blocks, long long n, InvertState* state,
-                       int step, int max_iterations, double max_tol, double mean_tol, hipStream_t s);
-void launch_inv_update(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing,
-                       const InvertState* state, int step, int enforce_boundary, hipStream_t s);
+// Residual partials of the initial estimate (2 * blocks floats: sums, then maxima) and state reset.
+void launch_inv_begin(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
+                      InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s);
+// One iteration; partials holds two such buffers (4 * blocks floats), alternating by step parity.
+void launch_inv_iter(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing, InvertState* state,
+                     float* partials, int blocks, int step, int max_iterations, double max_tol, double mean_tol,
+                     int enforce_boundary, hipStream_t s);
 int inv_blocks(const Dims& d);
 
 // --- linear-stage Mattes MI ---

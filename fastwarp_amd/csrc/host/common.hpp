@@ -5,10 +5,12 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <functional>
+#include <initializer_list>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -81,12 +83,41 @@ inline Vec3 vadd(const Vec3& a, const Vec3& b) { return {a[0] + b[0], a[1] + b[1
 inline Vec3 vsub(const Vec3& a, const Vec3& b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
 inline double vnorm(const Vec3& a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
 
-// Number of worker threads of the host loops: FW_THREADS when set, else the hardware count.
+// Wall-clock spans of host work. While a sink is installed on the calling thread (a device
+// context does so while it profiles), every HostTimer reports its name and duration to it.
+struct HostSpanSink {
+  virtual void host_span(const char* name, double seconds) = 0;
+  virtual ~HostSpanSink() = default;
+};
+inline HostSpanSink*& host_span_sink() {
+  thread_local HostSpanSink* sink = nullptr;
+  return sink;
+}
+class HostTimer {
+ public:
+  explicit HostTimer(const char* name) : name_(name), sink_(host_span_sink()) {
+    if (sink_) t0_ = std::chrono::steady_clock::now();
+  }
+  ~HostTimer() {
+    if (sink_) sink_->host_span(name_, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count());
+  }
+  HostTimer(const HostTimer&) = delete;
+  HostTimer& operator=(const HostTimer&) = delete;
+
+ private:
+  const char* name_;
+  HostSpanSink* sink_;
+  std::chrono::steady_clock::time_point t0_;
+};
+
+// Number of worker threads of the host loops: FW_THREADS, else OMP_NUM_THREADS, else the hardware count.
 inline unsigned host_threads() {
   static unsigned n = [] {
-    if (const char* s = std::getenv("FW_THREADS")) {
-      long v = std::strtol(s, nullptr, 10);
-      if (v > 0) return static_cast<unsigned>(v);
+    for (const char* name : {"FW_THREADS", "OMP_NUM_THREADS"}) {
+      if (const char* s = std::getenv(name)) {
+        long v = std::strtol(s, nullptr, 10);
+        if (v > 0) return static_cast<unsigned>(v);
+      }
     }
     unsigned h = std::thread::hardware_concurrency();
     return h == 0 ? 1u : h;

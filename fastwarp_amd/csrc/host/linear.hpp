@@ -95,6 +95,7 @@ struct PreparedSamples {
 
 // linear.rs:251-279 `prepare_samples` with MiBinning::new (mi.rs:209-236).
 inline PreparedSamples prepare_samples(const Image& fixed, const Image& moving, const std::vector<Vec3>& samples, size_t bins) {
+  HostTimer host_timer("prepare_samples");
   if (bins <= 4) invalid("mattes", "at least five histogram bins are required");
   double flo = 0, fhi = 0;
   if (fixed_sample_range(fixed, samples, flo, fhi) == 0) invalid("mattes", "no fixed-image samples were observed");

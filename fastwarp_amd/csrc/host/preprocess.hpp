@@ -11,13 +11,31 @@ class IntensityHistogram {
  public:
   explicit IntensityHistogram(const std::vector<double>& values) {
     constexpr size_t kBins = 256;
+    // range and counts over fixed chunks, merged in chunk order
+    constexpr size_t kGrain = size_t(1) << 16;
+    const size_t chunks = (values.size() + kGrain - 1) / kGrain;
+    struct Range {
+      double lo = std::numeric_limits<double>::max(), hi = std::numeric_limits<double>::lowest();
+      size_t finite = 0;
+    };
+    std::vector<Range> ranges(chunks);
+    parallel_chunks(values.size(), kGrain, [&](size_t c, size_t b, size_t e) {
+      Range r;
+      for (size_t i = b; i < e; ++i) {
+        const double v = values[i];
+        if (!std::isfinite(v)) continue;
+        ++r.finite;
+        r.lo = std::min(r.lo, v);
+        r.hi = std::max(r.hi, v);
+      }
+      ranges[c] = r;
+    });
     double lo = std::numeric_limits<double>::max(), hi = std::numeric_limits<double>::lowest();
     size_t finite = 0;
-    for (double v : values) {
-      if (!std::isfinite(v)) continue;
-      ++finite;
-      lo = std::min(lo, v);
-      hi = std::max(hi, v);
+    for (const Range& r : ranges) {
+      finite += r.finite;
+      lo = std::min(lo, r.lo);
+      hi = std::max(hi, r.hi);
     }
     if (finite == 0) invalid("preprocess", "image has no finite values");
     const double margin = ((hi - lo) / double(kBins)) / 10.0;
@@ -31,9 +49,16 @@ class IntensityHistogram {
     }
     min_[kBins - 1] = lo + double((float(kBins) - 1.0f) * interval);
     max_[kBins - 1] = upper;
+    step_ = double(interval) > 0.0 ? double(interval) : 1.0;
+    std::vector<uint64_t> partial(chunks * kBins, 0);
+    parallel_chunks(values.size(), kGrain, [&](size_t c, size_t b, size_t e) {
+      uint64_t* counts = partial.data() + c * kBins;
+      for (size_t i = b; i < e; ++i)
+        if (std::isfinite(values[i])) ++counts[index_of(values[i])];
+    });
     std::vector<uint64_t> counts(kBins, 0);
-    for (double v : values)
-      if (std::isfinite(v)) ++counts[index_of(v)];
+    for (size_t c = 0; c < chunks; ++c)
+      for (size_t j = 0; j < kBins; ++j) counts[j] += partial[c * kBins + j];
     freq_.assign(counts.begin(), counts.end());
   }
 
@@ -41,9 +66,16 @@ class IntensityHistogram {
     if (v < min_[0]) return 0;
     const size_t n = min_.size();
     if (v >= max_[n - 1]) return n - 1;
-    const size_t pp = size_t(std::upper_bound(min_.begin(), min_.end(), v) - min_.begin());
-    return pp == 0 ? 0 : pp - 1;
+    // the largest j with min_[j] <= v (upper_bound - 1): start from the uniform-bin guess and walk
+    const double guess = (v - min_[0]) / step_;
+    size_t j = guess <= 0.0 ? 0 : std::min(n - 1, size_t(guess));
+    while (j + 1 < n && v >= min_[j + 1]) ++j;
+    while (j > 0 && v < min_[j]) --j;
+    return j;
   }
+
+  const std::vector<double>& bin_minima() const { return min_; }
+  const std::vector<double>& bin_maxima() const { return max_; }
 
   double quantile(double p) const {
     const size_t size = freq_.size();
@@ -85,11 +117,13 @@ class IntensityHistogram {
 
  private:
   std::vector<double> min_, max_, freq_;
+  double step_ = 1.0;
 };
 
 // preprocess.rs:159-232: the quantile window rounded to f32, then the linear map of [lo, hi] onto
 // [out_min, out_max] with clamping; non-finite voxels pass unchanged.
 inline Image preprocess_volume(const Image& vol, float lower, float upper, double out_min = 0.0, double out_max = 1.0) {
+  HostTimer host_timer("preprocess");
   IntensityHistogram h(vol.data);
   const double lo = double(float(h.quantile(double(lower))));
   const double hi = double(float(h.quantile(double(upper))));

@@ -177,6 +177,7 @@ inline Image smooth_recursive_gaussian(const Image& vol, const Vec3& sigma_mm) {
 
 // pyramid.rs:441-465 `sigma_mm_per_axis`, `smooth_for_level`: a copy for sigma <= 0.
 inline Image smooth_for_level(const Image& vol, double sigma, SigmaUnits units) {
+  HostTimer host_timer("deriche");
   if (sigma <= 0.0) return vol;
   const Vec3& s = vol.grid.spacing;
   const Vec3 mm = units == SigmaUnits::Voxels ? Vec3{sigma * s[0], sigma * s[1], sigma * s[2]} : Vec3{sigma, sigma, sigma};

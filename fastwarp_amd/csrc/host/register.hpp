@@ -41,10 +41,13 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
     invalid("register", "fixed and moving must have the same, non-zero number of channels");
   if (plan.run_syn && plan.syn.channel_weights.size() != fixed.size())
     invalid("stage plan", "the number of channels differs from the number of channel weights");
-  for (const auto* set : {&fixed, &moving})
-    for (const Image& v : *set)
-      for (double x : v.data)
-        if (!std::isfinite(x)) invalid("register", "an input image has non-finite (NaN or infinite) voxels");
+  {
+    HostTimer t_check("validate");
+    for (const auto* set : {&fixed, &moving})
+      for (const Image& v : *set)
+        for (double x : v.data)
+          if (!std::isfinite(x)) invalid("register", "an input image has non-finite (NaN or infinite) voxels");
+  }
   RegisterOutput out;
   auto t = clock::now();
   std::vector<SynChannel> channels;

@@ -63,6 +63,7 @@ struct SeedSequence {
 // jittered by N(0, 1) * spacing / 3 per axis (x first); `keep` filters after the jitter.
 template <typename Keep>
 std::vector<Vec3> metric_sample_points(const Grid& g, float percentage, SeedSequence& seeds, Keep keep) {
+  HostTimer host_timer("sample_points");
   Mt19937 rng(seeds.next_seed());
   const size_t stride = size_t(std::max(1.0, std::ceil(1.0 / double(percentage))));
   const size_t n = g.num_voxels();

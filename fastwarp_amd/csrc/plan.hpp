@@ -132,13 +132,17 @@ void run_register(B& backend, E& engine, const fw_plan* plan, const fw_grid* fix
   const RegisterPlan p = plan_from(plan);
   const Grid fg = grid_of(fixed_grid), mg = grid_of(moving_grid);
   std::vector<Image> f, m;
-  for (int32_t c = 0; c < plan->channels; ++c) {
-    if (!fixed || !moving || !fixed[c] || !moving[c]) invalid("register", "null image data");
-    f.emplace_back(fg, std::vector<double>(fixed[c], fixed[c] + fg.num_voxels()));
-    m.emplace_back(mg, std::vector<double>(moving[c], moving[c] + mg.num_voxels()));
+  {
+    HostTimer t_in("copy_inputs");
+    for (int32_t c = 0; c < plan->channels; ++c) {
+      if (!fixed || !moving || !fixed[c] || !moving[c]) invalid("register", "null image data");
+      f.emplace_back(fg, std::vector<double>(fixed[c], fixed[c] + fg.num_voxels()));
+      m.emplace_back(mg, std::vector<double>(moving[c], moving[c] + mg.num_voxels()));
+    }
   }
   const RegisterOutput o = register_full(backend, engine, p, f, m, cancel);
   result_to(o, result);
+  HostTimer t_out("write_fields");
   auto write = [](const VectorField& v, double* dst) {
     if (!dst) return;
     for (size_t i = 0; i < v.data.size(); ++i)

@@ -233,6 +233,28 @@ static void preprocess() {
   CHECK(p.data[0] == 0.0 && p.data[99] <= 1.0 && p.data[99] > 0.99, "window to [0, 1]");
   const Image w = preprocess_volume(v, 0.1f, 0.9f);
   CHECK(w.data[0] == 0.0 && w.data[99] == 1.0, "winsorised ends clamp");
+  // the bin lookup equals ITK's search (the largest bin whose minimum is <= v, clamped at both
+  // ends) on random values, on every bin edge and on the neighbours of each edge
+  std::vector<double> r(20000);
+  Mt19937 mt(7);
+  for (auto& x : r) x = -2.0 + 7.0 * (mt.integer() / 4294967296.0);
+  const IntensityHistogram hr(r);
+  const auto& mn = hr.bin_minima();
+  const auto& mx = hr.bin_maxima();
+  std::vector<double> probes = r;
+  for (double e : mn) probes.insert(probes.end(), {e, std::nextafter(e, -1e9), std::nextafter(e, 1e9)});
+  probes.insert(probes.end(), {mx.back(), std::nextafter(mx.back(), -1e9), -1e9, 1e9});
+  for (double v : probes) {
+    size_t want;
+    if (v < mn[0]) want = 0;
+    else if (v >= mx.back()) want = mn.size() - 1;
+    else {
+      want = 0;
+      for (size_t j = 0; j < mn.size(); ++j)
+        if (mn[j] <= v) want = j;
+    }
+    CHECK(hr.index_of(v) == want, "histogram bin lookup");
+  }
   ok("preprocess");
 }
 
