@@ -81,6 +81,64 @@ def cpu_baseline(pair, gpu_result) -> dict:
     }
 
 
+def kernels(repeats: int) -> int:
+    """Each hot-path op on the 1 mm pair at full resolution (182x218x182), `repeats` launches
+    after one warm-up: one JSON line per kernel family with its mean launch time (HIP events on
+    the context's stream) and algorithmic bytes per launch against the HBM peak."""
+    from fastwarp_amd._abi import InvertParams, LinearTransform, MiBinning
+    from fastwarp_amd.backend import HipBackend, MiLevel
+    from fastwarp_amd.phantom import CONFIGS, make_pair
+
+    pair = make_pair(**CONFIGS["1mm"])
+    g = pair.grid
+    b = HipBackend(0)
+    f = b.upload_image(g, pair.fixed[0])
+    m = b.upload_image(g, pair.moving[0])
+    u = b.upload_field(g, pair.displacement)
+    n = pair.fixed[0].size
+    # samples of the finest linear level: every 4th voxel, jittered by a third of a voxel
+    rng = np.random.default_rng(0)
+    idx = np.arange(0, n, 4)
+    size, origin, spacing = np.array(list(g.size)), np.array(list(g.origin)), np.array(list(g.spacing))
+    ijk = np.stack([idx % size[0], (idx // size[0]) % size[1], idx // (size[0] * size[1])], 1).astype(float)
+    pts = origin + (ijk + rng.normal(0, 1 / 3, ijk.shape)) * spacing
+    fv, mv = pair.fixed[0], pair.moving[0]
+    binning = MiBinning.from_ranges(32, (fv.min(), fv.max()), (mv.min(), mv.max()))
+    fb = np.clip(((fv[idx] - binning.fixed_min) / binning.fixed_bin_size + 2).astype(np.uint32), 2, 29)
+    level = MiLevel(b, m, pts, fb, binning)
+    centre = tuple(origin + (size - 1) / 2 * spacing)
+    t_rigid = LinearTransform.make(0, [0.01, -0.02, 0.015, 0.5, -0.3, 0.2], centre)
+    no_stop = InvertParams(20, 0.0, 0.0, 1)
+
+    ops = {
+        "cc_pair r4": lambda: b.cc_value_derivative_pair(f, m, 4),
+        "smooth_field var3": lambda: b.smooth_field(b.copy_field(u), 3.0),
+        "compose": lambda: b.compose(u, u),
+        "warp": lambda: b.warp_through_field(m, u),
+        "invert (20 iterations)": lambda: b.invert(u, b.zero_field(g), no_stop),
+        "max_scaled_norm_pair": lambda: b.max_scaled_norm_pair(u, u),
+        "scale": lambda: b.scale(u, 1.0),
+        "mi value (1.8M samples)": lambda: level.value(t_rigid),
+        "mi value+derivative": lambda: level.value_and_derivative(t_rigid),
+    }
+    for name, op in ops.items():
+        op()
+        b.set_profiling(True)
+        for _ in range(repeats):
+            op()
+        prof = b.take_profile()
+        b.set_profiling(False)
+        for k, (launches, ms, nbytes) in sorted(prof.items()):
+            if k.startswith("host:") or launches == 0:
+                continue
+            mean_us = 1e3 * ms / launches
+            gbs = nbytes / launches / (mean_us * 1e-6) / 1e9 if nbytes else 0.0
+            print(json.dumps({"op": name, "kernel": k, "grid": size.tolist(), "launches": launches,
+                              "mean_us": round(mean_us, 2), "bytes_per_launch": round(nbytes / launches),
+                              "achieved_gbs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,10 +146,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-engine baseline")
     ap.add_argument("--config", default="1mm", choices=["1mm", "2mm"])
+    ap.add_argument("--kernels", action="store_true", help="time each hot-path kernel at 1 mm instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box of this build has a single MI355X (asked for {args.gpus})", file=sys.stderr)
         return 2
+    if args.kernels:
+        return kernels(max(args.steps, 1) * 10)
 
     from fastwarp_amd.backend import HipBackend
     from fastwarp_amd.phantom import CONFIGS, make_pair
@@ -146,6 +207,9 @@ def main() -> int:
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "phases_s": {"linear": round(last.result.seconds_linear, 3), "syn": round(last.result.seconds_syn, 3),
                      "preprocess": round(last.result.seconds_preprocess, 3)},
+        "syn_levels": [{"iterations": last.result.syn_iterations[l],
+                        "mean_inversion_steps": round(last.result.syn_inversion_steps[l] / max(4 * last.result.syn_iterations[l], 1), 2)}
+                       for l in range(last.result.syn_levels)],
         "host_s_per_step": dict(sorted(host.items(), key=lambda kv: -kv[1])),
         "device_peak_bytes": int(peak),
         "roofline": roofline(profile),

@@ -223,4 +223,5 @@ class Result(C.Structure):
         ("linear_level_seconds", (C.c_double * 8) * 2),
         ("linear_iteration_seconds", (C.c_double * 8) * 2),
         ("linear_evaluations", (C.c_int64 * 8) * 2),
+        ("syn_inversion_steps", C.c_int64 * 8),
     ]

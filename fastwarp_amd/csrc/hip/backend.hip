@@ -193,11 +193,24 @@ HipBackend::HipBackend(int device) : ctx_(std::make_shared<DeviceContext>(device
 HImage HipBackend::upload_image(const Image& v) {
   HostTimer host_timer("upload_image");
   const size_t n = v.grid.num_voxels();
-  std::vector<float> f(n);
-  parallel_for(n, [&](size_t b, size_t e) {
-    for (size_t i = b; i < e; ++i) f[i] = float(v.data[i]);
+  auto* stage = static_cast<float*>(ctx_->pinned(n * sizeof(float)));
+  constexpr size_t kGrain = size_t(1) << 16;
+  std::vector<float> bound((n + kGrain - 1) / kGrain, 0.0f);
+  parallel_chunks(n, kGrain, [&](size_t c, size_t b, size_t e) {
+    float m = 0.0f;
+    for (size_t i = b; i < e; ++i) {
+      const float x = float(v.data[i]);
+      stage[i] = x;
+      if (std::isfinite(x)) m = std::max(m, std::fabs(x));
+    }
+    bound[c] = m;
   });
-  return upload_image_f32(v.grid, f.data());
+  float m = 0.0f;
+  for (float x : bound) m = std::max(m, x);
+  HImage im{v.grid, DevBuf(ctx_.get(), n), m};
+  check(hipMemcpyAsync(im.buf.get(), stage, n * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()), "upload image");
+  ctx_->sync();
+  return im;
 }
 
 HImage HipBackend::upload_image_f32(const Grid& g, const float* data) {
@@ -238,6 +251,21 @@ HField HipBackend::upload_field(const VectorField& v) {
         "upload field");
   ctx_->sync();
   return f;
+}
+
+void HipBackend::download_field_to(const HField& f, double* aos) {
+  HostTimer host_timer("download_field");
+  const size_t n = f.grid.num_voxels();
+  auto* stage = static_cast<float*>(ctx_->pinned(3 * n * sizeof(float)));
+  check(hipMemcpyAsync(stage, f.buf.get(), 3 * n * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "download field");
+  ctx_->sync();
+  parallel_for(n, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      aos[3 * i] = double(stage[i]);
+      aos[3 * i + 1] = double(stage[n + i]);
+      aos[3 * i + 2] = double(stage[2 * n + i]);
+    }
+  });
 }
 
 VectorField HipBackend::download_field(const HField& f) {

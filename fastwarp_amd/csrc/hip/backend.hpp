@@ -179,6 +179,8 @@ class HipBackend {
   HField zero_field(const Grid& g);
   HField upload_field(const VectorField& f);
   VectorField download_field(const HField& f);
+  // Writes the field as 3n interleaved f64 values (x, y, z per voxel) into `aos`.
+  void download_field_to(const HField& f, double* aos);
   Grid field_grid(const HField& f) const { return f.grid; }
   HField copy_field(const HField& f);
 

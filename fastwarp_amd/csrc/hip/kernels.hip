@@ -459,6 +459,91 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
 }
 
 // ------------------------------------------------------------------------------------------
+// Recursive Gaussian (pyramid.rs:184-305, ITK RecursiveGaussianImageFilter, zero order): one
+// thread per line along `axis`. The causal pass writes its result to `out`; the anti-causal pass
+// walks back, adds its own result and overwrites `out`. Lines of neighbouring threads are
+// adjacent in x for the y and z axes (coalesced); x lines stay in the same cache lines for 16
+// consecutive voxels.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                      int axis, DericheCoefficients c) {
+  const long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long plane = (long long)d.nx * d.ny;
+  long long lines, base, stride;
+  int len;
+  if (axis == 0) {
+    lines = (long long)d.ny * d.nz;
+    base = line * d.nx;
+    stride = 1;
+    len = d.nx;
+  } else if (axis == 1) {
+    lines = (long long)d.nx * d.nz;
+    base = (line % d.nx) + (line / d.nx) * plane;
+    stride = d.nx;
+    len = d.ny;
+  } else {
+    lines = plane;
+    base = line;
+    stride = plane;
+    len = d.nz;
+  }
+  if (line >= lines) return;
+  const float* p = in + base;
+  float* q = out + base;
+  // causal: y[i] = sum n_k x[i-k] - sum d_k y[i-1-k], with x and y extended by x[0] at the start
+  {
+    const float v1 = p[0];
+    float x0 = v1, x1 = v1, x2 = v1, x3 = v1;  // x[i], x[i-1], x[i-2], x[i-3]
+    float y1 = 0.0f, y2 = 0.0f, y3 = 0.0f, y4 = 0.0f;
+    for (int i = 0; i < len; ++i) {
+      x3 = x2;
+      x2 = x1;
+      x1 = x0;
+      x0 = p[(long long)i * stride];
+      float y = x0 * c.n[0] + x1 * c.n[1] + x2 * c.n[2] + x3 * c.n[3];
+      // outputs before the start of the line are replaced by the boundary terms v1 * bn_k
+      const float sub = (i >= 1 ? y1 * c.d[0] : v1 * c.bn[0]) + (i >= 2 ? y2 * c.d[1] : v1 * c.bn[1]) +
+                        (i >= 3 ? y3 * c.d[2] : v1 * c.bn[2]) + (i >= 4 ? y4 * c.d[3] : v1 * c.bn[3]);
+      y -= sub;
+      y4 = y3;
+      y3 = y2;
+      y2 = y1;
+      y1 = y;
+      q[(long long)i * stride] = y;
+    }
+  }
+  // anti-causal: a[i-1] = sum m_k x[i+k] - sum d_k a[i+k], with x extended by x[len-1] at the end
+  {
+    const float v2 = p[(long long)(len - 1) * stride];
+    float x0 = v2, x1 = v2, x2 = v2, x3 = v2;  // x[i], x[i+1], x[i+2], x[i+3] for a[i-1]
+    float a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;  // a[i], a[i+1], a[i+2], a[i+3]
+    for (int t = 0; t < len; ++t) {
+      const int i = len - t;  // this step produces a[i-1]
+      // x[i..i+3]: x[i] is the next input inward, past the end it is v2
+      x3 = x2;
+      x2 = x1;
+      x1 = x0;
+      x0 = i <= len - 1 ? p[(long long)i * stride] : v2;
+      float y = x0 * c.m[0] + x1 * c.m[1] + x2 * c.m[2] + x3 * c.m[3];
+      const float sub = (t >= 1 ? a1 * c.d[0] : v2 * c.bm[0]) + (t >= 2 ? a2 * c.d[1] : v2 * c.bm[1]) +
+                        (t >= 3 ? a3 * c.d[2] : v2 * c.bm[2]) + (t >= 4 ? a4 * c.d[3] : v2 * c.bm[3]);
+      y -= sub;
+      a4 = a3;
+      a3 = a2;
+      a2 = a1;
+      a1 = y;
+      const long long o = (long long)(i - 1) * stride;
+      q[o] = q[o] + y;
+    }
+  }
+}
+
+void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s) {
+  const long long lines = axis == 0 ? (long long)d.ny * d.nz : axis == 1 ? (long long)d.nx * d.nz : (long long)d.nx * d.ny;
+  hipLaunchKernelGGL(k_deriche_axis, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, in, out, d, axis, c);
+}
+
+// ------------------------------------------------------------------------------------------
 // Separable Gaussian (field_ops.rs:508-574): taps clamped at the edges; the final pass zeroes
 // every face voxel and blends w1 * smooth + w2 * original.
 // ------------------------------------------------------------------------------------------

@@ -111,36 +111,12 @@ void launch_inv_iter(const float* field, float* psi, Dims d, const float l[9], f
                      int enforce_boundary, hipStream_t s);
 int inv_blocks(const Dims& d);
 
-// --- linear-stage Mattes MI ---
-struct MiLinearArgs {
-  const float4* samples;  // (x, y, z) virtual continuous coordinates relative to the anchor, w = fixed bin bits
-  long long count;
-  const float* moving;
-  Dims md;
-  float a[12];            // moving continuous index = A[0:9] * s + A[9:12]
-  float moving_min, moving_max, inv_bin_size, normalized_min;
-  int bins;
-  int chunk;              // samples per block
-  unsigned long long* hist_partial;  // per block: bins*bins joint + bins fixed marginal, 2^-32 fixed point
-  unsigned int* valid_partial;       // per block: contributing samples
+// --- recursive Gaussian (Deriche, 4th order) along one axis, constant extension at both ends ---
+struct DericheCoefficients {
+  float n[4], d[4], m[4], bn[4], bm[4];
 };
-void launch_mi_hist(const MiLinearArgs& a, int blocks, hipStream_t s);
-void launch_mi_merge(const unsigned long long* partial, const unsigned int* valid_partial, int blocks, int cells,
-                     unsigned long long* out, unsigned long long* valid_out, hipStream_t s);
+// `in` and `out` must differ; every line needs at least 4 voxels.
+void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s);
 
-struct MiDerivArgs {
-  MiLinearArgs base;
-  const float* ratio;    // bins*bins f32 weights
-  float probe_o[3][3];   // central-difference probe offsets (index units)
-  float probe_inv_spacing[3];
-  float jac_outer[9];    // outer matrix (columns left-multiplied)
-  float center[3];       // optimised transform centre relative to the anchor (physical)
-  float euler[6];        // cx sx cy sy cz sz for Euler3D
-  float to_phys[12];     // sample continuous coords -> physical point relative to the anchor
-  int kind;              // 0 Euler3D, 1 affine
-  int params;
-  double* partial;       // per block: params doubles
-};
-void launch_mi_deriv(const MiDerivArgs& a, int blocks, hipStream_t s);
 
 }  // namespace fw::hip

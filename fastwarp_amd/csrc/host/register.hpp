@@ -33,7 +33,7 @@ struct RegisterOutput {
 
 template <typename B, typename E>
 RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, const std::vector<Image>& fixed,
-                             const std::vector<Image>& moving, const volatile int32_t* cancel) {
+                             const std::vector<Image>& moving, const volatile int32_t* cancel, FieldOutputs outputs = {}) {
   using clock = std::chrono::steady_clock;
   auto secs = [](clock::time_point a) { return std::chrono::duration<double>(clock::now() - a).count(); };
   const auto t_all = clock::now();
@@ -69,7 +69,7 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
   if (plan.run_syn) {
     backend.release_idle_buffers();
     t = clock::now();
-    out.syn = syn_fields(backend, plan.syn, channels, out.total, &cache, cancel);
+    out.syn = syn_fields(backend, plan.syn, channels, out.total, &cache, cancel, outputs);
     out.seconds_syn = secs(t);
   }
   out.seconds_total = secs(t_all);

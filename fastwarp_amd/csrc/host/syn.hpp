@@ -71,9 +71,15 @@ struct SynLevelTrace {
   double seconds = 0.0;
 };
 
+// Per-level traces of one SyN run. The fields themselves go straight to the caller's arrays.
 struct SynFields {
-  VectorField warp, inverse_warp;
   std::vector<SynLevelTrace> levels;
+};
+
+// Where the SyN fields go: 3n interleaved f64 values each on the fixed grid; null skips one.
+struct FieldOutputs {
+  double* forward = nullptr;
+  double* inverse = nullptr;
 };
 
 namespace detail {
@@ -103,7 +109,7 @@ struct Side {
 // syn.rs:776-986 `syn_fields` from a linear start `initial` (maps fixed points to moving points).
 template <typename B>
 SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel>& channels, const AffineTransform& initial,
-                     SmoothCache* cache, const volatile int32_t* cancel) {
+                     SmoothCache* cache, const volatile int32_t* cancel, FieldOutputs outputs = {}) {
   using FieldH = typename B::FieldH;
   using ImageH = typename B::ImageH;
   recipe.validate();
@@ -229,10 +235,8 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel
     out.levels.push_back(std::move(trace));
   }
   // syn.rs:969-980: SyN = Minv o F forward, Finv o M inverse
-  FieldH forward = b.compose(*m2m_inv, *f2m);
-  FieldH inverse = b.compose(*f2m_inv, *m2m);
-  out.warp = b.download_field(forward);
-  out.inverse_warp = b.download_field(inverse);
+  if (outputs.forward) b.download_field_to(b.compose(*m2m_inv, *f2m), outputs.forward);
+  if (outputs.inverse) b.download_field_to(b.compose(*f2m_inv, *m2m), outputs.inverse);
   return out;
 }
 

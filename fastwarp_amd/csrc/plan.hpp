@@ -95,8 +95,13 @@ inline void result_to(const RegisterOutput& o, fw_result* r) {
     r->syn_last_energy[l] = o.syn.levels[l].iterations.empty() ? 0.0 : o.syn.levels[l].iterations.back().energy;
     r->syn_level_seconds[l] = o.syn.levels[l].seconds;
     double it = 0.0;
-    for (const auto& i : o.syn.levels[l].iterations) it += i.seconds;
+    int64_t steps = 0;
+    for (const auto& i : o.syn.levels[l].iterations) {
+      it += i.seconds;
+      for (const InvertStats& st : i.inversions) steps += st.iterations;
+    }
     r->syn_iteration_seconds[l] = it;
+    r->syn_inversion_steps[l] = steps;
   }
   const LinearStageResult* st[2] = {&o.linear.rigid, &o.linear.affine};
   for (int s = 0; s < 2; ++s) {
@@ -140,18 +145,8 @@ void run_register(B& backend, E& engine, const fw_plan* plan, const fw_grid* fix
       m.emplace_back(mg, std::vector<double>(moving[c], moving[c] + mg.num_voxels()));
     }
   }
-  const RegisterOutput o = register_full(backend, engine, p, f, m, cancel);
+  const RegisterOutput o = register_full(backend, engine, p, f, m, cancel, FieldOutputs{forward, inverse});
   result_to(o, result);
-  HostTimer t_out("write_fields");
-  auto write = [](const VectorField& v, double* dst) {
-    if (!dst) return;
-    for (size_t i = 0; i < v.data.size(); ++i)
-      for (int c = 0; c < 3; ++c) dst[3 * i + c] = v.data[i][c];
-  };
-  if (p.run_syn) {
-    write(o.syn.warp, forward);
-    write(o.syn.inverse_warp, inverse);
-  }
 }
 
 }  // namespace fw

@@ -225,6 +225,7 @@ typedef struct {
   double linear_level_seconds[2][8];
   double linear_iteration_seconds[2][8];
   int64_t linear_evaluations[2][8]; /* metric evaluations, line searches included */
+  int64_t syn_inversion_steps[8];    /* executed fixed-point steps of the level's four inversions per iteration, summed */
 } fw_result;
 
 /* ---- profiling (the reference's WgpuBackend::set_profiling / take_profile,

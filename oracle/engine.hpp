@@ -19,6 +19,10 @@ class CpuBackend {
   Image upload_image(const Image& v) { return v; }
   VectorField zero_field(const Grid& g) { return VectorField::zeros(g); }
   VectorField download_field(const VectorField& f) { return f; }
+  void download_field_to(const VectorField& f, double* aos) {
+    for (size_t i = 0; i < f.data.size(); ++i)
+      for (int c = 0; c < 3; ++c) aos[3 * i + c] = f.data[i][c];
+  }
   Grid field_grid(const VectorField& f) const { return f.grid; }
   Image warp_through_field(const Image& src, const VectorField& f, const VectorField* pre, const AffineTransform* a) {
     return oracle::warp_through_field(src, f, pre, a);
