@@ -113,6 +113,7 @@ def kernels(repeats: int) -> int:
     ops = {
         "cc_pair r4": lambda: b.cc_value_derivative_pair(f, m, 4),
         "smooth_field var3": lambda: b.smooth_field(b.copy_field(u), 3.0),
+        "smooth_image sigma 3 vox": lambda: b.smooth_image(f, (3.0, 3.0, 3.0)),
         "compose": lambda: b.compose(u, u),
         "warp": lambda: b.warp_through_field(m, u),
         "invert (20 iterations)": lambda: b.invert(u, b.zero_field(g), no_stop),

@@ -60,6 +60,8 @@ _SIGNATURES = [
     ("fw_cc_pair", C.c_int, [_P, _P, _P, _P, C.c_uint32, _D, _D, _PP, _PP]),
     ("fw_mi_dense_pair", C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, _D, _D, _PP, _PP]),
     ("fw_smooth_field", C.c_int, [_P, _P, C.c_double, _PP]),
+    ("fw_smooth_image", C.c_int, [_P, _P, _D, _PP]),
+    ("fw_sample_values", C.c_int, [_P, _P, _D, C.c_int64, _D, _D]),
     ("fw_max_scaled_norm_pair", C.c_int, [_P, _P, _P, _D]),
     ("fw_l2_norm", C.c_int, [_P, _P, _D]),
     ("fw_scale", C.c_int, [_P, _P, C.c_double]),
@@ -331,6 +333,19 @@ class HipBackend:
         return self._mi(fixed, moving, None, None, bins, True)
 
     # --- field algebra ---
+    def smooth_image(self, image: DeviceImage, sigma_mm) -> DeviceImage:
+        """Level smoothing on the device (pyramid.rs:429-437), per-axis sigma in mm."""
+        s = np.ascontiguousarray(sigma_mm, dtype=np.float64).reshape(3)
+        return self._img(lib().fw_smooth_image, image.handle, dptr(s))
+
+    def sample_values(self, image: DeviceImage, points: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """(nearest-voxel, linear) values at physical points, NaN outside."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        m = pts.shape[0]
+        near, lin = np.empty(m), np.empty(m)
+        self._check(lib().fw_sample_values(self.ctx, image.handle, dptr(pts), m, dptr(near), dptr(lin)))
+        return near, lin
+
     def smooth_field(self, field: DeviceField, variance: float) -> DeviceField:
         return self._fld(lib().fw_smooth_field, field.take(), float(variance))
 

@@ -197,6 +197,29 @@ int fw_field_copy(fw_ctx* ctx, const fw_field* field, fw_field** out) {
 
 void fw_field_free(fw_ctx*, fw_field* field) { delete field; }
 
+int fw_smooth_image(fw_ctx* ctx, const fw_image* src, const double sigma_mm[3], fw_image** out) {
+  return guard(ctx, [&] {
+    need(src, "source");
+    need(sigma_mm, "sigma");
+    need(out, "output");
+    *out = new fw_image{ctx->backend.smooth_image_mm(src->h, {sigma_mm[0], sigma_mm[1], sigma_mm[2]})};
+  });
+}
+
+int fw_sample_values(fw_ctx* ctx, const fw_image* img, const double* points, int64_t m, double* nearest, double* linear) {
+  return guard(ctx, [&] {
+    need(img, "image");
+    if (m < 0) fw::invalid("sample values", "negative count");
+    if (m > 0) need(points, "points");
+    std::vector<fw::Vec3> pts(static_cast<size_t>(m));
+    for (int64_t i = 0; i < m; ++i) pts[size_t(i)] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    std::vector<double> a, b;
+    ctx->backend.sample_values(img->h, pts, a, b);
+    if (nearest) std::copy(a.begin(), a.end(), nearest);
+    if (linear) std::copy(b.begin(), b.end(), linear);
+  });
+}
+
 int fw_warp(fw_ctx* ctx, const fw_image* src, const fw_field* field, const fw_field* pre, const fw_affine* affine,
             fw_image** out) {
   return guard(ctx, [&] {

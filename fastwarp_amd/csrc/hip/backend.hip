@@ -4,6 +4,7 @@
 #include "backend.hpp"
 
 #include <cmath>
+#include <cstring>
 
 #include "../host/gauss_taps.hpp"
 #include "geometry.hpp"
@@ -27,6 +28,7 @@ DeviceContext::~DeviceContext() {
   for (auto& s : scratch_)
     if (s.p) hipFree(s.p);
   if (pinned_) hipHostFree(pinned_);
+  if (mailbox_) hipHostFree(mailbox_);
   for (auto& kv : taps_) hipFree(kv.second.first);
   for (auto& s : spans_) {
     (void)hipEventDestroy(s.a);
@@ -101,9 +103,34 @@ void* DeviceContext::pinned(size_t bytes) {
   return pinned_;
 }
 
+volatile uint32_t* DeviceContext::mailbox_host() {
+  if (!mailbox_) {
+    void* p = nullptr;
+    check(hipHostMalloc(&p, 256, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc mailbox");
+    mailbox_ = static_cast<uint32_t*>(p);
+    std::memset(mailbox_, 0, 256);
+    void* d = nullptr;
+    check(hipHostGetDevicePointer(&d, p, 0), "hipHostGetDevicePointer");
+    mailbox_dev_ = static_cast<uint32_t*>(d);
+  }
+  return mailbox_;
+}
+
+uint32_t* DeviceContext::mailbox_device() {
+  (void)mailbox_host();
+  return mailbox_dev_;
+}
+
 void DeviceContext::sync() {
   HostTimer t("sync_wait");
   check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+const std::vector<double>& DeviceContext::host_taps(double variance, size_t length) {
+  auto key = std::make_pair(variance, length);
+  auto it = host_taps_.find(key);
+  if (it == host_taps_.end()) it = host_taps_.emplace(key, gaussian_operator_taps(variance, 0.001, length)).first;
+  return it->second;
 }
 
 const float* DeviceContext::taps(double variance, size_t length, int& radius) {
@@ -433,14 +460,149 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
 
 // ------------------------------------------------------------------------- field ops (a8-a13) ---
 
+HImage HipBackend::smooth_image(const HImage& src, double sigma, SigmaUnits units) {
+  const Vec3& sp = src.grid.spacing;
+  if (sigma <= 0.0) return smooth_image_mm(src, {0.0, 0.0, 0.0});
+  const Vec3 mm = units == SigmaUnits::Voxels ? Vec3{sigma * sp[0], sigma * sp[1], sigma * sp[2]} : Vec3{sigma, sigma, sigma};
+  return smooth_image_mm(src, mm);
+}
+
+HImage HipBackend::smooth_image_mm(const HImage& src, const Vec3& sigma_mm) {
+  const Grid& g = src.grid;
+  const size_t n = g.num_voxels();
+  HImage out{g, DevBuf(ctx_.get(), n), src.absmax};
+  hipStream_t s = ctx_->stream();
+  if (!(sigma_mm[0] > 0.0 || sigma_mm[1] > 0.0 || sigma_mm[2] > 0.0)) {
+    check(hipMemcpyAsync(out.buf.get(), src.buf.get(), n * sizeof(float), hipMemcpyDeviceToDevice, s), "copy image");
+    return out;
+  }
+  for (int axis = 0; axis < 3; ++axis) {
+    if (g.size[axis] < 4) invalid("smoothing", "every axis needs at least 4 voxels");
+    if (!(sigma_mm[axis] > 0.0)) invalid("smoothing", "sigma must be greater than zero");
+  }
+  const Dims d = dims_of(g);
+  DevBuf tmp(ctx_.get(), n);
+  // z into out, x into tmp, y into out (pyramid.rs:429-437 order)
+  const float* in = src.buf.get();
+  float* bufs[3] = {out.buf.get(), tmp.get(), out.buf.get()};
+  const int axes[3] = {2, 0, 1};
+  for (int q = 0; q < 3; ++q) {
+    const Deriche k = Deriche::zero_order(sigma_mm[axes[q]], g.spacing[axes[q]]);
+    DericheCoefficients c{};
+    for (int t = 0; t < 4; ++t) {
+      c.n[t] = float(k.n[t]);
+      c.d[t] = float(k.d[t]);
+      c.m[t] = float(k.m[t]);
+      c.bn[t] = float(k.bn[t]);
+      c.bm[t] = float(k.bm[t]);
+    }
+    Timed t(*ctx_, "k_deriche_axis", 20.0 * double(n));
+    launch_deriche_axis(in, bufs[q], d, axes[q], c, s);
+    in = bufs[q];
+  }
+  // the bound of the smoothed image: its own maximum magnitude
+  const int nb = norm_blocks((long long)n);
+  float* pm = static_cast<float*>(ctx_->scratch(2, nb * sizeof(float)));
+  {
+    Timed t(*ctx_, "k_abs_max", 4.0 * double(n));
+    launch_abs_max(out.buf.get(), (long long)n, pm, nb, s);
+  }
+  float* host = static_cast<float*>(ctx_->pinned(nb * sizeof(float)));
+  check(hipMemcpyAsync(host, pm, nb * sizeof(float), hipMemcpyDeviceToHost, s), "abs max partials");
+  ctx_->sync();
+  float m = 0.0f;
+  for (int b = 0; b < nb; ++b)
+    if (std::isfinite(host[b])) m = std::max(m, host[b]);
+  out.absmax = m;
+  return out;
+}
+
+void HipBackend::sample_values(const HImage& img, const std::vector<Vec3>& points, std::vector<double>& nearest,
+                               std::vector<double>& linear) {
+  const size_t m = points.size();
+  nearest.assign(m, std::nan(""));
+  linear.assign(m, std::nan(""));
+  if (m == 0) return;
+  const Grid& g = img.grid;
+  hipStream_t s = ctx_->stream();
+  auto* stage = static_cast<float*>(ctx_->pinned(4 * m * sizeof(float)));
+  parallel_for(m, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      const Vec3 c = g.point_to_index(points[i]);
+      stage[4 * i] = float(c[0]);
+      stage[4 * i + 1] = float(c[1]);
+      stage[4 * i + 2] = float(c[2]);
+      stage[4 * i + 3] = 0.0f;
+    }
+  });
+  DevBuf idx(ctx_.get(), 4 * m), values(ctx_.get(), 2 * m);
+  check(hipMemcpyAsync(idx.get(), stage, 4 * m * sizeof(float), hipMemcpyHostToDevice, s), "sample indices");
+  {
+    Timed t(*ctx_, "k_sample_values", 16.0 * double(m) + 8.0 * double(m));
+    launch_sample_values(img.buf.get(), dims_of(g), reinterpret_cast<const float4*>(idx.get()), (long long)m,
+                         values.get(), values.get() + m, s);
+  }
+  // the stage is reused for the values: stream order puts this copy after the index upload
+  check(hipMemcpyAsync(stage, values.get(), 2 * m * sizeof(float), hipMemcpyDeviceToHost, s), "sample values");
+  ctx_->sync();
+  parallel_for(m, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      nearest[i] = double(stage[i]);
+      linear[i] = double(stage[m + i]);
+    }
+  });
+}
+
+bool HipBackend::finite_range(const HImage& img, double& lo, double& hi) {
+  const long long n = (long long)img.grid.num_voxels();
+  const int nb = norm_blocks(n);
+  float* p = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
+  {
+    Timed t(*ctx_, "k_finite_range", 4.0 * double(n));
+    launch_finite_range(img.buf.get(), n, p, p + nb, nb, ctx_->stream());
+  }
+  float* host = static_cast<float*>(ctx_->pinned(2 * nb * sizeof(float)));
+  check(hipMemcpyAsync(host, p, 2 * nb * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "range partials");
+  ctx_->sync();
+  float l = INFINITY, h = -INFINITY;
+  for (int b = 0; b < nb; ++b) {
+    l = std::min(l, host[b]);
+    h = std::max(h, host[nb + b]);
+  }
+  if (!(l <= h)) return false;
+  lo = double(l);
+  hi = double(h);
+  return true;
+}
+
 HField HipBackend::smooth_field(HField field, double variance) {
   if (variance <= 0.0) return field;
   const Grid& g = field.grid;
   const Dims d = dims_of(g);
   const size_t n = g.num_voxels();
-  DevBuf t1(ctx_.get(), 3 * n), t2(ctx_.get(), 3 * n);
   const double w1 = variance < 0.5 ? 1.0 - variance / 0.5 : 1.0;
   const double w2 = 1.0 - w1;
+  {
+    GaussTaps3 t{};
+    int* radii[3] = {&t.rx, &t.ry, &t.rz};
+    float* dst[3] = {t.x, t.y, t.z};
+    bool fits = true;
+    for (int a = 0; a < 3; ++a) {
+      const std::vector<double>& taps = ctx_->host_taps(variance, g.size[a]);
+      *radii[a] = int(taps.size() / 2);
+      if (taps.size() > 17) {
+        fits = false;
+        break;
+      }
+      for (size_t q = 0; q < taps.size(); ++q) dst[a][q] = float(taps[q]);
+    }
+    if (fits) {
+      HField out{g, DevBuf(ctx_.get(), 3 * n)};
+      Timed tm(*ctx_, "k_gauss3", 24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0));
+      if (launch_gauss3(field.buf.get(), out.buf.get(), d, t, float(w1), float(w2), ctx_->stream())) return out;
+    }
+  }
+  DevBuf t1(ctx_.get(), 3 * n), t2(ctx_.get(), 3 * n);
   int r = 0;
   const double nb = double(n);
   const float* tx = ctx_->taps(variance, g.size[0], r);
@@ -537,17 +699,23 @@ std::pair<HField, InvertStats> HipBackend::invert(const HField& field, HField in
   const float3 isp = inv_spacing(g);
   const int nb = inv_blocks(d);
   auto* state = static_cast<InvertState*>(ctx_->scratch(4, sizeof(InvertState)));
-  auto* partials = static_cast<float*>(ctx_->scratch(5, 4 * nb * sizeof(float)));
+  auto* partials = static_cast<float*>(ctx_->scratch(5, 4 * size_t(nb) * sizeof(float)));
   hipStream_t s = ctx_->stream();
   const int max_it = int(p.max_iterations);
+  // The iterations' launches are enqueued one by one; once the device has reported (through the
+  // mailbox) that this inversion stopped, the remaining ones, which would return at once, are not.
+  volatile uint32_t* mail = ctx_->mailbox_host();
+  uint32_t* mail_dev = ctx_->mailbox_device();
+  const uint32_t tag = ctx_->next_tag();
   {
     Timed t(*ctx_, "k_inv_begin", 24.0 * double(d.n()));
     launch_inv_begin(field.buf.get(), initial.buf.get(), d, l, isp, state, partials, nb, max_it, s);
   }
   for (int step = 0; step < max_it; ++step) {
+    if (step > 0 && *mail == tag) break;
     Timed t(*ctx_, "k_inv_iter", 0.0);
     launch_inv_iter(field.buf.get(), initial.buf.get(), d, l, isp, state, partials, nb, step, max_it,
-                    p.max_error_tolerance, p.mean_error_tolerance, p.enforce_boundary_condition ? 1 : 0, s);
+                    p.max_error_tolerance, p.mean_error_tolerance, p.enforce_boundary_condition ? 1 : 0, mail_dev, tag, s);
   }
   auto* host = static_cast<InvertState*>(ctx_->pinned(sizeof(InvertState)));
   check(hipMemcpyAsync(host, state, sizeof(InvertState), hipMemcpyDeviceToHost, s), "invert state");

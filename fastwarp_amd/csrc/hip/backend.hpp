@@ -20,6 +20,7 @@
 #include "../host/backend_types.hpp"
 #include "../host/grid.hpp"
 #include "../host/mi_host.hpp"
+#include "../host/pyramid.hpp"
 #include "../host/transform.hpp"
 #include "kernels.hpp"
 #include "mi_kernels.hpp"
@@ -46,9 +47,16 @@ class DeviceContext : public HostSpanSink {
   void* scratch(int slot, size_t bytes);
   // Pinned host memory for read-backs (one growable buffer).
   void* pinned(size_t bytes);
+  // A word of coherent host memory the device may store to (a hint channel: kernels report that
+  // an iterative loop has stopped, so the host stops enqueueing its steps). Host and device views.
+  volatile uint32_t* mailbox_host();
+  uint32_t* mailbox_device();
+  uint32_t next_tag() { return ++tag_; }
   void sync();
   // Device copy of the Gaussian taps of (variance, axis length), cached.
   const float* taps(double variance, size_t length, int& radius);
+  // The same taps on the host (f64), cached.
+  const std::vector<double>& host_taps(double variance, size_t length);
 
   // Per-kernel timing (the reference's profiler.rs:95-170): while on, launches are bracketed by
   // events; take_profile() waits, adds the spans up per kernel family and clears them.
@@ -72,8 +80,12 @@ class DeviceContext : public HostSpanSink {
   };
   Scratch scratch_[8];
   void* pinned_ = nullptr;
+  uint32_t* mailbox_ = nullptr;
+  uint32_t* mailbox_dev_ = nullptr;
+  uint32_t tag_ = 0;
   size_t pinned_bytes_ = 0;
   std::map<std::pair<double, size_t>, std::pair<void*, int>> taps_;
+  std::map<std::pair<double, size_t>, std::vector<double>> host_taps_;
   bool profiling_ = false;
   struct Span {
     std::string name;
@@ -196,6 +208,17 @@ class HipBackend {
   MetricPair<HField> mi_pair(const HImage& fixed, const HImage& moving, const HImage* fixed_mask, const HImage* moving_mask,
                              size_t bins, bool want_norms);
 
+  // Level smoothing on the device (pyramid.rs:429-465, f32 recursive Gaussian, axes z, x, y);
+  // sigma <= 0 gives a copy. The result's bound is its measured maximum magnitude.
+  HImage smooth_image(const HImage& src, double sigma, SigmaUnits units);
+  HImage smooth_image_mm(const HImage& src, const Vec3& sigma_mm);
+  // Nearest-voxel and linear values at physical points (NaN outside), as the linear stages'
+  // sample preparation reads them (mi.rs:161-207, linear.rs:251-279).
+  void sample_values(const HImage& img, const std::vector<Vec3>& points, std::vector<double>& nearest,
+                     std::vector<double>& linear);
+  // Finite minimum and maximum; false when the image has no finite value.
+  bool finite_range(const HImage& img, double& lo, double& hi);
+
   HField smooth_field(HField field, double variance);
   std::pair<double, double> max_scaled_norm_pair(const HField& a, const HField* b);
   double l2_norm(const HField& f);
@@ -234,22 +257,22 @@ class MiLevel {
   MiBinning binning_;
 };
 
-// A LinearEngine (linear.rs:288-320) whose levels own their uploaded moving image.
+// A LinearEngine (linear.rs:288-320) whose levels share the device-resident smoothed moving image.
 class HipLinearEngine {
  public:
   explicit HipLinearEngine(HipBackend& b) : backend_(b) {}
   struct Level {
-    std::unique_ptr<HImage> moving;
+    std::shared_ptr<const HImage> moving;
     std::unique_ptr<MiLevel> metric;
     double value(const LinearMovingTransform& t) { return metric->value(t); }
     std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t) {
       return metric->value_and_derivative(t);
     }
   };
-  Level prepare_level(const Image& moving, const std::vector<Vec3>& points, const std::vector<uint32_t>& fixed_bin,
-                      const MiBinning& binning) {
+  Level prepare_level(std::shared_ptr<const HImage> moving, const std::vector<Vec3>& points,
+                      const std::vector<uint32_t>& fixed_bin, const MiBinning& binning) {
     Level l;
-    l.moving = std::make_unique<HImage>(backend_.upload_image(moving));
+    l.moving = std::move(moving);
     l.metric = std::make_unique<MiLevel>(backend_, *l.moving, points, fixed_bin, binning);
     return l;
   }

@@ -40,18 +40,64 @@ __device__ __forceinline__ float tri(const float* __restrict__ img, const Dims& 
   axis_weight(cx, d.nx, bx, wx);
   axis_weight(cy, d.ny, by, wy);
   axis_weight(cz, d.nz, bz, wz);
-  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+  // all eight loads stay inside the buffer (a missing upper neighbour repeats the base, and its
+  // weight is zero), so they are issued together and the blends select instead of branching
+  const long long sx = bx + 1 < d.nx ? 1 : 0;
+  const long long sy = by + 1 < d.ny ? (long long)d.nx : 0;
+  const long long sz = bz + 1 < d.nz ? (long long)d.nx * d.ny : 0;
   const float* p = img + lin(d, bx, by, bz);
-  auto row = [&](const float* q) {
-    const float v0 = q[0];
-    return wx > 0.0f ? v0 + (q[1] - v0) * wx : v0;
-  };
-  auto plane = [&](const float* q) {
-    const float v0 = row(q);
-    return wy > 0.0f ? v0 + (row(q + sy) - v0) * wy : v0;
-  };
-  const float v0 = plane(p);
-  return wz > 0.0f ? v0 + (plane(p + sz) - v0) * wz : v0;
+  const float v0 = p[0], v1 = p[sx], v2 = p[sy], v3 = p[sy + sx];
+  const float v4 = p[sz], v5 = p[sz + sx], v6 = p[sz + sy], v7 = p[sz + sy + sx];
+  const float r00 = wx > 0.0f ? v0 + (v1 - v0) * wx : v0;
+  const float r01 = wx > 0.0f ? v2 + (v3 - v2) * wx : v2;
+  const float r10 = wx > 0.0f ? v4 + (v5 - v4) * wx : v4;
+  const float r11 = wx > 0.0f ? v6 + (v7 - v6) * wx : v6;
+  const float p0 = wy > 0.0f ? r00 + (r01 - r00) * wy : r00;
+  const float p1 = wy > 0.0f ? r10 + (r11 - r10) * wy : r10;
+  return wz > 0.0f ? p0 + (p1 - p0) * wz : p0;
+}
+
+// All three planes of a field at one continuous index (the same rule as tri()): the eight corner
+// offsets and weights are computed once and all 24 loads are issued before any blend.
+__device__ __forceinline__ void tri3(const float* __restrict__ u, const Dims& d, float cx, float cy, float cz,
+                                     float& ox, float& oy, float& oz) {
+  int bx, by, bz;
+  float wx, wy, wz;
+  axis_weight(cx, d.nx, bx, wx);
+  axis_weight(cy, d.ny, by, wy);
+  axis_weight(cz, d.nz, bz, wz);
+  const long long n = d.n();
+  const long long sx = bx + 1 < d.nx ? 1 : 0;
+  const long long sy = by + 1 < d.ny ? (long long)d.nx : 0;
+  const long long sz = bz + 1 < d.nz ? (long long)d.nx * d.ny : 0;
+  const long long g = lin(d, bx, by, bz);
+  float v[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float* p = u + c * n + g;
+    v[c][0] = p[0];
+    v[c][1] = p[sx];
+    v[c][2] = p[sy];
+    v[c][3] = p[sy + sx];
+    v[c][4] = p[sz];
+    v[c][5] = p[sz + sx];
+    v[c][6] = p[sz + sy];
+    v[c][7] = p[sz + sy + sx];
+  }
+  float r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float r00 = wx > 0.0f ? v[c][0] + (v[c][1] - v[c][0]) * wx : v[c][0];
+    const float r01 = wx > 0.0f ? v[c][2] + (v[c][3] - v[c][2]) * wx : v[c][2];
+    const float r10 = wx > 0.0f ? v[c][4] + (v[c][5] - v[c][4]) * wx : v[c][4];
+    const float r11 = wx > 0.0f ? v[c][6] + (v[c][7] - v[c][6]) * wx : v[c][6];
+    const float p0 = wy > 0.0f ? r00 + (r01 - r00) * wy : r00;
+    const float p1 = wy > 0.0f ? r10 + (r11 - r10) * wy : r10;
+    r[c] = wz > 0.0f ? p0 + (p1 - p0) * wz : p0;
+  }
+  ox = r[0];
+  oy = r[1];
+  oz = r[2];
 }
 
 // Nearest voxel with rounding half up (crates/fastwarp-reg/src/mask.rs:99-111); -1 outside.

@@ -303,6 +303,60 @@ void launch_abs_max(const float* x, long long n, float* partial, int blocks, hip
   hipLaunchKernelGGL(k_abs_max, dim3(blocks), dim3(256), 0, s, x, n, partial);
 }
 
+// Finite minimum and maximum of x per block; a block without a finite value reports +inf / -inf.
+__global__ __launch_bounds__(256) void k_finite_range(const float* __restrict__ x, long long n, float* partial_min,
+                                                      float* partial_max) {
+  float lo = INFINITY, hi = -INFINITY;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = x[i];
+    if (isfinite(v)) {
+      lo = fminf(lo, v);
+      hi = fmaxf(hi, v);
+    }
+  }
+  __shared__ float red[2][4];
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fminf(lo, __shfl_down(lo, off, 64));
+    hi = fmaxf(hi, __shfl_down(hi, off, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = lo;
+    red[1][w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+      lo = fminf(lo, red[0][q]);
+      hi = fmaxf(hi, red[1][q]);
+    }
+    partial_min[blockIdx.x] = lo;
+    partial_max[blockIdx.x] = hi;
+  }
+}
+
+void launch_finite_range(const float* x, long long n, float* partial_min, float* partial_max, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_finite_range, dim3(blocks), dim3(256), 0, s, x, n, partial_min, partial_max);
+}
+
+// Image values at continuous indices (mi.rs:161-207 and interp.rs:43-87 rules): the nearest voxel,
+// rounding half up, NaN outside [0, n); the ITK linear interpolation, NaN outside [-0.5, n - 0.5).
+__global__ __launch_bounds__(256) void k_sample_values(const float* __restrict__ img, Dims d,
+                                                       const float4* __restrict__ idx, long long m,
+                                                       float* __restrict__ nearest, float* __restrict__ linear) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const float4 c = idx[t];
+  const long long q = nearest_index(d, c.x, c.y, c.z);
+  nearest[t] = q >= 0 ? img[q] : NAN;
+  linear[t] = inside(d, c.x, c.y, c.z) ? tri(img, d, c.x, c.y, c.z) : NAN;
+}
+
+void launch_sample_values(const float* img, Dims d, const float4* idx, long long m, float* nearest, float* linear,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_sample_values, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, img, d, idx, m, nearest, linear);
+}
+
 // ------------------------------------------------------------------------------------------
 // Elementwise scale and axpy over the three planes (field_ops.rs:216-244).
 // ------------------------------------------------------------------------------------------
@@ -359,10 +413,8 @@ __global__ __launch_bounds__(256) void k_warp(WarpArgs a) {
     py += p.l[3] * ux + p.l[4] * uy + p.l[5] * uz;
     pz += p.l[6] * ux + p.l[7] * uy + p.l[8] * uz;
     if (inside(a.pd, px, py, pz)) {
-      const long long pn = a.pd.n();
-      const float dx = tri(a.pre, a.pd, px, py, pz);
-      const float dy = tri(a.pre + pn, a.pd, px, py, pz);
-      const float dz = tri(a.pre + 2 * pn, a.pd, px, py, pz);
+      float dx, dy, dz;
+      tri3(a.pre, a.pd, px, py, pz, dx, dy, dz);
       cx += a.pre_l[0] * dx + a.pre_l[1] * dy + a.pre_l[2] * dz;
       cy += a.pre_l[3] * dx + a.pre_l[4] * dy + a.pre_l[5] * dz;
       cz += a.pre_l[6] * dx + a.pre_l[7] * dy + a.pre_l[8] * dz;
@@ -391,7 +443,7 @@ __global__ __launch_bounds__(256) void k_resample(ResampleArgs a) {
   const int k = blockIdx.z;
   if (i >= a.od.nx || j >= a.od.ny) return;
   const long long g = lin(a.od, i, j, k);
-  const long long n = a.od.n(), sn = a.sd.n();
+  const long long n = a.od.n();
   const float fi = (float)i, fj = (float)j, fk = (float)k;
   const IndexMap& m = a.to_src;
   const float cx = fmaf(m.m[0], fi, fmaf(m.m[1], fj, fmaf(m.m[2], fk, m.c0[0])));
@@ -399,9 +451,7 @@ __global__ __launch_bounds__(256) void k_resample(ResampleArgs a) {
   const float cz = fmaf(m.m[6], fi, fmaf(m.m[7], fj, fmaf(m.m[8], fk, m.c0[2])));
   float vx = 0.0f, vy = 0.0f, vz = 0.0f;
   if (inside(a.sd, cx, cy, cz)) {
-    vx = tri(a.src, a.sd, cx, cy, cz);
-    vy = tri(a.src + sn, a.sd, cx, cy, cz);
-    vz = tri(a.src + 2 * sn, a.sd, cx, cy, cz);
+    tri3(a.src, a.sd, cx, cy, cz, vx, vy, vz);
   }
   a.out[g] = vx;
   a.out[g + n] = vy;
@@ -426,10 +476,11 @@ __device__ __forceinline__ void compose_at(const float* __restrict__ u, const Di
   oy = wy;
   oz = wz;
   if (inside(d, cx, cy, cz)) {
-    const long long n = d.n();
-    ox += tri(u, d, cx, cy, cz);
-    oy += tri(u + n, d, cx, cy, cz);
-    oz += tri(u + 2 * n, d, cx, cy, cz);
+    float ux, uy, uz;
+    tri3(u, d, cx, cy, cz, ux, uy, uz);
+    ox += ux;
+    oy += uy;
+    oz += uz;
   }
 }
 
@@ -456,6 +507,174 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
   const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
   hipLaunchKernelGGL(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out);
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused separable Gaussian of a field (field_ops.rs:508-574), one pass over HBM: a block owns a
+// 64 x 16 (x, y) tile of one component over a run of z planes. Each input plane (edge-clamped) is
+// staged in LDS with its x/y halo, smoothed along x then y (runs of 4 outputs per thread, so each
+// LDS value is read once per run), and the y-smoothed plane enters a ring of 2rz+1 planes held in
+// registers; the z tap sum of the ring is the output plane rz behind. The arithmetic is the
+// three-pass filter's, in the same order (x, then y, then z, each an fma chain over the taps in
+// order), so the result is bit-identical to it. The final blend w1 * s + w2 * u and the zeroed
+// faces are applied on the way out.
+// ------------------------------------------------------------------------------------------
+constexpr int kG3TX = 64, kG3TY = 16, kG3MaxR = 8;
+
+template <int RZ>
+__global__ __launch_bounds__(256) void k_gauss3(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                GaussTaps3 t, int zrun, float w1, float w2) {
+  constexpr int W = 2 * RZ + 1;
+  __shared__ float tile[kG3TY + 2 * kG3MaxR][kG3TX + 2 * kG3MaxR];
+  __shared__ float xs[kG3TY + 2 * kG3MaxR][kG3TX];
+  const int rx = t.rx, ry = t.ry;
+  const int LX = kG3TX + 2 * rx, LY = kG3TY + 2 * ry;
+  const int comp = blockIdx.z % 3;
+  const int chunk = blockIdx.z / 3;
+  const int x0 = blockIdx.x * kG3TX, y0 = blockIdx.y * kG3TY;
+  const int z0 = chunk * zrun;
+  const int z1 = min(z0 + zrun, d.nz);
+  const long long n = d.n();
+  const long long plane = (long long)d.nx * d.ny;
+  const float* src = in + comp * n;
+  float* dst = out + comp * n;
+  const int tid = threadIdx.x;
+  const int tx = tid & 63, ty = (tid >> 6) * 4;  // y-pass: column tx, rows ty..ty+3
+  float ring[W][4];
+#pragma unroll
+  for (int q = 0; q < W; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ring[q][r] = 0.0f;
+  // This thread's tile elements (e = tid + 256 q) and their edge-clamped offsets within a plane,
+  // the same for every plane; the next plane is fetched into registers while this one is filtered.
+  constexpr int kPer = ((kG3TX + 2 * kG3MaxR) * (kG3TY + 2 * kG3MaxR) + 255) / 256;
+  int off[kPer], slot[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = tid + 256 * q;
+    const int ly = e / LX, lx = e - ly * LX;
+    int gx = x0 - rx + lx, gy = y0 - ry + ly;
+    gx = gx < 0 ? 0 : (gx > d.nx - 1 ? d.nx - 1 : gx);
+    gy = gy < 0 ? 0 : (gy > d.ny - 1 ? d.ny - 1 : gy);
+    off[q] = e < LX * LY ? gx + d.nx * gy : -1;
+    slot[q] = ly * (kG3TX + 2 * kG3MaxR) + lx;
+  }
+  float pre[kPer];
+  auto fetch = [&](int zin) {
+    const int zc = zin < 0 ? 0 : (zin > d.nz - 1 ? d.nz - 1 : zin);
+    const float* p = src + plane * zc;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q)
+      if (off[q] >= 0) pre[q] = p[off[q]];
+  };
+  fetch(z0 - RZ);
+
+  for (int zin = z0 - RZ; zin < z1 + RZ; ++zin) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q)
+      if (off[q] >= 0) (&tile[0][0])[slot[q]] = pre[q];
+    __syncthreads();
+    if (zin + 1 < z1 + RZ) fetch(zin + 1);
+    // x pass: runs of 4 outputs along x
+    for (int e = tid; e < LY * (kG3TX / 4); e += 256) {
+      const int ly = e / (kG3TX / 4), lx = (e - ly * (kG3TX / 4)) * 4;
+      float v[4 + 2 * kG3MaxR];
+#pragma unroll
+      for (int q = 0; q < 4 + 2 * kG3MaxR; ++q)
+        if (q < 4 + 2 * rx) v[q] = tile[ly][lx + q];
+      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll
+      for (int k = 0; k <= 2 * kG3MaxR; ++k) {
+        if (k <= 2 * rx) {
+          const float w = t.x[k];
+          a0 = fmaf(w, v[k], a0);
+          a1 = fmaf(w, v[k + 1], a1);
+          a2 = fmaf(w, v[k + 2], a2);
+          a3 = fmaf(w, v[k + 3], a3);
+        }
+      }
+      xs[ly][lx] = a0;
+      xs[ly][lx + 1] = a1;
+      xs[ly][lx + 2] = a2;
+      xs[ly][lx + 3] = a3;
+    }
+    __syncthreads();
+    // y pass: rows ty..ty+3 of column tx
+    float v[4 + 2 * kG3MaxR];
+#pragma unroll
+    for (int q = 0; q < 4 + 2 * kG3MaxR; ++q)
+      if (q < 4 + 2 * ry) v[q] = xs[ty + q][tx];
+    float b[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k <= 2 * kG3MaxR; ++k) {
+      if (k <= 2 * ry) {
+        const float w = t.y[k];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) b[r] = fmaf(w, v[k + r], b[r]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < W - 1; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ring[q][r] = ring[q + 1][r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ring[W - 1][r] = b[r];
+    const int k = zin - RZ;
+    if (k >= z0) {
+      const int x = x0 + tx;
+      if (x < d.nx) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int y = y0 + ty + r;
+          if (y >= d.ny) continue;
+          const long long g = (long long)x + (long long)d.nx * y + plane * k;
+          float o = 0.0f;
+          if (!(x == 0 || y == 0 || k == 0 || x == d.nx - 1 || y == d.ny - 1 || k == d.nz - 1)) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int q = 0; q < W; ++q) acc = fmaf(t.z[q], ring[q][r], acc);
+            o = w2 != 0.0f ? acc * w1 + src[g] * w2 : acc * w1;
+          }
+          dst[g] = o;
+        }
+      }
+    }
+  }
+}
+
+// The z-run per block: about 512 blocks (two per CU), runs at least 4 * rz planes long, so the
+// 2 * rz halo planes each run re-reads stay a minority of its input.
+static int g3_zrun(const Dims& d, int rz) {
+  const long long tiles = (long long)((d.nx + kG3TX - 1) / kG3TX) * ((d.ny + kG3TY - 1) / kG3TY) * 3;
+  long long chunks = (512 + tiles - 1) / tiles;
+  int run = (int)((d.nz + chunks - 1) / chunks);
+  const int least = 4 * rz > 8 ? 4 * rz : 8;
+  return run < least ? least : run;
+}
+
+bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s) {
+  if (t.rx > kG3MaxR || t.ry > kG3MaxR || t.rz > kG3MaxR || t.rz < 1) return false;
+  const int run = g3_zrun(d, t.rz);
+  const dim3 grid((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * ((d.nz + run - 1) / run));
+  switch (t.rz) {
+#define FW_G3_CASE(R)                                                                       \
+  case R:                                                                                   \
+    hipLaunchKernelGGL(k_gauss3<R>, grid, dim3(256), 0, s, in, out, d, t, run, w1, w2);     \
+    break;
+    FW_G3_CASE(1)
+    FW_G3_CASE(2)
+    FW_G3_CASE(3)
+    FW_G3_CASE(4)
+    FW_G3_CASE(5)
+    FW_G3_CASE(6)
+    FW_G3_CASE(7)
+    FW_G3_CASE(8)
+#undef FW_G3_CASE
+    default:
+      return false;
+  }
+  return true;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -490,16 +709,20 @@ __global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ 
   if (line >= lines) return;
   const float* p = in + base;
   float* q = out + base;
+  // The filter has unit DC gain, so it runs on x - x[0] and x[0] is added back: the recursion then
+  // carries only the line's variation, which keeps the f32 rounding of its coefficients (whose
+  // DC gain is ill-conditioned at large sigma) from scaling with the mean intensity.
+  const float shift = p[0];
   // causal: y[i] = sum n_k x[i-k] - sum d_k y[i-1-k], with x and y extended by x[0] at the start
   {
-    const float v1 = p[0];
+    const float v1 = 0.0f;
     float x0 = v1, x1 = v1, x2 = v1, x3 = v1;  // x[i], x[i-1], x[i-2], x[i-3]
     float y1 = 0.0f, y2 = 0.0f, y3 = 0.0f, y4 = 0.0f;
     for (int i = 0; i < len; ++i) {
       x3 = x2;
       x2 = x1;
       x1 = x0;
-      x0 = p[(long long)i * stride];
+      x0 = p[(long long)i * stride] - shift;
       float y = x0 * c.n[0] + x1 * c.n[1] + x2 * c.n[2] + x3 * c.n[3];
       // outputs before the start of the line are replaced by the boundary terms v1 * bn_k
       const float sub = (i >= 1 ? y1 * c.d[0] : v1 * c.bn[0]) + (i >= 2 ? y2 * c.d[1] : v1 * c.bn[1]) +
@@ -514,7 +737,7 @@ __global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ 
   }
   // anti-causal: a[i-1] = sum m_k x[i+k] - sum d_k a[i+k], with x extended by x[len-1] at the end
   {
-    const float v2 = p[(long long)(len - 1) * stride];
+    const float v2 = p[(long long)(len - 1) * stride] - shift;
     float x0 = v2, x1 = v2, x2 = v2, x3 = v2;  // x[i], x[i+1], x[i+2], x[i+3] for a[i-1]
     float a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;  // a[i], a[i+1], a[i+2], a[i+3]
     for (int t = 0; t < len; ++t) {
@@ -523,7 +746,7 @@ __global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ 
       x3 = x2;
       x2 = x1;
       x1 = x0;
-      x0 = i <= len - 1 ? p[(long long)i * stride] : v2;
+      x0 = i <= len - 1 ? p[(long long)i * stride] - shift : v2;
       float y = x0 * c.m[0] + x1 * c.m[1] + x2 * c.m[2] + x3 * c.m[3];
       const float sub = (t >= 1 ? a1 * c.d[0] : v2 * c.bm[0]) + (t >= 2 ? a2 * c.d[1] : v2 * c.bm[1]) +
                         (t >= 3 ? a3 * c.d[2] : v2 * c.bm[2]) + (t >= 4 ? a4 * c.d[3] : v2 * c.bm[3]);
@@ -533,7 +756,7 @@ __global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ 
       a2 = a1;
       a1 = y;
       const long long o = (long long)(i - 1) * stride;
-      q[o] = q[o] + y;
+      q[o] = (q[o] + y) + shift;
     }
   }
 }
@@ -595,18 +818,34 @@ void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const floa
 // Inversion (field_ops.rs:315-392, ITK InvertDisplacementFieldImageFilter): one launch per
 // iteration. Iteration i runs only if state->stop[i & 1] is clear. Every block first reduces the
 // residual partials of psi_i that the previous launch left (the same fixed-order reduction in
-// every block, so all blocks agree on mean_i and max_i without a grid barrier). It then applies
-// the update psi_{i+1} = psi_i + eps * clip(-e_i, eps * max_i), with e_i recomputed at each voxel.
-// If another iteration may follow, it also writes the partials of e_{i+1}. The residual at x
-// gathers only the forward field, never psi at other voxels, so psi is updated in place. Block 0
-// publishes the statistics and stop[(i + 1) & 1], a word no block of this launch reads.
+// every block, so all blocks agree on mean_i and max_i without a grid barrier or a device-scope
+// fence; a release at device scope would write back the whole L2 of each XCD). It then applies
+// psi_{i+1} = psi_i + eps * clip(-e_i, eps * max_i), with e_i recomputed at each voxel, and, if
+// another iteration may follow, writes the partials of e_{i+1}. The residual at x gathers only the
+// forward field, never psi at other voxels, so psi is updated in place. Block 0 publishes the
+// statistics and stop[(i + 1) & 1], a word no block of the same launch reads. Each thread takes
+// kInvVoxels voxels per step of its grid-stride loop so their gathers are in flight together.
 // ------------------------------------------------------------------------------------------
 constexpr int kInvThreads = 256;
+constexpr int kInvVoxels = 4;
+constexpr int kInvMaxBlocks = 1024;
 
 __device__ __forceinline__ void inv_residual_at(const float* __restrict__ field, const Dims& d, const float* l,
                                                 float3 inv_sp, int i, int j, int k, float wx, float wy, float wz,
                                                 float& ex, float& ey, float& ez, float& nrm) {
-  compose_at(field, d, l, i, j, k, wx, wy, wz, ex, ey, ez);
+  const float cx = (float)i + (l[0] * wx + l[1] * wy + l[2] * wz);
+  const float cy = (float)j + (l[3] * wx + l[4] * wy + l[5] * wz);
+  const float cz = (float)k + (l[6] * wx + l[7] * wy + l[8] * wz);
+  ex = wx;
+  ey = wy;
+  ez = wz;
+  if (inside(d, cx, cy, cz)) {
+    float ux, uy, uz;
+    tri3(field, d, cx, cy, cz, ux, uy, uz);
+    ex += ux;
+    ey += uy;
+    ez += uz;
+  }
   const float px = ex * inv_sp.x, py = ey * inv_sp.y, pz = ez * inv_sp.z;
   nrm = sqrtf(px * px + py * py + pz * pz);
 }
@@ -618,20 +857,6 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   s = t;
 }
 
-// Sum and max over the block, broadcast to every thread.
-__device__ __forceinline__ void block_sum_max_all(float s, float m, float& total, float& mx) {
-  __shared__ float sh[2];
-  const float bs = block_sum(s);
-  const float bm = block_max(m);
-  if (threadIdx.x == 0) {
-    sh[0] = bs;
-    sh[1] = bm;
-  }
-  __syncthreads();
-  total = sh[0];
-  mx = sh[1];
-}
-
 __device__ __forceinline__ void inv_write_partials(float s, float m, float* partials, int blocks) {
   const float bs = block_sum(s);
   const float bm = block_max(m);
@@ -641,19 +866,32 @@ __device__ __forceinline__ void inv_write_partials(float s, float m, float* part
   }
 }
 
+__device__ __forceinline__ void inv_coords(const Dims& d, long long g, int& i, int& j, int& k) {
+  i = (int)(g % d.nx);
+  const long long r = g / d.nx;
+  j = (int)(r % d.ny);
+  k = (int)(r / d.ny);
+}
+
 __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restrict__ field, const float* __restrict__ psi,
                                                            Dims d, L9 l, float3 inv_sp, InvertState* state,
                                                            float* partials, int max_iterations) {
   const long long n = d.n();
+  const long long span = (long long)gridDim.x * kInvThreads;
   float s = 0.0f, c = 0.0f, m = 0.0f;
-  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(g % d.nx);
-    const long long r = g / d.nx;
-    const int j = (int)(r % d.ny), k = (int)(r / d.ny);
-    float ex, ey, ez, nrm;
-    inv_residual_at(field, d, l.v, inv_sp, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez, nrm);
-    m = nan_max(m, nrm);
-    kahan_add(s, c, nrm);
+  for (long long base = (long long)blockIdx.x * kInvThreads + threadIdx.x; base < n; base += span * kInvVoxels) {
+#pragma unroll
+    for (int v = 0; v < kInvVoxels; ++v) {
+      const long long g = base + v * span;
+      if (g < n) {
+        int i, j, k;
+        inv_coords(d, g, i, j, k);
+        float ex, ey, ez, nrm;
+        inv_residual_at(field, d, l.v, inv_sp, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez, nrm);
+        m = nan_max(m, nrm);
+        kahan_add(s, c, nrm);
+      }
+    }
   }
   inv_write_partials(s, m, partials, gridDim.x);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -668,7 +906,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
 __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi, Dims d,
                                                           L9 l, float3 inv_sp, InvertState* state, float* partials,
                                                           int step, int max_iterations, float max_tol, float mean_tol,
-                                                          int enforce) {
+                                                          int enforce, uint32_t* mailbox, uint32_t tag) {
   if (state->stop[step & 1]) {
     // once stopped, stay stopped: the next launch reads the other word
     if (blockIdx.x == 0 && threadIdx.x == 0) state->stop[(step + 1) & 1] = 1;
@@ -682,53 +920,70 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
     kahan_add(s, c, in[b]);
     m = nan_max(m, in[blocks + b]);
   }
-  float total, mx;
-  block_sum_max_all(s, m, total, mx);
+  __shared__ float stats[2];
+  {
+    const float bs = block_sum(s);
+    const float bm = block_max(m);
+    if (threadIdx.x == 0) {
+      stats[0] = bs;
+      stats[1] = bm;
+    }
+    __syncthreads();
+  }
   const long long n = d.n();
-  const float mean = total / (float)n;
+  const float mean = stats[0] / (float)n;
+  const float mx = stats[1];
   const bool next = (step + 1) < max_iterations && mx > max_tol && mean > mean_tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     state->mean_error = mean;
     state->max_error = mx;
     state->executed = step + 1;
     state->stop[(step + 1) & 1] = next ? 0 : 1;
+    // a hint for the host: later launches of this inversion would return at once
+    if (!next) __hip_atomic_store(mailbox, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   const float eps = step == 0 ? 0.75f : 0.5f;
   const float limit = eps * mx;
+  const long long span = (long long)gridDim.x * kInvThreads;
   s = 0.0f;
   c = 0.0f;
   m = 0.0f;
-  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(g % d.nx);
-    const long long r = g / d.nx;
-    const int j = (int)(r % d.ny), k = (int)(r / d.ny);
-    const float wx = psi[g], wy = psi[g + n], wz = psi[g + 2 * n];
-    float ex, ey, ez, nrm;
-    inv_residual_at(field, d, l.v, inv_sp, i, j, k, wx, wy, wz, ex, ey, ez, nrm);
-    float ux = -ex, uy = -ey, uz = -ez;
-    if (nrm > limit) {
-      const float f = limit / nrm;
-      ux *= f;
-      uy *= f;
-      uz *= f;
-    }
-    float nx = wx + ux * eps, ny = wy + uy * eps, nz = wz + uz * eps;
-    if (enforce && (i == 0 || j == 0 || k == 0 || i == d.nx - 1 || j == d.ny - 1 || k == d.nz - 1)) nx = ny = nz = 0.0f;
-    psi[g] = nx;
-    psi[g + n] = ny;
-    psi[g + 2 * n] = nz;
-    if (next) {
-      inv_residual_at(field, d, l.v, inv_sp, i, j, k, nx, ny, nz, ex, ey, ez, nrm);
-      m = nan_max(m, nrm);
-      kahan_add(s, c, nrm);
+  for (long long base = (long long)blockIdx.x * kInvThreads + threadIdx.x; base < n; base += span * kInvVoxels) {
+#pragma unroll
+    for (int v = 0; v < kInvVoxels; ++v) {
+      const long long g = base + v * span;
+      if (g < n) {
+        int i, j, k;
+        inv_coords(d, g, i, j, k);
+        const float wx = psi[g], wy = psi[g + n], wz = psi[g + 2 * n];
+        float ex, ey, ez, nrm;
+        inv_residual_at(field, d, l.v, inv_sp, i, j, k, wx, wy, wz, ex, ey, ez, nrm);
+        float ux = -ex, uy = -ey, uz = -ez;
+        if (nrm > limit) {
+          const float f = limit / nrm;
+          ux *= f;
+          uy *= f;
+          uz *= f;
+        }
+        float nx = wx + ux * eps, ny = wy + uy * eps, nz = wz + uz * eps;
+        if (enforce && (i == 0 || j == 0 || k == 0 || i == d.nx - 1 || j == d.ny - 1 || k == d.nz - 1)) nx = ny = nz = 0.0f;
+        psi[g] = nx;
+        psi[g + n] = ny;
+        psi[g + 2 * n] = nz;
+        if (next) {
+          inv_residual_at(field, d, l.v, inv_sp, i, j, k, nx, ny, nz, ex, ey, ez, nrm);
+          m = nan_max(m, nrm);
+          kahan_add(s, c, nrm);
+        }
+      }
     }
   }
   if (next) inv_write_partials(s, m, out, blocks);
 }
 
 int inv_blocks(const Dims& d) {
-  long long b = (d.n() + kInvThreads - 1) / kInvThreads;
-  return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+  const long long b = (d.n() + kInvThreads * kInvVoxels - 1) / (kInvThreads * kInvVoxels);
+  return (int)(b < 1 ? 1 : (b > kInvMaxBlocks ? kInvMaxBlocks : b));
 }
 
 void launch_inv_begin(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
@@ -741,11 +996,11 @@ void launch_inv_begin(const float* field, const float* psi, Dims d, const float 
 
 void launch_inv_iter(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing, InvertState* state,
                      float* partials, int blocks, int step, int max_iterations, double max_tol, double mean_tol,
-                     int enforce_boundary, hipStream_t s) {
+                     int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
   hipLaunchKernelGGL(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, d, lv, inv_spacing, state,
-                     partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary);
+                     partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
 }
 
 

@@ -55,6 +55,11 @@ void launch_field_norms(const float* a, const float* b, long long n, float3 inv_
 int norm_blocks(long long n);
 // In-place scaled max of image magnitudes (for fixed-point scales): out[block] = max |x|.
 void launch_abs_max(const float* x, long long n, float* partial, int blocks, hipStream_t s);
+// Per-block finite minimum / maximum (+inf / -inf for a block without finite values).
+void launch_finite_range(const float* x, long long n, float* partial_min, float* partial_max, int blocks, hipStream_t s);
+// Nearest-voxel and linear values at m continuous indices (x, y, z; w unused); NaN outside.
+void launch_sample_values(const float* img, Dims d, const float4* idx, long long m, float* nearest, float* linear,
+                          hipStream_t s);
 
 // --- elementwise ---
 void launch_scale(float* x, long long count, float s, hipStream_t st);
@@ -92,10 +97,17 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
 // --- Gaussian smoothing, separable, clamped edges; final pass zeroes faces and blends ---
 void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const float* taps_dev, int radius,
                        const float* orig, float w1, float w2, int final_pass, hipStream_t s);
+// All three axes in one pass over HBM (`in` is also the blend's original); taps per axis, radius
+// at most 8 each (else nothing is launched and false is returned).
+struct GaussTaps3 {
+  float x[17], y[17], z[17];
+  int rx, ry, rz;
+};
+bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s);
 
 // --- inversion (ITK InvertDisplacementFieldImageFilter) ---
 struct InvertState {
-  // written by block 0 of each iteration's launch; read by the host after the last launch
+  // written by block 0 of each launch; read by the host after the last launch
   float mean_error;
   float max_error;
   int32_t executed;
@@ -105,10 +117,11 @@ struct InvertState {
 // Residual partials of the initial estimate (2 * blocks floats: sums, then maxima) and state reset.
 void launch_inv_begin(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
                       InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s);
-// One iteration; partials holds two such buffers (4 * blocks floats), alternating by step parity.
+// One iteration; `partials` holds two such buffers (4 * blocks floats), alternating by step parity.
+// The launch that ends the loop stores `tag` to `mailbox` (host memory) as a hint.
 void launch_inv_iter(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing, InvertState* state,
                      float* partials, int blocks, int step, int max_iterations, double max_tol, double mean_tol,
-                     int enforce_boundary, hipStream_t s);
+                     int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s);
 int inv_blocks(const Dims& d);
 
 // --- recursive Gaussian (Deriche, 4th order) along one axis, constant extension at both ends ---

@@ -5,6 +5,7 @@
 // fixed_bin, binning) returns a LinearMetric.
 #pragma once
 
+#include <future>
 #include <map>
 #include <memory>
 #include <optional>
@@ -50,33 +51,67 @@ struct LinearRecipe {
 };
 
 // Smoothed full-resolution image pairs shared by the rigid, affine and SyN stages of one
-// registration (the reference's SmoothCache, linear.rs:569-754), keyed by sigma.
-struct SmoothCache {
-  std::map<double, std::pair<std::shared_ptr<Image>, std::shared_ptr<Image>>> pairs;
-  std::pair<std::shared_ptr<Image>, std::shared_ptr<Image>> get(const Image& fixed, const Image& moving, double sigma, SigmaUnits u) {
-    auto it = pairs.find(sigma);
-    if (it != pairs.end()) return it->second;
-    auto f = std::make_shared<Image>(smooth_for_level(fixed, sigma, u));
-    auto m = std::make_shared<Image>(smooth_for_level(moving, sigma, u));
-    pairs[sigma] = {f, m};
-    return {f, m};
+// registration (the reference's SmoothCache, linear.rs:569-754), keyed by channel and sigma and
+// held as backend images (device-resident on the HIP backend). Each channel's preprocessed pair is
+// uploaded once, on first use; sigma <= 0 is the upload itself.
+template <typename B>
+class SmoothCache {
+ public:
+  using ImageH = typename B::ImageH;
+  using Ptr = std::shared_ptr<const ImageH>;
+  struct Pair {
+    Ptr fixed, moving;
+  };
+
+  SmoothCache(B& backend, std::vector<std::pair<const Image*, const Image*>> channels)
+      : b_(backend), host_(std::move(channels)) {}
+
+  B& backend() { return b_; }
+  const Grid& fixed_grid(size_t channel) const { return host_.at(channel).first->grid; }
+
+  Pair get(size_t channel, double sigma, SigmaUnits units) {
+    if (channel >= host_.size()) invalid("smoothing", "no such channel");
+    const double key = sigma > 0.0 ? sigma : 0.0;
+    auto it = pairs_.find({channel, key});
+    if (it != pairs_.end()) return it->second;
+    Pair p;
+    if (key == 0.0) {
+      p.fixed = std::make_shared<const ImageH>(b_.upload_image(*host_[channel].first));
+      p.moving = std::make_shared<const ImageH>(b_.upload_image(*host_[channel].second));
+    } else {
+      const Pair raw = get(channel, 0.0, units);
+      HostTimer t("smooth_images");
+      p.fixed = std::make_shared<const ImageH>(b_.smooth_image(*raw.fixed, sigma, units));
+      p.moving = std::make_shared<const ImageH>(b_.smooth_image(*raw.moving, sigma, units));
+    }
+    pairs_[{channel, key}] = p;
+    return p;
   }
+
+  // Finite range of a cached moving image (mi.rs:209-236 reads it for the moving bins).
+  std::pair<double, double> moving_range(size_t channel, double sigma, SigmaUnits units) {
+    const double key = sigma > 0.0 ? sigma : 0.0;
+    auto it = ranges_.find({channel, key});
+    if (it != ranges_.end()) return it->second;
+    const Pair p = get(channel, sigma, units);
+    double lo = 0.0, hi = 0.0;
+    if (!b_.finite_range(*p.moving, lo, hi)) invalid("mattes", "the moving image has no finite voxel");
+    ranges_[{channel, key}] = {lo, hi};
+    return {lo, hi};
+  }
+
+ private:
+  B& b_;
+  std::vector<std::pair<const Image*, const Image*>> host_;
+  std::map<std::pair<size_t, double>, Pair> pairs_;
+  std::map<std::pair<size_t, double>, std::pair<double, double>> ranges_;
 };
 
-// mi.rs:161-169 and 193-207: nearest-voxel value at a point, none outside.
-inline bool nearest_value(const Image& vol, const Vec3& p, double& out) {
-  size_t idx;
-  if (!nearest_voxel(vol.grid, p, idx)) return false;
-  out = vol.data[idx];
-  return true;
-}
-
 // mi.rs:193-207 `fixed_sample_range`: count and range of the finite nearest-voxel values.
-inline size_t fixed_sample_range(const Image& fixed, const std::vector<Vec3>& samples, double& lo, double& hi) {
+inline size_t fixed_sample_range(const std::vector<double>& nearest, double& lo, double& hi) {
   size_t count = 0;
-  for (const Vec3& s : samples) {
-    double v;
-    if (!nearest_value(fixed, s, v) || !std::isfinite(v)) continue;
+  for (double v : nearest) {
+    if (!std::isfinite(v)) continue;
     if (count == 0) lo = hi = v;
     else {
       lo = std::min(lo, v);
@@ -93,34 +128,23 @@ struct PreparedSamples {
   MiBinning binning;
 };
 
-// linear.rs:251-279 `prepare_samples` with MiBinning::new (mi.rs:209-236).
-inline PreparedSamples prepare_samples(const Image& fixed, const Image& moving, const std::vector<Vec3>& samples, size_t bins) {
+// linear.rs:251-279 `prepare_samples` with MiBinning::new (mi.rs:209-236), from the fixed image's
+// nearest-voxel and linear values at the raw sample points (NaN outside) and the moving range:
+// samples whose linear fixed value is not finite are dropped, the rest get their fixed bin.
+inline PreparedSamples prepare_samples(const std::vector<Vec3>& samples, const std::vector<double>& nearest,
+                                       const std::vector<double>& linear, std::pair<double, double> moving_range,
+                                       size_t bins) {
   HostTimer host_timer("prepare_samples");
   if (bins <= 4) invalid("mattes", "at least five histogram bins are required");
   double flo = 0, fhi = 0;
-  if (fixed_sample_range(fixed, samples, flo, fhi) == 0) invalid("mattes", "no fixed-image samples were observed");
+  if (fixed_sample_range(nearest, flo, fhi) == 0) invalid("mattes", "no fixed-image samples were observed");
   if (fhi - flo <= kEps) invalid("mattes", "the fixed image is constant over the samples");
-  double mlo = 0, mhi = 0;
-  bool any = false;
-  for (double v : moving.data) {
-    if (!std::isfinite(v)) continue;
-    if (!any) {
-      mlo = mhi = v;
-      any = true;
-    } else {
-      mlo = std::min(mlo, v);
-      mhi = std::max(mhi, v);
-    }
-  }
-  if (!any) invalid("mattes", "the moving image has no finite voxel");
   PreparedSamples out;
-  out.binning = MiBinning::from_ranges(bins, flo, fhi, mlo, mhi);
+  out.binning = MiBinning::from_ranges(bins, flo, fhi, moving_range.first, moving_range.second);
   std::vector<int64_t> bin(samples.size(), -1);
   parallel_for(samples.size(), [&](size_t b, size_t e) {
-    for (size_t i = b; i < e; ++i) {
-      double v;
-      if (linear_at_point(fixed, samples[i], v) && std::isfinite(v)) bin[i] = int64_t(out.binning.fixed_bin(v));
-    }
+    for (size_t i = b; i < e; ++i)
+      if (std::isfinite(linear[i])) bin[i] = int64_t(out.binning.fixed_bin(linear[i]));
   });
   for (size_t i = 0; i < samples.size(); ++i)
     if (bin[i] >= 0) {
@@ -158,21 +182,46 @@ struct LevelCache {
 };
 
 // linear.rs:873-1068 `run_linear_stage_cached_async`.
-template <typename Engine>
-LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, const Image& fixed, const Image& moving,
-                                   const AffineTransform& outer, const ParametricTransform& start, SmoothCache& cache,
+template <typename Engine, typename B>
+LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, SmoothCache<B>& cache,
+                                   const AffineTransform& outer, const ParametricTransform& start,
                                    LevelCache<Engine>* levels_cache, const volatile int32_t* cancel) {
+  const Grid& fixed_grid = cache.fixed_grid(0);
   recipe.validate();
   if (start.count() != (recipe.kind == LinearKind::Rigid ? 6u : 12u))
     invalid("linear stage", "start transform does not match the stage kind");
   SeedSequence seeds(recipe.seed);
   LinearMovingTransform current{outer, start};
   LinearStageResult result;
+  // Level l draws its sample points with seed + l whether or not it samples (each level takes one
+  // seed, sampling.rs:159-163), so the next level's points are drawn on a worker thread while this
+  // level optimises (the reference prefetches them the same way, linear.rs:773-869).
+  auto level_grid = [&](size_t level) {
+    return level_virtual_grid(fixed_grid, shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing));
+  };
+  auto needs_samples = [&](size_t level) {
+    return !(levels_cache && (levels_cache->skipped.count(level) || levels_cache->levels.count(level)));
+  };
+  auto draw = [&recipe](Grid grid, uint32_t seed) {
+    SeedSequence s(seed);
+    return metric_sample_points(grid, recipe.sampling_percentage, s);
+  };
+  std::future<std::vector<Vec3>> prefetched;
+  size_t prefetched_level = recipe.levels();
   for (size_t level = 0; level < recipe.levels(); ++level) {
     check_cancel(cancel);
     const auto t0 = std::chrono::steady_clock::now();
-    const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed.grid.spacing);
-    const Grid grid = level_virtual_grid(fixed.grid, factors);
+    const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing);
+    const Grid grid = level_grid(level);
+    std::optional<std::vector<Vec3>> ready;
+    if (prefetched_level == level && prefetched.valid()) {
+      HostTimer wait("sample_wait");
+      ready = prefetched.get();
+    }
+    if (level + 1 < recipe.levels() && needs_samples(level + 1)) {
+      prefetched = std::async(std::launch::async, draw, level_grid(level + 1), uint32_t(recipe.seed + level + 1));
+      prefetched_level = level + 1;
+    }
     const double sigma = recipe.smoothing_sigmas[level];
     LinearLevelTrace tr;
     tr.level = level;
@@ -187,18 +236,27 @@ LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, c
       metric = levels_cache->levels[level];
       tr.samples = levels_cache->sample_counts[level];
     } else {
-      auto [fs, ms] = cache.get(fixed, moving, sigma, recipe.sigma_units);
-      const std::vector<Vec3> raw = metric_sample_points(grid, recipe.sampling_percentage, seeds);
+      const auto images = cache.get(0, sigma, recipe.sigma_units);
+      std::vector<Vec3> raw;
+      if (ready) {
+        seeds.next_seed();
+        raw = std::move(*ready);
+      } else {
+        raw = metric_sample_points(grid, recipe.sampling_percentage, seeds);
+      }
+      std::vector<double> nearest, linear;
+      cache.backend().sample_values(*images.fixed, raw, nearest, linear);
       double lo = 0, hi = 0;
-      const size_t observed = fixed_sample_range(*fs, raw, lo, hi);
+      const size_t observed = fixed_sample_range(nearest, lo, hi);
       if (observed == 0) tr.skipped = "no sample point lands on the fixed image";
       else if (hi - lo <= kEps) tr.skipped = "the fixed image takes a single value at its sample points";
       if (tr.skipped.empty()) {
-        PreparedSamples prep = prepare_samples(*fs, *ms, raw, recipe.bins);
+        PreparedSamples prep = prepare_samples(raw, nearest, linear, cache.moving_range(0, sigma, recipe.sigma_units), recipe.bins);
         if (prep.points.empty()) tr.skipped = "no sample point lies inside the fixed image";
         else {
           tr.samples = prep.points.size();
-          metric = std::make_shared<typename Engine::Level>(engine.prepare_level(*ms, prep.points, prep.fixed_bin, prep.binning));
+          metric = std::make_shared<typename Engine::Level>(
+              engine.prepare_level(images.moving, prep.points, prep.fixed_bin, prep.binning));
         }
       }
       if (levels_cache) {
@@ -264,10 +322,9 @@ struct LinearChainResult {
 
 // linear.rs:1148-1212 `run_linear_chain_cached_async`: centre of mass, Rigid, Affine, collapsed
 // about the last linear transform's centre.
-template <typename Engine>
+template <typename Engine, typename B>
 LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine, const Image& fixed_raw,
-                                   const Image& moving_raw, const Image& fixed, const Image& moving, SmoothCache& cache,
-                                   const volatile int32_t* cancel) {
+                                   const Image& moving_raw, SmoothCache<B>& cache, const volatile int32_t* cancel) {
   if (rigid.kind != LinearKind::Rigid || affine.kind != LinearKind::Affine)
     invalid("linear chain", "expected a rigid then an affine recipe");
   LinearChainResult out;
@@ -276,11 +333,11 @@ LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, co
                              rigid.smoothing_sigmas == affine.smoothing_sigmas && rigid.bins == affine.bins &&
                              rigid.sampling_percentage == affine.sampling_percentage && rigid.sigma_units == affine.sigma_units;
   LevelCache<Engine> levels;
-  out.rigid = run_linear_stage(engine, rigid, fixed, moving, out.initial, ParametricTransform::euler3d({0, 0, 0, 0, 0, 0}, out.initial.center),
-                               cache, same_sampling ? &levels : nullptr, cancel);
+  out.rigid = run_linear_stage(engine, rigid, cache, out.initial, ParametricTransform::euler3d({0, 0, 0, 0, 0, 0}, out.initial.center),
+                               same_sampling ? &levels : nullptr, cancel);
   const AffineTransform after_rigid = AffineTransform::compose(out.initial, out.rigid.affine);
   const ParametricTransform start = ParametricTransform::affine(AffineTransform{kIdentity3, {0, 0, 0}, out.rigid.affine.center});
-  out.affine = run_linear_stage(engine, affine, fixed, moving, after_rigid, start, cache, same_sampling ? &levels : nullptr, cancel);
+  out.affine = run_linear_stage(engine, affine, cache, after_rigid, start, same_sampling ? &levels : nullptr, cancel);
   out.total = with_center(AffineTransform::compose(after_rigid, out.affine.affine), out.affine.affine.center);
   return out;
 }

@@ -55,11 +55,12 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
     channels.push_back({preprocess_volume(fixed[c], plan.winsorize_lower, plan.winsorize_upper),
                         preprocess_volume(moving[c], plan.winsorize_lower, plan.winsorize_upper)});
   out.seconds_preprocess = secs(t);
-  SmoothCache cache;
+  std::vector<std::pair<const Image*, const Image*>> pre;
+  for (const SynChannel& c : channels) pre.push_back({&c.fixed, &c.moving});
+  SmoothCache<B> cache(backend, std::move(pre));
   t = clock::now();
   if (plan.run_linear) {
-    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, fixed[0], moving[0], channels[0].fixed, channels[0].moving,
-                                  cache, cancel);
+    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, fixed[0], moving[0], cache, cancel);
     out.initial = out.linear.initial;
     out.total = out.linear.total;
   } else {
@@ -69,7 +70,7 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
   if (plan.run_syn) {
     backend.release_idle_buffers();
     t = clock::now();
-    out.syn = syn_fields(backend, plan.syn, channels, out.total, &cache, cancel, outputs);
+    out.syn = syn_fields(backend, plan.syn, channels, out.total, cache, cancel, outputs);
     out.seconds_syn = secs(t);
   }
   out.seconds_total = secs(t_all);

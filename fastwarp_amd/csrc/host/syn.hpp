@@ -109,7 +109,7 @@ struct Side {
 // syn.rs:776-986 `syn_fields` from a linear start `initial` (maps fixed points to moving points).
 template <typename B>
 SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel>& channels, const AffineTransform& initial,
-                     SmoothCache* cache, const volatile int32_t* cancel, FieldOutputs outputs = {}) {
+                     SmoothCache<B>& cache, const volatile int32_t* cancel, FieldOutputs outputs = {}) {
   using FieldH = typename B::FieldH;
   using ImageH = typename B::ImageH;
   recipe.validate();
@@ -133,17 +133,8 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel
     const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing);
     const Grid grid = level_virtual_grid(fixed_grid, factors);
     const double sigma = recipe.smoothing_sigmas[level];
-    std::vector<ImageH> fixed_smooth, moving_smooth;
-    for (size_t c = 0; c < nc; ++c) {
-      if (c == 0 && cache) {
-        auto [f, m] = cache->get(channels[c].fixed, channels[c].moving, sigma, recipe.sigma_units);
-        fixed_smooth.push_back(b.upload_image(*f));
-        moving_smooth.push_back(b.upload_image(*m));
-      } else {
-        fixed_smooth.push_back(b.upload_image(smooth_for_level(channels[c].fixed, sigma, recipe.sigma_units)));
-        moving_smooth.push_back(b.upload_image(smooth_for_level(channels[c].moving, sigma, recipe.sigma_units)));
-      }
-    }
+    std::vector<typename SmoothCache<B>::Pair> smooth;
+    for (size_t c = 0; c < nc; ++c) smooth.push_back(cache.get(c, sigma, recipe.sigma_units));
     if (!f2m) {
       f2m = b.zero_field(grid);
       f2m_inv = b.zero_field(grid);
@@ -169,8 +160,8 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel
       // 1. resample every channel into the middle (syn.rs:895-911)
       detail::Side<B> fs, ms;
       for (size_t c = 0; c < nc; ++c) {
-        ImageH fmid = b.warp_through_field(fixed_smooth[c], *f2m_inv, nullptr, nullptr);
-        ImageH mmid = b.warp_through_field(moving_smooth[c], *m2m_inv, nullptr, &initial);
+        ImageH fmid = b.warp_through_field(*smooth[c].fixed, *f2m_inv, nullptr, nullptr);
+        ImageH mmid = b.warp_through_field(*smooth[c].moving, *m2m_inv, nullptr, &initial);
         // 2. the metric both ways (syn.rs:337-420 `side_updates`)
         const bool single = nc == 1;
         MetricPair<FieldH> pair = recipe.metric(c) == SynMetric::Cc

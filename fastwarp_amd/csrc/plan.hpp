@@ -141,8 +141,13 @@ void run_register(B& backend, E& engine, const fw_plan* plan, const fw_grid* fix
     HostTimer t_in("copy_inputs");
     for (int32_t c = 0; c < plan->channels; ++c) {
       if (!fixed || !moving || !fixed[c] || !moving[c]) invalid("register", "null image data");
-      f.emplace_back(fg, std::vector<double>(fixed[c], fixed[c] + fg.num_voxels()));
-      m.emplace_back(mg, std::vector<double>(moving[c], moving[c] + mg.num_voxels()));
+      auto copy = [](const double* src, size_t n) {
+        std::vector<double> v(n);
+        parallel_for(n, [&](size_t b, size_t e) { std::copy(src + b, src + e, v.begin() + long(b)); });
+        return v;
+      };
+      f.emplace_back(fg, copy(fixed[c], fg.num_voxels()));
+      m.emplace_back(mg, copy(moving[c], mg.num_voxels()));
     }
   }
   const RegisterOutput o = register_full(backend, engine, p, f, m, cancel, FieldOutputs{forward, inverse});

@@ -173,6 +173,15 @@ int fw_mi_level_value_and_derivative(fw_mi_level* level, const fw_linear_transfo
                                      double* derivative);
 void fw_mi_level_free(fw_mi_level* level);
 
+/* ---- level images on the device (pyramid.rs:429-465 smooth_recursive_gaussian; the reference
+ *      smooths on the host in f64 and uploads, this engine smooths the uploaded image in place of
+ *      that: same filter, axes z, x, y, f32 arithmetic) ---- */
+/* sigma_mm per axis, each > 0 (or all three 0 for a copy); every axis needs 4 voxels. */
+int fw_smooth_image(fw_ctx* ctx, const fw_image* src, const double sigma_mm[3], fw_image** out);
+/* Values of `img` at m physical points (3m f64): the nearest voxel, rounding half up, and the
+ * linear interpolation (mi.rs:161-207, linear.rs:251-279); NaN outside. Either output may be NULL. */
+int fw_sample_values(fw_ctx* ctx, const fw_image* img, const double* points, int64_t m, double* nearest, double* linear);
+
 /* ---- whole registrations (stages.rs:286 register_stages_async with the HIP backend) ---- */
 
 /* The stage plan of one full normalisation: centre of mass, Rigid, Affine (Mattes MI), SyN (CC).

@@ -43,6 +43,24 @@ def _check(st: int):
         raise OracleError(lib().fwo_last_error().decode())
 
 
+def smooth_image(grid: Grid, data, sigma_mm) -> np.ndarray:
+    """smooth_recursive_gaussian (pyramid.rs:429-437), f64."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    s = np.ascontiguousarray(sigma_mm, dtype=np.float64).reshape(3)
+    out = np.empty(grid.n)
+    _check(lib().fwo_smooth_image(C.byref(grid), dptr(d), dptr(s), dptr(out)))
+    return out
+
+
+def sample_values(grid: Grid, data, points) -> tuple[np.ndarray, np.ndarray]:
+    """(nearest-voxel, linear) values at physical points, NaN outside (mi.rs:161-169, interp.rs:43-87)."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    near, lin = np.empty(len(pts)), np.empty(len(pts))
+    _check(lib().fwo_sample_values(C.byref(grid), dptr(d), dptr(pts), C.c_int64(len(pts)), dptr(near), dptr(lin)))
+    return near, lin
+
+
 def warp(src_grid: Grid, src, field_grid: Grid, field, pre_grid: Grid | None = None, pre=None, affine: Affine | None = None,
          nearest: bool = False, product=None) -> np.ndarray:
     s = np.ascontiguousarray(src, dtype=np.float64)

@@ -69,6 +69,30 @@ extern "C" {
 
 const char* fwo_last_error() { return g_error.c_str(); }
 
+// pyramid.rs:429-437 `smooth_recursive_gaussian` in f64 (the drivers' host restatement).
+int fwo_smooth_image(const fw_grid* grid, const double* data, const double sigma_mm[3], double* out) {
+  return guard([&] {
+    Image v = image_from(grid, data);
+    Image r = fw::smooth_recursive_gaussian(v, {sigma_mm[0], sigma_mm[1], sigma_mm[2]});
+    std::copy(r.data.begin(), r.data.end(), out);
+  });
+}
+
+// mi.rs:161-169 nearest-voxel value and interp.rs:43-87 linear value at physical points, NaN outside.
+int fwo_sample_values(const fw_grid* grid, const double* data, const double* points, int64_t m, double* nearest,
+                      double* linear) {
+  return guard([&] {
+    Image v = image_from(grid, data);
+    for (int64_t i = 0; i < m; ++i) {
+      const fw::Vec3 p{points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+      const auto q = oracle::nearest(v.grid, p);
+      nearest[i] = q ? v.data[*q] : std::nan("");
+      const auto l = oracle::lerp_point(v, p);
+      linear[i] = l ? *l : std::nan("");
+    }
+  });
+}
+
 int fwo_warp(const fw_grid* src_grid, const double* src, const fw_grid* field_grid, const double* field,
              const fw_grid* pre_grid, const double* pre, const fw_affine* affine, int nearest, const double* product,
              double* out) {

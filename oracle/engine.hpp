@@ -17,6 +17,29 @@ class CpuBackend {
 
   void release_idle_buffers() {}
   Image upload_image(const Image& v) { return v; }
+  Image smooth_image(const Image& v, double sigma, SigmaUnits units) { return smooth_for_level(v, sigma, units); }
+  // mi.rs:161-207 and interp.rs:43-87: nearest-voxel and linear values, NaN outside.
+  void sample_values(const Image& img, const std::vector<Vec3>& points, std::vector<double>& nearest,
+                     std::vector<double>& linear) {
+    nearest.assign(points.size(), std::nan(""));
+    linear.assign(points.size(), std::nan(""));
+    for (size_t i = 0; i < points.size(); ++i) {
+      size_t idx;
+      if (nearest_voxel(img.grid, points[i], idx)) nearest[i] = img.data[idx];
+      double v;
+      if (linear_at_point(img, points[i], v)) linear[i] = v;
+    }
+  }
+  bool finite_range(const Image& img, double& lo, double& hi) {
+    bool any = false;
+    for (double v : img.data) {
+      if (!std::isfinite(v)) continue;
+      lo = any ? std::min(lo, v) : v;
+      hi = any ? std::max(hi, v) : v;
+      any = true;
+    }
+    return any;
+  }
   VectorField zero_field(const Grid& g) { return VectorField::zeros(g); }
   VectorField download_field(const VectorField& f) { return f; }
   void download_field_to(const VectorField& f, double* aos) {
@@ -85,10 +108,10 @@ class CpuLinearEngine {
       return {e.value, e.derivative};
     }
   };
-  Level prepare_level(const Image& moving, const std::vector<Vec3>& points, const std::vector<uint32_t>& fixed_bin,
-                      const MiBinning& b) {
+  Level prepare_level(std::shared_ptr<const Image> moving, const std::vector<Vec3>& points,
+                      const std::vector<uint32_t>& fixed_bin, const MiBinning& b) {
     Level l;
-    l.moving = moving;
+    l.moving = *moving;
     l.points = points;
     l.fixed_bin = fixed_bin;
     l.binning = Binning::from_ranges(b.bins, b.fixed_min, b.fixed_max, b.moving_min, b.moving_max);
