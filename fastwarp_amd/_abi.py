@@ -202,6 +202,20 @@ class Plan(C.Structure):
             self.syn_iterations[i], self.syn_shrink[i], self.syn_sigmas[i] = it, sh, sg
 
 
+class Capacity(C.Structure):
+    """fw_capacity: device bytes of one registration on this engine (include/fastwarp_hip.h)."""
+
+    _fields_ = [
+        ("fixed_voxels", C.c_uint64),
+        ("moving_voxels", C.c_uint64),
+        ("smoothing_cache_bytes", C.c_uint64),
+        ("linear_bytes", C.c_uint64),
+        ("syn_bytes", C.c_uint64),
+        ("peak_bytes", C.c_uint64),
+        ("largest_buffer_bytes", C.c_uint64),
+    ]
+
+
 class Result(C.Structure):
     """fw_result: the collapsed affine, phase timings and per-level counts of one registration."""
 

@@ -14,7 +14,8 @@ from pathlib import Path
 
 import numpy as np
 
-from ._abi import Affine, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, Plan, Result, aos, dptr, u32ptr
+from ._abi import (Affine, Capacity, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, Plan, Result, aos, dptr,
+                   u32ptr)
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfastwarp_hip.so"
 
@@ -80,6 +81,14 @@ _SIGNATURES = [
     ("fw_mi_level_free", None, [_P]),
     ("fw_set_profiling", C.c_int, [_P, C.c_int]),
     ("fw_take_profile", C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    ("fw_plan_capacity", C.c_int, [C.POINTER(Plan), C.POINTER(Grid), C.POINTER(Grid), C.POINTER(Capacity)]),
+    ("fw_memory_budget", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("fw_fits", C.c_int, [_P, C.POINTER(Plan), C.POINTER(Grid), C.POINTER(Grid), C.POINTER(C.c_int32)]),
+    (
+        "fw_registrations_at_once",
+        C.c_int,
+        [_P, C.POINTER(Plan), C.POINTER(Grid), C.POINTER(Grid), C.c_uint32, C.POINTER(C.c_uint32)],
+    ),
     ("fw_plan_cohort", None, [C.POINTER(Plan), C.c_int32]),
     (
         "fw_register",
@@ -88,6 +97,14 @@ _SIGNATURES = [
          C.POINTER(C.POINTER(C.c_double)), _D, _D, C.POINTER(Result), C.POINTER(C.c_int32)],
     ),
 ]
+
+
+def plan_capacity(plan: Plan, fixed_grid: Grid, moving_grid: Grid | None = None) -> Capacity:
+    """Device bytes one registration of `plan` needs on this engine (no device is touched)."""
+    out = Capacity()
+    if lib().fw_plan_capacity(C.byref(plan), C.byref(fixed_grid), C.byref(moving_grid or fixed_grid), C.byref(out)) != FW_OK:
+        raise RegError(-1, "capacity: invalid plan or grid")
+    return out
 
 
 def exported_symbols() -> list[str]:
@@ -211,6 +228,23 @@ class HipBackend:
         out = C.c_void_p()
         self._check(fn(self.ctx, *args, C.byref(out)))
         return DeviceField(self, out)
+
+    # --- admission (capacity.rs, fits_gpu / registrations_at_once) ---
+    def memory_budget(self) -> int:
+        b = C.c_uint64()
+        self._check(lib().fw_memory_budget(self.ctx, C.byref(b)))
+        return b.value
+
+    def fits(self, plan: Plan, fixed_grid: Grid, moving_grid: Grid | None = None) -> bool:
+        f = C.c_int32()
+        self._check(lib().fw_fits(self.ctx, C.byref(plan), C.byref(fixed_grid), C.byref(moving_grid or fixed_grid), C.byref(f)))
+        return bool(f.value)
+
+    def registrations_at_once(self, plan: Plan, fixed_grid: Grid, moving_grid: Grid | None = None, max: int = 1 << 16) -> int:
+        n = C.c_uint32()
+        self._check(lib().fw_registrations_at_once(self.ctx, C.byref(plan), C.byref(fixed_grid), C.byref(moving_grid or fixed_grid),
+                                                   max, C.byref(n)))
+        return n.value
 
     # --- transfers ---
     def release_idle_buffers(self):

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "hip/backend.hpp"
+#include "host/capacity.hpp"
 #include "plan.hpp"
 
 using fw::Grid;
@@ -482,6 +483,56 @@ int fw_take_profile(fw_ctx* ctx, char* text, size_t capacity) {
     const size_t n = std::min(p.size(), capacity - 1);
     std::memcpy(text, p.data(), n);
     text[n] = '\0';
+  });
+}
+
+static fw::CapacityPlan capacity_of(const fw_plan* plan, const fw_grid* fixed_grid, const fw_grid* moving_grid) {
+  need(plan, "plan");
+  need(fixed_grid, "fixed grid");
+  need(moving_grid, "moving grid");
+  const fw::RegisterPlan p = fw::plan_from(plan);
+  return fw::plan_capacity(p, fw::grid_of(fixed_grid), fw::grid_of(moving_grid), size_t(std::max(plan->channels, 0)));
+}
+
+static void fill(const fw::CapacityPlan& c, fw_capacity* out) {
+  out->fixed_voxels = c.fixed_voxels;
+  out->moving_voxels = c.moving_voxels;
+  out->smoothing_cache_bytes = c.smoothing_cache_bytes;
+  out->linear_bytes = c.linear_bytes;
+  out->syn_bytes = c.syn_bytes;
+  out->peak_bytes = c.peak_bytes;
+  out->largest_buffer_bytes = c.largest_buffer_bytes;
+}
+
+int fw_plan_capacity(const fw_plan* plan, const fw_grid* fixed_grid, const fw_grid* moving_grid, fw_capacity* out) {
+  try {
+    need(out, "output");
+    fill(capacity_of(plan, fixed_grid, moving_grid), out);
+    return FW_OK;
+  } catch (const std::exception&) {
+    return FW_ERR_INVALID;
+  }
+}
+
+int fw_memory_budget(fw_ctx* ctx, uint64_t* bytes) {
+  return guard(ctx, [&] {
+    need(bytes, "output");
+    *bytes = ctx->backend.memory_budget();
+  });
+}
+
+int fw_fits(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const fw_grid* moving_grid, int32_t* fits) {
+  return guard(ctx, [&] {
+    need(fits, "output");
+    *fits = capacity_of(plan, fixed_grid, moving_grid).peak_bytes <= ctx->backend.memory_budget() ? 1 : 0;
+  });
+}
+
+int fw_registrations_at_once(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const fw_grid* moving_grid,
+                             uint32_t max, uint32_t* count) {
+  return guard(ctx, [&] {
+    need(count, "output");
+    *count = fw::registrations_at_once(capacity_of(plan, fixed_grid, moving_grid), ctx->backend.memory_budget(), max);
   });
 }
 

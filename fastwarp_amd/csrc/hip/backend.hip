@@ -4,6 +4,7 @@
 #include "backend.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "../host/gauss_taps.hpp"
@@ -29,6 +30,7 @@ DeviceContext::~DeviceContext() {
     if (s.p) hipFree(s.p);
   if (pinned_) hipHostFree(pinned_);
   if (mailbox_) hipHostFree(mailbox_);
+  for (auto& c : stage_) hipHostFree(c.p);
   for (auto& kv : taps_) hipFree(kv.second.first);
   for (auto& s : spans_) {
     (void)hipEventDestroy(s.a);
@@ -101,6 +103,29 @@ void* DeviceContext::pinned(size_t bytes) {
     check(hipHostMalloc(&pinned_, pinned_bytes_, hipHostMallocDefault), "hipHostMalloc");
   }
   return pinned_;
+}
+
+void* DeviceContext::stage(size_t bytes) {
+  bytes = round_bytes(bytes);
+  for (auto& c : stage_)
+    if (c.capacity - c.used >= bytes) {
+      void* p = c.p + c.used;
+      c.used += bytes;
+      return p;
+    }
+  StageChunk c;
+  c.capacity = std::max<size_t>(bytes, size_t(1) << 20);
+  void* p = nullptr;
+  check(hipHostMalloc(&p, c.capacity, hipHostMallocDefault), "hipHostMalloc stage");
+  c.p = static_cast<char*>(p);
+  c.used = bytes;
+  stage_.push_back(c);
+  return c.p;
+}
+
+void DeviceContext::settle() {
+  sync();
+  for (auto& c : stage_) c.used = 0;
 }
 
 volatile uint32_t* DeviceContext::mailbox_host() {
@@ -216,6 +241,18 @@ void DevBuf::reset() {
 HipBackend::HipBackend(int device) : ctx_(std::make_shared<DeviceContext>(device)) {}
 
 // --------------------------------------------------------------------- transfers (a1, a2) ---
+
+uint64_t HipBackend::memory_budget() const {
+  if (const char* env = std::getenv("FASTWARP_GPU_MEMORY_BUDGET_MB")) {
+    char* end = nullptr;
+    const unsigned long long mb = std::strtoull(env, &end, 10);
+    if (end != env) return uint64_t(mb) << 20;
+  }
+  size_t free_bytes = 0, total = 0;
+  check(hipMemGetInfo(&free_bytes, &total), "hipMemGetInfo");
+  const uint64_t reserve = uint64_t(1) << 30;
+  return free_bytes > reserve ? uint64_t(free_bytes) - reserve : 0;
+}
 
 HImage HipBackend::upload_image(const Image& v) {
   HostTimer host_timer("upload_image");
@@ -460,6 +497,85 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
 
 // ------------------------------------------------------------------------- field ops (a8-a13) ---
 
+MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImage& moving, size_t radius) {
+  if (fixed.grid != moving.grid) invalid("hip backend", "cc: the images are not on one grid");
+  if (radius == 0 || radius > 8) invalid("hip backend", "cc radius " + std::to_string(radius) + " outside 1..=8");
+  const Grid& g = fixed.grid;
+  const size_t n = g.num_voxels();
+  MetricPair<HField> out;
+  out.fields[0] = HField{g, DevBuf(ctx_.get(), 3 * n)};
+  out.fields[1] = HField{g, DevBuf(ctx_.get(), 3 * n)};
+  CcArgs a{};
+  a.fixed = fixed.buf.get();
+  a.moving = moving.buf.get();
+  a.valid = nullptr;
+  a.d = dims_of(g);
+  a.radius = int(radius);
+  a.tile_z = std::max(1, cc_tile_z);
+  a.qf = fixed_point_scale(fixed.absmax);
+  a.qm = fixed_point_scale(moving.absmax);
+  a.inv_qf = 1.0f / a.qf;
+  a.inv_qm = 1.0f / a.qm;
+  a.probes = probes_of(g);
+  a.d_fixed = out.fields[0].buf.get();
+  a.d_moving = out.fields[1].buf.get();
+  const dim3 gr = cc_grid(a.d, a.tile_z);
+  const size_t blocks = size_t(gr.x) * gr.y * gr.z;
+  // the partials live in a buffer of this call's own: the next evaluation may run before settle()
+  DevBuf partials(ctx_.get(), 2 * blocks);
+  a.partial_cc = partials.get();
+  a.partial_count = reinterpret_cast<int*>(partials.get() + blocks);
+  {
+    Timed t(*ctx_, "k_cc_pair", 56.0 * double(n));
+    launch_cc_pair(a, ctx_->stream());
+  }
+  const int tx = radius <= 5 ? 64 : 32;
+  const size_t real = size_t((a.d.nx + tx - 1) / tx) * ((a.d.ny + 3) / 4) * ((a.d.nz + a.tile_z - 1) / a.tile_z);
+  char* host = static_cast<char*>(ctx_->stage(real * (sizeof(float) + sizeof(int))));
+  check(hipMemcpyAsync(host, a.partial_cc, real * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "cc partials");
+  check(hipMemcpyAsync(host + real * sizeof(float), a.partial_count, real * sizeof(int), hipMemcpyDeviceToHost,
+                       ctx_->stream()),
+        "cc counts");
+  out.later = Pending<std::array<double, 2>>(std::function<std::array<double, 2>()>([host, real] {
+    const float* pcc = reinterpret_cast<const float*>(host);
+    const int* pcount = reinterpret_cast<const int*>(host + real * sizeof(float));
+    double total = 0.0;
+    long long count = 0;
+    for (size_t b = 0; b < real; ++b) {
+      total += double(pcc[b]);
+      count += pcount[b];
+    }
+    if (count == 0) invalid("cc metric", "no virtual voxel maps inside both image buffers");
+    const double v = -total / double(count);
+    return std::array<double, 2>{v, v};
+  }));
+  return out;
+}
+
+Pending<std::pair<double, double>> HipBackend::scale_to_step(HField& a, HField& b, double step) {
+  if (a.grid != b.grid) invalid("hip backend", "update pair: grids differ");
+  const long long n = long(a.grid.num_voxels());
+  const int nb = norm_blocks(n);
+  float* pm = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
+  float* ps = static_cast<float*>(ctx_->scratch(3, 2 * nb * sizeof(float)));
+  DevBuf out(ctx_.get(), 4);  // two factors, two maxima
+  hipStream_t s = ctx_->stream();
+  {
+    Timed t(*ctx_, "k_field_norms", 24.0 * double(n));
+    launch_field_norms(a.buf.get(), b.buf.get(), n, inv_spacing(a.grid), pm, ps, nb, s);
+  }
+  launch_step_factors(pm, nb, 2, step, out.get(), out.get() + 2, s);
+  {
+    Timed t(*ctx_, "k_scale", 48.0 * double(n));
+    launch_scale_by(a.buf.get(), 3 * n, out.get(), s);
+    launch_scale_by(b.buf.get(), 3 * n, out.get() + 1, s);
+  }
+  float* host = static_cast<float*>(ctx_->stage(2 * sizeof(float)));
+  check(hipMemcpyAsync(host, out.get() + 2, 2 * sizeof(float), hipMemcpyDeviceToHost, s), "update maxima");
+  return Pending<std::pair<double, double>>(
+      std::function<std::pair<double, double>()>([host] { return std::make_pair(double(host[0]), double(host[1])); }));
+}
+
 HImage HipBackend::smooth_image(const HImage& src, double sigma, SigmaUnits units) {
   const Vec3& sp = src.grid.spacing;
   if (sigma <= 0.0) return smooth_image_mm(src, {0.0, 0.0, 0.0});
@@ -691,6 +807,13 @@ HField HipBackend::compose(const HField& displacement, const HField& warping) {
 }
 
 std::pair<HField, InvertStats> HipBackend::invert(const HField& field, HField initial, const InvertParams& p) {
+  auto r = invert_deferred(field, std::move(initial), p);
+  ctx_->settle();
+  return {std::move(r.first), r.second.get()};
+}
+
+std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField& field, HField initial,
+                                                                    const InvertParams& p) {
   if (initial.grid != field.grid) invalid("invert", "initial estimate grid differs");
   const Grid& g = field.grid;
   const Dims d = dims_of(g);
@@ -707,26 +830,34 @@ std::pair<HField, InvertStats> HipBackend::invert(const HField& field, HField in
   volatile uint32_t* mail = ctx_->mailbox_host();
   uint32_t* mail_dev = ctx_->mailbox_device();
   const uint32_t tag = ctx_->next_tag();
+  DevBuf residual(ctx_.get(), 3 * g.num_voxels());
   {
-    Timed t(*ctx_, "k_inv_begin", 24.0 * double(d.n()));
-    launch_inv_begin(field.buf.get(), initial.buf.get(), d, l, isp, state, partials, nb, max_it, s);
+    Timed t(*ctx_, "k_inv_begin", 36.0 * double(d.n()));
+    launch_inv_begin(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, partials, nb, max_it, s);
   }
   for (int step = 0; step < max_it; ++step) {
     if (step > 0 && *mail == tag) break;
     Timed t(*ctx_, "k_inv_iter", 0.0);
-    launch_inv_iter(field.buf.get(), initial.buf.get(), d, l, isp, state, partials, nb, step, max_it,
+    launch_inv_iter(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, partials, nb, step, max_it,
                     p.max_error_tolerance, p.mean_error_tolerance, p.enforce_boundary_condition ? 1 : 0, mail_dev, tag, s);
   }
-  auto* host = static_cast<InvertState*>(ctx_->pinned(sizeof(InvertState)));
+  auto* host = static_cast<InvertState*>(ctx_->stage(sizeof(InvertState)));
   check(hipMemcpyAsync(host, state, sizeof(InvertState), hipMemcpyDeviceToHost, s), "invert state");
-  ctx_->sync();
-  InvertStats st;
-  st.iterations = uint32_t(host->executed);
-  // each executed iteration reads psi and the field and writes psi
-  if (ctx_->profiling()) ctx_->add_bytes("k_inv_iter", 36.0 * double(d.n()) * host->executed);
-  st.mean_error_norm = host->executed ? double(host->mean_error) : 0.0;
-  st.max_error_norm = host->executed ? double(host->max_error) : 0.0;
-  return {std::move(initial), st};
+  DeviceContext* ctx = ctx_.get();
+  const double voxels = double(d.n());
+  const bool profiling = ctx_->profiling();
+  return {std::move(initial), Pending<InvertStats>(std::function<InvertStats()>([host, ctx, voxels, profiling] {
+            InvertStats st;
+            st.iterations = uint32_t(host->executed);
+            // an iteration that another follows reads psi, the stored residual and the field and
+            // writes psi and the new residual (60 B per voxel); the last one reads psi and the
+            // residual and writes psi (36 B)
+            if (profiling && host->executed > 0)
+              ctx->add_bytes("k_inv_iter", voxels * (60.0 * double(host->executed - 1) + 36.0));
+            st.mean_error_norm = host->executed ? double(host->mean_error) : 0.0;
+            st.max_error_norm = host->executed ? double(host->max_error) : 0.0;
+            return st;
+          }))};
 }
 
 HField HipBackend::resample_field(const HField& field, const Grid& grid) {

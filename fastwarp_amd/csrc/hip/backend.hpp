@@ -47,6 +47,11 @@ class DeviceContext : public HostSpanSink {
   void* scratch(int slot, size_t bytes);
   // Pinned host memory for read-backs (one growable buffer).
   void* pinned(size_t bytes);
+  // Pinned host space for a read-back whose value is wanted only at the next settle(): a bump
+  // allocation valid until then (chunks are added, never moved, while copies may be in flight).
+  void* stage(size_t bytes);
+  // Waits for the stream and recycles the staging space.
+  void settle();
   // A word of coherent host memory the device may store to (a hint channel: kernels report that
   // an iterative loop has stopped, so the host stops enqueueing its steps). Host and device views.
   volatile uint32_t* mailbox_host();
@@ -80,6 +85,11 @@ class DeviceContext : public HostSpanSink {
   };
   Scratch scratch_[8];
   void* pinned_ = nullptr;
+  struct StageChunk {
+    char* p = nullptr;
+    size_t capacity = 0, used = 0;
+  };
+  std::vector<StageChunk> stage_;
   uint32_t* mailbox_ = nullptr;
   uint32_t* mailbox_dev_ = nullptr;
   uint32_t tag_ = 0;
@@ -185,6 +195,9 @@ class HipBackend {
   DeviceContext& ctx() { return *ctx_; }
 
   void release_idle_buffers() { ctx_->release_idle(); }
+  // Device bytes a new registration may use: FASTWARP_GPU_MEMORY_BUDGET_MB when set, else the
+  // device's free memory less 1 GiB kept in reserve (capacity.rs:162-180).
+  uint64_t memory_budget() const;
   HImage upload_image(const Image& v);
   HImage upload_image_f32(const Grid& g, const float* data);
   Image download_image(const HImage& im);
@@ -226,6 +239,14 @@ class HipBackend {
   void axpy(HField& acc, double a, const HField& x);
   HField compose(const HField& displacement, const HField& warping);
   std::pair<HField, InvertStats> invert(const HField& field, HField initial, const InvertParams& p);
+
+  // Deferred forms for the SyN loop: results read back at the next settle().
+  MetricPair<HField> cc_pair_deferred(const HImage& fixed, const HImage& moving, size_t radius);
+  // Scales both updates so each one's largest scaled voxel norm is `step` (step / max, or step when
+  // the maximum is not positive), on the device; the two maxima come back at settle().
+  Pending<std::pair<double, double>> scale_to_step(HField& a, HField& b, double step);
+  std::pair<HField, Pending<InvertStats>> invert_deferred(const HField& field, HField initial, const InvertParams& p);
+  void settle() { ctx_->settle(); }
   HField resample_field(const HField& field, const Grid& grid);
 
   // CC tile height in z of the fused kernel.

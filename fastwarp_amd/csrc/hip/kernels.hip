@@ -291,6 +291,34 @@ void launch_field_norms(const float* a, const float* b, long long n, float3 inv_
   hipLaunchKernelGGL(k_field_norms, dim3(blocks), dim3(256), 0, s, a, b, n, inv_spacing, partial_max, partial_sq);
 }
 
+// Update factors of the SyN loop (syn.rs:422-445): for each of `fields` fields, the NaN-propagating
+// maximum of its partial maxima and factor = step / max (step when the maximum is not positive),
+// divided in f64 and rounded once to f32, as the host computed it. One block.
+__global__ __launch_bounds__(256) void k_step_factors(const float* __restrict__ partial_max, int blocks, int fields,
+                                                      double step, float* factors, float* maxima) {
+  for (int f = 0; f < fields; ++f) {
+    float m = 0.0f;
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x) m = nan_max(m, partial_max[f * blocks + b]);
+    const float r = block_max(m);
+    if (threadIdx.x == 0) {
+      maxima[f] = r;
+      factors[f] = r > 0.0f ? (float)(step / (double)r) : (float)step;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_step_factors(const float* partial_max, int blocks, int fields, double step, float* factors, float* maxima,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_step_factors, dim3(1), dim3(256), 0, s, partial_max, blocks, fields, step, factors, maxima);
+}
+
+__global__ __launch_bounds__(256) void k_scale_by(float* __restrict__ x, long long count, const float* __restrict__ factor) {
+  const float f = *factor;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x)
+    x[i] *= f;
+}
+
 __global__ __launch_bounds__(256) void k_abs_max(const float* __restrict__ x, long long n, float* partial) {
   float m = 0.0f;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -373,6 +401,10 @@ __global__ __launch_bounds__(256) void k_axpy(float* __restrict__ acc, const flo
 static int stream_blocks(long long count) {
   long long b = (count + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+void launch_scale_by(float* x, long long count, const float* factor, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_by, dim3(stream_blocks(count)), dim3(256), 0, st, x, count, factor);
 }
 
 void launch_scale(float* x, long long count, float s, hipStream_t st) {
@@ -816,23 +848,25 @@ void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const floa
 
 // ------------------------------------------------------------------------------------------
 // Inversion (field_ops.rs:315-392, ITK InvertDisplacementFieldImageFilter): one launch per
-// iteration. Iteration i runs only if state->stop[i & 1] is clear. Every block first reduces the
-// residual partials of psi_i that the previous launch left (the same fixed-order reduction in
-// every block, so all blocks agree on mean_i and max_i without a grid barrier or a device-scope
-// fence; a release at device scope would write back the whole L2 of each XCD). It then applies
-// psi_{i+1} = psi_i + eps * clip(-e_i, eps * max_i), with e_i recomputed at each voxel, and, if
-// another iteration may follow, writes the partials of e_{i+1}. The residual at x gathers only the
-// forward field, never psi at other voxels, so psi is updated in place. Block 0 publishes the
-// statistics and stop[(i + 1) & 1], a word no block of the same launch reads. Each thread takes
-// kInvVoxels voxels per step of its grid-stride loop so their gathers are in flight together.
+// iteration. The residual e_i = psi_i + field(x + psi_i) is kept in a field buffer: the launch
+// for iteration i reads it, applies psi_{i+1} = psi_i + eps * clip(-e_i, eps * max_i), and, if
+// another iteration may follow, gathers the field once at x + psi_{i+1} to store e_{i+1} and
+// write per-block partials of its scaled norms. The residual at x gathers only the forward field,
+// never psi at other voxels, so psi and e are updated in place.
+// Iteration i runs only if state->stop[i & 1] is clear. Every block first reduces the partials
+// that the previous launch left (the same fixed-order reduction in every block, so all blocks
+// agree on mean_i and max_i without a grid barrier or a device-scope fence; a release at device
+// scope would write back the whole L2 of each XCD). Block 0 publishes the statistics and
+// stop[(i + 1) & 1], a word no block of the same launch reads. Each thread takes kInvVoxels voxels
+// per step of its grid-stride loop so their gathers are in flight together.
 // ------------------------------------------------------------------------------------------
 constexpr int kInvThreads = 256;
 constexpr int kInvVoxels = 4;
-constexpr int kInvMaxBlocks = 1024;
+constexpr int kInvMaxBlocks = 2048;
 
-__device__ __forceinline__ void inv_residual_at(const float* __restrict__ field, const Dims& d, const float* l,
-                                                float3 inv_sp, int i, int j, int k, float wx, float wy, float wz,
-                                                float& ex, float& ey, float& ez, float& nrm) {
+__device__ __forceinline__ void inv_residual_at(const float* __restrict__ field, const Dims& d, const float* l, int i,
+                                                int j, int k, float wx, float wy, float wz, float& ex, float& ey,
+                                                float& ez) {
   const float cx = (float)i + (l[0] * wx + l[1] * wy + l[2] * wz);
   const float cy = (float)j + (l[3] * wx + l[4] * wy + l[5] * wz);
   const float cz = (float)k + (l[6] * wx + l[7] * wy + l[8] * wz);
@@ -846,8 +880,11 @@ __device__ __forceinline__ void inv_residual_at(const float* __restrict__ field,
     ey += uy;
     ez += uz;
   }
+}
+
+__device__ __forceinline__ float scaled_norm(float3 inv_sp, float ex, float ey, float ez) {
   const float px = ex * inv_sp.x, py = ey * inv_sp.y, pz = ez * inv_sp.z;
-  nrm = sqrtf(px * px + py * py + pz * pz);
+  return sqrtf(px * px + py * py + pz * pz);
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
@@ -874,8 +911,8 @@ __device__ __forceinline__ void inv_coords(const Dims& d, long long g, int& i, i
 }
 
 __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restrict__ field, const float* __restrict__ psi,
-                                                           Dims d, L9 l, float3 inv_sp, InvertState* state,
-                                                           float* partials, int max_iterations) {
+                                                           float* __restrict__ res, Dims d, L9 l, float3 inv_sp,
+                                                           InvertState* state, float* partials, int max_iterations) {
   const long long n = d.n();
   const long long span = (long long)gridDim.x * kInvThreads;
   float s = 0.0f, c = 0.0f, m = 0.0f;
@@ -886,8 +923,12 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
       if (g < n) {
         int i, j, k;
         inv_coords(d, g, i, j, k);
-        float ex, ey, ez, nrm;
-        inv_residual_at(field, d, l.v, inv_sp, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez, nrm);
+        float ex, ey, ez;
+        inv_residual_at(field, d, l.v, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez);
+        res[g] = ex;
+        res[g + n] = ey;
+        res[g + 2 * n] = ez;
+        const float nrm = scaled_norm(inv_sp, ex, ey, ez);
         m = nan_max(m, nrm);
         kahan_add(s, c, nrm);
       }
@@ -903,10 +944,11 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
   }
 }
 
-__global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi, Dims d,
-                                                          L9 l, float3 inv_sp, InvertState* state, float* partials,
-                                                          int step, int max_iterations, float max_tol, float mean_tol,
-                                                          int enforce, uint32_t* mailbox, uint32_t tag) {
+__global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi,
+                                                          float* __restrict__ res, Dims d, L9 l, float3 inv_sp,
+                                                          InvertState* state, float* partials, int step,
+                                                          int max_iterations, float max_tol, float mean_tol, int enforce,
+                                                          uint32_t* mailbox, uint32_t tag) {
   if (state->stop[step & 1]) {
     // once stopped, stay stopped: the next launch reads the other word
     if (blockIdx.x == 0 && threadIdx.x == 0) state->stop[(step + 1) & 1] = 1;
@@ -956,8 +998,8 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
         int i, j, k;
         inv_coords(d, g, i, j, k);
         const float wx = psi[g], wy = psi[g + n], wz = psi[g + 2 * n];
-        float ex, ey, ez, nrm;
-        inv_residual_at(field, d, l.v, inv_sp, i, j, k, wx, wy, wz, ex, ey, ez, nrm);
+        float ex = res[g], ey = res[g + n], ez = res[g + 2 * n];
+        const float nrm = scaled_norm(inv_sp, ex, ey, ez);
         float ux = -ex, uy = -ey, uz = -ez;
         if (nrm > limit) {
           const float f = limit / nrm;
@@ -971,9 +1013,13 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
         psi[g + n] = ny;
         psi[g + 2 * n] = nz;
         if (next) {
-          inv_residual_at(field, d, l.v, inv_sp, i, j, k, nx, ny, nz, ex, ey, ez, nrm);
-          m = nan_max(m, nrm);
-          kahan_add(s, c, nrm);
+          inv_residual_at(field, d, l.v, i, j, k, nx, ny, nz, ex, ey, ez);
+          res[g] = ex;
+          res[g + n] = ey;
+          res[g + 2 * n] = ez;
+          const float r = scaled_norm(inv_sp, ex, ey, ez);
+          m = nan_max(m, r);
+          kahan_add(s, c, r);
         }
       }
     }
@@ -986,20 +1032,20 @@ int inv_blocks(const Dims& d) {
   return (int)(b < 1 ? 1 : (b > kInvMaxBlocks ? kInvMaxBlocks : b));
 }
 
-void launch_inv_begin(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
+void launch_inv_begin(const float* field, const float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
                       InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  hipLaunchKernelGGL(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, d, lv, inv_spacing, state,
+  hipLaunchKernelGGL(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
                      partials, max_iterations);
 }
 
-void launch_inv_iter(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing, InvertState* state,
-                     float* partials, int blocks, int step, int max_iterations, double max_tol, double mean_tol,
-                     int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s) {
+void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
+                     InvertState* state, float* partials, int blocks, int step, int max_iterations, double max_tol,
+                     double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  hipLaunchKernelGGL(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, d, lv, inv_spacing, state,
+  hipLaunchKernelGGL(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
                      partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
 }
 

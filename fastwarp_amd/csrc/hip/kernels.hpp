@@ -63,6 +63,12 @@ void launch_sample_values(const float* img, Dims d, const float4* idx, long long
 
 // --- elementwise ---
 void launch_scale(float* x, long long count, float s, hipStream_t st);
+// x *= *factor, the factor read on the device.
+void launch_scale_by(float* x, long long count, const float* factor, hipStream_t st);
+// Per field f < fields: maxima[f] = max of partial_max[f * blocks ...] (NaN propagates),
+// factors[f] = step / maxima[f] (step when not positive). One block.
+void launch_step_factors(const float* partial_max, int blocks, int fields, double step, float* factors, float* maxima,
+                         hipStream_t s);
 void launch_axpy(float* acc, const float* x, long long count, float a, hipStream_t st);
 
 // --- warp / resample ---
@@ -114,14 +120,15 @@ struct InvertState {
   int32_t stop[2];  // stop[i & 1] set: iteration i does not run
   int32_t pad;
 };
-// Residual partials of the initial estimate (2 * blocks floats: sums, then maxima) and state reset.
-void launch_inv_begin(const float* field, const float* psi, Dims d, const float l[9], float3 inv_spacing,
+// The residual of the initial estimate into `residual` (a field), its partials (2 * blocks floats:
+// sums, then maxima) and the state reset.
+void launch_inv_begin(const float* field, const float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
                       InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s);
 // One iteration; `partials` holds two such buffers (4 * blocks floats), alternating by step parity.
 // The launch that ends the loop stores `tag` to `mailbox` (host memory) as a hint.
-void launch_inv_iter(const float* field, float* psi, Dims d, const float l[9], float3 inv_spacing, InvertState* state,
-                     float* partials, int blocks, int step, int max_iterations, double max_tol, double mean_tol,
-                     int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s);
+void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
+                     InvertState* state, float* partials, int blocks, int step, int max_iterations, double max_tol,
+                     double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s);
 int inv_blocks(const Dims& d);
 
 // --- recursive Gaussian (Deriche, 4th order) along one axis, constant extension at both ends ---

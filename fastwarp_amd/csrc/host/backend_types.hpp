@@ -2,9 +2,36 @@
 // (crates/fastwarp-reg/src/field_ops.rs:276-307) and a metric evaluated in both directions.
 #pragma once
 
+#include <array>
 #include <cstdint>
+#include <functional>
+#include <utility>
 
 namespace fw {
+
+// A result a backend reads back from the device later: get() is valid once the backend has
+// settled (settle() waits for the device and for every read-back issued since the last settle).
+// Backends that compute on the host hand out results that are ready at once.
+template <typename T>
+class Pending {
+ public:
+  Pending() = default;
+  explicit Pending(T value) : value_(std::move(value)), ready_(true) {}
+  explicit Pending(std::function<T()> read) : read_(std::move(read)) {}
+  T get() {
+    if (!ready_) {
+      value_ = read_();
+      ready_ = true;
+      read_ = nullptr;
+    }
+    return value_;
+  }
+
+ private:
+  T value_{};
+  bool ready_ = false;
+  std::function<T()> read_;
+};
 
 // InvertParams::SYN by default (field_ops.rs:290-295).
 struct InvertParams {
@@ -26,6 +53,8 @@ struct MetricPair {
   double values[2] = {0, 0};
   F fields[2];
   double norms[2] = {0, 0};
+  // Set instead of `values` by the deferred evaluation: both values, once the backend settled.
+  Pending<std::array<double, 2>> later;
 };
 
 }  // namespace fw

@@ -157,60 +157,75 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel
       check_cancel(cancel);
       ++current;
       const auto t_it = std::chrono::steady_clock::now();
+      // The iteration is issued without waiting on the device: the metric values, the update
+      // maxima and the inversion statistics are read back once, at the settle below, where the
+      // reference's finiteness checks run on them (same errors, after the device work).
       // 1. resample every channel into the middle (syn.rs:895-911)
       detail::Side<B> fs, ms;
+      const bool single = nc == 1;
+      std::vector<Pending<std::array<double, 2>>> later(nc);
       for (size_t c = 0; c < nc; ++c) {
         ImageH fmid = b.warp_through_field(*smooth[c].fixed, *f2m_inv, nullptr, nullptr);
         ImageH mmid = b.warp_through_field(*smooth[c].moving, *m2m_inv, nullptr, &initial);
-        // 2. the metric both ways (syn.rs:337-420 `side_updates`)
-        const bool single = nc == 1;
-        MetricPair<FieldH> pair = recipe.metric(c) == SynMetric::Cc
-                                      ? b.cc_pair(fmid, mmid, nullptr, recipe.cc_radius, !single)
-                                      : b.mi_pair(fmid, mmid, nullptr, nullptr, recipe.mi_bins, !single);
-        fs.add(b, normalised[c], pair.values[0], std::move(pair.fields[0]), single ? nullptr : &pair.norms[0]);
-        ms.add(b, normalised[c], pair.values[1], std::move(pair.fields[1]), single ? nullptr : &pair.norms[1]);
+        // 2. the metric both ways (syn.rs:337-420 `side_updates`); several channels need the
+        // derivative norms for their weights at once, a single one needs its values only later
+        MetricPair<FieldH> pair;
+        if (recipe.metric(c) == SynMetric::Cc && single) {
+          pair = b.cc_pair_deferred(fmid, mmid, recipe.cc_radius);
+          later[c] = std::move(pair.later);
+        } else {
+          pair = recipe.metric(c) == SynMetric::Cc ? b.cc_pair(fmid, mmid, nullptr, recipe.cc_radius, !single)
+                                                   : b.mi_pair(fmid, mmid, nullptr, nullptr, recipe.mi_bins, !single);
+          later[c] = Pending<std::array<double, 2>>(std::array<double, 2>{pair.values[0], pair.values[1]});
+        }
+        fs.add(b, normalised[c], 0.0, std::move(pair.fields[0]), single ? nullptr : &pair.norms[0]);
+        ms.add(b, normalised[c], 0.0, std::move(pair.fields[1]), single ? nullptr : &pair.norms[1]);
       }
-      double weighted[2];
       FieldH grads[2];
       detail::Side<B>* sides[2] = {&fs, &ms};
       for (int s = 0; s < 2; ++s) {
         detail::Side<B>& side = *sides[s];
         if (!side.gradient) invalid("syn", "no channel produced a gradient");
-        if (nc == 1) {
-          weighted[s] = side.values[0];
-        } else {
-          b.scale(*side.gradient, side.total_magnitude / double(nc));
-          weighted[s] = 0.0;
-          for (size_t c = 0; c < nc; ++c) weighted[s] += side.values[c] * normalised[c];
-        }
+        if (nc > 1) b.scale(*side.gradient, side.total_magnitude / double(nc));
         grads[s] = b.smooth_field(std::move(*side.gradient), recipe.update_field_variance);
       }
       // 3. scale each update so its largest voxel step is the gradient step
-      auto [fmax, mmax] = b.max_scaled_norm_pair(grads[0], &grads[1]);
-      if (!(std::isfinite(fmax) && std::isfinite(mmax))) invalid("syn", "the metric gradient is not finite");
-      double scales[2] = {recipe.gradient_step, recipe.gradient_step};
-      if (fmax > 0.0) scales[0] /= fmax;
-      if (mmax > 0.0) scales[1] /= mmax;
-      b.scale(grads[0], scales[0]);
-      b.scale(grads[1], scales[1]);
+      Pending<std::pair<double, double>> maxima = b.scale_to_step(grads[0], grads[1], recipe.gradient_step);
       // 4-5. compose into the total fields and re-estimate both inverses (syn.rs:422-445)
       SynIteration it;
       std::optional<FieldH>* fields[2] = {&f2m, &m2m};
       std::optional<FieldH>* inverses[2] = {&f2m_inv, &m2m_inv};
+      Pending<InvertStats> stats[4];
       for (int s = 0; s < 2; ++s) {
         FieldH composed = b.compose(grads[s], **fields[s]);
         fields[s]->reset();
         FieldH tmp = b.smooth_field(std::move(composed), recipe.total_field_variance);
-        auto inv = b.invert(tmp, std::move(**inverses[s]), inv_params);
-        auto fwd = b.invert(inv.first, std::move(tmp), inv_params);
-        for (const InvertStats& st : {inv.second, fwd.second})
-          if (!(std::isfinite(st.mean_error_norm) && std::isfinite(st.max_error_norm)))
-            invalid("syn", "the displacement field is not finite");
+        auto inv = b.invert_deferred(tmp, std::move(**inverses[s]), inv_params);
+        auto fwd = b.invert_deferred(inv.first, std::move(tmp), inv_params);
         *inverses[s] = std::move(inv.first);
         *fields[s] = std::move(fwd.first);
-        it.inversions[2 * s] = inv.second;
-        it.inversions[2 * s + 1] = fwd.second;
+        stats[2 * s] = std::move(inv.second);
+        stats[2 * s + 1] = std::move(fwd.second);
       }
+      b.settle();
+      const auto [fmax, mmax] = maxima.get();
+      if (!(std::isfinite(fmax) && std::isfinite(mmax))) invalid("syn", "the metric gradient is not finite");
+      for (int q = 0; q < 4; ++q) {
+        it.inversions[q] = stats[q].get();
+        if (!(std::isfinite(it.inversions[q].mean_error_norm) && std::isfinite(it.inversions[q].max_error_norm)))
+          invalid("syn", "the displacement field is not finite");
+      }
+      double weighted[2] = {0.0, 0.0};
+      for (size_t c = 0; c < nc; ++c) {
+        const std::array<double, 2> v = later[c].get();
+        fs.values[c] = v[0];
+        ms.values[c] = v[1];
+        weighted[0] += single ? v[0] : v[0] * normalised[c];
+        weighted[1] += single ? v[1] : v[1] * normalised[c];
+      }
+      double scales[2] = {recipe.gradient_step, recipe.gradient_step};
+      if (fmax > 0.0) scales[0] /= fmax;
+      if (mmax > 0.0) scales[1] /= mmax;
       // 6. energy of the first metric into the monitor (syn.rs:941-948)
       it.energy = 0.5 * (fs.values[0] + ms.values[0]);
       it.energy_weighted = 0.5 * (weighted[0] + weighted[1]);

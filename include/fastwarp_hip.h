@@ -245,6 +245,27 @@ int fw_set_profiling(fw_ctx* ctx, int on);
  * (NUL-terminated, at most `capacity` bytes) and clears the totals. */
 int fw_take_profile(fw_ctx* ctx, char* text, size_t capacity);
 
+/* ---- admission: device memory of one registration (the reference's capacity model,
+ *      crates/fastwarp-gpu/src/capacity.rs:162-180, 511-715, and fits_gpu /
+ *      registrations_at_once, crates/fastwarp-py/src/lib.rs:322-380, for this engine's buffers) ---- */
+typedef struct {
+  uint64_t fixed_voxels, moving_voxels;
+  uint64_t smoothing_cache_bytes; /* both images of every channel at every distinct sigma */
+  uint64_t linear_bytes;          /* peak of the linear chain (0 when it does not run) */
+  uint64_t syn_bytes;             /* peak of the SyN stage (0 when it does not run) */
+  uint64_t peak_bytes;            /* what one registration needs: the larger peak plus overhead */
+  uint64_t largest_buffer_bytes;  /* the largest single allocation */
+} fw_capacity;
+/* Pure, no device: FW_ERR_INVALID for a bad plan or grid. */
+int fw_plan_capacity(const fw_plan* plan, const fw_grid* fixed_grid, const fw_grid* moving_grid, fw_capacity* out);
+/* Free device memory of the context's device less 1 GiB kept in reserve, or the environment
+ * override FASTWARP_GPU_MEMORY_BUDGET_MB (capacity.rs:162-180). */
+int fw_memory_budget(fw_ctx* ctx, uint64_t* bytes);
+/* Whether one registration of `plan` fits the budget now; how many fit at once (at most `max`). */
+int fw_fits(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const fw_grid* moving_grid, int32_t* fits);
+int fw_registrations_at_once(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const fw_grid* moving_grid,
+                             uint32_t max, uint32_t* count);
+
 /* Fills `plan` with the cohort recipe: Rigid/Affine MI 32 bins, 25 % regular sampling,
  * 1000x500x250x100, shrink 8/4/2/1, sigmas 3/2/1/0 vox; SyN[0.1,3,0], CC radius 4,
  * 100x70x50x20, shrink 8/4/2/1, sigmas 3/2/1/0 vox (linear.rs:112-128, recipe.rs:93-112). */

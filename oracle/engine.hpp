@@ -86,6 +86,26 @@ class CpuBackend {
   std::pair<double, double> max_scaled_norm_pair(const VectorField& a, const VectorField* b) {
     return {max_scaled_norm(a), b ? max_scaled_norm(*b) : 0.0};
   }
+  // The deferred forms of the SyN driver: computed at once on the host.
+  MetricPair<VectorField> cc_pair_deferred(const Image& f, const Image& m, size_t radius) {
+    MetricPair<VectorField> out = cc_pair(f, m, nullptr, radius, false);
+    out.later = Pending<std::array<double, 2>>(std::array<double, 2>{out.values[0], out.values[1]});
+    return out;
+  }
+  Pending<std::pair<double, double>> scale_to_step(VectorField& a, VectorField& b, double step) {
+    const auto m = max_scaled_norm_pair(a, &b);
+    if (std::isfinite(m.first) && std::isfinite(m.second)) {
+      scale(a, m.first > 0.0 ? step / m.first : step);
+      scale(b, m.second > 0.0 ? step / m.second : step);
+    }
+    return Pending<std::pair<double, double>>(m);
+  }
+  std::pair<VectorField, Pending<fw::InvertStats>> invert_deferred(const VectorField& field, VectorField initial,
+                                                                  const fw::InvertParams& p) {
+    auto r = invert(field, std::move(initial), p);
+    return {std::move(r.first), Pending<fw::InvertStats>(r.second)};
+  }
+  void settle() {}
   VectorField compose(const VectorField& u, const VectorField& w) { return oracle::compose(u, w); }
   std::pair<VectorField, fw::InvertStats> invert(const VectorField& field, VectorField initial, const fw::InvertParams& p) {
     oracle::InvertParams q{p.max_iterations, p.mean_error_tolerance, p.max_error_tolerance, p.enforce_boundary_condition};
