@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
+#include <thread>
 
 #include "../host/gauss_taps.hpp"
 #include "geometry.hpp"
@@ -714,8 +716,10 @@ HField HipBackend::smooth_field(HField field, double variance) {
     }
     if (fits) {
       HField out{g, DevBuf(ctx_.get(), 3 * n)};
-      Timed tm(*ctx_, "k_gauss3", 24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0));
-      if (launch_gauss3(field.buf.get(), out.buf.get(), d, t, float(w1), float(w2), ctx_->stream())) return out;
+      DevBuf tmp(ctx_.get(), 3 * n);
+      // x/y pass: read u, write the intermediate; z pass: read it (and u for a blend), write
+      Timed tm(*ctx_, "k_gauss_xy_z", 48.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0));
+      if (launch_gauss3(field.buf.get(), tmp.get(), out.buf.get(), d, t, float(w1), float(w2), ctx_->stream())) return out;
     }
   }
   DevBuf t1(ctx_.get(), 3 * n), t2(ctx_.get(), 3 * n);
@@ -835,8 +839,23 @@ std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField
     Timed t(*ctx_, "k_inv_begin", 36.0 * double(d.n()));
     launch_inv_begin(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, partials, nb, max_it, s);
   }
+  // At most kAhead launches are queued beyond the one the device has started: the host waits on
+  // the progress word (a hint; a bounded wait) so that it does not queue launches past the stop.
+  constexpr int kAhead = 3;
+  const auto started = [&]() -> int {
+    const uint32_t w = mail[1];
+    return (w >> 8) == (tag & 0xffffffu) ? int(w & 0xffu) : 0;
+  };
   for (int step = 0; step < max_it; ++step) {
-    if (step > 0 && *mail == tag) break;
+    if (step > 0 && mail[0] == tag) break;
+    if (step >= kAhead) {
+      const auto t0 = std::chrono::steady_clock::now();
+      while (started() < step - kAhead + 1 && mail[0] != tag) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
+        std::this_thread::yield();
+      }
+      if (mail[0] == tag) break;
+    }
     Timed t(*ctx_, "k_inv_iter", 0.0);
     launch_inv_iter(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, partials, nb, step, max_it,
                     p.max_error_tolerance, p.mean_error_tolerance, p.enforce_boundary_condition ? 1 : 0, mail_dev, tag, s);

@@ -542,167 +542,156 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
 }
 
 // ------------------------------------------------------------------------------------------
-// Fused separable Gaussian of a field (field_ops.rs:508-574), one pass over HBM: a block owns a
-// 64 x 16 (x, y) tile of one component over a run of z planes. Each input plane (edge-clamped) is
-// staged in LDS with its x/y halo, smoothed along x then y (runs of 4 outputs per thread, so each
-// LDS value is read once per run), and the y-smoothed plane enters a ring of 2rz+1 planes held in
-// registers; the z tap sum of the ring is the output plane rz behind. The arithmetic is the
-// three-pass filter's, in the same order (x, then y, then z, each an fma chain over the taps in
-// order), so the result is bit-identical to it. The final blend w1 * s + w2 * u and the zeroed
+// Separable Gaussian of a field (field_ops.rs:508-574) in two passes over HBM.
+// k_gauss_xy: one block per 64 x 16 (x, y) tile of one plane of one component; the plane tile is
+// staged in LDS with its edge-clamped x/y halo, smoothed along x (runs of 4 outputs per thread,
+// so each LDS value is read once per run) and then along y (4 rows per thread).
+// k_gauss_z: one thread per (x, y) column of one component over a run of planes; the z window
+// slides through a ring of 2rz+1 registers, and the final blend w1 * s + w2 * u and the zeroed
 // faces are applied on the way out.
+// The arithmetic is the three-pass filter's in the same order (x, then y, then z, each an fma
+// chain over the taps in order), so the result is bit-identical to it.
 // ------------------------------------------------------------------------------------------
 constexpr int kG3TX = 64, kG3TY = 16, kG3MaxR = 8;
 
-template <int RZ>
-__global__ __launch_bounds__(256) void k_gauss3(const float* __restrict__ in, float* __restrict__ out, Dims d,
-                                                GaussTaps3 t, int zrun, float w1, float w2) {
-  constexpr int W = 2 * RZ + 1;
+__global__ __launch_bounds__(256) void k_gauss_xy(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                  GaussTaps3 t) {
   __shared__ float tile[kG3TY + 2 * kG3MaxR][kG3TX + 2 * kG3MaxR];
   __shared__ float xs[kG3TY + 2 * kG3MaxR][kG3TX];
   const int rx = t.rx, ry = t.ry;
   const int LX = kG3TX + 2 * rx, LY = kG3TY + 2 * ry;
-  const int comp = blockIdx.z % 3;
-  const int chunk = blockIdx.z / 3;
+  const int comp = blockIdx.z / d.nz, z = blockIdx.z - comp * d.nz;
   const int x0 = blockIdx.x * kG3TX, y0 = blockIdx.y * kG3TY;
-  const int z0 = chunk * zrun;
-  const int z1 = min(z0 + zrun, d.nz);
   const long long n = d.n();
-  const long long plane = (long long)d.nx * d.ny;
-  const float* src = in + comp * n;
-  float* dst = out + comp * n;
+  const float* src = in + comp * n + (long long)d.nx * d.ny * z;
+  float* dst = out + comp * n + (long long)d.nx * d.ny * z;
   const int tid = threadIdx.x;
-  const int tx = tid & 63, ty = (tid >> 6) * 4;  // y-pass: column tx, rows ty..ty+3
-  float ring[W][4];
-#pragma unroll
-  for (int q = 0; q < W; ++q)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) ring[q][r] = 0.0f;
-  // This thread's tile elements (e = tid + 256 q) and their edge-clamped offsets within a plane,
-  // the same for every plane; the next plane is fetched into registers while this one is filtered.
-  constexpr int kPer = ((kG3TX + 2 * kG3MaxR) * (kG3TY + 2 * kG3MaxR) + 255) / 256;
-  int off[kPer], slot[kPer];
-#pragma unroll
-  for (int q = 0; q < kPer; ++q) {
-    const int e = tid + 256 * q;
+  for (int e = tid; e < LX * LY; e += 256) {
     const int ly = e / LX, lx = e - ly * LX;
     int gx = x0 - rx + lx, gy = y0 - ry + ly;
     gx = gx < 0 ? 0 : (gx > d.nx - 1 ? d.nx - 1 : gx);
     gy = gy < 0 ? 0 : (gy > d.ny - 1 ? d.ny - 1 : gy);
-    off[q] = e < LX * LY ? gx + d.nx * gy : -1;
-    slot[q] = ly * (kG3TX + 2 * kG3MaxR) + lx;
+    tile[ly][lx] = src[gx + d.nx * gy];
   }
-  float pre[kPer];
-  auto fetch = [&](int zin) {
-    const int zc = zin < 0 ? 0 : (zin > d.nz - 1 ? d.nz - 1 : zin);
-    const float* p = src + plane * zc;
-#pragma unroll
-    for (int q = 0; q < kPer; ++q)
-      if (off[q] >= 0) pre[q] = p[off[q]];
-  };
-  fetch(z0 - RZ);
-
-  for (int zin = z0 - RZ; zin < z1 + RZ; ++zin) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kPer; ++q)
-      if (off[q] >= 0) (&tile[0][0])[slot[q]] = pre[q];
-    __syncthreads();
-    if (zin + 1 < z1 + RZ) fetch(zin + 1);
-    // x pass: runs of 4 outputs along x
-    for (int e = tid; e < LY * (kG3TX / 4); e += 256) {
-      const int ly = e / (kG3TX / 4), lx = (e - ly * (kG3TX / 4)) * 4;
-      float v[4 + 2 * kG3MaxR];
-#pragma unroll
-      for (int q = 0; q < 4 + 2 * kG3MaxR; ++q)
-        if (q < 4 + 2 * rx) v[q] = tile[ly][lx + q];
-      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-#pragma unroll
-      for (int k = 0; k <= 2 * kG3MaxR; ++k) {
-        if (k <= 2 * rx) {
-          const float w = t.x[k];
-          a0 = fmaf(w, v[k], a0);
-          a1 = fmaf(w, v[k + 1], a1);
-          a2 = fmaf(w, v[k + 2], a2);
-          a3 = fmaf(w, v[k + 3], a3);
-        }
-      }
-      xs[ly][lx] = a0;
-      xs[ly][lx + 1] = a1;
-      xs[ly][lx + 2] = a2;
-      xs[ly][lx + 3] = a3;
-    }
-    __syncthreads();
-    // y pass: rows ty..ty+3 of column tx
+  __syncthreads();
+  for (int e = tid; e < LY * (kG3TX / 4); e += 256) {
+    const int ly = e / (kG3TX / 4), lx = (e - ly * (kG3TX / 4)) * 4;
     float v[4 + 2 * kG3MaxR];
 #pragma unroll
     for (int q = 0; q < 4 + 2 * kG3MaxR; ++q)
-      if (q < 4 + 2 * ry) v[q] = xs[ty + q][tx];
-    float b[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (q < 4 + 2 * rx) v[q] = tile[ly][lx + q];
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
 #pragma unroll
     for (int k = 0; k <= 2 * kG3MaxR; ++k) {
-      if (k <= 2 * ry) {
-        const float w = t.y[k];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) b[r] = fmaf(w, v[k + r], b[r]);
+      if (k <= 2 * rx) {
+        const float w = t.x[k];
+        a0 = fmaf(w, v[k], a0);
+        a1 = fmaf(w, v[k + 1], a1);
+        a2 = fmaf(w, v[k + 2], a2);
+        a3 = fmaf(w, v[k + 3], a3);
       }
     }
+    xs[ly][lx] = a0;
+    xs[ly][lx + 1] = a1;
+    xs[ly][lx + 2] = a2;
+    xs[ly][lx + 3] = a3;
+  }
+  __syncthreads();
+  const int tx = tid & 63, ty = (tid >> 6) * 4;
+  float v[4 + 2 * kG3MaxR];
 #pragma unroll
-    for (int q = 0; q < W - 1; ++q)
+  for (int q = 0; q < 4 + 2 * kG3MaxR; ++q)
+    if (q < 4 + 2 * ry) v[q] = xs[ty + q][tx];
+  float b[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ring[q][r] = ring[q + 1][r];
+  for (int k = 0; k <= 2 * kG3MaxR; ++k) {
+    if (k <= 2 * ry) {
+      const float w = t.y[k];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) ring[W - 1][r] = b[r];
-    const int k = zin - RZ;
-    if (k >= z0) {
-      const int x = x0 + tx;
-      if (x < d.nx) {
+      for (int r = 0; r < 4; ++r) b[r] = fmaf(w, v[k + r], b[r]);
+    }
+  }
+  const int x = x0 + tx;
+  if (x < d.nx) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int y = y0 + ty + r;
-          if (y >= d.ny) continue;
-          const long long g = (long long)x + (long long)d.nx * y + plane * k;
-          float o = 0.0f;
-          if (!(x == 0 || y == 0 || k == 0 || x == d.nx - 1 || y == d.ny - 1 || k == d.nz - 1)) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int q = 0; q < W; ++q) acc = fmaf(t.z[q], ring[q][r], acc);
-            o = w2 != 0.0f ? acc * w1 + src[g] * w2 : acc * w1;
-          }
-          dst[g] = o;
-        }
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int y = y0 + ty + r;
+      if (y < d.ny) dst[x + d.nx * y] = b[r];
     }
   }
 }
 
-// The z-run per block: about 512 blocks (two per CU), runs at least 4 * rz planes long, so the
-// 2 * rz halo planes each run re-reads stay a minority of its input.
-static int g3_zrun(const Dims& d, int rz) {
-  const long long tiles = (long long)((d.nx + kG3TX - 1) / kG3TX) * ((d.ny + kG3TY - 1) / kG3TY) * 3;
-  long long chunks = (512 + tiles - 1) / tiles;
+template <int RZ>
+__global__ __launch_bounds__(256) void k_gauss_z(const float* __restrict__ in, const float* __restrict__ orig,
+                                                 float* __restrict__ out, Dims d, GaussTaps3 t, int zrun, float w1,
+                                                 float w2) {
+  constexpr int W = 2 * RZ + 1;
+  const int comp = blockIdx.z % 3, chunk = blockIdx.z / 3;
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= d.nx || y >= d.ny) return;
+  const int z0 = chunk * zrun, z1 = min(z0 + zrun, d.nz);
+  const long long n = d.n(), plane = (long long)d.nx * d.ny;
+  const long long col = (long long)x + (long long)d.nx * y;
+  const float* src = in + comp * n + col;
+  const float* org = orig + comp * n + col;
+  float* dst = out + comp * n + col;
+  const bool edge_xy = x == 0 || y == 0 || x == d.nx - 1 || y == d.ny - 1;
+  float ring[W];
+#pragma unroll
+  for (int q = 0; q < W; ++q) ring[q] = 0.0f;
+  auto plane_of = [&](int zin) { return zin < 0 ? 0 : (zin > d.nz - 1 ? d.nz - 1 : zin); };
+  float next = src[plane * plane_of(z0 - RZ)];
+  for (int zin = z0 - RZ; zin < z1 + RZ; ++zin) {
+    const float cur = next;
+    if (zin + 1 < z1 + RZ) next = src[plane * plane_of(zin + 1)];
+#pragma unroll
+    for (int q = 0; q < W - 1; ++q) ring[q] = ring[q + 1];
+    ring[W - 1] = cur;
+    const int k = zin - RZ;
+    if (k >= z0) {
+      float o = 0.0f;
+      if (!(edge_xy || k == 0 || k == d.nz - 1)) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < W; ++q) acc = fmaf(t.z[q], ring[q], acc);
+        o = w2 != 0.0f ? acc * w1 + org[plane * k] * w2 : acc * w1;
+      }
+      dst[plane * k] = o;
+    }
+  }
+}
+
+// The z-run per block: about 2048 blocks, runs at least 4 * rz planes long, so the 2 * rz halo
+// planes each run re-reads stay a minority of its input.
+static int gz_zrun(const Dims& d, int rz) {
+  const long long cols = (long long)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * 3;
+  const long long chunks = (2048 + cols - 1) / cols;
   int run = (int)((d.nz + chunks - 1) / chunks);
   const int least = 4 * rz > 8 ? 4 * rz : 8;
   return run < least ? least : run;
 }
 
-bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s) {
-  if (t.rx > kG3MaxR || t.ry > kG3MaxR || t.rz > kG3MaxR || t.rz < 1) return false;
-  const int run = g3_zrun(d, t.rz);
-  const dim3 grid((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * ((d.nz + run - 1) / run));
+bool launch_gauss3(const float* in, float* tmp, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s) {
+  if (t.rx > kG3MaxR || t.ry > kG3MaxR || t.rz > kG3MaxR || t.rz < 1 || 3LL * d.nz > 65535) return false;
+  const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * d.nz);
+  hipLaunchKernelGGL(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t);
+  const int run = gz_zrun(d, t.rz);
+  const dim3 gz((d.nx + 63) / 64, (d.ny + 3) / 4, 3 * ((d.nz + run - 1) / run));
   switch (t.rz) {
-#define FW_G3_CASE(R)                                                                       \
-  case R:                                                                                   \
-    hipLaunchKernelGGL(k_gauss3<R>, grid, dim3(256), 0, s, in, out, d, t, run, w1, w2);     \
+#define FW_GZ_CASE(R)                                                                              \
+  case R:                                                                                          \
+    hipLaunchKernelGGL(k_gauss_z<R>, gz, dim3(256), 0, s, tmp, in, out, d, t, run, w1, w2);        \
     break;
-    FW_G3_CASE(1)
-    FW_G3_CASE(2)
-    FW_G3_CASE(3)
-    FW_G3_CASE(4)
-    FW_G3_CASE(5)
-    FW_G3_CASE(6)
-    FW_G3_CASE(7)
-    FW_G3_CASE(8)
-#undef FW_G3_CASE
+    FW_GZ_CASE(1)
+    FW_GZ_CASE(2)
+    FW_GZ_CASE(3)
+    FW_GZ_CASE(4)
+    FW_GZ_CASE(5)
+    FW_GZ_CASE(6)
+    FW_GZ_CASE(7)
+    FW_GZ_CASE(8)
+#undef FW_GZ_CASE
     default:
       return false;
   }
@@ -955,6 +944,10 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
     return;
   }
   const int blocks = gridDim.x;
+  // progress hint for the host: this inversion has started iteration `step`
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    __hip_atomic_store(mailbox + 1, ((tag & 0xffffffu) << 8) | (uint32_t)(step + 1), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
   const float* in = partials + (step & 1) * 2 * blocks;
   float* out = partials + ((step + 1) & 1) * 2 * blocks;
   float s = 0.0f, c = 0.0f, m = 0.0f;

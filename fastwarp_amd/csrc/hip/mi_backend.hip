@@ -71,13 +71,15 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
   a.bins = int(binning_.bins);
   const size_t bins = binning_.bins;
   const size_t cells = bins * bins + bins + 1;
+  const int blocks = mi_sampled_blocks(int(count_));
   a.hist = static_cast<unsigned long long*>(ctx_->scratch(0, cells * sizeof(unsigned long long)));
+  a.hist_slots = static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * cells * sizeof(unsigned long long)));
   hipStream_t s = ctx_->stream();
-  check(hipMemsetAsync(a.hist, 0, cells * sizeof(unsigned long long), s), "clear histogram");
-  if (count_ > 0) {
+  {
     Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
-    launch_mi_sampled_hist(a, mi_sampled_blocks(int(count_)), s);
+    launch_mi_sampled_hist(a, blocks, s);
   }
+  launch_mi_sum_slots(a.hist_slots, blocks, int(cells), a.hist, s);
   auto* host = static_cast<unsigned long long*>(ctx_->pinned(cells * sizeof(unsigned long long)));
   check(hipMemcpyAsync(host, a.hist, cells * sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "read histogram");
   ctx_->sync();

@@ -84,13 +84,13 @@ int mi_sampled_blocks(int count) {
   return b < 1 ? 1 : (b > 512 ? 512 : b);
 }
 
+// The block's histogram (LDS, 64-bit atomics) is written to its own slot of `hist_slots`; a second
+// kernel adds the slots, so the global histogram needs neither clearing nor global atomics.
 __global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
-  __shared__ unsigned long long h[kMiMaxBins * kMiMaxBins + kMiMaxBins];
-  __shared__ unsigned int valid;
+  extern __shared__ unsigned long long h[];
   const int bins = a.bins;
   const int cells = bins * bins + bins;
-  for (int i = threadIdx.x; i < cells; i += blockDim.x) h[i] = 0ull;
-  if (threadIdx.x == 0) valid = 0u;
+  for (int i = threadIdx.x; i < cells + 1; i += blockDim.x) h[i] = 0ull;
   __syncthreads();
   unsigned int mine = 0;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < a.count; s += gridDim.x * blockDim.x) {
@@ -100,21 +100,35 @@ __global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
     atomicAdd(&h[bins * bins + e.fbin], 1ull);
     unsigned long long* row = h + e.fbin * bins;
 #pragma unroll
-    for (int k = -1; k <= 2; ++k) {
-      const int m = e.mbin + k;
+    for (int q = -1; q <= 2; ++q) {
+      const int m = e.mbin + q;
       const float w = bspline3((float)m - e.term);
       if (w > 0.0f) atomicAdd(&row[m], to_fixed(w));
     }
   }
-  if (mine) atomicAdd(&valid, mine);
+  if (mine) atomicAdd(&h[cells], (unsigned long long)mine);
   __syncthreads();
-  for (int i = threadIdx.x; i < cells; i += blockDim.x)
-    if (h[i]) atomicAdd(&a.hist[i], h[i]);
-  if (threadIdx.x == 0 && valid) atomicAdd(&a.hist[cells], (unsigned long long)valid);
+  unsigned long long* slot = a.hist_slots + (size_t)blockIdx.x * (cells + 1);
+  for (int i = threadIdx.x; i < cells + 1; i += blockDim.x) slot[i] = h[i];
 }
 
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_mi_sampled_hist, dim3(blocks), dim3(256), 0, s, a);
+  const size_t lds = size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
+  hipLaunchKernelGGL(k_mi_sampled_hist, dim3(blocks), dim3(256), lds, s, a);
+}
+
+// hist[c] = sum of the slots' cell c, for every cell and the valid count (one thread per cell).
+__global__ __launch_bounds__(256) void k_mi_sum_slots(const unsigned long long* __restrict__ slots, int nslots,
+                                                      int cells, unsigned long long* __restrict__ hist) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  unsigned long long t = 0ull;
+  for (int b = 0; b < nslots; ++b) t += slots[(size_t)b * cells + c];
+  hist[c] = t;
+}
+
+void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, hipStream_t s) {
+  hipLaunchKernelGGL(k_mi_sum_slots, dim3((cells + 255) / 256), dim3(256), 0, s, slots, nslots, cells, hist);
 }
 
 // Per sample: sum_k B3'(k - term) * ratio[f, k] * (grad M . dT/dtheta_p) for every parameter

@@ -26,6 +26,7 @@ struct MiSampledArgs {
   float moving_min, moving_max;
   int bins;
   unsigned long long* hist;  // bins*bins joint, then bins fixed marginal (counts), then the valid count
+  unsigned long long* hist_slots;  // per block of the histogram pass: one such histogram
   // derivative pass only
   const float* ratio;     // bins*bins weights
   Probes probes;          // central-difference probes of the moving grid
@@ -36,6 +37,8 @@ struct MiSampledArgs {
   double* partial;        // per block: 12 doubles
 };
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s);
+// Adds `nslots` per-block histograms of `cells` u64 entries into `hist`.
+void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, hipStream_t s);
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
 int mi_sampled_blocks(int count);
 
