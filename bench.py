@@ -208,8 +208,11 @@ def main() -> int:
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "phases_s": {"linear": round(last.result.seconds_linear, 3), "syn": round(last.result.seconds_syn, 3),
                      "preprocess": round(last.result.seconds_preprocess, 3)},
+        "linear_levels": [[{"iterations": last.result.linear_iterations[st][l], "evaluations": last.result.linear_evaluations[st][l],
+                            "seconds": round(last.result.linear_level_seconds[st][l], 4)} for l in range(4)] for st in range(2)],
         "syn_levels": [{"iterations": last.result.syn_iterations[l],
-                        "mean_inversion_steps": round(last.result.syn_inversion_steps[l] / max(4 * last.result.syn_iterations[l], 1), 2)}
+                        "mean_inversion_steps": round(last.result.syn_inversion_steps[l] / max(4 * last.result.syn_iterations[l], 1), 2),
+                        "seconds": round(last.result.syn_level_seconds[l], 4)}
                        for l in range(last.result.syn_levels)],
         "host_s_per_step": dict(sorted(host.items(), key=lambda kv: -kv[1])),
         "device_peak_bytes": int(peak),

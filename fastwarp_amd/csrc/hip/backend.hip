@@ -32,6 +32,8 @@ DeviceContext::~DeviceContext() {
     if (s.p) hipFree(s.p);
   if (pinned_) hipHostFree(pinned_);
   if (mailbox_) hipHostFree(mailbox_);
+  for (auto& m : mapped_)
+    if (m.host) hipHostFree(m.host);
   for (auto& c : stage_) hipHostFree(c.p);
   for (auto& kv : taps_) hipFree(kv.second.first);
   for (auto& s : spans_) {
@@ -105,6 +107,21 @@ void* DeviceContext::pinned(size_t bytes) {
     check(hipHostMalloc(&pinned_, pinned_bytes_, hipHostMallocDefault), "hipHostMalloc");
   }
   return pinned_;
+}
+
+void* DeviceContext::mapped(int slot, size_t bytes, void** device) {
+  Mapped& m = mapped_[slot];
+  if (m.bytes < bytes) {
+    if (m.host) {
+      sync();
+      hipHostFree(m.host);
+    }
+    m.bytes = round_bytes(bytes * 2);
+    check(hipHostMalloc(&m.host, m.bytes, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc mapped");
+    check(hipHostGetDevicePointer(&m.dev, m.host, 0), "hipHostGetDevicePointer");
+  }
+  *device = m.dev;
+  return m.host;
 }
 
 void* DeviceContext::stage(size_t bytes) {
@@ -449,9 +466,7 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
     Timed t(*ctx_, "k_cc_pair", (valid ? 60.0 : 56.0) * double(n));
     launch_cc_pair(a, ctx_->stream());
   }
-  // the launched grid may be smaller than `blocks` (wider x tiles): count the real one
-  const int tx = radius <= 5 ? 64 : 32;
-  const size_t real = size_t((a.d.nx + tx - 1) / tx) * ((a.d.ny + 3) / 4) * ((a.d.nz + a.tile_z - 1) / a.tile_z);
+  const size_t real = blocks;
   int nb = 0;
   float* pm = nullptr;
   float* ps = nullptr;
@@ -531,8 +546,7 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
     Timed t(*ctx_, "k_cc_pair", 56.0 * double(n));
     launch_cc_pair(a, ctx_->stream());
   }
-  const int tx = radius <= 5 ? 64 : 32;
-  const size_t real = size_t((a.d.nx + tx - 1) / tx) * ((a.d.ny + 3) / 4) * ((a.d.nz + a.tile_z - 1) / a.tile_z);
+  const size_t real = blocks;
   char* host = static_cast<char*>(ctx_->stage(real * (sizeof(float) + sizeof(int))));
   check(hipMemcpyAsync(host, a.partial_cc, real * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "cc partials");
   check(hipMemcpyAsync(host + real * sizeof(float), a.partial_count, real * sizeof(int), hipMemcpyDeviceToHost,

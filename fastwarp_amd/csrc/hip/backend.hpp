@@ -47,6 +47,9 @@ class DeviceContext : public HostSpanSink {
   void* scratch(int slot, size_t bytes);
   // Pinned host memory for read-backs (one growable buffer).
   void* pinned(size_t bytes);
+  // Coherent host memory that kernels write directly (results the host reads after a sync), one
+  // growable buffer per slot (0..3): the host pointer, and the device pointer in `device`.
+  void* mapped(int slot, size_t bytes, void** device);
   // Pinned host space for a read-back whose value is wanted only at the next settle(): a bump
   // allocation valid until then (chunks are added, never moved, while copies may be in flight).
   void* stage(size_t bytes);
@@ -90,6 +93,12 @@ class DeviceContext : public HostSpanSink {
     size_t capacity = 0, used = 0;
   };
   std::vector<StageChunk> stage_;
+  struct Mapped {
+    void* host = nullptr;
+    void* dev = nullptr;
+    size_t bytes = 0;
+  };
+  Mapped mapped_[4];
   uint32_t* mailbox_ = nullptr;
   uint32_t* mailbox_dev_ = nullptr;
   uint32_t tag_ = 0;

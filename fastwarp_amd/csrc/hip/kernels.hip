@@ -207,17 +207,243 @@ __global__ __launch_bounds__(TX* TY) void k_cc_pair(CcArgs a) {
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// CC pair, second form: the same exact integer window sums, organised to do less integer work per
+// output. A block owns a 64 x 8 (x, y) tile over a run of tile_z planes. Each thread keeps the
+// running z-window sums of its columns of the tile and halo in registers: one plane enters and
+// one leaves per step (exact in integers, so a running sum never drifts), read from global
+// memory (the leaving plane was read 2r+1 steps before and sits in cache). The z sums of the
+// step's plane go to LDS; x sums slide along runs of 4 outputs per thread; y sums slide along
+// pairs of rows per thread; each thread then finishes two output voxels exactly as above.
+// First moments stay in 32 bits through the z and x sums (at most 17 x 17 terms of |q| <= 2^22).
+// ------------------------------------------------------------------------------------------
+constexpr int kCcTX = 64, kCcTY = 8, kCcThreads = 512, kCcRun = 2;
+
+template <int R, bool MASKED, bool DIAGONAL>
+__global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
+  constexpr int W = 2 * R + 1;
+  constexpr int LX = kCcTX + 2 * R, LY = kCcTY + 2 * R;
+  constexpr int NC = LX * LY;
+  constexpr int KC = (NC + kCcThreads - 1) / kCcThreads;  // columns per thread
+  // z sums of the current plane, by column
+  __shared__ int za[LY][LX], zb[LY][LX];
+  __shared__ long long zaa[LY][LX], zbb[LY][LX], zab[LY][LX];
+  __shared__ int zv[MASKED ? LY : 1][MASKED ? LX : 1];
+  // x sums over the window, by row of the tile and halo rows
+  __shared__ int xa[LY][kCcTX], xb[LY][kCcTX];
+  __shared__ long long xaa[LY][kCcTX], xbb[LY][kCcTX], xab[LY][kCcTX];
+  __shared__ int xv[MASKED ? LY : 1][MASKED ? kCcTX : 1];
+
+  const Dims d = a.d;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * kCcTX, y0 = blockIdx.y * kCcTY, z0 = blockIdx.z * a.tile_z;
+  const int z1 = min(z0 + a.tile_z, d.nz);
+  const long long plane = (long long)d.nx * d.ny;
+  const long long n3 = d.n();
+
+  // the columns this thread owns: plane offset, or -1 outside the image
+  long long off[KC];
+  int sa[KC], sb[KC], sv[KC];
+  long long saa[KC], sbb[KC], sab[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int c = tid + kCcThreads * k;
+    off[k] = -1;
+    if (c < NC) {
+      const int gx = x0 - R + c % LX, gy = y0 - R + c / LX;
+      if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) off[k] = (long long)gx + (long long)d.nx * gy;
+    }
+    sa[k] = sb[k] = sv[k] = 0;
+    saa[k] = sbb[k] = sab[k] = 0;
+  }
+  // quantised values of one voxel (zero outside the image, the mask, or the z range)
+  auto load = [&](long long o, int z, int& qa, int& qb, int& qv) {
+    qa = qb = qv = 0;
+    if (o < 0 || z < 0 || z >= d.nz) return;
+    const long long g = o + plane * z;
+    if (MASKED && a.valid[g] == 0.0f) return;
+    qa = __float2int_rn(a.fixed[g] * a.qf);
+    qb = __float2int_rn(a.moving[g] * a.qm);
+    qv = 1;
+  };
+  // warm-up: the window of plane z0
+  for (int z = z0 - R; z <= z0 + R; ++z) {
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      int qa, qb, qv;
+      load(off[k], z, qa, qb, qv);
+      sa[k] += qa;
+      sb[k] += qb;
+      sv[k] += qv;
+      saa[k] += (long long)qa * qa;
+      sbb[k] += (long long)qb * qb;
+      sab[k] += (long long)qa * qb;
+    }
+  }
+
+  const int tx = tid & 63, ty = tid >> 6;  // y pass and output: column tx, row ty
+  float cc_sum = 0.0f, cc_comp = 0.0f;
+  int counted = 0;
+  for (int z = z0; z < z1; ++z) {
+    // the values entering and leaving the window for the next plane, fetched early
+    int ea[KC], eb[KC], ev[KC], la[KC], lb[KC], lv[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      load(off[k], z + R + 1, ea[k], eb[k], ev[k]);
+      load(off[k], z - R, la[k], lb[k], lv[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int c = tid + kCcThreads * k;
+      if (c < NC) {
+        const int ly = c / LX, lx = c % LX;
+        za[ly][lx] = sa[k];
+        zb[ly][lx] = sb[k];
+        zaa[ly][lx] = saa[k];
+        zbb[ly][lx] = sbb[k];
+        zab[ly][lx] = sab[k];
+        if constexpr (MASKED) zv[ly][lx] = sv[k];
+      }
+    }
+    __syncthreads();
+    // x pass: runs of kCcRun outputs along x
+    for (int e = tid; e < LY * (kCcTX / kCcRun); e += kCcThreads) {
+      const int ly = e / (kCcTX / kCcRun), lx = (e % (kCcTX / kCcRun)) * kCcRun;
+      int s1 = 0, s2 = 0, s3 = 0;
+      long long s11 = 0, s22 = 0, s12 = 0;
+#pragma unroll 3
+      for (int t = 0; t < W; ++t) {
+        s1 += za[ly][lx + t];
+        s2 += zb[ly][lx + t];
+        s11 += zaa[ly][lx + t];
+        s22 += zbb[ly][lx + t];
+        s12 += zab[ly][lx + t];
+        if constexpr (MASKED) s3 += zv[ly][lx + t];
+      }
+#pragma unroll 1
+      for (int r = 0; r < kCcRun; ++r) {
+        if (r > 0) {
+          const int in = lx + r + W - 1, out = lx + r - 1;
+          s1 += za[ly][in] - za[ly][out];
+          s2 += zb[ly][in] - zb[ly][out];
+          s11 += zaa[ly][in] - zaa[ly][out];
+          s22 += zbb[ly][in] - zbb[ly][out];
+          s12 += zab[ly][in] - zab[ly][out];
+          if constexpr (MASKED) s3 += zv[ly][in] - zv[ly][out];
+        }
+        xa[ly][lx + r] = s1;
+        xb[ly][lx + r] = s2;
+        xaa[ly][lx + r] = s11;
+        xbb[ly][lx + r] = s22;
+        xab[ly][lx + r] = s12;
+        if constexpr (MASKED) xv[ly][lx + r] = s3;
+      }
+    }
+    __syncthreads();
+    // y pass: row ty of column tx
+    long long wa = 0, wb = 0, waa = 0, wbb = 0, wab = 0;
+    int wv = 0;
+#pragma unroll 3
+    for (int t = 0; t < W; ++t) {
+      wa += xa[ty + t][tx];
+      wb += xb[ty + t][tx];
+      waa += xaa[ty + t][tx];
+      wbb += xbb[ty + t][tx];
+      wab += xab[ty + t][tx];
+      if constexpr (MASKED) wv += xv[ty + t][tx];
+    }
+    {
+      const int x = x0 + tx, y = y0 + ty;
+      if (x < d.nx && y < d.ny) {
+        const long long g = (long long)x + (long long)d.nx * y + plane * z;
+        float dfx = 0.0f, dfy = 0.0f, dfz = 0.0f, dmx = 0.0f, dmy = 0.0f, dmz = 0.0f;
+        const bool centre_valid = !MASKED || a.valid[g] != 0.0f;
+        if (centre_valid) {
+          const long long n = MASKED ? (long long)wv
+                                     : (long long)window_count(x, d.nx, R) * window_count(y, d.ny, R) *
+                                           window_count(z, d.nz, R);
+          const float inv_n = 1.0f / (float)n;
+          long long qa_, ra_, qb_, rb_;
+          floor_divmod(wa, n, inv_n, qa_, ra_);
+          floor_divmod(wb, n, inv_n, qb_, rb_);
+          const long long eaa = waa - n * qa_ * qa_ - 2 * qa_ * ra_;
+          const long long ebb = wbb - n * qb_ * qb_ - 2 * qb_ * rb_;
+          const long long eab = wab - n * qa_ * qb_ - qb_ * ra_ - qa_ * rb_;
+          const float s_ff = ((float)eaa - (float)(ra_ * ra_) * inv_n) * (a.inv_qf * a.inv_qf);
+          const float s_mm = ((float)ebb - (float)(rb_ * rb_) * inv_n) * (a.inv_qm * a.inv_qm);
+          const float s_fm = ((float)eab - (float)(ra_ * rb_) * inv_n) * (a.inv_qf * a.inv_qm);
+          const float fmean = ((float)qa_ + (float)ra_ * inv_n) * a.inv_qf;
+          const float mmean = ((float)qb_ + (float)rb_ * inv_n) * a.inv_qm;
+          const float product = s_ff * s_mm;
+          float cc = 1.0f;
+          if (fabsf(product) > kEps64) cc = s_fm * s_fm / product;
+          const float yk = cc - cc_comp;
+          const float tk = cc_sum + yk;
+          cc_comp = (tk - cc_sum) - yk;
+          cc_sum = tk;
+          counted += 1;
+          if (s_ff > kEps64 && s_mm > kEps64) {
+            const float fi = a.fixed[g] - fmean;
+            const float mi = a.moving[g] - mmean;
+            const float k1 = 2.0f * s_fm / product;
+            const float cf = k1 * (fi - s_fm / s_mm * mi);
+            const float cm = k1 * (mi - s_fm / s_ff * fi);
+            float gm[3], gf[3];
+            if constexpr (DIAGONAL) {
+              diagonal_gradient(a.moving, d, a.probes, x, y, z, gm);
+              diagonal_gradient(a.fixed, d, a.probes, x, y, z, gf);
+            } else {
+#pragma unroll 1
+              for (int q = 0; q < 3; ++q) {
+                gm[q] = central_difference(a.moving, d, a.probes, q, x, y, z);
+                gf[q] = central_difference(a.fixed, d, a.probes, q, x, y, z);
+              }
+            }
+            dfx = cf * gm[0];
+            dfy = cf * gm[1];
+            dfz = cf * gm[2];
+            dmx = cm * gf[0];
+            dmy = cm * gf[1];
+            dmz = cm * gf[2];
+          }
+        }
+        a.d_fixed[g] = dfx;
+        a.d_fixed[g + n3] = dfy;
+        a.d_fixed[g + 2 * n3] = dfz;
+        a.d_moving[g] = dmx;
+        a.d_moving[g + n3] = dmy;
+        a.d_moving[g + 2 * n3] = dmz;
+      }
+    }
+    // slide every owned column's window to the next plane
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      sa[k] += ea[k] - la[k];
+      sb[k] += eb[k] - lb[k];
+      sv[k] += ev[k] - lv[k];
+      saa[k] += (long long)ea[k] * ea[k] - (long long)la[k] * la[k];
+      sbb[k] += (long long)eb[k] * eb[k] - (long long)lb[k] * lb[k];
+      sab[k] += (long long)ea[k] * eb[k] - (long long)la[k] * lb[k];
+    }
+    __syncthreads();
+  }
+  const float total = block_sum(cc_sum);
+  const float cnt = block_sum((float)counted);
+  if (tid == 0) {
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    a.partial_cc[b] = total;
+    a.partial_count[b] = (int)cnt;
+  }
+}
+
 template <int R, bool M>
 static void launch_cc_r(const CcArgs& a, hipStream_t s) {
-  constexpr int TX = R <= 5 ? 64 : 32;
-  constexpr int TY = 4;
-  const dim3 grid((a.d.nx + TX - 1) / TX, (a.d.ny + TY - 1) / TY, (a.d.nz + a.tile_z - 1) / a.tile_z);
-  hipLaunchKernelGGL((k_cc_pair<R, TX, TY, M>), grid, dim3(TX * TY), 0, s, a);
+  if (a.probes.diagonal) hipLaunchKernelGGL((k_cc2<R, M, true>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
+  else hipLaunchKernelGGL((k_cc2<R, M, false>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
 }
 
 dim3 cc_grid(const Dims& d, int tile_z) {
-  // The radius decides TX; callers size the partial buffers with the largest grid (TX = 32).
-  return dim3((d.nx + 31) / 32, (d.ny + 3) / 4, (d.nz + tile_z - 1) / tile_z);
+  return dim3((d.nx + kCcTX - 1) / kCcTX, (d.ny + kCcTY - 1) / kCcTY, (d.nz + tile_z - 1) / tile_z);
 }
 
 void launch_cc_pair(const CcArgs& a, hipStream_t s) {

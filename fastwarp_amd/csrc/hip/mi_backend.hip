@@ -34,22 +34,35 @@ MiLevel::MiLevel(HipBackend& backend, const HImage& moving, const std::vector<Ve
     invalid("hip backend", "the linear Mattes metric supports 5 to 64 histogram bins, not " + std::to_string(binning.bins));
   if (fixed_bin.size() != points.size()) invalid("hip backend", "one fixed bin per sample point is required");
   if (points.size() > size_t(1) << 30) invalid("hip backend", "too many sample points");
+  // the anchor: the points' mean, summed over fixed chunks combined in chunk order
+  constexpr size_t kGrain = size_t(1) << 14;
+  const size_t m = points.size();
+  std::vector<Vec3> part((m + kGrain - 1) / kGrain, Vec3{0, 0, 0});
+  parallel_chunks(m, kGrain, [&](size_t c, size_t b, size_t e) {
+    Vec3 acc{0, 0, 0};
+    for (size_t i = b; i < e; ++i)
+      for (int a = 0; a < 3; ++a) acc[a] += points[i][a];
+    part[c] = acc;
+  });
   Vec3 mean{0, 0, 0};
-  for (const Vec3& p : points)
+  for (const Vec3& p : part)
     for (int a = 0; a < 3; ++a) mean[a] += p[a];
-  if (!points.empty())
-    for (int a = 0; a < 3; ++a) mean[a] /= double(points.size());
+  if (m > 0)
+    for (int a = 0; a < 3; ++a) mean[a] /= double(m);
   anchor_ = mean;
-  std::vector<float> rec(4 * std::max<size_t>(points.size(), 1));
-  for (size_t i = 0; i < points.size(); ++i) {
-    for (int a = 0; a < 3; ++a) rec[4 * i + a] = float(points[i][a] - anchor_[a]);
-    if (fixed_bin[i] < 2 || fixed_bin[i] > binning.bins - 3) invalid("hip backend", "fixed bin outside [2, bins - 3]");
-    int32_t b = int32_t(fixed_bin[i]);
-    std::memcpy(&rec[4 * i + 3], &b, 4);
-  }
-  samples_ = DevBuf(ctx_, rec.size());
-  check(hipMemcpyAsync(samples_.get(), rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()),
-        "upload samples");
+  for (uint32_t b : fixed_bin)
+    if (b < 2 || b > binning.bins - 3) invalid("hip backend", "fixed bin outside [2, bins - 3]");
+  const size_t floats = 4 * std::max<size_t>(m, 1);
+  auto* rec = static_cast<float*>(ctx_->pinned(floats * sizeof(float)));
+  parallel_for(m, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      for (int a = 0; a < 3; ++a) rec[4 * i + a] = float(points[i][a] - anchor_[a]);
+      const int32_t bin = int32_t(fixed_bin[i]);
+      std::memcpy(&rec[4 * i + 3], &bin, 4);
+    }
+  });
+  samples_ = DevBuf(ctx_, floats);
+  check(hipMemcpyAsync(samples_.get(), rec, floats * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()), "upload samples");
   ctx_->sync();
 }
 
@@ -79,9 +92,10 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
     Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
     launch_mi_sampled_hist(a, blocks, s);
   }
-  launch_mi_sum_slots(a.hist_slots, blocks, int(cells), a.hist, s);
-  auto* host = static_cast<unsigned long long*>(ctx_->pinned(cells * sizeof(unsigned long long)));
-  check(hipMemcpyAsync(host, a.hist, cells * sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "read histogram");
+  // the sums go straight to coherent host memory; the sync makes them visible
+  void* dev_hist = nullptr;
+  auto* host = static_cast<unsigned long long*>(ctx_->mapped(0, cells * sizeof(unsigned long long), &dev_hist));
+  launch_mi_sum_slots(a.hist_slots, blocks, int(cells), static_cast<unsigned long long*>(dev_hist), s);
   ctx_->sync();
   const uint64_t valid = host[cells - 1];
   HistogramFinish fin =
@@ -100,10 +114,11 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
   std::vector<double> ratio;
   const double value = histogram(t, a, &ratio);
   const size_t bins = binning_.bins;
-  std::vector<float> rf(ratio.begin(), ratio.end());
-  auto* dev_ratio = static_cast<float*>(ctx_->scratch(1, rf.size() * sizeof(float)));
+  auto* rf = static_cast<float*>(ctx_->pinned(ratio.size() * sizeof(float)));
+  for (size_t i = 0; i < ratio.size(); ++i) rf[i] = float(ratio[i]);
+  auto* dev_ratio = static_cast<float*>(ctx_->scratch(1, ratio.size() * sizeof(float)));
   hipStream_t s = ctx_->stream();
-  check(hipMemcpyAsync(dev_ratio, rf.data(), rf.size() * sizeof(float), hipMemcpyHostToDevice, s), "upload ratio");
+  check(hipMemcpyAsync(dev_ratio, rf, ratio.size() * sizeof(float), hipMemcpyHostToDevice, s), "upload ratio");
   a.ratio = dev_ratio;
   a.probes = probes_of(moving_->grid);
   put(a.outer, t.outer.matrix);
@@ -119,13 +134,14 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
   if (t.optimised.kind == ParamKind::Euler3d && t.optimised.zyx)
     invalid("hip backend", "the ZYX Euler order is not supported by the device derivative");
   const int blocks = mi_sampled_blocks(int(count_));
-  a.partial = static_cast<double*>(ctx_->scratch(2, size_t(blocks) * 12 * sizeof(double)));
+  // the block partials go straight to coherent host memory; the sync makes them visible
+  void* dev_partial = nullptr;
+  auto* host = static_cast<double*>(ctx_->mapped(1, size_t(blocks) * 12 * sizeof(double), &dev_partial));
+  a.partial = static_cast<double*>(dev_partial);
   {
     Timed tm(*ctx_, "k_mi_sampled_deriv", 20.0 * double(count_));
     launch_mi_sampled_deriv(a, blocks, s);
   }
-  auto* host = static_cast<double*>(ctx_->pinned(size_t(blocks) * 12 * sizeof(double)));
-  check(hipMemcpyAsync(host, a.partial, size_t(blocks) * 12 * sizeof(double), hipMemcpyDeviceToHost, s), "read derivative");
   ctx_->sync();
   const size_t np = t.optimised.count();
   std::vector<double> d(np, 0.0);

@@ -55,6 +55,86 @@ __device__ __forceinline__ float central_difference_at(const float* __restrict__
   return (tri(img, d, px, py, pz) - tri(img, d, mx, my, mz)) * pr.inv_spacing[a];
 }
 
+// The moving-image gradient of the linear stage's derivative at a continuous index: the three
+// central differences of interp.rs:102-121. On a grid whose probes lie on the grid axes (a signed
+// permutation of the axes) and where the 3 x 3 x 3 block from floor(c - 0.5) lies inside the
+// image, every probe's trilinear stencil lies in that block. Its 27 values are loaded once. A
+// linear interpolation at a point of the block is a product of per-axis weight triples (1 - w, w, 0)
+// or (0, 1 - w, w), so the block is first contracted over the two axes the probes share with the
+// centre, and each difference is the third axis's weight difference applied to that contraction
+// (the same values as interpolating each probe, up to rounding order). Elsewhere the probes are
+// interpolated one by one.
+__device__ __forceinline__ void axis_triple(float c, int s, float w3[3]) {
+  const float f = floorf(c);
+  const bool first = (int)f == s;
+  const float dd = c - f;
+  const float w = dd > 0.0f ? dd : 0.0f;
+  w3[0] = first ? 1.0f - w : 0.0f;
+  w3[1] = first ? w : 1.0f - w;
+  w3[2] = first ? 0.0f : w;
+}
+
+__device__ __forceinline__ void moving_gradient(const float* __restrict__ img, const Dims& d, const Probes& pr, float cx,
+                                                float cy, float cz, float& g0, float& g1, float& g2) {
+  if (pr.diagonal) {
+    const int sx = (int)floorf(cx - 0.5f), sy = (int)floorf(cy - 0.5f), sz = (int)floorf(cz - 0.5f);
+    if (sx >= 0 && sy >= 0 && sz >= 0 && sx + 2 <= d.nx - 1 && sy + 2 <= d.ny - 1 && sz + 2 <= d.nz - 1) {
+      float v[3][3][3];
+      const long long ys = d.nx, zs = (long long)d.nx * d.ny;
+      const float* p = img + (long long)sx + ys * sy + zs * sz;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) v[k][j][i] = p[i + ys * j + zs * k];
+      float wx[3], wy[3], wz[3];
+      axis_triple(cx, sx, wx);
+      axis_triple(cy, sy, wy);
+      axis_triple(cz, sz, wz);
+      // contractions over two axes at the centre's weights: X over (y, z), Y over (x, z), Z over (x, y)
+      float X[3], Y[3], Z[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int t = 0; t < 3; ++t) {
+            ax = fmaf(wy[r] * wz[t], v[t][r][q], ax);
+            ay = fmaf(wx[r] * wz[t], v[t][q][r], ay);
+            az = fmaf(wx[r] * wy[t], v[q][t][r], az);
+          }
+        X[q] = ax;
+        Y[q] = ay;
+        Z[q] = az;
+      }
+      const float c3[3] = {cx, cy, cz};
+      const int s3[3] = {sx, sy, sz};
+      float g[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        // the probes of physical axis a move along one grid axis
+        const int ax = pr.o[a][0] != 0.0f ? 0 : (pr.o[a][1] != 0.0f ? 1 : 2);
+        const float h = pr.o[a][ax];
+        float hi[3], lo[3];
+        axis_triple(c3[ax] + h, s3[ax], hi);
+        axis_triple(c3[ax] - h, s3[ax], lo);
+        const float* line = ax == 0 ? X : (ax == 1 ? Y : Z);
+        const float dv = (hi[0] - lo[0]) * line[0] + (hi[1] - lo[1]) * line[1] + (hi[2] - lo[2]) * line[2];
+        g[a] = dv * pr.inv_spacing[a];
+      }
+      g0 = g[0];
+      g1 = g[1];
+      g2 = g[2];
+      return;
+    }
+  }
+  g0 = central_difference_at(img, d, pr, 0, cx, cy, cz);
+  g1 = central_difference_at(img, d, pr, 1, cx, cy, cz);
+  g2 = central_difference_at(img, d, pr, 2, cx, cy, cz);
+}
+
 struct SampleEval {
   bool ok;
   int fbin, mbin;
@@ -153,9 +233,8 @@ __global__ __launch_bounds__(256) void k_mi_sampled_deriv(MiSampledArgs a) {
       w += bspline3_d((float)m - e.term) * ratio[e.fbin * bins + m];
     }
     if (w == 0.0f) continue;
-    float g0 = central_difference_at(a.moving, a.md, a.probes, 0, e.cx, e.cy, e.cz);
-    float g1 = central_difference_at(a.moving, a.md, a.probes, 1, e.cx, e.cy, e.cz);
-    float g2 = central_difference_at(a.moving, a.md, a.probes, 2, e.cx, e.cy, e.cz);
+    float g0, g1, g2;
+    moving_gradient(a.moving, a.md, a.probes, e.cx, e.cy, e.cz, g0, g1, g2);
     if (!(isfinite(g0) && isfinite(g1) && isfinite(g2))) g0 = g1 = g2 = 0.0f;
     // g' = outer^T grad: the Jacobian columns are left-multiplied by the outer matrix
     const float q0 = a.outer[0] * g0 + a.outer[3] * g1 + a.outer[6] * g2;

@@ -89,6 +89,25 @@ struct Volume {
 
 using Image = Volume<double>;
 
+// A read-only view of a caller's f64 volume (no copy): the same `grid` / `data` surface as Image
+// for the code that only reads its inputs.
+struct ImageView {
+  struct Values {
+    const double* p = nullptr;
+    size_t n = 0;
+    size_t size() const { return n; }
+    const double& operator[](size_t i) const { return p[i]; }
+    const double* data() const { return p; }
+    const double* begin() const { return p; }
+    const double* end() const { return p + n; }
+  };
+  Grid grid;
+  Values data;
+  ImageView() = default;
+  ImageView(const Grid& g, const double* values) : grid(g), data{values, g.num_voxels()} {}
+  explicit ImageView(const Image& im) : grid(im.grid), data{im.data.data(), im.data.size()} {}
+};
+
 // A displacement field on a grid, (dx, dy, dz) LPS mm per voxel, x-fastest (field_ops.rs:58-63).
 struct VectorField {
   Grid grid;

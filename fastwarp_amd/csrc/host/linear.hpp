@@ -109,15 +109,38 @@ class SmoothCache {
 
 // mi.rs:193-207 `fixed_sample_range`: count and range of the finite nearest-voxel values.
 inline size_t fixed_sample_range(const std::vector<double>& nearest, double& lo, double& hi) {
-  size_t count = 0;
-  for (double v : nearest) {
-    if (!std::isfinite(v)) continue;
-    if (count == 0) lo = hi = v;
-    else {
-      lo = std::min(lo, v);
-      hi = std::max(hi, v);
+  constexpr size_t kGrain = size_t(1) << 14;
+  const size_t chunks = (nearest.size() + kGrain - 1) / kGrain;
+  struct Part {
+    size_t count = 0;
+    double lo = 0, hi = 0;
+  };
+  std::vector<Part> parts(chunks);
+  parallel_chunks(nearest.size(), kGrain, [&](size_t c, size_t b, size_t e) {
+    Part p;
+    for (size_t i = b; i < e; ++i) {
+      const double v = nearest[i];
+      if (!std::isfinite(v)) continue;
+      if (p.count == 0) p.lo = p.hi = v;
+      else {
+        p.lo = std::min(p.lo, v);
+        p.hi = std::max(p.hi, v);
+      }
+      ++p.count;
     }
-    ++count;
+    parts[c] = p;
+  });
+  size_t count = 0;
+  for (const Part& p : parts) {
+    if (p.count == 0) continue;
+    if (count == 0) {
+      lo = p.lo;
+      hi = p.hi;
+    } else {
+      lo = std::min(lo, p.lo);
+      hi = std::max(hi, p.hi);
+    }
+    count += p.count;
   }
   return count;
 }
@@ -141,16 +164,29 @@ inline PreparedSamples prepare_samples(const std::vector<Vec3>& samples, const s
   if (fhi - flo <= kEps) invalid("mattes", "the fixed image is constant over the samples");
   PreparedSamples out;
   out.binning = MiBinning::from_ranges(bins, flo, fhi, moving_range.first, moving_range.second);
-  std::vector<int64_t> bin(samples.size(), -1);
-  parallel_for(samples.size(), [&](size_t b, size_t e) {
-    for (size_t i = b; i < e; ++i)
-      if (std::isfinite(linear[i])) bin[i] = int64_t(out.binning.fixed_bin(linear[i]));
+  // keep the samples whose linear value is finite, in order: count per chunk, then fill each
+  // chunk's slice of the output in parallel
+  constexpr size_t kGrain = size_t(1) << 14;
+  const size_t m = samples.size();
+  const size_t chunks = (m + kGrain - 1) / kGrain;
+  std::vector<size_t> kept(chunks + 1, 0);
+  parallel_chunks(m, kGrain, [&](size_t c, size_t b, size_t e) {
+    size_t k = 0;
+    for (size_t i = b; i < e; ++i) k += std::isfinite(linear[i]) ? 1 : 0;
+    kept[c + 1] = k;
   });
-  for (size_t i = 0; i < samples.size(); ++i)
-    if (bin[i] >= 0) {
-      out.points.push_back(samples[i]);
-      out.fixed_bin.push_back(uint32_t(bin[i]));
-    }
+  for (size_t c = 0; c < chunks; ++c) kept[c + 1] += kept[c];
+  out.points.resize(kept[chunks]);
+  out.fixed_bin.resize(kept[chunks]);
+  parallel_chunks(m, kGrain, [&](size_t c, size_t b, size_t e) {
+    size_t o = kept[c];
+    for (size_t i = b; i < e; ++i)
+      if (std::isfinite(linear[i])) {
+        out.points[o] = samples[i];
+        out.fixed_bin[o] = uint32_t(out.binning.fixed_bin(linear[i]));
+        ++o;
+      }
+  });
   return out;
 }
 
@@ -194,8 +230,8 @@ LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, S
   LinearMovingTransform current{outer, start};
   LinearStageResult result;
   // Level l draws its sample points with seed + l whether or not it samples (each level takes one
-  // seed, sampling.rs:159-163), so the next level's points are drawn on a worker thread while this
-  // level optimises (the reference prefetches them the same way, linear.rs:773-869).
+  // seed, sampling.rs:159-163), so the later levels' points are drawn on worker threads while the
+  // earlier levels optimise (the reference prefetches them on a thread too, linear.rs:773-869).
   auto level_grid = [&](size_t level) {
     return level_virtual_grid(fixed_grid, shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing));
   };
@@ -206,21 +242,20 @@ LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, S
     SeedSequence s(seed);
     return metric_sample_points(grid, recipe.sampling_percentage, s);
   };
-  std::future<std::vector<Vec3>> prefetched;
-  size_t prefetched_level = recipe.levels();
+  // every level that samples starts drawing at once, each on its own worker thread
+  std::vector<std::future<std::vector<Vec3>>> prefetched(recipe.levels());
+  for (size_t level = 1; level < recipe.levels(); ++level)
+    if (needs_samples(level))
+      prefetched[level] = std::async(std::launch::async, draw, level_grid(level), uint32_t(recipe.seed + level));
   for (size_t level = 0; level < recipe.levels(); ++level) {
     check_cancel(cancel);
     const auto t0 = std::chrono::steady_clock::now();
     const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing);
     const Grid grid = level_grid(level);
     std::optional<std::vector<Vec3>> ready;
-    if (prefetched_level == level && prefetched.valid()) {
+    if (prefetched[level].valid()) {
       HostTimer wait("sample_wait");
-      ready = prefetched.get();
-    }
-    if (level + 1 < recipe.levels() && needs_samples(level + 1)) {
-      prefetched = std::async(std::launch::async, draw, level_grid(level + 1), uint32_t(recipe.seed + level + 1));
-      prefetched_level = level + 1;
+      ready = prefetched[level].get();
     }
     const double sigma = recipe.smoothing_sigmas[level];
     LinearLevelTrace tr;
@@ -293,23 +328,43 @@ LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, S
 
 // init.rs:47-86: intensity-weighted centre (ITK ImageMomentsCalculator) and the initial
 // translation moving centre - fixed centre about the fixed centre.
-inline Vec3 centre_of_gravity(const Image& vol) {
+// Summed per plane on the host threads, the planes' sums then added in plane order.
+template <typename V>
+Vec3 centre_of_gravity(const V& vol) {
+  HostTimer host_timer("centre_of_mass");
   const Grid& g = vol.grid;
+  const size_t nx = g.size[0], ny = g.size[1], nz = g.size[2];
+  struct Part {
+    double mass = 0.0;
+    Vec3 cg{0, 0, 0};
+  };
+  std::vector<Part> parts(nz);
+  parallel_chunks(nz, 1, [&](size_t k, size_t, size_t) {
+    Part p;
+    for (size_t j = 0; j < ny; ++j) {
+      const double* row = vol.data.data() + nx * (j + ny * k);
+      for (size_t i = 0; i < nx; ++i) {
+        const double v = row[i];
+        if (v == 0.0) continue;
+        const Vec3 x = g.index_to_point({double(i), double(j), double(k)});
+        p.mass += v;
+        for (int a = 0; a < 3; ++a) p.cg[a] += x[a] * v;
+      }
+    }
+    parts[k] = p;
+  });
   double mass = 0.0;
   Vec3 cg{0, 0, 0};
-  for (size_t l = 0; l < vol.data.size(); ++l) {
-    const double v = vol.data[l];
-    if (v == 0.0) continue;
-    auto [i, j, k] = g.ijk(l);
-    const Vec3 p = g.index_to_point({double(i), double(j), double(k)});
-    mass += v;
-    for (int a = 0; a < 3; ++a) cg[a] += p[a] * v;
+  for (const Part& p : parts) {
+    mass += p.mass;
+    for (int a = 0; a < 3; ++a) cg[a] += p.cg[a];
   }
   if (mass == 0.0) invalid("initial transform", "total mass of the image is zero");
   return {cg[0] / mass, cg[1] / mass, cg[2] / mass};
 }
 
-inline AffineTransform initial_transform_com(const Image& fixed, const Image& moving) {
+template <typename V>
+AffineTransform initial_transform_com(const V& fixed, const V& moving) {
   const Vec3 fc = centre_of_gravity(fixed), mc = centre_of_gravity(moving);
   return AffineTransform{kIdentity3, vsub(mc, fc), fc};
 }
@@ -322,9 +377,9 @@ struct LinearChainResult {
 
 // linear.rs:1148-1212 `run_linear_chain_cached_async`: centre of mass, Rigid, Affine, collapsed
 // about the last linear transform's centre.
-template <typename Engine, typename B>
-LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine, const Image& fixed_raw,
-                                   const Image& moving_raw, SmoothCache<B>& cache, const volatile int32_t* cancel) {
+template <typename Engine, typename B, typename V>
+LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine, const V& fixed_raw,
+                                   const V& moving_raw, SmoothCache<B>& cache, const volatile int32_t* cancel) {
   if (rigid.kind != LinearKind::Rigid || affine.kind != LinearKind::Affine)
     invalid("linear chain", "expected a rigid then an affine recipe");
   LinearChainResult out;

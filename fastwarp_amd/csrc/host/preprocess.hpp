@@ -9,7 +9,8 @@ namespace fw {
 // preprocess.rs:40-150: the ImageToHistogramFilter histogram and its quantiles.
 class IntensityHistogram {
  public:
-  explicit IntensityHistogram(const std::vector<double>& values) {
+  template <typename Values>
+  explicit IntensityHistogram(const Values& values) {
     constexpr size_t kBins = 256;
     // range and counts over fixed chunks, merged in chunk order
     constexpr size_t kGrain = size_t(1) << 16;
@@ -122,7 +123,8 @@ class IntensityHistogram {
 
 // preprocess.rs:159-232: the quantile window rounded to f32, then the linear map of [lo, hi] onto
 // [out_min, out_max] with clamping; non-finite voxels pass unchanged.
-inline Image preprocess_volume(const Image& vol, float lower, float upper, double out_min = 0.0, double out_max = 1.0) {
+template <typename V>
+Image preprocess_volume(const V& vol, float lower, float upper, double out_min = 0.0, double out_max = 1.0) {
   HostTimer host_timer("preprocess");
   IntensityHistogram h(vol.data);
   const double lo = double(float(h.quantile(double(lower))));

@@ -31,9 +31,9 @@ struct RegisterOutput {
   double seconds_preprocess = 0.0, seconds_linear = 0.0, seconds_syn = 0.0, seconds_total = 0.0;
 };
 
-template <typename B, typename E>
-RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, const std::vector<Image>& fixed,
-                             const std::vector<Image>& moving, const volatile int32_t* cancel, FieldOutputs outputs = {}) {
+template <typename B, typename E, typename V>
+RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, const std::vector<V>& fixed,
+                             const std::vector<V>& moving, const volatile int32_t* cancel, FieldOutputs outputs = {}) {
   using clock = std::chrono::steady_clock;
   auto secs = [](clock::time_point a) { return std::chrono::duration<double>(clock::now() - a).count(); };
   const auto t_all = clock::now();
@@ -43,10 +43,20 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
     invalid("stage plan", "the number of channels differs from the number of channel weights");
   {
     HostTimer t_check("validate");
+    bool finite = true;
     for (const auto* set : {&fixed, &moving})
-      for (const Image& v : *set)
-        for (double x : v.data)
-          if (!std::isfinite(x)) invalid("register", "an input image has non-finite (NaN or infinite) voxels");
+      for (const V& v : *set) {
+        std::atomic<bool> ok{true};
+        parallel_for(v.data.size(), [&](size_t b, size_t e) {
+          for (size_t i = b; i < e; ++i)
+            if (!std::isfinite(v.data[i])) {
+              ok = false;
+              return;
+            }
+        });
+        finite = finite && ok;
+      }
+    if (!finite) invalid("register", "an input image has non-finite (NaN or infinite) voxels");
   }
   RegisterOutput out;
   auto t = clock::now();

@@ -136,19 +136,12 @@ void run_register(B& backend, E& engine, const fw_plan* plan, const fw_grid* fix
                   const volatile int32_t* cancel) {
   const RegisterPlan p = plan_from(plan);
   const Grid fg = grid_of(fixed_grid), mg = grid_of(moving_grid);
-  std::vector<Image> f, m;
-  {
-    HostTimer t_in("copy_inputs");
-    for (int32_t c = 0; c < plan->channels; ++c) {
-      if (!fixed || !moving || !fixed[c] || !moving[c]) invalid("register", "null image data");
-      auto copy = [](const double* src, size_t n) {
-        std::vector<double> v(n);
-        parallel_for(n, [&](size_t b, size_t e) { std::copy(src + b, src + e, v.begin() + long(b)); });
-        return v;
-      };
-      f.emplace_back(fg, copy(fixed[c], fg.num_voxels()));
-      m.emplace_back(mg, copy(moving[c], mg.num_voxels()));
-    }
+  // the caller's arrays are read in place
+  std::vector<ImageView> f, m;
+  for (int32_t c = 0; c < plan->channels; ++c) {
+    if (!fixed || !moving || !fixed[c] || !moving[c]) invalid("register", "null image data");
+    f.emplace_back(fg, fixed[c]);
+    m.emplace_back(mg, moving[c]);
   }
   const RegisterOutput o = register_full(backend, engine, p, f, m, cancel, FieldOutputs{forward, inverse});
   result_to(o, result);
