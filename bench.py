@@ -27,6 +27,15 @@ def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel: str) -> int | None:
+    """HBM bytes per launch of `kernel` in this workload from the committed PMC summary
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel), if present."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    return json.loads(path.read_text()).get("registration_1mm", {}).get(kernel)
+
+
 def roofline(profile: dict) -> dict:
     """The kernel with the largest total time: algorithmic bytes per launch over its mean duration."""
     name, (launches, ms, nbytes) = max(profile.items(), key=lambda kv: kv[1][1])
@@ -39,7 +48,8 @@ def roofline(profile: dict) -> dict:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(name),
+        "algorithmic_bytes_per_launch": round(nbytes / launches),
         "launches": launches,
         "mean_launch_us": round(per_launch_ms * 1e3, 2),
         "share_of_gpu_time": round(ms / sum(v[1] for v in profile.values()), 4),

@@ -421,13 +421,13 @@ HImage HipBackend::warp_mask_through_field(const HImage& mask, const HField& fie
     a.product = out.buf.get();
     a.out = out.buf.get();
     out.absmax = 1.0f;
-    Timed t(*ctx_, "k_warp_mask", 24.0 * double(field.grid.num_voxels()));
+    Timed t(*ctx_, "k_warp", 24.0 * double(field.grid.num_voxels()));
     launch_warp(a, ctx_->stream());
     return out;
   }
   HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), 1.0f};
   a.out = out.buf.get();
-  Timed t(*ctx_, "k_warp_mask", 20.0 * double(field.grid.num_voxels()));
+  Timed t(*ctx_, "k_warp", 20.0 * double(field.grid.num_voxels()));
   launch_warp(a, ctx_->stream());
   return out;
 }
@@ -463,7 +463,7 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
   a.partial_cc = static_cast<float*>(ctx_->scratch(0, blocks * sizeof(float)));
   a.partial_count = static_cast<int*>(ctx_->scratch(1, blocks * sizeof(int)));
   {
-    Timed t(*ctx_, "k_cc_pair", (valid ? 60.0 : 56.0) * double(n));
+    Timed t(*ctx_, "k_cc2", (valid ? 60.0 : 56.0) * double(n));
     launch_cc_pair(a, ctx_->stream());
   }
   const size_t real = blocks;
@@ -543,7 +543,7 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
   a.partial_cc = partials.get();
   a.partial_count = reinterpret_cast<int*>(partials.get() + blocks);
   {
-    Timed t(*ctx_, "k_cc_pair", 56.0 * double(n));
+    Timed t(*ctx_, "k_cc2", 56.0 * double(n));
     launch_cc_pair(a, ctx_->stream());
   }
   const size_t real = blocks;
@@ -582,8 +582,11 @@ Pending<std::pair<double, double>> HipBackend::scale_to_step(HField& a, HField& 
   }
   launch_step_factors(pm, nb, 2, step, out.get(), out.get() + 2, s);
   {
-    Timed t(*ctx_, "k_scale", 48.0 * double(n));
+    Timed t(*ctx_, "k_scale_by", 24.0 * double(n));
     launch_scale_by(a.buf.get(), 3 * n, out.get(), s);
+  }
+  {
+    Timed t(*ctx_, "k_scale_by", 24.0 * double(n));
     launch_scale_by(b.buf.get(), 3 * n, out.get() + 1, s);
   }
   float* host = static_cast<float*>(ctx_->stage(2 * sizeof(float)));
@@ -728,12 +731,20 @@ HField HipBackend::smooth_field(HField field, double variance) {
       }
       for (size_t q = 0; q < taps.size(); ++q) dst[a][q] = float(taps[q]);
     }
-    if (fits) {
+    if (fits && gauss3_fits(d, t)) {
       HField out{g, DevBuf(ctx_.get(), 3 * n)};
       DevBuf tmp(ctx_.get(), 3 * n);
-      // x/y pass: read u, write the intermediate; z pass: read it (and u for a blend), write
-      Timed tm(*ctx_, "k_gauss_xy_z", 48.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0));
-      if (launch_gauss3(field.buf.get(), tmp.get(), out.buf.get(), d, t, float(w1), float(w2), ctx_->stream())) return out;
+      {
+        // read u, write the x/y-smoothed intermediate
+        Timed tm(*ctx_, "k_gauss_xy", 24.0 * double(n));
+        launch_gauss_xy(field.buf.get(), tmp.get(), d, t, ctx_->stream());
+      }
+      {
+        // read the intermediate (and u for a blend), write the result
+        Timed tm(*ctx_, "k_gauss_z", 24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0));
+        launch_gauss_z(tmp.get(), field.buf.get(), out.buf.get(), d, t, float(w1), float(w2), ctx_->stream());
+      }
+      return out;
     }
   }
   DevBuf t1(ctx_.get(), 3 * n), t2(ctx_.get(), 3 * n);
@@ -741,17 +752,17 @@ HField HipBackend::smooth_field(HField field, double variance) {
   const double nb = double(n);
   const float* tx = ctx_->taps(variance, g.size[0], r);
   {
-    Timed t(*ctx_, "k_gauss_x", 24.0 * nb);
+    Timed t(*ctx_, "k_gauss_axis", 24.0 * nb);
     launch_gauss_axis(field.buf.get(), t1.get(), d, 0, tx, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
   }
   const float* ty = ctx_->taps(variance, g.size[1], r);
   {
-    Timed t(*ctx_, "k_gauss_y", 24.0 * nb);
+    Timed t(*ctx_, "k_gauss_axis", 24.0 * nb);
     launch_gauss_axis(t1.get(), t2.get(), d, 1, ty, r, nullptr, 1.0f, 0.0f, 0, ctx_->stream());
   }
   const float* tz = ctx_->taps(variance, g.size[2], r);
   {
-    Timed t(*ctx_, "k_gauss_z", 36.0 * nb);
+    Timed t(*ctx_, "k_gauss_axis", 36.0 * nb);
     launch_gauss_axis(t2.get(), t1.get(), d, 2, tz, r, field.buf.get(), float(w1), float(w2), 1, ctx_->stream());
   }
   return HField{g, std::move(t1)};

@@ -20,196 +20,12 @@ namespace fw::hip {
 // CC pair (metric/cc.rs:170-321 and wgpu_backend.rs:1354-1520 semantics, fused).
 //
 // Both images are quantised to fixed point (|value| * q <= 2^22, one q per image) and every
-// window sum is an exact 64-bit integer, so the sums carry no rounding at all and do not depend
-// on tiling. A block owns a TX x TY column of outputs over tile_z planes: each input plane is
-// staged in LDS with an r-voxel halo, summed along x then y, and the z window slides through a
-// ring of 2r+1 plane sums held in registers (the loop is unrolled by 2r+1 so every ring index is
-// static). Centred moments come from the integer sums by a floor division (s = q n + r), which
-// keeps the cancellation inside exact integer arithmetic. Each output voxel then writes both
+// window sum is an exact integer, so the sums carry no rounding at all and do not depend on
+// tiling. Centred moments come from the integer sums by a floor division (s = q n + r), which
+// keeps the cancellation inside exact integer arithmetic. Each output voxel writes both
 // derivative directions; localCC is symmetric, so one value partial serves both.
-// ------------------------------------------------------------------------------------------
-template <int R, int TX, int TY, bool MASKED>
-__global__ __launch_bounds__(TX* TY) void k_cc_pair(CcArgs a) {
-  constexpr int W = 2 * R + 1;
-  constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
-  constexpr int NT = TX * TY;
-  __shared__ int qa[LY][LX];
-  __shared__ int qb[LY][LX];
-  __shared__ int qv[MASKED ? LY : 1][MASKED ? LX : 1];
-  // x-window sums: first moments fit 32 bits (|q| <= 2^22, at most 17 terms), second moments not.
-  __shared__ int xa[LY][TX], xb[LY][TX];
-  __shared__ long long xaa[LY][TX], xbb[LY][TX], xab[LY][TX];
-  __shared__ int xv[MASKED ? LY : 1][MASKED ? TX : 1];
-
-  const Dims d = a.d;
-  const int tid = threadIdx.x;
-  const int tx = tid % TX, ty = tid / TX;
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.tile_z;
-  const int x = x0 + tx, y = y0 + ty;
-  const int zcount = min(a.tile_z, d.nz - z0);
-  const int steps = zcount + 2 * R;
-  const long long sz = (long long)d.nx * d.ny;
-  const long long n3 = d.n();
-
-  // Shift register of the last W xy-plane sums (slot 0 leaves next); xy sums of the first
-  // moments fit 32 bits (at most 17 x 17 terms of |q| <= 2^22).
-  int ra[W], rb[W], rv[MASKED ? W : 1];
-  long long raa[W], rbb[W], rab[W];
-#pragma unroll
-  for (int q = 0; q < W; ++q) {
-    ra[q] = rb[q] = 0;
-    raa[q] = rbb[q] = rab[q] = 0;
-  }
-#pragma unroll
-  for (int q = 0; q < (MASKED ? W : 1); ++q) rv[q] = 0;
-  long long sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
-  int sv = 0;
-  float cc_sum = 0.0f, cc_comp = 0.0f;
-  int counted = 0;
-
-  for (int step = 0; step < steps; ++step) {
-    const int zin = z0 - R + step;
-    int pa = 0, pb = 0, pv = 0;
-    long long paa = 0, pbb = 0, pab = 0;
-    if (zin >= 0 && zin < d.nz) {
-      __syncthreads();
-      for (int t = tid; t < LX * LY; t += NT) {
-        const int lx = t % LX, ly = t / LX;
-        const int gx = x0 - R + lx, gy = y0 - R + ly;
-        int va = 0, vb = 0, vv = 0;
-        if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) {
-          const long long g = (long long)gx + (long long)d.nx * gy + sz * zin;
-          const bool ok = !MASKED || a.valid[g] != 0.0f;
-          if (ok) {
-            va = __float2int_rn(a.fixed[g] * a.qf);
-            vb = __float2int_rn(a.moving[g] * a.qm);
-            vv = 1;
-          }
-        }
-        qa[ly][lx] = va;
-        qb[ly][lx] = vb;
-        if constexpr (MASKED) qv[ly][lx] = vv;
-      }
-      __syncthreads();
-      for (int t = tid; t < LY * TX; t += NT) {
-        const int ly = t / TX, lx = t % TX;
-        int s1 = 0, s2 = 0, s3 = 0;
-        long long s11 = 0, s22 = 0, s12 = 0;
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-          const int u = qa[ly][lx + k], v = qb[ly][lx + k];
-          s1 += u;
-          s2 += v;
-          s11 += (long long)u * u;
-          s22 += (long long)v * v;
-          s12 += (long long)u * v;
-          if constexpr (MASKED) s3 += qv[ly][lx + k];
-        }
-        xa[ly][lx] = s1;
-        xb[ly][lx] = s2;
-        xaa[ly][lx] = s11;
-        xbb[ly][lx] = s22;
-        xab[ly][lx] = s12;
-        if constexpr (MASKED) xv[ly][lx] = s3;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < W; ++k) {
-        pa += xa[ty + k][tx];
-        pb += xb[ty + k][tx];
-        paa += xaa[ty + k][tx];
-        pbb += xbb[ty + k][tx];
-        pab += xab[ty + k][tx];
-        if constexpr (MASKED) pv += xv[ty + k][tx];
-      }
-    }
-    sa += (long long)(pa - ra[0]);
-    sb += (long long)(pb - rb[0]);
-    saa += paa - raa[0];
-    sbb += pbb - rbb[0];
-    sab += pab - rab[0];
-    if constexpr (MASKED) sv += pv - rv[0];
-#pragma unroll
-    for (int q = 0; q < W - 1; ++q) {
-      ra[q] = ra[q + 1];
-      rb[q] = rb[q + 1];
-      raa[q] = raa[q + 1];
-      rbb[q] = rbb[q + 1];
-      rab[q] = rab[q + 1];
-      if constexpr (MASKED) rv[q] = rv[q + 1];
-    }
-    ra[W - 1] = pa;
-    rb[W - 1] = pb;
-    raa[W - 1] = paa;
-    rbb[W - 1] = pbb;
-    rab[W - 1] = pab;
-    if constexpr (MASKED) rv[W - 1] = pv;
-
-    if (step >= 2 * R && x < d.nx && y < d.ny) {
-      const int z = zin - R;
-      const long long g = (long long)x + (long long)d.nx * y + sz * z;
-      float dfx = 0.0f, dfy = 0.0f, dfz = 0.0f, dmx = 0.0f, dmy = 0.0f, dmz = 0.0f;
-      const bool centre_valid = !MASKED || a.valid[g] != 0.0f;
-      if (centre_valid) {
-        const long long n = MASKED ? (long long)sv
-                                   : (long long)window_count(x, d.nx, R) * window_count(y, d.ny, R) *
-                                         window_count(z, d.nz, R);
-        const float inv_n = 1.0f / (float)n;
-        long long qa_, ra_, qb_, rb_;
-        floor_divmod(sa, n, inv_n, qa_, ra_);
-        floor_divmod(sb, n, inv_n, qb_, rb_);
-        // Sums of squared deviations from the integer part of the mean, exact in wrapping
-        // 64-bit arithmetic (the true values are small), then the fractional correction.
-        const long long eaa = saa - n * qa_ * qa_ - 2 * qa_ * ra_;
-        const long long ebb = sbb - n * qb_ * qb_ - 2 * qb_ * rb_;
-        const long long eab = sab - n * qa_ * qb_ - qb_ * ra_ - qa_ * rb_;
-        const float s_ff = ((float)eaa - (float)(ra_ * ra_) * inv_n) * (a.inv_qf * a.inv_qf);
-        const float s_mm = ((float)ebb - (float)(rb_ * rb_) * inv_n) * (a.inv_qm * a.inv_qm);
-        const float s_fm = ((float)eab - (float)(ra_ * rb_) * inv_n) * (a.inv_qf * a.inv_qm);
-        const float fmean = ((float)qa_ + (float)ra_ * inv_n) * a.inv_qf;
-        const float mmean = ((float)qb_ + (float)rb_ * inv_n) * a.inv_qm;
-        const float product = s_ff * s_mm;
-        float cc = 1.0f;
-        if (fabsf(product) > kEps64) cc = s_fm * s_fm / product;
-        const float yk = cc - cc_comp;
-        const float tk = cc_sum + yk;
-        cc_comp = (tk - cc_sum) - yk;
-        cc_sum = tk;
-        counted += 1;
-        if (s_ff > kEps64 && s_mm > kEps64) {
-          const float fi = a.fixed[g] - fmean;
-          const float mi = a.moving[g] - mmean;
-          const float k1 = 2.0f * s_fm / product;
-          const float cf = k1 * (fi - s_fm / s_mm * mi);
-          const float cm = k1 * (mi - s_fm / s_ff * fi);
-          dfx = cf * central_difference(a.moving, d, a.probes, 0, x, y, z);
-          dfy = cf * central_difference(a.moving, d, a.probes, 1, x, y, z);
-          dfz = cf * central_difference(a.moving, d, a.probes, 2, x, y, z);
-          dmx = cm * central_difference(a.fixed, d, a.probes, 0, x, y, z);
-          dmy = cm * central_difference(a.fixed, d, a.probes, 1, x, y, z);
-          dmz = cm * central_difference(a.fixed, d, a.probes, 2, x, y, z);
-        }
-      }
-      a.d_fixed[g] = dfx;
-      a.d_fixed[g + n3] = dfy;
-      a.d_fixed[g + 2 * n3] = dfz;
-      a.d_moving[g] = dmx;
-      a.d_moving[g + n3] = dmy;
-      a.d_moving[g + 2 * n3] = dmz;
-    }
-  }
-  const float total = block_sum(cc_sum);
-  const float cnt = block_sum((float)counted);
-  if (tid == 0) {
-    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    a.partial_cc[b] = total;
-    a.partial_count[b] = (int)cnt;
-  }
-}
-
-// ------------------------------------------------------------------------------------------
-// CC pair, second form: the same exact integer window sums, organised to do less integer work per
-// output. A block owns a 64 x 8 (x, y) tile over a run of tile_z planes. Each thread keeps the
+//
+// The work is organised to keep the integer operations per output low. A block owns a 64 x 8 (x, y) tile over a run of tile_z planes. Each thread keeps the
 // running z-window sums of its columns of the tile and halo in registers: one plane enters and
 // one leaves per step (exact in integers, so a running sum never drifts), read from global
 // memory (the leaving plane was read 2r+1 steps before and sits in cache). The z sums of the
@@ -898,10 +714,17 @@ static int gz_zrun(const Dims& d, int rz) {
   return run < least ? least : run;
 }
 
-bool launch_gauss3(const float* in, float* tmp, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s) {
-  if (t.rx > kG3MaxR || t.ry > kG3MaxR || t.rz > kG3MaxR || t.rz < 1 || 3LL * d.nz > 65535) return false;
+bool gauss3_fits(const Dims& d, const GaussTaps3& t) {
+  return t.rx <= kG3MaxR && t.ry <= kG3MaxR && t.rz <= kG3MaxR && t.rz >= 1 && 3LL * d.nz <= 65535;
+}
+
+void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s) {
   const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * d.nz);
   hipLaunchKernelGGL(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t);
+}
+
+void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
+                    hipStream_t s) {
   const int run = gz_zrun(d, t.rz);
   const dim3 gz((d.nx + 63) / 64, (d.ny + 3) / 4, 3 * ((d.nz + run - 1) / run));
   switch (t.rz) {
@@ -919,9 +742,8 @@ bool launch_gauss3(const float* in, float* tmp, float* out, Dims d, const GaussT
     FW_GZ_CASE(8)
 #undef FW_GZ_CASE
     default:
-      return false;
+      break;
   }
-  return true;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -104,13 +104,15 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
 void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const float* taps_dev, int radius,
                        const float* orig, float w1, float w2, int final_pass, hipStream_t s);
 // x and y in one pass into `tmp`, then z (with the blend against `in` and the zeroed faces) into
-// `out`; taps per axis, radius at most 8 each (else nothing is launched and false is returned).
+// `out`; taps per axis, radius at most 8 each (gauss3_fits).
 struct GaussTaps3 {
   float x[17], y[17], z[17];
   int rx, ry, rz;
 };
-bool launch_gauss3(const float* in, float* tmp, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
-                   hipStream_t s);
+bool gauss3_fits(const Dims& d, const GaussTaps3& t);
+void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s);
+void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
+                    hipStream_t s);
 
 // --- inversion (ITK InvertDisplacementFieldImageFilter) ---
 struct InvertState {

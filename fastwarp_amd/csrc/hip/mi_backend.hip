@@ -95,7 +95,10 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
   // the sums go straight to coherent host memory; the sync makes them visible
   void* dev_hist = nullptr;
   auto* host = static_cast<unsigned long long*>(ctx_->mapped(0, cells * sizeof(unsigned long long), &dev_hist));
-  launch_mi_sum_slots(a.hist_slots, blocks, int(cells), static_cast<unsigned long long*>(dev_hist), s);
+  {
+    Timed tm(*ctx_, "k_mi_sum_slots", 8.0 * double(cells) * double(blocks + 1));
+    launch_mi_sum_slots(a.hist_slots, blocks, int(cells), static_cast<unsigned long long*>(dev_hist), s);
+  }
   ctx_->sync();
   const uint64_t valid = host[cells - 1];
   HistogramFinish fin =

@@ -197,18 +197,37 @@ void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_mi_sampled_hist, dim3(blocks), dim3(256), lds, s, a);
 }
 
-// hist[c] = sum of the slots' cell c, for every cell and the valid count (one thread per cell).
-__global__ __launch_bounds__(256) void k_mi_sum_slots(const unsigned long long* __restrict__ slots, int nslots,
-                                                      int cells, unsigned long long* __restrict__ hist) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  unsigned long long t = 0ull;
-  for (int b = 0; b < nslots; ++b) t += slots[(size_t)b * cells + c];
-  hist[c] = t;
+// hist[c] = sum of the slots' cell c, for every cell and the valid count. A block of 16 waves
+// takes 64 cells: lane = cell, wave w adds slots w, w + 16, ... in four independent chains (so the
+// loads of one wave overlap), and the 16 wave sums are added in LDS.
+__global__ __launch_bounds__(1024) void k_mi_sum_slots(const unsigned long long* __restrict__ slots, int nslots,
+                                                       int cells, unsigned long long* __restrict__ hist) {
+  __shared__ unsigned long long part[16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  unsigned long long t0 = 0ull, t1 = 0ull, t2 = 0ull, t3 = 0ull;
+  if (c < cells) {
+    int b = w;
+    for (; b + 48 < nslots; b += 64) {
+      t0 += slots[(size_t)b * cells + c];
+      t1 += slots[(size_t)(b + 16) * cells + c];
+      t2 += slots[(size_t)(b + 32) * cells + c];
+      t3 += slots[(size_t)(b + 48) * cells + c];
+    }
+    for (; b < nslots; b += 16) t0 += slots[(size_t)b * cells + c];
+  }
+  part[w][lane] = (t0 + t1) + (t2 + t3);
+  __syncthreads();
+  if (w == 0 && c < cells) {
+    unsigned long long t = 0ull;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t += part[q][lane];
+    hist[c] = t;
+  }
 }
 
 void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_mi_sum_slots, dim3((cells + 255) / 256), dim3(256), 0, s, slots, nslots, cells, hist);
+  hipLaunchKernelGGL(k_mi_sum_slots, dim3((cells + 63) / 64), dim3(1024), 0, s, slots, nslots, cells, hist);
 }
 
 // Per sample: sum_k B3'(k - term) * ratio[f, k] * (grad M . dT/dtheta_p) for every parameter
