@@ -80,6 +80,7 @@ _SIGNATURES = [
     ("fw_mi_level_value_and_derivative", C.c_int, [_P, C.POINTER(LinearTransform), _D, _D]),
     ("fw_mi_level_free", None, [_P]),
     ("fw_set_profiling", C.c_int, [_P, C.c_int]),
+    ("fw_set_profiling_focus", C.c_int, [_P, C.c_char_p, C.c_int]),
     ("fw_take_profile", C.c_int, [_P, C.c_char_p, C.c_size_t]),
     ("fw_plan_capacity", C.c_int, [C.POINTER(Plan), C.POINTER(Grid), C.POINTER(Grid), C.POINTER(Capacity)]),
     ("fw_memory_budget", C.c_int, [_P, C.POINTER(C.c_uint64)]),
@@ -205,14 +206,21 @@ class HipBackend:
     def set_profiling(self, on: bool):
         self._check(lib().fw_set_profiling(self.ctx, int(on)))
 
-    def take_profile(self) -> dict[str, tuple[int, float, float]]:
-        """Per kernel family: (launches, total milliseconds, algorithmic bytes)."""
+    def set_profiling_focus(self, family: str | None = None, every: int = 1):
+        """Time only every `every`-th launch of `family` (None: every launch of every family)."""
+        self._check(lib().fw_set_profiling_focus(self.ctx, (family or "").encode(), int(every)))
+
+    def take_profile(self) -> dict[str, tuple[int, float, float, int]]:
+        """Per kernel family: (launches, milliseconds of all launches, algorithmic bytes, timed
+        launches). With a focus, a family's milliseconds are its timed launches' mean times its
+        launches, and NaN for a family with no timed launch."""
         buf = C.create_string_buffer(1 << 16)
         self._check(lib().fw_take_profile(self.ctx, buf, len(buf)))
         out = {}
         for line in buf.value.decode().splitlines():
-            name, launches, ms, nbytes = line.split()
-            out[name] = (int(launches), float(ms), float(nbytes))
+            name, launches, ms, nbytes, timed = line.split()
+            n, t = int(launches), int(timed)
+            out[name] = (n, float(ms) * n / t if t else float("nan"), float(nbytes), t)
         return out
 
     def _check(self, status: int):

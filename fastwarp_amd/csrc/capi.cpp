@@ -475,6 +475,10 @@ int fw_set_profiling(fw_ctx* ctx, int on) {
   return guard(ctx, [&] { ctx->backend.ctx().set_profiling(on != 0); });
 }
 
+int fw_set_profiling_focus(fw_ctx* ctx, const char* family, int every) {
+  return guard(ctx, [&] { ctx->backend.ctx().set_profiling_focus(family ? std::string(family) : std::string(), every); });
+}
+
 int fw_take_profile(fw_ctx* ctx, char* text, size_t capacity) {
   return guard(ctx, [&] {
     need(text, "output");

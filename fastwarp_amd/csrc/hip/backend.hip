@@ -40,7 +40,6 @@ DeviceContext::~DeviceContext() {
     (void)hipEventDestroy(s.a);
     (void)hipEventDestroy(s.b);
   }
-  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(stream_);
 }
 
@@ -203,21 +202,38 @@ void DeviceContext::set_profiling(bool on) {
 void DeviceContext::host_span(const char* name, double seconds) {
   Total& t = totals_[std::string("host:") + name];
   t.launches += 1;
+  t.timed += 1;
   t.ms += seconds * 1e3;
 }
 
+// Every span gets new events: an event that has been carried by a launch before is not handed
+// to another (its earlier binding is not relied on to be replaced).
 hipEvent_t DeviceContext::event() {
-  if (!events_.empty()) {
-    hipEvent_t e = events_.back();
-    events_.pop_back();
-    return e;
-  }
   hipEvent_t e = nullptr;
   check(hipEventCreate(&e), "hipEventCreate");
   return e;
 }
 
+void DeviceContext::set_profiling_focus(const std::string& family, int every) {
+  if (every < 1) invalid("profiling", "the timing interval must be at least 1");
+  focus_ = family;
+  focus_every_ = every;
+  focus_seen_ = 0;
+}
+
+bool DeviceContext::should_time(const char* family) {
+  if (focus_.empty()) return true;
+  if (focus_ != family) return false;
+  return focus_seen_++ % uint64_t(focus_every_) == 0;
+}
+
 void DeviceContext::span(const char* name, double bytes, hipEvent_t a, hipEvent_t b) { spans_.push_back({name, bytes, a, b}); }
+
+void DeviceContext::count(const char* name, double bytes) {
+  Total& t = totals_[name];
+  t.launches += 1;
+  t.bytes += bytes;
+}
 
 void DeviceContext::add_bytes(const char* name, double bytes) { totals_[name].bytes += bytes; }
 
@@ -228,17 +244,18 @@ std::string DeviceContext::take_profile() {
     check(hipEventElapsedTime(&ms, s.a, s.b), "hipEventElapsedTime");
     Total& t = totals_[s.name];
     t.launches += 1;
+    t.timed += 1;
     t.ms += double(ms);
     t.bytes += s.bytes;
-    events_.push_back(s.a);
-    events_.push_back(s.b);
+    (void)hipEventDestroy(s.a);
+    (void)hipEventDestroy(s.b);
   }
   spans_.clear();
   std::string out;
   char line[256];
   for (auto& kv : totals_) {
-    std::snprintf(line, sizeof line, "%s %llu %.6f %.0f\n", kv.first.c_str(), (unsigned long long)kv.second.launches, kv.second.ms,
-                  kv.second.bytes);
+    std::snprintf(line, sizeof line, "%s %llu %.6f %.0f %llu\n", kv.first.c_str(), (unsigned long long)kv.second.launches,
+                  kv.second.ms, kv.second.bytes, (unsigned long long)kv.second.timed);
     out += line;
   }
   totals_.clear();

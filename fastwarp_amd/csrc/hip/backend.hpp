@@ -66,12 +66,21 @@ class DeviceContext : public HostSpanSink {
   // The same taps on the host (f64), cached.
   const std::vector<double>& host_taps(double variance, size_t length);
 
-  // Per-kernel timing (the reference's profiler.rs:95-170): while on, launches are bracketed by
-  // events; take_profile() waits, adds the spans up per kernel family and clears them.
+  // Per-kernel timing (the reference's profiler.rs:95-170): while on, launches carry events;
+  // take_profile() waits, adds the spans up per kernel family and clears them.
   void set_profiling(bool on);
   bool profiling() const { return profiling_; }
+  // Limits the timing to every `every`-th launch of one kernel family (an empty name: every
+  // launch of every family). The other launches are still counted, with their bytes, untimed, so
+  // a timed run pays the events' cost on few launches.
+  void set_profiling_focus(const std::string& family, int every);
+  // Whether the next launch of `family` is to be timed (advances the focus family's counter).
+  bool should_time(const char* family);
   hipEvent_t event();
+  void drop(hipEvent_t e) { (void)hipEventDestroy(e); }
   void span(const char* name, double bytes, hipEvent_t a, hipEvent_t b);
+  // An untimed launch of a family, with its algorithmic bytes.
+  void count(const char* name, double bytes);
   void add_bytes(const char* name, double bytes);
   // Host work of the drivers while profiling, listed as "host:<name>" (sync waits included).
   void host_span(const char* name, double seconds) override;
@@ -106,35 +115,48 @@ class DeviceContext : public HostSpanSink {
   std::map<std::pair<double, size_t>, std::pair<void*, int>> taps_;
   std::map<std::pair<double, size_t>, std::vector<double>> host_taps_;
   bool profiling_ = false;
+  std::string focus_;
+  int focus_every_ = 1;
+  uint64_t focus_seen_ = 0;
   struct Span {
     std::string name;
     double bytes;
     hipEvent_t a, b;
   };
   std::vector<Span> spans_;
-  std::vector<hipEvent_t> events_;
   struct Total {
-    uint64_t launches = 0;
-    double ms = 0.0, bytes = 0.0;
+    uint64_t launches = 0, timed = 0;  // all launches, and those with events
+    double ms = 0.0, bytes = 0.0;      // ms of the timed launches; bytes of all
   };
   std::map<std::string, Total> totals_;
 };
 
-// Brackets the launches of its scope with events when the context is profiling.
+// A timing scope: while the context profiles, the launches made inside it carry its events (or,
+// for a family outside the profiling focus, are only counted).
 class Timed {
  public:
   Timed(DeviceContext& ctx, const char* name, double bytes) : ctx_(ctx), name_(name), bytes_(bytes) {
-    if (ctx_.profiling()) {
-      a_ = ctx_.event();
-      (void)hipEventRecord(a_, ctx_.stream());
-    }
+    if (!ctx_.profiling()) return;
+    active_ = true;
+    if (!ctx_.should_time(name_)) return;
+    ev_.start = ctx_.event();
+    ev_.stop = ctx_.event();
+    outer_ = current_launch_events();
+    current_launch_events() = &ev_;
+    timing_ = true;
   }
   ~Timed() {
-    if (a_) {
-      hipEvent_t b = ctx_.event();
-      (void)hipEventRecord(b, ctx_.stream());
-      ctx_.span(name_, bytes_, a_, b);
+    if (!active_) return;
+    if (timing_) {
+      current_launch_events() = outer_;
+      if (ev_.launches > 0) {
+        ctx_.span(name_, bytes_, ev_.start, ev_.stop);
+        return;
+      }
+      ctx_.drop(ev_.start);
+      ctx_.drop(ev_.stop);
     }
+    ctx_.count(name_, bytes_);
   }
   Timed(const Timed&) = delete;
   Timed& operator=(const Timed&) = delete;
@@ -143,7 +165,9 @@ class Timed {
   DeviceContext& ctx_;
   const char* name_;
   double bytes_;
-  hipEvent_t a_ = nullptr;
+  bool active_ = false, timing_ = false;
+  LaunchEvents ev_;
+  LaunchEvents* outer_ = nullptr;
 };
 
 // A pooled device buffer of f32 values (move-only; returns to the pool on destruction).

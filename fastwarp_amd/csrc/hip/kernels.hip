@@ -16,6 +16,11 @@
 
 namespace fw::hip {
 
+LaunchEvents*& current_launch_events() {
+  static thread_local LaunchEvents* open = nullptr;
+  return open;
+}
+
 // ------------------------------------------------------------------------------------------
 // CC pair (metric/cc.rs:170-321 and wgpu_backend.rs:1354-1520 semantics, fused).
 //
@@ -254,8 +259,8 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
 
 template <int R, bool M>
 static void launch_cc_r(const CcArgs& a, hipStream_t s) {
-  if (a.probes.diagonal) hipLaunchKernelGGL((k_cc2<R, M, true>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
-  else hipLaunchKernelGGL((k_cc2<R, M, false>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
+  if (a.probes.diagonal) launch((k_cc2<R, M, true>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
+  else launch((k_cc2<R, M, false>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
 }
 
 dim3 cc_grid(const Dims& d, int tile_z) {
@@ -330,7 +335,7 @@ int norm_blocks(long long n) {
 
 void launch_field_norms(const float* a, const float* b, long long n, float3 inv_spacing, float* partial_max,
                         float* partial_sq, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_field_norms, dim3(blocks), dim3(256), 0, s, a, b, n, inv_spacing, partial_max, partial_sq);
+  launch(k_field_norms, dim3(blocks), dim3(256), 0, s, a, b, n, inv_spacing, partial_max, partial_sq);
 }
 
 // Update factors of the SyN loop (syn.rs:422-445): for each of `fields` fields, the NaN-propagating
@@ -352,7 +357,7 @@ __global__ __launch_bounds__(256) void k_step_factors(const float* __restrict__ 
 
 void launch_step_factors(const float* partial_max, int blocks, int fields, double step, float* factors, float* maxima,
                          hipStream_t s) {
-  hipLaunchKernelGGL(k_step_factors, dim3(1), dim3(256), 0, s, partial_max, blocks, fields, step, factors, maxima);
+  launch(k_step_factors, dim3(1), dim3(256), 0, s, partial_max, blocks, fields, step, factors, maxima);
 }
 
 __global__ __launch_bounds__(256) void k_scale_by(float* __restrict__ x, long long count, const float* __restrict__ factor) {
@@ -370,7 +375,7 @@ __global__ __launch_bounds__(256) void k_abs_max(const float* __restrict__ x, lo
 }
 
 void launch_abs_max(const float* x, long long n, float* partial, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_abs_max, dim3(blocks), dim3(256), 0, s, x, n, partial);
+  launch(k_abs_max, dim3(blocks), dim3(256), 0, s, x, n, partial);
 }
 
 // Finite minimum and maximum of x per block; a block without a finite value reports +inf / -inf.
@@ -406,7 +411,7 @@ __global__ __launch_bounds__(256) void k_finite_range(const float* __restrict__ 
 }
 
 void launch_finite_range(const float* x, long long n, float* partial_min, float* partial_max, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_finite_range, dim3(blocks), dim3(256), 0, s, x, n, partial_min, partial_max);
+  launch(k_finite_range, dim3(blocks), dim3(256), 0, s, x, n, partial_min, partial_max);
 }
 
 // Image values at continuous indices (mi.rs:161-207 and interp.rs:43-87 rules): the nearest voxel,
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(256) void k_sample_values(const float* __restrict__
 
 void launch_sample_values(const float* img, Dims d, const float4* idx, long long m, float* nearest, float* linear,
                           hipStream_t s) {
-  hipLaunchKernelGGL(k_sample_values, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, img, d, idx, m, nearest, linear);
+  launch(k_sample_values, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, img, d, idx, m, nearest, linear);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -446,15 +451,15 @@ static int stream_blocks(long long count) {
 }
 
 void launch_scale_by(float* x, long long count, const float* factor, hipStream_t st) {
-  hipLaunchKernelGGL(k_scale_by, dim3(stream_blocks(count)), dim3(256), 0, st, x, count, factor);
+  launch(k_scale_by, dim3(stream_blocks(count)), dim3(256), 0, st, x, count, factor);
 }
 
 void launch_scale(float* x, long long count, float s, hipStream_t st) {
-  hipLaunchKernelGGL(k_scale, dim3(stream_blocks(count)), dim3(256), 0, st, x, count, s);
+  launch(k_scale, dim3(stream_blocks(count)), dim3(256), 0, st, x, count, s);
 }
 
 void launch_axpy(float* acc, const float* x, long long count, float a, hipStream_t st) {
-  hipLaunchKernelGGL(k_axpy, dim3(stream_blocks(count)), dim3(256), 0, st, acc, x, count, a);
+  launch(k_axpy, dim3(stream_blocks(count)), dim3(256), 0, st, acc, x, count, a);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -507,7 +512,7 @@ __global__ __launch_bounds__(256) void k_warp(WarpArgs a) {
 
 void launch_warp(const WarpArgs& a, hipStream_t s) {
   const dim3 grid((a.od.nx + 63) / 64, (a.od.ny + 3) / 4, a.od.nz);
-  hipLaunchKernelGGL(k_warp, grid, dim3(256), 0, s, a);
+  launch(k_warp, grid, dim3(256), 0, s, a);
 }
 
 // Field resampling onto another grid (field_ops.rs:580-605): linear, zero outside the old buffer.
@@ -534,7 +539,7 @@ __global__ __launch_bounds__(256) void k_resample(ResampleArgs a) {
 
 void launch_resample(const ResampleArgs& a, hipStream_t s) {
   const dim3 grid((a.od.nx + 63) / 64, (a.od.ny + 3) / 4, a.od.nz);
-  hipLaunchKernelGGL(k_resample, grid, dim3(256), 0, s, a);
+  launch(k_resample, grid, dim3(256), 0, s, a);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -580,7 +585,7 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
   const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
-  hipLaunchKernelGGL(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out);
+  launch(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -720,7 +725,7 @@ bool gauss3_fits(const Dims& d, const GaussTaps3& t) {
 
 void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s) {
   const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * d.nz);
-  hipLaunchKernelGGL(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t);
+  launch(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t);
 }
 
 void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
@@ -730,7 +735,7 @@ void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const
   switch (t.rz) {
 #define FW_GZ_CASE(R)                                                                              \
   case R:                                                                                          \
-    hipLaunchKernelGGL(k_gauss_z<R>, gz, dim3(256), 0, s, tmp, in, out, d, t, run, w1, w2);        \
+    launch(k_gauss_z<R>, gz, dim3(256), 0, s, tmp, in, out, d, t, run, w1, w2);        \
     break;
     FW_GZ_CASE(1)
     FW_GZ_CASE(2)
@@ -832,7 +837,7 @@ __global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ 
 
 void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s) {
   const long long lines = axis == 0 ? (long long)d.ny * d.nz : axis == 1 ? (long long)d.nx * d.nz : (long long)d.nx * d.ny;
-  hipLaunchKernelGGL(k_deriche_axis, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, in, out, d, axis, c);
+  launch(k_deriche_axis, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, in, out, d, axis, c);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -880,7 +885,7 @@ __global__ __launch_bounds__(256) void k_gauss_axis(const float* __restrict__ in
 void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const float* taps_dev, int radius,
                        const float* orig, float w1, float w2, int final_pass, hipStream_t s) {
   const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
-  hipLaunchKernelGGL(k_gauss_axis, grid, dim3(256), 0, s, in, out, d, axis, taps_dev, radius, orig, w1, w2, final_pass);
+  launch(k_gauss_axis, grid, dim3(256), 0, s, in, out, d, axis, taps_dev, radius, orig, w1, w2, final_pass);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1077,7 +1082,7 @@ void launch_inv_begin(const float* field, const float* psi, float* residual, Dim
                       InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  hipLaunchKernelGGL(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
+  launch(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
                      partials, max_iterations);
 }
 
@@ -1086,7 +1091,7 @@ void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, co
                      double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  hipLaunchKernelGGL(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
+  launch(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
                      partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
 }
 

@@ -5,9 +5,33 @@
 #include <cstddef>
 #include <cstdint>
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 namespace fw::hip {
+
+// Events carried by the launches of an open timing scope on this thread: the scope's first launch
+// records `start` when it begins, every launch records `stop` when it ends, so the span is the
+// kernels' own execution (hipExtLaunchKernelGGL), with no marker packets between them.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+  int launches = 0;
+};
+
+// The timing scope open on this thread, or nullptr.
+LaunchEvents*& current_launch_events();
+
+// Every kernel of the library is launched through here.
+template <typename... Params, typename... Args>
+inline void launch(void (*kernel)(Params...), dim3 grid, dim3 block, uint32_t lds, hipStream_t s, Args... args) {
+  LaunchEvents* ev = current_launch_events();
+  if (ev == nullptr) {
+    kernel<<<grid, block, lds, s>>>(args...);
+    return;
+  }
+  hipExtLaunchKernelGGL(kernel, grid, block, lds, s, ev->launches == 0 ? ev->start : nullptr, ev->stop, 0u, args...);
+  ++ev->launches;
+}
 
 // A continuous-index map c = m * (i, j, k) + c0 (+ l * u for a displacement u), f32 row-major.
 struct IndexMap {

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
 
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
   const size_t lds = size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
-  hipLaunchKernelGGL(k_mi_sampled_hist, dim3(blocks), dim3(256), lds, s, a);
+  launch(k_mi_sampled_hist, dim3(blocks), dim3(256), lds, s, a);
 }
 
 // hist[c] = sum of the slots' cell c, for every cell and the valid count. A block of 16 waves
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(1024) void k_mi_sum_slots(const unsigned long long*
 }
 
 void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_mi_sum_slots, dim3((cells + 63) / 64), dim3(1024), 0, s, slots, nslots, cells, hist);
+  launch(k_mi_sum_slots, dim3((cells + 63) / 64), dim3(1024), 0, s, slots, nslots, cells, hist);
 }
 
 // Per sample: sum_k B3'(k - term) * ratio[f, k] * (grad M . dT/dtheta_p) for every parameter
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_mi_sampled_deriv(MiSampledArgs a) {
 }
 
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_mi_sampled_deriv, dim3(blocks), dim3(256), 0, s, a);
+  launch(k_mi_sampled_deriv, dim3(blocks), dim3(256), 0, s, a);
 }
 
 // ------------------------------------------------------------------------------ dense pair ---
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void k_mi_dense_range(MiDenseArgs a) {
 }
 
 void launch_mi_dense_range(const MiDenseArgs& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_mi_dense_range, dim3(blocks), dim3(256), 0, s, a);
+  launch(k_mi_dense_range, dim3(blocks), dim3(256), 0, s, a);
 }
 
 __device__ __forceinline__ int threshold_bin(const float* thr, int bins, float v) {
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_mi_dense_hist(MiDenseArgs a) {
 
 void launch_mi_dense_hist(const MiDenseArgs& a, int blocks, hipStream_t s) {
   const size_t shmem = size_t(2) * (a.bins * a.bins + a.bins) * sizeof(unsigned long long);
-  hipLaunchKernelGGL(k_mi_dense_hist, dim3(blocks), dim3(256), shmem, s, a);
+  launch(k_mi_dense_hist, dim3(blocks), dim3(256), shmem, s, a);
 }
 
 // Per voxel sum_k B3'(m_k - term) ratio[f, m_k] * grad(moving) for both directions (mi.rs:650-666).
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(256) void k_mi_dense_deriv(MiDenseArgs a) {
 
 void launch_mi_dense_deriv(const MiDenseArgs& a, hipStream_t s) {
   const dim3 grid((a.d.nx + 63) / 64, (a.d.ny + 3) / 4, a.d.nz);
-  hipLaunchKernelGGL(k_mi_dense_deriv, grid, dim3(256), 0, s, a);
+  launch(k_mi_dense_deriv, grid, dim3(256), 0, s, a);
 }
 
 }  // namespace fw::hip

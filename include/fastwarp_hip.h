@@ -239,10 +239,16 @@ typedef struct {
 
 /* ---- profiling (the reference's WgpuBackend::set_profiling / take_profile,
  *      crates/fastwarp-gpu/src/profiler.rs:95-170) ---- */
-/* While on, every kernel launch is bracketed by HIP events on the context's stream. */
+/* While on, every kernel launch carries HIP events on the context's stream
+ * (hipExtLaunchKernelGGL: the span is the kernel's own execution). */
 int fw_set_profiling(fw_ctx* ctx, int on);
-/* Writes one line per kernel family, "name launches total_ms algorithmic_bytes", into `text`
- * (NUL-terminated, at most `capacity` bytes) and clears the totals. */
+/* Times only every `every`-th launch of the kernel family `family` (NULL or "": every launch of
+ * every family); the other launches are counted with their bytes but carry no events. No
+ * counterpart in the reference: it keeps the cost of the events off most launches of a timed run. */
+int fw_set_profiling_focus(fw_ctx* ctx, const char* family, int every);
+/* Writes one line per kernel family, "name launches timed_ms algorithmic_bytes timed_launches",
+ * into `text` (NUL-terminated, at most `capacity` bytes) and clears the totals; timed_ms covers
+ * the timed launches only. Host spans of the drivers are listed as "host:<name>". */
 int fw_take_profile(fw_ctx* ctx, char* text, size_t capacity);
 
 /* ---- admission: device memory of one registration (the reference's capacity model,
