@@ -212,9 +212,9 @@ int fw_sample_values(fw_ctx* ctx, const fw_image* img, const double* points, int
     need(img, "image");
     if (m < 0) fw::invalid("sample values", "negative count");
     if (m > 0) need(points, "points");
-    std::vector<fw::Vec3> pts(static_cast<size_t>(m));
+    fw::HostVec<fw::Vec3> pts(static_cast<size_t>(m));
     for (int64_t i = 0; i < m; ++i) pts[size_t(i)] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
-    std::vector<double> a, b;
+    fw::HostVec<double> a, b;
     ctx->backend.sample_values(img->h, pts, a, b);
     if (nearest) std::copy(a.begin(), a.end(), nearest);
     if (linear) std::copy(b.begin(), b.end(), linear);
@@ -302,9 +302,9 @@ int fw_mi_level_create(fw_ctx* ctx, const fw_image* moving, const double* points
       need(points, "points");
       need(fixed_bin, "fixed bins");
     }
-    std::vector<fw::Vec3> pts(static_cast<size_t>(m));
+    fw::HostVec<fw::Vec3> pts(static_cast<size_t>(m));
     for (int64_t i = 0; i < m; ++i) pts[size_t(i)] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
-    std::vector<uint32_t> fb(fixed_bin, fixed_bin + m);
+    fw::HostVec<uint32_t> fb(fixed_bin, fixed_bin + m);
     fw::MiBinning b;
     b.bins = size_t(binning->bins);
     b.fixed_min = binning->fixed_min;

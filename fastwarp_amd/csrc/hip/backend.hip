@@ -669,11 +669,11 @@ HImage HipBackend::smooth_image_mm(const HImage& src, const Vec3& sigma_mm) {
   return out;
 }
 
-void HipBackend::sample_values(const HImage& img, const std::vector<Vec3>& points, std::vector<double>& nearest,
-                               std::vector<double>& linear) {
+void HipBackend::sample_values(const HImage& img, const HostVec<Vec3>& points, HostVec<double>& nearest,
+                               HostVec<double>& linear) {
   const size_t m = points.size();
-  nearest.assign(m, std::nan(""));
-  linear.assign(m, std::nan(""));
+  nearest.resize(m);  // every element is written below
+  linear.resize(m);
   if (m == 0) return;
   const Grid& g = img.grid;
   hipStream_t s = ctx_->stream();

@@ -260,8 +260,8 @@ class HipBackend {
   HImage smooth_image_mm(const HImage& src, const Vec3& sigma_mm);
   // Nearest-voxel and linear values at physical points (NaN outside), as the linear stages'
   // sample preparation reads them (mi.rs:161-207, linear.rs:251-279).
-  void sample_values(const HImage& img, const std::vector<Vec3>& points, std::vector<double>& nearest,
-                     std::vector<double>& linear);
+  void sample_values(const HImage& img, const HostVec<Vec3>& points, HostVec<double>& nearest,
+                     HostVec<double>& linear);
   // Finite minimum and maximum; false when the image has no finite value.
   bool finite_range(const HImage& img, double& lo, double& hi);
 
@@ -295,7 +295,7 @@ class HipBackend {
 class MiLevel {
  public:
   // `moving` must outlive the level. points: physical sample points; fixed_bin: their bins.
-  MiLevel(HipBackend& backend, const HImage& moving, const std::vector<Vec3>& points, const std::vector<uint32_t>& fixed_bin,
+  MiLevel(HipBackend& backend, const HImage& moving, const HostVec<Vec3>& points, const HostVec<uint32_t>& fixed_bin,
           const MiBinning& binning);
   double value(const LinearMovingTransform& t);
   std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t);
@@ -323,8 +323,8 @@ class HipLinearEngine {
       return metric->value_and_derivative(t);
     }
   };
-  Level prepare_level(std::shared_ptr<const HImage> moving, const std::vector<Vec3>& points,
-                      const std::vector<uint32_t>& fixed_bin, const MiBinning& binning) {
+  Level prepare_level(std::shared_ptr<const HImage> moving, const HostVec<Vec3>& points,
+                      const HostVec<uint32_t>& fixed_bin, const MiBinning& binning) {
     Level l;
     l.moving = std::move(moving);
     l.metric = std::make_unique<MiLevel>(backend_, *l.moving, points, fixed_bin, binning);

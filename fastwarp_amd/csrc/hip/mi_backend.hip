@@ -2,6 +2,7 @@
 // crates/fastwarp-gpu/src/wgpu_backend/mi.rs:144-524) and the dense SyN pair
 // (wgpu_backend/mi/dense.rs:241-487). The device returns exact integer histograms; the host
 // finishes them in f64 with the same code path for both (metric/mi.rs:702-769).
+#include <atomic>
 #include <cstring>
 
 #include "backend.hpp"
@@ -27,8 +28,8 @@ std::vector<double> marginal_from(const unsigned long long* h, size_t bins) {
 
 // --------------------------------------------------------------------- linear stage level ---
 
-MiLevel::MiLevel(HipBackend& backend, const HImage& moving, const std::vector<Vec3>& points,
-                 const std::vector<uint32_t>& fixed_bin, const MiBinning& binning)
+MiLevel::MiLevel(HipBackend& backend, const HImage& moving, const HostVec<Vec3>& points,
+                 const HostVec<uint32_t>& fixed_bin, const MiBinning& binning)
     : ctx_(&backend.ctx()), moving_(&moving), count_(points.size()), binning_(binning) {
   if (binning.bins < 5 || binning.bins > size_t(kMiMaxBins))
     invalid("hip backend", "the linear Mattes metric supports 5 to 64 histogram bins, not " + std::to_string(binning.bins));
@@ -50,17 +51,21 @@ MiLevel::MiLevel(HipBackend& backend, const HImage& moving, const std::vector<Ve
   if (m > 0)
     for (int a = 0; a < 3; ++a) mean[a] /= double(m);
   anchor_ = mean;
-  for (uint32_t b : fixed_bin)
-    if (b < 2 || b > binning.bins - 3) invalid("hip backend", "fixed bin outside [2, bins - 3]");
   const size_t floats = 4 * std::max<size_t>(m, 1);
   auto* rec = static_cast<float*>(ctx_->pinned(floats * sizeof(float)));
+  std::atomic<bool> bad_bin{false};
   parallel_for(m, [&](size_t b, size_t e) {
+    bool bad = false;
     for (size_t i = b; i < e; ++i) {
       for (int a = 0; a < 3; ++a) rec[4 * i + a] = float(points[i][a] - anchor_[a]);
-      const int32_t bin = int32_t(fixed_bin[i]);
+      const uint32_t fb = fixed_bin[i];
+      bad |= fb < 2 || fb > binning.bins - 3;
+      const int32_t bin = int32_t(fb);
       std::memcpy(&rec[4 * i + 3], &bin, 4);
     }
+    if (bad) bad_bin.store(true);
   });
+  if (bad_bin.load()) invalid("hip backend", "fixed bin outside [2, bins - 3]");
   samples_ = DevBuf(ctx_, floats);
   check(hipMemcpyAsync(samples_.get(), rec, floats * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()), "upload samples");
   ctx_->sync();

@@ -7,20 +7,131 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
+#include <exception>
 #include <functional>
+#include <mutex>
 #include <initializer_list>
 #include <limits>
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace fw {
 
 using Vec3 = std::array<double, 3>;
 using Mat3 = std::array<std::array<double, 3>, 3>;
+
+// Large host blocks kept for reuse: a registration allocates the same large buffers (sample
+// points, sample values, preprocessed volumes) every time, and a block that has been touched
+// once costs no page faults when it is handed out again. A request takes the smallest cached
+// block that holds it, if that block is at most a quarter larger; blocks beyond the cache's
+// limit (FW_HOST_CACHE_MB, default 2048 MiB) go back to the system, oldest first.
+class HostBlockCache {
+ public:
+  static constexpr size_t kMinBytes = size_t(1) << 20;
+  static HostBlockCache& get() {
+    static HostBlockCache* cache = new HostBlockCache();  // never destroyed: blocks live until exit
+    return *cache;
+  }
+  void* take(size_t bytes) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      size_t best = free_.size();
+      for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].bytes >= bytes && free_[i].bytes <= bytes + bytes / 4 &&
+            (best == free_.size() || free_[i].bytes < free_[best].bytes))
+          best = i;
+      if (best != free_.size()) {
+        const Block b = free_[best];
+        free_.erase(free_.begin() + std::ptrdiff_t(best));
+        cached_ -= b.bytes;
+        live_.push_back(b);
+        return b.p;
+      }
+    }
+    void* p = ::operator new(bytes);
+    std::lock_guard<std::mutex> lk(mu_);
+    live_.push_back({p, bytes});
+    return p;
+  }
+  void give(void* p) {
+    std::vector<void*> release;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto it = std::find_if(live_.begin(), live_.end(), [&](const Block& b) { return b.p == p; });
+      if (it == live_.end()) {
+        release.push_back(p);
+      } else {
+        free_.push_back(*it);
+        cached_ += it->bytes;
+        live_.erase(it);
+        while (cached_ > limit_ && !free_.empty()) {
+          release.push_back(free_.front().p);
+          cached_ -= free_.front().bytes;
+          free_.erase(free_.begin());
+        }
+      }
+    }
+    for (void* q : release) ::operator delete(q);
+  }
+
+ private:
+  HostBlockCache() {
+    if (const char* s = std::getenv("FW_HOST_CACHE_MB")) {
+      const long v = std::strtol(s, nullptr, 10);
+      if (v >= 0) limit_ = size_t(v) << 20;
+    }
+  }
+  struct Block {
+    void* p;
+    size_t bytes;
+  };
+  std::mutex mu_;
+  std::vector<Block> free_, live_;
+  size_t cached_ = 0, limit_ = size_t(2048) << 20;
+};
+
+// An allocator whose value-less construct leaves the element default-initialised (indeterminate
+// for arithmetic types and arrays of them), so that resizing a large host buffer does not write it
+// serially first: the parallel loop that fills it is the first to touch its pages. Blocks of a
+// MiB and more come from the HostBlockCache.
+template <typename T>
+struct DefaultInit : std::allocator<T> {
+  using value_type = T;
+  template <typename U>
+  struct rebind {
+    using other = DefaultInit<U>;
+  };
+  DefaultInit() = default;
+  template <typename U>
+  DefaultInit(const DefaultInit<U>&) noexcept {}
+  T* allocate(size_t n) {
+    const size_t bytes = n * sizeof(T);
+    if (bytes >= HostBlockCache::kMinBytes) return static_cast<T*>(HostBlockCache::get().take(bytes));
+    return std::allocator<T>::allocate(n);
+  }
+  void deallocate(T* p, size_t n) {
+    if (n * sizeof(T) >= HostBlockCache::kMinBytes) HostBlockCache::get().give(p);
+    else std::allocator<T>::deallocate(p, n);
+  }
+  template <typename U>
+  void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using HostVec = std::vector<T, DefaultInit<T>>;
 
 constexpr Mat3 kIdentity3 = {{{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}}};
 constexpr double kEps = std::numeric_limits<double>::epsilon();
@@ -128,28 +239,120 @@ inline unsigned host_threads() {
 // Runs body(begin, end) over [0, n) in contiguous chunks on the host threads. Each chunk is a
 // fixed function of n and `grain`, so per-chunk results combined in chunk order do not depend on
 // the thread count.
+//
+// The loops run on one process-wide pool of host_threads() - 1 parked workers plus the caller (a
+// new thread per loop would cost tens of microseconds per thread). One loop runs on the pool at a
+// time; a loop started while the pool is busy (from another thread, or from inside a loop body)
+// gets threads of its own.
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool* pool = new HostPool(host_threads());  // never destroyed: its workers stay parked
+    return *pool;
+  }
+  // body(c) for every chunk c < chunks; false (nothing run) if the pool is busy. Every worker
+  // takes part in every loop (those that find no chunk left return at once), so no worker can
+  // still hold a loop's body once try_run has returned.
+  bool try_run(size_t chunks, const std::function<void(size_t)>& body) {
+    std::unique_lock<std::mutex> run(run_mu_, std::try_to_lock);
+    if (!run.owns_lock()) return false;
+    job_ = &body;
+    chunks_ = chunks;
+    next_.store(0);
+    error_ = nullptr;
+    busy_.store(unsigned(workers_.size()));
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      generation_.fetch_add(1, std::memory_order_release);
+    }
+    wake_.notify_all();
+    work();
+    if (!spin_until([&] { return busy_.load(std::memory_order_acquire) == 0; })) {
+      std::unique_lock<std::mutex> lk(mu_);
+      done_.wait(lk, [&] { return busy_.load(std::memory_order_acquire) == 0; });
+    }
+    job_ = nullptr;
+    if (error_) std::rethrow_exception(error_);
+    return true;
+  }
+
+ private:
+  explicit HostPool(unsigned threads) {
+    for (unsigned t = 1; t < threads; ++t) workers_.emplace_back([this] { loop(); });
+    for (auto& w : workers_) w.detach();
+  }
+  // Polls `ready` for about 100 microseconds (a parked thread takes longer than that to wake).
+  template <typename F>
+  static bool spin_until(F ready) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned i = 1;; ++i) {
+      if (ready()) return true;
+      if ((i & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(100)) return ready();
+    }
+  }
+  void work() {
+    for (;;) {
+      const size_t c = next_.fetch_add(1);
+      if (c >= chunks_) return;
+      try {
+        (*job_)(c);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!error_) error_ = std::current_exception();
+        next_.store(chunks_);
+      }
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      if (!spin_until([&] { return generation_.load(std::memory_order_acquire) != seen; })) {
+        std::unique_lock<std::mutex> lk(mu_);
+        wake_.wait(lk, [&] { return generation_.load(std::memory_order_acquire) != seen; });
+      }
+      seen = generation_.load(std::memory_order_acquire);
+      work();
+      if (busy_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_.notify_one();
+      }
+    }
+  }
+  std::mutex run_mu_, mu_;
+  std::condition_variable wake_, done_;
+  std::vector<std::thread> workers_;
+  const std::function<void(size_t)>* job_ = nullptr;
+  size_t chunks_ = 0;
+  std::atomic<size_t> next_{0};
+  std::atomic<unsigned> busy_{0};
+  std::atomic<uint64_t> generation_{0};
+  std::exception_ptr error_;
+};
+
 inline void parallel_chunks(size_t n, size_t grain, const std::function<void(size_t, size_t, size_t)>& body) {
   if (n == 0) return;
   grain = std::max<size_t>(grain, 1);
   const size_t chunks = (n + grain - 1) / grain;
+  auto chunk = [&](size_t c) { body(c, c * grain, std::min(n, (c + 1) * grain)); };
   const unsigned threads = static_cast<unsigned>(std::min<size_t>(host_threads(), chunks));
   if (threads <= 1) {
-    for (size_t c = 0; c < chunks; ++c) body(c, c * grain, std::min(n, (c + 1) * grain));
+    for (size_t c = 0; c < chunks; ++c) chunk(c);
     return;
   }
+  if (HostPool::get().try_run(chunks, chunk)) return;
   std::atomic<size_t> next{0};
   auto worker = [&] {
     for (;;) {
-      size_t c = next.fetch_add(1);
+      const size_t c = next.fetch_add(1);
       if (c >= chunks) return;
-      body(c, c * grain, std::min(n, (c + 1) * grain));
+      chunk(c);
     }
   };
-  std::vector<std::thread> pool;
-  pool.reserve(threads - 1);
-  for (unsigned t = 1; t < threads; ++t) pool.emplace_back(worker);
+  std::vector<std::thread> own;
+  own.reserve(threads - 1);
+  for (unsigned t = 1; t < threads; ++t) own.emplace_back(worker);
   worker();
-  for (auto& t : pool) t.join();
+  for (auto& t : own) t.join();
 }
 
 // parallel_chunks with a default grain that gives every thread several chunks.

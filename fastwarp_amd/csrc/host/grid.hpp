@@ -79,12 +79,16 @@ struct Grid {
 template <typename T>
 struct Volume {
   Grid grid;
-  std::vector<T> data;
+  HostVec<T> data;
   Volume() = default;
-  Volume(const Grid& g, std::vector<T> d) : grid(g), data(std::move(d)) {
+  Volume(const Grid& g, HostVec<T> d) : grid(g), data(std::move(d)) { check(); }
+  Volume(const Grid& g, const std::vector<T>& d) : grid(g), data(d.begin(), d.end()) { check(); }
+  static Volume zeros(const Grid& g) { return Volume(g, HostVec<T>(g.num_voxels(), T{})); }
+
+ private:
+  void check() const {
     if (data.size() != grid.num_voxels()) invalid("volume data", "length differs from the voxel count");
   }
-  static Volume zeros(const Grid& g) { return Volume(g, std::vector<T>(g.num_voxels(), T{})); }
 };
 
 using Image = Volume<double>;

@@ -108,7 +108,7 @@ class SmoothCache {
 };
 
 // mi.rs:193-207 `fixed_sample_range`: count and range of the finite nearest-voxel values.
-inline size_t fixed_sample_range(const std::vector<double>& nearest, double& lo, double& hi) {
+inline size_t fixed_sample_range(const HostVec<double>& nearest, double& lo, double& hi) {
   constexpr size_t kGrain = size_t(1) << 14;
   const size_t chunks = (nearest.size() + kGrain - 1) / kGrain;
   struct Part {
@@ -146,16 +146,16 @@ inline size_t fixed_sample_range(const std::vector<double>& nearest, double& lo,
 }
 
 struct PreparedSamples {
-  std::vector<Vec3> points;
-  std::vector<uint32_t> fixed_bin;
+  HostVec<Vec3> points;
+  HostVec<uint32_t> fixed_bin;
   MiBinning binning;
 };
 
 // linear.rs:251-279 `prepare_samples` with MiBinning::new (mi.rs:209-236), from the fixed image's
 // nearest-voxel and linear values at the raw sample points (NaN outside) and the moving range:
 // samples whose linear fixed value is not finite are dropped, the rest get their fixed bin.
-inline PreparedSamples prepare_samples(const std::vector<Vec3>& samples, const std::vector<double>& nearest,
-                                       const std::vector<double>& linear, std::pair<double, double> moving_range,
+inline PreparedSamples prepare_samples(const HostVec<Vec3>& samples, const HostVec<double>& nearest,
+                                       const HostVec<double>& linear, std::pair<double, double> moving_range,
                                        size_t bins) {
   HostTimer host_timer("prepare_samples");
   if (bins <= 4) invalid("mattes", "at least five histogram bins are required");
@@ -243,7 +243,7 @@ LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, S
     return metric_sample_points(grid, recipe.sampling_percentage, s);
   };
   // every level that samples starts drawing at once, each on its own worker thread
-  std::vector<std::future<std::vector<Vec3>>> prefetched(recipe.levels());
+  std::vector<std::future<HostVec<Vec3>>> prefetched(recipe.levels());
   for (size_t level = 1; level < recipe.levels(); ++level)
     if (needs_samples(level))
       prefetched[level] = std::async(std::launch::async, draw, level_grid(level), uint32_t(recipe.seed + level));
@@ -252,7 +252,7 @@ LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, S
     const auto t0 = std::chrono::steady_clock::now();
     const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing);
     const Grid grid = level_grid(level);
-    std::optional<std::vector<Vec3>> ready;
+    std::optional<HostVec<Vec3>> ready;
     if (prefetched[level].valid()) {
       HostTimer wait("sample_wait");
       ready = prefetched[level].get();
@@ -272,14 +272,14 @@ LinearStageResult run_linear_stage(Engine& engine, const LinearRecipe& recipe, S
       tr.samples = levels_cache->sample_counts[level];
     } else {
       const auto images = cache.get(0, sigma, recipe.sigma_units);
-      std::vector<Vec3> raw;
+      HostVec<Vec3> raw;
       if (ready) {
         seeds.next_seed();
         raw = std::move(*ready);
       } else {
         raw = metric_sample_points(grid, recipe.sampling_percentage, seeds);
       }
-      std::vector<double> nearest, linear;
+      HostVec<double> nearest, linear;
       cache.backend().sample_values(*images.fixed, raw, nearest, linear);
       double lo = 0, hi = 0;
       const size_t observed = fixed_sample_range(nearest, lo, hi);

@@ -132,7 +132,7 @@ Image preprocess_volume(const V& vol, float lower, float upper, double out_min =
   if (!(hi > lo)) invalid("preprocess", "degenerate intensity window");
   const double scale = (out_max - out_min) / (hi - lo);
   const double shift = out_min - lo * scale;
-  std::vector<double> out(vol.data.size());
+  HostVec<double> out(vol.data.size());
   parallel_for(out.size(), [&](size_t b, size_t e) {
     for (size_t i = b; i < e; ++i) {
       const double x = vol.data[i];

@@ -144,7 +144,7 @@ struct Deriche {
 };
 
 // Filters every line of `data` along `axis` in place, lines in parallel (pyramid.rs:366-419).
-inline void deriche_axis(std::vector<double>& data, const std::array<size_t, 3>& size, int axis, const Deriche& k) {
+inline void deriche_axis(HostVec<double>& data, const std::array<size_t, 3>& size, int axis, const Deriche& k) {
   const size_t nx = size[0], ny = size[1], nz = size[2];
   const size_t ln = size[axis];
   const size_t stride = axis == 0 ? 1 : axis == 1 ? nx : nx * ny;
@@ -166,7 +166,7 @@ inline void deriche_axis(std::vector<double>& data, const std::array<size_t, 3>&
 
 // pyramid.rs:429-437: axes z, x, y with per-axis sigma in mm; every axis needs 4 voxels.
 inline Image smooth_recursive_gaussian(const Image& vol, const Vec3& sigma_mm) {
-  std::vector<double> out = vol.data;
+  HostVec<double> out = vol.data;
   for (int axis : {2, 0, 1}) {
     if (vol.grid.size[axis] < 4) invalid("smoothing", "every axis needs at least 4 voxels");
     if (!(sigma_mm[axis] > 0.0)) invalid("smoothing", "sigma must be greater than zero");

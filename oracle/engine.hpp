@@ -19,8 +19,8 @@ class CpuBackend {
   Image upload_image(const Image& v) { return v; }
   Image smooth_image(const Image& v, double sigma, SigmaUnits units) { return smooth_for_level(v, sigma, units); }
   // mi.rs:161-207 and interp.rs:43-87: nearest-voxel and linear values, NaN outside.
-  void sample_values(const Image& img, const std::vector<Vec3>& points, std::vector<double>& nearest,
-                     std::vector<double>& linear) {
+  void sample_values(const Image& img, const HostVec<Vec3>& points, HostVec<double>& nearest,
+                     HostVec<double>& linear) {
     nearest.assign(points.size(), std::nan(""));
     linear.assign(points.size(), std::nan(""));
     for (size_t i = 0; i < points.size(); ++i) {
@@ -128,12 +128,12 @@ class CpuLinearEngine {
       return {e.value, e.derivative};
     }
   };
-  Level prepare_level(std::shared_ptr<const Image> moving, const std::vector<Vec3>& points,
-                      const std::vector<uint32_t>& fixed_bin, const MiBinning& b) {
+  Level prepare_level(std::shared_ptr<const Image> moving, const HostVec<Vec3>& points,
+                      const HostVec<uint32_t>& fixed_bin, const MiBinning& b) {
     Level l;
     l.moving = *moving;
-    l.points = points;
-    l.fixed_bin = fixed_bin;
+    l.points.assign(points.begin(), points.end());
+    l.fixed_bin.assign(fixed_bin.begin(), fixed_bin.end());
     l.binning = Binning::from_ranges(b.bins, b.fixed_min, b.fixed_max, b.moving_min, b.moving_max);
     return l;
   }

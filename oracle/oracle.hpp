@@ -384,7 +384,8 @@ struct Binning {
   size_t fixed_bin(double v) const { return clamp_bin(v / fixed_bin_size - fixed_normalized_min); }
 };
 
-inline bool finite_range(const std::vector<double>& v, const std::vector<uint8_t>* keep, double& lo, double& hi) {
+template <typename Vec>
+inline bool finite_range(const Vec& v, const std::vector<uint8_t>* keep, double& lo, double& hi) {
   bool any = false;
   for (size_t i = 0; i < v.size(); ++i) {
     if (keep && !(*keep)[i]) continue;
