@@ -451,6 +451,17 @@ HImage HipBackend::warp_mask_through_field(const HImage& mask, const HField& fie
 
 // ----------------------------------------------------------------------------- CC (a5, a6) ---
 
+// The z run of a CC block: the longest of max_tile, max_tile / 2, ..., 1 planes whose grid still
+// has 512 blocks (two per CU), so that a small level is not left to a few long-running blocks.
+static int cc_tile_for(const Dims& d, int max_tile) {
+  int t = std::max(1, max_tile);
+  for (; t > 1; t /= 2) {
+    const dim3 g = cc_grid(d, t);
+    if (size_t(g.x) * g.y * g.z >= 512) break;
+  }
+  return t;
+}
+
 MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving, const HImage* valid, size_t radius,
                                        bool want_norms) {
   if (fixed.grid != moving.grid) invalid("hip backend", "cc: the images are not on one grid");
@@ -467,7 +478,7 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
   a.valid = valid ? valid->buf.get() : nullptr;
   a.d = dims_of(g);
   a.radius = int(radius);
-  a.tile_z = std::max(1, cc_tile_z);
+  a.tile_z = cc_tile_for(a.d, cc_tile_z);
   a.qf = fixed_point_scale(fixed.absmax);
   a.qm = fixed_point_scale(moving.absmax);
   a.inv_qf = 1.0f / a.qf;
@@ -545,7 +556,7 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
   a.valid = nullptr;
   a.d = dims_of(g);
   a.radius = int(radius);
-  a.tile_z = std::max(1, cc_tile_z);
+  a.tile_z = cc_tile_for(a.d, cc_tile_z);
   a.qf = fixed_point_scale(fixed.absmax);
   a.qm = fixed_point_scale(moving.absmax);
   a.inv_qf = 1.0f / a.qf;

@@ -12,6 +12,25 @@ namespace {
 
 constexpr float kEps64 = 2.220446049250313e-16f;  // f64 machine epsilon, the reference's guard
 
+// The workgroups of a dispatch go to the 8 XCDs of the MI355X in turn (workgroup b to XCD b % 8),
+// and every XCD has an L2 of its own. xcd_slab renumbers them so that XCD x owns one contiguous
+// range of logical ids: tiles that are neighbours in memory then run on one XCD, and a stencil or
+// gather that reaches into the neighbouring tile finds the lines in that XCD's L2 instead of
+// fetching them from HBM once more. A bijection on [0, n) whatever the placement really is.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_slab(unsigned b, unsigned n) {
+  const unsigned per = n / kXcds, extra = n % kXcds;
+  const unsigned x = b % kXcds, k = b / kXcds;
+  return x * per + (x < extra ? x : extra) + k;
+}
+
+// The logical (x, y, z) block of a 3D grid under xcd_slab (x fastest, as the dispatch order).
+__device__ __forceinline__ uint3 xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned l = xcd_slab(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+  return make_uint3(l % gx, (l / gx) % gy, l / (gx * gy));
+}
+
 __device__ __forceinline__ long long lin(const Dims& d, int i, int j, int k) {
   return (long long)i + (long long)d.nx * ((long long)j + (long long)d.ny * k);
 }

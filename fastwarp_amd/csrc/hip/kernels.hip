@@ -57,7 +57,8 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
 
   const Dims d = a.d;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * kCcTX, y0 = blockIdx.y * kCcTY, z0 = blockIdx.z * a.tile_z;
+  const uint3 bl = xcd_block();
+  const int x0 = bl.x * kCcTX, y0 = bl.y * kCcTY, z0 = bl.z * a.tile_z;
   const int z1 = min(z0 + a.tile_z, d.nz);
   const long long plane = (long long)d.nx * d.ny;
   const long long n3 = d.n();
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
   const float total = block_sum(cc_sum);
   const float cnt = block_sum((float)counted);
   if (tid == 0) {
-    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int b = bl.x + gridDim.x * (bl.y + gridDim.y * bl.z);
     a.partial_cc[b] = total;
     a.partial_count[b] = (int)cnt;
   }
@@ -468,9 +469,10 @@ void launch_axpy(float* acc, const float* x, long long count, float a, hipStream
 // buffer and zero outside; in mask mode the nearest voxel decides {0, 1}.
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_warp(WarpArgs a) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = blockIdx.z;
+  const uint3 bl = xcd_block();
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  const int k = bl.z;
   if (i >= a.od.nx || j >= a.od.ny) return;
   const long long g = lin(a.od, i, j, k);
   const long long n = a.od.n();
@@ -517,9 +519,10 @@ void launch_warp(const WarpArgs& a, hipStream_t s) {
 
 // Field resampling onto another grid (field_ops.rs:580-605): linear, zero outside the old buffer.
 __global__ __launch_bounds__(256) void k_resample(ResampleArgs a) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = blockIdx.z;
+  const uint3 bl = xcd_block();
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  const int k = bl.z;
   if (i >= a.od.nx || j >= a.od.ny) return;
   const long long g = lin(a.od, i, j, k);
   const long long n = a.od.n();
@@ -569,9 +572,10 @@ struct L9 {
 
 __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u, const float* __restrict__ w, Dims d, L9 l,
                                                  float* __restrict__ out) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = blockIdx.z;
+  const uint3 bl = xcd_block();
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  const int k = bl.z;
   if (i >= d.nx || j >= d.ny) return;
   const long long g = lin(d, i, j, k), n = d.n();
   float ox, oy, oz;
@@ -607,8 +611,9 @@ __global__ __launch_bounds__(256) void k_gauss_xy(const float* __restrict__ in, 
   __shared__ float xs[kG3TY + 2 * kG3MaxR][kG3TX];
   const int rx = t.rx, ry = t.ry;
   const int LX = kG3TX + 2 * rx, LY = kG3TY + 2 * ry;
-  const int comp = blockIdx.z / d.nz, z = blockIdx.z - comp * d.nz;
-  const int x0 = blockIdx.x * kG3TX, y0 = blockIdx.y * kG3TY;
+  const uint3 bl = xcd_block();
+  const int comp = bl.z / d.nz, z = bl.z - comp * d.nz;
+  const int x0 = bl.x * kG3TX, y0 = bl.y * kG3TY;
   const long long n = d.n();
   const float* src = in + comp * n + (long long)d.nx * d.ny * z;
   float* dst = out + comp * n + (long long)d.nx * d.ny * z;
@@ -673,9 +678,10 @@ __global__ __launch_bounds__(256) void k_gauss_z(const float* __restrict__ in, c
                                                  float* __restrict__ out, Dims d, GaussTaps3 t, int zrun, float w1,
                                                  float w2) {
   constexpr int W = 2 * RZ + 1;
-  const int comp = blockIdx.z % 3, chunk = blockIdx.z / 3;
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const uint3 bl = xcd_block();
+  const int comp = bl.z % 3, chunk = bl.z / 3;
+  const int x = bl.x * 64 + (threadIdx.x & 63);
+  const int y = bl.y * 4 + (threadIdx.x >> 6);
   if (x >= d.nx || y >= d.ny) return;
   const int z0 = chunk * zrun, z1 = min(z0 + zrun, d.nz);
   const long long n = d.n(), plane = (long long)d.nx * d.ny;
@@ -847,9 +853,10 @@ void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const De
 __global__ __launch_bounds__(256) void k_gauss_axis(const float* __restrict__ in, float* __restrict__ out, Dims d, int axis,
                                                     const float* __restrict__ taps, int radius,
                                                     const float* __restrict__ orig, float w1, float w2, int final_pass) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = blockIdx.z;
+  const uint3 bl = xcd_block();
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  const int k = bl.z;
   if (i >= d.nx || j >= d.ny) return;
   const long long g = lin(d, i, j, k), n = d.n();
   if (final_pass && (i == 0 || j == 0 || k == 0 || i == d.nx - 1 || j == d.ny - 1 || k == d.nz - 1)) {
@@ -936,13 +943,28 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   s = t;
 }
 
-__device__ __forceinline__ void inv_write_partials(float s, float m, float* partials, int blocks) {
+__device__ __forceinline__ void inv_write_partials(float s, float m, float* partials, int blocks, unsigned chunk) {
   const float bs = block_sum(s);
   const float bm = block_max(m);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = bs;
-    partials[blocks + blockIdx.x] = bm;
+    partials[chunk] = bs;
+    partials[blocks + chunk] = bm;
   }
+}
+
+// The voxels of block b: the contiguous range of its logical chunk (xcd_slab), in steps of
+// kInvThreads * kInvVoxels; chunk lengths are multiples of kInvThreads.
+struct InvRange {
+  unsigned chunk;
+  long long lo, hi;
+};
+__device__ __forceinline__ InvRange inv_range(long long n) {
+  const long long len = ((n + gridDim.x - 1) / gridDim.x + kInvThreads - 1) / kInvThreads * kInvThreads;
+  InvRange r;
+  r.chunk = xcd_slab(blockIdx.x, gridDim.x);
+  r.lo = (long long)r.chunk * len;
+  r.hi = r.lo + len < n ? r.lo + len : n;
+  return r;
 }
 
 __device__ __forceinline__ void inv_coords(const Dims& d, long long g, int& i, int& j, int& k) {
@@ -956,13 +978,13 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
                                                            float* __restrict__ res, Dims d, L9 l, float3 inv_sp,
                                                            InvertState* state, float* partials, int max_iterations) {
   const long long n = d.n();
-  const long long span = (long long)gridDim.x * kInvThreads;
+  const InvRange rg = inv_range(n);
   float s = 0.0f, c = 0.0f, m = 0.0f;
-  for (long long base = (long long)blockIdx.x * kInvThreads + threadIdx.x; base < n; base += span * kInvVoxels) {
+  for (long long base = rg.lo + threadIdx.x; base < rg.hi; base += kInvThreads * kInvVoxels) {
 #pragma unroll
     for (int v = 0; v < kInvVoxels; ++v) {
-      const long long g = base + v * span;
-      if (g < n) {
+      const long long g = base + v * kInvThreads;
+      if (g < rg.hi) {
         int i, j, k;
         inv_coords(d, g, i, j, k);
         float ex, ey, ez;
@@ -976,7 +998,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
       }
     }
   }
-  inv_write_partials(s, m, partials, gridDim.x);
+  inv_write_partials(s, m, partials, gridDim.x, rg.chunk);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     state->mean_error = 0.0f;
     state->max_error = 0.0f;
@@ -1032,15 +1054,15 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
   }
   const float eps = step == 0 ? 0.75f : 0.5f;
   const float limit = eps * mx;
-  const long long span = (long long)gridDim.x * kInvThreads;
+  const InvRange rg = inv_range(n);
   s = 0.0f;
   c = 0.0f;
   m = 0.0f;
-  for (long long base = (long long)blockIdx.x * kInvThreads + threadIdx.x; base < n; base += span * kInvVoxels) {
+  for (long long base = rg.lo + threadIdx.x; base < rg.hi; base += kInvThreads * kInvVoxels) {
 #pragma unroll
     for (int v = 0; v < kInvVoxels; ++v) {
-      const long long g = base + v * span;
-      if (g < n) {
+      const long long g = base + v * kInvThreads;
+      if (g < rg.hi) {
         int i, j, k;
         inv_coords(d, g, i, j, k);
         const float wx = psi[g], wy = psi[g + n], wz = psi[g + 2 * n];
@@ -1070,7 +1092,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
       }
     }
   }
-  if (next) inv_write_partials(s, m, out, blocks);
+  if (next) inv_write_partials(s, m, out, blocks, rg.chunk);
 }
 
 int inv_blocks(const Dims& d) {

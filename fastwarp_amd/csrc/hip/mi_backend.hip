@@ -141,20 +141,23 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
   a.kind = t.optimised.kind == ParamKind::Affine ? 1 : 0;
   if (t.optimised.kind == ParamKind::Euler3d && t.optimised.zyx)
     invalid("hip backend", "the ZYX Euler order is not supported by the device derivative");
-  const int blocks = mi_sampled_blocks(int(count_));
-  // the block partials go straight to coherent host memory; the sync makes them visible
-  void* dev_partial = nullptr;
-  auto* host = static_cast<double*>(ctx_->mapped(1, size_t(blocks) * 12 * sizeof(double), &dev_partial));
-  a.partial = static_cast<double*>(dev_partial);
+  const int blocks = mi_deriv_blocks(int(count_));
+  const size_t np = t.optimised.count();
+  a.partial = static_cast<double*>(ctx_->scratch(7, size_t(blocks) * 12 * sizeof(double)));
   {
     Timed tm(*ctx_, "k_mi_sampled_deriv", 20.0 * double(count_));
     launch_mi_sampled_deriv(a, blocks, s);
   }
+  // the blocks' partials are added on the device, straight into coherent host memory; the sync
+  // makes the sums visible
+  void* dev_sum = nullptr;
+  auto* host = static_cast<double*>(ctx_->mapped(1, 12 * sizeof(double), &dev_sum));
+  {
+    Timed tm(*ctx_, "k_mi_deriv_finish", 8.0 * 12.0 * double(blocks));
+    launch_mi_deriv_finish(a.partial, blocks, int(np), static_cast<double*>(dev_sum), s);
+  }
   ctx_->sync();
-  const size_t np = t.optimised.count();
-  std::vector<double> d(np, 0.0);
-  for (int bl = 0; bl < blocks; ++bl)
-    for (size_t k = 0; k < np; ++k) d[k] += host[size_t(bl) * 12 + k];
+  std::vector<double> d(host, host + np);
   (void)bins;
   return {value, d};
 }

@@ -135,6 +135,23 @@ __device__ __forceinline__ void moving_gradient(const float* __restrict__ img, c
   g2 = central_difference_at(img, d, pr, 2, cx, cy, cz);
 }
 
+// The samples of block b: one contiguous range per logical block (xcd_slab). The samples are in
+// raster order of the fixed grid, so a block's moving-image gathers stay in one region and
+// neighbouring regions run on the same XCD.
+struct SampleRange {
+  unsigned chunk;
+  int lo, hi;
+};
+__device__ __forceinline__ SampleRange sample_range(int count) {
+  const int len = ((count + (int)gridDim.x - 1) / (int)gridDim.x + 255) / 256 * 256;
+  SampleRange r;
+  r.chunk = xcd_slab(blockIdx.x, gridDim.x);
+  const long long lo = (long long)r.chunk * len;
+  r.lo = lo < count ? (int)lo : count;
+  r.hi = lo + len < count ? (int)(lo + len) : count;
+  return r;
+}
+
 struct SampleEval {
   bool ok;
   int fbin, mbin;
@@ -159,9 +176,17 @@ __device__ __forceinline__ SampleEval eval_sample(const MiSampledArgs& a, float4
 
 }  // namespace
 
+// One sample per thread while that gives at most 512 blocks (the small levels: their launches are
+// one chain of dependent loads per thread, so the chain is kept short), several per thread beyond.
 int mi_sampled_blocks(int count) {
-  int b = (count + 2047) / 2048;
+  int b = (count + 255) / 256;
   return b < 1 ? 1 : (b > 512 ? 512 : b);
+}
+
+// The derivative kernel's partials are reduced on the device, so its grid may be wider.
+int mi_deriv_blocks(int count) {
+  int b = (count + 255) / 256;
+  return b < 1 ? 1 : (b > kMiDerivMaxBlocks ? kMiDerivMaxBlocks : b);
 }
 
 // The block's histogram (LDS, 64-bit atomics) is written to its own slot of `hist_slots`; a second
@@ -173,7 +198,8 @@ __global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
   for (int i = threadIdx.x; i < cells + 1; i += blockDim.x) h[i] = 0ull;
   __syncthreads();
   unsigned int mine = 0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < a.count; s += gridDim.x * blockDim.x) {
+  const SampleRange rg = sample_range(a.count);
+  for (int s = rg.lo + threadIdx.x; s < rg.hi; s += blockDim.x) {
     const SampleEval e = eval_sample(a, a.samples[s]);
     if (!e.ok) continue;
     ++mine;
@@ -234,14 +260,15 @@ void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells,
 // (mi.rs:405-433 with the ratio weights of :752-757), accumulated in FP64.
 __global__ __launch_bounds__(256) void k_mi_sampled_deriv(MiSampledArgs a) {
   __shared__ float ratio[kMiMaxBins * kMiMaxBins];
-  __shared__ double red[256];
+  __shared__ double red[4 * 12];
   const int bins = a.bins;
   for (int i = threadIdx.x; i < bins * bins; i += blockDim.x) ratio[i] = a.ratio[i];
   __syncthreads();
   double acc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < a.count; s += gridDim.x * blockDim.x) {
+  const SampleRange rg = sample_range(a.count);
+  for (int s = rg.lo + threadIdx.x; s < rg.hi; s += blockDim.x) {
     const float4 smp = a.samples[s];
     const SampleEval e = eval_sample(a, smp);
     if (!e.ok) continue;
@@ -289,21 +316,64 @@ __global__ __launch_bounds__(256) void k_mi_sampled_deriv(MiSampledArgs a) {
       acc[5] += (double)(w * q2);
     }
   }
+  // fixed tree: a butterfly within each wave, then the four wave sums in order
   const int np = a.kind == 1 ? 12 : 6;
-  for (int k = 0; k < np; ++k) {
-    red[threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) a.partial[blockIdx.x * 12 + k] = red[0];
-    __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    if (k >= np) break;
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[w * 12 + k] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < np) {
+    const int k = threadIdx.x;
+    a.partial[rg.chunk * 12 + k] = (red[k] + red[12 + k]) + (red[24 + k] + red[36 + k]);
   }
 }
 
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s) {
   launch(k_mi_sampled_deriv, dim3(blocks), dim3(256), 0, s, a);
+}
+
+// out[k] = sum over the blocks of partial[b * 12 + k], k < np, in a fixed order: 16 waves take
+// consecutive slices of the blocks, each lane adds its blocks' rows (all loads in flight at
+// once), a butterfly adds each wave's lanes, and the 16 wave sums are added in order.
+__global__ __launch_bounds__(1024) void k_mi_deriv_finish(const double* __restrict__ partial, int blocks, int np,
+                                                          double* __restrict__ out) {
+  __shared__ double wsum[16][12];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int per = (blocks + 15) / 16;
+  const int b0 = w * per, b1 = min(blocks, b0 + per);
+  double acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  for (int b = b0 + lane; b < b1; b += 64) {
+    const double2* row = reinterpret_cast<const double2*>(partial + (size_t)b * 12);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const double2 v = row[q];
+      acc[2 * q] += v.x;
+      acc[2 * q + 1] += v.y;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) wsum[w][k] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < np) {
+    double t = 0.0;
+    for (int q = 0; q < 16; ++q) t += wsum[q][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, hipStream_t s) {
+  launch(k_mi_deriv_finish, dim3(1), dim3(1024), 0, s, partial, blocks, np, out);
 }
 
 // ------------------------------------------------------------------------------ dense pair ---

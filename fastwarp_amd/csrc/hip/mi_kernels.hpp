@@ -41,6 +41,10 @@ void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s);
 void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, hipStream_t s);
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
 int mi_sampled_blocks(int count);
+constexpr int kMiDerivMaxBlocks = 2048;
+int mi_deriv_blocks(int count);
+// out[k] (k < np) = the blocks' derivative partials added in a fixed order.
+void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, hipStream_t s);
 
 // Dense pair on one grid (mi.rs:578-676 in both directions, wgpu_backend/mi/dense.rs:241-487).
 struct MiDenseArgs {

@@ -240,43 +240,50 @@ inline unsigned host_threads() {
 // fixed function of n and `grain`, so per-chunk results combined in chunk order do not depend on
 // the thread count.
 //
-// The loops run on one process-wide pool of host_threads() - 1 parked workers plus the caller (a
-// new thread per loop would cost tens of microseconds per thread). One loop runs on the pool at a
-// time; a loop started while the pool is busy (from another thread, or from inside a loop body)
-// gets threads of its own.
+// The loops run on one process-wide pool of host_threads() - 1 parked workers (a new thread per
+// loop would cost tens of microseconds per thread). Every loop is a job in the pool's list; the
+// calling thread works on its own job while idle workers take chunks of the oldest job with
+// chunks left, so loops started on several threads at once (the drivers' main thread and the
+// sample prefetch) share the workers instead of each starting a full set of threads. A loop body
+// may itself run a loop.
 class HostPool {
  public:
   static HostPool& get() {
     static HostPool* pool = new HostPool(host_threads());  // never destroyed: its workers stay parked
     return *pool;
   }
-  // body(c) for every chunk c < chunks; false (nothing run) if the pool is busy. Every worker
-  // takes part in every loop (those that find no chunk left return at once), so no worker can
-  // still hold a loop's body once try_run has returned.
-  bool try_run(size_t chunks, const std::function<void(size_t)>& body) {
-    std::unique_lock<std::mutex> run(run_mu_, std::try_to_lock);
-    if (!run.owns_lock()) return false;
-    job_ = &body;
-    chunks_ = chunks;
-    next_.store(0);
-    error_ = nullptr;
-    busy_.store(unsigned(workers_.size()));
+  // body(c) for every chunk c < chunks; the first exception a chunk throws is rethrown here.
+  void run(size_t chunks, const std::function<void(size_t)>& body) {
+    Job job;
+    job.body = &body;
+    job.chunks = chunks;
     {
       std::lock_guard<std::mutex> lk(mu_);
+      jobs_.push_back(&job);
       generation_.fetch_add(1, std::memory_order_release);
     }
     wake_.notify_all();
-    work();
-    if (!spin_until([&] { return busy_.load(std::memory_order_acquire) == 0; })) {
-      std::unique_lock<std::mutex> lk(mu_);
-      done_.wait(lk, [&] { return busy_.load(std::memory_order_acquire) == 0; });
+    work_on(job);
+    // every chunk is claimed now; wait for the workers still running one of them
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      jobs_.erase(std::find(jobs_.begin(), jobs_.end(), &job));
     }
-    job_ = nullptr;
-    if (error_) std::rethrow_exception(error_);
-    return true;
+    if (!spin_until([&] { return job.users.load(std::memory_order_acquire) == 0; })) {
+      std::unique_lock<std::mutex> lk(mu_);
+      done_.wait(lk, [&] { return job.users.load(std::memory_order_acquire) == 0; });
+    }
+    if (job.error) std::rethrow_exception(job.error);
   }
 
  private:
+  struct Job {
+    const std::function<void(size_t)>* body = nullptr;
+    size_t chunks = 0;
+    std::atomic<size_t> next{0};
+    std::atomic<unsigned> users{0};  // workers inside work_on (raised under mu_)
+    std::exception_ptr error;
+  };
   explicit HostPool(unsigned threads) {
     for (unsigned t = 1; t < threads; ++t) workers_.emplace_back([this] { loop(); });
     for (auto& w : workers_) w.detach();
@@ -290,43 +297,52 @@ class HostPool {
       if ((i & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(100)) return ready();
     }
   }
-  void work() {
+  void work_on(Job& job) {
     for (;;) {
-      const size_t c = next_.fetch_add(1);
-      if (c >= chunks_) return;
+      const size_t c = job.next.fetch_add(1);
+      if (c >= job.chunks) return;
       try {
-        (*job_)(c);
+        (*job.body)(c);
       } catch (...) {
         std::lock_guard<std::mutex> lk(mu_);
-        if (!error_) error_ = std::current_exception();
-        next_.store(chunks_);
+        if (!job.error) job.error = std::current_exception();
+        job.next.store(job.chunks);
       }
     }
   }
   void loop() {
     uint64_t seen = 0;
     for (;;) {
+      Job* job = nullptr;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (Job* j : jobs_)
+          if (j->next.load() < j->chunks) {
+            job = j;
+            break;
+          }
+        if (job) job->users.fetch_add(1, std::memory_order_relaxed);
+        else seen = generation_.load(std::memory_order_acquire);
+      }
+      if (job) {
+        work_on(*job);
+        if (job->users.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+          std::lock_guard<std::mutex> lk(mu_);
+          done_.notify_all();
+        }
+        continue;
+      }
       if (!spin_until([&] { return generation_.load(std::memory_order_acquire) != seen; })) {
         std::unique_lock<std::mutex> lk(mu_);
         wake_.wait(lk, [&] { return generation_.load(std::memory_order_acquire) != seen; });
       }
-      seen = generation_.load(std::memory_order_acquire);
-      work();
-      if (busy_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
-        std::lock_guard<std::mutex> lk(mu_);
-        done_.notify_one();
-      }
     }
   }
-  std::mutex run_mu_, mu_;
+  std::mutex mu_;
   std::condition_variable wake_, done_;
   std::vector<std::thread> workers_;
-  const std::function<void(size_t)>* job_ = nullptr;
-  size_t chunks_ = 0;
-  std::atomic<size_t> next_{0};
-  std::atomic<unsigned> busy_{0};
+  std::vector<Job*> jobs_;
   std::atomic<uint64_t> generation_{0};
-  std::exception_ptr error_;
 };
 
 inline void parallel_chunks(size_t n, size_t grain, const std::function<void(size_t, size_t, size_t)>& body) {
@@ -334,25 +350,11 @@ inline void parallel_chunks(size_t n, size_t grain, const std::function<void(siz
   grain = std::max<size_t>(grain, 1);
   const size_t chunks = (n + grain - 1) / grain;
   auto chunk = [&](size_t c) { body(c, c * grain, std::min(n, (c + 1) * grain)); };
-  const unsigned threads = static_cast<unsigned>(std::min<size_t>(host_threads(), chunks));
-  if (threads <= 1) {
+  if (chunks == 1 || host_threads() <= 1) {
     for (size_t c = 0; c < chunks; ++c) chunk(c);
     return;
   }
-  if (HostPool::get().try_run(chunks, chunk)) return;
-  std::atomic<size_t> next{0};
-  auto worker = [&] {
-    for (;;) {
-      const size_t c = next.fetch_add(1);
-      if (c >= chunks) return;
-      chunk(c);
-    }
-  };
-  std::vector<std::thread> own;
-  own.reserve(threads - 1);
-  for (unsigned t = 1; t < threads; ++t) own.emplace_back(worker);
-  worker();
-  for (auto& t : own) t.join();
+  HostPool::get().run(chunks, chunk);
 }
 
 // parallel_chunks with a default grain that gives every thread several chunks.

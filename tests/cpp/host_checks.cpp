@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <thread>
 
 #include "../../fastwarp_amd/csrc/host/gauss_taps.hpp"
 #include "../../fastwarp_amd/csrc/host/linear.hpp"
@@ -266,7 +267,53 @@ static void preprocess() {
   ok("preprocess");
 }
 
+// The host loops: every element of every loop is visited exactly once when loops run from several
+// threads at once and from inside other loops' bodies, and a body's exception reaches the caller.
+static void host_loops() {
+  auto loop_sum = [](size_t n, size_t grain, bool nested) {
+    std::vector<uint64_t> part((n + grain - 1) / grain, 0);
+    parallel_chunks(n, grain, [&](size_t c, size_t b, size_t e) {
+      uint64_t t = 0;
+      for (size_t i = b; i < e; ++i) t += i;
+      if (nested && c % 7 == 0) {
+        std::vector<uint64_t> inner(8, 0);
+        parallel_chunks(800, 100, [&](size_t ic, size_t ib, size_t ie) {
+          for (size_t i = ib; i < ie; ++i) inner[ic] += 1;
+        });
+        for (uint64_t v : inner) t += v - 100;  // adds zero when every inner element ran once
+      }
+      part[c] = t;
+    });
+    uint64_t total = 0;
+    for (uint64_t v : part) total += v;
+    return total;
+  };
+  std::vector<std::thread> callers;
+  std::atomic<int> bad{0};
+  for (int t = 0; t < 4; ++t)
+    callers.emplace_back([&, t] {
+      for (int r = 0; r < 200; ++r) {
+        const size_t n = 1000 + size_t(r * 37 + t * 11);
+        if (loop_sum(n, 16 + size_t(r % 5), (r + t) % 3 == 0) != uint64_t(n) * (n - 1) / 2) bad.fetch_add(1);
+      }
+    });
+  for (auto& c : callers) c.join();
+  CHECK(bad.load() == 0, "concurrent and nested host loops");
+  bool caught = false;
+  try {
+    parallel_chunks(1000, 10, [](size_t c, size_t, size_t) {
+      if (c == 57) throw Error("test", "chunk 57");
+    });
+  } catch (const Error& e) {
+    caught = e.message == "chunk 57";
+  }
+  CHECK(caught, "a body's exception reaches the caller");
+  CHECK(loop_sum(5000, 50, false) == 5000ull * 4999 / 2, "the pool works after an exception");
+  ok("host_loops");
+}
+
 int main() {
+  host_loops();
   mt19937();
   regular_sampling();
   shrink();
