@@ -149,9 +149,9 @@ void DeviceContext::settle() {
 volatile uint32_t* DeviceContext::mailbox_host() {
   if (!mailbox_) {
     void* p = nullptr;
-    check(hipHostMalloc(&p, 256, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc mailbox");
+    check(hipHostMalloc(&p, kMailboxBytes, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc mailbox");
     mailbox_ = static_cast<uint32_t*>(p);
-    std::memset(mailbox_, 0, 256);
+    std::memset(mailbox_, 0, kMailboxBytes);
     void* d = nullptr;
     check(hipHostGetDevicePointer(&d, p, 0), "hipHostGetDevicePointer");
     mailbox_dev_ = static_cast<uint32_t*>(d);
@@ -162,6 +162,24 @@ volatile uint32_t* DeviceContext::mailbox_host() {
 uint32_t* DeviceContext::mailbox_device() {
   (void)mailbox_host();
   return mailbox_dev_;
+}
+
+void DeviceContext::wait_done(int count, uint32_t tag) {
+  HostTimer t("sync_wait");
+  if (count < 1 || count > kDoneWords) invalid("device", "completion word count outside 1.." + std::to_string(kDoneWords));
+  volatile uint32_t* w = mailbox_host() + kDoneBase;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int b = 0; b < count; ++b) {
+    for (unsigned spins = 1; w[b] != tag; ++spins) {
+      if ((spins & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+        // late: wait for the stream (which reports a device error, if any) and look once more
+        check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        if (w[b] != tag) invalid("device", "a kernel completed without writing its completion word");
+        break;
+      }
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 void DeviceContext::sync() {

@@ -60,6 +60,12 @@ class DeviceContext : public HostSpanSink {
   volatile uint32_t* mailbox_host();
   uint32_t* mailbox_device();
   uint32_t next_tag() { return ++tag_; }
+  // Completion words (in the mailbox): a kernel whose results go to coherent host memory stores
+  // its tag into words [0, count) with a system-scope release once they are written, and
+  // wait_done() spins until every word holds the tag: cheaper than a stream synchronisation.
+  static constexpr int kDoneWords = 512;
+  uint32_t* done_words_device() { return mailbox_device() + kDoneBase; }
+  void wait_done(int count, uint32_t tag);
   void sync();
   // Device copy of the Gaussian taps of (variance, axis length), cached.
   const float* taps(double variance, size_t length, int& radius);
@@ -108,6 +114,8 @@ class DeviceContext : public HostSpanSink {
     size_t bytes = 0;
   };
   Mapped mapped_[4];
+  static constexpr int kDoneBase = 64;  // words 0..63: the inversion's progress and stop words
+  static constexpr size_t kMailboxBytes = sizeof(uint32_t) * (kDoneBase + kDoneWords);
   uint32_t* mailbox_ = nullptr;
   uint32_t* mailbox_dev_ = nullptr;
   uint32_t tag_ = 0;

@@ -97,14 +97,16 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
     Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
     launch_mi_sampled_hist(a, blocks, s);
   }
-  // the sums go straight to coherent host memory; the sync makes them visible
+  // the sums go straight to coherent host memory; the completion words say when they are there
   void* dev_hist = nullptr;
   auto* host = static_cast<unsigned long long*>(ctx_->mapped(0, cells * sizeof(unsigned long long), &dev_hist));
+  const uint32_t tag = ctx_->next_tag();
   {
     Timed tm(*ctx_, "k_mi_sum_slots", 8.0 * double(cells) * double(blocks + 1));
-    launch_mi_sum_slots(a.hist_slots, blocks, int(cells), static_cast<unsigned long long*>(dev_hist), s);
+    launch_mi_sum_slots(a.hist_slots, blocks, int(cells), static_cast<unsigned long long*>(dev_hist),
+                        ctx_->done_words_device(), tag, s);
   }
-  ctx_->sync();
+  ctx_->wait_done(mi_sum_slots_blocks(int(cells)), tag);
   const uint64_t valid = host[cells - 1];
   HistogramFinish fin =
       finish_histogram(bins, binning_.moving_bin_size, valid, joint_from(host, bins), marginal_from(host, bins), ratio != nullptr);
@@ -148,15 +150,16 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
     Timed tm(*ctx_, "k_mi_sampled_deriv", 20.0 * double(count_));
     launch_mi_sampled_deriv(a, blocks, s);
   }
-  // the blocks' partials are added on the device, straight into coherent host memory; the sync
-  // makes the sums visible
+  // the blocks' partials are added on the device, straight into coherent host memory; the
+  // completion word says when the sums are there
   void* dev_sum = nullptr;
   auto* host = static_cast<double*>(ctx_->mapped(1, 12 * sizeof(double), &dev_sum));
+  const uint32_t tag = ctx_->next_tag();
   {
     Timed tm(*ctx_, "k_mi_deriv_finish", 8.0 * 12.0 * double(blocks));
-    launch_mi_deriv_finish(a.partial, blocks, int(np), static_cast<double*>(dev_sum), s);
+    launch_mi_deriv_finish(a.partial, blocks, int(np), static_cast<double*>(dev_sum), ctx_->done_words_device(), tag, s);
   }
-  ctx_->sync();
+  ctx_->wait_done(1, tag);
   std::vector<double> d(host, host + np);
   (void)bins;
   return {value, d};

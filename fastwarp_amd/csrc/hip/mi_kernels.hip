@@ -227,7 +227,8 @@ void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
 // takes 64 cells: lane = cell, wave w adds slots w, w + 16, ... in four independent chains (so the
 // loads of one wave overlap), and the 16 wave sums are added in LDS.
 __global__ __launch_bounds__(1024) void k_mi_sum_slots(const unsigned long long* __restrict__ slots, int nslots,
-                                                       int cells, unsigned long long* __restrict__ hist) {
+                                                       int cells, unsigned long long* __restrict__ hist, uint32_t* done,
+                                                       uint32_t tag) {
   __shared__ unsigned long long part[16][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + lane;
@@ -250,10 +251,15 @@ __global__ __launch_bounds__(1024) void k_mi_sum_slots(const unsigned long long*
     for (int q = 0; q < 16; ++q) t += part[q][lane];
     hist[c] = t;
   }
+  // the wave that wrote this block's cells releases them to the host with its completion word
+  if (threadIdx.x == 0) __hip_atomic_store(done + blockIdx.x, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, hipStream_t s) {
-  launch(k_mi_sum_slots, dim3((cells + 63) / 64), dim3(1024), 0, s, slots, nslots, cells, hist);
+int mi_sum_slots_blocks(int cells) { return (cells + 63) / 64; }
+
+void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, uint32_t* done,
+                         uint32_t tag, hipStream_t s) {
+  launch(k_mi_sum_slots, dim3(mi_sum_slots_blocks(cells)), dim3(1024), 0, s, slots, nslots, cells, hist, done, tag);
 }
 
 // Per sample: sum_k B3'(k - term) * ratio[f, k] * (grad M . dT/dtheta_p) for every parameter
@@ -341,7 +347,7 @@ void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s) 
 // consecutive slices of the blocks, each lane adds its blocks' rows (all loads in flight at
 // once), a butterfly adds each wave's lanes, and the 16 wave sums are added in order.
 __global__ __launch_bounds__(1024) void k_mi_deriv_finish(const double* __restrict__ partial, int blocks, int np,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out, uint32_t* done, uint32_t tag) {
   __shared__ double wsum[16][12];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int per = (blocks + 15) / 16;
@@ -370,10 +376,13 @@ __global__ __launch_bounds__(1024) void k_mi_deriv_finish(const double* __restri
     for (int q = 0; q < 16; ++q) t += wsum[q][threadIdx.x];
     out[threadIdx.x] = t;
   }
+  // wave 0 wrote the sums; it releases them to the host with the completion word
+  if (threadIdx.x == 0) __hip_atomic_store(done, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, hipStream_t s) {
-  launch(k_mi_deriv_finish, dim3(1), dim3(1024), 0, s, partial, blocks, np, out);
+void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, uint32_t* done, uint32_t tag,
+                            hipStream_t s) {
+  launch(k_mi_deriv_finish, dim3(1), dim3(1024), 0, s, partial, blocks, np, out, done, tag);
 }
 
 // ------------------------------------------------------------------------------ dense pair ---

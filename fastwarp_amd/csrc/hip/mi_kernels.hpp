@@ -38,13 +38,19 @@ struct MiSampledArgs {
 };
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s);
 // Adds `nslots` per-block histograms of `cells` u64 entries into `hist`.
-void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, hipStream_t s);
+// hist[c] = the slots' cell c added up (hist in coherent host memory); block b then stores `tag` into
+// done[b] with a system-scope release, b < mi_sum_slots_blocks(cells).
+int mi_sum_slots_blocks(int cells);
+void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, uint32_t* done,
+                         uint32_t tag, hipStream_t s);
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
 int mi_sampled_blocks(int count);
 constexpr int kMiDerivMaxBlocks = 2048;
 int mi_deriv_blocks(int count);
-// out[k] (k < np) = the blocks' derivative partials added in a fixed order.
-void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, hipStream_t s);
+// out[k] (k < np) = the blocks' derivative partials added in a fixed order; then done[0] = tag
+// (system-scope release).
+void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, uint32_t* done, uint32_t tag,
+                            hipStream_t s);
 
 // Dense pair on one grid (mi.rs:578-676 in both directions, wgpu_backend/mi/dense.rs:241-487).
 struct MiDenseArgs {
