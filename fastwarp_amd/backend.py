@@ -77,6 +77,7 @@ _SIGNATURES = [
     ("fw_resample_field", C.c_int, [_P, _P, C.POINTER(Grid), _PP]),
     ("fw_mi_level_create", C.c_int, [_P, _P, _D, C.POINTER(C.c_uint32), C.c_int64, C.POINTER(MiBinning), _PP]),
     ("fw_mi_level_value", C.c_int, [_P, C.POINTER(LinearTransform), _D]),
+    ("fw_mi_level_values", C.c_int, [_P, C.POINTER(LinearTransform), C.c_int64, _D]),
     ("fw_mi_level_value_and_derivative", C.c_int, [_P, C.POINTER(LinearTransform), _D, _D]),
     ("fw_mi_level_free", None, [_P]),
     ("fw_set_profiling", C.c_int, [_P, C.c_int]),
@@ -463,6 +464,13 @@ class MiLevel:
         v = C.c_double()
         self._b._check(lib().fw_mi_level_value(self.handle, C.byref(t), C.byref(v)))
         return v.value
+
+    def values(self, ts: list) -> np.ndarray:
+        """The values of several transforms in batched passes (each equals value(t))."""
+        arr = (LinearTransform * len(ts))(*ts)
+        out = np.zeros(len(ts))
+        self._b._check(lib().fw_mi_level_values(self.handle, arr, len(ts), dptr(out)))
+        return out
 
     def value_and_derivative(self, t: LinearTransform):
         v = C.c_double()

@@ -337,6 +337,20 @@ int fw_mi_level_value(fw_mi_level* level, const fw_linear_transform* t, double* 
   });
 }
 
+int fw_mi_level_values(fw_mi_level* level, const fw_linear_transform* transforms, int64_t count, double* values) {
+  if (!level) return FW_ERR_INVALID;
+  return guard(level->ctx, [&] {
+    if (count < 0) fw::invalid("mi values", "negative count");
+    if (count == 0) return;
+    need(transforms, "transforms");
+    need(values, "output");
+    std::vector<fw::LinearMovingTransform> ts;
+    for (int64_t i = 0; i < count; ++i) ts.push_back(linear_from(&transforms[i]));
+    const std::vector<double> v = level->level.values(ts);
+    std::copy(v.begin(), v.end(), values);
+  });
+}
+
 int fw_mi_level_value_and_derivative(fw_mi_level* level, const fw_linear_transform* t, double* value,
                                      double* derivative) {
   if (!level) return FW_ERR_INVALID;

@@ -306,10 +306,18 @@ class MiLevel {
   MiLevel(HipBackend& backend, const HImage& moving, const HostVec<Vec3>& points, const HostVec<uint32_t>& fixed_bin,
           const MiBinning& binning);
   double value(const LinearMovingTransform& t);
+  // The values of several transforms, batched into as few histogram passes as the LDS allows.
+  std::vector<double> values(const std::vector<LinearMovingTransform>& ts);
+  // Whether batching pays on this level: below about half a million samples a pass is bound by
+  // its latency, and an extra transform in it costs a few microseconds against a round trip of
+  // some twenty; on larger levels every transform costs as much as a pass of its own.
+  bool batch_pays() const { return count_ <= kBatchPaysSamples && mi_batch_capacity(int(binning_.bins)) > 1; }
+  static constexpr size_t kBatchPaysSamples = 500000;
   std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t);
   size_t count() const { return count_; }
 
  private:
+  void batch_values(const LinearMovingTransform* ts, size_t k, std::vector<double>& out);
   double histogram(const LinearMovingTransform& t, MiSampledArgs& args, std::vector<double>* ratio);
   DeviceContext* ctx_;
   const HImage* moving_;
@@ -327,6 +335,8 @@ class HipLinearEngine {
     std::shared_ptr<const HImage> moving;
     std::unique_ptr<MiLevel> metric;
     double value(const LinearMovingTransform& t) { return metric->value(t); }
+    std::vector<double> values(const std::vector<LinearMovingTransform>& ts) { return metric->values(ts); }
+    bool batch_pays() const { return metric->batch_pays(); }
     std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t) {
       return metric->value_and_derivative(t);
     }

@@ -119,6 +119,65 @@ double MiLevel::value(const LinearMovingTransform& t) {
   return histogram(t, a, nullptr);
 }
 
+std::vector<double> MiLevel::values(const std::vector<LinearMovingTransform>& ts) {
+  std::vector<double> out;
+  out.reserve(ts.size());
+  const size_t cap = size_t(mi_batch_capacity(int(binning_.bins)));
+  for (size_t i = 0; i < ts.size();) {
+    const size_t k = std::min(cap, ts.size() - i);
+    if (k == 1) out.push_back(value(ts[i]));
+    else batch_values(&ts[i], k, out);
+    i += k;
+  }
+  return out;
+}
+
+// Up to kMiMaxBatch transforms in one histogram pass and one slot sum.
+void MiLevel::batch_values(const LinearMovingTransform* ts, size_t k, std::vector<double>& out) {
+  const Grid& mg = moving_->grid;
+  MiSampledArgs a{};
+  MiBatch batch{};
+  for (size_t q = 0; q < k; ++q) {
+    const AffineTransform T = ts[q].collapsed();
+    const Mat3 A = mat3_mul(mg.index_from_offset, T.matrix);
+    const Vec3 b = mat3_mul_vec(mg.index_from_offset, vsub(vadd(mat3_mul_vec(T.matrix, anchor_), T.offset()), mg.origin));
+    put(batch.a[q], A);
+    put(batch.b[q], b);
+  }
+  a.samples = reinterpret_cast<const float4*>(samples_.get());
+  a.count = int(count_);
+  a.moving = moving_->buf.get();
+  a.md = dims_of(mg);
+  a.inv_bin_size = float(1.0 / binning_.moving_bin_size);
+  a.moving_min = float(binning_.moving_min);
+  a.moving_max = float(binning_.moving_max);
+  a.bins = int(binning_.bins);
+  const size_t bins = binning_.bins;
+  const size_t cells = bins * bins + bins + 1;
+  const size_t all = k * cells;
+  const int blocks = mi_sampled_blocks(int(count_));
+  a.hist_slots = static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * all * sizeof(unsigned long long)));
+  hipStream_t s = ctx_->stream();
+  {
+    Timed tm(*ctx_, "k_mi_sampled_hist_batch", (16.0 + 4.0 * double(k)) * double(count_));
+    launch_mi_sampled_hist_batch(a, batch, int(k), blocks, s);
+  }
+  void* dev_hist = nullptr;
+  auto* host = static_cast<unsigned long long*>(ctx_->mapped(0, all * sizeof(unsigned long long), &dev_hist));
+  const uint32_t tag = ctx_->next_tag();
+  {
+    Timed tm(*ctx_, "k_mi_sum_slots", 8.0 * double(all) * double(blocks + 1));
+    launch_mi_sum_slots(a.hist_slots, blocks, int(all), static_cast<unsigned long long*>(dev_hist), ctx_->done_words_device(),
+                        tag, s);
+  }
+  ctx_->wait_done(mi_sum_slots_blocks(int(all)), tag);
+  for (size_t q = 0; q < k; ++q) {
+    const unsigned long long* h = host + q * cells;
+    out.push_back(finish_histogram(bins, binning_.moving_bin_size, h[cells - 1], joint_from(h, bins), marginal_from(h, bins), false)
+                      .value);
+  }
+}
+
 std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const LinearMovingTransform& t) {
   MiSampledArgs a;
   std::vector<double> ratio;

@@ -218,6 +218,67 @@ __global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
   for (int i = threadIdx.x; i < cells + 1; i += blockDim.x) slot[i] = h[i];
 }
 
+// K transforms per pass: each sample record is read once and binned under every transform, into
+// the K histograms of the block's LDS (the samples, the anchor and the moving image are shared).
+template <int K>
+__global__ __launch_bounds__(256) void k_mi_sampled_hist_batch(MiSampledArgs a, MiBatch batch) {
+  extern __shared__ unsigned long long h[];
+  const int bins = a.bins;
+  const int cells = bins * bins + bins;
+  const int C = cells + 1;
+  for (int i = threadIdx.x; i < K * C; i += blockDim.x) h[i] = 0ull;
+  __syncthreads();
+  unsigned int mine[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) mine[k] = 0;
+  const SampleRange rg = sample_range(a.count);
+  for (int s = rg.lo + threadIdx.x; s < rg.hi; s += blockDim.x) {
+    const float4 smp = a.samples[s];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      MiSampledArgs ak = a;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) ak.a[q] = batch.a[k][q];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) ak.b[q] = batch.b[k][q];
+      const SampleEval e = eval_sample(ak, smp);
+      if (!e.ok) continue;
+      ++mine[k];
+      unsigned long long* hk = h + k * C;
+      atomicAdd(&hk[bins * bins + e.fbin], 1ull);
+      unsigned long long* row = hk + e.fbin * bins;
+#pragma unroll
+      for (int q = -1; q <= 2; ++q) {
+        const int m = e.mbin + q;
+        const float w = bspline3((float)m - e.term);
+        if (w > 0.0f) atomicAdd(&row[m], to_fixed(w));
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (mine[k]) atomicAdd(&h[k * C + cells], (unsigned long long)mine[k]);
+  __syncthreads();
+  unsigned long long* slot = a.hist_slots + (size_t)blockIdx.x * K * C;
+  for (int i = threadIdx.x; i < K * C; i += blockDim.x) slot[i] = h[i];
+}
+
+int mi_batch_capacity(int bins) {
+  const size_t one = size_t(bins * bins + bins + 1) * sizeof(unsigned long long);
+  const size_t fit = (size_t(64) << 10) / one;
+  return int(fit < 1 ? 1 : (fit > size_t(kMiMaxBatch) ? kMiMaxBatch : fit));
+}
+
+void launch_mi_sampled_hist_batch(const MiSampledArgs& a, const MiBatch& batch, int count, int blocks, hipStream_t s) {
+  const size_t lds = size_t(count) * size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
+  switch (count) {
+    case 2: launch(k_mi_sampled_hist_batch<2>, dim3(blocks), dim3(256), (uint32_t)lds, s, a, batch); break;
+    case 3: launch(k_mi_sampled_hist_batch<3>, dim3(blocks), dim3(256), (uint32_t)lds, s, a, batch); break;
+    case 4: launch(k_mi_sampled_hist_batch<4>, dim3(blocks), dim3(256), (uint32_t)lds, s, a, batch); break;
+    default: break;
+  }
+}
+
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
   const size_t lds = size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
   launch(k_mi_sampled_hist, dim3(blocks), dim3(256), lds, s, a);

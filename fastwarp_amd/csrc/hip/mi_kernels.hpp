@@ -37,6 +37,16 @@ struct MiSampledArgs {
   double* partial;        // per block: 12 doubles
 };
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s);
+// Several transforms over the same samples in one pass (a.a / a.b unused): transform k maps the
+// samples with batch.a[k], batch.b[k]; block b's slot holds one histogram per transform, in order.
+constexpr int kMiMaxBatch = 4;
+struct MiBatch {
+  float a[kMiMaxBatch][9];
+  float b[kMiMaxBatch][3];
+};
+// The number of transforms one batched pass takes for `bins` (its LDS holds a histogram each).
+int mi_batch_capacity(int bins);
+void launch_mi_sampled_hist_batch(const MiSampledArgs& a, const MiBatch& batch, int count, int blocks, hipStream_t s);
 // Adds `nslots` per-block histograms of `cells` u64 entries into `hist`.
 // hist[c] = the slots' cell c added up (hist in coherent host memory); block b then stores `tag` into
 // done[b] with a system-scope release, b < mi_sum_slots_blocks(cells).

@@ -6,6 +6,9 @@
 
 #include <atomic>
 #include <chrono>
+#include <limits>
+#include <type_traits>
+#include <utility>
 
 #include "convergence.hpp"
 #include "grid.hpp"
@@ -91,12 +94,61 @@ inline void check_cancel(const volatile int32_t* cancel) {
   if (cancel && *cancel) throw Cancelled();
 }
 
-// optimizer.rs:262-300 `LineSearch::golden` (RESPHI = 2 - 1.618034).
+// optimizer.rs:262-300 `LineSearch::golden` (RESPHI = 2 - 1.618034), as two pure steps over the
+// bracket so that the plain search and the speculative one below share them.
+struct GoldenBracket {
+  double a, b, c;
+  double metric_b;  // the reference's sentinel f64::MAX while f(b) has not been evaluated
+  uint32_t iterations;
+};
+
+// The next probe x of bracket g (and the bracket with the iteration counted), or false when the
+// search ends there, with its return value in `result`.
+inline bool golden_probe(const GoldenBracket& g, const CgSettings& st, double& x, double& result, GoldenBracket& next) {
+  const double resphi = 2.0 - 1.618034;
+  if (g.iterations > st.max_line_search_iterations) {
+    result = (g.c + g.a) / 2.0;
+    return false;
+  }
+  next = g;
+  ++next.iterations;
+  x = (g.c - g.b > g.b - g.a) ? g.b + resphi * (g.c - g.b) : g.b - resphi * (g.b - g.a);
+  if (std::fabs(g.c - g.a) < st.epsilon * (std::fabs(g.b) + std::fabs(x))) {
+    result = (g.c + g.a) / 2.0;
+    return false;
+  }
+  return true;
+}
+
+// The bracket after probing x: f(x) = mx, f(b) = mb.
+inline GoldenBracket golden_update(GoldenBracket g, double x, double mx, double mb) {
+  constexpr double kMax = std::numeric_limits<double>::max();
+  g.metric_b = mb;
+  if (mx < mb) {
+    if (g.c - g.b > g.b - g.a) {
+      g.a = g.b;
+      g.b = x;
+    } else {
+      g.c = g.b;
+      g.b = x;
+    }
+    g.metric_b = mx;
+  } else if (g.c - g.b > g.b - g.a) {
+    g.c = x;
+  } else if (mx == kMax) {
+    g.c = g.b;
+    g.b = x;
+    g.metric_b = mx;
+  } else {
+    g.a = x;
+  }
+  return g;
+}
+
 template <typename Metric>
 double golden_search(Metric& metric, const LinearMovingTransform& start, const std::vector<double>& dir, const CgSettings& st,
                      double a, double b, double c, uint64_t& evaluations) {
-  const double resphi = 2.0 - 1.618034;
-  uint32_t iterations = 0;
+  constexpr double kMax = std::numeric_limits<double>::max();
   const std::vector<double> base = start.optimised.parameters();
   auto value_at = [&](double step) {
     ++evaluations;
@@ -104,32 +156,77 @@ double golden_search(Metric& metric, const LinearMovingTransform& start, const s
     for (size_t i = 0; i < p.size(); ++i) p[i] = base[i] + step * dir[i];
     return metric.value(start.with_parameters(p));
   };
-  double metric_b = std::numeric_limits<double>::max();
+  GoldenBracket g{a, b, c, kMax, 0};
   for (;;) {
-    if (iterations > st.max_line_search_iterations) return (c + a) / 2.0;
-    ++iterations;
-    const double x = (c - b > b - a) ? b + resphi * (c - b) : b - resphi * (b - a);
-    if (std::fabs(c - a) < st.epsilon * (std::fabs(b) + std::fabs(x))) return (c + a) / 2.0;
-    const double metric_x = value_at(x);
-    if (metric_b == std::numeric_limits<double>::max()) metric_b = value_at(b);
-    if (metric_x < metric_b) {
-      if (c - b > b - a) {
-        a = b;
-        b = x;
-      } else {
-        c = b;
-        b = x;
+    double x = 0.0, result = 0.0;
+    GoldenBracket next;
+    if (!golden_probe(g, st, x, result, next)) return result;
+    const double mx = value_at(x);
+    const double mb = g.metric_b == kMax ? value_at(g.b) : g.metric_b;
+    g = golden_update(next, x, mx, mb);
+  }
+}
+
+// Metrics that can evaluate several transforms in one pass (values(ts)) and say whether that pays
+// (batch_pays(): an extra transform in a pass costs less than a pass of its own).
+template <typename M, typename = void>
+struct has_batched_values : std::false_type {};
+template <typename M>
+struct has_batched_values<M, std::void_t<decltype(std::declval<M&>().values(std::declval<const std::vector<LinearMovingTransform>&>())),
+                                         decltype(std::declval<const M&>().batch_pays())>> : std::true_type {};
+
+// The same search, two probes per pass: with the probe x0 of the current bracket (and f(b) when it
+// is not known yet) the pass also evaluates the probe that follows each outcome of x0 (f(x0) below
+// f(b), or not). Which bracket follows depends on f(x0) only through that comparison, and the next
+// probe on the bracket only, so the search visits exactly the brackets of golden_search, with the
+// same values (a metric value is a function of the transform), and returns the same step; only the
+// evaluations golden_search makes are counted. The probe of the outcome not taken is discarded.
+template <typename Metric>
+double golden_search_speculative(Metric& metric, const LinearMovingTransform& start, const std::vector<double>& dir,
+                                 const CgSettings& st, double a, double b, double c, uint64_t& evaluations) {
+  constexpr double kMax = std::numeric_limits<double>::max();
+  const std::vector<double> base = start.optimised.parameters();
+  auto at = [&](double step) {
+    std::vector<double> p(base.size());
+    for (size_t i = 0; i < p.size(); ++i) p[i] = base[i] + step * dir[i];
+    return start.with_parameters(p);
+  };
+  GoldenBracket g{a, b, c, kMax, 0};
+  for (;;) {
+    double x0 = 0.0, result = 0.0;
+    GoldenBracket n0;
+    if (!golden_probe(g, st, x0, result, n0)) return result;
+    const bool need_b = g.metric_b == kMax;
+    // the brackets after x0 if f(x0) < f(b) ("lower") and if not, f(x0) finite ("higher"): their
+    // geometry does not depend on the values themselves
+    const GoldenBracket lower = golden_update(n0, x0, 0.0, 1.0), higher = golden_update(n0, x0, 1.0, 0.0);
+    double x1[2] = {0.0, 0.0}, r1[2] = {0.0, 0.0};
+    GoldenBracket n1[2];
+    const bool go[2] = {golden_probe(lower, st, x1[0], r1[0], n1[0]), golden_probe(higher, st, x1[1], r1[1], n1[1])};
+    std::vector<LinearMovingTransform> ts{at(x0)};
+    if (need_b) ts.push_back(at(g.b));
+    int slot[2] = {-1, -1};
+    for (int o = 0; o < 2; ++o)
+      if (go[o]) {
+        slot[o] = int(ts.size());
+        ts.push_back(at(x1[o]));
       }
-      metric_b = metric_x;
-    } else if (c - b > b - a) {
-      c = x;
-    } else if (metric_x == std::numeric_limits<double>::max()) {
-      c = b;
-      b = x;
-      metric_b = metric_x;
-    } else {
-      a = x;
+    const std::vector<double> v = metric.values(ts);
+    const double mx = v[0];
+    const double mb = need_b ? v[1] : g.metric_b;
+    evaluations += need_b ? 2 : 1;
+    const GoldenBracket g1 = golden_update(n0, x0, mx, mb);
+    int o = -1;
+    if (mx < mb) o = 0;
+    else if (n0.c - n0.b > n0.b - n0.a || mx != kMax) o = 1;
+    if (o < 0 || g1.metric_b == kMax) {
+      // f(x0) or f(b) is the sentinel: the next step evaluates b again, which this pass did not
+      g = g1;
+      continue;
     }
+    if (!go[o]) return r1[o];
+    ++evaluations;
+    g = golden_update(n1[o], x1[o], v[size_t(slot[o])], g1.metric_b);
   }
 }
 
@@ -177,7 +274,16 @@ std::pair<LinearMovingTransform, CgTrace> optimise_level(Metric& metric, const s
     if (!(gamma >= 0.0 && gamma <= 5.0)) gamma = 0.0;
     last = gradient;
     for (size_t i = 0; i < n; ++i) conjugate[i] = gradient[i] + conjugate[i] * gamma;
-    lr = golden_search(metric, current, conjugate, st, lr * st.lower_limit, lr, lr * st.upper_limit, trace.line_search_evaluations);
+    bool speculate = false;
+    if constexpr (has_batched_values<Metric>::value) speculate = metric.batch_pays();
+    if (speculate) {
+      if constexpr (has_batched_values<Metric>::value)
+        lr = golden_search_speculative(metric, current, conjugate, st, lr * st.lower_limit, lr, lr * st.upper_limit,
+                                       trace.line_search_evaluations);
+    } else {
+      lr = golden_search(metric, current, conjugate, st, lr * st.lower_limit, lr, lr * st.upper_limit,
+                         trace.line_search_evaluations);
+    }
     std::vector<double> p = current.optimised.parameters();
     for (size_t i = 0; i < n; ++i) p[i] += lr * conjugate[i];
     current = current.with_parameters(p);

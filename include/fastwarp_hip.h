@@ -168,6 +168,11 @@ int fw_resample_field(fw_ctx* ctx, const fw_field* field, const fw_grid* grid, f
 int fw_mi_level_create(fw_ctx* ctx, const fw_image* moving, const double* points, const uint32_t* fixed_bin, int64_t m,
                        const fw_mi_binning* binning, fw_mi_level** out);
 int fw_mi_level_value(fw_mi_level* level, const fw_linear_transform* t, double* value);
+/* values[i] = the value of transforms[i], i < count, computed in as few histogram passes as the
+ * device allows (up to four transforms share one pass over the samples); each equals
+ * fw_mi_level_value of that transform. No counterpart in the reference: the line search's
+ * speculative probes (two golden-section steps per pass) use it. */
+int fw_mi_level_values(fw_mi_level* level, const fw_linear_transform* transforms, int64_t count, double* values);
 /* derivative receives 6 or 12 values. */
 int fw_mi_level_value_and_derivative(fw_mi_level* level, const fw_linear_transform* t, double* value,
                                      double* derivative);

@@ -172,6 +172,57 @@ static void optimizer() {
   ok("optimizer");
 }
 
+// The speculative line search (two probes per pass) against the plain golden search it stands for:
+// the same step and the same evaluation count on rugged 1D profiles, flat stretches, ties and the
+// f64::MAX sentinel, for many brackets.
+struct Profile {
+  int kind;
+  uint64_t evaluations = 0;
+  double f(double s) const {
+    switch (kind % 5) {
+      case 0: return (s - 0.37) * (s - 0.37);
+      case 1: return std::floor(8.0 * std::fabs(s - 0.8)) / 8.0;  // plateaus: ties in both directions
+      case 2: return s > 1.3 ? std::numeric_limits<double>::max() : std::sin(9.0 * s) + 0.1 * s;
+      case 3: return std::fabs(s - 0.01) < 0.3 ? std::numeric_limits<double>::max() : -s;
+      default: return std::cos(23.0 * s) * std::exp(-s);
+    }
+  }
+  double step_of(const LinearMovingTransform& t) const { return t.optimised.params[3]; }
+  double value(const LinearMovingTransform& t) {
+    ++evaluations;
+    return f(step_of(t));
+  }
+  bool batch_pays() const { return true; }
+  std::vector<double> values(const std::vector<LinearMovingTransform>& ts) {
+    std::vector<double> v;
+    for (const auto& t : ts) v.push_back(f(step_of(t)));
+    return v;
+  }
+};
+
+static void speculative_search() {
+  const LinearMovingTransform t{AffineTransform::identity(), ParametricTransform::euler3d({0, 0, 0, 0, 0, 0}, {0, 0, 0})};
+  const std::vector<double> dir{0, 0, 0, 1, 0, 0};
+  int cases = 0;
+  for (int kind = 0; kind < 5; ++kind)
+    for (double lr : {0.05, 0.3, 0.5, 1.0, 1.7, 4.0})
+      for (double eps : {0.2, 0.05, 0.001})
+        for (uint32_t cap : {2u, 20u}) {
+          CgSettings st;
+          st.epsilon = eps;
+          st.max_line_search_iterations = cap;
+          Profile plain{kind}, spec{kind};
+          uint64_t e_plain = 0, e_spec = 0;
+          const double a = golden_search(plain, t, dir, st, 0.0, lr, 2.0 * lr, e_plain);
+          const double b = golden_search_speculative(spec, t, dir, st, 0.0, lr, 2.0 * lr, e_spec);
+          CHECK(a == b, "speculative search step");
+          CHECK(e_plain == e_spec && e_plain == plain.evaluations, "speculative search evaluation count");
+          ++cases;
+        }
+  CHECK(cases == 180, "speculative search cases");
+  ok("speculative_search");
+}
+
 // transform/param.rs:398-430: the Euler and affine Jacobians against central differences.
 static void jacobians() {
   for (int kind = 0; kind < 2; ++kind) {
@@ -320,6 +371,7 @@ int main() {
   deriche();
   convergence();
   optimizer();
+  speculative_search();
   jacobians();
   thresholds();
   taps();

@@ -60,6 +60,36 @@ def test_sampled_mi_matches_oracle(backend, name, kind, params, outer, oblique):
     assert rel <= 5e-4, (d, want["derivative"])
 
 
+@pytest.mark.parametrize("bins", [32, 12, 64])
+def test_batched_values_equal_single_values(backend, bins):
+    """One pass over the samples for up to four transforms (the line search's speculative probes):
+    the integer histograms make every batched value bit-identical to its single evaluation."""
+    g = cases.grid((40, 36, 30), (1.2, 1.0, 1.4), (-20.0, -15.0, -18.0), oblique=0.3)
+    f = smooth_vol(g, (0, 0, 0))
+    m = smooth_vol(g, (1.1, -0.8, 0.6), 0.3)
+    rng = np.random.default_rng(11)
+    pts = np.array([g.index_to_point(p) for p in rng.uniform([4, 4, 4], np.array(g.shape_xyz) - 5, size=(9000, 3))])
+    b = MiBinning.from_ranges(bins, (f.min(), f.max()), (m.min(), m.max()))
+    fb = fixed_bins(g, f, pts, b)
+    level = MiLevel(backend, backend.upload_image(g, m), pts, fb, b)
+    centre = tuple(g.index_to_point(np.array(g.shape_xyz) / 2))
+    ts = []
+    for q in range(7):
+        if q % 3 == 2:
+            p = np.array([1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0]) + rng.normal(0, 0.02, 12)
+            ts.append(LinearTransform.make(1, p, centre, Affine.make(translation=(0.1, 0.0, -0.1))))
+        else:
+            ts.append(LinearTransform.make(0, np.concatenate([rng.normal(0, 0.04, 3), rng.normal(0, 1.5, 3)]), centre))
+    single = np.array([level.value(t) for t in ts])
+    for n in range(1, 8):
+        assert np.array_equal(level.values(ts[:n]), single[:n]), n
+    from fastwarp_amd.backend import RegError
+
+    away = LinearTransform.make(0, [0, 0, 0, 500.0, 0, 0], centre)
+    with pytest.raises(RegError, match="outside the moving image buffer"):
+        level.values([ts[0], away, ts[1]])
+
+
 def test_sampled_mi_is_deterministic_and_errors_without_overlap(backend):
     g = cases.grid((24, 24, 24))
     f = smooth_vol(g, (0, 0, 0))
