@@ -175,7 +175,7 @@ def main() -> int:
 
     from fastwarp_amd.backend import HipBackend
     from fastwarp_amd.phantom import CONFIGS, make_pair
-    from fastwarp_amd.register import cohort_plan, plan_2mm, register
+    from fastwarp_amd.register import cohort_plan, plan_2mm, register, register_resident
 
     cfg = CONFIGS[args.config]
     t = time.perf_counter()
@@ -183,6 +183,19 @@ def main() -> int:
     log(f"phantom {cfg['shape_xyz']} at {cfg['spacing']} mm ready in {time.perf_counter() - t:.1f} s")
     plan = cohort_plan(1) if args.config == "1mm" else plan_2mm(1)
     backend = HipBackend(0)
+    # The raw volumes are in HBM before the timed region starts (fw_upload_image), and the output
+    # fields stay there (fw_register_resident); the host-buffer path (fw_register: PCIe both ways
+    # and host preprocessing) is timed after it and reported beside it.
+    fixed_dev = [backend.upload_image(pair.grid, pair.fixed[0])]
+    moving_dev = [backend.upload_image(pair.grid, pair.moving[0])]
+
+    def step():
+        r = register_resident(backend, plan, fixed_dev, moving_dev)
+        for f in (r.forward, r.inverse):
+            if f is not None:
+                f.free()
+        return r
+
     # The per-kernel breakdown comes from the last warmup step, every launch timed; the timed
     # steps time every 4th launch of the dominant kernel only (the events cost the GPU a few
     # microseconds per timed launch).
@@ -191,7 +204,7 @@ def main() -> int:
         if w == args.warmup - 1:
             backend.set_profiling(True)
         t = time.perf_counter()
-        r = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving)
+        r = step()
         log(f"warmup {w + 1}/{args.warmup}: {time.perf_counter() - t:.2f} s (linear {r.result.seconds_linear:.2f}, syn {r.result.seconds_syn:.2f})")
     if args.warmup:
         breakdown = {k: v for k, v in backend.take_profile().items() if not k.startswith("host:")}
@@ -204,7 +217,7 @@ def main() -> int:
     t_all = time.perf_counter()
     for k in range(args.steps):
         t = time.perf_counter()
-        last = register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving)
+        last = step()
         lat.append(time.perf_counter() - t)
         log(f"step {k + 1}/{args.steps}: {lat[-1]:.2f} s (linear {last.result.seconds_linear:.2f}, syn {last.result.seconds_syn:.2f})")
     total = time.perf_counter() - t_all
@@ -216,6 +229,14 @@ def main() -> int:
         breakdown = profile
         focus = dominant(profile)
     held, peak = backend.memory_stats()
+    # the host-buffer path on the same pair: one untimed run, then as many timed as above (at most 3)
+    register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving)
+    host_steps = max(1, min(args.steps, 3))
+    t = time.perf_counter()
+    for _ in range(host_steps):
+        register(backend, plan, pair.grid, pair.fixed, pair.grid, pair.moving)
+    host_s = (time.perf_counter() - t) / host_steps
+    log(f"host-buffer path: {host_s:.3f} s per registration")
     line = {
         "metric": METRIC,
         "value": round(3600.0 * args.steps / total, 3),
@@ -235,8 +256,10 @@ def main() -> int:
             "grid": list(cfg["shape_xyz"]),
             "spacing_mm": cfg["spacing"],
             "channels": 1,
-            "path": "plain one-at-a-time HIP",
+            "path": "plain one-at-a-time HIP, raw volumes resident in HBM, fields left in HBM",
         },
+        "host_buffers": {"value": round(3600.0 / host_s, 3), "unit": "registrations/hour", "ms_per_step": round(1000.0 * host_s, 1),
+                         "path": "fw_register: f64 host volumes in, host preprocessing, PCIe uploads, fields downloaded to the host"},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "phases_s": {"linear": round(last.result.seconds_linear, 3), "syn": round(last.result.seconds_syn, 3),
                      "preprocess": round(last.result.seconds_preprocess, 3)},

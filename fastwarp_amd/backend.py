@@ -63,6 +63,8 @@ _SIGNATURES = [
     ("fw_smooth_field", C.c_int, [_P, _P, C.c_double, _PP]),
     ("fw_smooth_image", C.c_int, [_P, _P, _D, _PP]),
     ("fw_sample_values", C.c_int, [_P, _P, _D, C.c_int64, _D, _D]),
+    ("fw_preprocess_image", C.c_int, [_P, _P, C.c_float, C.c_float, _PP]),
+    ("fw_centre_of_gravity", C.c_int, [_P, _P, _D]),
     ("fw_max_scaled_norm_pair", C.c_int, [_P, _P, _P, _D]),
     ("fw_l2_norm", C.c_int, [_P, _P, _D]),
     ("fw_scale", C.c_int, [_P, _P, C.c_double]),
@@ -98,6 +100,8 @@ _SIGNATURES = [
         [_P, C.POINTER(Plan), C.POINTER(Grid), C.POINTER(C.POINTER(C.c_double)), C.POINTER(Grid),
          C.POINTER(C.POINTER(C.c_double)), _D, _D, C.POINTER(Result), C.POINTER(C.c_int32)],
     ),
+    ("fw_register_resident", C.c_int, [_P, C.POINTER(Plan), C.POINTER(_P), C.POINTER(_P), _PP, _PP, C.POINTER(Result),
+                                       C.POINTER(C.c_int32)]),
 ]
 
 
@@ -380,6 +384,16 @@ class HipBackend:
         """Level smoothing on the device (pyramid.rs:429-437), per-axis sigma in mm."""
         s = np.ascontiguousarray(sigma_mm, dtype=np.float64).reshape(3)
         return self._img(lib().fw_smooth_image, image.handle, dptr(s))
+
+    def preprocess_image(self, raw: DeviceImage, lower: float = 0.005, upper: float = 0.995) -> DeviceImage:
+        """Winsorise and window a raw image on the device (preprocess.rs:159-232)."""
+        return self._img(lib().fw_preprocess_image, raw.handle, C.c_float(lower), C.c_float(upper))
+
+    def centre_of_gravity(self, image: DeviceImage) -> np.ndarray:
+        """Intensity-weighted centre (mm) of an image on the device (init.rs:47-86)."""
+        c = np.zeros(3)
+        self._check(lib().fw_centre_of_gravity(self.ctx, image.handle, dptr(c)))
+        return c
 
     def sample_values(self, image: DeviceImage, points: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         """(nearest-voxel, linear) values at physical points, NaN outside."""

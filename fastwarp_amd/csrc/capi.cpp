@@ -207,6 +207,23 @@ int fw_smooth_image(fw_ctx* ctx, const fw_image* src, const double sigma_mm[3], 
   });
 }
 
+int fw_preprocess_image(fw_ctx* ctx, const fw_image* raw, float lower, float upper, fw_image** out) {
+  return guard(ctx, [&] {
+    need(raw, "image");
+    need(out, "output");
+    *out = new fw_image{ctx->backend.preprocess_image(raw->h, lower, upper)};
+  });
+}
+
+int fw_centre_of_gravity(fw_ctx* ctx, const fw_image* img, double centre[3]) {
+  return guard(ctx, [&] {
+    need(img, "image");
+    need(centre, "output");
+    const fw::Vec3 c = ctx->backend.centre_of_gravity(img->h);
+    for (int a = 0; a < 3; ++a) centre[a] = c[a];
+  });
+}
+
 int fw_sample_values(fw_ctx* ctx, const fw_image* img, const double* points, int64_t m, double* nearest, double* linear) {
   return guard(ctx, [&] {
     need(img, "image");
@@ -556,6 +573,28 @@ int fw_registrations_at_once(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fi
 
 void fw_plan_cohort(fw_plan* plan, int32_t channels) {
   if (plan) fw::cohort_plan(plan, channels);
+}
+
+int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
+                         fw_field** forward, fw_field** inverse, fw_result* result, const volatile int32_t* cancel) {
+  return guard(ctx, [&] {
+    need(plan, "plan");
+    if (!fixed || !moving) fw::invalid("register", "null image list");
+    const fw::RegisterPlan p = fw::plan_from(plan);
+    std::vector<const HImage*> f, m;
+    for (int32_t c = 0; c < plan->channels; ++c) {
+      if (!fixed[c] || !moving[c]) fw::invalid("register", "null image");
+      f.push_back(&fixed[c]->h);
+      m.push_back(&moving[c]->h);
+    }
+    fw::hip::HipLinearEngine engine(ctx->backend);
+    std::optional<HField> fwd, inv;
+    const fw::RegisterOutput o = fw::register_resident(ctx->backend, engine, p, f, m, cancel, forward ? &fwd : nullptr,
+                                                       inverse ? &inv : nullptr);
+    fw::result_to(o, result);
+    if (forward) *forward = fwd ? new fw_field{std::move(*fwd)} : nullptr;
+    if (inverse) *inverse = inv ? new fw_field{std::move(*inv)} : nullptr;
+  });
 }
 
 int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,

@@ -10,6 +10,7 @@
 #include <thread>
 
 #include "../host/gauss_taps.hpp"
+#include "../host/preprocess.hpp"
 #include "geometry.hpp"
 
 namespace fw::hip {
@@ -732,6 +733,102 @@ void HipBackend::sample_values(const HImage& img, const HostVec<Vec3>& points, H
       linear[i] = double(stage[m + i]);
     }
   });
+}
+
+// preprocess.rs:159-232 on an image in HBM: the finite range and the non-finite count, the 256-bin
+// histogram (bins by f32 thresholds equal to the f64 bins), the quantile window on the host (a few
+// hundred f64 operations) and the window map on the device.
+HImage HipBackend::preprocess_image(const HImage& raw, float lower, float upper, double out_min, double out_max) {
+  const Grid& g = raw.grid;
+  const long long n = (long long)g.num_voxels();
+  const int nb = norm_blocks(n);
+  hipStream_t s = ctx_->stream();
+  auto* part = static_cast<float*>(ctx_->scratch(2, 3 * size_t(nb) * sizeof(float)));
+  {
+    Timed t(*ctx_, "k_range_and_bad", 4.0 * double(n));
+    launch_range_and_bad(raw.buf.get(), n, part, part + nb, reinterpret_cast<unsigned int*>(part + 2 * nb), nb, s);
+  }
+  auto* host = static_cast<float*>(ctx_->pinned(3 * size_t(nb) * sizeof(float)));
+  check(hipMemcpyAsync(host, part, 3 * size_t(nb) * sizeof(float), hipMemcpyDeviceToHost, s), "range partials");
+  ctx_->sync();
+  float lo = INFINITY, hi = -INFINITY;
+  uint64_t bad = 0;
+  for (int b = 0; b < nb; ++b) {
+    lo = std::min(lo, host[b]);
+    hi = std::max(hi, host[nb + b]);
+    unsigned int c;
+    std::memcpy(&c, &host[2 * nb + b], sizeof c);
+    bad += c;
+  }
+  if (bad) invalid("register", "an input image has non-finite (NaN or infinite) voxels");
+  if (!(lo <= hi)) invalid("preprocess", "image has no finite values");
+  const IntensityHistogram edges(double(lo), double(hi), std::vector<uint64_t>(kIntensityBins, 0));
+  const std::vector<float> th = edges.f32_thresholds();
+  auto* th_host = static_cast<float*>(ctx_->stage(th.size() * sizeof(float)));
+  std::memcpy(th_host, th.data(), th.size() * sizeof(float));
+  auto* th_dev = static_cast<float*>(ctx_->scratch(3, th.size() * sizeof(float)));
+  check(hipMemcpyAsync(th_dev, th_host, th.size() * sizeof(float), hipMemcpyHostToDevice, s), "bin thresholds");
+  auto* slots = static_cast<unsigned long long*>(ctx_->scratch(6, size_t(nb) * kIntensityBins * sizeof(unsigned long long)));
+  {
+    Timed t(*ctx_, "k_intensity_hist", 4.0 * double(n));
+    launch_intensity_hist(raw.buf.get(), n, th_dev, slots, nb, s);
+  }
+  void* dev_counts = nullptr;
+  auto* counts = static_cast<unsigned long long*>(ctx_->mapped(0, kIntensityBins * sizeof(unsigned long long), &dev_counts));
+  const uint32_t tag = ctx_->next_tag();
+  {
+    Timed t(*ctx_, "k_mi_sum_slots", 8.0 * double(kIntensityBins) * double(nb + 1));
+    launch_mi_sum_slots(slots, nb, kIntensityBins, static_cast<unsigned long long*>(dev_counts), ctx_->done_words_device(),
+                        tag, s);
+  }
+  ctx_->wait_done(mi_sum_slots_blocks(kIntensityBins), tag);
+  ctx_->settle();
+  const IntensityHistogram h(double(lo), double(hi), std::vector<uint64_t>(counts, counts + kIntensityBins));
+  const IntensityWindow w = intensity_window(h, lower, upper, out_min, out_max);
+  const float sh = float(w.scale), th_ = float(w.shift);
+  const float sl = float(w.scale - double(sh)), tl = float(w.shift - double(th_));
+  HImage out{g, DevBuf(ctx_.get(), size_t(n)), 0.0f};
+  {
+    Timed t(*ctx_, "k_window", 8.0 * double(n));
+    launch_window(raw.buf.get(), out.buf.get(), n, float(w.lo), float(w.hi), sh, sl, th_, tl, float(out_min), float(out_max), s);
+  }
+  // the bound of the windowed image: its own maximum magnitude, as upload_image takes it
+  auto* pm = static_cast<float*>(ctx_->scratch(2, size_t(nb) * sizeof(float)));
+  {
+    Timed t(*ctx_, "k_abs_max", 4.0 * double(n));
+    launch_abs_max(out.buf.get(), n, pm, nb, s);
+  }
+  auto* mh = static_cast<float*>(ctx_->pinned(size_t(nb) * sizeof(float)));
+  check(hipMemcpyAsync(mh, pm, size_t(nb) * sizeof(float), hipMemcpyDeviceToHost, s), "abs max partials");
+  ctx_->sync();
+  float m = 0.0f;
+  for (int b = 0; b < nb; ++b)
+    if (std::isfinite(mh[b])) m = std::max(m, mh[b]);
+  out.absmax = m;
+  return out;
+}
+
+// init.rs:47-86 (ITK ImageMomentsCalculator) on an image in HBM: compensated f32 block sums of
+// v, v i, v j, v k over the non-zero voxels, added in f64 on the host in block order.
+Vec3 HipBackend::centre_of_gravity(const HImage& img) {
+  const Grid& g = img.grid;
+  const Dims d = dims_of(g);
+  const dim3 gr = moments_grid(d);
+  const size_t blocks = size_t(gr.x) * gr.y * gr.z;
+  auto* part = static_cast<float*>(ctx_->scratch(2, 4 * blocks * sizeof(float)));
+  hipStream_t s = ctx_->stream();
+  {
+    Timed t(*ctx_, "k_moments", 4.0 * double(g.num_voxels()));
+    launch_moments(img.buf.get(), d, part, s);
+  }
+  auto* host = static_cast<float*>(ctx_->pinned(4 * blocks * sizeof(float)));
+  check(hipMemcpyAsync(host, part, 4 * blocks * sizeof(float), hipMemcpyDeviceToHost, s), "moment partials");
+  ctx_->sync();
+  double sum[4] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t b = 0; b < blocks; ++b)
+    for (int q = 0; q < 4; ++q) sum[q] += double(host[4 * b + q]);
+  if (sum[0] == 0.0) invalid("initial transform", "total mass of the image is zero");
+  return g.index_to_point({sum[1] / sum[0], sum[2] / sum[0], sum[3] / sum[0]});
 }
 
 bool HipBackend::finite_range(const HImage& img, double& lo, double& hi) {

@@ -272,6 +272,10 @@ class HipBackend {
                      HostVec<double>& linear);
   // Finite minimum and maximum; false when the image has no finite value.
   bool finite_range(const HImage& img, double& lo, double& hi);
+  // The winsorised, windowed copy of an image already in HBM (preprocess.rs:159-232), raising the
+  // registration's error for non-finite voxels; and its intensity-weighted centre (init.rs:47-86).
+  HImage preprocess_image(const HImage& raw, float lower, float upper, double out_min = 0.0, double out_max = 1.0);
+  Vec3 centre_of_gravity(const HImage& img);
 
   HField smooth_field(HField field, double variance);
   std::pair<double, double> max_scaled_norm_pair(const HField& a, const HField* b);

@@ -415,6 +415,164 @@ void launch_finite_range(const float* x, long long n, float* partial_min, float*
   launch(k_finite_range, dim3(blocks), dim3(256), 0, s, x, n, partial_min, partial_max);
 }
 
+// ------------------------------------------------------------------------------------------
+// Device preprocessing of raw input images (preprocess.rs:40-232 and ITK ImageMomentsCalculator
+// as init.rs:47-86 uses it), so that a registration can start from images already in HBM.
+// ------------------------------------------------------------------------------------------
+
+static int stream_blocks(long long count);
+
+// Per block: finite minimum, maximum and the count of non-finite voxels.
+__global__ __launch_bounds__(256) void k_range_and_bad(const float* __restrict__ x, long long n, float* partial_min,
+                                                       float* partial_max, unsigned int* partial_bad) {
+  float lo = INFINITY, hi = -INFINITY;
+  unsigned int bad = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = x[i];
+    if (isfinite(v)) {
+      lo = fminf(lo, v);
+      hi = fmaxf(hi, v);
+    } else {
+      ++bad;
+    }
+  }
+  __shared__ float red[2][4];
+  __shared__ unsigned int nb[4];
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fminf(lo, __shfl_down(lo, off, 64));
+    hi = fmaxf(hi, __shfl_down(hi, off, 64));
+    bad += __shfl_down(bad, off, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = lo;
+    red[1][w] = hi;
+    nb[w] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; ++q) {
+      lo = fminf(lo, red[0][q]);
+      hi = fmaxf(hi, red[1][q]);
+      bad += nb[q];
+    }
+    partial_min[blockIdx.x] = lo;
+    partial_max[blockIdx.x] = hi;
+    partial_bad[blockIdx.x] = bad;
+  }
+}
+
+void launch_range_and_bad(const float* x, long long n, float* partial_min, float* partial_max, unsigned int* partial_bad,
+                          int blocks, hipStream_t s) {
+  launch(k_range_and_bad, dim3(blocks), dim3(256), 0, s, x, n, partial_min, partial_max, partial_bad);
+}
+
+// The 256-bin intensity histogram of the finite voxels. A voxel's bin is the number of bin-start
+// thresholds (bins 1..255) at or below it: thresholds[k - 1] is the smallest f32 whose f64 value
+// is at least the f64 start of bin k, so the f32 comparison equals the f64 one and the bins are
+// those of the f64 histogram. Per-block counts go to the block's slot (u64).
+__global__ __launch_bounds__(256) void k_intensity_hist(const float* __restrict__ x, long long n,
+                                                        const float* __restrict__ thresholds,
+                                                        unsigned long long* __restrict__ slots) {
+  __shared__ float t[kIntensityBins - 1];
+  __shared__ unsigned int h[kIntensityBins];
+  for (int i = threadIdx.x; i < kIntensityBins - 1; i += blockDim.x) t[i] = thresholds[i];
+  for (int i = threadIdx.x; i < kIntensityBins; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = x[i];
+    if (!isfinite(v)) continue;
+    int lo = 0, cnt = kIntensityBins - 1;  // count of thresholds <= v, by bisection
+    while (cnt > 0) {
+      const int half = cnt >> 1;
+      if (t[lo + half] <= v) {
+        lo += half + 1;
+        cnt -= half + 1;
+      } else {
+        cnt = half;
+      }
+    }
+    atomicAdd(&h[lo], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kIntensityBins; i += blockDim.x)
+    slots[(size_t)blockIdx.x * kIntensityBins + i] = (unsigned long long)h[i];
+}
+
+void launch_intensity_hist(const float* x, long long n, const float* thresholds, unsigned long long* slots, int blocks,
+                           hipStream_t s) {
+  launch(k_intensity_hist, dim3(blocks), dim3(256), 0, s, x, n, thresholds, slots);
+}
+
+// out = x below lo: out_min; above hi: out_max; else x * scale + shift, with scale and shift given
+// as f32 pairs (hi + lo) and the product's rounding error recovered by an fma, so the result is the
+// f64 map rounded once to f32 (up to ties within about 2^-46 of a rounding boundary).
+__global__ __launch_bounds__(256) void k_window(const float* __restrict__ x, float* __restrict__ out, long long n, float lo,
+                                                float hi, float sh, float sl, float th, float tl, float out_min,
+                                                float out_max) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = x[i];
+    float r;
+    if (!isfinite(v)) r = v;
+    else if (v < lo) r = out_min;
+    else if (v > hi) r = out_max;
+    else {
+      // no contraction here: an fma formed from p + th would break the two-sum
+#pragma clang fp contract(off)
+      const float p = v * sh;
+      const float pe = fmaf(v, sh, -p);  // p + pe = v * sh exactly
+      const float s = p + th;            // two-sum of p and th
+      const float bb = s - p;
+      const float se = (p - (s - bb)) + (th - bb);
+      r = s + (se + (pe + fmaf(v, sl, tl)));
+    }
+    out[i] = r;
+  }
+}
+
+void launch_window(const float* x, float* out, long long n, float lo, float hi, float sh, float sl, float th, float tl,
+                   float out_min, float out_max, hipStream_t s) {
+  launch(k_window, dim3(stream_blocks(n)), dim3(256), 0, s, x, out, n, lo, hi, sh, sl, th, tl, out_min, out_max);
+}
+
+// Per block of a (64 x 4)-column, kMomentPlanes-plane chunk: the sums of v, v * i, v * j, v * k over
+// its non-zero voxels (index space), compensated f32; the host adds the blocks in f64.
+constexpr int kMomentPlanes = 16;
+__global__ __launch_bounds__(256) void k_moments(const float* __restrict__ x, Dims d, float* __restrict__ partial) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k0 = blockIdx.z * kMomentPlanes, k1 = min(k0 + kMomentPlanes, d.nz);
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}, c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (i < d.nx && j < d.ny) {
+    for (int k = k0; k < k1; ++k) {
+      const float v = x[lin(d, i, j, k)];
+      if (v == 0.0f) continue;
+      const float terms[4] = {v, v * (float)i, v * (float)j, v * (float)k};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float y = terms[q] - c[q];
+        const float t = s[q] + y;
+        c[q] = (t - s[q]) - y;
+        s[q] = t;
+      }
+    }
+  }
+  const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float t = block_sum(s[q]);
+    if (threadIdx.x == 0) partial[(size_t)b * 4 + q] = t;
+  }
+}
+
+dim3 moments_grid(const Dims& d) {
+  return dim3((unsigned)((d.nx + 63) / 64), (unsigned)((d.ny + 3) / 4), (unsigned)((d.nz + kMomentPlanes - 1) / kMomentPlanes));
+}
+
+void launch_moments(const float* x, Dims d, float* partial, hipStream_t s) {
+  launch(k_moments, moments_grid(d), dim3(256), 0, s, x, d, partial);
+}
+
 // Image values at continuous indices (mi.rs:161-207 and interp.rs:43-87 rules): the nearest voxel,
 // rounding half up, NaN outside [0, n); the ITK linear interpolation, NaN outside [-0.5, n - 0.5).
 __global__ __launch_bounds__(256) void k_sample_values(const float* __restrict__ img, Dims d,

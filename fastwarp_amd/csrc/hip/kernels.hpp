@@ -82,6 +82,16 @@ void launch_abs_max(const float* x, long long n, float* partial, int blocks, hip
 // Per-block finite minimum / maximum (+inf / -inf for a block without finite values).
 void launch_finite_range(const float* x, long long n, float* partial_min, float* partial_max, int blocks, hipStream_t s);
 // Nearest-voxel and linear values at m continuous indices (x, y, z; w unused); NaN outside.
+// Device preprocessing (preprocess.rs, init.rs:47-86).
+constexpr int kIntensityBins = 256;
+void launch_range_and_bad(const float* x, long long n, float* partial_min, float* partial_max, unsigned int* partial_bad,
+                          int blocks, hipStream_t s);
+void launch_intensity_hist(const float* x, long long n, const float* thresholds, unsigned long long* slots, int blocks,
+                           hipStream_t s);
+void launch_window(const float* x, float* out, long long n, float lo, float hi, float sh, float sl, float th, float tl,
+                   float out_min, float out_max, hipStream_t s);
+dim3 moments_grid(const Dims& d);
+void launch_moments(const float* x, Dims d, float* partial, hipStream_t s);
 void launch_sample_values(const float* img, Dims d, const float4* idx, long long m, float* nearest, float* linear,
                           hipStream_t s);
 

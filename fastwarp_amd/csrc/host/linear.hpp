@@ -63,14 +63,25 @@ class SmoothCache {
     Ptr fixed, moving;
   };
 
+  // Host images, uploaded on first use.
   SmoothCache(B& backend, std::vector<std::pair<const Image*, const Image*>> channels)
-      : b_(backend), host_(std::move(channels)) {}
+      : b_(backend), host_(std::move(channels)) {
+    for (const auto& c : host_) grids_.push_back(c.first->grid);
+  }
+  // Images already on the device (the unsmoothed level of every channel).
+  SmoothCache(B& backend, std::vector<Pair> resident) : b_(backend) {
+    for (size_t c = 0; c < resident.size(); ++c) {
+      grids_.push_back(resident[c].fixed->grid);
+      pairs_[{c, 0.0}] = resident[c];
+    }
+  }
 
   B& backend() { return b_; }
-  const Grid& fixed_grid(size_t channel) const { return host_.at(channel).first->grid; }
+  const Grid& fixed_grid(size_t channel) const { return grids_.at(channel); }
+  size_t channels() const { return grids_.size(); }
 
   Pair get(size_t channel, double sigma, SigmaUnits units) {
-    if (channel >= host_.size()) invalid("smoothing", "no such channel");
+    if (channel >= grids_.size()) invalid("smoothing", "no such channel");
     const double key = sigma > 0.0 ? sigma : 0.0;
     auto it = pairs_.find({channel, key});
     if (it != pairs_.end()) return it->second;
@@ -103,6 +114,7 @@ class SmoothCache {
  private:
   B& b_;
   std::vector<std::pair<const Image*, const Image*>> host_;
+  std::vector<Grid> grids_;
   std::map<std::pair<size_t, double>, Pair> pairs_;
   std::map<std::pair<size_t, double>, std::pair<double, double>> ranges_;
 };
@@ -377,13 +389,13 @@ struct LinearChainResult {
 
 // linear.rs:1148-1212 `run_linear_chain_cached_async`: centre of mass, Rigid, Affine, collapsed
 // about the last linear transform's centre.
-template <typename Engine, typename B, typename V>
-LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine, const V& fixed_raw,
-                                   const V& moving_raw, SmoothCache<B>& cache, const volatile int32_t* cancel) {
+template <typename Engine, typename B>
+LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine,
+                                   const AffineTransform& initial, SmoothCache<B>& cache, const volatile int32_t* cancel) {
   if (rigid.kind != LinearKind::Rigid || affine.kind != LinearKind::Affine)
     invalid("linear chain", "expected a rigid then an affine recipe");
   LinearChainResult out;
-  out.initial = initial_transform_com(fixed_raw, moving_raw);
+  out.initial = initial;
   const bool same_sampling = rigid.seed == affine.seed && rigid.shrink_factors == affine.shrink_factors &&
                              rigid.smoothing_sigmas == affine.smoothing_sigmas && rigid.bins == affine.bins &&
                              rigid.sampling_percentage == affine.sampling_percentage && rigid.sigma_units == affine.sigma_units;
@@ -395,6 +407,13 @@ LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, co
   out.affine = run_linear_stage(engine, affine, cache, after_rigid, start, same_sampling ? &levels : nullptr, cancel);
   out.total = with_center(AffineTransform::compose(after_rigid, out.affine.affine), out.affine.affine.center);
   return out;
+}
+
+// The same from the raw first channel: the centre-of-mass initialisation first.
+template <typename Engine, typename B, typename V>
+LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine, const V& fixed_raw,
+                                   const V& moving_raw, SmoothCache<B>& cache, const volatile int32_t* cancel) {
+  return run_linear_chain(engine, rigid, affine, initial_transform_com(fixed_raw, moving_raw), cache, cancel);
 }
 
 }  // namespace fw

@@ -11,7 +11,6 @@ class IntensityHistogram {
  public:
   template <typename Values>
   explicit IntensityHistogram(const Values& values) {
-    constexpr size_t kBins = 256;
     // range and counts over fixed chunks, merged in chunk order
     constexpr size_t kGrain = size_t(1) << 16;
     const size_t chunks = (values.size() + kGrain - 1) / kGrain;
@@ -39,18 +38,7 @@ class IntensityHistogram {
       hi = std::max(hi, r.hi);
     }
     if (finite == 0) invalid("preprocess", "image has no finite values");
-    const double margin = ((hi - lo) / double(kBins)) / 10.0;
-    const double upper = hi + margin;
-    const float interval = (float(upper) - float(lo)) / float(kBins);
-    min_.assign(kBins, 0.0);
-    max_.assign(kBins, 0.0);
-    for (size_t j = 0; j + 1 < kBins; ++j) {
-      min_[j] = lo + double(float(j) * interval);
-      max_[j] = lo + double((float(j) + 1.0f) * interval);
-    }
-    min_[kBins - 1] = lo + double((float(kBins) - 1.0f) * interval);
-    max_[kBins - 1] = upper;
-    step_ = double(interval) > 0.0 ? double(interval) : 1.0;
+    set_edges(lo, hi);
     std::vector<uint64_t> partial(chunks * kBins, 0);
     parallel_chunks(values.size(), kGrain, [&](size_t c, size_t b, size_t e) {
       uint64_t* counts = partial.data() + c * kBins;
@@ -61,6 +49,26 @@ class IntensityHistogram {
     for (size_t c = 0; c < chunks; ++c)
       for (size_t j = 0; j < kBins; ++j) counts[j] += partial[c * kBins + j];
     freq_.assign(counts.begin(), counts.end());
+  }
+
+  // From the finite range and the bin counts computed elsewhere (the device's histogram of the
+  // same bins, see f32_thresholds).
+  IntensityHistogram(double lo, double hi, const std::vector<uint64_t>& counts) {
+    if (counts.size() != kBins) invalid("preprocess", "the histogram needs 256 counts");
+    set_edges(lo, hi);
+    freq_.assign(counts.begin(), counts.end());
+  }
+
+  // thresholds[k - 1] (k = 1..255) is the smallest f32 whose f64 value is at least the start of
+  // bin k: for an f32 intensity v, index_of(v) is the number of thresholds at or below v.
+  std::vector<float> f32_thresholds() const {
+    std::vector<float> t(kBins - 1);
+    for (size_t k = 1; k < kBins; ++k) {
+      float f = float(min_[k]);
+      if (double(f) < min_[k]) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+      t[k - 1] = f;
+    }
+    return t;
   }
 
   size_t index_of(double v) const {
@@ -117,9 +125,43 @@ class IntensityHistogram {
   }
 
  private:
+  static constexpr size_t kBins = 256;
+  // preprocess.rs:40-150: bin edges of [lo, hi + margin] in f32 steps, as ITK's histogram
+  void set_edges(double lo, double hi) {
+    const double margin = ((hi - lo) / double(kBins)) / 10.0;
+    const double upper = hi + margin;
+    const float interval = (float(upper) - float(lo)) / float(kBins);
+    min_.assign(kBins, 0.0);
+    max_.assign(kBins, 0.0);
+    for (size_t j = 0; j + 1 < kBins; ++j) {
+      min_[j] = lo + double(float(j) * interval);
+      max_[j] = lo + double((float(j) + 1.0f) * interval);
+    }
+    min_[kBins - 1] = lo + double((float(kBins) - 1.0f) * interval);
+    max_[kBins - 1] = upper;
+    step_ = double(interval) > 0.0 ? double(interval) : 1.0;
+  }
   std::vector<double> min_, max_, freq_;
   double step_ = 1.0;
 };
+
+// preprocess.rs:159-232: the quantile window rounded to f32 and the linear map of [lo, hi] onto
+// [out_min, out_max].
+struct IntensityWindow {
+  double lo = 0.0, hi = 1.0, scale = 1.0, shift = 0.0, out_min = 0.0, out_max = 1.0;
+};
+inline IntensityWindow intensity_window(const IntensityHistogram& h, float lower, float upper, double out_min = 0.0,
+                                        double out_max = 1.0) {
+  IntensityWindow w;
+  w.lo = double(float(h.quantile(double(lower))));
+  w.hi = double(float(h.quantile(double(upper))));
+  if (!(w.hi > w.lo)) invalid("preprocess", "degenerate intensity window");
+  w.scale = (out_max - out_min) / (w.hi - w.lo);
+  w.shift = out_min - w.lo * w.scale;
+  w.out_min = out_min;
+  w.out_max = out_max;
+  return w;
+}
 
 // preprocess.rs:159-232: the quantile window rounded to f32, then the linear map of [lo, hi] onto
 // [out_min, out_max] with clamping; non-finite voxels pass unchanged.
@@ -127,11 +169,8 @@ template <typename V>
 Image preprocess_volume(const V& vol, float lower, float upper, double out_min = 0.0, double out_max = 1.0) {
   HostTimer host_timer("preprocess");
   IntensityHistogram h(vol.data);
-  const double lo = double(float(h.quantile(double(lower))));
-  const double hi = double(float(h.quantile(double(upper))));
-  if (!(hi > lo)) invalid("preprocess", "degenerate intensity window");
-  const double scale = (out_max - out_min) / (hi - lo);
-  const double shift = out_min - lo * scale;
+  const IntensityWindow w = intensity_window(h, lower, upper, out_min, out_max);
+  const double lo = w.lo, hi = w.hi, scale = w.scale, shift = w.shift;
   HostVec<double> out(vol.data.size());
   parallel_for(out.size(), [&](size_t b, size_t e) {
     for (size_t i = b; i < e; ++i) {

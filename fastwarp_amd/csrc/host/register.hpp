@@ -80,7 +80,55 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
   if (plan.run_syn) {
     backend.release_idle_buffers();
     t = clock::now();
-    out.syn = syn_fields(backend, plan.syn, channels, out.total, cache, cancel, outputs);
+    out.syn = syn_fields(backend, plan.syn, out.total, cache, cancel, outputs);
+    out.seconds_syn = secs(t);
+  }
+  out.seconds_total = secs(t_all);
+  return out;
+}
+
+// The same normalisation from raw images already on the device: every channel is preprocessed
+// and the centre of mass taken on the device, and the fields stay there (forward_device,
+// inverse_device, when given). The backend provides preprocess_image and centre_of_gravity.
+template <typename B, typename E>
+RegisterOutput register_resident(B& backend, E& engine, const RegisterPlan& plan,
+                                 const std::vector<const typename B::ImageH*>& fixed,
+                                 const std::vector<const typename B::ImageH*>& moving, const volatile int32_t* cancel,
+                                 std::optional<typename B::FieldH>* forward_device = nullptr,
+                                 std::optional<typename B::FieldH>* inverse_device = nullptr) {
+  using clock = std::chrono::steady_clock;
+  using ImageH = typename B::ImageH;
+  auto secs = [](clock::time_point a) { return std::chrono::duration<double>(clock::now() - a).count(); };
+  const auto t_all = clock::now();
+  if (fixed.empty() || fixed.size() != moving.size())
+    invalid("register", "fixed and moving must have the same, non-zero number of channels");
+  if (plan.run_syn && plan.syn.channel_weights.size() != fixed.size())
+    invalid("stage plan", "the number of channels differs from the number of channel weights");
+  for (size_t c = 0; c < fixed.size(); ++c)
+    if (!fixed[c] || !moving[c]) invalid("register", "missing image");
+  RegisterOutput out;
+  auto t = clock::now();
+  std::vector<typename SmoothCache<B>::Pair> resident;
+  for (size_t c = 0; c < fixed.size(); ++c)
+    resident.push_back({std::make_shared<const ImageH>(backend.preprocess_image(*fixed[c], plan.winsorize_lower, plan.winsorize_upper)),
+                        std::make_shared<const ImageH>(backend.preprocess_image(*moving[c], plan.winsorize_lower, plan.winsorize_upper))});
+  out.seconds_preprocess = secs(t);
+  SmoothCache<B> cache(backend, std::move(resident));
+  t = clock::now();
+  const Vec3 fc = backend.centre_of_gravity(*fixed[0]), mc = backend.centre_of_gravity(*moving[0]);
+  const AffineTransform com{kIdentity3, vsub(mc, fc), fc};
+  if (plan.run_linear) {
+    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, com, cache, cancel);
+    out.initial = out.linear.initial;
+    out.total = out.linear.total;
+  } else {
+    out.initial = out.total = com;
+  }
+  out.seconds_linear = secs(t);
+  if (plan.run_syn) {
+    backend.release_idle_buffers();
+    t = clock::now();
+    out.syn = syn_fields(backend, plan.syn, out.total, cache, cancel, FieldOutputs{}, forward_device, inverse_device);
     out.seconds_syn = secs(t);
   }
   out.seconds_total = secs(t_all);

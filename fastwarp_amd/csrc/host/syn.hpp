@@ -107,23 +107,28 @@ struct Side {
 }  // namespace detail
 
 // syn.rs:776-986 `syn_fields` from a linear start `initial` (maps fixed points to moving points).
+// The channels' images come from `cache` (its channels' fixed grids must agree). The fields go to
+// the host buffers of `outputs` and, when given, stay on the device in *forward_device and
+// *inverse_device.
 template <typename B>
-SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel>& channels, const AffineTransform& initial,
-                     SmoothCache<B>& cache, const volatile int32_t* cancel, FieldOutputs outputs = {}) {
+SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initial, SmoothCache<B>& cache,
+                     const volatile int32_t* cancel, FieldOutputs outputs = {},
+                     std::optional<typename B::FieldH>* forward_device = nullptr,
+                     std::optional<typename B::FieldH>* inverse_device = nullptr) {
   using FieldH = typename B::FieldH;
   using ImageH = typename B::ImageH;
   recipe.validate();
-  if (channels.empty()) invalid("syn", "no channels");
-  if (channels.size() != recipe.channel_weights.size())
+  if (cache.channels() == 0) invalid("syn", "no channels");
+  if (cache.channels() != recipe.channel_weights.size())
     invalid("syn", "the number of channels differs from the number of channel weights");
-  const Grid fixed_grid = channels[0].fixed.grid;
-  for (const auto& c : channels)
-    if (!c.fixed.grid.approx_eq(fixed_grid, 1e-9)) invalid("syn", "fixed images of the channels differ in grid");
+  const Grid fixed_grid = cache.fixed_grid(0);
+  for (size_t c = 0; c < cache.channels(); ++c)
+    if (!cache.fixed_grid(c).approx_eq(fixed_grid, 1e-9)) invalid("syn", "fixed images of the channels differ in grid");
   double wsum = 0.0;
   for (double w : recipe.channel_weights) wsum += w;
   std::vector<double> normalised;
   for (double w : recipe.channel_weights) normalised.push_back(w / wsum);
-  const size_t nc = channels.size();
+  const size_t nc = cache.channels();
   const InvertParams inv_params;  // InvertParams::SYN
   std::optional<FieldH> f2m, f2m_inv, m2m, m2m_inv;
   SynFields out;
@@ -241,8 +246,16 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const std::vector<SynChannel
     out.levels.push_back(std::move(trace));
   }
   // syn.rs:969-980: SyN = Minv o F forward, Finv o M inverse
-  if (outputs.forward) b.download_field_to(b.compose(*m2m_inv, *f2m), outputs.forward);
-  if (outputs.inverse) b.download_field_to(b.compose(*f2m_inv, *m2m), outputs.inverse);
+  if (outputs.forward || forward_device) {
+    FieldH f = b.compose(*m2m_inv, *f2m);
+    if (outputs.forward) b.download_field_to(f, outputs.forward);
+    if (forward_device) *forward_device = std::move(f);
+  }
+  if (outputs.inverse || inverse_device) {
+    FieldH f = b.compose(*f2m_inv, *m2m);
+    if (outputs.inverse) b.download_field_to(f, outputs.inverse);
+    if (inverse_device) *inverse_device = std::move(f);
+  }
   return out;
 }
 

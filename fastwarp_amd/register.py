@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._abi import Affine, Grid, Plan, Result, dptr
-from .backend import FW_OK, HipBackend, RegError, lib
+from .backend import FW_OK, DeviceField, DeviceImage, HipBackend, RegError, lib
 
 
 def cohort_plan(channels: int = 1) -> Plan:
@@ -62,3 +62,30 @@ def register(backend: HipBackend, plan: Plan, fixed_grid: Grid, fixed, moving_gr
     if st != FW_OK:
         raise RegError(st, h.fw_last_error(backend.ctx).decode())
     return Registration(res.affine, fwd, inv, res)
+
+
+@dataclass
+class ResidentRegistration:
+    affine: Affine
+    forward: DeviceField | None  # on the fixed grid, in HBM
+    inverse: DeviceField | None
+    result: Result
+
+
+def register_resident(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], moving: list[DeviceImage],
+                      fields: bool = True, cancel: C.c_int32 | None = None) -> ResidentRegistration:
+    """One full normalisation from raw images already on the device (upload_image handles, kept);
+    the fields stay on the device."""
+    h = _sig()
+    fp = (C.c_void_p * len(fixed))(*[im.handle for im in fixed])
+    mp = (C.c_void_p * len(moving))(*[im.handle for im in moving])
+    fwd, inv = C.c_void_p(), C.c_void_p()
+    want = fields and plan.run_syn
+    res = Result()
+    st = h.fw_register_resident(backend.ctx, C.byref(plan), fp, mp, C.byref(fwd) if want else None,
+                                C.byref(inv) if want else None, C.byref(res), C.byref(cancel) if cancel is not None else None)
+    if st != FW_OK:
+        raise RegError(st, h.fw_last_error(backend.ctx).decode())
+    wrap = lambda p: DeviceField(backend, p) if p.value else None
+    return ResidentRegistration(res.affine, wrap(fwd) if want else None, wrap(inv) if want else None, res)
+

@@ -186,6 +186,12 @@ int fw_smooth_image(fw_ctx* ctx, const fw_image* src, const double sigma_mm[3], 
 /* Values of `img` at m physical points (3m f64): the nearest voxel, rounding half up, and the
  * linear interpolation (mi.rs:161-207, linear.rs:251-279); NaN outside. Either output may be NULL. */
 int fw_sample_values(fw_ctx* ctx, const fw_image* img, const double* points, int64_t m, double* nearest, double* linear);
+/* The winsorised copy of a raw image on the device: the 256-bin quantile window [lower, upper]
+ * mapped onto [0, 1] (preprocess.rs:159-232; the histogram bins equal the f64 ones for the f32
+ * intensities). Fails, as fw_register does, when a voxel is not finite. */
+int fw_preprocess_image(fw_ctx* ctx, const fw_image* raw, float lower, float upper, fw_image** out);
+/* The intensity-weighted centre of an image on the device, physical mm (init.rs:47-86). */
+int fw_centre_of_gravity(fw_ctx* ctx, const fw_image* img, double centre[3]);
 
 /* ---- whole registrations (stages.rs:286 register_stages_async with the HIP backend) ---- */
 
@@ -289,6 +295,14 @@ void fw_plan_cohort(fw_plan* plan, int32_t channels);
 int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,
                 const fw_grid* moving_grid, const double* const* moving, double* forward, double* inverse,
                 fw_result* result, const volatile int32_t* cancel);
+
+/* The same registration from raw images already on the device (fw_upload_image handles, which it
+ * does not consume): winsorising, windowing and the centre of mass run on the device, and the
+ * SyN fields stay there (*forward / *inverse, new handles, may be NULL). No counterpart in the
+ * reference, whose images cross PCIe per level: it serves callers that keep a cohort's volumes
+ * resident in HBM (and the throughput measurement of the device path). */
+int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
+                         fw_field** forward, fw_field** inverse, fw_result* result, const volatile int32_t* cancel);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,22 @@ def smooth_image(grid: Grid, data, sigma_mm) -> np.ndarray:
     return out
 
 
+def preprocess(grid: Grid, data, lower: float = 0.005, upper: float = 0.995) -> np.ndarray:
+    """Winsorise by the 256-bin histogram quantiles and window to [0, 1] (preprocess.rs:159-232), f64."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    out = np.empty(grid.n)
+    _check(lib().fwo_preprocess(C.byref(grid), dptr(d), C.c_float(lower), C.c_float(upper), dptr(out)))
+    return out
+
+
+def centre_of_gravity(grid: Grid, data) -> np.ndarray:
+    """Intensity-weighted centre in mm (init.rs:47-86), f64."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    c = np.zeros(3)
+    _check(lib().fwo_centre_of_gravity(C.byref(grid), dptr(d), dptr(c)))
+    return c
+
+
 def sample_values(grid: Grid, data, points) -> tuple[np.ndarray, np.ndarray]:
     """(nearest-voxel, linear) values at physical points, NaN outside (mi.rs:161-169, interp.rs:43-87)."""
     d = np.ascontiguousarray(data, dtype=np.float64)

@@ -78,6 +78,24 @@ int fwo_smooth_image(const fw_grid* grid, const double* data, const double sigma
   });
 }
 
+// preprocess.rs:159-232 on the host in f64 (the host code both engines share).
+int fwo_preprocess(const fw_grid* grid, const double* data, float lower, float upper, double* out) {
+  return guard([&] {
+    Image v = image_from(grid, data);
+    Image r = fw::preprocess_volume(v, lower, upper);
+    std::copy(r.data.begin(), r.data.end(), out);
+  });
+}
+
+// init.rs:47-86 on the host in f64.
+int fwo_centre_of_gravity(const fw_grid* grid, const double* data, double* centre) {
+  return guard([&] {
+    Image v = image_from(grid, data);
+    const fw::Vec3 c = fw::centre_of_gravity(v);
+    for (int a = 0; a < 3; ++a) centre[a] = c[a];
+  });
+}
+
 // mi.rs:161-169 nearest-voxel value and interp.rs:43-87 linear value at physical points, NaN outside.
 int fwo_sample_values(const fw_grid* grid, const double* data, const double* points, int64_t m, double* nearest,
                       double* linear) {

@@ -293,6 +293,32 @@ static void preprocess() {
   CHECK(p.data[0] == 0.0 && p.data[99] <= 1.0 && p.data[99] > 0.99, "window to [0, 1]");
   const Image w = preprocess_volume(v, 0.1f, 0.9f);
   CHECK(w.data[0] == 0.0 && w.data[99] == 1.0, "winsorised ends clamp");
+  // the device's f32 bin rule (count of f32 thresholds at or below v) equals index_of on f32
+  // intensities: random values over and beyond the range, every threshold and its neighbours
+  {
+    Mt19937 rt(99);
+    std::vector<double> fv(30000);
+    for (auto& x : fv) x = double(float(-0.3 + 1.9 * (rt.integer() / 4294967296.0)));
+    const IntensityHistogram hf(fv);
+    const std::vector<float> th = hf.f32_thresholds();
+    CHECK(th.size() == 255, "255 thresholds");
+    std::vector<float> probes;
+    for (double x : fv) probes.push_back(float(x));
+    for (float t : th) probes.insert(probes.end(), {t, std::nextafter(t, -1e30f), std::nextafter(t, 1e30f)});
+    probes.insert(probes.end(), {-1e30f, 1e30f, float(hf.bin_maxima().back())});
+    for (float x : probes) {
+      size_t count = 0;
+      for (float t : th) count += t <= x ? 1 : 0;
+      CHECK(count == hf.index_of(double(x)), "f32 threshold bins");
+    }
+    // the histogram rebuilt from its range and counts has the same quantiles
+    std::vector<uint64_t> counts(256, 0);
+    for (double x : fv) ++counts[hf.index_of(x)];
+    double lo = fv[0], hi = fv[0];
+    for (double x : fv) lo = std::min(lo, x), hi = std::max(hi, x);
+    const IntensityHistogram hc(lo, hi, counts);
+    for (double q : {0.005, 0.2, 0.5, 0.995}) CHECK(hc.quantile(q) == hf.quantile(q), "quantiles from counts");
+  }
   // the bin lookup equals ITK's search (the largest bin whose minimum is <= v, clamped at both
   // ends) on random values, on every bin edge and on the neighbours of each edge
   std::vector<double> r(20000);
