@@ -1003,9 +1003,16 @@ std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField
   uint32_t* mail_dev = ctx_->mailbox_device();
   const uint32_t tag = ctx_->next_tag();
   DevBuf residual(ctx_.get(), 3 * g.num_voxels());
+  // the inversion's report: a slot of a ring in coherent host memory, read after the settle
+  constexpr size_t kReports = 64;
+  void* report_dev = nullptr;
+  auto* reports = static_cast<InvertReport*>(ctx_->mapped(2, kReports * sizeof(InvertReport), &report_dev));
+  const size_t slot = tag % kReports;
+  InvertReport* report = static_cast<InvertReport*>(report_dev) + slot;
+  volatile InvertReport* host = reports + slot;
   {
     Timed t(*ctx_, "k_inv_begin", 36.0 * double(d.n()));
-    launch_inv_begin(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, partials, nb, max_it, s);
+    launch_inv_begin(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, report, partials, nb, max_it, s);
   }
   // At most kAhead launches are queued beyond the one the device has started: the host waits on
   // the progress word (a hint; a bounded wait) so that it does not queue launches past the stop.
@@ -1018,18 +1025,14 @@ std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField
     if (step > 0 && mail[0] == tag) break;
     if (step >= kAhead) {
       const auto t0 = std::chrono::steady_clock::now();
-      while (started() < step - kAhead + 1 && mail[0] != tag) {
-        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
-        std::this_thread::yield();
-      }
+      for (unsigned spins = 1; started() < step - kAhead + 1 && mail[0] != tag; ++spins)
+        if ((spins & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
       if (mail[0] == tag) break;
     }
     Timed t(*ctx_, "k_inv_iter", 0.0);
-    launch_inv_iter(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, partials, nb, step, max_it,
+    launch_inv_iter(field.buf.get(), initial.buf.get(), residual.get(), d, l, isp, state, report, partials, nb, step, max_it,
                     p.max_error_tolerance, p.mean_error_tolerance, p.enforce_boundary_condition ? 1 : 0, mail_dev, tag, s);
   }
-  auto* host = static_cast<InvertState*>(ctx_->stage(sizeof(InvertState)));
-  check(hipMemcpyAsync(host, state, sizeof(InvertState), hipMemcpyDeviceToHost, s), "invert state");
   DeviceContext* ctx = ctx_.get();
   const double voxels = double(d.n());
   const bool profiling = ctx_->profiling();

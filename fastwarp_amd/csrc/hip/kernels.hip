@@ -1134,7 +1134,8 @@ __device__ __forceinline__ void inv_coords(const Dims& d, long long g, int& i, i
 
 __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restrict__ field, const float* __restrict__ psi,
                                                            float* __restrict__ res, Dims d, L9 l, float3 inv_sp,
-                                                           InvertState* state, float* partials, int max_iterations) {
+                                                           InvertState* state, InvertReport* report, float* partials,
+                                                           int max_iterations) {
   const long long n = d.n();
   const InvRange rg = inv_range(n);
   float s = 0.0f, c = 0.0f, m = 0.0f;
@@ -1163,12 +1164,15 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
     state->executed = 0;
     state->stop[0] = max_iterations > 0 ? 0 : 1;
     state->stop[1] = 0;
+    report->mean_error = 0.0f;
+    report->max_error = 0.0f;
+    report->executed = 0;
   }
 }
 
 __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi,
                                                           float* __restrict__ res, Dims d, L9 l, float3 inv_sp,
-                                                          InvertState* state, float* partials, int step,
+                                                          InvertState* state, InvertReport* report, float* partials, int step,
                                                           int max_iterations, float max_tol, float mean_tol, int enforce,
                                                           uint32_t* mailbox, uint32_t tag) {
   if (state->stop[step & 1]) {
@@ -1207,6 +1211,9 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
     state->max_error = mx;
     state->executed = step + 1;
     state->stop[(step + 1) & 1] = next ? 0 : 1;
+    report->mean_error = mean;
+    report->max_error = mx;
+    report->executed = step + 1;
     // a hint for the host: later launches of this inversion would return at once
     if (!next) __hip_atomic_store(mailbox, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
@@ -1259,20 +1266,21 @@ int inv_blocks(const Dims& d) {
 }
 
 void launch_inv_begin(const float* field, const float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
-                      InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s) {
+                      InvertState* state, InvertReport* report, float* partials, int blocks, int max_iterations,
+                      hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  launch(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
-                     partials, max_iterations);
+  launch(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state, report,
+         partials, max_iterations);
 }
 
 void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
-                     InvertState* state, float* partials, int blocks, int step, int max_iterations, double max_tol,
-                     double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s) {
+                     InvertState* state, InvertReport* report, float* partials, int blocks, int step, int max_iterations,
+                     double max_tol, double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  launch(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state,
-                     partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
+  launch(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state, report,
+         partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
 }
 
 

@@ -157,15 +157,24 @@ struct InvertState {
   int32_t stop[2];  // stop[i & 1] set: iteration i does not run
   int32_t pad;
 };
+// What the host reads of one inversion, in coherent host memory: block 0 of the first launch and of
+// every executed step stores it, and the host reads it once the inversion's launches are done.
+struct InvertReport {
+  float mean_error;
+  float max_error;
+  int32_t executed;
+  int32_t pad;
+};
 // The residual of the initial estimate into `residual` (a field), its partials (2 * blocks floats:
-// sums, then maxima) and the state reset.
+// sums, then maxima) and the state and report reset.
 void launch_inv_begin(const float* field, const float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
-                      InvertState* state, float* partials, int blocks, int max_iterations, hipStream_t s);
+                      InvertState* state, InvertReport* report, float* partials, int blocks, int max_iterations,
+                      hipStream_t s);
 // One iteration; `partials` holds two such buffers (4 * blocks floats), alternating by step parity.
 // The launch that ends the loop stores `tag` to `mailbox` (host memory) as a hint.
 void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
-                     InvertState* state, float* partials, int blocks, int step, int max_iterations, double max_tol,
-                     double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s);
+                     InvertState* state, InvertReport* report, float* partials, int blocks, int step, int max_iterations,
+                     double max_tol, double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s);
 int inv_blocks(const Dims& d);
 
 // --- recursive Gaussian (Deriche, 4th order) along one axis, constant extension at both ends ---
