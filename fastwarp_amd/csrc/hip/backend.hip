@@ -585,20 +585,36 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
   a.d_moving = out.fields[1].buf.get();
   const dim3 gr = cc_grid(a.d, a.tile_z);
   const size_t blocks = size_t(gr.x) * gr.y * gr.z;
-  // the partials live in a buffer of this call's own: the next evaluation may run before settle()
-  DevBuf partials(ctx_.get(), 2 * blocks);
-  a.partial_cc = partials.get();
-  a.partial_count = reinterpret_cast<int*>(partials.get() + blocks);
-  {
+  const size_t real = blocks;
+  char* host = nullptr;
+  DevBuf partials;
+  if (blocks <= kCcSlotBlocks) {
+    // the kernel writes its block partials straight into a slot of a ring in coherent host memory,
+    // read after the settle (the next evaluation may run before it, in the next slot)
+    void* ring_dev = nullptr;
+    const size_t bytes = kCcSlotBlocks * (sizeof(float) + sizeof(int));
+    char* ring = static_cast<char*>(ctx_->mapped(4, kCcSlots * bytes, &ring_dev));
+    const size_t slot = ctx_->next_tag() % kCcSlots;
+    host = ring + slot * bytes;
+    a.partial_cc = reinterpret_cast<float*>(static_cast<char*>(ring_dev) + slot * bytes);
+    a.partial_count = reinterpret_cast<int*>(static_cast<char*>(ring_dev) + slot * bytes + real * sizeof(float));
     Timed t(*ctx_, "k_cc2", 56.0 * double(n));
     launch_cc_pair(a, ctx_->stream());
+  } else {
+    // the partials live in a buffer of this call's own, copied to staging memory
+    partials = DevBuf(ctx_.get(), 2 * blocks);
+    a.partial_cc = partials.get();
+    a.partial_count = reinterpret_cast<int*>(partials.get() + blocks);
+    {
+      Timed t(*ctx_, "k_cc2", 56.0 * double(n));
+      launch_cc_pair(a, ctx_->stream());
+    }
+    host = static_cast<char*>(ctx_->stage(real * (sizeof(float) + sizeof(int))));
+    check(hipMemcpyAsync(host, a.partial_cc, real * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "cc partials");
+    check(hipMemcpyAsync(host + real * sizeof(float), a.partial_count, real * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx_->stream()),
+          "cc counts");
   }
-  const size_t real = blocks;
-  char* host = static_cast<char*>(ctx_->stage(real * (sizeof(float) + sizeof(int))));
-  check(hipMemcpyAsync(host, a.partial_cc, real * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream()), "cc partials");
-  check(hipMemcpyAsync(host + real * sizeof(float), a.partial_count, real * sizeof(int), hipMemcpyDeviceToHost,
-                       ctx_->stream()),
-        "cc counts");
   out.later = Pending<std::array<double, 2>>(std::function<std::array<double, 2>()>([host, real] {
     const float* pcc = reinterpret_cast<const float*>(host);
     const int* pcount = reinterpret_cast<const int*>(host + real * sizeof(float));
@@ -621,23 +637,22 @@ Pending<std::pair<double, double>> HipBackend::scale_to_step(HField& a, HField& 
   const int nb = norm_blocks(n);
   float* pm = static_cast<float*>(ctx_->scratch(2, 2 * nb * sizeof(float)));
   float* ps = static_cast<float*>(ctx_->scratch(3, 2 * nb * sizeof(float)));
-  DevBuf out(ctx_.get(), 4);  // two factors, two maxima
+  DevBuf factors(ctx_.get(), 2);
   hipStream_t s = ctx_->stream();
   {
     Timed t(*ctx_, "k_field_norms", 24.0 * double(n));
     launch_field_norms(a.buf.get(), b.buf.get(), n, inv_spacing(a.grid), pm, ps, nb, s);
   }
-  launch_step_factors(pm, nb, 2, step, out.get(), out.get() + 2, s);
+  // the maxima go to a slot of a ring in coherent host memory, read after the settle
+  void* ring_dev = nullptr;
+  auto* ring = static_cast<float*>(ctx_->mapped(3, kMaximaSlots * 2 * sizeof(float), &ring_dev));
+  const size_t slot = ctx_->next_tag() % kMaximaSlots;
+  launch_step_factors(pm, nb, 2, step, factors.get(), static_cast<float*>(ring_dev) + 2 * slot, s);
   {
-    Timed t(*ctx_, "k_scale_by", 24.0 * double(n));
-    launch_scale_by(a.buf.get(), 3 * n, out.get(), s);
+    Timed t(*ctx_, "k_scale_by", 48.0 * double(n));
+    launch_scale_by2(a.buf.get(), b.buf.get(), 3 * n, factors.get(), s);
   }
-  {
-    Timed t(*ctx_, "k_scale_by", 24.0 * double(n));
-    launch_scale_by(b.buf.get(), 3 * n, out.get() + 1, s);
-  }
-  float* host = static_cast<float*>(ctx_->stage(2 * sizeof(float)));
-  check(hipMemcpyAsync(host, out.get() + 2, 2 * sizeof(float), hipMemcpyDeviceToHost, s), "update maxima");
+  volatile float* host = ring + 2 * slot;
   return Pending<std::pair<double, double>>(
       std::function<std::pair<double, double>()>([host] { return std::make_pair(double(host[0]), double(host[1])); }));
 }

@@ -113,7 +113,7 @@ class DeviceContext : public HostSpanSink {
     void* dev = nullptr;
     size_t bytes = 0;
   };
-  Mapped mapped_[4];
+  Mapped mapped_[6];
   static constexpr int kDoneBase = 64;  // words 0..63: the inversion's progress and stop words
   static constexpr size_t kMailboxBytes = sizeof(uint32_t) * (kDoneBase + kDoneWords);
   uint32_t* mailbox_ = nullptr;
@@ -296,6 +296,10 @@ class HipBackend {
 
   // CC tile height in z of the fused kernel.
   int cc_tile_z = 16;
+  // Slots of the coherent-host rings of update maxima and CC partials (results awaiting a settle).
+  static constexpr size_t kMaximaSlots = 64;
+  static constexpr size_t kCcSlots = 4;
+  static constexpr size_t kCcSlotBlocks = size_t(1) << 16;
 
  private:
   std::shared_ptr<DeviceContext> ctx_;

@@ -367,6 +367,17 @@ __global__ __launch_bounds__(256) void k_scale_by(float* __restrict__ x, long lo
     x[i] *= f;
 }
 
+// x *= factor[0] and y *= factor[1] in one launch: the first half of the grid takes x.
+__global__ __launch_bounds__(256) void k_scale_by2(float* __restrict__ x, float* __restrict__ y, long long count,
+                                                   const float* __restrict__ factor) {
+  const unsigned half = gridDim.x / 2;
+  const bool second = blockIdx.x >= half;
+  float* v = second ? y : x;
+  const float f = factor[second ? 1 : 0];
+  const long long b = second ? blockIdx.x - half : blockIdx.x;
+  for (long long i = b * blockDim.x + threadIdx.x; i < count; i += (long long)half * blockDim.x) v[i] *= f;
+}
+
 __global__ __launch_bounds__(256) void k_abs_max(const float* __restrict__ x, long long n, float* partial) {
   float m = 0.0f;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -611,6 +622,10 @@ static int stream_blocks(long long count) {
 
 void launch_scale_by(float* x, long long count, const float* factor, hipStream_t st) {
   launch(k_scale_by, dim3(stream_blocks(count)), dim3(256), 0, st, x, count, factor);
+}
+
+void launch_scale_by2(float* x, float* y, long long count, const float* factors, hipStream_t st) {
+  launch(k_scale_by2, dim3(2 * stream_blocks(count)), dim3(256), 0, st, x, y, count, factors);
 }
 
 void launch_scale(float* x, long long count, float s, hipStream_t st) {

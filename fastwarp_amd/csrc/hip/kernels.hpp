@@ -99,6 +99,8 @@ void launch_sample_values(const float* img, Dims d, const float4* idx, long long
 void launch_scale(float* x, long long count, float s, hipStream_t st);
 // x *= *factor, the factor read on the device.
 void launch_scale_by(float* x, long long count, const float* factor, hipStream_t st);
+// x *= factors[0], y *= factors[1] (both of `count` floats) in one launch.
+void launch_scale_by2(float* x, float* y, long long count, const float* factors, hipStream_t st);
 // Per field f < fields: maxima[f] = max of partial_max[f * blocks ...] (NaN propagates),
 // factors[f] = step / maxima[f] (step when not positive). One block.
 void launch_step_factors(const float* partial_max, int blocks, int fields, double step, float* factors, float* maxima,
