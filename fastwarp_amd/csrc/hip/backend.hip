@@ -446,6 +446,23 @@ HImage HipBackend::warp_through_field(const HImage& src, const HField& field, co
   return out;
 }
 
+std::pair<HImage, HImage> HipBackend::warp_pair(const HImage& src0, const HField& field0, const AffineTransform* affine0,
+                                                const HImage& src1, const HField& field1, const AffineTransform* affine1) {
+  if (field0.grid != field1.grid)
+    return {warp_through_field(src0, field0, nullptr, affine0), warp_through_field(src1, field1, nullptr, affine1)};
+  WarpArgs a0 = warp_args(src0.grid, field0, nullptr, affine0), a1 = warp_args(src1.grid, field1, nullptr, affine1);
+  const Grid& g = field0.grid;
+  HImage out0{g, DevBuf(ctx_.get(), g.num_voxels()), src0.absmax}, out1{g, DevBuf(ctx_.get(), g.num_voxels()), src1.absmax};
+  a0.src = src0.buf.get();
+  a0.out = out0.buf.get();
+  a1.src = src1.buf.get();
+  a1.out = out1.buf.get();
+  const double no = double(g.num_voxels());
+  Timed t(*ctx_, "k_warp", 32.0 * no + 4.0 * (std::min(no, double(src0.grid.num_voxels())) + std::min(no, double(src1.grid.num_voxels()))));
+  launch_warp2(a0, a1, ctx_->stream());
+  return {std::move(out0), std::move(out1)};
+}
+
 HImage HipBackend::warp_mask_through_field(const HImage& mask, const HField& field, const HField* pre,
                                            const AffineTransform* affine, HImage* product) {
   if (product && product->grid != field.grid) invalid("warp mask", "the product image is not on the grid of the field");
@@ -868,6 +885,38 @@ bool HipBackend::finite_range(const HImage& img, double& lo, double& hi) {
   return true;
 }
 
+std::pair<HField, HField> HipBackend::smooth_field_pair(HField a, HField b, double variance) {
+  if (variance <= 0.0) return {std::move(a), std::move(b)};
+  const Grid& g = a.grid;
+  if (b.grid != g) return {smooth_field(std::move(a), variance), smooth_field(std::move(b), variance)};
+  const Dims d = dims_of(g);
+  const size_t n = g.num_voxels();
+  const double w1 = variance < 0.5 ? 1.0 - variance / 0.5 : 1.0;
+  const double w2 = 1.0 - w1;
+  GaussTaps3 t{};
+  int* radii[3] = {&t.rx, &t.ry, &t.rz};
+  float* dst[3] = {t.x, t.y, t.z};
+  for (int q = 0; q < 3; ++q) {
+    const std::vector<double>& taps = ctx_->host_taps(variance, g.size[q]);
+    *radii[q] = int(taps.size() / 2);
+    if (taps.size() > 17) return {smooth_field(std::move(a), variance), smooth_field(std::move(b), variance)};
+    for (size_t k = 0; k < taps.size(); ++k) dst[q][k] = float(taps[k]);
+  }
+  if (!gauss3_fits(d, t)) return {smooth_field(std::move(a), variance), smooth_field(std::move(b), variance)};
+  HField out0{g, DevBuf(ctx_.get(), 3 * n)}, out1{g, DevBuf(ctx_.get(), 3 * n)};
+  DevBuf tmp0(ctx_.get(), 3 * n), tmp1(ctx_.get(), 3 * n);
+  {
+    Timed tm(*ctx_, "k_gauss_xy", 48.0 * double(n));
+    launch_gauss_xy(a.buf.get(), tmp0.get(), d, t, ctx_->stream(), b.buf.get(), tmp1.get());
+  }
+  {
+    Timed tm(*ctx_, "k_gauss_z", 48.0 * double(n) + (w2 != 0.0 ? 24.0 * double(n) : 0.0));
+    launch_gauss_z(tmp0.get(), a.buf.get(), out0.buf.get(), d, t, float(w1), float(w2), ctx_->stream(), tmp1.get(),
+                   b.buf.get(), out1.buf.get());
+  }
+  return {std::move(out0), std::move(out1)};
+}
+
 HField HipBackend::smooth_field(HField field, double variance) {
   if (variance <= 0.0) return field;
   const Grid& g = field.grid;
@@ -997,6 +1046,19 @@ std::pair<HField, InvertStats> HipBackend::invert(const HField& field, HField in
   auto r = invert_deferred(field, std::move(initial), p);
   ctx_->settle();
   return {std::move(r.first), r.second.get()};
+}
+
+std::pair<HField, HField> HipBackend::compose_pair(const HField& u0, const HField& w0, const HField& u1, const HField& w1) {
+  if (u0.grid != w0.grid || u1.grid != w1.grid) invalid("compose", "displacement and warping grids differ");
+  if (w0.grid != w1.grid) return {compose(u0, w0), compose(u1, w1)};
+  const Grid& g = w0.grid;
+  HField out0{g, DevBuf(ctx_.get(), 3 * g.num_voxels())}, out1{g, DevBuf(ctx_.get(), 3 * g.num_voxels())};
+  float l[9];
+  put(l, g.index_from_offset);
+  Timed t(*ctx_, "k_compose", 72.0 * double(g.num_voxels()));
+  launch_compose2(u0.buf.get(), w0.buf.get(), out0.buf.get(), u1.buf.get(), w1.buf.get(), out1.buf.get(), dims_of(g), l,
+                  ctx_->stream());
+  return {std::move(out0), std::move(out1)};
 }
 
 std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField& field, HField initial,

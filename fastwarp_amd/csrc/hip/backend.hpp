@@ -272,6 +272,12 @@ class HipBackend {
                      HostVec<double>& linear);
   // Finite minimum and maximum; false when the image has no finite value.
   bool finite_range(const HImage& img, double& lo, double& hi);
+  // Both sides of a SyN step in single launches (the SyN driver's pair operations): two warps onto
+  // one grid, two compositions, two update smoothings. Each result equals the single operation's.
+  std::pair<HImage, HImage> warp_pair(const HImage& src0, const HField& field0, const AffineTransform* affine0,
+                                      const HImage& src1, const HField& field1, const AffineTransform* affine1);
+  std::pair<HField, HField> compose_pair(const HField& u0, const HField& w0, const HField& u1, const HField& w1);
+  std::pair<HField, HField> smooth_field_pair(HField a, HField b, double variance);
   // The winsorised, windowed copy of an image already in HBM (preprocess.rs:159-232), raising the
   // registration's error for non-finite voxels; and its intensity-weighted centre (init.rs:47-86).
   HImage preprocess_image(const HImage& raw, float lower, float upper, double out_min = 0.0, double out_max = 1.0);

@@ -641,12 +641,7 @@ void launch_axpy(float* acc, const float* x, long long count, float a, hipStream
 // continuous index to_src.m * x + to_src.c0 + to_src.l * (u(x) [+ pre(q)]), trilinear inside the
 // buffer and zero outside; in mask mode the nearest voxel decides {0, 1}.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_warp(WarpArgs a) {
-  const uint3 bl = xcd_block();
-  const int i = bl.x * 64 + (threadIdx.x & 63);
-  const int j = bl.y * 4 + (threadIdx.x >> 6);
-  const int k = bl.z;
-  if (i >= a.od.nx || j >= a.od.ny) return;
+__device__ __forceinline__ void warp_at(const WarpArgs& a, int i, int j, int k) {
   const long long g = lin(a.od, i, j, k);
   const long long n = a.od.n();
   const float ux = a.field[g], uy = a.field[g + n], uz = a.field[g + 2 * n];
@@ -685,9 +680,33 @@ __global__ __launch_bounds__(256) void k_warp(WarpArgs a) {
   a.out[g] = v;
 }
 
+__global__ __launch_bounds__(256) void k_warp(WarpArgs a) {
+  const uint3 bl = xcd_block();
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  if (i >= a.od.nx || j >= a.od.ny) return;
+  warp_at(a, i, j, bl.z);
+}
+
+// Two warps onto one output grid in one launch: planes [0, nz) of the grid take a0, [nz, 2 nz) a1.
+__global__ __launch_bounds__(256) void k_warp2(WarpArgs a0, WarpArgs a1) {
+  const uint3 bl = xcd_block();
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  const bool second = (int)bl.z >= a0.od.nz;
+  const WarpArgs& a = second ? a1 : a0;
+  if (i >= a.od.nx || j >= a.od.ny) return;
+  warp_at(a, i, j, second ? (int)bl.z - a0.od.nz : (int)bl.z);
+}
+
 void launch_warp(const WarpArgs& a, hipStream_t s) {
   const dim3 grid((a.od.nx + 63) / 64, (a.od.ny + 3) / 4, a.od.nz);
   launch(k_warp, grid, dim3(256), 0, s, a);
+}
+
+void launch_warp2(const WarpArgs& a0, const WarpArgs& a1, hipStream_t s) {
+  const dim3 grid((a0.od.nx + 63) / 64, (a0.od.ny + 3) / 4, 2 * a0.od.nz);
+  launch(k_warp2, grid, dim3(256), 0, s, a0, a1);
 }
 
 // Field resampling onto another grid (field_ops.rs:580-605): linear, zero outside the old buffer.
@@ -743,12 +762,19 @@ struct L9 {
   float v[9];
 };
 
-__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u, const float* __restrict__ w, Dims d, L9 l,
-                                                 float* __restrict__ out) {
+// Planes [0, nz) of the grid compose (u, w) into out; with a second triple, planes [nz, 2 nz)
+// compose (u1, w1) into out1.
+__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u0, const float* __restrict__ w0, Dims d, L9 l,
+                                                 float* __restrict__ out0, const float* __restrict__ u1,
+                                                 const float* __restrict__ w1, float* __restrict__ out1) {
   const uint3 bl = xcd_block();
   const int i = bl.x * 64 + (threadIdx.x & 63);
   const int j = bl.y * 4 + (threadIdx.x >> 6);
-  const int k = bl.z;
+  const bool second = (int)bl.z >= d.nz;
+  const int k = second ? (int)bl.z - d.nz : (int)bl.z;
+  const float* u = second ? u1 : u0;
+  const float* w = second ? w1 : w0;
+  float* out = second ? out1 : out0;
   if (i >= d.nx || j >= d.ny) return;
   const long long g = lin(d, i, j, k), n = d.n();
   float ox, oy, oz;
@@ -762,7 +788,16 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
   const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
-  launch(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out);
+  launch(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out, (const float*)nullptr, (const float*)nullptr,
+         (float*)nullptr);
+}
+
+void launch_compose2(const float* u0, const float* w0, float* out0, const float* u1, const float* w1, float* out1, Dims d,
+                     const float l[9], hipStream_t s) {
+  L9 lv;
+  for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
+  const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, 2 * d.nz);
+  launch(k_compose, grid, dim3(256), 0, s, u0, w0, d, lv, out0, u1, w1, out1);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -778,18 +813,21 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
 // ------------------------------------------------------------------------------------------
 constexpr int kG3TX = 64, kG3TY = 16, kG3MaxR = 8;
 
+// Grid z runs over component x plane; components 3..5 (a second field, when the grid has 6 nz
+// planes) read in1 and write out1.
 __global__ __launch_bounds__(256) void k_gauss_xy(const float* __restrict__ in, float* __restrict__ out, Dims d,
-                                                  GaussTaps3 t) {
+                                                  GaussTaps3 t, const float* __restrict__ in1, float* __restrict__ out1) {
   __shared__ float tile[kG3TY + 2 * kG3MaxR][kG3TX + 2 * kG3MaxR];
   __shared__ float xs[kG3TY + 2 * kG3MaxR][kG3TX];
   const int rx = t.rx, ry = t.ry;
   const int LX = kG3TX + 2 * rx, LY = kG3TY + 2 * ry;
   const uint3 bl = xcd_block();
-  const int comp = bl.z / d.nz, z = bl.z - comp * d.nz;
+  const int cz = bl.z / d.nz, z = bl.z - cz * d.nz;
+  const int comp = cz % 3;
   const int x0 = bl.x * kG3TX, y0 = bl.y * kG3TY;
   const long long n = d.n();
-  const float* src = in + comp * n + (long long)d.nx * d.ny * z;
-  float* dst = out + comp * n + (long long)d.nx * d.ny * z;
+  const float* src = (cz < 3 ? in : in1) + comp * n + (long long)d.nx * d.ny * z;
+  float* dst = (cz < 3 ? out : out1) + comp * n + (long long)d.nx * d.ny * z;
   const int tid = threadIdx.x;
   for (int e = tid; e < LX * LY; e += 256) {
     const int ly = e / LX, lx = e - ly * LX;
@@ -847,12 +885,21 @@ __global__ __launch_bounds__(256) void k_gauss_xy(const float* __restrict__ in, 
 }
 
 template <int RZ>
+// Grid z runs over z chunk x component; with `fields` 2, components 3..5 read in1 / orig1 and
+// write out1.
 __global__ __launch_bounds__(256) void k_gauss_z(const float* __restrict__ in, const float* __restrict__ orig,
                                                  float* __restrict__ out, Dims d, GaussTaps3 t, int zrun, float w1,
-                                                 float w2) {
+                                                 float w2, int fields, const float* __restrict__ in1,
+                                                 const float* __restrict__ orig1, float* __restrict__ out1) {
   constexpr int W = 2 * RZ + 1;
   const uint3 bl = xcd_block();
-  const int comp = bl.z % 3, chunk = bl.z / 3;
+  const int cz = bl.z % (3 * fields), chunk = bl.z / (3 * fields);
+  const int comp = cz % 3;
+  if (cz >= 3) {
+    in = in1;
+    orig = orig1;
+    out = out1;
+  }
   const int x = bl.x * 64 + (threadIdx.x & 63);
   const int y = bl.y * 4 + (threadIdx.x >> 6);
   if (x >= d.nx || y >= d.ny) return;
@@ -899,22 +946,25 @@ static int gz_zrun(const Dims& d, int rz) {
 }
 
 bool gauss3_fits(const Dims& d, const GaussTaps3& t) {
-  return t.rx <= kG3MaxR && t.ry <= kG3MaxR && t.rz <= kG3MaxR && t.rz >= 1 && 3LL * d.nz <= 65535;
+  return t.rx <= kG3MaxR && t.ry <= kG3MaxR && t.rz <= kG3MaxR && t.rz >= 1 && 6LL * d.nz <= 65535;
 }
 
-void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s) {
-  const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * d.nz);
-  launch(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t);
+void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s, const float* in1,
+                     float* tmp1) {
+  const int fields = in1 ? 2 : 1;
+  const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * fields * d.nz);
+  launch(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t, in1, tmp1);
 }
 
 void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
-                    hipStream_t s) {
+                    hipStream_t s, const float* tmp1, const float* in1, float* out1) {
   const int run = gz_zrun(d, t.rz);
-  const dim3 gz((d.nx + 63) / 64, (d.ny + 3) / 4, 3 * ((d.nz + run - 1) / run));
+  const int fields = tmp1 ? 2 : 1;
+  const dim3 gz((d.nx + 63) / 64, (d.ny + 3) / 4, 3 * fields * ((d.nz + run - 1) / run));
   switch (t.rz) {
 #define FW_GZ_CASE(R)                                                                              \
   case R:                                                                                          \
-    launch(k_gauss_z<R>, gz, dim3(256), 0, s, tmp, in, out, d, t, run, w1, w2);        \
+    launch(k_gauss_z<R>, gz, dim3(256), 0, s, tmp, in, out, d, t, run, w1, w2, fields, tmp1, in1, out1); \
     break;
     FW_GZ_CASE(1)
     FW_GZ_CASE(2)

@@ -123,6 +123,8 @@ struct WarpArgs {
   float* out;
 };
 void launch_warp(const WarpArgs& a, hipStream_t s);
+// Two warps with the same output dims in one launch.
+void launch_warp2(const WarpArgs& a0, const WarpArgs& a1, hipStream_t s);
 
 struct ResampleArgs {
   const float* src;  // 3 planes
@@ -135,6 +137,9 @@ void launch_resample(const ResampleArgs& a, hipStream_t s);
 
 // --- compose: out = w + u(x + w) on one grid; l maps a displacement to index offsets ---
 void launch_compose(const float* displacement, const float* warping, Dims d, const float l[9], float* out, hipStream_t s);
+// Two compositions on one grid in one launch.
+void launch_compose2(const float* u0, const float* w0, float* out0, const float* u1, const float* w1, float* out1, Dims d,
+                     const float l[9], hipStream_t s);
 
 // --- Gaussian smoothing, separable, clamped edges; final pass zeroes faces and blends ---
 void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const float* taps_dev, int radius,
@@ -146,9 +151,12 @@ struct GaussTaps3 {
   int rx, ry, rz;
 };
 bool gauss3_fits(const Dims& d, const GaussTaps3& t);
-void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s);
+// With in1 / tmp1 (and tmp1 / in1 / out1 for the z pass) a second field of the same grid is
+// smoothed in the same launch.
+void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s, const float* in1 = nullptr,
+                     float* tmp1 = nullptr);
 void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
-                    hipStream_t s);
+                    hipStream_t s, const float* tmp1 = nullptr, const float* in1 = nullptr, float* out1 = nullptr);
 
 // --- inversion (ITK InvertDisplacementFieldImageFilter) ---
 struct InvertState {

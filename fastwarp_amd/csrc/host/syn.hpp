@@ -84,6 +84,35 @@ struct FieldOutputs {
 
 namespace detail {
 
+// Backends that run both sides of a step in single launches (warp_pair, compose_pair,
+// smooth_field_pair); the others get the two single operations, in the same order.
+template <typename B, typename = void>
+struct has_pair_ops : std::false_type {};
+template <typename B>
+struct has_pair_ops<B, std::void_t<decltype(&B::compose_pair), decltype(&B::warp_pair), decltype(&B::smooth_field_pair)>>
+    : std::true_type {};
+
+template <typename B>
+std::pair<typename B::ImageH, typename B::ImageH> warp_both(B& b, const typename B::ImageH& f, const typename B::FieldH& finv,
+                                                            const typename B::ImageH& m, const typename B::FieldH& minv,
+                                                            const AffineTransform& initial) {
+  if constexpr (has_pair_ops<B>::value) return b.warp_pair(f, finv, nullptr, m, minv, &initial);
+  else return {b.warp_through_field(f, finv, nullptr, nullptr), b.warp_through_field(m, minv, nullptr, &initial)};
+}
+
+template <typename B>
+std::pair<typename B::FieldH, typename B::FieldH> compose_both(B& b, const typename B::FieldH& u0, const typename B::FieldH& w0,
+                                                               const typename B::FieldH& u1, const typename B::FieldH& w1) {
+  if constexpr (has_pair_ops<B>::value) return b.compose_pair(u0, w0, u1, w1);
+  else return {b.compose(u0, w0), b.compose(u1, w1)};
+}
+
+template <typename B>
+std::pair<typename B::FieldH, typename B::FieldH> smooth_both(B& b, typename B::FieldH f0, typename B::FieldH f1, double variance) {
+  if constexpr (has_pair_ops<B>::value) return b.smooth_field_pair(std::move(f0), std::move(f1), variance);
+  else return {b.smooth_field(std::move(f0), variance), b.smooth_field(std::move(f1), variance)};
+}
+
 // syn.rs:203-297 `SideAccumulator` / `SmoothedSide` for one side of the update.
 template <typename B>
 struct Side {
@@ -170,8 +199,8 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
       const bool single = nc == 1;
       std::vector<Pending<std::array<double, 2>>> later(nc);
       for (size_t c = 0; c < nc; ++c) {
-        ImageH fmid = b.warp_through_field(*smooth[c].fixed, *f2m_inv, nullptr, nullptr);
-        ImageH mmid = b.warp_through_field(*smooth[c].moving, *m2m_inv, nullptr, &initial);
+        auto mids = detail::warp_both(b, *smooth[c].fixed, *f2m_inv, *smooth[c].moving, *m2m_inv, initial);
+        ImageH fmid = std::move(mids.first), mmid = std::move(mids.second);
         // 2. the metric both ways (syn.rs:337-420 `side_updates`); several channels need the
         // derivative norms for their weights at once, a single one needs its values only later
         MetricPair<FieldH> pair;
@@ -192,7 +221,11 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
         detail::Side<B>& side = *sides[s];
         if (!side.gradient) invalid("syn", "no channel produced a gradient");
         if (nc > 1) b.scale(*side.gradient, side.total_magnitude / double(nc));
-        grads[s] = b.smooth_field(std::move(*side.gradient), recipe.update_field_variance);
+      }
+      {
+        auto smoothed = detail::smooth_both(b, std::move(*fs.gradient), std::move(*ms.gradient), recipe.update_field_variance);
+        grads[0] = std::move(smoothed.first);
+        grads[1] = std::move(smoothed.second);
       }
       // 3. scale each update so its largest voxel step is the gradient step
       Pending<std::pair<double, double>> maxima = b.scale_to_step(grads[0], grads[1], recipe.gradient_step);
@@ -201,10 +234,13 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
       std::optional<FieldH>* fields[2] = {&f2m, &m2m};
       std::optional<FieldH>* inverses[2] = {&f2m_inv, &m2m_inv};
       Pending<InvertStats> stats[4];
+      auto composed = detail::compose_both(b, grads[0], **fields[0], grads[1], **fields[1]);
+      fields[0]->reset();
+      fields[1]->reset();
+      auto totals = detail::smooth_both(b, std::move(composed.first), std::move(composed.second), recipe.total_field_variance);
+      FieldH tmps[2] = {std::move(totals.first), std::move(totals.second)};
       for (int s = 0; s < 2; ++s) {
-        FieldH composed = b.compose(grads[s], **fields[s]);
-        fields[s]->reset();
-        FieldH tmp = b.smooth_field(std::move(composed), recipe.total_field_variance);
+        FieldH tmp = std::move(tmps[s]);
         auto inv = b.invert_deferred(tmp, std::move(**inverses[s]), inv_params);
         auto fwd = b.invert_deferred(inv.first, std::move(tmp), inv_params);
         *inverses[s] = std::move(inv.first);
