@@ -183,6 +183,16 @@ int mi_sampled_blocks(int count) {
   return b < 1 ? 1 : (b > 512 ? 512 : b);
 }
 
+#ifndef FW_MI_WIDE_THREADS
+#define FW_MI_WIDE_THREADS 1024
+#endif
+
+// Threads per block of the histogram passes: 256 while that gives each thread at most two samples,
+// wider beyond. A wider block keeps the number of block histograms (and of slots to add) and puts
+// more loads in flight per CU: at 512 blocks of 256 threads a CU holds two waves per SIMD, and the
+// passes are bound by the latency of the sample and moving-image loads.
+int mi_sampled_threads(int count) { return count > 2 * 512 * 256 ? FW_MI_WIDE_THREADS : 256; }
+
 // The derivative kernel's partials are reduced on the device, so its grid may be wider.
 int mi_deriv_blocks(int count) {
   int b = (count + 255) / 256;
@@ -191,7 +201,7 @@ int mi_deriv_blocks(int count) {
 
 // The block's histogram (LDS, 64-bit atomics) is written to its own slot of `hist_slots`; a second
 // kernel adds the slots, so the global histogram needs neither clearing nor global atomics.
-__global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
+__global__ __launch_bounds__(1024) void k_mi_sampled_hist(MiSampledArgs a) {
   extern __shared__ unsigned long long h[];
   const int bins = a.bins;
   const int cells = bins * bins + bins;
@@ -221,7 +231,7 @@ __global__ __launch_bounds__(256) void k_mi_sampled_hist(MiSampledArgs a) {
 // K transforms per pass: each sample record is read once and binned under every transform, into
 // the K histograms of the block's LDS (the samples, the anchor and the moving image are shared).
 template <int K>
-__global__ __launch_bounds__(256) void k_mi_sampled_hist_batch(MiSampledArgs a, MiBatch batch) {
+__global__ __launch_bounds__(1024) void k_mi_sampled_hist_batch(MiSampledArgs a, MiBatch batch) {
   extern __shared__ unsigned long long h[];
   const int bins = a.bins;
   const int cells = bins * bins + bins;
@@ -271,17 +281,18 @@ int mi_batch_capacity(int bins) {
 
 void launch_mi_sampled_hist_batch(const MiSampledArgs& a, const MiBatch& batch, int count, int blocks, hipStream_t s) {
   const size_t lds = size_t(count) * size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
+  const dim3 t(mi_sampled_threads(a.count));
   switch (count) {
-    case 2: launch(k_mi_sampled_hist_batch<2>, dim3(blocks), dim3(256), (uint32_t)lds, s, a, batch); break;
-    case 3: launch(k_mi_sampled_hist_batch<3>, dim3(blocks), dim3(256), (uint32_t)lds, s, a, batch); break;
-    case 4: launch(k_mi_sampled_hist_batch<4>, dim3(blocks), dim3(256), (uint32_t)lds, s, a, batch); break;
+    case 2: launch(k_mi_sampled_hist_batch<2>, dim3(blocks), t, (uint32_t)lds, s, a, batch); break;
+    case 3: launch(k_mi_sampled_hist_batch<3>, dim3(blocks), t, (uint32_t)lds, s, a, batch); break;
+    case 4: launch(k_mi_sampled_hist_batch<4>, dim3(blocks), t, (uint32_t)lds, s, a, batch); break;
     default: break;
   }
 }
 
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
   const size_t lds = size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
-  launch(k_mi_sampled_hist, dim3(blocks), dim3(256), lds, s, a);
+  launch(k_mi_sampled_hist, dim3(blocks), dim3(mi_sampled_threads(a.count)), lds, s, a);
 }
 
 // hist[c] = sum of the slots' cell c, for every cell and the valid count. A block of 16 waves

@@ -55,6 +55,7 @@ void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells,
                          uint32_t tag, hipStream_t s);
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
 int mi_sampled_blocks(int count);
+int mi_sampled_threads(int count);
 constexpr int kMiDerivMaxBlocks = 2048;
 int mi_deriv_blocks(int count);
 // out[k] (k < np) = the blocks' derivative partials added in a fixed order; then done[0] = tag

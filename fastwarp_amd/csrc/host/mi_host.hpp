@@ -91,8 +91,12 @@ struct HistogramFinish {
 inline HistogramFinish finish_histogram(size_t bins, double moving_bin_size, uint64_t num_valid, std::vector<double> joint,
                                         std::vector<double> fixed_marginal, bool want_ratio) {
   if (num_valid == 0) invalid("mattes", "all samples map outside the moving image buffer");
+  // Kahan sum. A zero term with sum - comp == sum in f64 leaves both unchanged (y = -comp,
+  // t = sum, comp' = (t - sum) - y = comp), so such terms are skipped: most of a joint
+  // histogram is empty, and the result is that of the full sequence.
   double sum = 0.0, comp = 0.0;
   for (double v : joint) {
+    if (v == 0.0 && sum - comp == sum) continue;
     const double y = v - comp;
     const double t = sum + y;
     comp = (t - sum) - y;
@@ -103,8 +107,8 @@ inline HistogramFinish finish_histogram(size_t bins, double moving_bin_size, uin
   for (double& v : joint) v *= norm;
   for (double& v : fixed_marginal) v *= norm;
   std::vector<double> mm(bins, 0.0);
-  for (size_t c = 0; c < bins; ++c)
-    for (size_t r = 0; r < bins; ++r) mm[c] += joint[r * bins + c];
+  for (size_t r = 0; r < bins; ++r)
+    for (size_t c = 0; c < bins; ++c) mm[c] += joint[r * bins + c];
   const double n_factor = -1.0 / (moving_bin_size * double(num_valid));
   HistogramFinish out;
   if (want_ratio) out.ratio.assign(bins * bins, 0.0);
