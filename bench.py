@@ -99,6 +99,37 @@ def cpu_baseline(pair, gpu_result) -> dict:
     }
 
 
+def served(pair, plan, slots: int, requests: int) -> dict:
+    """The same workload as a served stream (fw_server_*): `requests` registrations submitted at
+    once to `slots` slots, f32 volumes in from host memory and both fields back to host memory per
+    request, against the same requests through a single slot (one at a time, identical work)."""
+    from fastwarp_amd.serve import Server
+
+    f = [np.asarray(v, dtype=np.float32) for v in pair.fixed]
+    m = [np.asarray(v, dtype=np.float32) for v in pair.moving]
+
+    def stream(n_slots: int, n: int):
+        with Server(slots=n_slots) as s:
+            for t in [s.submit(plan, pair.grid, f, pair.grid, m) for _ in range(n_slots)]:  # every slot warm
+                s.wait(t)
+            t0 = time.perf_counter()
+            got = [s.wait(t) for t in [s.submit(plan, pair.grid, f, pair.grid, m) for _ in range(n)]]
+            return time.perf_counter() - t0, got
+
+    one_s, _ = stream(1, max(2, requests // 3))
+    one = max(2, requests // 3) / one_s * 3600.0
+    wall, got = stream(slots, requests)
+    lat = np.array([g.timing.total_s for g in got])
+    value = requests / wall * 3600.0
+    log(f"served: {slots} slots, {requests} requests in {wall:.2f} s ({value:.0f}/h; one slot {one:.0f}/h)")
+    return {"value": round(value, 3), "unit": "registrations/hour", "slots": slots, "requests": requests,
+            "latency_s": {q: round(float(np.percentile(lat, int(q[1:]))), 3) for q in ("p50", "p95", "p99")},
+            "run_s_mean": round(float(np.mean([g.timing.run_s for g in got])), 3),
+            "one_slot_value": round(one, 3), "vs_one_slot": round(value / one, 3),
+            "path": "fw_server: requests submitted at once, admitted by HBM footprint to slots with their own HIP streams, "
+                    "f32 volumes uploaded per request, both fields downloaded to the host per request"}
+
+
 def kernels(repeats: int) -> int:
     """Each hot-path op on the 1 mm pair at full resolution (182x218x182), `repeats` launches
     after one warm-up: one JSON line per kernel family with its mean launch time (HIP events on
@@ -166,6 +197,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-engine baseline")
     ap.add_argument("--config", default="1mm", choices=["1mm", "2mm"])
     ap.add_argument("--kernels", action="store_true", help="time each hot-path kernel at 1 mm instead")
+    ap.add_argument("--slots", type=int, default=4, help="slots of the served stream (0 skips it)")
+    ap.add_argument("--requests", type=int, default=12, help="requests of the served stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box of this build has a single MI355X (asked for {args.gpus})", file=sys.stderr)
@@ -276,6 +309,9 @@ def main() -> int:
         "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "gbs": round(v[2] / max(v[1], 1e-9) / 1e6, 1)}
                     for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][1])},
     }
+    if args.slots > 0:
+        backend.release_idle_buffers()
+        line["served"] = served(pair, plan, args.slots, args.requests)
     if not args.no_cpu:
         log("CPU-engine baseline (bounded sample)")
         line["cpu_baseline"] = cpu_baseline(pair, last.result)

@@ -239,3 +239,10 @@ class Result(C.Structure):
         ("linear_evaluations", (C.c_int64 * 8) * 2),
         ("syn_inversion_steps", C.c_int64 * 8),
     ]
+
+
+class ServedTiming(C.Structure):
+    """fw_served_timing: where a served request's time went."""
+
+    _fields_ = [("queued_s", C.c_double), ("run_s", C.c_double), ("total_s", C.c_double),
+                ("footprint_bytes", C.c_uint64), ("slot", C.c_int32), ("pad", C.c_int32)]

@@ -14,8 +14,8 @@ from pathlib import Path
 
 import numpy as np
 
-from ._abi import (Affine, Capacity, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, Plan, Result, aos, dptr,
-                   u32ptr)
+from ._abi import (Affine, Capacity, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, Plan, Result, ServedTiming,
+                   aos, dptr, u32ptr)
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfastwarp_hip.so"
 
@@ -102,6 +102,18 @@ _SIGNATURES = [
     ),
     ("fw_register_resident", C.c_int, [_P, C.POINTER(Plan), C.POINTER(_P), C.POINTER(_P), _PP, _PP, C.POINTER(Result),
                                        C.POINTER(C.c_int32)]),
+    ("fw_server_create", C.c_int, [C.c_int32, C.c_uint32, C.c_uint64, _PP]),
+    ("fw_server_destroy", None, [_P]),
+    ("fw_server_last_error", C.c_char_p, []),
+    (
+        "fw_server_submit",
+        C.c_int,
+        [_P, C.POINTER(Plan), C.POINTER(Grid), C.POINTER(C.POINTER(C.c_float)), C.POINTER(Grid),
+         C.POINTER(C.POINTER(C.c_float)), _D, _D, C.POINTER(C.c_uint64)],
+    ),
+    ("fw_server_wait", C.c_int, [_P, C.c_uint64, C.POINTER(Result), C.POINTER(ServedTiming)]),
+    ("fw_server_cancel", C.c_int, [_P, C.c_uint64]),
+    ("fw_server_counts", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
 ]
 
 

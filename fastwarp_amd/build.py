@@ -21,6 +21,7 @@ SOURCES = [
     CSRC / "hip" / "backend.hip",
     CSRC / "hip" / "mi_backend.hip",
     CSRC / "capi.cpp",
+    CSRC / "serve.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -334,9 +334,16 @@ HImage HipBackend::upload_image(const Image& v) {
 
 HImage HipBackend::upload_image_f32(const Grid& g, const float* data) {
   const size_t n = g.num_voxels();
+  constexpr size_t kGrain = size_t(1) << 16;
+  std::vector<float> bound((n + kGrain - 1) / kGrain, 0.0f);
+  parallel_chunks(n, kGrain, [&](size_t c, size_t b, size_t e) {
+    float m = 0.0f;
+    for (size_t i = b; i < e; ++i)
+      if (std::isfinite(data[i])) m = std::max(m, std::fabs(data[i]));
+    bound[c] = m;
+  });
   float m = 0.0f;
-  for (size_t i = 0; i < n; ++i)
-    if (std::isfinite(data[i])) m = std::max(m, std::fabs(data[i]));
+  for (float x : bound) m = std::max(m, x);
   HImage im{g, DevBuf(ctx_.get(), n), m};
   check(hipMemcpyAsync(im.buf.get(), data, n * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()), "upload image");
   ctx_->sync();

@@ -304,6 +304,53 @@ int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, con
 int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
                          fw_field** forward, fw_field** inverse, fw_result* result, const volatile int32_t* cancel);
 
+/* ---- Serving: continuous batching of whole registrations on one device ------------------------
+ * The reference runs several registrations on one GPU as independent device instances, as many
+ * as its capacity model fits at once (crates/fastwarp-cli/src/engine.rs:1015-1068
+ * `registrations_at_once`; python/fastwarp/_api.py:1104-1130 runs up to four at once on worker
+ * threads, each on a backend from the idle pool, engine.rs:143-149). A server keeps
+ * `slots` such instances (one fw_ctx each: its own HIP stream and buffer pool, on its own host
+ * thread) and admits requests to them as slots and device memory free up: a request's footprint
+ * is fw_plan_capacity's peak_bytes plus 256 MiB per instance (engine.rs:1000), admitted in
+ * arrival order while it fits the budget beside the running ones. Requests overlap on the device,
+ * and a request's result is the same bits alone and in any mix (no state is shared between slots
+ * and every reduction has a fixed order).
+ *   Images are f32 host arrays (x-fastest, one per channel) read by the request's slot when it
+ * starts; they, and forward / inverse (3n interleaved f64 on the fixed grid, may be NULL), must
+ * stay valid until fw_server_wait returns for the ticket. Every call is thread-safe except
+ * fw_server_destroy, which cancels what has not ended, stops the slots and drops the tickets no
+ * one waited for; no other call on the server may be in progress. Errors: the int status, and
+ * fw_server_last_error() for the calling thread's last failure (a request's own message after
+ * fw_server_wait). No counterpart in the reference's FFI: the entry points serve callers that
+ * stream registrations through one GPU (a request loop), in place of one device per call. */
+typedef struct fw_server fw_server;
+typedef struct {
+  double queued_s;          /* submission to start on a slot */
+  double run_s;             /* start to end: uploads, registration, field downloads */
+  double total_s;           /* submission to end (the request's latency) */
+  uint64_t footprint_bytes; /* device bytes it was admitted with */
+  int32_t slot;             /* the slot that ran it (-1: it never started) */
+  int32_t pad;
+} fw_served_timing;
+/* `slots` 0 takes 4 (the reference's four at once, _api.py:1104; also the HIP hardware queues a
+ * process gets by default), at most 16; a budget of
+ * 0 takes fw_memory_budget's rule (free device memory less 1 GiB, or FASTWARP_GPU_MEMORY_BUDGET_MB). */
+int fw_server_create(int32_t device, uint32_t slots, uint64_t budget_bytes, fw_server** out);
+void fw_server_destroy(fw_server* server);
+const char* fw_server_last_error(void);
+/* Queues one registration (fw_register's arguments, f32 images); FW_ERR_INVALID for a bad plan,
+ * grid or null image, or a footprint beyond the whole budget. */
+int fw_server_submit(fw_server* server, const fw_plan* plan, const fw_grid* fixed_grid, const float* const* fixed,
+                     const fw_grid* moving_grid, const float* const* moving, double* forward, double* inverse,
+                     uint64_t* ticket);
+/* Blocks until the request has ended; returns its status (FW_ERR_CANCELLED when cancelled) and
+ * fills result and timing (either may be NULL). A ticket is waited for once. */
+int fw_server_wait(fw_server* server, uint64_t ticket, fw_result* result, fw_served_timing* timing);
+/* A waiting request ends at once; a running one at its next level or iteration. */
+int fw_server_cancel(fw_server* server, uint64_t ticket);
+/* Requests waiting and running, and the device bytes admitted to the running ones. */
+int fw_server_counts(fw_server* server, uint32_t* waiting, uint32_t* running, uint64_t* bytes_in_use);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,9 +3,11 @@
 // passing check and exits non-zero on the first failure.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 #include <thread>
 
+#include "../../fastwarp_amd/csrc/host/admission.hpp"
 #include "../../fastwarp_amd/csrc/host/gauss_taps.hpp"
 #include "../../fastwarp_amd/csrc/host/linear.hpp"
 #include "../../fastwarp_amd/csrc/host/mi_host.hpp"
@@ -344,6 +346,110 @@ static void preprocess() {
   ok("preprocess");
 }
 
+// The product's Mattes finish (which skips the Kahan steps that cannot change the sum) equals the
+// oracle's restatement of mi.rs:702-769 bit for bit: value and derivative weights, on sparse and
+// dense joint histograms of fixed-point counts with wide dynamic ranges.
+static void mi_finish() {
+  std::mt19937_64 rng(77);
+  int cases = 0;
+  for (size_t bins : {5u, 12u, 32u, 64u})
+    for (double density : {0.02, 0.1, 0.3, 1.0})
+      for (int rep = 0; rep < 25; ++rep) {
+        std::vector<double> joint(bins * bins, 0.0), marg(bins, 0.0);
+        uint64_t valid = 0;
+        for (size_t f = 0; f < bins; ++f)
+          for (size_t m = 0; m < bins; ++m) {
+            if (double(rng() % 1000000) / 1e6 >= density) continue;
+            // fixed-point counts (2^-32 units) from a few units up to 2^31 samples' worth: totals
+            // beyond 2^53 units make the Kahan compensation non-zero
+            const int shift = int(rng() % 62);
+            const uint64_t c = (rng() >> 1) >> shift;
+            joint[f * bins + m] = double(c) / 4294967296.0;
+            const uint64_t n = 1 + c / 4294967296u;
+            marg[f] += double(n);
+            valid += n;
+          }
+        for (bool ratio : {false, true}) {
+          bool fw_ok = true, or_ok = true;
+          HistogramFinish a;
+          oracle::HistogramResult b;
+          try {
+            a = finish_histogram(bins, 0.031, valid, joint, marg, ratio);
+          } catch (const Error&) {
+            fw_ok = false;
+          }
+          try {
+            b = oracle::finish_histogram(bins, 0.031, valid, joint, marg, ratio);
+          } catch (const Error&) {
+            or_ok = false;
+          }
+          CHECK(fw_ok == or_ok, "mattes finish: the same inputs are rejected");
+          if (!fw_ok) continue;
+          CHECK(std::memcmp(&a.value, &b.value, sizeof(double)) == 0, "mattes finish: value bit for bit");
+          CHECK(a.ratio.size() == b.ratio.size() &&
+                    (a.ratio.empty() || std::memcmp(a.ratio.data(), b.ratio.data(), a.ratio.size() * sizeof(double)) == 0),
+                "mattes finish: derivative weights bit for bit");
+          ++cases;
+        }
+      }
+  CHECK(cases > 600, "mattes finish: enough non-degenerate cases");
+  ok("mi_finish");
+}
+
+// Admission by footprint: arrival order, the budget never exceeded, a head that does not fit
+// holds the queue back until memory frees, cancelled waiters leave without a trace, and a
+// simulated stream of random requests drains with every request run exactly once.
+static void admission() {
+  AdmissionQueue q(100);
+  CHECK(q.fits_alone(100) && !q.fits_alone(101), "admission: a request larger than the budget never fits");
+  CHECK(!q.can_admit(), "admission: nothing to admit in an empty queue");
+  q.push(1, 40);
+  q.push(2, 50);
+  q.push(3, 30);
+  q.push(4, 5);
+  CHECK(q.can_admit() && q.admit() == 1, "admission: the head starts first");
+  CHECK(q.can_admit() && q.admit() == 2, "admission: the next fits beside it");
+  CHECK(q.in_use() == 90 && !q.can_admit(), "admission: a head that does not fit waits");
+  CHECK(q.waiting() == 2 && q.running() == 2, "admission: counts");
+  q.finish(1);
+  CHECK(q.in_use() == 50 && q.can_admit() && q.admit() == 3, "admission: a finish frees its footprint");
+  CHECK(q.remove(4) && !q.remove(4) && q.waiting() == 0, "admission: a waiting request can be removed once");
+  q.finish(2);
+  q.finish(3);
+  q.finish(3);  // a second finish is ignored
+  CHECK(q.in_use() == 0 && q.running() == 0, "admission: all memory back");
+  // random stream over a few slots: arrivals, completions in random order
+  std::mt19937 rng(11);
+  AdmissionQueue r(1000);
+  std::vector<uint64_t> running, order;
+  uint64_t next = 1, started = 0;
+  const size_t slots = 3;
+  std::vector<uint64_t> size_of(401, 0);
+  while (started < 400) {
+    if (next <= 400 && rng() % 3 != 0) {
+      size_of[next] = 50 + rng() % 600;
+      r.push(next, size_of[next]);
+      ++next;
+    }
+    while (running.size() < slots && r.can_admit()) {
+      const uint64_t id = r.admit();
+      order.push_back(id);
+      running.push_back(id);
+      ++started;
+    }
+    uint64_t sum = 0;
+    for (uint64_t id : running) sum += size_of[id];
+    CHECK(sum == r.in_use() && sum <= 1000, "admission: the budget holds at every step");
+    if (!running.empty() && (rng() % 2 == 0 || next > 400)) {
+      const size_t k = rng() % running.size();
+      r.finish(running[k]);
+      running.erase(running.begin() + std::ptrdiff_t(k));
+    }
+  }
+  for (size_t i = 0; i < order.size(); ++i) CHECK(order[i] == i + 1, "admission: requests start in arrival order");
+  ok("admission");
+}
+
 // The host loops: every element of every loop is visited exactly once when loops run from several
 // threads at once and from inside other loops' bodies, and a body's exception reaches the caller.
 static void host_loops() {
@@ -402,6 +508,8 @@ int main() {
   thresholds();
   taps();
   preprocess();
+  mi_finish();
+  admission();
   std::printf("all host checks passed\n");
   return 0;
 }

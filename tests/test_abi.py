@@ -47,6 +47,17 @@ def test_a_context_on_a_machine_without_a_gpu_fails_cleanly():
         backend.HipBackend(0)
 
 
+def test_a_server_on_a_machine_without_a_gpu_fails_cleanly():
+    import os
+
+    from fastwarp_amd.serve import Server
+
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    with pytest.raises(backend.RegError):
+        Server(0, slots=2)
+
+
 def test_oracle_library_loads():
     import oracle
 
