@@ -534,7 +534,8 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
   a.partial_cc = static_cast<float*>(ctx_->scratch(0, blocks * sizeof(float)));
   a.partial_count = static_cast<int*>(ctx_->scratch(1, blocks * sizeof(int)));
   {
-    Timed t(*ctx_, "k_cc2", (valid ? 60.0 : 56.0) * double(n));
+    // F and M read, both derivative fields written (the mask too when there is one)
+    Timed t(*ctx_, "k_cc2", (valid ? 36.0 : 32.0) * double(n));
     launch_cc_pair(a, ctx_->stream());
   }
   const size_t real = blocks;
@@ -622,7 +623,7 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
     host = ring + slot * bytes;
     a.partial_cc = reinterpret_cast<float*>(static_cast<char*>(ring_dev) + slot * bytes);
     a.partial_count = reinterpret_cast<int*>(static_cast<char*>(ring_dev) + slot * bytes + real * sizeof(float));
-    Timed t(*ctx_, "k_cc2", 56.0 * double(n));
+    Timed t(*ctx_, "k_cc2", 32.0 * double(n));
     launch_cc_pair(a, ctx_->stream());
   } else {
     // the partials live in a buffer of this call's own, copied to staging memory
@@ -630,7 +631,7 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
     a.partial_cc = partials.get();
     a.partial_count = reinterpret_cast<int*>(partials.get() + blocks);
     {
-      Timed t(*ctx_, "k_cc2", 56.0 * double(n));
+      Timed t(*ctx_, "k_cc2", 32.0 * double(n));
       launch_cc_pair(a, ctx_->stream());
     }
     host = static_cast<char*>(ctx_->stage(real * (sizeof(float) + sizeof(int))));
@@ -717,7 +718,8 @@ HImage HipBackend::smooth_image_mm(const HImage& src, const Vec3& sigma_mm) {
       c.bn[t] = float(k.bn[t]);
       c.bm[t] = float(k.bm[t]);
     }
-    Timed t(*ctx_, "k_deriche_axis", 20.0 * double(n));
+    // one read and one write per voxel (the anti-causal pass re-reads its line from cache)
+    Timed t(*ctx_, "k_deriche_axis", 8.0 * double(n));
     launch_deriche_axis(in, bufs[q], d, axes[q], c, s);
     in = bufs[q];
   }

@@ -302,7 +302,8 @@ MetricPair<HField> HipBackend::mi_pair(const HImage& fixed, const HImage& moving
   a.d0 = out.fields[0].buf.get();
   a.d1 = out.fields[1].buf.get();
   {
-    Timed tm(*ctx_, "k_mi_dense_deriv", 56.0 * double(n));
+    // F and M (and the masks) read, both derivative fields written
+    Timed tm(*ctx_, "k_mi_dense_deriv", (32.0 + (a.fmask ? 4.0 : 0.0) + (a.mmask ? 4.0 : 0.0)) * double(n));
     launch_mi_dense_deriv(a, s);
   }
   if (want_norms) {
