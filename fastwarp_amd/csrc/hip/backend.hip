@@ -1075,6 +1075,8 @@ std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField
   if (initial.grid != field.grid) invalid("invert", "initial estimate grid differs");
   const Grid& g = field.grid;
   const Dims d = dims_of(g);
+  // the kernels split voxel indices with 32-bit divisions
+  if (d.n() >= (1ll << 31)) invalid("invert", "grids of 2^31 voxels or more are not supported");
   float l[9];
   put(l, g.index_from_offset);
   const float3 isp = inv_spacing(g);

@@ -13,6 +13,7 @@
 #include <climits>
 
 #include "device_common.hpp"
+#include "fastdiv.hpp"
 
 namespace fw::hip {
 
@@ -1136,6 +1137,13 @@ constexpr int kInvThreads = 256;
 constexpr int kInvVoxels = 4;
 constexpr int kInvMaxBlocks = 2048;
 
+// The row and plane divisors of a grid's linear index (grids below 2^31 voxels, which the
+// inversion's caller checks).
+struct InvIndex {
+  FastDiv x, y;
+};
+static InvIndex inv_index(const Dims& d) { return InvIndex{make_fastdiv((uint32_t)d.nx), make_fastdiv((uint32_t)d.ny)}; }
+
 __device__ __forceinline__ void inv_residual_at(const float* __restrict__ field, const Dims& d, const float* l, int i,
                                                 int j, int k, float wx, float wy, float wz, float& ex, float& ey,
                                                 float& ez) {
@@ -1190,15 +1198,18 @@ __device__ __forceinline__ InvRange inv_range(long long n) {
   return r;
 }
 
-__device__ __forceinline__ void inv_coords(const Dims& d, long long g, int& i, int& j, int& k) {
-  i = (int)(g % d.nx);
-  const long long r = g / d.nx;
-  j = (int)(r % d.ny);
-  k = (int)(r / d.ny);
+// (i, j, k) of voxel g < 2^31 by two multiply-high divisions (nx, ny in `ix`).
+__device__ __forceinline__ void inv_coords(const InvIndex& ix, long long g, int& i, int& j, int& k) {
+  const uint32_t u = (uint32_t)g;
+  const uint32_t r = fast_div(u, ix.x);
+  i = (int)(u - r * ix.x.d);
+  const uint32_t q = fast_div(r, ix.y);
+  j = (int)(r - q * ix.y.d);
+  k = (int)q;
 }
 
 __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restrict__ field, const float* __restrict__ psi,
-                                                           float* __restrict__ res, Dims d, L9 l, float3 inv_sp,
+                                                           float* __restrict__ res, Dims d, InvIndex ix, L9 l, float3 inv_sp,
                                                            InvertState* state, InvertReport* report, float* partials,
                                                            int max_iterations) {
   const long long n = d.n();
@@ -1210,7 +1221,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
       const long long g = base + v * kInvThreads;
       if (g < rg.hi) {
         int i, j, k;
-        inv_coords(d, g, i, j, k);
+        inv_coords(ix, g, i, j, k);
         float ex, ey, ez;
         inv_residual_at(field, d, l.v, i, j, k, psi[g], psi[g + n], psi[g + 2 * n], ex, ey, ez);
         res[g] = ex;
@@ -1236,7 +1247,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
 }
 
 __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi,
-                                                          float* __restrict__ res, Dims d, L9 l, float3 inv_sp,
+                                                          float* __restrict__ res, Dims d, InvIndex ix, L9 l, float3 inv_sp,
                                                           InvertState* state, InvertReport* report, float* partials, int step,
                                                           int max_iterations, float max_tol, float mean_tol, int enforce,
                                                           uint32_t* mailbox, uint32_t tag) {
@@ -1294,7 +1305,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
       const long long g = base + v * kInvThreads;
       if (g < rg.hi) {
         int i, j, k;
-        inv_coords(d, g, i, j, k);
+        inv_coords(ix, g, i, j, k);
         const float wx = psi[g], wy = psi[g + n], wz = psi[g + 2 * n];
         float ex = res[g], ey = res[g + n], ez = res[g + 2 * n];
         const float nrm = scaled_norm(inv_sp, ex, ey, ez);
@@ -1335,8 +1346,8 @@ void launch_inv_begin(const float* field, const float* psi, float* residual, Dim
                       hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  launch(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state, report,
-         partials, max_iterations);
+  launch(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, inv_index(d), lv, inv_spacing, state,
+         report, partials, max_iterations);
 }
 
 void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
@@ -1344,8 +1355,8 @@ void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, co
                      double max_tol, double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  launch(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, lv, inv_spacing, state, report,
-         partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
+  launch(k_inv_iter, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, inv_index(d), lv, inv_spacing, state,
+         report, partials, step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
 }
 
 

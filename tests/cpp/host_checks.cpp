@@ -7,6 +7,7 @@
 #include <random>
 #include <thread>
 
+#include "../../fastwarp_amd/csrc/hip/fastdiv.hpp"
 #include "../../fastwarp_amd/csrc/host/admission.hpp"
 #include "../../fastwarp_amd/csrc/host/gauss_taps.hpp"
 #include "../../fastwarp_amd/csrc/host/linear.hpp"
@@ -396,6 +397,32 @@ static void mi_finish() {
   ok("mi_finish");
 }
 
+// The inversion kernels' multiply-high division equals integer division for every divisor a grid
+// axis can have and every index below 2^31: around each multiple of the divisor, at the top of the
+// range, and at random.
+static void fastdiv() {
+  std::mt19937 rng(5);
+  const uint32_t top = (uint32_t(1) << 31) - 1;
+  for (uint32_t d = 1; d <= 4096; ++d) {
+    const FastDiv f = make_fastdiv(d);
+    auto check = [&](uint32_t n) { return fast_div(n, f) == n / d; };
+    bool good = check(0) && check(top) && check(top - 1) && check(top - d);
+    for (uint32_t k = 1; k < 64 && good; ++k) {
+      const uint64_t q = uint64_t(k) * d;
+      good = check(uint32_t(q - 1)) && check(uint32_t(q)) && check(uint32_t(q + 1));
+    }
+    for (uint64_t k = top / d; k > top / d - 4 && good; --k) good = check(uint32_t(k * d)) && check(uint32_t(k * d - 1));
+    for (int r = 0; r < 2000 && good; ++r) good = check(rng() & top);
+    CHECK(good, "fast division by a grid axis");
+  }
+  for (uint32_t d : {5000u, 65535u, 100003u, (uint32_t(1) << 20) + 7u, top})
+    for (int r = 0; r < 20000; ++r) {
+      const uint32_t n = rng() & top;
+      CHECK(fast_div(n, make_fastdiv(d)) == n / d, "fast division by a large divisor");
+    }
+  ok("fastdiv");
+}
+
 // Admission by footprint: arrival order, the budget never exceeded, a head that does not fit
 // holds the queue back until memory frees, cancelled waiters leave without a trace, and a
 // simulated stream of random requests drains with every request run exactly once.
@@ -510,6 +537,7 @@ int main() {
   preprocess();
   mi_finish();
   admission();
+  fastdiv();
   std::printf("all host checks passed\n");
   return 0;
 }
