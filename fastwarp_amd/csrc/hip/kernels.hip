@@ -885,6 +885,83 @@ __global__ __launch_bounds__(256) void k_gauss_xy(const float* __restrict__ in, 
   }
 }
 
+// The same pass for equal x and y radii R known at compile time (the usual case: isotropic
+// variance on an isotropic grid): the tile rows are padded to whole 16-byte vectors, every x run
+// reads its 4 + 2R inputs as float4, the loops have no run-time bounds, and the tap chains are
+// those of k_gauss_xy, so the results are the same bits.
+template <int R>
+__global__ __launch_bounds__(256) void k_gauss_xy_r(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                    GaussTaps3 t, const float* __restrict__ in1, float* __restrict__ out1) {
+  constexpr int LX = kG3TX + 2 * R, LY = kG3TY + 2 * R;
+  constexpr int NV = (4 + 2 * R + 3) / 4;  // float4 reads per x run
+  constexpr int LXP0 = (LX + 3) / 4 * 4;
+  constexpr int LXP = LXP0 > kG3TX - 4 + 4 * NV ? LXP0 : kG3TX - 4 + 4 * NV;
+  __shared__ __align__(16) float tile[LY][LXP];
+  __shared__ __align__(16) float xs[LY][kG3TX];
+  const uint3 bl = xcd_block();
+  const int cz = bl.z / d.nz, z = bl.z - cz * d.nz;
+  const int comp = cz % 3;
+  const int x0 = bl.x * kG3TX, y0 = bl.y * kG3TY;
+  const long long n = d.n();
+  const float* src = (cz < 3 ? in : in1) + comp * n + (long long)d.nx * d.ny * z;
+  float* dst = (cz < 3 ? out : out1) + comp * n + (long long)d.nx * d.ny * z;
+  const int tid = threadIdx.x;
+  for (int ly = tid >> 6; ly < LY; ly += 4) {
+    int gy = y0 - R + ly;
+    gy = gy < 0 ? 0 : (gy > d.ny - 1 ? d.ny - 1 : gy);
+    const float* row = src + (long long)d.nx * gy;
+    for (int lx = tid & 63; lx < LX; lx += 64) {
+      int gx = x0 - R + lx;
+      gx = gx < 0 ? 0 : (gx > d.nx - 1 ? d.nx - 1 : gx);
+      tile[ly][lx] = row[gx];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < LY * (kG3TX / 4); e += 256) {
+    const int ly = e >> 4, lx = (e & 15) * 4;
+    float v[4 * NV];
+    const float4* p = reinterpret_cast<const float4*>(&tile[ly][lx]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const float4 f = p[q];
+      v[4 * q] = f.x;
+      v[4 * q + 1] = f.y;
+      v[4 * q + 2] = f.z;
+      v[4 * q + 3] = f.w;
+    }
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll
+    for (int k = 0; k <= 2 * R; ++k) {
+      const float w = t.x[k];
+      a0 = fmaf(w, v[k], a0);
+      a1 = fmaf(w, v[k + 1], a1);
+      a2 = fmaf(w, v[k + 2], a2);
+      a3 = fmaf(w, v[k + 3], a3);
+    }
+    *reinterpret_cast<float4*>(&xs[ly][lx]) = make_float4(a0, a1, a2, a3);
+  }
+  __syncthreads();
+  const int tx = tid & 63, ty = (tid >> 6) * 4;
+  float v[4 + 2 * R];
+#pragma unroll
+  for (int q = 0; q < 4 + 2 * R; ++q) v[q] = xs[ty + q][tx];
+  float b[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k <= 2 * R; ++k) {
+    const float w = t.y[k];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) b[r] = fmaf(w, v[k + r], b[r]);
+  }
+  const int x = x0 + tx;
+  if (x < d.nx) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int y = y0 + ty + r;
+      if (y < d.ny) dst[x + d.nx * y] = b[r];
+    }
+  }
+}
+
 template <int RZ>
 // Grid z runs over z chunk x component; with `fields` 2, components 3..5 read in1 / orig1 and
 // write out1.
@@ -954,6 +1031,25 @@ void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, h
                      float* tmp1) {
   const int fields = in1 ? 2 : 1;
   const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * fields * d.nz);
+  if (t.rx == t.ry) {
+    switch (t.rx) {
+#define FW_GXY_CASE(R)                                                        \
+  case R:                                                                     \
+    launch(k_gauss_xy_r<R>, gxy, dim3(256), 0, s, in, tmp, d, t, in1, tmp1); \
+    return;
+      FW_GXY_CASE(1)
+      FW_GXY_CASE(2)
+      FW_GXY_CASE(3)
+      FW_GXY_CASE(4)
+      FW_GXY_CASE(5)
+      FW_GXY_CASE(6)
+      FW_GXY_CASE(7)
+      FW_GXY_CASE(8)
+#undef FW_GXY_CASE
+      default:
+        break;
+    }
+  }
   launch(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t, in1, tmp1);
 }
 
