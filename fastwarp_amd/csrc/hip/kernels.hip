@@ -962,6 +962,147 @@ __global__ __launch_bounds__(256) void k_gauss_xy_r(const float* __restrict__ in
   }
 }
 
+// All three axes in one pass when the three radii are equal (R): a block owns a 64 x 16 (x, y) tile
+// of one component over a run of z planes. Each input plane (edge-clamped, with its x/y halo) is
+// smoothed along x and y exactly as in k_gauss_xy_r; a thread keeps the x/y results of its four
+// outputs for the last 2R + 1 planes in registers and finishes the z pass, the blend and the zeroed
+// faces as k_gauss_z does, so the results are the bits of the two-pass form. The next plane's tile
+// is fetched into registers while the current one is smoothed. Grid z runs over z chunk x
+// component; with `fields` 2, components 3..5 read in1 and write out1.
+template <int R>
+__global__ __launch_bounds__(256) void k_gauss3(const float* __restrict__ in, float* __restrict__ out, Dims d, GaussTaps3 t,
+                                                int zrun, float w1, float w2, int fields, const float* __restrict__ in1,
+                                                float* __restrict__ out1) {
+  constexpr int W = 2 * R + 1;
+  constexpr int LX = kG3TX + 2 * R, LY = kG3TY + 2 * R;
+  constexpr int NV = (4 + 2 * R + 3) / 4;
+  constexpr int LXP0 = (LX + 3) / 4 * 4;
+  constexpr int LXP = LXP0 > kG3TX - 4 + 4 * NV ? LXP0 : kG3TX - 4 + 4 * NV;
+  constexpr int PER = (LX * LY + 255) / 256;  // tile elements per thread
+  __shared__ __align__(16) float tile[LY][LXP];
+  __shared__ __align__(16) float xs[LY][kG3TX];
+  const uint3 bl = xcd_block();
+  const int cz = bl.z % (3 * fields), chunk = bl.z / (3 * fields);
+  const int comp = cz % 3;
+  const long long n = d.n(), plane = (long long)d.nx * d.ny;
+  const float* src = (cz < 3 ? in : in1) + comp * n;
+  float* dst = (cz < 3 ? out : out1) + comp * n;
+  const int x0 = bl.x * kG3TX, y0 = bl.y * kG3TY;
+  const int z0 = chunk * zrun, z1 = min(z0 + zrun, d.nz);
+  const int tid = threadIdx.x;
+  // this thread's tile elements: clamped offset within a plane, -1 past the tile
+  int toff[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + 256 * q;
+    toff[q] = -1;
+    if (e < LX * LY) {
+      const int ly = e / LX, lx = e - ly * LX;
+      int gx = x0 - R + lx, gy = y0 - R + ly;
+      gx = gx < 0 ? 0 : (gx > d.nx - 1 ? d.nx - 1 : gx);
+      gy = gy < 0 ? 0 : (gy > d.ny - 1 ? d.ny - 1 : gy);
+      toff[q] = gx + d.nx * gy;
+    }
+  }
+  float pre[PER];
+  auto fetch = [&](int zin) {
+    const float* p = src + plane * (zin < 0 ? 0 : (zin > d.nz - 1 ? d.nz - 1 : zin));
+#pragma unroll
+    for (int q = 0; q < PER; ++q) pre[q] = toff[q] >= 0 ? p[toff[q]] : 0.0f;
+  };
+  const int tx = tid & 63, ty = (tid >> 6) * 4;
+  const int x = x0 + tx;
+  float ring[W][4];
+#pragma unroll
+  for (int q = 0; q < W; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ring[q][r] = 0.0f;
+  fetch(z0 - R);
+  for (int zin = z0 - R; zin < z1 + R; ++zin) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+      if (toff[q] >= 0) {
+        const int e = tid + 256 * q;
+        const int ly = e / LX, lx = e - ly * LX;
+        tile[ly][lx] = pre[q];
+      }
+    __syncthreads();
+    if (zin + 1 < z1 + R) fetch(zin + 1);
+    for (int e = tid; e < LY * (kG3TX / 4); e += 256) {
+      const int ly = e >> 4, lx = (e & 15) * 4;
+      float v[4 * NV];
+      const float4* p = reinterpret_cast<const float4*>(&tile[ly][lx]);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const float4 f = p[q];
+        v[4 * q] = f.x;
+        v[4 * q + 1] = f.y;
+        v[4 * q + 2] = f.z;
+        v[4 * q + 3] = f.w;
+      }
+      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll
+      for (int k = 0; k <= 2 * R; ++k) {
+        const float w = t.x[k];
+        a0 = fmaf(w, v[k], a0);
+        a1 = fmaf(w, v[k + 1], a1);
+        a2 = fmaf(w, v[k + 2], a2);
+        a3 = fmaf(w, v[k + 3], a3);
+      }
+      *reinterpret_cast<float4*>(&xs[ly][lx]) = make_float4(a0, a1, a2, a3);
+    }
+    __syncthreads();
+    float v[4 + 2 * R];
+#pragma unroll
+    for (int q = 0; q < 4 + 2 * R; ++q) v[q] = xs[ty + q][tx];
+    float b[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k <= 2 * R; ++k) {
+      const float w = t.y[k];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) b[r] = fmaf(w, v[k + r], b[r]);
+    }
+#pragma unroll
+    for (int q = 0; q < W - 1; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ring[q][r] = ring[q + 1][r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ring[W - 1][r] = b[r];
+    const int k = zin - R;
+    if (k >= z0 && x < d.nx) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int y = y0 + ty + r;
+        if (y < d.ny) {
+          const long long g = plane * k + x + (long long)d.nx * y;
+          float o = 0.0f;
+          if (!(x == 0 || y == 0 || x == d.nx - 1 || y == d.ny - 1 || k == 0 || k == d.nz - 1)) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int q = 0; q < W; ++q) acc = fmaf(t.z[q], ring[q][r], acc);
+            o = w2 != 0.0f ? acc * w1 + src[g] * w2 : acc * w1;
+          }
+          dst[g] = o;
+        }
+      }
+    }
+  }
+}
+
+// z runs of the one-pass form: about kG3Blocks blocks, each run at least 4R planes long (a run
+// smooths 2R planes beyond its own along x and y).
+#ifndef FW_G3_BLOCKS
+#define FW_G3_BLOCKS 2048
+#endif
+constexpr int kG3Blocks = FW_G3_BLOCKS;
+static int g3_zrun(const Dims& d, int r, int fields) {
+  const long long cols = (long long)((d.nx + kG3TX - 1) / kG3TX) * ((d.ny + kG3TY - 1) / kG3TY) * 3 * fields;
+  const long long chunks = (kG3Blocks + cols - 1) / cols;
+  int run = (int)((d.nz + chunks - 1) / chunks);
+  const int least = 4 * r;
+  return run < least ? least : run;
+}
+
 template <int RZ>
 // Grid z runs over z chunk x component; with `fields` 2, components 3..5 read in1 / orig1 and
 // write out1.
@@ -1051,6 +1192,31 @@ void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, h
     }
   }
   launch(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t, in1, tmp1);
+}
+
+bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s,
+                   const float* in1, float* out1) {
+  if (t.rx != t.ry || t.ry != t.rz) return false;
+  const int fields = in1 ? 2 : 1;
+  const int run = g3_zrun(d, t.rz, fields);
+  const dim3 g((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * fields * ((d.nz + run - 1) / run));
+  switch (t.rz) {
+#define FW_G3_CASE(R)                                                                      \
+  case R:                                                                                  \
+    launch(k_gauss3<R>, g, dim3(256), 0, s, in, out, d, t, run, w1, w2, fields, in1, out1); \
+    return true;
+    FW_G3_CASE(1)
+    FW_G3_CASE(2)
+    FW_G3_CASE(3)
+    FW_G3_CASE(4)
+    FW_G3_CASE(5)
+    FW_G3_CASE(6)
+    FW_G3_CASE(7)
+    FW_G3_CASE(8)
+#undef FW_G3_CASE
+    default:
+      return false;
+  }
 }
 
 void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,

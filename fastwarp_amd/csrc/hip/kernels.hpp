@@ -157,6 +157,11 @@ void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, h
                      float* tmp1 = nullptr);
 void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
                     hipStream_t s, const float* tmp1 = nullptr, const float* in1 = nullptr, float* out1 = nullptr);
+// All three axes in one launch when the three radii are equal (false, and nothing launched,
+// otherwise); the same results as launch_gauss_xy then launch_gauss_z. With in1 / out1 a second
+// field of the same grid is smoothed in the same launch.
+bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s,
+                   const float* in1 = nullptr, float* out1 = nullptr);
 
 // --- inversion (ITK InvertDisplacementFieldImageFilter) ---
 struct InvertState {
