@@ -913,7 +913,7 @@ std::pair<HField, HField> HipBackend::smooth_field_pair(HField a, HField b, doub
   }
   if (!gauss3_fits(d, t)) return {smooth_field(std::move(a), variance), smooth_field(std::move(b), variance)};
   HField out0{g, DevBuf(ctx_.get(), 3 * n)}, out1{g, DevBuf(ctx_.get(), 3 * n)};
-  if (t.rx == t.ry && t.ry == t.rz) {
+  if (t.rx == t.ry && t.ry == t.rz && gauss3_pays(d)) {
     // both fields read once (twice with a blend), both results written
     Timed tm(*ctx_, "k_gauss3", 48.0 * double(n) + (w2 != 0.0 ? 24.0 * double(n) : 0.0));
     if (!launch_gauss3(a.buf.get(), out0.buf.get(), d, t, float(w1), float(w2), ctx_->stream(), b.buf.get(), out1.buf.get()))
@@ -956,7 +956,7 @@ HField HipBackend::smooth_field(HField field, double variance) {
     }
     if (fits && gauss3_fits(d, t)) {
       HField out{g, DevBuf(ctx_.get(), 3 * n)};
-      if (t.rx == t.ry && t.ry == t.rz) {
+      if (t.rx == t.ry && t.ry == t.rz && gauss3_pays(d)) {
         // u read once (twice with a blend), the result written
         Timed tm(*ctx_, "k_gauss3", 24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0));
         if (!launch_gauss3(field.buf.get(), out.buf.get(), d, t, float(w1), float(w2), ctx_->stream()))

@@ -1194,6 +1194,8 @@ void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, h
   launch(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t, in1, tmp1);
 }
 
+bool gauss3_pays(const Dims& d) { return d.n() >= 4000000; }
+
 bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s,
                    const float* in1, float* out1) {
   if (t.rx != t.ry || t.ry != t.rz) return false;

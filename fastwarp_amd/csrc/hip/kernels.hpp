@@ -162,6 +162,10 @@ void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const
 // field of the same grid is smoothed in the same launch.
 bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s,
                    const float* in1 = nullptr, float* out1 = nullptr);
+// Whether the one-pass form is the faster on this grid: it walks each block's planes in sequence,
+// which pays on large grids only (below about four million voxels the two-pass grid, one block per
+// plane tile, finishes first).
+bool gauss3_pays(const Dims& d);
 
 // --- inversion (ITK InvertDisplacementFieldImageFilter) ---
 struct InvertState {

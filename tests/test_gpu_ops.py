@@ -151,6 +151,16 @@ def test_smooth_matches_oracle(backend, name, variance):
     assert cases.rel_err(got, want) <= 2e-6
 
 
+@pytest.mark.parametrize("variance", [3.0, 0.25])
+def test_smooth_matches_oracle_on_a_large_grid(backend, variance):
+    """Grids of four million voxels and more take the one-pass three-axis kernel."""
+    g = cases.grid((171, 160, 150), (1.0, 1.0, 1.0), (-85.0, -80.0, -75.0))
+    f = cases.field(g, 2.0, 17)
+    want = oracle.smooth(g, f, variance)
+    got = backend.download_field(backend.smooth_field(backend.upload_field(g, f), variance))
+    assert cases.rel_err(got, want) <= 2e-6
+
+
 @pytest.mark.parametrize("name", ["odd", "oblique", "medium"])
 def test_norms_scale_axpy(backend, name):
     g = GRIDS[name]
