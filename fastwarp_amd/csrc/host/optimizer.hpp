@@ -90,8 +90,9 @@ inline double estimate_step_scale(const std::array<Vec3, 8>& corners, const Line
   return max_voxel_shift(corners, t, s) / factor;
 }
 
+// The flag is set from another thread: read it with an atomic load.
 inline void check_cancel(const volatile int32_t* cancel) {
-  if (cancel && *cancel) throw Cancelled();
+  if (cancel && __atomic_load_n(cancel, __ATOMIC_RELAXED)) throw Cancelled();
 }
 
 // optimizer.rs:262-300 `LineSearch::golden` (RESPHI = 2 - 1.618034), as two pure steps over the

@@ -1,6 +1,9 @@
 // Checks of the product's host code (the C++ drivers' building blocks) against the reference's
 // published values and closed forms. Built and run by tests/test_host.py; prints "ok <name>" per
 // passing check and exits non-zero on the first failure.
+#include <atomic>
+#include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -13,6 +16,7 @@
 #include "../../fastwarp_amd/csrc/host/linear.hpp"
 #include "../../fastwarp_amd/csrc/host/mi_host.hpp"
 #include "../../fastwarp_amd/csrc/host/preprocess.hpp"
+#include "../../fastwarp_amd/csrc/host/request_loop.hpp"
 #include "../../oracle/oracle.hpp"
 
 using namespace fw;
@@ -397,6 +401,120 @@ static void mi_finish() {
   ok("mi_finish");
 }
 
+// The server's request loop with stand-in slots: results and errors reach the caller that waits,
+// no more requests run at once than slots and budget allow, they start in arrival order, waiting
+// and running requests cancel, and a loop with requests in flight shuts down promptly.
+static void request_loop() {
+  struct Slot {
+    uint32_t index;
+  };
+  struct Job {
+    int input = 0, output = 0, sleep_us = 0;
+    int fail = 0;  // 1: invalid input, 2: a device error
+  };
+  std::atomic<int> running{0}, peak{0}, order{0};
+  std::vector<int> starts(200, -1);
+  auto run = [&](Slot&, Job& j, const volatile int32_t* cancel) {
+    const int now = ++running;
+    int p = peak.load();
+    while (now > p && !peak.compare_exchange_weak(p, now)) {
+    }
+    starts[size_t(order++)] = j.input;
+    for (int t = 0; t < j.sleep_us; t += 200) {
+      if (__atomic_load_n(cancel, __ATOMIC_RELAXED)) {
+        --running;
+        throw Cancelled();
+      }
+      std::this_thread::sleep_for(std::chrono::microseconds(200));
+    }
+    --running;
+    if (j.fail == 1) throw Error("test", "bad input " + std::to_string(j.input));
+    if (j.fail == 2) throw Error("hip", "device lost");
+    j.output = 2 * j.input + 1;
+  };
+  auto make = [](uint32_t i) { return std::make_unique<Slot>(Slot{i}); };
+  {
+    RequestLoop<Slot, Job> loop(3, 1000, make, run);
+    std::vector<uint64_t> ids;
+    for (int i = 0; i < 40; ++i) ids.push_back(loop.submit(100, Job{i, 0, 1000, 0}));
+    for (int i = 0; i < 40; ++i) {
+      auto [o, j] = loop.wait(ids[size_t(i)]);
+      CHECK(o.status == Ended::Ok && j.output == 2 * i + 1 && o.slot >= 0 && o.slot < 3, "request loop: results");
+      CHECK(o.footprint == 100 && std::fabs(o.total_s - o.queued_s - o.run_s) < 1e-6, "request loop: timing");
+    }
+    CHECK(peak.load() == 3, "request loop: as many at once as slots");
+    for (int i = 0; i < 40; ++i) CHECK(starts[size_t(i)] == i, "request loop: arrival order");
+    bool threw = false;
+    try {
+      loop.wait(ids[0]);
+    } catch (const Error&) {
+      threw = true;
+    }
+    CHECK(threw, "request loop: a ticket is waited for once");
+  }
+  {
+    peak = 0;
+    RequestLoop<Slot, Job> loop(3, 250, make, run);
+    std::vector<uint64_t> ids;
+    for (int i = 0; i < 12; ++i) ids.push_back(loop.submit(100, Job{i, 0, 1000, 0}));
+    for (uint64_t id : ids) loop.wait(id);
+    CHECK(peak.load() == 2, "request loop: the budget bounds what runs at once");
+    bool threw = false;
+    try {
+      loop.submit(251, Job{});
+    } catch (const Error&) {
+      threw = true;
+    }
+    CHECK(threw, "request loop: a request beyond the budget is refused");
+  }
+  {
+    RequestLoop<Slot, Job> loop(1, 1000, make, run);
+    const uint64_t a = loop.submit(10, Job{1, 0, 2000000, 0});
+    const uint64_t b = loop.submit(10, Job{2, 0, 1000, 0});
+    const uint64_t c = loop.submit(10, Job{3, 0, 1000, 1});
+    const uint64_t d = loop.submit(10, Job{4, 0, 1000, 2});
+    loop.cancel(b);
+    auto [ob, jb] = loop.wait(b);
+    CHECK(ob.status == Ended::Cancelled && ob.slot == -1 && jb.output == 0, "request loop: a waiting request cancels at once");
+    while (running.load() == 0) std::this_thread::sleep_for(std::chrono::microseconds(100));
+    loop.cancel(a);
+    auto [oa, ja] = loop.wait(a);
+    CHECK(oa.status == Ended::Cancelled && oa.slot == 0 && oa.run_s < 1.0, "request loop: a running request cancels");
+    auto [oc, jc] = loop.wait(c);
+    CHECK(oc.status == Ended::Invalid && oc.error.find("bad input 3") != std::string::npos, "request loop: an error reaches its caller");
+    auto [od, jd] = loop.wait(d);
+    CHECK(od.status == Ended::Device, "request loop: a device error is one");
+    uint32_t w = 9, r = 9;
+    uint64_t bytes = 9;
+    loop.counts(&w, &r, &bytes);
+    CHECK(w == 0 && r == 0 && bytes == 0, "request loop: empty at the end");
+  }
+  {
+    const auto t0 = std::chrono::steady_clock::now();
+    {
+      RequestLoop<Slot, Job> loop(2, 1000, make, run);
+      for (int i = 0; i < 6; ++i) loop.submit(10, Job{i, 0, 2000000, 0});
+      std::this_thread::sleep_for(std::chrono::milliseconds(5));
+    }
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    CHECK(s < 1.0, "request loop: shutdown cancels what is in flight");
+  }
+  bool threw = false;
+  try {
+    RequestLoop<Slot, Job> loop(
+        3, 1000,
+        [](uint32_t i) -> std::unique_ptr<Slot> {
+          if (i == 1) throw Error("hip", "no device");
+          return std::make_unique<Slot>(Slot{i});
+        },
+        run);
+  } catch (const Error& e) {
+    threw = e.what_ == "hip";
+  }
+  CHECK(threw, "request loop: a slot that cannot start fails the loop");
+  ok("request_loop");
+}
+
 // The inversion kernels' multiply-high division equals integer division for every divisor a grid
 // axis can have and every index below 2^31: around each multiple of the divisor, at the top of the
 // range, and at random.
@@ -538,6 +656,7 @@ int main() {
   mi_finish();
   admission();
   fastdiv();
+  request_loop();
   std::printf("all host checks passed\n");
   return 0;
 }
