@@ -144,6 +144,8 @@ def kernels(repeats: int) -> int:
     f = b.upload_image(g, pair.fixed[0])
     m = b.upload_image(g, pair.moving[0])
     u = b.upload_field(g, pair.displacement)
+    # a second, different field, so that the composition reads two fields' worth of data
+    u2 = b.upload_field(g, 0.7 * pair.displacement[::-1].copy())
     n = pair.fixed[0].size
     # samples of the finest linear level: every 4th voxel, jittered by a third of a voxel
     rng = np.random.default_rng(0)
@@ -163,7 +165,7 @@ def kernels(repeats: int) -> int:
         "cc_pair r4": lambda: b.cc_value_derivative_pair(f, m, 4),
         "smooth_field var3": lambda: b.smooth_field(b.copy_field(u), 3.0),
         "smooth_image sigma 3 vox": lambda: b.smooth_image(f, (3.0, 3.0, 3.0)),
-        "compose": lambda: b.compose(u, u),
+        "compose": lambda: b.compose(u, u2),
         "warp": lambda: b.warp_through_field(m, u),
         "invert (20 iterations)": lambda: b.invert(u, b.zero_field(g), no_stop),
         "max_scaled_norm_pair": lambda: b.max_scaled_norm_pair(u, u),

@@ -240,18 +240,24 @@ __device__ __forceinline__ int window_count(int pos, int n, int r) {
   return hi - lo + 1;
 }
 
-// Floor division of a window sum by its voxel count: s = q * n + r with 0 <= r < n.
-__device__ __forceinline__ void floor_divmod(long long s, long long n, float inv_n, long long& q, long long& r) {
-  q = (long long)floorf((float)s * inv_n);
-  r = s - q * n;
-  while (r < 0) {
-    q -= 1;
-    r += n;
+// Floor division of a window sum by its voxel count: s = q * n + r with 0 <= r < n, for
+// |s| < 2^32 and 1 <= n < 2^12 (so |q| < 2^31). An f32 estimate of q is refined once from its
+// remainder (exact in 64 bits), which leaves it within a step of the floor.
+__device__ __forceinline__ void floor_divmod(long long s, int n, float inv_n, int& q, int& r) {
+  int qe = (int)floorf((float)s * inv_n);
+  long long rem = s - (long long)qe * n;
+  qe += (int)floorf((float)rem * inv_n);
+  rem = s - (long long)qe * n;
+  while (rem < 0) {
+    qe -= 1;
+    rem += n;
   }
-  while (r >= n) {
-    q += 1;
-    r -= n;
+  while (rem >= n) {
+    qe += 1;
+    rem -= n;
   }
+  q = qe;
+  r = (int)rem;
 }
 
 }  // namespace

@@ -182,16 +182,17 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
         float dfx = 0.0f, dfy = 0.0f, dfz = 0.0f, dmx = 0.0f, dmy = 0.0f, dmz = 0.0f;
         const bool centre_valid = !MASKED || a.valid[g] != 0.0f;
         if (centre_valid) {
-          const long long n = MASKED ? (long long)wv
-                                     : (long long)window_count(x, d.nx, R) * window_count(y, d.ny, R) *
-                                           window_count(z, d.nz, R);
+          // the window's voxel count (at most (2R + 1)^3)
+          const int n = MASKED ? wv : window_count(x, d.nx, R) * window_count(y, d.ny, R) * window_count(z, d.nz, R);
           const float inv_n = 1.0f / (float)n;
-          long long qa_, ra_, qb_, rb_;
+          // means q + r / n (|q| <= 2^22, 0 <= r < n); the centred sums exactly in 64 bits:
+          // sum (a - q)^2 - ... = waa - q (n q + 2 r), and the cross term alike
+          int qa_, ra_, qb_, rb_;
           floor_divmod(wa, n, inv_n, qa_, ra_);
           floor_divmod(wb, n, inv_n, qb_, rb_);
-          const long long eaa = waa - n * qa_ * qa_ - 2 * qa_ * ra_;
-          const long long ebb = wbb - n * qb_ * qb_ - 2 * qb_ * rb_;
-          const long long eab = wab - n * qa_ * qb_ - qb_ * ra_ - qa_ * rb_;
+          const long long eaa = waa - (long long)qa_ * ((long long)n * qa_ + 2 * ra_);
+          const long long ebb = wbb - (long long)qb_ * ((long long)n * qb_ + 2 * rb_);
+          const long long eab = wab - (long long)qa_ * ((long long)n * qb_ + rb_) - (long long)qb_ * ra_;
           const float s_ff = ((float)eaa - (float)(ra_ * ra_) * inv_n) * (a.inv_qf * a.inv_qf);
           const float s_mm = ((float)ebb - (float)(rb_ * rb_) * inv_n) * (a.inv_qm * a.inv_qm);
           const float s_fm = ((float)eab - (float)(ra_ * rb_) * inv_n) * (a.inv_qf * a.inv_qm);
