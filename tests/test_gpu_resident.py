@@ -78,7 +78,7 @@ def test_resident_registration_matches_the_host_path(backend):
     m_h, m_d = np.array(list(host.affine.matrix)), np.array(list(dev.affine.matrix))
     t_h, t_d = np.array(list(host.affine.translation)), np.array(list(dev.affine.translation))
     # the inputs differ by the window's last-ulp roundings and the f32 moment sums (about 1e-5 mm),
-    # which the optimiser may amplify through its comparisons: the north-star bounds apply
+    # which the optimiser may amplify through its comparisons: the registration accuracy bounds apply
     assert np.max(np.abs(m_h - m_d)) <= 1e-3, (m_h, m_d)
     assert np.max(np.abs(t_h - t_d)) <= 0.05, (t_h, t_d)
     spacing = 2.0

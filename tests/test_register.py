@@ -1,5 +1,5 @@
 """Whole registrations: the f64 CPU engine recovers a known transform (CPU), and the HIP engine
-matches it on the same phantom and plan within the north-star bounds (GPU): affine parameters
+matches it on the same phantom and plan within the accuracy bounds (GPU): affine parameters
 within 1e-3 (translations within 0.05 mm), displacement fields mean |d| <= 0.1 voxel and
 p99 <= 0.5 voxel, per-level final SyN energies within 1e-4 relative."""
 from __future__ import annotations
