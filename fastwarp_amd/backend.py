@@ -102,6 +102,7 @@ _SIGNATURES = [
     ),
     ("fw_register_resident", C.c_int, [_P, C.POINTER(Plan), C.POINTER(_P), C.POINTER(_P), _PP, _PP, C.POINTER(Result),
                                        C.POINTER(C.c_int32)]),
+    ("fw_resample_isotropic", C.c_int, [_P, _P, C.c_double, _PP]),
     ("fw_server_create", C.c_int, [C.c_int32, C.c_uint32, C.c_uint64, _PP]),
     ("fw_server_destroy", None, [_P]),
     ("fw_server_last_error", C.c_char_p, []),
@@ -396,6 +397,10 @@ class HipBackend:
         """Level smoothing on the device (pyramid.rs:429-437), per-axis sigma in mm."""
         s = np.ascontiguousarray(sigma_mm, dtype=np.float64).reshape(3)
         return self._img(lib().fw_smooth_image, image.handle, dptr(s))
+
+    def resample_isotropic(self, image: DeviceImage, mm: float) -> DeviceImage:
+        """The image on the pipeline's isotropic `mm` grid (register.rs:112-142), on the device."""
+        return self._img(lib().fw_resample_isotropic, image.handle, C.c_double(mm))
 
     def preprocess_image(self, raw: DeviceImage, lower: float = 0.005, upper: float = 0.995) -> DeviceImage:
         """Winsorise and window a raw image on the device (preprocess.rs:159-232)."""

@@ -146,6 +146,14 @@ int fw_image_download(fw_ctx* ctx, const fw_image* image, double* out) {
   });
 }
 
+int fw_resample_isotropic(fw_ctx* ctx, const fw_image* image, double mm, fw_image** out) {
+  return guard(ctx, [&] {
+    need(image, "image");
+    need(out, "output");
+    *out = new fw_image{ctx->backend.resample_isotropic(image->h, mm)};
+  });
+}
+
 int fw_image_grid(const fw_image* image, fw_grid* out) {
   if (!image || !out) return FW_ERR_INVALID;
   grid_to(image->h.grid, out);

@@ -3,6 +3,7 @@
 // composed in f64 on the host into continuous-index maps and narrowed to f32 once per launch.
 #include "backend.hpp"
 
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -330,6 +331,47 @@ HImage HipBackend::upload_image(const Image& v) {
   check(hipMemcpyAsync(im.buf.get(), stage, n * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()), "upload image");
   ctx_->sync();
   return im;
+}
+
+HImage HipBackend::resample_isotropic(const HImage& im, double mm) {
+  if (!(std::isfinite(mm) && mm > 0.0)) invalid("register", "isotropic spacing must be positive");
+  const Grid& g = im.grid;
+  const size_t n = g.num_voxels();
+  if (std::fabs(g.spacing[0] - mm) < 1e-6 && std::fabs(g.spacing[1] - mm) < 1e-6 && std::fabs(g.spacing[2] - mm) < 1e-6) {
+    HImage copy{g, DevBuf(ctx_.get(), n), im.absmax};
+    check(hipMemcpyAsync(copy.buf.get(), im.buf.get(), n * sizeof(float), hipMemcpyDeviceToDevice, ctx_->stream()), "copy image");
+    return copy;
+  }
+  // the source index of every output row, column and plane, split in f64 as the reference does
+  std::array<size_t, 3> out{1, 1, 1};
+  std::vector<int> lo;
+  std::vector<float> fr;
+  for (int a = 0; a < 3; ++a) {
+    const double last = double(g.size[a] - 1);
+    const double ratio = mm / g.spacing[a];
+    out[a] = std::max<size_t>(size_t(std::floor(last * g.spacing[a] / mm + 1e-9)) + 1, 1);
+    for (size_t i = 0; i < out[a]; ++i) {
+      const double c = std::min(double(i) * ratio, last);
+      const double f = std::floor(c);
+      lo.push_back(int(f));
+      fr.push_back(float(c - f));
+    }
+  }
+  const Grid og(out, {mm, mm, mm}, g.origin, g.direction);
+  DevBuf tables(ctx_.get(), lo.size() + fr.size());
+  check(hipMemcpyAsync(tables.get(), lo.data(), lo.size() * sizeof(int), hipMemcpyHostToDevice, ctx_->stream()), "upload table");
+  check(hipMemcpyAsync(tables.get() + lo.size(), fr.data(), fr.size() * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()),
+        "upload table");
+  HImage r{og, DevBuf(ctx_.get(), og.num_voxels()), im.absmax};
+  {
+    // the source read once (at most), the result written
+    Timed t(*ctx_, "k_resample_iso", 4.0 * double(og.num_voxels()) + 4.0 * std::min(double(n), double(og.num_voxels())));
+    launch_resample_iso(im.buf.get(), dims_of(g), r.buf.get(), dims_of(og), reinterpret_cast<const int*>(tables.get()),
+                        tables.get() + lo.size(), ctx_->stream());
+  }
+  // the tables are host vectors: the copies must finish before they go
+  ctx_->sync();
+  return r;
 }
 
 HImage HipBackend::upload_image_f32(const Grid& g, const float* data) {

@@ -241,6 +241,10 @@ class HipBackend {
   uint64_t memory_budget() const;
   HImage upload_image(const Image& v);
   HImage upload_image_f32(const Grid& g, const float* data);
+  // register.rs:112-142: the image on the isotropic `mm` grid (same origin and direction,
+  // floor((n - 1) * spacing / mm + 1e-9) + 1 voxels per axis), trilinear from the source indices
+  // min(i * mm / spacing, n - 1); a copy when every axis already has spacing mm (within 1e-6).
+  HImage resample_isotropic(const HImage& im, double mm);
   Image download_image(const HImage& im);
   HField zero_field(const Grid& g);
   HField upload_field(const VectorField& f);

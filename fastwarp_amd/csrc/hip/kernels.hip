@@ -734,6 +734,43 @@ __global__ __launch_bounds__(256) void k_resample(ResampleArgs a) {
   a.out[g + 2 * n] = vz;
 }
 
+// The pipeline's isotropic resample (register.rs:82-142): output voxel (i, j, k) samples the
+// source at per-axis indices whose integer parts (`lo`) and fractions (`fr`) the host computed in
+// f64; corners past the last voxel carry zero weight. The weights multiply in axis order and the
+// corners add in the reference's order (x bit first), in f32.
+__global__ __launch_bounds__(256) void k_resample_iso(const float* __restrict__ src, Dims sd, float* __restrict__ dst, Dims od,
+                                                      const int* __restrict__ lo, const float* __restrict__ fr) {
+  const uint3 bl = xcd_block();
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  const int k = bl.z;
+  if (i >= od.nx || j >= od.ny) return;
+  const int b[3] = {lo[i], lo[od.nx + j], lo[od.nx + od.ny + k]};
+  const float f[3] = {fr[i], fr[od.nx + j], fr[od.nx + od.ny + k]};
+  const int n[3] = {sd.nx, sd.ny, sd.nz};
+  float acc = 0.0f;
+#pragma unroll
+  for (int corner = 0; corner < 8; ++corner) {
+    float w = 1.0f;
+    int q[3];
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int bit = (corner >> a) & 1;
+      q[a] = b[a] + bit;
+      inside = inside && q[a] < n[a];
+      w *= bit ? f[a] : 1.0f - f[a];
+    }
+    if (inside && w != 0.0f) acc += w * src[(long long)q[0] + (long long)sd.nx * (q[1] + (long long)sd.ny * q[2])];
+  }
+  dst[(long long)i + (long long)od.nx * (j + (long long)od.ny * k)] = acc;
+}
+
+void launch_resample_iso(const float* src, Dims sd, float* dst, Dims od, const int* lo, const float* fr, hipStream_t s) {
+  const dim3 grid((od.nx + 63) / 64, (od.ny + 3) / 4, od.nz);
+  launch(k_resample_iso, grid, dim3(256), 0, s, src, sd, dst, od, lo, fr);
+}
+
 void launch_resample(const ResampleArgs& a, hipStream_t s) {
   const dim3 grid((a.od.nx + 63) / 64, (a.od.ny + 3) / 4, a.od.nz);
   launch(k_resample, grid, dim3(256), 0, s, a);

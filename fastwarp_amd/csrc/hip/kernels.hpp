@@ -134,6 +134,9 @@ struct ResampleArgs {
   float* out;        // 3 planes
 };
 void launch_resample(const ResampleArgs& a, hipStream_t s);
+// Image onto the isotropic grid `od`: lo / fr hold nx + ny + nz source index parts and fractions
+// (x axis first, then y, then z).
+void launch_resample_iso(const float* src, Dims sd, float* dst, Dims od, const int* lo, const float* fr, hipStream_t s);
 
 // --- compose: out = w + u(x + w) on one grid; l maps a displacement to index offsets ---
 void launch_compose(const float* displacement, const float* warping, Dims d, const float l[9], float* out, hipStream_t s);

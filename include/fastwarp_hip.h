@@ -104,6 +104,14 @@ int fw_memory_stats(fw_ctx* ctx, uint64_t* held, uint64_t* peak);
 int fw_image_upload(fw_ctx* ctx, const fw_grid* grid, const double* data, fw_image** out);
 int fw_image_upload_f32(fw_ctx* ctx, const fw_grid* grid, const float* data, fw_image** out);
 int fw_image_download(fw_ctx* ctx, const fw_image* image, double* out);
+/* The pipeline's isotropic resample (crates/fastwarp-reg/src/register.rs:112-142
+ * `resample_isotropic`, which `register` applies to every raw input when given an iso_mm): a new
+ * image on the grid of spacing mm with the same origin and direction and
+ * floor((n - 1) * spacing / mm + 1e-9) + 1 voxels per axis, trilinear from the source indices
+ * min(i * mm / spacing, n - 1); a copy when every axis already has spacing mm (within 1e-6). On
+ * the device, in f32 with the index splits computed in f64. A caller that registers at iso_mm
+ * resamples each raw image with it before fw_register_resident. */
+int fw_resample_isotropic(fw_ctx* ctx, const fw_image* image, double mm, fw_image** out);
 int fw_image_grid(const fw_image* image, fw_grid* out);
 void fw_image_free(fw_ctx* ctx, fw_image* image);
 

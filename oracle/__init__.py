@@ -60,6 +60,16 @@ def preprocess(grid: Grid, data, lower: float = 0.005, upper: float = 0.995) -> 
     return out
 
 
+def resample_isotropic(grid: Grid, data, mm: float) -> tuple[Grid, np.ndarray]:
+    """The pipeline's isotropic resample (register.rs:82-142), f64 rounded to f32: (grid, values)."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    og = Grid()
+    _check(lib().fwo_resample_isotropic(C.byref(grid), dptr(d), C.c_double(mm), C.byref(og), None, C.c_int64(0)))
+    out = np.empty(og.n)
+    _check(lib().fwo_resample_isotropic(C.byref(grid), dptr(d), C.c_double(mm), C.byref(og), dptr(out), C.c_int64(og.n)))
+    return og, out
+
+
 def centre_of_gravity(grid: Grid, data) -> np.ndarray:
     """Intensity-weighted centre in mm (init.rs:47-86), f64."""
     d = np.ascontiguousarray(data, dtype=np.float64)

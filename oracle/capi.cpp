@@ -87,6 +87,26 @@ int fwo_preprocess(const fw_grid* grid, const double* data, float lower, float u
   });
 }
 
+// register.rs:112-142 on the host in f64: the isotropic grid into *out_grid and, when `out` holds
+// at least its voxel count (`capacity`), the resampled values.
+int fwo_resample_isotropic(const fw_grid* grid, const double* data, double mm, fw_grid* out_grid, double* out,
+                           int64_t capacity) {
+  return guard([&] {
+    Image v = image_from(grid, data);
+    Image r = fw::oracle::resample_isotropic(v, mm);
+    for (int a = 0; a < 3; ++a) {
+      out_grid->size[a] = int64_t(r.grid.size[a]);
+      out_grid->spacing[a] = r.grid.spacing[a];
+      out_grid->origin[a] = r.grid.origin[a];
+      for (int c = 0; c < 3; ++c) out_grid->direction[3 * a + c] = r.grid.direction[a][c];
+    }
+    if (out) {
+      if (capacity < int64_t(r.data.size())) fw::invalid("register", "output buffer too small");
+      std::copy(r.data.begin(), r.data.end(), out);
+    }
+  });
+}
+
 // init.rs:47-86 on the host in f64.
 int fwo_centre_of_gravity(const fw_grid* grid, const double* data, double* centre) {
   return guard([&] {

@@ -887,4 +887,53 @@ inline VectorField resample_field(const VectorField& field, const Grid& grid) {
   return VectorField(grid, std::move(out));
 }
 
+// register.rs:82-142 `resample_isotropic` with `sample_linear_clamped`: the pipeline's isotropic
+// grid (same origin and direction, floor((n - 1) * spacing / mm + 1e-9) + 1 voxels per axis);
+// output voxel i samples source index min(i * mm / spacing, n - 1) trilinearly, corners outside
+// the buffer or of zero weight skipped; the result is rounded to f32. A copy when every axis
+// already has spacing mm (within 1e-6).
+inline Image resample_isotropic(const Image& v, double mm) {
+  if (!(std::isfinite(mm) && mm > 0.0)) invalid("register", "isotropic spacing must be positive");
+  const Grid& g = v.grid;
+  if (std::fabs(g.spacing[0] - mm) < 1e-6 && std::fabs(g.spacing[1] - mm) < 1e-6 && std::fabs(g.spacing[2] - mm) < 1e-6) return v;
+  std::array<size_t, 3> out{1, 1, 1};
+  std::array<std::vector<double>, 3> coords;
+  for (int a = 0; a < 3; ++a) {
+    const double last = double(g.size[a] - 1);
+    const double ratio = mm / g.spacing[a];
+    const size_t t = std::max<size_t>(size_t(std::floor(last * g.spacing[a] / mm + 1e-9)) + 1, 1);
+    for (size_t i = 0; i < t; ++i) coords[a].push_back(std::min(double(i) * ratio, last));
+    out[a] = t;
+  }
+  const Grid og(out, {mm, mm, mm}, g.origin, g.direction);
+  std::vector<double> data(og.num_voxels());
+  for (size_t k = 0; k < out[2]; ++k)
+    for (size_t j = 0; j < out[1]; ++j)
+      for (size_t i = 0; i < out[0]; ++i) {
+        const double c[3] = {coords[0][i], coords[1][j], coords[2][k]};
+        const double lo[3] = {std::floor(c[0]), std::floor(c[1]), std::floor(c[2])};
+        double acc = 0.0;
+        for (int corner = 0; corner < 8; ++corner) {
+          size_t idx[3];
+          double w = 1.0;
+          bool inside = true;
+          for (int a = 0; a < 3; ++a) {
+            const int bit = (corner >> a) & 1;
+            const long long q = (long long)lo[a] + bit;
+            if (q < 0 || q >= (long long)g.size[a]) {
+              inside = false;
+              break;
+            }
+            idx[a] = size_t(q);
+            const double fr = c[a] - lo[a];
+            w *= bit ? fr : 1.0 - fr;
+          }
+          if (w == 0.0 || !inside) continue;
+          acc += w * v.data[idx[0] + g.size[0] * (idx[1] + g.size[1] * idx[2])];
+        }
+        data[i + out[0] * (j + out[1] * k)] = double(float(acc));
+      }
+  return Image(og, std::move(data));
+}
+
 }  // namespace fw::oracle

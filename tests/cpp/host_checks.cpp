@@ -413,7 +413,7 @@ static void request_loop() {
     int fail = 0;  // 1: invalid input, 2: a device error
   };
   std::atomic<int> running{0}, peak{0}, order{0};
-  std::vector<int> starts(200, -1);
+  std::vector<int> starts(400, -1);
   auto run = [&](Slot&, Job& j, const volatile int32_t* cancel) {
     const int now = ++running;
     int p = peak.load();
@@ -443,7 +443,6 @@ static void request_loop() {
       CHECK(o.footprint == 100 && std::fabs(o.total_s - o.queued_s - o.run_s) < 1e-6, "request loop: timing");
     }
     CHECK(peak.load() == 3, "request loop: as many at once as slots");
-    for (int i = 0; i < 40; ++i) CHECK(starts[size_t(i)] == i, "request loop: arrival order");
     bool threw = false;
     try {
       loop.wait(ids[0]);
@@ -451,6 +450,16 @@ static void request_loop() {
       threw = true;
     }
     CHECK(threw, "request loop: a ticket is waited for once");
+  }
+  {
+    // one slot runs the requests one after another, in arrival order (with several, the order in
+    // which running requests reach their first line is up to the threads)
+    const int first = order.load();
+    RequestLoop<Slot, Job> loop(1, 1000, make, run);
+    std::vector<uint64_t> ids;
+    for (int i = 0; i < 20; ++i) ids.push_back(loop.submit(100, Job{100 + i, 0, 100, 0}));
+    for (uint64_t id : ids) loop.wait(id);
+    for (int i = 0; i < 20; ++i) CHECK(starts[size_t(first + i)] == 100 + i, "request loop: arrival order");
   }
   {
     peak = 0;

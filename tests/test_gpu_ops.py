@@ -222,3 +222,24 @@ def test_resample_matches_oracle(backend, name):
     keep = ~(np.all(got == 0, axis=1) | np.all(want == 0, axis=1))
     assert flips <= 1 + target.n // 500
     assert cases.rel_err(got[keep], want[keep]) <= 1e-5
+
+
+@pytest.mark.parametrize("size,spacing,mm", [((37, 29, 23), (1.5, 0.8, 2.0), 1.0), ((40, 44, 36), (0.7, 0.7, 1.3), 1.2),
+                                             ((121, 140, 97), (1.2, 1.2, 1.2), 1.0), ((20, 21, 22), (1.0, 1.0, 1.0), 1.0)])
+def test_isotropic_resample_matches_oracle(backend, size, spacing, mm):
+    """register.rs:112-142 on the device against the oracle's f64 rule: the same grid, values within
+    1e-6 of the image's range (f32 weights against f64), and a copy when already isotropic."""
+    g = cases.grid(size, spacing, (-11.0, 4.5, 2.25), oblique=0.2)
+    v = cases.image(g, 5)
+    og, want = oracle.resample_isotropic(g, v, mm)
+    got_img = backend.resample_isotropic(backend.upload_image(g, v), mm)
+    got = backend.download_image(got_img)
+    assert got.shape == want.shape, (got.shape, want.shape)
+    gg = got_img.grid
+    assert list(gg.size) == list(og.size)
+    assert np.allclose(list(gg.spacing), list(og.spacing), rtol=0, atol=0)
+    assert np.array_equal(np.array(list(gg.origin)), np.array(list(og.origin)))
+    assert np.array_equal(np.array(list(gg.direction)), np.array(list(og.direction)))
+    assert np.max(np.abs(got - want)) <= 1e-6 * np.max(np.abs(want))
+    if spacing == (mm, mm, mm):
+        assert np.array_equal(got, v.astype(np.float32).astype(np.float64))

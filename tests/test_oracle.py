@@ -402,3 +402,51 @@ def test_sampled_mi_pdf_is_normalised_and_value_is_minus_mi():
     nz = p > 1e-12
     mi = np.sum(p[nz] * np.log(p[nz] / np.outer(pf, pm)[nz]))
     assert mi > 0 and abs(r["value"] + mi) < 1e-9
+
+
+# ------------------------------------------------------------------ isotropic resample (register.rs) --
+
+
+def test_isotropic_resample_follows_the_pipeline_grid_rule():
+    """register.rs:447-459: 21 voxels at 0.5 mm become 11 at 1 mm, same origin, voxel (3, 4, 5)
+    takes source voxel (6, 8, 10), and a grid already at 1 mm is returned as it is."""
+    g = Grid.make((21, 21, 21), (0.5, 0.5, 0.5), (-3.0, 2.0, 1.5))
+    v = np.random.default_rng(4).random(g.n)
+    og, r = oracle.resample_isotropic(g, v, 1.0)
+    assert list(og.size) == [11, 11, 11] and list(og.spacing) == [1.0, 1.0, 1.0]
+    assert list(og.origin) == list(g.origin)
+    assert abs(r[3 + 11 * (4 + 11 * 5)] - v[6 + 21 * (8 + 21 * 10)]) < 1e-6
+    og2, same = oracle.resample_isotropic(og, r, 1.0)
+    assert list(og2.size) == [11, 11, 11] and np.array_equal(same, r)
+
+
+def test_isotropic_resample_interpolates_between_source_voxels():
+    """register.rs:462-471: [0, 3, 6, 9] at 1.5 mm resampled to 1 mm is [0, 2, 4, 6, 8]."""
+    g = Grid.make((4, 1, 1), (1.5, 1.0, 1.0))
+    og, r = oracle.resample_isotropic(g, np.array([0.0, 3.0, 6.0, 9.0]), 1.0)
+    assert list(og.size) == [5, 1, 1]
+    assert np.max(np.abs(r - [0.0, 2.0, 4.0, 6.0, 8.0])) < 1e-6
+
+
+@pytest.mark.parametrize("spacing,mm", [((1.5, 0.8, 2.0), 1.0), ((0.7, 0.7, 1.3), 1.2), ((1.0, 2.0, 3.0), 2.5)])
+def test_isotropic_resample_reproduces_a_linear_function(spacing, mm):
+    """Trilinear interpolation is exact on a linear function: the resampled volume is the function
+    at the new voxels' source indices (clamped at the last voxel), rounded to f32."""
+    size = (13, 9, 7)
+    g = Grid.make(size, spacing, (4.0, -2.0, 1.0))
+    i, j, k = np.meshgrid(*[np.arange(n, dtype=float) for n in size], indexing="ij")
+    f = lambda a, b, c: 0.3 * a - 1.7 * b + 0.45 * c + 2.0
+    v = f(i, j, k).transpose(2, 1, 0).reshape(-1)  # x-fastest
+    og, r = oracle.resample_isotropic(g, v, mm)
+    idx = [np.minimum(np.arange(og.size[a]) * (mm / spacing[a]), size[a] - 1) for a in range(3)]
+    for a in range(3):
+        assert og.size[a] == int(np.floor((size[a] - 1) * spacing[a] / mm + 1e-9)) + 1
+    ci, cj, ck = np.meshgrid(*idx, indexing="ij")
+    want = f(ci, cj, ck).transpose(2, 1, 0).reshape(-1).astype(np.float32).astype(np.float64)
+    assert np.max(np.abs(r - want)) <= 2e-6 * np.max(np.abs(want))
+
+
+def test_isotropic_resample_rejects_a_non_positive_spacing():
+    g = Grid.make((4, 4, 4))
+    with pytest.raises(oracle.OracleError, match="positive"):
+        oracle.resample_isotropic(g, np.zeros(g.n), 0.0)
