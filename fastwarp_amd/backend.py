@@ -103,6 +103,8 @@ _SIGNATURES = [
     ("fw_register_resident", C.c_int, [_P, C.POINTER(Plan), C.POINTER(_P), C.POINTER(_P), _PP, _PP, C.POINTER(Result),
                                        C.POINTER(C.c_int32)]),
     ("fw_resample_isotropic", C.c_int, [_P, _P, C.c_double, _PP]),
+    ("fw_write_affine_mat", C.c_int, [C.c_char_p, C.POINTER(Affine)]),
+    ("fw_read_affine_mat", C.c_int, [C.c_char_p, C.POINTER(Affine)]),
     ("fw_server_create", C.c_int, [C.c_int32, C.c_uint32, C.c_uint64, _PP]),
     ("fw_server_destroy", None, [_P]),
     ("fw_server_last_error", C.c_char_p, []),
@@ -116,6 +118,20 @@ _SIGNATURES = [
     ("fw_server_cancel", C.c_int, [_P, C.c_uint64]),
     ("fw_server_counts", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
 ]
+
+
+def write_affine_mat(path, affine: Affine) -> None:
+    """The affine as ANTs writes it (0GenericAffine.mat, itk_mat.rs:150-190)."""
+    if lib().fw_write_affine_mat(str(path).encode(), C.byref(affine)) != FW_OK:
+        raise RegError(FW_ERR_INVALID, lib().fw_last_error(None).decode())
+
+
+def read_affine_mat(path) -> Affine:
+    """The first transform of an ITK MATLAB transform file as an affine (itk_mat.rs:102-133)."""
+    out = Affine()
+    if lib().fw_read_affine_mat(str(path).encode(), C.byref(out)) != FW_OK:
+        raise RegError(FW_ERR_INVALID, lib().fw_last_error(None).decode())
+    return out
 
 
 def plan_capacity(plan: Plan, fixed_grid: Grid, moving_grid: Grid | None = None) -> Capacity:

@@ -8,6 +8,7 @@
 
 #include "hip/backend.hpp"
 #include "host/capacity.hpp"
+#include "host/itk_mat.hpp"
 #include "plan.hpp"
 
 using fw::Grid;
@@ -577,6 +578,35 @@ int fw_registrations_at_once(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fi
     need(count, "output");
     *count = fw::registrations_at_once(capacity_of(plan, fixed_grid, moving_grid), ctx->backend.memory_budget(), max);
   });
+}
+
+int fw_write_affine_mat(const char* path, const fw_affine* affine) {
+  try {
+    need(path, "path");
+    need(affine, "affine");
+    fw::write_affine_mat(path, affine_from(affine));
+    return FW_OK;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return FW_ERR_INVALID;
+  }
+}
+
+int fw_read_affine_mat(const char* path, fw_affine* out) {
+  try {
+    need(path, "path");
+    need(out, "output");
+    const fw::AffineTransform t = fw::read_affine_mat(path);
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) out->matrix[3 * r + c] = t.matrix[r][c];
+      out->translation[r] = t.translation[r];
+      out->center[r] = t.center[r];
+    }
+    return FW_OK;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return FW_ERR_INVALID;
+  }
 }
 
 void fw_plan_cohort(fw_plan* plan, int32_t channels) {

@@ -312,6 +312,16 @@ int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, con
 int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
                          fw_field** forward, fw_field** inverse, fw_result* result, const volatile int32_t* cancel);
 
+/* ---- Output files (crates/fastwarp-reg/src/transform/itk_mat.rs) ----
+ * The affine of a registration as ANTs writes it (0GenericAffine.mat): an ITK MATLAB Level-4 file
+ * of two f64 column vectors, `AffineTransform_double_3_3` (matrix row-major, then translation) and
+ * `fixed` (the centre), 193 bytes (format_mat / write_mat_file, itk_mat.rs:150-190). Reading takes
+ * the first transform of the file: the double or float forms of AffineTransform,
+ * MatrixOffsetTransformBase, Rigid3DTransform and TranslationTransform (parse_mat, itk_mat.rs:102-133
+ * with itk_params.rs:58-102). Host only, no device; on failure fw_last_error(NULL) gives the text. */
+int fw_write_affine_mat(const char* path, const fw_affine* affine);
+int fw_read_affine_mat(const char* path, fw_affine* out);
+
 /* ---- Serving: continuous batching of whole registrations on one device ------------------------
  * The reference runs several registrations on one GPU as independent device instances, as many
  * as its capacity model fits at once (crates/fastwarp-cli/src/engine.rs:1015-1068
