@@ -313,7 +313,11 @@ def main() -> int:
     }
     if args.slots > 0:
         backend.release_idle_buffers()
-        line["served"] = served(pair, plan, args.slots, args.requests)
+        try:
+            line["served"] = served(pair, plan, args.slots, args.requests)
+        except Exception as e:  # the plain line is still reported
+            log(f"served stream failed: {e}")
+            line["served"] = {"error": str(e)}
     if not args.no_cpu:
         log("CPU-engine baseline (bounded sample)")
         line["cpu_baseline"] = cpu_baseline(pair, last.result)
