@@ -58,6 +58,18 @@ def test_a_server_on_a_machine_without_a_gpu_fails_cleanly():
         Server(0, slots=2)
 
 
+def test_the_header_is_plain_c(tmp_path):
+    """The boundary is a C ABI: the header compiles as C99 with every warning on."""
+    import subprocess
+    from pathlib import Path
+
+    src = tmp_path / "use.c"
+    src.write_text('#include "fastwarp_hip.h"\nint main(void) { fw_served_timing t; fw_affine a; (void)t; (void)a; return 0; }\n')
+    include = Path(__file__).resolve().parent.parent / "include"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", f"-I{include}", "-fsyntax-only", str(src)],
+                   check=True)
+
+
 def test_oracle_library_loads():
     import oracle
 
