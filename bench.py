@@ -203,7 +203,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=12, help="requests of the served stream")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box of this build has a single MI355X (asked for {args.gpus})", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for {args.gpus}): registrations are independent and would shard across "
+              "GPUs by request, one server per GPU", file=sys.stderr)
         return 2
     if args.kernels:
         return kernels(max(args.steps, 1) * 10)
