@@ -74,3 +74,16 @@ def test_oracle_library_loads():
     import oracle
 
     assert abs(oracle.cubic_bspline(0.0) - 4.0 / 6.0) < 1e-15
+
+
+def test_served_stream_mix_follows_its_recipe():
+    """The served-stream driver's request mix: about 70 % full normalisations, spacings 2, 1.5 and
+    1 mm, two channels only on full requests, the same mix for the same seed."""
+    from fastwarp_amd.serve_stream import GRIDS, make_mix
+
+    mix = make_mix(2000)
+    full = [m for m in mix if m[2]]
+    assert 0.66 < len(full) / len(mix) < 0.74
+    assert {m[0] for m in mix} == set(GRIDS) and {m[1] for m in full} == {1, 2}
+    assert all(m[1] == 1 for m in mix if not m[2])
+    assert make_mix(50) == make_mix(50)
