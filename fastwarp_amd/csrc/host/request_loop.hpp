@@ -9,7 +9,10 @@
 // slot thread i. Payload: a request's inputs and outputs, owned by the loop from submission until
 // it has been waited for. `run(slot, payload, cancel)` runs one request; it may throw
 // (Cancelled, Error or any std::exception) and should poll `*cancel` with an atomic load (the
-// loop sets it from another thread with an atomic store).
+// loop sets it from another thread with an atomic store). `held(slot)` is the device bytes the
+// slot's pool keeps once a request has ended, and `release(slot)` gives them back; the loop
+// charges what a slot keeps against the budget (admission.hpp) and has an idle slot release it
+// whenever the head of the queue does not fit.
 #pragma once
 
 #include <chrono>
@@ -43,10 +46,13 @@ class RequestLoop {
  public:
   using MakeSlot = std::function<std::unique_ptr<Slot>(uint32_t)>;
   using Run = std::function<void(Slot&, Payload&, const volatile int32_t*)>;
+  using Held = std::function<uint64_t(Slot&)>;
+  using Release = std::function<void(Slot&)>;
 
   // Starts `slots` threads; throws (Error "hip" with the first message) when a slot cannot be
   // built, after stopping the others.
-  RequestLoop(uint32_t slots, uint64_t budget, MakeSlot make_slot, Run run) : queue_(budget), run_(std::move(run)) {
+  RequestLoop(uint32_t slots, uint64_t budget, MakeSlot make_slot, Run run, Held held, Release release)
+      : queue_(budget), run_(std::move(run)), held_(std::move(held)), release_(std::move(release)) {
     std::vector<std::string> errors(slots);
     uint32_t ready = 0;
     for (uint32_t s = 0; s < slots; ++s)
@@ -97,12 +103,14 @@ class RequestLoop {
     return r->id;
   }
 
-  // Blocks until the request has ended and hands it back (once; the ticket is then unknown).
+  // Blocks until the request has ended and hands it back (once: the ticket is unknown to every
+  // later or concurrent wait).
   std::pair<Outcome, Payload> wait(uint64_t id) {
     std::unique_lock<std::mutex> lk(mu_);
     auto it = requests_.find(id);
-    if (it == requests_.end()) invalid("server", "unknown ticket " + std::to_string(id));
+    if (it == requests_.end() || it->second->claimed) invalid("server", "unknown ticket " + std::to_string(id));
     std::shared_ptr<Request> r = it->second;
+    r->claimed = true;
     done_cv_.wait(lk, [&] { return r->done; });
     requests_.erase(id);
     Outcome o;
@@ -146,6 +154,7 @@ class RequestLoop {
     Clock::time_point submitted, started, finished;
     int32_t slot = -1;
     bool done = false;
+    bool claimed = false;  // a caller waits for it
     Ended status = Ended::Ok;
     std::string error;
   };
@@ -162,13 +171,38 @@ class RequestLoop {
   }
 
   void serve(uint32_t index, Slot& slot) {
+    uint64_t kept = 0;          // idle bytes this slot's pool keeps, as charged to the queue
+    bool released = false;      // nothing more to give back until the next request
     for (;;) {
       std::shared_ptr<Request> r;
       {
         std::unique_lock<std::mutex> lk(mu_);
-        work_cv_.wait(lk, [&] { return stopping_ || queue_.can_admit(); });
-        if (stopping_) return;
-        r = requests_.at(queue_.admit());
+        work_cv_.wait(lk, [&] {
+          return stopping_ || queue_.can_admit() || (kept > 0 && !released && queue_.has_waiting());
+        });
+        if (stopping_) {
+          queue_.retain(kept, 0);
+          return;
+        }
+        if (!queue_.can_admit()) {
+          // the head does not fit beside what is held: give this slot's idle buffers back
+          lk.unlock();
+          uint64_t now = kept;
+          try {
+            release_(slot);
+            now = held_(slot);
+          } catch (const std::exception&) {
+          }
+          lk.lock();
+          queue_.retain(kept, now);
+          kept = now;
+          released = true;
+          work_cv_.notify_all();
+          continue;
+        }
+        r = requests_.at(queue_.admit(kept));
+        kept = 0;
+        released = false;
         r->started = Clock::now();
         r->slot = int32_t(index);
       }
@@ -186,8 +220,15 @@ class RequestLoop {
         status = Ended::Invalid;
         error = e.what();
       }
+      uint64_t now = 0;
+      try {
+        now = held_(slot);
+      } catch (const std::exception&) {
+      }
       std::lock_guard<std::mutex> lk(mu_);
       queue_.finish(r->id);
+      queue_.retain(0, now);
+      kept = now;
       finish_locked(*r, status, error);
       work_cv_.notify_all();
     }
@@ -216,6 +257,8 @@ class RequestLoop {
   std::condition_variable work_cv_, done_cv_, ready_cv_;
   AdmissionQueue queue_;
   Run run_;
+  Held held_;
+  Release release_;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> requests_;
   std::vector<std::thread> threads_;
   bool stopping_ = false;

@@ -54,14 +54,21 @@ inline void execute(hip::HipBackend& b, Job& r, const volatile int32_t* cancel) 
 class Server {
  public:
   Server(int device, uint32_t slots, uint64_t budget)
-      : loop_(check_slots(slots), budget, [device](uint32_t) { return std::make_unique<hip::HipBackend>(device); },
-              [](hip::HipBackend& b, Job& job, const volatile int32_t* cancel) {
-                // the slot keeps its pool for the next request of the same size, but not beyond
-                // the footprint this one was admitted with
+      : loop_(
+            check_slots(slots), budget, [device](uint32_t) { return std::make_unique<hip::HipBackend>(device); },
+            [](hip::HipBackend& b, Job& job, const volatile int32_t* cancel) {
+              // the slot keeps its pool for the next request of the same size, but not beyond the
+              // footprint this one was admitted with, and not at all after a failure
+              try {
                 execute(b, job, cancel);
-                const uint64_t keep = footprint_of(job);
-                if (b.ctx().held() > keep) b.release_idle_buffers();
-              }) {}
+              } catch (...) {
+                b.release_idle_buffers();
+                throw;
+              }
+              if (b.ctx().held() > footprint_of(job)) b.release_idle_buffers();
+            },
+            [](hip::HipBackend& b) { return uint64_t(b.ctx().held()); },
+            [](hip::HipBackend& b) { b.release_idle_buffers(); }) {}
 
   uint64_t submit(const fw_plan* plan, const fw_grid* fixed_grid, const float* const* fixed, const fw_grid* moving_grid,
                   const float* const* moving, double* forward, double* inverse) {

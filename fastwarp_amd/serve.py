@@ -10,20 +10,20 @@ alone and in any mix.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from dataclasses import dataclass
 
 import numpy as np
 
 from ._abi import Affine, Grid, Plan, Result, ServedTiming, dptr
-from .backend import FW_OK, RegError, lib
+from .backend import FW_ERR_INVALID, FW_OK, RegError, lib
 
 
 @dataclass
 class Ticket:
-    """A queued request; holds its input and output arrays until it has been waited for."""
+    """A queued request: its id and the output arrays its fields are written into."""
 
     id: int
-    keep: tuple
     forward: np.ndarray | None
     inverse: np.ndarray | None
 
@@ -52,6 +52,11 @@ class Server:
 
     def __init__(self, device: int = 0, slots: int = 0, budget_bytes: int = 0):
         self.handle = C.c_void_p()
+        # ticket id -> the arrays and structs the native slots read and write until the request
+        # has ended: held here, not by the Ticket, so that a ticket dropped without a wait cannot
+        # free memory a slot still uses
+        self._keep: dict[int, tuple] = {}
+        self._lock = threading.Lock()
         st = lib().fw_server_create(device, slots, budget_bytes, C.byref(self.handle))
         if st != FW_OK:
             raise _error(st)
@@ -69,13 +74,23 @@ class Server:
                                     C.byref(ticket))
         if st != FW_OK:
             raise _error(st)
-        return Ticket(ticket.value, (fa, fp, ma, mp, pl, fg, mg), fwd, inv)
+        self._keep[ticket.value] = (fa, fp, ma, mp, pl, fg, mg, fwd, inv)
+        return Ticket(ticket.value, fwd, inv)
 
     def wait(self, ticket: Ticket) -> Served:
+        """Blocks until the request has ended; raises RegError when it failed or was cancelled.
+        A ticket is waited for once: a second wait, also a concurrent one, is refused."""
+        with self._lock:
+            keep = self._keep.pop(ticket.id, None)
+        if keep is None:
+            raise RegError(FW_ERR_INVALID, f"server: unknown ticket {ticket.id}")
         res, timing = Result(), ServedTiming()
+        # `keep` holds the request's arrays until the native wait has returned, i.e. until the
+        # request has ended and its slot no longer touches them
         st = lib().fw_server_wait(self.handle, ticket.id, C.byref(res), C.byref(timing))
         if st != FW_OK:
             raise _error(st)
+        del keep
         return Served(res.affine, ticket.forward, ticket.inverse, res, timing)
 
     def cancel(self, ticket: Ticket) -> None:
@@ -92,9 +107,11 @@ class Server:
         return w.value, r.value, b.value
 
     def close(self) -> None:
+        """Cancels what has not ended and stops the slots, then lets go of the requests' arrays."""
         if self.handle:
             lib().fw_server_destroy(self.handle)
             self.handle = C.c_void_p()
+        self._keep.clear()
 
     def __enter__(self) -> "Server":
         return self

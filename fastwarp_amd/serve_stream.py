@@ -58,37 +58,54 @@ def plan_for(channels: int, full: bool):
 
 def run_stream(server: Server, mix, pairs, rate_per_s: float | None, seed: int = 5):
     """Submits the mix at Poisson arrival times (all at once when `rate_per_s` is None) and waits
-    for every result; returns (seconds from the first submission to the last result, per-request
-    latencies)."""
+    for every result, failed ones included; returns (seconds from the first submission to the last
+    result, per-request latencies, failure messages). A request that could not be submitted counts
+    as failed, and the stream still waits for every request it did submit."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_per_s, len(mix)) if rate_per_s else np.zeros(len(mix))
     tickets = [None] * len(mix)
     submitted = [0.0] * len(mix)
     latency = [0.0] * len(mix)
+    failures: list[str] = []
     ready = threading.Semaphore(0)
 
     def collect():
         # the server times each request from its submission to its end
         for i in range(len(mix)):
             ready.acquire()
-            latency[i] = server.wait(tickets[i]).timing.total_s
+            if tickets[i] is None:
+                continue
+            try:
+                latency[i] = server.wait(tickets[i]).timing.total_s
+            except Exception as e:  # a failed request: counted, and the stream goes on
+                failures.append(f"request {i}: {e}")
 
     waiter = threading.Thread(target=collect)
     waiter.start()
     t0 = time.perf_counter()
     due = t0
-    for i, (spacing, channels, full) in enumerate(mix):
-        due += gaps[i]
-        pause = due - time.perf_counter()
-        if pause > 0:
-            time.sleep(pause)
-        grid, f, m = pairs[(spacing, channels)]
-        submitted[i] = time.perf_counter()
-        tickets[i] = server.submit(plan_for(channels, full), grid, f, grid, m, fields=full)
-        ready.release()
-    waiter.join()
+    released = 0
+    try:
+        for i, (spacing, channels, full) in enumerate(mix):
+            due += gaps[i]
+            pause = due - time.perf_counter()
+            if pause > 0:
+                time.sleep(pause)
+            grid, f, m = pairs[(spacing, channels)]
+            submitted[i] = time.perf_counter()
+            try:
+                tickets[i] = server.submit(plan_for(channels, full), grid, f, grid, m, fields=full)
+            except Exception as e:
+                failures.append(f"request {i} not submitted: {e}")
+            ready.release()
+            released += 1
+    finally:
+        # the collector waits for every submitted ticket, also when submission stopped early
+        for _ in range(len(mix) - released):
+            ready.release()
+        waiter.join()
     wall = max(sub - t0 + lat for sub, lat in zip(submitted, latency))
-    return wall, latency
+    return wall, latency, failures
 
 
 def main(argv=None) -> int:
@@ -101,11 +118,12 @@ def main(argv=None) -> int:
     pairs = pairs_for(mix)
     with Server(slots=args.slots) as s:
         # warm every slot on every kind of request, then the saturated rate on a prefix of the mix
-        run_stream(s, sorted(set(mix)), pairs, None)
+        _, _, warm_failures = run_stream(s, sorted(set(mix)), pairs, None)
         prefix = mix[: max(8, len(mix) // 4)]
-        sat_wall, _ = run_stream(s, prefix, pairs, None)
+        sat_wall, _, sat_failures = run_stream(s, prefix, pairs, None)
         saturated = len(prefix) / sat_wall
-        wall, lat = run_stream(s, mix, pairs, args.load * saturated)
+        wall, lat, failures = run_stream(s, mix, pairs, args.load * saturated)
+    failures = warm_failures + sat_failures + failures
     lat = np.array(lat)
     kinds = {}
     for spacing, channels, full in mix:
@@ -116,8 +134,9 @@ def main(argv=None) -> int:
                       "arrival_per_hour": round(3600.0 * args.load * saturated, 1),
                       "served_per_hour": round(3600.0 * len(mix) / wall, 1),
                       "latency_s": {q: round(float(np.percentile(lat, int(q[1:]))), 3) for q in ("p50", "p95", "p99")},
-                      "kinds": dict(sorted(kinds.items()))}), flush=True)
-    return 0
+                      "kinds": dict(sorted(kinds.items())), "failed": len(failures),
+                      "failures": failures[:8]}), flush=True)
+    return 1 if failures else 0
 
 
 if __name__ == "__main__":

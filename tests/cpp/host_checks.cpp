@@ -407,11 +407,14 @@ static void mi_finish() {
 static void request_loop() {
   struct Slot {
     uint32_t index;
+    uint64_t pool = 0;  // idle bytes the slot keeps between requests
   };
   struct Job {
     int input = 0, output = 0, sleep_us = 0;
     int fail = 0;  // 1: invalid input, 2: a device error
   };
+  auto none = [](Slot&) { return uint64_t(0); };
+  auto keep = [](Slot&) {};
   std::atomic<int> running{0}, peak{0}, order{0};
   std::vector<int> starts(400, -1);
   auto run = [&](Slot&, Job& j, const volatile int32_t* cancel) {
@@ -434,7 +437,7 @@ static void request_loop() {
   };
   auto make = [](uint32_t i) { return std::make_unique<Slot>(Slot{i}); };
   {
-    RequestLoop<Slot, Job> loop(3, 1000, make, run);
+    RequestLoop<Slot, Job> loop(3, 1000, make, run, none, keep);
     std::vector<uint64_t> ids;
     for (int i = 0; i < 40; ++i) ids.push_back(loop.submit(100, Job{i, 0, 1000, 0}));
     for (int i = 0; i < 40; ++i) {
@@ -455,7 +458,7 @@ static void request_loop() {
     // one slot runs the requests one after another, in arrival order (with several, the order in
     // which running requests reach their first line is up to the threads)
     const int first = order.load();
-    RequestLoop<Slot, Job> loop(1, 1000, make, run);
+    RequestLoop<Slot, Job> loop(1, 1000, make, run, none, keep);
     std::vector<uint64_t> ids;
     for (int i = 0; i < 20; ++i) ids.push_back(loop.submit(100, Job{100 + i, 0, 100, 0}));
     for (uint64_t id : ids) loop.wait(id);
@@ -463,7 +466,7 @@ static void request_loop() {
   }
   {
     peak = 0;
-    RequestLoop<Slot, Job> loop(3, 250, make, run);
+    RequestLoop<Slot, Job> loop(3, 250, make, run, none, keep);
     std::vector<uint64_t> ids;
     for (int i = 0; i < 12; ++i) ids.push_back(loop.submit(100, Job{i, 0, 1000, 0}));
     for (uint64_t id : ids) loop.wait(id);
@@ -477,7 +480,7 @@ static void request_loop() {
     CHECK(threw, "request loop: a request beyond the budget is refused");
   }
   {
-    RequestLoop<Slot, Job> loop(1, 1000, make, run);
+    RequestLoop<Slot, Job> loop(1, 1000, make, run, none, keep);
     const uint64_t a = loop.submit(10, Job{1, 0, 2000000, 0});
     const uint64_t b = loop.submit(10, Job{2, 0, 1000, 0});
     const uint64_t c = loop.submit(10, Job{3, 0, 1000, 1});
@@ -501,7 +504,7 @@ static void request_loop() {
   {
     const auto t0 = std::chrono::steady_clock::now();
     {
-      RequestLoop<Slot, Job> loop(2, 1000, make, run);
+      RequestLoop<Slot, Job> loop(2, 1000, make, run, none, keep);
       for (int i = 0; i < 6; ++i) loop.submit(10, Job{i, 0, 2000000, 0});
       std::this_thread::sleep_for(std::chrono::milliseconds(5));
     }
@@ -516,11 +519,76 @@ static void request_loop() {
           if (i == 1) throw Error("hip", "no device");
           return std::make_unique<Slot>(Slot{i});
         },
-        run);
+        run, none, keep);
   } catch (const Error& e) {
     threw = e.what_ == "hip";
   }
   CHECK(threw, "request loop: a slot that cannot start fails the loop");
+  {
+    // two callers wait for one ticket at once: one gets the request, the other an unknown ticket
+    RequestLoop<Slot, Job> loop(1, 1000, make, run, none, keep);
+    const uint64_t id = loop.submit(10, Job{7, 0, 20000, 0});
+    std::atomic<int> got{0}, unknown{0};
+    std::vector<std::thread> waiters;
+    for (int t = 0; t < 2; ++t)
+      waiters.emplace_back([&] {
+        try {
+          auto [o, j] = loop.wait(id);
+          if (o.status == Ended::Ok && j.output == 15) ++got;
+        } catch (const Error& e) {
+          if (e.message.find("unknown ticket") != std::string::npos) ++unknown;
+        }
+      });
+    for (auto& w : waiters) w.join();
+    CHECK(got.load() == 1 && unknown.load() == 1, "request loop: a second concurrent wait is refused");
+  }
+  {
+    // slots that keep their buffers between requests: what runs plus what the idle pools keep
+    // never exceeds the budget, since a running request may not reuse its slot's kept buffers
+    // (each run allocates its whole footprint beside them), and a head that does not fit makes
+    // idle slots give their buffers back. Failing requests keep their buffers too.
+    std::mutex mu;
+    uint64_t device = 0, device_peak = 0;
+    std::atomic<int> over{0};
+    auto run_keep = [&](Slot& s, Job& j, const volatile int32_t*) {
+      const uint64_t fp = uint64_t(j.input);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        device += fp;
+        device_peak = std::max(device_peak, device);
+        if (device > 1000) ++over;
+      }
+      std::this_thread::sleep_for(std::chrono::microseconds(j.sleep_us));
+      std::lock_guard<std::mutex> lk(mu);
+      // the request's buffers stay in the pool: it keeps the larger of the old pool and this one
+      const uint64_t kept = std::max(s.pool, fp);
+      device = device - fp - s.pool + kept;
+      s.pool = kept;
+      if (j.fail) throw Error("test", "failed with its buffers kept");
+    };
+    auto held = [&](Slot& s) {
+      std::lock_guard<std::mutex> lk(mu);
+      return s.pool;
+    };
+    auto release = [&](Slot& s) {
+      std::lock_guard<std::mutex> lk(mu);
+      device -= s.pool;
+      s.pool = 0;
+    };
+    std::mt19937 rng(3);
+    {
+      RequestLoop<Slot, Job> loop(4, 1000, make, run_keep, held, release);
+      std::vector<uint64_t> ids;
+      for (int i = 0; i < 300; ++i) {
+        const int fp = 50 + int(rng() % 500);
+        ids.push_back(loop.submit(uint64_t(fp), Job{fp, 0, int(rng() % 300), int(rng() % 7 == 0)}));
+      }
+      int ok_count = 0;
+      for (uint64_t id : ids) ok_count += loop.wait(id).first.status == Ended::Ok;
+      CHECK(ok_count > 200, "request loop: kept buffers: requests run");
+    }
+    CHECK(over.load() == 0 && device_peak <= 1000, "request loop: kept buffers count against the budget");
+  }
   ok("request_loop");
 }
 

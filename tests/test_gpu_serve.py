@@ -126,4 +126,23 @@ def test_footprint_beyond_the_budget_is_refused():
         with pytest.raises(RegError, match="beyond the budget"):
             s.submit(plan, grid, fixed, grid, moving)
         with pytest.raises(RegError, match="unknown ticket"):
-            s.wait(Ticket(12345, (), None, None))
+            s.wait(Ticket(12345, None, None))
+
+
+def test_dropped_tickets_and_a_second_wait():
+    """Tickets dropped without a wait keep their arrays alive inside the server until it stops
+    (the slots still read the inputs and write the fields), and a ticket is waited for once."""
+    import gc
+
+    reqs = requests()
+    s = Server(slots=2)
+    for r in reqs:
+        s.submit(r[0], r[1], [v.copy() for v in r[2]], r[1], [v.copy() for v in r[3]])
+    gc.collect()
+    s.close()
+    with Server(slots=2) as s:
+        t = s.submit(reqs[1][0], reqs[1][1], reqs[1][2], reqs[1][1], reqs[1][3])
+        got = s.wait(t)
+        assert got.forward is not None and np.all(np.isfinite(got.forward))
+        with pytest.raises(RegError, match="unknown ticket"):
+            s.wait(t)
