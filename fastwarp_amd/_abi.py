@@ -189,6 +189,8 @@ class Plan(C.Structure):
         ("winsorize_upper", C.c_double),
         ("run_linear", C.c_int32),
         ("run_syn", C.c_int32),
+        ("initial_kind", C.c_int32),
+        ("initial_affine", Affine),
     ]
 
     def set_linear(self, iterations, shrink, sigmas):
@@ -239,6 +241,32 @@ class Result(C.Structure):
         ("linear_evaluations", (C.c_int64 * 8) * 2),
         ("syn_inversion_steps", C.c_int64 * 8),
     ]
+
+
+class TraceStep(C.Structure):
+    """fw_trace_step: one CG iteration (stage 0 rigid, 1 affine) or SyN iteration (stage 2)."""
+
+    _fields_ = [
+        ("stage", C.c_int32),
+        ("level", C.c_int32),
+        ("iteration", C.c_int32),
+        ("pad", C.c_int32),
+        ("value", C.c_double),
+        ("convergence_value", C.c_double),
+        ("update_scale", C.c_double * 2),
+        ("inversion_steps", C.c_int32 * 4),
+        ("inversion_mean_error", C.c_double * 4),
+        ("inversion_max_error", C.c_double * 4),
+    ]
+
+
+def read_trace(fn) -> list[TraceStep]:
+    """The records of the last registration on this thread from `fn` (fw_last_trace or the
+    oracle's counterpart)."""
+    n = fn(None, 0)
+    buf = (TraceStep * max(n, 1))()
+    fn(buf, n)
+    return list(buf[:n])
 
 
 class ServedTiming(C.Structure):

@@ -15,7 +15,7 @@ from pathlib import Path
 import numpy as np
 
 from ._abi import (Affine, Capacity, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, Plan, Result, ServedTiming,
-                   aos, dptr, u32ptr)
+                   TraceStep, aos, dptr, read_trace, u32ptr)
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfastwarp_hip.so"
 
@@ -102,6 +102,7 @@ _SIGNATURES = [
     ),
     ("fw_register_resident", C.c_int, [_P, C.POINTER(Plan), C.POINTER(_P), C.POINTER(_P), _PP, _PP, C.POINTER(Result),
                                        C.POINTER(C.c_int32)]),
+    ("fw_last_trace", C.c_int32, [C.POINTER(TraceStep), C.c_int32]),
     ("fw_resample_isotropic", C.c_int, [_P, _P, C.c_double, _PP]),
     ("fw_write_affine_mat", C.c_int, [C.c_char_p, C.POINTER(Affine)]),
     ("fw_read_affine_mat", C.c_int, [C.c_char_p, C.POINTER(Affine)]),
@@ -175,9 +176,10 @@ class DeviceImage:
         return g
 
     def free(self):
-        if self.handle:
+        # a handle outliving its closed context is dropped: the context took the device memory
+        if self.handle and self._b.ctx:
             lib().fw_image_free(self._b.ctx, self.handle)
-            self.handle = None
+        self.handle = None
 
     def take(self) -> C.c_void_p:
         h, self.handle = self.handle, None
@@ -198,9 +200,9 @@ class DeviceField(DeviceImage):
         return g
 
     def free(self):
-        if self.handle:
+        if self.handle and self._b.ctx:
             lib().fw_field_free(self._b.ctx, self.handle)
-            self.handle = None
+        self.handle = None
 
 
 @dataclass

@@ -613,6 +613,8 @@ void fw_plan_cohort(fw_plan* plan, int32_t channels) {
   if (plan) fw::cohort_plan(plan, channels);
 }
 
+int32_t fw_last_trace(fw_trace_step* out, int32_t capacity) { return fw::copy_last_trace(out, capacity); }
+
 int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
                          fw_field** forward, fw_field** inverse, fw_result* result, const volatile int32_t* cancel) {
   return guard(ctx, [&] {

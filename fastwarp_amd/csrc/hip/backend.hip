@@ -338,7 +338,7 @@ HImage HipBackend::resample_isotropic(const HImage& im, double mm) {
   const Grid& g = im.grid;
   const size_t n = g.num_voxels();
   if (std::fabs(g.spacing[0] - mm) < 1e-6 && std::fabs(g.spacing[1] - mm) < 1e-6 && std::fabs(g.spacing[2] - mm) < 1e-6) {
-    HImage copy{g, DevBuf(ctx_.get(), n), im.absmax};
+    HImage copy{g, DevBuf(ctx_.get(), n), im.bound};
     check(hipMemcpyAsync(copy.buf.get(), im.buf.get(), n * sizeof(float), hipMemcpyDeviceToDevice, ctx_->stream()), "copy image");
     return copy;
   }
@@ -362,7 +362,7 @@ HImage HipBackend::resample_isotropic(const HImage& im, double mm) {
   check(hipMemcpyAsync(tables.get(), lo.data(), lo.size() * sizeof(int), hipMemcpyHostToDevice, ctx_->stream()), "upload table");
   check(hipMemcpyAsync(tables.get() + lo.size(), fr.data(), fr.size() * sizeof(float), hipMemcpyHostToDevice, ctx_->stream()),
         "upload table");
-  HImage r{og, DevBuf(ctx_.get(), og.num_voxels()), im.absmax};
+  HImage r{og, DevBuf(ctx_.get(), og.num_voxels()), im.bound};
   {
     // the source read once (at most), the result written
     Timed t(*ctx_, "k_resample_iso", 4.0 * double(og.num_voxels()) + 4.0 * std::min(double(n), double(og.num_voxels())));
@@ -486,7 +486,7 @@ static WarpArgs warp_args(const Grid& src, const HField& field, const HField* pr
 
 HImage HipBackend::warp_through_field(const HImage& src, const HField& field, const HField* pre, const AffineTransform* affine) {
   WarpArgs a = warp_args(src.grid, field, pre, affine);
-  HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), src.absmax};
+  HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), src.bound};
   a.src = src.buf.get();
   a.out = out.buf.get();
   const double no = double(field.grid.num_voxels());
@@ -501,7 +501,7 @@ std::pair<HImage, HImage> HipBackend::warp_pair(const HImage& src0, const HField
     return {warp_through_field(src0, field0, nullptr, affine0), warp_through_field(src1, field1, nullptr, affine1)};
   WarpArgs a0 = warp_args(src0.grid, field0, nullptr, affine0), a1 = warp_args(src1.grid, field1, nullptr, affine1);
   const Grid& g = field0.grid;
-  HImage out0{g, DevBuf(ctx_.get(), g.num_voxels()), src0.absmax}, out1{g, DevBuf(ctx_.get(), g.num_voxels()), src1.absmax};
+  HImage out0{g, DevBuf(ctx_.get(), g.num_voxels()), src0.bound}, out1{g, DevBuf(ctx_.get(), g.num_voxels()), src1.bound};
   a0.src = src0.buf.get();
   a0.out = out0.buf.get();
   a1.src = src1.buf.get();
@@ -520,14 +520,14 @@ HImage HipBackend::warp_mask_through_field(const HImage& mask, const HField& fie
   a.nearest = 1;
   if (product) {
     HImage out = std::move(*product);
+    out.bound *= mask.bound;
     a.product = out.buf.get();
     a.out = out.buf.get();
-    out.absmax = 1.0f;
     Timed t(*ctx_, "k_warp", 24.0 * double(field.grid.num_voxels()));
     launch_warp(a, ctx_->stream());
     return out;
   }
-  HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), 1.0f};
+  HImage out{field.grid, DevBuf(ctx_.get(), field.grid.num_voxels()), mask.bound};
   a.out = out.buf.get();
   Timed t(*ctx_, "k_warp", 20.0 * double(field.grid.num_voxels()));
   launch_warp(a, ctx_->stream());
@@ -564,10 +564,10 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
   a.d = dims_of(g);
   a.radius = int(radius);
   a.tile_z = cc_tile_for(a.d, cc_tile_z);
-  a.qf = fixed_point_scale(fixed.absmax);
-  a.qm = fixed_point_scale(moving.absmax);
-  a.inv_qf = 1.0f / a.qf;
-  a.inv_qm = 1.0f / a.qm;
+  a.scale_f = cc_split_scale(fixed.bound);
+  a.scale_m = cc_split_scale(moving.bound);
+  a.step_f = 1.0f / a.scale_f;
+  a.step_m = 1.0f / a.scale_m;
   a.probes = probes_of(g);
   a.d_fixed = out.fields[0].buf.get();
   a.d_moving = out.fields[1].buf.get();
@@ -643,10 +643,10 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
   a.d = dims_of(g);
   a.radius = int(radius);
   a.tile_z = cc_tile_for(a.d, cc_tile_z);
-  a.qf = fixed_point_scale(fixed.absmax);
-  a.qm = fixed_point_scale(moving.absmax);
-  a.inv_qf = 1.0f / a.qf;
-  a.inv_qm = 1.0f / a.qm;
+  a.scale_f = cc_split_scale(fixed.bound);
+  a.scale_m = cc_split_scale(moving.bound);
+  a.step_f = 1.0f / a.scale_f;
+  a.step_m = 1.0f / a.scale_m;
   a.probes = probes_of(g);
   a.d_fixed = out.fields[0].buf.get();
   a.d_moving = out.fields[1].buf.get();
@@ -734,7 +734,7 @@ HImage HipBackend::smooth_image(const HImage& src, double sigma, SigmaUnits unit
 HImage HipBackend::smooth_image_mm(const HImage& src, const Vec3& sigma_mm) {
   const Grid& g = src.grid;
   const size_t n = g.num_voxels();
-  HImage out{g, DevBuf(ctx_.get(), n), src.absmax};
+  HImage out{g, DevBuf(ctx_.get(), n), src.bound};
   hipStream_t s = ctx_->stream();
   if (!(sigma_mm[0] > 0.0 || sigma_mm[1] > 0.0 || sigma_mm[2] > 0.0)) {
     check(hipMemcpyAsync(out.buf.get(), src.buf.get(), n * sizeof(float), hipMemcpyDeviceToDevice, s), "copy image");
@@ -754,31 +754,17 @@ HImage HipBackend::smooth_image_mm(const HImage& src, const Vec3& sigma_mm) {
     const Deriche k = Deriche::zero_order(sigma_mm[axes[q]], g.spacing[axes[q]]);
     DericheCoefficients c{};
     for (int t = 0; t < 4; ++t) {
-      c.n[t] = float(k.n[t]);
-      c.d[t] = float(k.d[t]);
-      c.m[t] = float(k.m[t]);
-      c.bn[t] = float(k.bn[t]);
-      c.bm[t] = float(k.bm[t]);
+      c.n[t] = k.n[t];
+      c.d[t] = k.d[t];
+      c.m[t] = k.m[t];
+      c.bn[t] = k.bn[t];
+      c.bm[t] = k.bm[t];
     }
     // one read and one write per voxel (the anti-causal pass re-reads its line from cache)
     Timed t(*ctx_, "k_deriche_axis", 8.0 * double(n));
     launch_deriche_axis(in, bufs[q], d, axes[q], c, s);
     in = bufs[q];
   }
-  // the bound of the smoothed image: its own maximum magnitude
-  const int nb = norm_blocks((long long)n);
-  float* pm = static_cast<float*>(ctx_->scratch(2, nb * sizeof(float)));
-  {
-    Timed t(*ctx_, "k_abs_max", 4.0 * double(n));
-    launch_abs_max(out.buf.get(), (long long)n, pm, nb, s);
-  }
-  float* host = static_cast<float*>(ctx_->pinned(nb * sizeof(float)));
-  check(hipMemcpyAsync(host, pm, nb * sizeof(float), hipMemcpyDeviceToHost, s), "abs max partials");
-  ctx_->sync();
-  float m = 0.0f;
-  for (int b = 0; b < nb; ++b)
-    if (std::isfinite(host[b])) m = std::max(m, host[b]);
-  out.absmax = m;
   return out;
 }
 
@@ -870,24 +856,11 @@ HImage HipBackend::preprocess_image(const HImage& raw, float lower, float upper,
   const IntensityWindow w = intensity_window(h, lower, upper, out_min, out_max);
   const float sh = float(w.scale), th_ = float(w.shift);
   const float sl = float(w.scale - double(sh)), tl = float(w.shift - double(th_));
-  HImage out{g, DevBuf(ctx_.get(), size_t(n)), 0.0f};
+  HImage out{g, DevBuf(ctx_.get(), size_t(n)), float(std::max(std::fabs(out_min), std::fabs(out_max)))};
   {
     Timed t(*ctx_, "k_window", 8.0 * double(n));
     launch_window(raw.buf.get(), out.buf.get(), n, float(w.lo), float(w.hi), sh, sl, th_, tl, float(out_min), float(out_max), s);
   }
-  // the bound of the windowed image: its own maximum magnitude, as upload_image takes it
-  auto* pm = static_cast<float*>(ctx_->scratch(2, size_t(nb) * sizeof(float)));
-  {
-    Timed t(*ctx_, "k_abs_max", 4.0 * double(n));
-    launch_abs_max(out.buf.get(), n, pm, nb, s);
-  }
-  auto* mh = static_cast<float*>(ctx_->pinned(size_t(nb) * sizeof(float)));
-  check(hipMemcpyAsync(mh, pm, size_t(nb) * sizeof(float), hipMemcpyDeviceToHost, s), "abs max partials");
-  ctx_->sync();
-  float m = 0.0f;
-  for (int b = 0; b < nb; ++b)
-    if (std::isfinite(mh[b])) m = std::max(m, mh[b]);
-  out.absmax = m;
   return out;
 }
 

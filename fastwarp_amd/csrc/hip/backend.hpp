@@ -208,12 +208,13 @@ class DevBuf {
   size_t bytes_ = 0;
 };
 
-// A scalar image on the device (f32, x-fastest) with an upper bound of |value| (for the
-// fixed-point scale of the CC sums; warps of an image keep its bound).
+// A scalar image on the device (f32, x-fastest) with an upper bound of |value| (taken on the host
+// at upload and carried through the ops that cannot grow it: warps, smoothing, resampling). The
+// CC sums split every value at a power-of-two step of this bound.
 struct HImage {
   Grid grid;
   DevBuf buf;
-  float absmax = 0.0f;
+  float bound = 0.0f;
 };
 
 // A displacement field on the device: three f32 planes x, y, z.

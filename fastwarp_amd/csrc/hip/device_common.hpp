@@ -163,37 +163,6 @@ __device__ __forceinline__ float central_difference(const float* __restrict__ im
   return (tri(img, d, px, py, pz) - tri(img, d, mx, my, mz)) * pr.inv_spacing[a];
 }
 
-// All three central differences of an image at voxel (i, j, k) on a grid whose probes lie on the
-// grid axes (Probes::diagonal): the seven-point stencil is loaded once (neighbours past an edge
-// repeat the centre), and each component follows central_difference's axis-aligned rule.
-__device__ __forceinline__ void diagonal_gradient(const float* __restrict__ img, const Dims& d, const Probes& pr, int i,
-                                                  int j, int k, float out[3]) {
-  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
-  const float* p = img + lin(d, i, j, k);
-  const float c0 = p[0];
-  const int pos[3] = {i, j, k};
-  const int len[3] = {d.nx, d.ny, d.nz};
-  const long long stride[3] = {1, sy, sz};
-  float up[3], down[3];
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    const float hi = p[pos[ax] + 1 < len[ax] ? stride[ax] : 0];
-    const float lo = p[pos[ax] >= 1 ? -stride[ax] : 0];
-    up[ax] = c0 + (hi - c0) * 0.5f;
-    down[ax] = pos[ax] >= 1 ? lo + (c0 - lo) * 0.5f : c0;
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int ax = pr.o[a][0] != 0.0f ? 0 : (pr.o[a][1] != 0.0f ? 1 : 2);
-    const float sgn = pr.o[a][ax];
-    const int q = ax == 0 ? i : (ax == 1 ? j : k);
-    const int n = ax == 0 ? d.nx : (ax == 1 ? d.ny : d.nz);
-    const float u = ax == 0 ? up[0] : (ax == 1 ? up[1] : up[2]);
-    const float w = ax == 0 ? down[0] : (ax == 1 ? down[1] : down[2]);
-    const float diff = sgn > 0.0f ? u - w : w - u;
-    out[a] = q >= n - 1 ? 0.0f : diff * pr.inv_spacing[a];
-  }
-}
 
 __device__ __forceinline__ float wave_sum(float v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -240,25 +209,6 @@ __device__ __forceinline__ int window_count(int pos, int n, int r) {
   return hi - lo + 1;
 }
 
-// Floor division of a window sum by its voxel count: s = q * n + r with 0 <= r < n, for
-// |s| < 2^32 and 1 <= n < 2^12 (so |q| < 2^31). An f32 estimate of q is refined once from its
-// remainder (exact in 64 bits), which leaves it within a step of the floor.
-__device__ __forceinline__ void floor_divmod(long long s, int n, float inv_n, int& q, int& r) {
-  int qe = (int)floorf((float)s * inv_n);
-  long long rem = s - (long long)qe * n;
-  qe += (int)floorf((float)rem * inv_n);
-  rem = s - (long long)qe * n;
-  while (rem < 0) {
-    qe -= 1;
-    rem += n;
-  }
-  while (rem >= n) {
-    qe += 1;
-    rem -= n;
-  }
-  q = qe;
-  r = (int)rem;
-}
 
 }  // namespace
 

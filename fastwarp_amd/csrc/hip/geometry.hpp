@@ -55,12 +55,14 @@ inline Probes probes_of(const Grid& g) {
   return p;
 }
 
-inline float fixed_point_scale(float absmax) {
-  if (!(absmax > 0.0f) || !std::isfinite(absmax)) return float(1 << 22);
-  int e = 0;
-  std::frexp(double(absmax), &e);  // absmax < 2^e
-  return float(std::ldexp(1.0, 22 - e));
-}
 
+// The split scale of the CC sums for an image bounded by `bound`: 2^19 / 2^e with 2^e >= bound,
+// so that |v| * scale <= 2^19 (2^20 if the bound is exceeded up to twice).
+inline float cc_split_scale(float bound) {
+  if (!(bound > 0.0f) || !std::isfinite(bound)) return float(1 << 19);
+  int e = 0;
+  std::frexp(double(bound), &e);  // bound < 2^e
+  return float(std::ldexp(1.0, 19 - e));
+}
 
 }  // namespace fw::hip

@@ -25,36 +25,126 @@ LaunchEvents*& current_launch_events() {
 // ------------------------------------------------------------------------------------------
 // CC pair (metric/cc.rs:170-321 and wgpu_backend.rs:1354-1520 semantics, fused).
 //
-// Both images are quantised to fixed point (|value| * q <= 2^22, one q per image) and every
-// window sum is an exact integer, so the sums carry no rounding at all and do not depend on
-// tiling. Centred moments come from the integer sums by a floor division (s = q n + r), which
-// keeps the cancellation inside exact integer arithmetic. Each output voxel writes both
-// derivative directions; localCC is symmetric, so one value partial serves both.
+// The window sums are those of the f32 voxel values with f64 precision and no rounding history:
+// each value is split at a power-of-two step of its image's bound (v = h q + l, h an integer), the
+// integer parts are summed exactly in f64 and the remainders in f64 beside them, and the two are
+// combined once per window. So a window of zeros sums to zero (as the oracle's direct box sums do,
+// which matters for its degenerate-window rule), and a window of tiny values (the edge of a
+// zero-padded region, where the local variance is 1e-12 and the coefficient is the largest of the
+// image) keeps its relative precision, which a fixed-point quantisation at the bound loses. On
+// gfx950 the vector FP64 rate equals the non-packed FP32 rate. The centred moments, the local CC
+// and the coefficients follow cc.rs:270-300 in f64; the derivative is the coefficient times the
+// f32 central difference, rounded once to f32. Each output voxel writes both derivative
+// directions; localCC is symmetric, so one value partial serves both.
 //
-// The work is organised to keep the integer operations per output low. A block owns a 64 x 8 (x, y) tile over a run of tile_z planes. Each thread keeps the
-// running z-window sums of its columns of the tile and halo in registers: one plane enters and
-// one leaves per step (exact in integers, so a running sum never drifts), read from global
-// memory (the leaving plane was read 2r+1 steps before and sits in cache). The z sums of the
-// step's plane go to LDS; x sums slide along runs of 4 outputs per thread; y sums slide along
-// pairs of rows per thread; each thread then finishes two output voxels exactly as above.
-// First moments stay in 32 bits through the z and x sums (at most 17 x 17 terms of |q| <= 2^22).
+// A block owns a 64 x 8 (x, y) tile over a run of tile_z planes. Each thread keeps the running
+// z-window sums of its columns of the tile and halo in registers: one plane enters and one leaves
+// per step, read from global memory (the leaving plane was read 2r+1 steps before and sits in
+// cache). The z sums of the step's plane go to LDS; the x pass sums pairs of outputs along x
+// (sharing their middle 2r columns, summed directly) and writes its sums back over the z sums after
+// a barrier, so one LDS buffer serves both passes; the y pass sums 2r+1 rows per output voxel.
 // ------------------------------------------------------------------------------------------
-constexpr int kCcTX = 64, kCcTY = 8, kCcThreads = 512, kCcRun = 2;
+constexpr int kCcTX = 64, kCcTY = 8;
+
+// Threads per block: one per output voxel of the 64 x 8 tile. Each thread owns up to
+// ceil((64 + 2R) (8 + 2R) / 512) tile-and-halo columns (three at radius 4); the running sums of a
+// column take 20 registers, which leaves one block of eight waves per CU.
+template <int R>
+constexpr int cc_threads() {
+  return kCcTX * kCcTY;
+}
+
+// Central differences along the three physical axes on an axis-aligned grid (probes at +-0.5 on
+// one grid axis each): the difference of the two linear interpolations is half the difference of
+// the two neighbours (the centre cancels), computed as such so that it carries one rounding.
+__device__ __forceinline__ void axis_gradient(const float* __restrict__ img, const Dims& d, const Probes& pr, int i, int j,
+                                              int k, float out[3]) {
+  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+  const float* p = img + lin(d, i, j, k);
+  const int pos[3] = {i, j, k};
+  const int len[3] = {d.nx, d.ny, d.nz};
+  const long long stride[3] = {1, sy, sz};
+  float half_diff[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    // the lower probe at pos - 0.5 clamps to the centre on the first voxel
+    const float hi = pos[ax] + 1 < len[ax] ? p[stride[ax]] : 0.0f;
+    const float lo = pos[ax] >= 1 ? p[-stride[ax]] : p[0];
+    half_diff[ax] = 0.5f * (hi - lo);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int ax = pr.o[a][0] != 0.0f ? 0 : (pr.o[a][1] != 0.0f ? 1 : 2);
+    const float sgn = pr.o[a][ax];
+    const float h = ax == 0 ? half_diff[0] : (ax == 1 ? half_diff[1] : half_diff[2]);
+    // the upper probe of the last voxel lies at n - 0.5, outside the buffer
+    out[a] = pos[ax] >= len[ax] - 1 ? 0.0f : (sgn > 0.0f ? h : -h) * pr.inv_spacing[a];
+  }
+}
+
+__device__ float block_sum_f64(double v) {
+  __shared__ double red[16];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
+  return (float)t;
+}
+
+// One voxel's values split at the power-of-two step q = 1 / scale: v = h q + l with h an integer
+// (|h| <= 2^20 for |v| <= twice the bound) and l exact in f32 (|l| <= q / 2).
+struct CcSplit {
+  float h, l;
+};
+__device__ __forceinline__ CcSplit cc_split(float v, float scale, float step) {
+  const float h = rintf(v * scale);
+  return {h, v - h * step};
+}
+
+// The running sums of one column over the window's planes, for both images: the integer parts
+// (sums of h, h^2 and h_a h_b, exact in f64: at most 17^3 terms below 2^40) and the corrections of
+// the remainders (sums of l, l (2 h q + l) and h_a q l_b + l_a h_b q + l_a l_b), whose rounding is
+// relative to their own size (about q). A plane enters and one leaves per step; the integer parts
+// return exactly to zero with the values, so a window of zeros has zero sums up to the rounding of
+// the corrections (below 1e-19 for a bound of 1).
+struct CcColumn {
+  double e1a, e1b, e2a, e2b, e2ab;
+  double c1a, c1b, c2a, c2b, c2ab;
+};
+
+__device__ __forceinline__ void cc_column_step(CcColumn& col, const CcSplit& ea, const CcSplit& eb, const CcSplit& la,
+                                               const CcSplit& lb, float qa, float qb) {
+  const double hae = ea.h, hbe = eb.h, hal = la.h, hbl = lb.h;
+  col.e1a += hae - hal;
+  col.e1b += hbe - hbl;
+  col.e2a += fma(hae, hae, -hal * hal);
+  col.e2b += fma(hbe, hbe, -hbl * hbl);
+  col.e2ab += fma(hae, hbe, -hal * hbl);
+  const double lae = ea.l, lbe = eb.l, lal = la.l, lbl = lb.l;
+  const double Hae = hae * (double)qa, Hbe = hbe * (double)qb, Hal = hal * (double)qa, Hbl = hbl * (double)qb;
+  col.c1a += lae - lal;
+  col.c1b += lbe - lbl;
+  col.c2a += lae * (2.0 * Hae + lae) - lal * (2.0 * Hal + lal);
+  col.c2b += lbe * (2.0 * Hbe + lbe) - lbl * (2.0 * Hbl + lbl);
+  col.c2ab += fma(Hae, lbe, lae * (Hbe + lbe)) - fma(Hal, lbl, lal * (Hbl + lbl));
+}
 
 template <int R, bool MASKED, bool DIAGONAL>
-__global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
+__global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
+  constexpr int kCcThreads = cc_threads<R>();
   constexpr int W = 2 * R + 1;
   constexpr int LX = kCcTX + 2 * R, LY = kCcTY + 2 * R;
   constexpr int NC = LX * LY;
   constexpr int KC = (NC + kCcThreads - 1) / kCcThreads;  // columns per thread
-  // z sums of the current plane, by column
-  __shared__ int za[LY][LX], zb[LY][LX];
-  __shared__ long long zaa[LY][LX], zbb[LY][LX], zab[LY][LX];
+  constexpr int RUNS = LY * (kCcTX / 2);                   // x pass: pairs of outputs per plane
+  constexpr int RPT = (RUNS + kCcThreads - 1) / kCcThreads;
+  // z sums of the current plane by column, then (in place) the x sums by output column
+  __shared__ double za[LY][LX], zb[LY][LX], zaa[LY][LX], zbb[LY][LX], zab[LY][LX];
   __shared__ int zv[MASKED ? LY : 1][MASKED ? LX : 1];
-  // x sums over the window, by row of the tile and halo rows
-  __shared__ int xa[LY][kCcTX], xb[LY][kCcTX];
-  __shared__ long long xaa[LY][kCcTX], xbb[LY][kCcTX], xab[LY][kCcTX];
-  __shared__ int xv[MASKED ? LY : 1][MASKED ? kCcTX : 1];
 
   const Dims d = a.d;
   const int tid = threadIdx.x;
@@ -63,11 +153,13 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
   const int z1 = min(z0 + a.tile_z, d.nz);
   const long long plane = (long long)d.nx * d.ny;
   const long long n3 = d.n();
+  const float sa_ = a.scale_f, sb_ = a.scale_m, qa = a.step_f, qb = a.step_m;
+  const double qa2 = (double)qa * qa, qb2 = (double)qb * qb, qab = (double)qa * qb;
 
   // the columns this thread owns: plane offset, or -1 outside the image
   long long off[KC];
-  int sa[KC], sb[KC], sv[KC];
-  long long saa[KC], sbb[KC], sab[KC];
+  CcColumn col[KC];
+  int sv[KC];
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int c = tid + kCcThreads * k;
@@ -76,40 +168,40 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
       const int gx = x0 - R + c % LX, gy = y0 - R + c / LX;
       if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) off[k] = (long long)gx + (long long)d.nx * gy;
     }
-    sa[k] = sb[k] = sv[k] = 0;
-    saa[k] = sbb[k] = sab[k] = 0;
+    col[k] = CcColumn{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    sv[k] = 0;
   }
-  // quantised values of one voxel (zero outside the image, the mask, or the z range)
-  auto load = [&](long long o, int z, int& qa, int& qb, int& qv) {
-    qa = qb = qv = 0;
+  // the split values of one voxel (zero outside the image, the mask, or the z range)
+  auto load = [&](long long o, int z, CcSplit& va, CcSplit& vb, int& vv) {
+    va = vb = CcSplit{0.0f, 0.0f};
+    vv = 0;
     if (o < 0 || z < 0 || z >= d.nz) return;
     const long long g = o + plane * z;
     if (MASKED && a.valid[g] == 0.0f) return;
-    qa = __float2int_rn(a.fixed[g] * a.qf);
-    qb = __float2int_rn(a.moving[g] * a.qm);
-    qv = 1;
+    va = cc_split(a.fixed[g], sa_, qa);
+    vb = cc_split(a.moving[g], sb_, qb);
+    vv = 1;
   };
   // warm-up: the window of plane z0
+  const CcSplit none{0.0f, 0.0f};
   for (int z = z0 - R; z <= z0 + R; ++z) {
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
-      int qa, qb, qv;
-      load(off[k], z, qa, qb, qv);
-      sa[k] += qa;
-      sb[k] += qb;
-      sv[k] += qv;
-      saa[k] += (long long)qa * qa;
-      sbb[k] += (long long)qb * qb;
-      sab[k] += (long long)qa * qb;
+      CcSplit va, vb;
+      int vv;
+      load(off[k], z, va, vb, vv);
+      cc_column_step(col[k], va, vb, none, none, qa, qb);
+      sv[k] += vv;
     }
   }
 
-  const int tx = tid & 63, ty = tid >> 6;  // y pass and output: column tx, row ty
-  float cc_sum = 0.0f, cc_comp = 0.0f;
+  const int tx = tid & 63, ty = tid >> 6;  // y pass and output: column tx, row ty (ty < 8)
+  double cc_sum = 0.0;
   int counted = 0;
   for (int z = z0; z < z1; ++z) {
     // the values entering and leaving the window for the next plane, fetched early
-    int ea[KC], eb[KC], ev[KC], la[KC], lb[KC], lv[KC];
+    CcSplit ea[KC], eb[KC], la[KC], lb[KC];
+    int ev[KC], lv[KC];
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       load(off[k], z + R + 1, ea[k], eb[k], ev[k]);
@@ -120,62 +212,83 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
       const int c = tid + kCcThreads * k;
       if (c < NC) {
         const int ly = c / LX, lx = c % LX;
-        za[ly][lx] = sa[k];
-        zb[ly][lx] = sb[k];
-        zaa[ly][lx] = saa[k];
-        zbb[ly][lx] = sbb[k];
-        zab[ly][lx] = sab[k];
+        const CcColumn& s = col[k];
+        za[ly][lx] = fma(s.e1a, (double)qa, s.c1a);
+        zb[ly][lx] = fma(s.e1b, (double)qb, s.c1b);
+        zaa[ly][lx] = fma(s.e2a, qa2, s.c2a);
+        zbb[ly][lx] = fma(s.e2b, qb2, s.c2b);
+        zab[ly][lx] = fma(s.e2ab, qab, s.c2ab);
         if constexpr (MASKED) zv[ly][lx] = sv[k];
       }
     }
     __syncthreads();
-    // x pass: runs of kCcRun outputs along x
-    for (int e = tid; e < LY * (kCcTX / kCcRun); e += kCcThreads) {
-      const int ly = e / (kCcTX / kCcRun), lx = (e % (kCcTX / kCcRun)) * kCcRun;
-      int s1 = 0, s2 = 0, s3 = 0;
-      long long s11 = 0, s22 = 0, s12 = 0;
-#pragma unroll 3
-      for (int t = 0; t < W; ++t) {
-        s1 += za[ly][lx + t];
-        s2 += zb[ly][lx + t];
-        s11 += zaa[ly][lx + t];
-        s22 += zbb[ly][lx + t];
-        s12 += zab[ly][lx + t];
-        if constexpr (MASKED) s3 += zv[ly][lx + t];
-      }
-#pragma unroll 1
-      for (int r = 0; r < kCcRun; ++r) {
-        if (r > 0) {
-          const int in = lx + r + W - 1, out = lx + r - 1;
-          s1 += za[ly][in] - za[ly][out];
-          s2 += zb[ly][in] - zb[ly][out];
-          s11 += zaa[ly][in] - zaa[ly][out];
-          s22 += zbb[ly][in] - zbb[ly][out];
-          s12 += zab[ly][in] - zab[ly][out];
-          if constexpr (MASKED) s3 += zv[ly][in] - zv[ly][out];
+    // x pass: pairs of outputs along x share their middle 2r columns (summed directly, no running
+    // sum); held in registers until every pair has read its inputs, then written over the z sums
+    double xs[RPT][2][5];
+    int xv[RPT][2];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int e = tid + kCcThreads * q;
+      if (RUNS % kCcThreads == 0 || e < RUNS) {
+        const int ly = e / (kCcTX / 2), lx = (e % (kCcTX / 2)) * 2;
+        double m1 = 0.0, m2 = 0.0, m11 = 0.0, m22 = 0.0, m12 = 0.0;
+        int m3 = 0;
+#pragma unroll
+        for (int t = 1; t < W; ++t) {
+          m1 += za[ly][lx + t];
+          m2 += zb[ly][lx + t];
+          m11 += zaa[ly][lx + t];
+          m22 += zbb[ly][lx + t];
+          m12 += zab[ly][lx + t];
+          if constexpr (MASKED) m3 += zv[ly][lx + t];
         }
-        xa[ly][lx + r] = s1;
-        xb[ly][lx + r] = s2;
-        xaa[ly][lx + r] = s11;
-        xbb[ly][lx + r] = s22;
-        xab[ly][lx + r] = s12;
-        if constexpr (MASKED) xv[ly][lx + r] = s3;
+        xs[q][0][0] = za[ly][lx] + m1;
+        xs[q][0][1] = zb[ly][lx] + m2;
+        xs[q][0][2] = zaa[ly][lx] + m11;
+        xs[q][0][3] = zbb[ly][lx] + m22;
+        xs[q][0][4] = zab[ly][lx] + m12;
+        xs[q][1][0] = m1 + za[ly][lx + W];
+        xs[q][1][1] = m2 + zb[ly][lx + W];
+        xs[q][1][2] = m11 + zaa[ly][lx + W];
+        xs[q][1][3] = m22 + zbb[ly][lx + W];
+        xs[q][1][4] = m12 + zab[ly][lx + W];
+        if constexpr (MASKED) {
+          xv[q][0] = zv[ly][lx] + m3;
+          xv[q][1] = m3 + zv[ly][lx + W];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int e = tid + kCcThreads * q;
+      if (RUNS % kCcThreads == 0 || e < RUNS) {
+        const int ly = e / (kCcTX / 2), lx = (e % (kCcTX / 2)) * 2;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          za[ly][lx + r] = xs[q][r][0];
+          zb[ly][lx + r] = xs[q][r][1];
+          zaa[ly][lx + r] = xs[q][r][2];
+          zbb[ly][lx + r] = xs[q][r][3];
+          zab[ly][lx + r] = xs[q][r][4];
+          if constexpr (MASKED) zv[ly][lx + r] = xv[q][r];
+        }
       }
     }
     __syncthreads();
     // y pass: row ty of column tx
-    long long wa = 0, wb = 0, waa = 0, wbb = 0, wab = 0;
-    int wv = 0;
+    if (ty < kCcTY) {
+      double wa = 0.0, wb = 0.0, waa = 0.0, wbb = 0.0, wab = 0.0;
+      int wv = 0;
 #pragma unroll 3
-    for (int t = 0; t < W; ++t) {
-      wa += xa[ty + t][tx];
-      wb += xb[ty + t][tx];
-      waa += xaa[ty + t][tx];
-      wbb += xbb[ty + t][tx];
-      wab += xab[ty + t][tx];
-      if constexpr (MASKED) wv += xv[ty + t][tx];
-    }
-    {
+      for (int t = 0; t < W; ++t) {
+        wa += za[ty + t][tx];
+        wb += zb[ty + t][tx];
+        waa += zaa[ty + t][tx];
+        wbb += zbb[ty + t][tx];
+        wab += zab[ty + t][tx];
+        if constexpr (MASKED) wv += zv[ty + t][tx];
+      }
       const int x = x0 + tx, y = y0 + ty;
       if (x < d.nx && y < d.ny) {
         const long long g = (long long)x + (long long)d.nx * y + plane * z;
@@ -184,38 +297,26 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
         if (centre_valid) {
           // the window's voxel count (at most (2R + 1)^3)
           const int n = MASKED ? wv : window_count(x, d.nx, R) * window_count(y, d.ny, R) * window_count(z, d.nz, R);
-          const float inv_n = 1.0f / (float)n;
-          // means q + r / n (|q| <= 2^22, 0 <= r < n); the centred sums exactly in 64 bits:
-          // sum (a - q)^2 - ... = waa - q (n q + 2 r), and the cross term alike
-          int qa_, ra_, qb_, rb_;
-          floor_divmod(wa, n, inv_n, qa_, ra_);
-          floor_divmod(wb, n, inv_n, qb_, rb_);
-          const long long eaa = waa - (long long)qa_ * ((long long)n * qa_ + 2 * ra_);
-          const long long ebb = wbb - (long long)qb_ * ((long long)n * qb_ + 2 * rb_);
-          const long long eab = wab - (long long)qa_ * ((long long)n * qb_ + rb_) - (long long)qb_ * ra_;
-          const float s_ff = ((float)eaa - (float)(ra_ * ra_) * inv_n) * (a.inv_qf * a.inv_qf);
-          const float s_mm = ((float)ebb - (float)(rb_ * rb_) * inv_n) * (a.inv_qm * a.inv_qm);
-          const float s_fm = ((float)eab - (float)(ra_ * rb_) * inv_n) * (a.inv_qf * a.inv_qm);
-          const float fmean = ((float)qa_ + (float)ra_ * inv_n) * a.inv_qf;
-          const float mmean = ((float)qb_ + (float)rb_ * inv_n) * a.inv_qm;
-          const float product = s_ff * s_mm;
-          float cc = 1.0f;
-          if (fabsf(product) > kEps64) cc = s_fm * s_fm / product;
-          const float yk = cc - cc_comp;
-          const float tk = cc_sum + yk;
-          cc_comp = (tk - cc_sum) - yk;
-          cc_sum = tk;
+          const double count = (double)n;
+          const double fmean = wa / count, mmean = wb / count;
+          const double s_ff = waa - fmean * wa - fmean * wa + count * fmean * fmean;
+          const double s_mm = wbb - mmean * wb - mmean * wb + count * mmean * mmean;
+          const double s_fm = wab - mmean * wa - fmean * wb + count * mmean * fmean;
+          const double product = s_ff * s_mm;
+          double cc = 1.0;
+          if (fabs(product) > (double)kEps64) cc = s_fm * s_fm / product;
+          cc_sum += cc;
           counted += 1;
-          if (s_ff > kEps64 && s_mm > kEps64) {
-            const float fi = a.fixed[g] - fmean;
-            const float mi = a.moving[g] - mmean;
-            const float k1 = 2.0f * s_fm / product;
-            const float cf = k1 * (fi - s_fm / s_mm * mi);
-            const float cm = k1 * (mi - s_fm / s_ff * fi);
+          if (s_ff > (double)kEps64 && s_mm > (double)kEps64) {
+            const double fi = (double)a.fixed[g] - fmean;
+            const double mi = (double)a.moving[g] - mmean;
+            const double k1 = 2.0 * s_fm / product;
+            const double cf = k1 * (fi - s_fm / s_mm * mi);
+            const double cm = k1 * (mi - s_fm / s_ff * fi);
             float gm[3], gf[3];
             if constexpr (DIAGONAL) {
-              diagonal_gradient(a.moving, d, a.probes, x, y, z, gm);
-              diagonal_gradient(a.fixed, d, a.probes, x, y, z, gf);
+              axis_gradient(a.moving, d, a.probes, x, y, z, gm);
+              axis_gradient(a.fixed, d, a.probes, x, y, z, gf);
             } else {
 #pragma unroll 1
               for (int q = 0; q < 3; ++q) {
@@ -223,12 +324,12 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
                 gf[q] = central_difference(a.fixed, d, a.probes, q, x, y, z);
               }
             }
-            dfx = cf * gm[0];
-            dfy = cf * gm[1];
-            dfz = cf * gm[2];
-            dmx = cm * gf[0];
-            dmy = cm * gf[1];
-            dmz = cm * gf[2];
+            dfx = (float)(cf * (double)gm[0]);
+            dfy = (float)(cf * (double)gm[1]);
+            dfz = (float)(cf * (double)gm[2]);
+            dmx = (float)(cm * (double)gf[0]);
+            dmy = (float)(cm * (double)gf[1]);
+            dmz = (float)(cm * (double)gf[2]);
           }
         }
         a.d_fixed[g] = dfx;
@@ -242,16 +343,12 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
     // slide every owned column's window to the next plane
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
-      sa[k] += ea[k] - la[k];
-      sb[k] += eb[k] - lb[k];
+      cc_column_step(col[k], ea[k], eb[k], la[k], lb[k], qa, qb);
       sv[k] += ev[k] - lv[k];
-      saa[k] += (long long)ea[k] * ea[k] - (long long)la[k] * la[k];
-      sbb[k] += (long long)eb[k] * eb[k] - (long long)lb[k] * lb[k];
-      sab[k] += (long long)ea[k] * eb[k] - (long long)la[k] * lb[k];
     }
     __syncthreads();
   }
-  const float total = block_sum(cc_sum);
+  const float total = block_sum_f64(cc_sum);
   const float cnt = block_sum((float)counted);
   if (tid == 0) {
     const int b = bl.x + gridDim.x * (bl.y + gridDim.y * bl.z);
@@ -262,8 +359,8 @@ __global__ __launch_bounds__(kCcThreads) void k_cc2(CcArgs a) {
 
 template <int R, bool M>
 static void launch_cc_r(const CcArgs& a, hipStream_t s) {
-  if (a.probes.diagonal) launch((k_cc2<R, M, true>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
-  else launch((k_cc2<R, M, false>), cc_grid(a.d, a.tile_z), dim3(kCcThreads), 0, s, a);
+  if (a.probes.diagonal) launch((k_cc2<R, M, true>), cc_grid(a.d, a.tile_z), dim3(cc_threads<R>()), 0, s, a);
+  else launch((k_cc2<R, M, false>), cc_grid(a.d, a.tile_z), dim3(cc_threads<R>()), 0, s, a);
 }
 
 dim3 cc_grid(const Dims& d, int tile_z) {
@@ -380,17 +477,6 @@ __global__ __launch_bounds__(256) void k_scale_by2(float* __restrict__ x, float*
   for (long long i = b * blockDim.x + threadIdx.x; i < count; i += (long long)half * blockDim.x) v[i] *= f;
 }
 
-__global__ __launch_bounds__(256) void k_abs_max(const float* __restrict__ x, long long n, float* partial) {
-  float m = 0.0f;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    m = nan_max(m, fabsf(x[i]));
-  const float r = block_max(m);
-  if (threadIdx.x == 0) partial[blockIdx.x] = r;
-}
-
-void launch_abs_max(const float* x, long long n, float* partial, int blocks, hipStream_t s) {
-  launch(k_abs_max, dim3(blocks), dim3(256), 0, s, x, n, partial);
-}
 
 // Finite minimum and maximum of x per block; a block without a finite value reports +inf / -inf.
 __global__ __launch_bounds__(256) void k_finite_range(const float* __restrict__ x, long long n, float* partial_min,
@@ -1315,54 +1401,53 @@ __global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ 
   if (line >= lines) return;
   const float* p = in + base;
   float* q = out + base;
-  // The filter has unit DC gain, so it runs on x - x[0] and x[0] is added back: the recursion then
-  // carries only the line's variation, which keeps the f32 rounding of its coefficients (whose
-  // DC gain is ill-conditioned at large sigma) from scaling with the mean intensity.
-  const float shift = p[0];
+  // The recursions run in f64 (the vector FP64 rate of gfx950 is that of FP32): in f32 their
+  // rounding grows with sigma as the poles approach 1. The causal part is stored in f32 and the
+  // anti-causal part added to it, so the result carries two f32 roundings.
   // causal: y[i] = sum n_k x[i-k] - sum d_k y[i-1-k], with x and y extended by x[0] at the start
   {
-    const float v1 = 0.0f;
-    float x0 = v1, x1 = v1, x2 = v1, x3 = v1;  // x[i], x[i-1], x[i-2], x[i-3]
-    float y1 = 0.0f, y2 = 0.0f, y3 = 0.0f, y4 = 0.0f;
+    const double v1 = p[0];
+    double x0 = v1, x1 = v1, x2 = v1, x3 = v1;  // x[i], x[i-1], x[i-2], x[i-3]
+    double y1 = 0.0, y2 = 0.0, y3 = 0.0, y4 = 0.0;
     for (int i = 0; i < len; ++i) {
       x3 = x2;
       x2 = x1;
       x1 = x0;
-      x0 = p[(long long)i * stride] - shift;
-      float y = x0 * c.n[0] + x1 * c.n[1] + x2 * c.n[2] + x3 * c.n[3];
+      x0 = p[(long long)i * stride];
+      double y = x0 * c.n[0] + x1 * c.n[1] + x2 * c.n[2] + x3 * c.n[3];
       // outputs before the start of the line are replaced by the boundary terms v1 * bn_k
-      const float sub = (i >= 1 ? y1 * c.d[0] : v1 * c.bn[0]) + (i >= 2 ? y2 * c.d[1] : v1 * c.bn[1]) +
-                        (i >= 3 ? y3 * c.d[2] : v1 * c.bn[2]) + (i >= 4 ? y4 * c.d[3] : v1 * c.bn[3]);
+      const double sub = (i >= 1 ? y1 * c.d[0] : v1 * c.bn[0]) + (i >= 2 ? y2 * c.d[1] : v1 * c.bn[1]) +
+                         (i >= 3 ? y3 * c.d[2] : v1 * c.bn[2]) + (i >= 4 ? y4 * c.d[3] : v1 * c.bn[3]);
       y -= sub;
       y4 = y3;
       y3 = y2;
       y2 = y1;
       y1 = y;
-      q[(long long)i * stride] = y;
+      q[(long long)i * stride] = (float)y;
     }
   }
   // anti-causal: a[i-1] = sum m_k x[i+k] - sum d_k a[i+k], with x extended by x[len-1] at the end
   {
-    const float v2 = p[(long long)(len - 1) * stride] - shift;
-    float x0 = v2, x1 = v2, x2 = v2, x3 = v2;  // x[i], x[i+1], x[i+2], x[i+3] for a[i-1]
-    float a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;  // a[i], a[i+1], a[i+2], a[i+3]
+    const double v2 = p[(long long)(len - 1) * stride];
+    double x0 = v2, x1 = v2, x2 = v2, x3 = v2;  // x[i], x[i+1], x[i+2], x[i+3] for a[i-1]
+    double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;  // a[i], a[i+1], a[i+2], a[i+3]
     for (int t = 0; t < len; ++t) {
       const int i = len - t;  // this step produces a[i-1]
       // x[i..i+3]: x[i] is the next input inward, past the end it is v2
       x3 = x2;
       x2 = x1;
       x1 = x0;
-      x0 = i <= len - 1 ? p[(long long)i * stride] - shift : v2;
-      float y = x0 * c.m[0] + x1 * c.m[1] + x2 * c.m[2] + x3 * c.m[3];
-      const float sub = (t >= 1 ? a1 * c.d[0] : v2 * c.bm[0]) + (t >= 2 ? a2 * c.d[1] : v2 * c.bm[1]) +
-                        (t >= 3 ? a3 * c.d[2] : v2 * c.bm[2]) + (t >= 4 ? a4 * c.d[3] : v2 * c.bm[3]);
+      x0 = i <= len - 1 ? (double)p[(long long)i * stride] : v2;
+      double y = x0 * c.m[0] + x1 * c.m[1] + x2 * c.m[2] + x3 * c.m[3];
+      const double sub = (t >= 1 ? a1 * c.d[0] : v2 * c.bm[0]) + (t >= 2 ? a2 * c.d[1] : v2 * c.bm[1]) +
+                         (t >= 3 ? a3 * c.d[2] : v2 * c.bm[2]) + (t >= 4 ? a4 * c.d[3] : v2 * c.bm[3]);
       y -= sub;
       a4 = a3;
       a3 = a2;
       a2 = a1;
       a1 = y;
       const long long o = (long long)(i - 1) * stride;
-      q[o] = (q[o] + y) + shift;
+      q[o] = (float)((double)q[o] + y);
     }
   }
 }

@@ -61,8 +61,8 @@ struct CcArgs {
   Dims d;
   int radius;
   int tile_z;
-  float qf, qm;          // fixed-point scales of the two images
-  float inv_qf, inv_qm;
+  float scale_f, scale_m;  // 2^19 / (the image's bound rounded up to a power of two)
+  float step_f, step_m;    // 1 / scale
   Probes probes;
   float* d_fixed;        // 3 planes: coefficient * grad(moving)
   float* d_moving;       // 3 planes: coefficient' * grad(fixed)
@@ -77,8 +77,6 @@ dim3 cc_grid(const Dims& d, int tile_z);
 void launch_field_norms(const float* a, const float* b, long long n, float3 inv_spacing, float* partial_max,
                         float* partial_sq, int blocks, hipStream_t s);
 int norm_blocks(long long n);
-// In-place scaled max of image magnitudes (for fixed-point scales): out[block] = max |x|.
-void launch_abs_max(const float* x, long long n, float* partial, int blocks, hipStream_t s);
 // Per-block finite minimum / maximum (+inf / -inf for a block without finite values).
 void launch_finite_range(const float* x, long long n, float* partial_min, float* partial_max, int blocks, hipStream_t s);
 // Nearest-voxel and linear values at m continuous indices (x, y, z; w unused); NaN outside.
@@ -201,7 +199,7 @@ int inv_blocks(const Dims& d);
 
 // --- recursive Gaussian (Deriche, 4th order) along one axis, constant extension at both ends ---
 struct DericheCoefficients {
-  float n[4], d[4], m[4], bn[4], bm[4];
+  double n[4], d[4], m[4], bn[4], bm[4];
 };
 // `in` and `out` must differ; every line needs at least 4 voxels.
 void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s);

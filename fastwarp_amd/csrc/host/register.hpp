@@ -12,11 +12,16 @@
 
 namespace fw {
 
+// stages.rs:74-86 `InitialTransform`: where the chain starts.
+enum class InitialKind { CentreOfMass, Identity, Affine };
+
 struct RegisterPlan {
   LinearRecipe rigid, affine;
   SynRecipe syn;
   float winsorize_lower = 0.005f, winsorize_upper = 0.995f;  // register.rs WINSORIZE_LOWER/UPPER
   bool run_linear = true, run_syn = true;
+  InitialKind initial_kind = InitialKind::CentreOfMass;
+  AffineTransform initial_affine;  // InitialKind::Affine
 
   RegisterPlan() {
     rigid.kind = LinearKind::Rigid;
@@ -69,12 +74,15 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
   for (const SynChannel& c : channels) pre.push_back({&c.fixed, &c.moving});
   SmoothCache<B> cache(backend, std::move(pre));
   t = clock::now();
+  const AffineTransform start = plan.initial_kind == InitialKind::CentreOfMass ? initial_transform_com(fixed[0], moving[0])
+                                : plan.initial_kind == InitialKind::Identity   ? AffineTransform{}
+                                                                                : plan.initial_affine;
   if (plan.run_linear) {
-    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, fixed[0], moving[0], cache, cancel);
+    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, start, cache, cancel);
     out.initial = out.linear.initial;
     out.total = out.linear.total;
   } else {
-    out.initial = out.total = initial_transform_com(fixed[0], moving[0]);
+    out.initial = out.total = start;
   }
   out.seconds_linear = secs(t);
   if (plan.run_syn) {
@@ -115,14 +123,19 @@ RegisterOutput register_resident(B& backend, E& engine, const RegisterPlan& plan
   out.seconds_preprocess = secs(t);
   SmoothCache<B> cache(backend, std::move(resident));
   t = clock::now();
-  const Vec3 fc = backend.centre_of_gravity(*fixed[0]), mc = backend.centre_of_gravity(*moving[0]);
-  const AffineTransform com{kIdentity3, vsub(mc, fc), fc};
+  AffineTransform start = plan.initial_affine;
+  if (plan.initial_kind == InitialKind::CentreOfMass) {
+    const Vec3 fc = backend.centre_of_gravity(*fixed[0]), mc = backend.centre_of_gravity(*moving[0]);
+    start = AffineTransform{kIdentity3, vsub(mc, fc), fc};
+  } else if (plan.initial_kind == InitialKind::Identity) {
+    start = AffineTransform{};
+  }
   if (plan.run_linear) {
-    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, com, cache, cancel);
+    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, start, cache, cancel);
     out.initial = out.linear.initial;
     out.total = out.linear.total;
   } else {
-    out.initial = out.total = com;
+    out.initial = out.total = start;
   }
   out.seconds_linear = secs(t);
   if (plan.run_syn) {

@@ -74,10 +74,83 @@ inline RegisterPlan plan_from(const fw_plan* p) {
   r.winsorize_upper = float(p->winsorize_upper);
   r.run_linear = p->run_linear != 0;
   r.run_syn = p->run_syn != 0;
+  // stages.rs:139-155: a given start must be finite and invertible
+  switch (p->initial_kind) {
+    case 0: r.initial_kind = InitialKind::CentreOfMass; break;
+    case 1: r.initial_kind = InitialKind::Identity; break;
+    case 2: {
+      r.initial_kind = InitialKind::Affine;
+      AffineTransform& t = r.initial_affine;
+      bool finite = true;
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) t.matrix[i][j] = p->initial_affine.matrix[3 * i + j];
+        t.translation[i] = p->initial_affine.translation[i];
+        t.center[i] = p->initial_affine.center[i];
+      }
+      for (int i = 0; i < 3; ++i) {
+        finite = finite && std::isfinite(t.translation[i]) && std::isfinite(t.center[i]);
+        for (int j = 0; j < 3; ++j) finite = finite && std::isfinite(t.matrix[i][j]);
+      }
+      if (!finite) invalid("stage plan", "the initial transform has non-finite values");
+      Mat3 inv;
+      if (!mat3_inverse(t.matrix, inv)) invalid("stage plan", "the initial transform is singular");
+      break;
+    }
+    default: invalid("stage plan", "initial_kind must be 0 (centre of mass), 1 (identity) or 2 (given affine)");
+  }
   return r;
 }
 
+// The per-iteration records of the calling thread's last registration (fw_last_trace).
+inline std::vector<fw_trace_step>& last_trace() {
+  static thread_local std::vector<fw_trace_step> t;
+  return t;
+}
+
+inline void trace_to(const RegisterOutput& o, std::vector<fw_trace_step>& t) {
+  t.clear();
+  const LinearStageResult* st[2] = {&o.linear.rigid, &o.linear.affine};
+  for (int s = 0; s < 2; ++s)
+    for (size_t l = 0; l < st[s]->levels.size(); ++l) {
+      const auto& values = st[s]->levels[l].cg.values;
+      for (size_t i = 0; i < values.size(); ++i) {
+        fw_trace_step r{};
+        r.stage = s;
+        r.level = int32_t(l);
+        r.iteration = int32_t(i);
+        r.value = values[i];
+        t.push_back(r);
+      }
+    }
+  for (size_t l = 0; l < o.syn.levels.size(); ++l)
+    for (size_t i = 0; i < o.syn.levels[l].iterations.size(); ++i) {
+      const SynIteration& it = o.syn.levels[l].iterations[i];
+      fw_trace_step r{};
+      r.stage = 2;
+      r.level = int32_t(l);
+      r.iteration = int32_t(i);
+      r.value = it.energy;
+      r.convergence_value = it.convergence_value;
+      r.update_scale[0] = it.fixed_update_scale;
+      r.update_scale[1] = it.moving_update_scale;
+      for (int q = 0; q < 4; ++q) {
+        r.inversion_steps[q] = int32_t(it.inversions[q].iterations);
+        r.inversion_mean_error[q] = it.inversions[q].mean_error_norm;
+        r.inversion_max_error[q] = it.inversions[q].max_error_norm;
+      }
+      t.push_back(r);
+    }
+}
+
+inline int32_t copy_last_trace(fw_trace_step* out, int32_t capacity) {
+  const auto& t = last_trace();
+  const size_t n = out && capacity > 0 ? std::min(t.size(), size_t(capacity)) : 0;
+  for (size_t i = 0; i < n; ++i) out[i] = t[i];
+  return int32_t(t.size());
+}
+
 inline void result_to(const RegisterOutput& o, fw_result* r) {
+  trace_to(o, last_trace());
   if (!r) return;
   *r = fw_result{};
   for (int i = 0; i < 3; ++i) {

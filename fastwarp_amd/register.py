@@ -8,7 +8,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._abi import Affine, Grid, Plan, Result, dptr
+from ._abi import Affine, Grid, Plan, Result, TraceStep, dptr, read_trace
 from .backend import FW_OK, DeviceField, DeviceImage, HipBackend, RegError, lib
 
 
@@ -89,3 +89,9 @@ def register_resident(backend: HipBackend, plan: Plan, fixed: list[DeviceImage],
     wrap = lambda p: DeviceField(backend, p) if p.value else None
     return ResidentRegistration(res.affine, wrap(fwd) if want else None, wrap(inv) if want else None, res)
 
+
+
+def last_trace() -> list[TraceStep]:
+    """Per-iteration records (CG values, SyN energies and inversions) of this thread's last
+    register / register_resident call."""
+    return read_trace(lib().fw_last_trace)

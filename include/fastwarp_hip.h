@@ -93,6 +93,8 @@ typedef struct {
 
 /* ---- context (WgpuBackend::new, release_idle_buffers backend.rs:50) ---- */
 int fw_ctx_create(int device, fw_ctx** out);
+/* Frees the context's device memory. Every image, field and MI level made on it must be freed
+ * before (their buffers return to its pool); a handle freed after this call is undefined. */
 void fw_ctx_destroy(fw_ctx* ctx);
 const char* fw_last_error(const fw_ctx* ctx);
 /* Backend::release_idle_buffers (backend.rs:48-50). */
@@ -231,8 +233,14 @@ typedef struct {
   int32_t syn_convergence_window;
 
   double winsorize_lower, winsorize_upper;
-  int32_t run_linear; /* 0: identity affine start, SyN only */
+  int32_t run_linear; /* 0: no linear stage, SyN starts from the initial transform */
   int32_t run_syn;    /* 0: linear stages only */
+  /* Where the chain starts (the reference's StagePlan::initial, crates/fastwarp-reg/src/stages.rs:74-86,
+   * ANTs' --initial-moving-transform): 0 the translation between the centres of intensity mass of
+   * the raw first channel, 1 the identity centred at the origin, 2 initial_affine (fixed points to
+   * moving points; finite and invertible, the first linear stage is centred at its centre). */
+  int32_t initial_kind;
+  fw_affine initial_affine;
 } fw_plan;
 
 /* Summary of one registration: the collapsed affine, per-phase seconds and per-level counts. */
@@ -255,6 +263,26 @@ typedef struct {
   int64_t linear_evaluations[2][8]; /* metric evaluations, line searches included */
   int64_t syn_inversion_steps[8];    /* executed fixed-point steps of the level's four inversions per iteration, summed */
 } fw_result;
+
+/* One iteration of the last registration run on the calling thread: a CG iteration of a linear
+ * level (stage 0 rigid, 1 affine; value = the metric value of the iteration, as the reference's
+ * CgTrace::values, optimizer.rs:324-430), or a SyN iteration (stage 2; value = the energy, with
+ * the convergence value, the two update scales and the four inversions' step counts and errors of
+ * syn.rs:96-128 `SynIteration`). */
+typedef struct {
+  int32_t stage, level, iteration, pad;
+  double value;
+  double convergence_value;
+  double update_scale[2];
+  int32_t inversion_steps[4];
+  double inversion_mean_error[4];
+  double inversion_max_error[4];
+} fw_trace_step;
+/* Copies up to `capacity` records of the calling thread's last fw_register / fw_register_resident
+ * (in stage, level, iteration order) into `out` (may be NULL with capacity 0) and returns how
+ * many there are. The reference returns these traces inside its stage results
+ * (crates/fastwarp-reg/src/syn.rs:96-128, optimizer.rs:324-430). */
+int32_t fw_last_trace(fw_trace_step* out, int32_t capacity);
 
 /* ---- profiling (the reference's WgpuBackend::set_profiling / take_profile,
  *      crates/fastwarp-gpu/src/profiler.rs:95-170) ---- */

@@ -30,6 +30,7 @@ def lib() -> C.CDLL:
         h.fwo_bessel.argtypes = [C.c_int, C.c_double]
         h.fwo_cubic_bspline.restype = C.c_double
         h.fwo_cubic_bspline.argtypes = [C.c_double, C.c_int]
+        h.fwo_last_trace.restype = C.c_int32
         _lib = h
     return _lib
 
@@ -240,3 +241,53 @@ def register(plan, fixed_grid: Grid, fixed, moving_grid: Grid, moving, fields: b
                        dptr(fwd) if fwd is not None else None, dptr(inv) if inv is not None else None, C.byref(res))
     )
     return res, fwd, inv
+
+
+def last_trace():
+    """Per-iteration records of this thread's last `register` call (fastwarp_amd._abi.TraceStep)."""
+    from fastwarp_amd._abi import read_trace
+
+    return read_trace(lib().fwo_last_trace)
+
+
+def register_lockstep(plan, fixed_grid: Grid, fixed, moving_grid: Grid, moving, trace, fields: bool = True,
+                      start: Affine | None = None):
+    """The CPU engine's run replaying another engine's discrete SyN decisions, taken from that
+    run's per-iteration trace (fastwarp_amd._abi.TraceStep records): every SyN level runs the
+    traced iteration count (convergence test off) and every inversion the traced step count.
+    With `start` (that run's collapsed linear result), the linear stages are skipped and the SyN
+    stage starts from it, so that the two SyN stages see the same inputs.
+    Returns (Result, forward, inverse, disagreements), where each disagreement is (inversion call,
+    body, the oracle's own decision (1 continue), max error, mean error) at a check where the
+    oracle's tolerance rule would have decided otherwise."""
+    from fastwarp_amd._abi import Plan
+
+    syn = [t for t in trace if t.stage == 2]
+    steps = [int(s) for t in syn for s in t.inversion_steps]
+    p = Plan()
+    C.pointer(p)[0] = plan
+    if start is not None:
+        p.run_linear = 0
+        p.initial_kind = 2
+        p.initial_affine = start
+    for level in range(p.syn_levels):
+        p.syn_iterations[level] = sum(1 for t in syn if t.level == level)
+    p.syn_convergence_threshold = -1e300
+    arr = (C.c_int32 * max(len(steps), 1))(*steps)
+    h = lib()
+    h.fwo_lockstep.argtypes = [C.POINTER(C.c_int32), C.c_int64]
+    h.fwo_lockstep(arr, len(steps))
+    try:
+        res, fwd, inv = register(p, fixed_grid, fixed, moving_grid, moving, fields=fields)
+    finally:
+        h.fwo_lockstep(arr, -1)
+    cap = 4096
+    call = (C.c_int64 * cap)()
+    body = (C.c_int32 * cap)()
+    nat = (C.c_int32 * cap)()
+    mx = (C.c_double * cap)()
+    mn = (C.c_double * cap)()
+    h.fwo_lockstep_disagreements.restype = C.c_int64
+    n = h.fwo_lockstep_disagreements(call, body, nat, mx, mn, cap)
+    dis = [(call[i], body[i], nat[i], mx[i], mn[i]) for i in range(min(n, cap))]
+    return res, fwd, inv, dis

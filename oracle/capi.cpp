@@ -297,12 +297,49 @@ double fwo_cubic_bspline(double u, int derivative) { return derivative ? cubic_b
 }  // extern "C"
 
 // The whole CPU engine: the generic drivers over the oracle's f64 ops (engine="cpu").
+extern "C" int32_t fwo_last_trace(fw_trace_step* out, int32_t capacity) { return fw::copy_last_trace(out, capacity); }
+
+namespace {
+InversionScript g_script;
+bool g_script_on = false;
+}  // namespace
+
 extern "C" int fwo_register(const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,
                             const fw_grid* moving_grid, const double* const* moving, double* forward, double* inverse,
                             fw_result* result) {
   return guard([&] {
     CpuBackend backend;
     CpuLinearEngine engine;
+    if (g_script_on) {
+      g_script.next = 0;
+      g_script.disagreements.clear();
+      g_script.disagreement_calls.clear();
+      backend.script = &g_script;
+    }
     fw::run_register(backend, engine, plan, fixed_grid, fixed, moving_grid, moving, forward, inverse, result, nullptr);
+    if (g_script_on && g_script.next != g_script.steps.size())
+      invalid("lockstep", "the run made fewer inversions than the script holds");
   });
+}
+
+// The next fwo_register calls replay these inversion step counts in call order (n < 0: off).
+extern "C" void fwo_lockstep(const int32_t* steps, int64_t n) {
+  g_script_on = n >= 0;
+  g_script.steps.assign(steps, steps + (n > 0 ? n : 0));
+}
+
+// The stop decisions of the last lockstep run where the oracle's own rule differed: per entry the
+// inversion call, the body count, the rule's decision (1 continue) and the errors it decided on.
+// Returns the count; copies up to `capacity`.
+extern "C" int64_t fwo_lockstep_disagreements(int64_t* call, int32_t* body, int32_t* natural, double* max_error,
+                                              double* mean_error, int64_t capacity) {
+  const auto& d = g_script.disagreements;
+  for (size_t i = 0; i < d.size() && int64_t(i) < capacity; ++i) {
+    call[i] = int64_t(g_script.disagreement_calls[i]);
+    body[i] = int32_t(d[i].body);
+    natural[i] = d[i].natural ? 1 : 0;
+    max_error[i] = d[i].max_error;
+    mean_error[i] = d[i].mean_error;
+  }
+  return int64_t(d.size());
 }

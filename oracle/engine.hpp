@@ -10,8 +10,19 @@
 
 namespace fw::oracle {
 
+// The lockstep comparison: the inversion step counts of another engine's run, replayed in call
+// order, and the stop decisions of the oracle's own rule that differ from them.
+struct InversionScript {
+  std::vector<int32_t> steps;
+  size_t next = 0;
+  std::vector<InvertDisagreement> disagreements;
+  std::vector<size_t> disagreement_calls;  // the call index of each disagreement
+};
+
 class CpuBackend {
  public:
+  InversionScript* script = nullptr;
+
   using ImageH = Image;
   using FieldH = VectorField;
 
@@ -109,7 +120,19 @@ class CpuBackend {
   VectorField compose(const VectorField& u, const VectorField& w) { return oracle::compose(u, w); }
   std::pair<VectorField, fw::InvertStats> invert(const VectorField& field, VectorField initial, const fw::InvertParams& p) {
     oracle::InvertParams q{p.max_iterations, p.mean_error_tolerance, p.max_error_tolerance, p.enforce_boundary_condition};
-    auto r = oracle::invert(field, &initial, q);
+    int forced = -1;
+    std::vector<InvertDisagreement>* dis = nullptr;
+    size_t before = 0, call = 0;
+    if (script) {
+      if (script->next >= script->steps.size()) invalid("lockstep", "the script has fewer inversions than the run");
+      call = script->next;
+      forced = script->steps[script->next++];
+      dis = &script->disagreements;
+      before = dis->size();
+    }
+    auto r = oracle::invert(field, &initial, q, forced, dis);
+    if (script)
+      for (size_t k = before; k < dis->size(); ++k) script->disagreement_calls.push_back(call);
     return {std::move(r.first), fw::InvertStats{r.second.iterations, r.second.mean_error_norm, r.second.max_error_norm}};
   }
   VectorField resample_field(const VectorField& f, const Grid& g) { return oracle::resample_field(f, g); }

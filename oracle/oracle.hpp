@@ -816,8 +816,21 @@ struct InvertStats {
   double max_error_norm = 0.0;
 };
 
-// field_ops.rs:315-392 `invert_displacement_field`
-inline std::pair<VectorField, InvertStats> invert(const VectorField& field, const VectorField* initial, const InvertParams& prm) {
+// A stop decision of the inversion loop that a scripted run took against the loop's own rule:
+// after `body` bodies the rule would have continued (`natural`) or stopped, on these errors.
+struct InvertDisagreement {
+  uint32_t body = 0;
+  bool natural = false;
+  double max_error = 0.0, mean_error = 0.0;
+};
+
+// field_ops.rs:315-392 `invert_displacement_field`. With `forced_steps` >= 0 the loop runs exactly
+// that many bodies instead of deciding by the tolerances (the lockstep comparison of a whole run
+// replays another engine's step counts); each check where the tolerance rule would have decided
+// otherwise is appended to `disagreements`.
+inline std::pair<VectorField, InvertStats> invert(const VectorField& field, const VectorField* initial, const InvertParams& prm,
+                                                  int forced_steps = -1,
+                                                  std::vector<InvertDisagreement>* disagreements = nullptr) {
   const Grid& g = field.grid;
   VectorField inv = initial ? *initial : VectorField::zeros(g);
   if (initial && initial->grid != g) invalid("invert", "initial estimate grid differs");
@@ -827,7 +840,10 @@ inline std::pair<VectorField, InvertStats> invert(const VectorField& field, cons
   const auto& n = g.size;
   for (;;) {
     const uint32_t current = iteration++;
-    if (!(current < prm.max_iterations && max_error > prm.max_error_tolerance && mean_error > prm.mean_error_tolerance)) break;
+    const bool natural = current < prm.max_iterations && max_error > prm.max_error_tolerance && mean_error > prm.mean_error_tolerance;
+    const bool go = forced_steps < 0 ? natural : current < uint32_t(forced_steps);
+    if (go != natural && disagreements) disagreements->push_back({current, natural, max_error, mean_error});
+    if (!go) break;
     ++executed;
     VectorField composed = compose(field, inv);
     std::vector<double> norms(composed.data.size());
