@@ -59,6 +59,7 @@ _SIGNATURES = [
     ("fw_warp", C.c_int, [_P, _P, _P, _P, C.POINTER(Affine), _PP]),
     ("fw_warp_mask", C.c_int, [_P, _P, _P, _P, C.POINTER(Affine), _P, _PP]),
     ("fw_cc_pair", C.c_int, [_P, _P, _P, _P, C.c_uint32, _D, _D, _PP, _PP]),
+    ("fw_cc_pair_noalloc", C.c_int, [C.POINTER(Grid), _P, _P, C.c_uint32, _P, _P, _P, _P]),
     ("fw_mi_dense_pair", C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, _D, _D, _PP, _PP]),
     ("fw_smooth_field", C.c_int, [_P, _P, C.c_double, _PP]),
     ("fw_smooth_image", C.c_int, [_P, _P, _D, _PP]),

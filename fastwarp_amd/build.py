@@ -20,6 +20,7 @@ SOURCES = [
     CSRC / "hip" / "mi_kernels.hip",
     CSRC / "hip" / "backend.hip",
     CSRC / "hip" / "mi_backend.hip",
+    CSRC / "hip" / "noalloc.hip",
     CSRC / "capi.cpp",
     CSRC / "serve.cpp",
 ]

@@ -85,6 +85,15 @@ int guard(fw_ctx* ctx, F&& f) {
   }
 }
 
+}  // namespace
+
+namespace fw {
+// The message fw_last_error(NULL) returns for a call with no context.
+void set_call_error(const std::string& message) { g_create_error = message; }
+}  // namespace fw
+
+namespace {
+
 void need(const void* p, const char* what) {
   if (!p) fw::invalid("fastwarp", std::string(what) + " is null");
 }

@@ -536,16 +536,6 @@ HImage HipBackend::warp_mask_through_field(const HImage& mask, const HField& fie
 
 // ----------------------------------------------------------------------------- CC (a5, a6) ---
 
-// The z run of a CC block: the longest of max_tile, max_tile / 2, ..., 1 planes whose grid still
-// has 512 blocks (two per CU), so that a small level is not left to a few long-running blocks.
-static int cc_tile_for(const Dims& d, int max_tile) {
-  int t = std::max(1, max_tile);
-  for (; t > 1; t /= 2) {
-    const dim3 g = cc_grid(d, t);
-    if (size_t(g.x) * g.y * g.z >= 512) break;
-  }
-  return t;
-}
 
 MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving, const HImage* valid, size_t radius,
                                        bool want_norms) {

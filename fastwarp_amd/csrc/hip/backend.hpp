@@ -306,7 +306,7 @@ class HipBackend {
   HField resample_field(const HField& field, const Grid& grid);
 
   // CC tile height in z of the fused kernel.
-  int cc_tile_z = 16;
+  int cc_tile_z = kCcTileZ;
   // Slots of the coherent-host rings of update maxima and CC partials (results awaiting a settle).
   static constexpr size_t kMaximaSlots = 64;
   static constexpr size_t kCcSlots = 4;

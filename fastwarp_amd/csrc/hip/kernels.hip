@@ -95,6 +95,14 @@ __device__ float block_sum_f64(double v) {
   return (float)t;
 }
 
+// The split scale for an image bounded by `bound` (the host's cc_split_scale, on the device).
+__device__ __forceinline__ float cc_split_scale_dev(float bound) {
+  if (!(bound > 0.0f) || !isfinite(bound)) return 524288.0f;
+  int e = 0;
+  frexpf(bound, &e);  // bound < 2^e
+  return ldexpf(1.0f, 19 - e);
+}
+
 // One voxel's values split at the power-of-two step q = 1 / scale: v = h q + l with h an integer
 // (|h| <= 2^20 for |v| <= twice the bound) and l exact in f32 (|l| <= q / 2).
 struct CcSplit {
@@ -153,7 +161,13 @@ __global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
   const int z1 = min(z0 + a.tile_z, d.nz);
   const long long plane = (long long)d.nx * d.ny;
   const long long n3 = d.n();
-  const float sa_ = a.scale_f, sb_ = a.scale_m, qa = a.step_f, qb = a.step_m;
+  float sa_ = a.scale_f, sb_ = a.scale_m, qa = a.step_f, qb = a.step_m;
+  if (a.bounds) {
+    sa_ = cc_split_scale_dev(a.bounds[0]);
+    sb_ = cc_split_scale_dev(a.bounds[1]);
+    qa = 1.0f / sa_;
+    qb = 1.0f / sb_;
+  }
   const double qa2 = (double)qa * qa, qb2 = (double)qb * qb, qab = (double)qa * qb;
 
   // the columns this thread owns: plane offset, or -1 outside the image
@@ -365,6 +379,17 @@ static void launch_cc_r(const CcArgs& a, hipStream_t s) {
 
 dim3 cc_grid(const Dims& d, int tile_z) {
   return dim3((d.nx + kCcTX - 1) / kCcTX, (d.ny + kCcTY - 1) / kCcTY, (d.nz + tile_z - 1) / tile_z);
+}
+
+// The z run of a CC block: the longest of max_tile, max_tile / 2, ..., 1 planes whose grid still
+// has 512 blocks (two per CU), so that a small level is not left to a few long-running blocks.
+int cc_tile_for(const Dims& d, int max_tile) {
+  int t = std::max(1, max_tile);
+  for (; t > 1; t /= 2) {
+    const dim3 g = cc_grid(d, t);
+    if (size_t(g.x) * g.y * g.z >= 512) break;
+  }
+  return t;
 }
 
 void launch_cc_pair(const CcArgs& a, hipStream_t s) {

@@ -63,6 +63,7 @@ struct CcArgs {
   int tile_z;
   float scale_f, scale_m;  // 2^19 / (the image's bound rounded up to a power of two)
   float step_f, step_m;    // 1 / scale
+  const float* bounds;     // device [2] (fixed, moving): when set, the scales come from these
   Probes probes;
   float* d_fixed;        // 3 planes: coefficient * grad(moving)
   float* d_moving;       // 3 planes: coefficient' * grad(fixed)
@@ -71,6 +72,9 @@ struct CcArgs {
 };
 void launch_cc_pair(const CcArgs& a, hipStream_t s);
 dim3 cc_grid(const Dims& d, int tile_z);
+// The longest z run of at most max_tile planes (halving) whose grid still has 512 blocks.
+int cc_tile_for(const Dims& d, int max_tile);
+constexpr int kCcTileZ = 16;  // the default longest z run
 
 // --- per-block partial reductions ---
 // Largest |u/spacing| of two fields (b may be null) and sums of squares.

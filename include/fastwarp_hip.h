@@ -142,6 +142,19 @@ int fw_warp_mask(fw_ctx* ctx, const fw_image* mask, const fw_field* field, const
 int fw_cc_pair(fw_ctx* ctx, const fw_image* fixed, const fw_image* moving, const fw_image* valid, uint32_t radius,
                double values[2], double norms[2], fw_field** d_fixed, fw_field** d_moving);
 
+/* No counterpart in the reference: fw_cc_pair (unmasked) for callers that cannot allocate (code
+ * under HIP graph capture, serving loops that forbid hipMalloc). Every buffer is the caller's
+ * device memory: fixed and moving (n f32 each, x-fastest, on `grid`), d_fixed and d_moving (3n f32,
+ * three planes each, written), values (2 f64, written: values[0] = values[1], as fw_cc_pair's).
+ * The work is enqueued on `stream` (a hipStream_t, NULL for the null stream) and the call returns
+ * without waiting; nothing is allocated. Block partials live in a static per-device scratch of
+ * 137,224 bytes, which limits the grid to 16,384 CC blocks of 64 x 8 x (at most 16) voxels (the
+ * 1 mm grid needs 1008); a larger grid is rejected with FW_ERR_INVALID. Calls that overlap on one
+ * GPU race on that scratch. Results are the same bits as fw_cc_pair's on the same images. Errors:
+ * the status, and fw_last_error(NULL) for the text. */
+int fw_cc_pair_noalloc(const fw_grid* grid, const float* fixed, const float* moving, uint32_t radius, float* d_fixed,
+                       float* d_moving, double* values, void* stream);
+
 /* Backend::mi_value_derivative_pair(_masked)(_with_norms) (backend.rs:185-243): dense Mattes MI in
  * both directions; masks may be NULL. */
 int fw_mi_dense_pair(fw_ctx* ctx, const fw_image* fixed, const fw_image* moving, const fw_image* fixed_mask,
