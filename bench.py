@@ -3,8 +3,9 @@
 A step is one full 1 mm normalisation (centre of mass, Rigid and Affine Mattes MI, SyN CC radius 4
 with the cohort schedule 8x4x2x1) of the seeded 182x218x182 phantom pair. W untimed warmup steps,
 then K timed steps; prints one JSON line with registrations/hour, per-step latency, the roofline of
-the dominant kernel (HIP events carried by every 4th of its launches in the timed region), the
-per-kernel breakdown of the last warmup step and the f64 CPU engine as a baseline.
+the dominant kernel (HIP events carried by every one of its launches in the timed region), the
+per-kernel breakdown of the last warmup step, the served stream, and one complete registration
+on the f64 CPU engine as a baseline (the 2 mm one by default).
 """
 from __future__ import annotations
 
@@ -28,26 +29,17 @@ def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str) -> int | None:
-    """HBM bytes per launch of `kernel` in this workload from the committed PMC summary
-    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel), if present."""
-    path = ROOT / "profiles" / "traffic.json"
-    if not path.exists():
-        return None
-    return json.loads(path.read_text()).get("registration_1mm", {}).get(kernel)
-
-
 def dominant(profile: dict) -> str:
     """The kernel family with the largest total time."""
     return max(profile.items(), key=lambda kv: kv[1][1])[0]
 
 
-def roofline(name: str, timed: dict, breakdown: dict) -> dict:
-    """Kernel `name` over the timed region: algorithmic bytes per launch over its mean duration
-    (HIP events carried by its timed launches); its share of GPU time from `breakdown`."""
+def roofline(name: str, timed: dict, steps: int, breakdown: dict) -> dict:
+    """Kernel `name` over the timed region, every launch carrying HIP events on the context's
+    stream: the algorithmic bytes of all its launches over the sum of their durations, the same
+    quantity a rocprofv3 kernel trace of the run gives (total bytes over total duration)."""
     launches, ms, nbytes, n_timed = timed[name]
-    per_launch_ms = ms / launches
-    achieved = (nbytes / launches) / (per_launch_ms * 1e-3) / 1e9
+    achieved = nbytes / (ms * 1e-3) / 1e9
     return {
         "kernel": name,
         "bound": "hbm",
@@ -55,47 +47,51 @@ def roofline(name: str, timed: dict, breakdown: dict) -> dict:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(name),
+        "traffic": None,
         "algorithmic_bytes_per_launch": round(nbytes / launches),
-        "launches": launches,
-        "timed_launches": n_timed,
-        "mean_launch_us": round(per_launch_ms * 1e3, 2),
+        "launches_per_step": round(launches / steps, 1),
+        "ms_per_step": round(ms / steps, 3),
+        "mean_launch_us": round(1e3 * ms / launches, 2),
         "share_of_gpu_time": round(breakdown[name][1] / sum(v[1] for v in breakdown.values()), 4),
     }
 
 
-def cpu_baseline(pair, gpu_result) -> dict:
-    """The f64 CPU engine (the oracle) on a bounded sample of the same workload: one iteration per
-    level of every stage on the 1 mm pair, timed per level, extrapolated with the iteration and
-    evaluation counts of the GPU run (set-up time of a level counted once)."""
-    import oracle
-    from fastwarp_amd.register import cohort_plan
+def cpu_info() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    plan = cohort_plan(1)
-    plan.set_linear([1, 1, 1, 1], [8, 4, 2, 1], [3.0, 2.0, 1.0, 0.0])
-    plan.set_syn([1, 1, 1, 1], [8, 4, 2, 1], [3.0, 2.0, 1.0, 0.0])
+
+def cpu_baseline(config: str) -> dict:
+    """The f64 CPU engine (the oracle's restatement of engine="cpu") running one complete plain
+    registration of `config` (every stage and level, to convergence as planned), timed end to end
+    on this host's cores."""
+    import oracle
+    from fastwarp_amd.phantom import CONFIGS, make_pair
+    from fastwarp_amd.register import cohort_plan, plan_2mm
+
+    cfg = CONFIGS[config]
+    pair = make_pair(**cfg)
+    plan = cohort_plan(1) if config == "1mm" else plan_2mm(1)
     t0 = time.perf_counter()
-    res, _, _ = oracle.register(plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=False)
+    res, _, _ = oracle.register(plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=True)
     wall = time.perf_counter() - t0
-    est = res.seconds_preprocess
-    for s in range(2):
-        for lv in range(4):
-            evals = max(res.linear_evaluations[s][lv], 1)
-            per_eval = res.linear_iteration_seconds[s][lv] / evals
-            setup = res.linear_level_seconds[s][lv] - res.linear_iteration_seconds[s][lv]
-            est += setup + per_eval * gpu_result.linear_evaluations[s][lv]
-    for lv in range(res.syn_levels):
-        per_it = res.syn_iteration_seconds[lv] / max(res.syn_iterations[lv], 1)
-        setup = res.syn_level_seconds[lv] - res.syn_iteration_seconds[lv]
-        est += setup + per_it * gpu_result.syn_iterations[lv]
     threads = int(os.environ.get("FW_THREADS") or os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     return {
-        "value": round(3600.0 / est, 3),
+        "value": round(3600.0 / wall, 3),
         "unit": "registrations/hour",
         "cores": threads,
         "kind": "port",
-        "sample": f"f64 CPU engine, 1 CG iteration and 1 SyN iteration per level on the 1 mm pair "
-        f"({wall:.1f} s measured), extrapolated to the GPU run's iteration/evaluation counts: {est:.1f} s per registration",
+        "sample": f"one complete {config} plain registration ({cfg['shape_xyz']} at {cfg['spacing']} mm, seed {cfg['seed']}: "
+        f"centre of mass, Rigid, Affine, SyN) on the f64 CPU engine, {wall:.1f} s end to end "
+        f"(linear {res.seconds_linear:.1f} s, SyN {res.seconds_syn:.1f} s) on {threads} threads of {os.cpu_count()} "
+        f"logical CPUs ({cpu_info()})",
+        "seconds": round(wall, 2),
+        "config": config,
     }
 
 
@@ -197,6 +193,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-engine baseline")
+    ap.add_argument("--cpu-config", default="2mm", choices=["1mm", "2mm"],
+                    help="the plain registration the CPU engine runs for the baseline (the 2 mm one by default)")
     ap.add_argument("--config", default="1mm", choices=["1mm", "2mm"])
     ap.add_argument("--kernels", action="store_true", help="time each hot-path kernel at 1 mm instead")
     ap.add_argument("--slots", type=int, default=4, help="slots of the served stream (0 skips it)")
@@ -233,8 +231,7 @@ def main() -> int:
         return r
 
     # The per-kernel breakdown comes from the last warmup step, every launch timed; the timed
-    # steps time every 4th launch of the dominant kernel only (the events cost the GPU a few
-    # microseconds per timed launch).
+    # steps time every launch of the dominant kernel family only.
     breakdown = None
     for w in range(args.warmup):
         if w == args.warmup - 1:
@@ -246,7 +243,7 @@ def main() -> int:
         breakdown = {k: v for k, v in backend.take_profile().items() if not k.startswith("host:")}
         backend.set_profiling(False)
         focus = dominant(breakdown)
-        backend.set_profiling_focus(focus, 4)
+        backend.set_profiling_focus(focus, 1)
     backend.set_profiling(True)
     lat = []
     last = None
@@ -307,7 +304,7 @@ def main() -> int:
                        for l in range(last.result.syn_levels)],
         "host_s_per_step": dict(sorted(host.items(), key=lambda kv: -kv[1])),
         "device_peak_bytes": int(peak),
-        "roofline": roofline(focus, profile, breakdown),
+        "roofline": roofline(focus, profile, args.steps, breakdown),
         "kernels_from": "the last warmup step, every launch timed" if args.warmup else f"the {args.steps} timed steps",
         "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "gbs": round(v[2] / max(v[1], 1e-9) / 1e6, 1)}
                     for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][1])},
@@ -320,8 +317,8 @@ def main() -> int:
             log(f"served stream failed: {e}")
             line["served"] = {"error": str(e)}
     if not args.no_cpu:
-        log("CPU-engine baseline (bounded sample)")
-        line["cpu_baseline"] = cpu_baseline(pair, last.result)
+        log(f"CPU-engine baseline: one complete {args.cpu_config} registration")
+        line["cpu_baseline"] = cpu_baseline(args.cpu_config)
     print(json.dumps(line), flush=True)
     return 0
 
