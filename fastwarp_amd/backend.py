@@ -80,6 +80,7 @@ _SIGNATURES = [
     ("fw_resample_field", C.c_int, [_P, _P, C.POINTER(Grid), _PP]),
     ("fw_mi_level_create", C.c_int, [_P, _P, _D, C.POINTER(C.c_uint32), C.c_int64, C.POINTER(MiBinning), _PP]),
     ("fw_mi_level_value", C.c_int, [_P, C.POINTER(LinearTransform), _D]),
+    ("fw_mi_level_histogram", C.c_int, [_P, C.POINTER(LinearTransform), _D, _D, C.POINTER(C.c_int64)]),
     ("fw_mi_level_values", C.c_int, [_P, C.POINTER(LinearTransform), C.c_int64, _D]),
     ("fw_mi_level_value_and_derivative", C.c_int, [_P, C.POINTER(LinearTransform), _D, _D]),
     ("fw_mi_level_free", None, [_P]),
@@ -529,10 +530,18 @@ class MiLevel:
         self._b._check(lib().fw_mi_level_value_and_derivative(self.handle, C.byref(t), C.byref(v), dptr(d)))
         return v.value, d
 
+    def histogram(self, t: LinearTransform, bins: int):
+        """The merged histogram before the finish: (joint (bins, bins), fixed marginal, valid count)."""
+        joint = np.zeros(bins * bins)
+        marginal = np.zeros(bins)
+        valid = C.c_int64()
+        self._b._check(lib().fw_mi_level_histogram(self.handle, C.byref(t), dptr(joint), dptr(marginal), C.byref(valid)))
+        return joint.reshape(bins, bins), marginal, valid.value
+
     def __del__(self):
         try:
-            if self.handle:
+            if self.handle and self._b.ctx:
                 lib().fw_mi_level_free(self.handle)
-                self.handle = None
+            self.handle = None
         except Exception:
             pass

@@ -372,6 +372,22 @@ int fw_mi_level_value(fw_mi_level* level, const fw_linear_transform* t, double* 
   });
 }
 
+int fw_mi_level_histogram(fw_mi_level* level, const fw_linear_transform* t, double* joint, double* fixed_marginal,
+                          int64_t* valid) {
+  if (!level) return FW_ERR_INVALID;
+  return guard(level->ctx, [&] {
+    need(joint, "joint output");
+    need(fixed_marginal, "marginal output");
+    need(valid, "valid output");
+    std::vector<double> j, m;
+    uint64_t v = 0;
+    level->level.histogram_cells(linear_from(t), j, m, v);
+    std::memcpy(joint, j.data(), j.size() * sizeof(double));
+    std::memcpy(fixed_marginal, m.data(), m.size() * sizeof(double));
+    *valid = int64_t(v);
+  });
+}
+
 int fw_mi_level_values(fw_mi_level* level, const fw_linear_transform* transforms, int64_t count, double* values) {
   if (!level) return FW_ERR_INVALID;
   return guard(level->ctx, [&] {

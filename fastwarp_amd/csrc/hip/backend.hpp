@@ -333,11 +333,17 @@ class MiLevel {
   bool batch_pays() const { return count_ <= kBatchPaysSamples && mi_batch_capacity(int(binning_.bins)) > 1; }
   static constexpr size_t kBatchPaysSamples = 500000;
   std::pair<double, std::vector<double>> value_and_derivative(const LinearMovingTransform& t);
+  // The histogram of one transform as the finish reads it: the joint cells (bins x bins, in
+  // samples: the B-spline weights summed), the fixed marginal counts and the valid count.
+  void histogram_cells(const LinearMovingTransform& t, std::vector<double>& joint, std::vector<double>& fixed_marginal,
+                       uint64_t& valid);
   size_t count() const { return count_; }
 
  private:
   void batch_values(const LinearMovingTransform* ts, size_t k, std::vector<double>& out);
-  double histogram(const LinearMovingTransform& t, MiSampledArgs& args, std::vector<double>* ratio);
+  double histogram(const LinearMovingTransform& t, MiSampledArgs& args, std::vector<double>* ratio,
+                   std::vector<double>* joint = nullptr, std::vector<double>* fixed_marginal = nullptr,
+                   uint64_t* valid_out = nullptr);
   DeviceContext* ctx_;
   const HImage* moving_;
   DevBuf samples_;

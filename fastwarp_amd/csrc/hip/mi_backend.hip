@@ -71,7 +71,8 @@ MiLevel::MiLevel(HipBackend& backend, const HImage& moving, const HostVec<Vec3>&
   ctx_->sync();
 }
 
-double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std::vector<double>* ratio) {
+double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std::vector<double>* ratio,
+                          std::vector<double>* joint, std::vector<double>* fixed_marginal, uint64_t* valid_out) {
   const Grid& mg = moving_->grid;
   const AffineTransform T = t.collapsed();
   const Mat3 A = mat3_mul(mg.index_from_offset, T.matrix);
@@ -108,10 +109,22 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
   }
   ctx_->wait_done(mi_sum_slots_blocks(int(cells)), tag);
   const uint64_t valid = host[cells - 1];
+  if (joint) *joint = joint_from(host, bins);
+  if (fixed_marginal) *fixed_marginal = marginal_from(host, bins);
+  if (valid_out) {
+    *valid_out = valid;
+    return 0.0;
+  }
   HistogramFinish fin =
       finish_histogram(bins, binning_.moving_bin_size, valid, joint_from(host, bins), marginal_from(host, bins), ratio != nullptr);
   if (ratio) *ratio = std::move(fin.ratio);
   return fin.value;
+}
+
+void MiLevel::histogram_cells(const LinearMovingTransform& t, std::vector<double>& joint, std::vector<double>& fixed_marginal,
+                              uint64_t& valid) {
+  MiSampledArgs a;
+  histogram(t, a, nullptr, &joint, &fixed_marginal, &valid);
 }
 
 double MiLevel::value(const LinearMovingTransform& t) {

@@ -191,6 +191,11 @@ int fw_resample_field(fw_ctx* ctx, const fw_field* field, const fw_grid* grid, f
 int fw_mi_level_create(fw_ctx* ctx, const fw_image* moving, const double* points, const uint32_t* fixed_bin, int64_t m,
                        const fw_mi_binning* binning, fw_mi_level** out);
 int fw_mi_level_value(fw_mi_level* level, const fw_linear_transform* t, double* value);
+/* GpuMiLevel::histogram (crates/fastwarp-gpu/src/wgpu_backend/mi.rs:312-377): the merged histogram
+ * of one transform before the finish: joint[bins * bins] (fixed bin major; the B-spline weights
+ * summed, in samples), fixed_marginal[bins] (sample counts) and the valid sample count. */
+int fw_mi_level_histogram(fw_mi_level* level, const fw_linear_transform* t, double* joint, double* fixed_marginal,
+                          int64_t* valid);
 /* values[i] = the value of transforms[i], i < count, computed in as few histogram passes as the
  * device allows (up to four transforms share one pass over the samples); each equals
  * fw_mi_level_value of that transform. No counterpart in the reference: the line search's

@@ -106,6 +106,17 @@ static void deriche() {
   }
   CHECK(std::fabs(sum - 1.0) < 1e-6, "deriche dc gain");
   CHECK(std::fabs(var - 9.0) < 0.9, "deriche variance");
+  // the Gaussian's shape to 1 % of its peak and a symmetric response to 1e-6 of the peak
+  const double peak = 1.0 / (3.0 * std::sqrt(2.0 * 3.14159265358979323846));
+  double worst = 0.0;
+  for (size_t i = 0; i < 81; ++i) {
+    const double x = double(i) - 40.0;
+    worst = std::max(worst, std::fabs(line[g.linear_index(i, 1, 1)] - peak * std::exp(-x * x / 18.0)));
+  }
+  CHECK(worst < 0.01 * peak, "deriche shape");
+  for (size_t i = 0; i < 40; ++i)
+    CHECK(std::fabs(line[g.linear_index(40 - i, 1, 1)] - line[g.linear_index(40 + i, 1, 1)]) < 1e-6 * peak + 1e-12,
+          "deriche symmetry");
   const Grid c({4, 4, 4}, {1, 1, 1}, {0, 0, 0}, kIdentity3);
   Image k(c, std::vector<double>(64, 7.0));
   Image s = smooth_recursive_gaussian(k, {1.5, 1.5, 1.5});
@@ -300,6 +311,24 @@ static void preprocess() {
   CHECK(p.data[0] == 0.0 && p.data[99] <= 1.0 && p.data[99] > 0.99, "window to [0, 1]");
   const Image w = preprocess_volume(v, 0.1f, 0.9f);
   CHECK(w.data[0] == 0.0 && w.data[99] == 1.0, "winsorised ends clamp");
+  // preprocess.rs:271-303: quantiles interpolate inside the bin like ITK's (a ramp of 2560 values
+  // has its quartiles at 640 and 1920), and the result is invariant to affine intensity changes
+  {
+    std::vector<double> ramp(2560);
+    for (size_t i = 0; i < ramp.size(); ++i) ramp[i] = double(i);
+    const IntensityHistogram hq(ramp);
+    CHECK(std::fabs(hq.quantile(0.25) - 640.0) < 5.0, "ramp first quartile");
+    CHECK(std::fabs(hq.quantile(0.75) - 1920.0) < 5.0, "ramp third quartile");
+    const Grid g5({500, 1, 1}, {1, 1, 1}, {0, 0, 0}, kIdentity3);
+    std::vector<double> a(500), b(500);
+    for (int i = 0; i < 500; ++i) {
+      a[i] = double((i * 37) % 101);
+      b[i] = 3.0 * a[i] + 40.0;
+    }
+    const Image pa = preprocess_volume(Image(g5, a), 0.005f, 0.995f);
+    const Image pb = preprocess_volume(Image(g5, b), 0.005f, 0.995f);
+    for (int i = 0; i < 500; ++i) CHECK(std::fabs(pa.data[i] - pb.data[i]) < 1e-5, "affine intensity invariance");
+  }
   // the device's f32 bin rule (count of f32 thresholds at or below v) equals index_of on f32
   // intensities: random values over and beyond the range, every threshold and its neighbours
   {

@@ -329,6 +329,69 @@ def test_warp_is_exact_on_an_affine_ramp_and_zero_outside():
     assert moved[7 + 8 * (3 + 7 * 2)] == 0.0
 
 
+def test_nearest_voxel_rounds_half_up_and_is_zero_outside():
+    """mask.rs:99-111 (ITK NearestNeighborInterpolateImageFunction): a point exactly half way
+    between voxels takes the upper one; a point whose rounded index leaves the buffer is outside.
+    The field grid sits half a voxel along x from the mask's, so every sample is a tie."""
+    src = Grid.make((6, 5, 4), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
+    dst = Grid.make((6, 5, 4), (1.0, 1.0, 1.0), (0.5, 0.0, 0.0))
+    i, j, k = cases.coords(src)
+    values = 1.0 + i + 10 * j + 100 * k
+    zero = np.zeros((dst.n, 3))
+    got = oracle.warp(src, values, dst, zero, nearest=True)
+    di, dj, dk = cases.coords(dst)
+    want = np.where(di < 5, 1.0 + (di + 1) + 10 * dj + 100 * dk, 0.0)
+    assert set(np.unique(got)) <= {0.0, 1.0}
+    # the warp of a mask reports membership: {0, 1} of the nearest voxel's non-zero value
+    assert np.array_equal(got, (want != 0).astype(float))
+    # a displacement of just under half a voxel back keeps the lower voxel everywhere
+    back = np.tile([-1e-9, 0.0, 0.0], (dst.n, 1))
+    below = oracle.warp(src, (values % 2 == 0).astype(float), dst, back, nearest=True)
+    assert np.array_equal(below, ((1.0 + di + 10 * dj + 100 * dk) % 2 == 0).astype(float))
+
+
+def test_warp_through_a_prior_field_follows_both_displacements():
+    """cpu/mod.rs:60-86: out(x) = I(q + pre(q)), q = x + u(x), with the prior field on its own grid;
+    constant displacements on a linear image give out(x) = I(x + u + pre) exactly where the
+    point is inside, and zero where it leaves the image."""
+    g = cases.grid((10, 9, 8), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
+    pre_g = cases.grid((12, 11, 10), (1.0, 1.0, 1.0), (-1.0, -1.0, -1.0))
+    i, j, k = cases.coords(g)
+    image = 3.0 * i - 2.0 * j + 0.5 * k + 7.0
+    u = np.tile([0.75, -0.5, 0.25], (g.n, 1))
+    pre = np.tile([1.0, 0.25, -0.5], (pre_g.n, 1))
+    got = oracle.warp(g, image, g, u, pre_grid=pre_g, pre=pre)
+    x, y, z = i + 1.75, j - 0.25, k - 0.25
+    inside = (x >= -0.5) & (x < 9.5) & (y >= -0.5) & (y < 8.5) & (z >= -0.5) & (z < 7.5)
+    # the linear interpolation of a linear image is exact inside the grid's voxel centres
+    interior = inside & (x <= 9) & (y >= 0) & (z >= 0)
+    want = 3.0 * x - 2.0 * y + 0.5 * z + 7.0
+    assert np.allclose(got[interior], want[interior], atol=1e-12)
+    assert np.all(got[~inside] == 0.0)
+    # without the prior the same point moves by u alone
+    alone = oracle.warp(g, image, g, u)
+    x1 = i + 0.75
+    ok = (x1 <= 9) & (j - 0.5 >= 0) & (k + 0.25 <= 7)
+    assert np.allclose(alone[ok], (3.0 * x1 - 2.0 * (j - 0.5) + 0.5 * (k + 0.25) + 7.0)[ok], atol=1e-12)
+
+
+def blob(g: Grid, centre) -> np.ndarray:
+    """init.rs:92-105: exp(-|p - c|^2 / 18) over the physical points of the grid."""
+    i, j, k = cases.coords(g)
+    pts = np.stack([i, j, k], 1) * np.array(list(g.spacing)) + np.array(list(g.origin))
+    return np.exp(-np.sum((pts - np.array(centre)) ** 2, axis=1) / 18.0)
+
+
+def test_centre_of_gravity_of_a_blob_is_its_centre():
+    """init.rs:107-140: the intensity centre of a Gaussian blob on an anisotropic grid is the blob's
+    centre (1e-3 mm); an image without mass is an error."""
+    g = Grid.make((40, 36, 32), (1.0, 1.5, 2.0), (-20.0, -30.0, -10.0))
+    c = oracle.centre_of_gravity(g, blob(g, (3.0, -4.0, 5.0)))
+    assert np.max(np.abs(c - np.array([3.0, -4.0, 5.0]))) < 1e-3, c
+    with pytest.raises(oracle.OracleError):
+        oracle.centre_of_gravity(Grid.make((4, 4, 4), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)), np.zeros(64))
+
+
 # ------------------------------------------------------------------------ sampled MI ------
 
 def smooth_vol(g: Grid, shift, phase=0.0) -> np.ndarray:
