@@ -21,6 +21,7 @@ void check(hipError_t e, const char* what) {
 }
 
 DeviceContext::DeviceContext(int device) : device_(device) {
+  wait_timeout_s_ = wait_timeout_from(std::getenv("FASTWARP_GPU_WAIT_TIMEOUT_S"));
   check(hipSetDevice(device), "hipSetDevice");
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
 }
@@ -166,6 +167,19 @@ uint32_t* DeviceContext::mailbox_device() {
   return mailbox_dev_;
 }
 
+// Waits for the stream by polling it, so that a device that stops making progress fails the call
+// after the wait timeout instead of blocking it for good (a device error is reported at once).
+static void wait_stream(hipStream_t stream, std::chrono::steady_clock::time_point t0, double limit_s) {
+  for (unsigned polls = 0;; ++polls) {
+    const hipError_t e = hipStreamQuery(stream);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) check(e, "hipStreamQuery");
+    if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > limit_s)
+      invalid("hip", "the device did not finish within " + std::to_string(limit_s) + " s (FASTWARP_GPU_WAIT_TIMEOUT_S)");
+    if (polls > 1000) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
 void DeviceContext::wait_done(int count, uint32_t tag) {
   HostTimer t("sync_wait");
   if (count < 1 || count > kDoneWords) invalid("device", "completion word count outside 1.." + std::to_string(kDoneWords));
@@ -173,12 +187,15 @@ void DeviceContext::wait_done(int count, uint32_t tag) {
   const auto t0 = std::chrono::steady_clock::now();
   for (int b = 0; b < count; ++b) {
     for (unsigned spins = 1; w[b] != tag; ++spins) {
-      if ((spins & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
-        // late: wait for the stream (which reports a device error, if any) and look once more
-        check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-        if (w[b] != tag) invalid("device", "a kernel completed without writing its completion word");
-        break;
-      }
+      if ((spins & 4095) != 0) continue;
+      // now and then: a device error is reported, a stream that has drained without the word is
+      // an error, and the wait fails after the wait timeout
+      const hipError_t e = hipStreamQuery(stream_);
+      if (e != hipSuccess && e != hipErrorNotReady) check(e, "hipStreamQuery");
+      if (e == hipSuccess && w[b] != tag) invalid("device", "a kernel completed without writing its completion word");
+      if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > wait_timeout_s_)
+        invalid("hip", "the device did not finish within " + std::to_string(wait_timeout_s_) +
+                           " s (FASTWARP_GPU_WAIT_TIMEOUT_S)");
     }
   }
   std::atomic_thread_fence(std::memory_order_acquire);
@@ -186,7 +203,7 @@ void DeviceContext::wait_done(int count, uint32_t tag) {
 
 void DeviceContext::sync() {
   HostTimer t("sync_wait");
-  check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  wait_stream(stream_, std::chrono::steady_clock::now(), wait_timeout_s_);
 }
 
 const std::vector<double>& DeviceContext::host_taps(double variance, size_t length) {

@@ -67,6 +67,9 @@ class DeviceContext : public HostSpanSink {
   uint32_t* done_words_device() { return mailbox_device() + kDoneBase; }
   void wait_done(int count, uint32_t tag);
   void sync();
+  // How long a wait on the device may take before it fails (device.rs:919-928
+  // FASTWARP_GPU_WAIT_TIMEOUT_S: positive finite seconds, 300 by default), read at creation.
+  double wait_timeout_s() const { return wait_timeout_s_; }
   // Device copy of the Gaussian taps of (variance, axis length), cached.
   const float* taps(double variance, size_t length, int& radius);
   // The same taps on the host (f64), cached.
@@ -94,6 +97,7 @@ class DeviceContext : public HostSpanSink {
 
  private:
   int device_;
+  double wait_timeout_s_ = 300.0;
   hipStream_t stream_{};
   std::map<size_t, std::vector<void*>> free_;
   uint64_t held_ = 0, peak_ = 0;

@@ -154,6 +154,18 @@ struct Cancelled : std::runtime_error {
   throw Error(what, message);
 }
 
+// device.rs:919-928: the FASTWARP_GPU_WAIT_TIMEOUT_S value if it parses as positive finite
+// seconds, else 300.
+inline double wait_timeout_from(const char* text) {
+  constexpr double kDefault = 300.0;
+  if (!text) return kDefault;
+  char* end = nullptr;
+  const double v = std::strtod(text, &end);
+  while (end && (*end == ' ' || *end == '\t' || *end == '\n')) ++end;
+  if (end == text || (end && *end != '\0') || !std::isfinite(v) || !(v > 0.0)) return kDefault;
+  return v;
+}
+
 inline Vec3 mat3_mul_vec(const Mat3& m, const Vec3& v) {
   return {m[0][0] * v[0] + m[0][1] * v[1] + m[0][2] * v[2],
           m[1][0] * v[0] + m[1][1] * v[1] + m[1][2] * v[2],

@@ -124,6 +124,20 @@ static void deriche() {
   ok("deriche");
 }
 
+// device.rs:919-928: FASTWARP_GPU_WAIT_TIMEOUT_S takes positive finite seconds, else 300.
+static void wait_timeout() {
+  CHECK(wait_timeout_from(nullptr) == 300.0, "timeout default");
+  CHECK(wait_timeout_from("2.5") == 2.5, "timeout seconds");
+  CHECK(wait_timeout_from(" 7 ") == 7.0, "timeout with spaces");
+  CHECK(wait_timeout_from("0") == 300.0, "timeout zero");
+  CHECK(wait_timeout_from("-1") == 300.0, "timeout negative");
+  CHECK(wait_timeout_from("inf") == 300.0, "timeout infinite");
+  CHECK(wait_timeout_from("nan") == 300.0, "timeout nan");
+  CHECK(wait_timeout_from("soon") == 300.0, "timeout text");
+  CHECK(wait_timeout_from("1e-9") == 1e-9, "timeout tiny");
+  ok("wait_timeout");
+}
+
 // convergence.rs:188-213: the closed form of the two-control-point fit.
 static void convergence() {
   WindowConvergenceMonitor m(10);
@@ -753,6 +767,7 @@ int main() {
   shrink();
   deriche();
   convergence();
+  wait_timeout();
   optimizer();
   speculative_search();
   jacobians();

@@ -6,7 +6,8 @@
 * the sampled MI histogram cells against the oracle's f64 PDF (north star: 1e-5);
 * the nearest-voxel rule's round-half-up ties (mask.rs:99-111);
 * a warp through a prior field against its closed form (cpu/mod.rs:60-86);
-* the device centre of gravity of a blob (init.rs:107-140).
+* the device centre of gravity of a blob (init.rs:107-140);
+* the wait timeout (device.rs:919-928), forced from the host.
 """
 from __future__ import annotations
 
@@ -116,3 +117,26 @@ def test_device_centre_of_gravity_of_a_blob_is_its_centre(backend):
     g = Grid.make((40, 36, 32), (1.0, 1.5, 2.0), (-20.0, -30.0, -10.0))
     c = backend.centre_of_gravity(backend.upload_image(g, blob(g, (3.0, -4.0, 5.0))))
     assert np.max(np.abs(np.asarray(c) - np.array([3.0, -4.0, 5.0]))) < 1e-3, c
+
+
+def test_a_wait_past_the_wait_timeout_fails_the_call():
+    """device.rs:919-928: FASTWARP_GPU_WAIT_TIMEOUT_S bounds every wait on the device; with a
+    nanosecond the first wait of a 1 mm CC pair gives up (its kernels still finish normally and
+    the context's destruction waits for them), and the error names the setting."""
+    import os
+
+    from fastwarp_amd.backend import HipBackend, RegError
+
+    g = cases.grid((182, 218, 182))
+    f, m = cases.image(g, 1), cases.image(g, 2)
+    os.environ["FASTWARP_GPU_WAIT_TIMEOUT_S"] = "1e-9"
+    try:
+        b = HipBackend(0)
+    finally:
+        del os.environ["FASTWARP_GPU_WAIT_TIMEOUT_S"]
+    try:
+        # the uploads wait for their copies too: whichever wait comes first gives up
+        with pytest.raises(RegError, match="FASTWARP_GPU_WAIT_TIMEOUT_S"):
+            b.cc_value_derivative_pair(b.upload_image(g, f), b.upload_image(g, m), 4)
+    finally:
+        b.close()
