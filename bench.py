@@ -4,8 +4,9 @@ A step is one full 1 mm normalisation (centre of mass, Rigid and Affine Mattes M
 with the cohort schedule 8x4x2x1) of the seeded 182x218x182 phantom pair. W untimed warmup steps,
 then K timed steps; prints one JSON line with registrations/hour, per-step latency, the roofline of
 the dominant kernel (HIP events carried by every one of its launches in the timed region), the
-per-kernel breakdown of the last warmup step, the served stream, and one complete registration
-on the f64 CPU engine as a baseline (the 2 mm one by default).
+per-kernel breakdown of the last warmup step, the served stream (256 mixed requests with Poisson
+arrivals at 80 % load against serial plain execution of the same mix), and one complete
+registration on the f64 CPU engine as a baseline (the 2 mm one by default).
 """
 from __future__ import annotations
 
@@ -95,35 +96,44 @@ def cpu_baseline(config: str) -> dict:
     }
 
 
-def served(pair, plan, slots: int, requests: int) -> dict:
-    """The same workload as a served stream (fw_server_*): `requests` registrations submitted at
-    once to `slots` slots, f32 volumes in from host memory and both fields back to host memory per
-    request, against the same requests through a single slot (one at a time, identical work)."""
+def served(slots: int, requests: int, load: float) -> dict:
+    """The served-stream workload: `requests` registrations (70 % full normalisations at 2, 1.5 or
+    1 mm with one or two channels, 30 % rigid+affine only; f32 volumes from host memory, fields back
+    to host memory) arriving as a Poisson stream at `load` times the rate the server saturates at
+    on the same mix, through `slots` slots; against serial plain execution of the same mix (one
+    slot, one request at a time, queued back to back) on a prefix of it."""
     from fastwarp_amd.serve import Server
+    from fastwarp_amd.serve_stream import make_mix, pairs_for, run_stream
 
-    f = [np.asarray(v, dtype=np.float32) for v in pair.fixed]
-    m = [np.asarray(v, dtype=np.float32) for v in pair.moving]
-
-    def stream(n_slots: int, n: int):
-        with Server(slots=n_slots) as s:
-            for t in [s.submit(plan, pair.grid, f, pair.grid, m) for _ in range(n_slots)]:  # every slot warm
-                s.wait(t)
-            t0 = time.perf_counter()
-            got = [s.wait(t) for t in [s.submit(plan, pair.grid, f, pair.grid, m) for _ in range(n)]]
-            return time.perf_counter() - t0, got
-
-    one_s, _ = stream(1, max(2, requests // 3))
-    one = max(2, requests // 3) / one_s * 3600.0
-    wall, got = stream(slots, requests)
-    lat = np.array([g.timing.total_s for g in got])
-    value = requests / wall * 3600.0
-    log(f"served: {slots} slots, {requests} requests in {wall:.2f} s ({value:.0f}/h; one slot {one:.0f}/h)")
-    return {"value": round(value, 3), "unit": "registrations/hour", "slots": slots, "requests": requests,
+    mix = make_mix(requests)
+    pairs = pairs_for(mix)
+    prefix = mix[: max(16, requests // 8)]
+    with Server(slots=1) as one:
+        run_stream(one, sorted(set(mix)), pairs, None)  # warm every kind of request
+        serial_wall, _, serial_fail = run_stream(one, prefix, pairs, None)
+    serial = len(prefix) / serial_wall
+    with Server(slots=slots) as s:
+        run_stream(s, sorted(set(mix)) * slots, pairs, None)
+        sat_wall, _, sat_fail = run_stream(s, prefix, pairs, None)
+        saturated = len(prefix) / sat_wall
+        wall, lat, fail = run_stream(s, mix, pairs, load * saturated)
+    lat = np.array(lat)
+    value = requests / wall
+    failures = serial_fail + sat_fail + fail
+    log(f"served: {slots} slots, {requests} Poisson requests at {load:.0%} load in {wall:.1f} s "
+        f"({3600 * value:.0f}/h; serial plain {3600 * serial:.0f}/h; saturated {3600 * saturated:.0f}/h)")
+    kinds = {}
+    for spacing, channels, full in mix:
+        key = f"{'full' if full else 'linear'} {spacing} mm x{channels}"
+        kinds[key] = kinds.get(key, 0) + 1
+    return {"value": round(3600.0 * value, 3), "unit": "registrations/hour", "slots": slots, "requests": requests,
+            "load": load, "arrival_per_hour": round(3600.0 * load * saturated, 1),
+            "saturated_per_hour": round(3600.0 * saturated, 1),
+            "serial_plain_per_hour": round(3600.0 * serial, 1), "vs_serial_plain": round(value / serial, 3),
             "latency_s": {q: round(float(np.percentile(lat, int(q[1:]))), 3) for q in ("p50", "p95", "p99")},
-            "run_s_mean": round(float(np.mean([g.timing.run_s for g in got])), 3),
-            "one_slot_value": round(one, 3), "vs_one_slot": round(value / one, 3),
-            "path": "fw_server: requests submitted at once, admitted by HBM footprint to slots with their own HIP streams, "
-                    "f32 volumes uploaded per request, both fields downloaded to the host per request"}
+            "mix": dict(sorted(kinds.items())), "failed": len(failures),
+            "path": "fw_server: Poisson arrivals, admitted by HBM footprint to slots with their own HIP streams, "
+                    "f32 volumes uploaded per request, fields downloaded to the host per full request"}
 
 
 def kernels(repeats: int) -> int:
@@ -198,7 +208,8 @@ def main() -> int:
     ap.add_argument("--config", default="1mm", choices=["1mm", "2mm"])
     ap.add_argument("--kernels", action="store_true", help="time each hot-path kernel at 1 mm instead")
     ap.add_argument("--slots", type=int, default=4, help="slots of the served stream (0 skips it)")
-    ap.add_argument("--requests", type=int, default=12, help="requests of the served stream")
+    ap.add_argument("--requests", type=int, default=256, help="requests of the served stream")
+    ap.add_argument("--load", type=float, default=0.8, help="arrival rate of the served stream as a share of saturation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for {args.gpus}): registrations are independent and would shard across "
@@ -312,7 +323,7 @@ def main() -> int:
     if args.slots > 0:
         backend.release_idle_buffers()
         try:
-            line["served"] = served(pair, plan, args.slots, args.requests)
+            line["served"] = served(args.slots, args.requests, args.load)
         except Exception as e:  # the plain line is still reported
             log(f"served stream failed: {e}")
             line["served"] = {"error": str(e)}
