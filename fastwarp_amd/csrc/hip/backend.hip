@@ -705,7 +705,7 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
   return out;
 }
 
-Pending<std::pair<double, double>> HipBackend::scale_to_step(HField& a, HField& b, double step) {
+std::pair<Pending<std::pair<double, double>>, DevBuf> HipBackend::step_factors(const HField& a, const HField& b, double step) {
   if (a.grid != b.grid) invalid("hip backend", "update pair: grids differ");
   const long long n = long(a.grid.num_voxels());
   const int nb = norm_blocks(n);
@@ -722,13 +722,20 @@ Pending<std::pair<double, double>> HipBackend::scale_to_step(HField& a, HField& 
   auto* ring = static_cast<float*>(ctx_->mapped(3, kMaximaSlots * 2 * sizeof(float), &ring_dev));
   const size_t slot = ctx_->next_tag() % kMaximaSlots;
   launch_step_factors(pm, nb, 2, step, factors.get(), static_cast<float*>(ring_dev) + 2 * slot, s);
+  volatile float* host = ring + 2 * slot;
+  return {Pending<std::pair<double, double>>(
+              std::function<std::pair<double, double>()>([host] { return std::make_pair(double(host[0]), double(host[1])); })),
+          std::move(factors)};
+}
+
+Pending<std::pair<double, double>> HipBackend::scale_to_step(HField& a, HField& b, double step) {
+  auto r = step_factors(a, b, step);
+  const long long n = long(a.grid.num_voxels());
   {
     Timed t(*ctx_, "k_scale_by", 48.0 * double(n));
-    launch_scale_by2(a.buf.get(), b.buf.get(), 3 * n, factors.get(), s);
+    launch_scale_by2(a.buf.get(), b.buf.get(), 3 * n, r.second.get(), ctx_->stream());
   }
-  volatile float* host = ring + 2 * slot;
-  return Pending<std::pair<double, double>>(
-      std::function<std::pair<double, double>()>([host] { return std::make_pair(double(host[0]), double(host[1])); }));
+  return std::move(r.first);
 }
 
 HImage HipBackend::smooth_image(const HImage& src, double sigma, SigmaUnits units) {
@@ -1102,7 +1109,20 @@ std::pair<HField, HField> HipBackend::compose_pair(const HField& u0, const HFiel
   put(l, g.index_from_offset);
   Timed t(*ctx_, "k_compose", 72.0 * double(g.num_voxels()));
   launch_compose2(u0.buf.get(), w0.buf.get(), out0.buf.get(), u1.buf.get(), w1.buf.get(), out1.buf.get(), dims_of(g), l,
-                  ctx_->stream());
+                  nullptr, ctx_->stream());
+  return {std::move(out0), std::move(out1)};
+}
+
+std::pair<HField, HField> HipBackend::compose_pair_scaled(const HField& u0, const HField& w0, const HField& u1, const HField& w1,
+                                                          const DevBuf& factors) {
+  if (u0.grid != w0.grid || u1.grid != w1.grid || w0.grid != w1.grid) invalid("compose", "the pair's grids differ");
+  const Grid& g = w0.grid;
+  HField out0{g, DevBuf(ctx_.get(), 3 * g.num_voxels())}, out1{g, DevBuf(ctx_.get(), 3 * g.num_voxels())};
+  float l[9];
+  put(l, g.index_from_offset);
+  Timed t(*ctx_, "k_compose", 72.0 * double(g.num_voxels()));
+  launch_compose2(u0.buf.get(), w0.buf.get(), out0.buf.get(), u1.buf.get(), w1.buf.get(), out1.buf.get(), dims_of(g), l,
+                  factors.get(), ctx_->stream());
   return {std::move(out0), std::move(out1)};
 }
 
@@ -1140,7 +1160,7 @@ std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField
   }
   // At most kAhead launches are queued beyond the one the device has started: the host waits on
   // the progress word (a hint; a bounded wait) so that it does not queue launches past the stop.
-  constexpr int kAhead = 3;
+  constexpr int kAhead = 1;
   const auto started = [&]() -> int {
     const uint32_t w = mail[1];
     return (w >> 8) == (tag & 0xffffffu) ? int(w & 0xffu) : 0;

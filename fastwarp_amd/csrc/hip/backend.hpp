@@ -305,6 +305,12 @@ class HipBackend {
   // Scales both updates so each one's largest scaled voxel norm is `step` (step / max, or step when
   // the maximum is not positive), on the device; the two maxima come back at settle().
   Pending<std::pair<double, double>> scale_to_step(HField& a, HField& b, double step);
+  // The same factors without scaling the fields (f32 factors on the device, the maxima at
+  // settle()), for compose_pair_scaled to apply while it interpolates the updates.
+  std::pair<Pending<std::pair<double, double>>, DevBuf> step_factors(const HField& a, const HField& b, double step);
+  // compose_pair with the updates u0, u1 multiplied by factors[0], factors[1] (device).
+  std::pair<HField, HField> compose_pair_scaled(const HField& u0, const HField& w0, const HField& u1, const HField& w1,
+                                                const DevBuf& factors);
   std::pair<HField, Pending<InvertStats>> invert_deferred(const HField& field, HField initial, const InvertParams& p);
   void settle() { ctx_->settle(); }
   HField resample_field(const HField& field, const Grid& grid);

@@ -891,32 +891,19 @@ void launch_resample(const ResampleArgs& a, hipStream_t s) {
 // Compose (field_ops.rs:251-272, ITK ComposeDisplacementFieldsImageFilter): out = w + u(x + w),
 // u interpolated on the shared grid at the continuous index (i, j, k) + l * w, zero outside.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void compose_at(const float* __restrict__ u, const Dims& d, const float* l, int i, int j,
-                                           int k, float wx, float wy, float wz, float& ox, float& oy, float& oz) {
-  const float cx = (float)i + (l[0] * wx + l[1] * wy + l[2] * wz);
-  const float cy = (float)j + (l[3] * wx + l[4] * wy + l[5] * wz);
-  const float cz = (float)k + (l[6] * wx + l[7] * wy + l[8] * wz);
-  ox = wx;
-  oy = wy;
-  oz = wz;
-  if (inside(d, cx, cy, cz)) {
-    float ux, uy, uz;
-    tri3(u, d, cx, cy, cz, ux, uy, uz);
-    ox += ux;
-    oy += uy;
-    oz += uz;
-  }
-}
 
 struct L9 {
   float v[9];
 };
 
 // Planes [0, nz) of the grid compose (u, w) into out; with a second triple, planes [nz, 2 nz)
-// compose (u1, w1) into out1.
+// compose (u1, w1) into out1. With `factors` (device, one per pair), u is the update before its
+// scaling: the interpolated update is multiplied by the pair's factor (the SyN loop's step / max,
+// so that the scaled update is never stored).
 __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u0, const float* __restrict__ w0, Dims d, L9 l,
                                                  float* __restrict__ out0, const float* __restrict__ u1,
-                                                 const float* __restrict__ w1, float* __restrict__ out1) {
+                                                 const float* __restrict__ w1, float* __restrict__ out1,
+                                                 const float* __restrict__ factors) {
   const uint3 bl = xcd_block();
   const int i = bl.x * 64 + (threadIdx.x & 63);
   const int j = bl.y * 4 + (threadIdx.x >> 6);
@@ -927,8 +914,24 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u0, c
   float* out = second ? out1 : out0;
   if (i >= d.nx || j >= d.ny) return;
   const long long g = lin(d, i, j, k), n = d.n();
-  float ox, oy, oz;
-  compose_at(u, d, l.v, i, j, k, w[g], w[g + n], w[g + 2 * n], ox, oy, oz);
+  const float wx = w[g], wy = w[g + n], wz = w[g + 2 * n];
+  const float cx = (float)i + (l.v[0] * wx + l.v[1] * wy + l.v[2] * wz);
+  const float cy = (float)j + (l.v[3] * wx + l.v[4] * wy + l.v[5] * wz);
+  const float cz = (float)k + (l.v[6] * wx + l.v[7] * wy + l.v[8] * wz);
+  float ox = wx, oy = wy, oz = wz;
+  if (inside(d, cx, cy, cz)) {
+    float ux, uy, uz;
+    tri3(u, d, cx, cy, cz, ux, uy, uz);
+    if (factors) {
+      const float f = factors[second ? 1 : 0];
+      ux *= f;
+      uy *= f;
+      uz *= f;
+    }
+    ox += ux;
+    oy += uy;
+    oz += uz;
+  }
   out[g] = ox;
   out[g + n] = oy;
   out[g + 2 * n] = oz;
@@ -939,15 +942,15 @@ void launch_compose(const float* displacement, const float* warping, Dims d, con
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
   const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
   launch(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out, (const float*)nullptr, (const float*)nullptr,
-         (float*)nullptr);
+         (float*)nullptr, (const float*)nullptr);
 }
 
 void launch_compose2(const float* u0, const float* w0, float* out0, const float* u1, const float* w1, float* out1, Dims d,
-                     const float l[9], hipStream_t s) {
+                     const float l[9], const float* factors, hipStream_t s) {
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
   const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, 2 * d.nz);
-  launch(k_compose, grid, dim3(256), 0, s, u0, w0, d, lv, out0, u1, w1, out1);
+  launch(k_compose, grid, dim3(256), 0, s, u0, w0, d, lv, out0, u1, w1, out1, factors);
 }
 
 // ------------------------------------------------------------------------------------------

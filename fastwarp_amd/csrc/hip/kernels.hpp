@@ -144,7 +144,7 @@ void launch_resample_iso(const float* src, Dims sd, float* dst, Dims od, const i
 void launch_compose(const float* displacement, const float* warping, Dims d, const float l[9], float* out, hipStream_t s);
 // Two compositions on one grid in one launch.
 void launch_compose2(const float* u0, const float* w0, float* out0, const float* u1, const float* w1, float* out1, Dims d,
-                     const float l[9], hipStream_t s);
+                     const float l[9], const float* factors, hipStream_t s);
 
 // --- Gaussian smoothing, separable, clamped edges; final pass zeroes faces and blends ---
 void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const float* taps_dev, int radius,

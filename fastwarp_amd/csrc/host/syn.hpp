@@ -107,6 +107,13 @@ std::pair<typename B::FieldH, typename B::FieldH> compose_both(B& b, const typen
   else return {b.compose(u0, w0), b.compose(u1, w1)};
 }
 
+// Backends that scale the two updates while composing them (step_factors computes step / max on
+// the device without scaling, compose_pair_scaled applies the factors as it interpolates).
+template <typename B, typename = void>
+struct has_scaled_compose : std::false_type {};
+template <typename B>
+struct has_scaled_compose<B, std::void_t<decltype(&B::step_factors), decltype(&B::compose_pair_scaled)>> : std::true_type {};
+
 template <typename B>
 std::pair<typename B::FieldH, typename B::FieldH> smooth_both(B& b, typename B::FieldH f0, typename B::FieldH f1, double variance) {
   if constexpr (has_pair_ops<B>::value) return b.smooth_field_pair(std::move(f0), std::move(f1), variance);
@@ -227,14 +234,22 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
         grads[0] = std::move(smoothed.first);
         grads[1] = std::move(smoothed.second);
       }
-      // 3. scale each update so its largest voxel step is the gradient step
-      Pending<std::pair<double, double>> maxima = b.scale_to_step(grads[0], grads[1], recipe.gradient_step);
-      // 4-5. compose into the total fields and re-estimate both inverses (syn.rs:422-445)
+      // 3. scale each update so its largest voxel step is the gradient step, and 4-5. compose
+      // into the total fields and re-estimate both inverses (syn.rs:422-445)
       SynIteration it;
       std::optional<FieldH>* fields[2] = {&f2m, &m2m};
       std::optional<FieldH>* inverses[2] = {&f2m_inv, &m2m_inv};
       Pending<InvertStats> stats[4];
-      auto composed = detail::compose_both(b, grads[0], **fields[0], grads[1], **fields[1]);
+      Pending<std::pair<double, double>> maxima;
+      std::pair<FieldH, FieldH> composed;
+      if constexpr (detail::has_scaled_compose<B>::value) {
+        auto factors = b.step_factors(grads[0], grads[1], recipe.gradient_step);
+        maxima = std::move(factors.first);
+        composed = b.compose_pair_scaled(grads[0], **fields[0], grads[1], **fields[1], factors.second);
+      } else {
+        maxima = b.scale_to_step(grads[0], grads[1], recipe.gradient_step);
+        composed = detail::compose_both(b, grads[0], **fields[0], grads[1], **fields[1]);
+      }
       fields[0]->reset();
       fields[1]->reset();
       auto totals = detail::smooth_both(b, std::move(composed.first), std::move(composed.second), recipe.total_field_variance);
