@@ -57,6 +57,7 @@ _SIGNATURES = [
     ("fw_field_copy", C.c_int, [_P, _P, _PP]),
     ("fw_field_free", None, [_P, _P]),
     ("fw_warp", C.c_int, [_P, _P, _P, _P, C.POINTER(Affine), _PP]),
+    ("fw_set_tuning", C.c_int, [_P, C.c_char_p, C.c_int64]),
     ("fw_warp_mask", C.c_int, [_P, _P, _P, _P, C.POINTER(Affine), _P, _PP]),
     ("fw_cc_pair", C.c_int, [_P, _P, _P, _P, C.c_uint32, _D, _D, _PP, _PP]),
     ("fw_cc_pair_noalloc", C.c_int, [C.POINTER(Grid), _P, _P, C.c_uint32, _P, _P, _P, _P]),
@@ -276,6 +277,10 @@ class HipBackend:
         return DeviceField(self, out)
 
     # --- admission (capacity.rs, fits_gpu / registrations_at_once) ---
+    def set_tuning(self, key: str, value: int):
+        """One launch-shape choice of this context (fw_set_tuning); value < 0 removes it."""
+        self._check(lib().fw_set_tuning(self.ctx, key.encode(), int(value)))
+
     def memory_budget(self) -> int:
         b = C.c_uint64()
         self._check(lib().fw_memory_budget(self.ctx, C.byref(b)))

@@ -122,6 +122,17 @@ int fw_release_idle(fw_ctx* ctx) {
   return guard(ctx, [&] { ctx->backend.release_idle_buffers(); });
 }
 
+int fw_set_tuning(fw_ctx* ctx, const char* key, int64_t value) {
+  return guard(ctx, [&] {
+    need(key, "key");
+    const std::string k(key);
+    const std::string knob = k.substr(0, k.find('@'));
+    if (value >= 0 && !fw::TuningTable::valid(knob, value))
+      fw::invalid("tuning", "unknown knob or value out of range: " + k + " = " + std::to_string(value));
+    ctx->backend.ctx().tuning().set(k, value);
+  });
+}
+
 int fw_memory_stats(fw_ctx* ctx, uint64_t* held, uint64_t* peak) {
   return guard(ctx, [&] {
     if (held) *held = ctx->backend.ctx().held();

@@ -3,6 +3,8 @@
 // composed in f64 on the host into continuous-index maps and narrowed to f32 once per launch.
 #include "backend.hpp"
 
+#include <dlfcn.h>
+
 #include <array>
 #include <cmath>
 #include <cstdlib>
@@ -20,7 +22,26 @@ void check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw Error("hip", std::string(what) + ": " + hipGetErrorString(e));
 }
 
-DeviceContext::DeviceContext(int device) : device_(device) {
+const TuningTable& default_tuning() {
+  static const TuningTable table = [] {
+    TuningTable t;
+    if (const char* env = std::getenv("FASTWARP_TUNING")) {
+      t.load(env);
+      return t;
+    }
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void*>(&default_tuning), &info) && info.dli_fname) {
+      std::string dir(info.dli_fname);
+      const size_t slash = dir.rfind('/');
+      dir = slash == std::string::npos ? "." : dir.substr(0, slash);
+      t.load(dir + "/../tuning/gfx950.json");
+    }
+    return t;
+  }();
+  return table;
+}
+
+DeviceContext::DeviceContext(int device) : device_(device), tuning_(default_tuning()) {
   wait_timeout_s_ = wait_timeout_from(std::getenv("FASTWARP_GPU_WAIT_TIMEOUT_S"));
   check(hipSetDevice(device), "hipSetDevice");
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -570,7 +591,8 @@ MetricPair<HField> HipBackend::cc_pair(const HImage& fixed, const HImage& moving
   a.valid = valid ? valid->buf.get() : nullptr;
   a.d = dims_of(g);
   a.radius = int(radius);
-  a.tile_z = cc_tile_for(a.d, cc_tile_z);
+  a.tile_z = cc_tile_for(a.d, int(ctx_->tuning().get("cc.tile_z", n, cc_tile_z)),
+                         int(ctx_->tuning().get("cc.min_blocks", n, 512)));
   a.scale_f = cc_split_scale(fixed.bound);
   a.scale_m = cc_split_scale(moving.bound);
   a.step_f = 1.0f / a.scale_f;
@@ -649,7 +671,8 @@ MetricPair<HField> HipBackend::cc_pair_deferred(const HImage& fixed, const HImag
   a.valid = nullptr;
   a.d = dims_of(g);
   a.radius = int(radius);
-  a.tile_z = cc_tile_for(a.d, cc_tile_z);
+  a.tile_z = cc_tile_for(a.d, int(ctx_->tuning().get("cc.tile_z", n, cc_tile_z)),
+                         int(ctx_->tuning().get("cc.min_blocks", n, 512)));
   a.scale_f = cc_split_scale(fixed.bound);
   a.scale_m = cc_split_scale(moving.bound);
   a.step_f = 1.0f / a.scale_f;
@@ -942,7 +965,7 @@ std::pair<HField, HField> HipBackend::smooth_field_pair(HField a, HField b, doub
   }
   if (!gauss3_fits(d, t)) return {smooth_field(std::move(a), variance), smooth_field(std::move(b), variance)};
   HField out0{g, DevBuf(ctx_.get(), 3 * n)}, out1{g, DevBuf(ctx_.get(), 3 * n)};
-  if (t.rx == t.ry && t.ry == t.rz && gauss3_pays(d)) {
+  if (t.rx == t.ry && t.ry == t.rz && ctx_->tuning().get("gauss.one_pass", uint64_t(d.n()), gauss3_pays(d) ? 1 : 0)) {
     // both fields read once (twice with a blend), both results written
     Timed tm(*ctx_, "k_gauss3", 48.0 * double(n) + (w2 != 0.0 ? 24.0 * double(n) : 0.0));
     if (!launch_gauss3(a.buf.get(), out0.buf.get(), d, t, float(w1), float(w2), ctx_->stream(), b.buf.get(), out1.buf.get()))
@@ -985,7 +1008,7 @@ HField HipBackend::smooth_field(HField field, double variance) {
     }
     if (fits && gauss3_fits(d, t)) {
       HField out{g, DevBuf(ctx_.get(), 3 * n)};
-      if (t.rx == t.ry && t.ry == t.rz && gauss3_pays(d)) {
+      if (t.rx == t.ry && t.ry == t.rz && ctx_->tuning().get("gauss.one_pass", uint64_t(d.n()), gauss3_pays(d) ? 1 : 0)) {
         // u read once (twice with a blend), the result written
         Timed tm(*ctx_, "k_gauss3", 24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0));
         if (!launch_gauss3(field.buf.get(), out.buf.get(), d, t, float(w1), float(w2), ctx_->stream()))
@@ -1136,7 +1159,7 @@ std::pair<HField, Pending<InvertStats>> HipBackend::invert_deferred(const HField
   float l[9];
   put(l, g.index_from_offset);
   const float3 isp = inv_spacing(g);
-  const int nb = inv_blocks(d);
+  const int nb = inv_blocks(d, int(ctx_->tuning().get("inv.max_blocks", uint64_t(d.n()), 2048)));
   auto* state = static_cast<InvertState*>(ctx_->scratch(4, sizeof(InvertState)));
   auto* partials = static_cast<float*>(ctx_->scratch(5, 4 * size_t(nb) * sizeof(float)));
   hipStream_t s = ctx_->stream();

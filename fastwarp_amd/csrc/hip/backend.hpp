@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../host/backend_types.hpp"
+#include "../host/tuning.hpp"
 #include "../host/grid.hpp"
 #include "../host/mi_host.hpp"
 #include "../host/pyramid.hpp"
@@ -70,6 +71,8 @@ class DeviceContext : public HostSpanSink {
   // How long a wait on the device may take before it fails (device.rs:919-928
   // FASTWARP_GPU_WAIT_TIMEOUT_S: positive finite seconds, 300 by default), read at creation.
   double wait_timeout_s() const { return wait_timeout_s_; }
+  // Launch-shape choices (the autotuner's table, loaded at creation; fw_set_tuning edits it).
+  TuningTable& tuning() { return tuning_; }
   // Device copy of the Gaussian taps of (variance, axis length), cached.
   const float* taps(double variance, size_t length, int& radius);
   // The same taps on the host (f64), cached.
@@ -98,6 +101,7 @@ class DeviceContext : public HostSpanSink {
  private:
   int device_;
   double wait_timeout_s_ = 300.0;
+  TuningTable tuning_;
   hipStream_t stream_{};
   std::map<size_t, std::vector<void*>> free_;
   uint64_t held_ = 0, peak_ = 0;
@@ -211,6 +215,10 @@ class DevBuf {
   float* p_ = nullptr;
   size_t bytes_ = 0;
 };
+
+// The launch-shape table every context starts from: the file FASTWARP_TUNING names, else
+// tuning/gfx950.json beside the library's directory (fastwarp_amd/tuning/ for lib/), else empty.
+const TuningTable& default_tuning();
 
 // A scalar image on the device (f32, x-fastest) with an upper bound of |value| (taken on the host
 // at upload and carried through the ops that cannot grow it: warps, smoothing, resampling). The

@@ -382,12 +382,12 @@ dim3 cc_grid(const Dims& d, int tile_z) {
 }
 
 // The z run of a CC block: the longest of max_tile, max_tile / 2, ..., 1 planes whose grid still
-// has 512 blocks (two per CU), so that a small level is not left to a few long-running blocks.
-int cc_tile_for(const Dims& d, int max_tile) {
+// has min_blocks blocks, so that a small level is not left to a few long-running blocks.
+int cc_tile_for(const Dims& d, int max_tile, int min_blocks) {
   int t = std::max(1, max_tile);
   for (; t > 1; t /= 2) {
     const dim3 g = cc_grid(d, t);
-    if (size_t(g.x) * g.y * g.z >= 512) break;
+    if (size_t(g.x) * g.y * g.z >= size_t(std::max(1, min_blocks))) break;
   }
   return t;
 }
@@ -1550,7 +1550,6 @@ void launch_gauss_axis(const float* in, float* out, Dims d, int axis, const floa
 // ------------------------------------------------------------------------------------------
 constexpr int kInvThreads = 256;
 constexpr int kInvVoxels = 4;
-constexpr int kInvMaxBlocks = 2048;
 
 // The row and plane divisors of a grid's linear index (grids below 2^31 voxels, which the
 // inversion's caller checks).
@@ -1751,9 +1750,10 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
   if (next) inv_write_partials(s, m, out, blocks, rg.chunk);
 }
 
-int inv_blocks(const Dims& d) {
+int inv_blocks(const Dims& d, int cap) {
   const long long b = (d.n() + kInvThreads * kInvVoxels - 1) / (kInvThreads * kInvVoxels);
-  return (int)(b < 1 ? 1 : (b > kInvMaxBlocks ? kInvMaxBlocks : b));
+  const long long c = cap < 1 ? 1 : cap;
+  return (int)(b < 1 ? 1 : (b > c ? c : b));
 }
 
 void launch_inv_begin(const float* field, const float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,

@@ -72,8 +72,8 @@ struct CcArgs {
 };
 void launch_cc_pair(const CcArgs& a, hipStream_t s);
 dim3 cc_grid(const Dims& d, int tile_z);
-// The longest z run of at most max_tile planes (halving) whose grid still has 512 blocks.
-int cc_tile_for(const Dims& d, int max_tile);
+// The longest z run of at most max_tile planes (halving) whose grid still has min_blocks blocks.
+int cc_tile_for(const Dims& d, int max_tile, int min_blocks = 512);
 constexpr int kCcTileZ = 16;  // the default longest z run
 
 // --- per-block partial reductions ---
@@ -199,7 +199,7 @@ void launch_inv_begin(const float* field, const float* psi, float* residual, Dim
 void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,
                      InvertState* state, InvertReport* report, float* partials, int blocks, int step, int max_iterations,
                      double max_tol, double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s);
-int inv_blocks(const Dims& d);
+int inv_blocks(const Dims& d, int cap = 2048);
 
 // --- recursive Gaussian (Deriche, 4th order) along one axis, constant extension at both ends ---
 struct DericheCoefficients {

@@ -90,7 +90,8 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
   a.bins = int(binning_.bins);
   const size_t bins = binning_.bins;
   const size_t cells = bins * bins + bins + 1;
-  const int blocks = mi_sampled_blocks(int(count_));
+  const int blocks = mi_sampled_blocks(int(count_), int(ctx_->tuning().get("mi.max_blocks", count_, 512)));
+  a.threads = int(ctx_->tuning().get("mi.threads", count_, 0));
   a.hist = static_cast<unsigned long long*>(ctx_->scratch(0, cells * sizeof(unsigned long long)));
   a.hist_slots = static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * cells * sizeof(unsigned long long)));
   hipStream_t s = ctx_->stream();
@@ -168,7 +169,8 @@ void MiLevel::batch_values(const LinearMovingTransform* ts, size_t k, std::vecto
   const size_t bins = binning_.bins;
   const size_t cells = bins * bins + bins + 1;
   const size_t all = k * cells;
-  const int blocks = mi_sampled_blocks(int(count_));
+  const int blocks = mi_sampled_blocks(int(count_), int(ctx_->tuning().get("mi.max_blocks", count_, 512)));
+  a.threads = int(ctx_->tuning().get("mi.threads", count_, 0));
   a.hist_slots = static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * all * sizeof(unsigned long long)));
   hipStream_t s = ctx_->stream();
   {
@@ -215,7 +217,7 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
   a.kind = t.optimised.kind == ParamKind::Affine ? 1 : 0;
   if (t.optimised.kind == ParamKind::Euler3d && t.optimised.zyx)
     invalid("hip backend", "the ZYX Euler order is not supported by the device derivative");
-  const int blocks = mi_deriv_blocks(int(count_));
+  const int blocks = mi_deriv_blocks(int(count_), int(ctx_->tuning().get("mi.deriv_blocks", count_, kMiDerivMaxBlocks)));
   const size_t np = t.optimised.count();
   a.partial = static_cast<double*>(ctx_->scratch(7, size_t(blocks) * 12 * sizeof(double)));
   {

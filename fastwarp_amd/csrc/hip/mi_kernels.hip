@@ -178,9 +178,9 @@ __device__ __forceinline__ SampleEval eval_sample(const MiSampledArgs& a, float4
 
 // One sample per thread while that gives at most 512 blocks (the small levels: their launches are
 // one chain of dependent loads per thread, so the chain is kept short), several per thread beyond.
-int mi_sampled_blocks(int count) {
+int mi_sampled_blocks(int count, int cap) {
   int b = (count + 255) / 256;
-  return b < 1 ? 1 : (b > 512 ? 512 : b);
+  return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
 #ifndef FW_MI_WIDE_THREADS
@@ -194,9 +194,10 @@ int mi_sampled_blocks(int count) {
 int mi_sampled_threads(int count) { return count > 2 * 512 * 256 ? FW_MI_WIDE_THREADS : 256; }
 
 // The derivative kernel's partials are reduced on the device, so its grid may be wider.
-int mi_deriv_blocks(int count) {
+int mi_deriv_blocks(int count, int cap) {
   int b = (count + 255) / 256;
-  return b < 1 ? 1 : (b > kMiDerivMaxBlocks ? kMiDerivMaxBlocks : b);
+  const int c = cap < kMiDerivMaxBlocks ? cap : kMiDerivMaxBlocks;
+  return b < 1 ? 1 : (b > c ? c : b);
 }
 
 // The block's histogram (LDS, 64-bit atomics) is written to its own slot of `hist_slots`; a second
@@ -281,7 +282,7 @@ int mi_batch_capacity(int bins) {
 
 void launch_mi_sampled_hist_batch(const MiSampledArgs& a, const MiBatch& batch, int count, int blocks, hipStream_t s) {
   const size_t lds = size_t(count) * size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
-  const dim3 t(mi_sampled_threads(a.count));
+  const dim3 t(a.threads > 0 ? a.threads : mi_sampled_threads(a.count));
   switch (count) {
     case 2: launch(k_mi_sampled_hist_batch<2>, dim3(blocks), t, (uint32_t)lds, s, a, batch); break;
     case 3: launch(k_mi_sampled_hist_batch<3>, dim3(blocks), t, (uint32_t)lds, s, a, batch); break;
@@ -292,7 +293,7 @@ void launch_mi_sampled_hist_batch(const MiSampledArgs& a, const MiBatch& batch, 
 
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s) {
   const size_t lds = size_t(a.bins * a.bins + a.bins + 1) * sizeof(unsigned long long);
-  launch(k_mi_sampled_hist, dim3(blocks), dim3(mi_sampled_threads(a.count)), lds, s, a);
+  launch(k_mi_sampled_hist, dim3(blocks), dim3(a.threads > 0 ? a.threads : mi_sampled_threads(a.count)), lds, s, a);
 }
 
 // hist[c] = sum of the slots' cell c, for every cell and the valid count. A block of 16 waves

@@ -35,6 +35,7 @@ struct MiSampledArgs {
   float trig[6];          // cx sx cy sy cz sz of the Euler angles
   int kind;               // 0 Euler3D (Rz Rx Ry), 1 affine
   double* partial;        // per block: 12 doubles
+  int threads;            // threads per block of the histogram passes (0: mi_sampled_threads)
 };
 void launch_mi_sampled_hist(const MiSampledArgs& a, int blocks, hipStream_t s);
 // Several transforms over the same samples in one pass (a.a / a.b unused): transform k maps the
@@ -54,10 +55,10 @@ int mi_sum_slots_blocks(int cells);
 void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, uint32_t* done,
                          uint32_t tag, hipStream_t s);
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
-int mi_sampled_blocks(int count);
+int mi_sampled_blocks(int count, int cap = 512);
 int mi_sampled_threads(int count);
 constexpr int kMiDerivMaxBlocks = 2048;
-int mi_deriv_blocks(int count);
+int mi_deriv_blocks(int count, int cap = kMiDerivMaxBlocks);
 // out[k] (k < np) = the blocks' derivative partials added in a fixed order; then done[0] = tag
 // (system-scope release).
 void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, uint32_t* done, uint32_t tag,

@@ -19,6 +19,7 @@
 #include "../host/grid.hpp"
 #include "device_common.hpp"
 #include "geometry.hpp"
+#include "backend.hpp"
 #include "kernels.hpp"
 
 namespace fw {
@@ -134,7 +135,10 @@ extern "C" int fw_cc_pair_noalloc(const fw_grid* grid, const float* fixed, const
     a.valid = nullptr;
     a.d = dims_of(g);
     a.radius = int(radius);
-    a.tile_z = cc_tile_for(a.d, kCcTileZ);
+    // the same z runs as fw_cc_pair on a context with the default table, so the same bits
+    const uint64_t nv = uint64_t(a.d.n());
+    a.tile_z = cc_tile_for(a.d, int(default_tuning().get("cc.tile_z", nv, kCcTileZ)),
+                           int(default_tuning().get("cc.min_blocks", nv, 512)));
     const dim3 gr = cc_grid(a.d, a.tile_z);
     const long long blocks = (long long)gr.x * gr.y * gr.z;
     if (blocks > kNoAllocMaxBlocks)

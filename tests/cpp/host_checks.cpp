@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <random>
 #include <thread>
 
@@ -17,6 +18,7 @@
 #include "../../fastwarp_amd/csrc/host/mi_host.hpp"
 #include "../../fastwarp_amd/csrc/host/preprocess.hpp"
 #include "../../fastwarp_amd/csrc/host/request_loop.hpp"
+#include "../../fastwarp_amd/csrc/host/tuning.hpp"
 #include "../../oracle/oracle.hpp"
 
 using namespace fw;
@@ -136,6 +138,43 @@ static void wait_timeout() {
   CHECK(wait_timeout_from("soon") == 300.0, "timeout text");
   CHECK(wait_timeout_from("1e-9") == 1e-9, "timeout tiny");
   ok("wait_timeout");
+}
+
+// The tuning table: size classes, class entries over plain ones over the fallback, the file format
+// (the autotuner's notes and invalid values skipped, a malformed file rejected whole).
+static void tuning() {
+  CHECK(TuningTable::size_class(1) == 0 && TuningTable::size_class(2) == 1 && TuningTable::size_class(3) == 1, "class small");
+  CHECK(TuningTable::size_class(182ull * 218 * 182) == 22 && TuningTable::size_class(1ull << 22) == 22, "class 1 mm");
+  TuningTable t;
+  CHECK(t.get("cc.tile_z", 1000, 16) == 16, "fallback");
+  t.set("cc.tile_z", 8);
+  t.set("cc.tile_z@9", 32);
+  CHECK(t.get("cc.tile_z", 1000, 16) == 32 && t.get("cc.tile_z", 1023, 16) == 32, "class entry");
+  CHECK(t.get("cc.tile_z", 1024, 16) == 8, "plain entry");
+  t.set("cc.tile_z", -1);
+  CHECK(t.get("cc.tile_z", 1024, 16) == 16 && t.size() == 1, "removed");
+  CHECK(TuningTable::valid("mi.threads", 512) && !TuningTable::valid("mi.threads", 384), "mi threads");
+  CHECK(TuningTable::valid("gauss.one_pass", 1) && !TuningTable::valid("gauss.one_pass", 2), "one pass");
+  CHECK(!TuningTable::valid("cc.tile_z", 0) && !TuningTable::valid("nonsense", 1), "unknown knob");
+  const char* path = "/tmp/fw_tuning_check.json";
+  {
+    std::ofstream f(path);
+    f << "{\n \"_command\": \"python -m x\", \"_seconds\": 1.5, \"_list\": [1, {\"a\": 2}],\n"
+         " \"cc.tile_z@22\": 32, \"mi.threads@20\": 384, \"inv.max_blocks\": 4096, \"bogus@3\": 5\n}\n";
+  }
+  TuningTable u;
+  CHECK(u.load(path), "load");
+  CHECK(u.size() == 2, "entries kept");
+  CHECK(u.get("cc.tile_z", 182ull * 218 * 182, 16) == 32 && u.get("mi.threads", 1u << 20, 0) == 0, "loaded values");
+  CHECK(u.get("inv.max_blocks", 5, 2048) == 4096, "loaded plain");
+  {
+    std::ofstream f(path);
+    f << "[1, 2]";
+  }
+  CHECK(!u.load(path) && u.size() == 2, "malformed kept the old table");
+  CHECK(!u.load("/nonexistent/tuning.json"), "missing file");
+  std::remove(path);
+  ok("tuning");
 }
 
 // convergence.rs:188-213: the closed form of the two-control-point fit.
@@ -768,6 +807,7 @@ int main() {
   deriche();
   convergence();
   wait_timeout();
+  tuning();
   optimizer();
   speculative_search();
   jacobians();
