@@ -7,6 +7,7 @@
 #include <string>
 
 #include "hip/backend.hpp"
+#include "hip/batch_syn.hpp"
 #include "host/capacity.hpp"
 #include "host/itk_mat.hpp"
 #include "plan.hpp"
@@ -671,6 +672,59 @@ int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const
     if (forward) *forward = fwd ? new fw_field{std::move(*fwd)} : nullptr;
     if (inverse) *inverse = inv ? new fw_field{std::move(*inv)} : nullptr;
   });
+}
+
+namespace {
+// The per-subject records of the calling thread's last fw_register_batch.
+std::vector<std::vector<fw_trace_step>>& last_batch_traces() {
+  static thread_local std::vector<std::vector<fw_trace_step>> t;
+  return t;
+}
+}  // namespace
+
+int fw_register_batch(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, uint32_t subjects,
+                      const fw_image* const* moving, fw_field** forward, fw_field** inverse, fw_result* results,
+                      const volatile int32_t* cancel) {
+  return guard(ctx, [&] {
+    need(plan, "plan");
+    if (!fixed || !moving) fw::invalid("register", "null image list");
+    if (subjects == 0) fw::invalid("register batch", "no subjects");
+    const fw::RegisterPlan p = fw::plan_from(plan);
+    const size_t nc = size_t(plan->channels);
+    std::vector<const HImage*> f;
+    std::vector<std::vector<const HImage*>> m(subjects);
+    for (size_t c = 0; c < nc; ++c) {
+      if (!fixed[c]) fw::invalid("register", "null image");
+      f.push_back(&fixed[c]->h);
+    }
+    for (size_t s = 0; s < subjects; ++s)
+      for (size_t c = 0; c < nc; ++c) {
+        const fw_image* im = moving[s * nc + c];
+        if (!im) fw::invalid("register", "null image");
+        m[s].push_back(&im->h);
+      }
+    fw::hip::HipLinearEngine engine(ctx->backend);
+    std::vector<std::optional<HField>> fwd, inv;
+    const std::vector<fw::RegisterOutput> o = fw::hip::register_batch(
+        ctx->backend, engine, p, f, m, cancel, forward ? &fwd : nullptr, inverse ? &inv : nullptr);
+    auto& traces = last_batch_traces();
+    traces.assign(subjects, {});
+    for (size_t s = 0; s < subjects; ++s) {
+      fw::result_to(o[s], results ? &results[s] : nullptr);
+      traces[s] = fw::last_trace();
+      if (forward) forward[s] = fwd[s] ? new fw_field{std::move(*fwd[s])} : nullptr;
+      if (inverse) inverse[s] = inv[s] ? new fw_field{std::move(*inv[s])} : nullptr;
+    }
+  });
+}
+
+int32_t fw_last_batch_trace(uint32_t subject, fw_trace_step* out, int32_t capacity) {
+  const auto& all = last_batch_traces();
+  if (subject >= all.size()) return -1;
+  const auto& t = all[subject];
+  const size_t n = out && capacity > 0 ? std::min(t.size(), size_t(capacity)) : 0;
+  for (size_t i = 0; i < n; ++i) out[i] = t[i];
+  return int32_t(t.size());
 }
 
 int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,

@@ -496,7 +496,7 @@ HField HipBackend::copy_field(const HField& f) {
 
 // ------------------------------------------------------------------------ warps (a3, a4) ---
 
-static WarpArgs warp_args(const Grid& src, const HField& field, const HField* pre, const AffineTransform* affine) {
+WarpArgs warp_args(const Grid& src, const HField& field, const HField* pre, const AffineTransform* affine) {
   WarpArgs a{};
   const Grid& og = field.grid;
   const Mat3 M = affine ? affine->matrix : kIdentity3;

@@ -49,7 +49,8 @@ class DeviceContext : public HostSpanSink {
   // Pinned host memory for read-backs (one growable buffer).
   void* pinned(size_t bytes);
   // Coherent host memory that kernels write directly (results the host reads after a sync), one
-  // growable buffer per slot (0..3): the host pointer, and the device pointer in `device`.
+  // growable buffer per slot (0..7): the host pointer, and the device pointer in `device`. Slots
+  // 6 and 7 belong to the batched SyN loop.
   void* mapped(int slot, size_t bytes, void** device);
   // Pinned host space for a read-back whose value is wanted only at the next settle(): a bump
   // allocation valid until then (chunks are added, never moved, while copies may be in flight).
@@ -121,7 +122,7 @@ class DeviceContext : public HostSpanSink {
     void* dev = nullptr;
     size_t bytes = 0;
   };
-  Mapped mapped_[6];
+  Mapped mapped_[8];
   static constexpr int kDoneBase = 64;  // words 0..63: the inversion's progress and stop words
   static constexpr size_t kMailboxBytes = sizeof(uint32_t) * (kDoneBase + kDoneWords);
   uint32_t* mailbox_ = nullptr;
@@ -238,6 +239,10 @@ struct HField {
 using fw::InvertParams;
 using fw::InvertStats;
 using fw::MetricPair;
+
+// The kernel arguments of a warp of an image on grid `src` through `field` (onto the field's
+// grid), with an optional prior field and affine; the caller sets src, out (and the mask fields).
+WarpArgs warp_args(const Grid& src, const HField& field, const HField* pre, const AffineTransform* affine);
 
 class HipBackend {
  public:

@@ -1,0 +1,630 @@
+// The batched SyN loop: the iteration of syn.hpp (crates/fastwarp-reg/src/syn.rs:776-986) for
+// several subjects at once, each step one launch over every active subject.
+#include "batch_syn.hpp"
+
+#include <chrono>
+#include <thread>
+
+#include "geometry.hpp"
+
+namespace fw::hip {
+namespace {
+
+using clock_type = std::chrono::steady_clock;
+
+struct Subject {
+  std::optional<HField> f2m, f2m_inv, m2m, m2m_inv;
+  std::optional<WindowConvergenceMonitor> monitor;
+  SynLevelTrace trace;
+  uint32_t current = 0;
+  bool active = false;
+};
+
+// smooth_field_pair over any number of fields on one grid: the same taps, pass shape and kernels,
+// so each result is the bits of smooth_field_pair / smooth_field on it.
+std::vector<HField> smooth_fields(HipBackend& b, std::vector<HField> in, double variance) {
+  if (variance <= 0.0 || in.empty()) return in;
+  DeviceContext& ctx = b.ctx();
+  const Grid g = in[0].grid;
+  const Dims d = dims_of(g);
+  const size_t n = g.num_voxels();
+  const double w1 = variance < 0.5 ? 1.0 - variance / 0.5 : 1.0;
+  const double w2 = 1.0 - w1;
+  GaussTaps3 t{};
+  int* radii[3] = {&t.rx, &t.ry, &t.rz};
+  float* dst[3] = {t.x, t.y, t.z};
+  bool fits = true;
+  for (int q = 0; q < 3 && fits; ++q) {
+    const std::vector<double>& taps = ctx.host_taps(variance, g.size[q]);
+    *radii[q] = int(taps.size() / 2);
+    if (taps.size() > 17) fits = false;
+    else
+      for (size_t k = 0; k < taps.size(); ++k) dst[q][k] = float(taps[k]);
+  }
+  const size_t m = in.size();
+  if (!fits || !gauss3_fits(d, t) || m > size_t(kBatchFields) || 3ull * m * size_t(d.nz) > 65535) {
+    // the pair form, two fields at a time (it falls back to single fields itself)
+    std::vector<HField> out;
+    for (size_t q = 0; q < m; q += 2) {
+      if (q + 1 < m) {
+        auto r = b.smooth_field_pair(std::move(in[q]), std::move(in[q + 1]), variance);
+        out.push_back(std::move(r.first));
+        out.push_back(std::move(r.second));
+      } else {
+        out.push_back(b.smooth_field(std::move(in[q]), variance));
+      }
+    }
+    return out;
+  }
+  std::vector<HField> out;
+  FieldPtrs p{};
+  p.count = int(m);
+  for (size_t q = 0; q < m; ++q) {
+    out.push_back(HField{g, DevBuf(&ctx, 3 * n)});
+    p.in[q] = in[q].buf.get();
+    p.orig[q] = in[q].buf.get();
+    p.out[q] = out[q].buf.get();
+  }
+  hipStream_t s = ctx.stream();
+  const double fields = double(m);
+  if (t.rx == t.ry && t.ry == t.rz && ctx.tuning().get("gauss.one_pass", uint64_t(d.n()), gauss3_pays(d) ? 1 : 0)) {
+    Timed tm(ctx, "k_gauss3", fields * (24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0)));
+    if (launch_gauss3_set(p, d, t, float(w1), float(w2), s)) return out;
+  }
+  std::vector<DevBuf> tmp;
+  FieldPtrs xy{}, z{};
+  xy.count = z.count = int(m);
+  for (size_t q = 0; q < m; ++q) {
+    tmp.emplace_back(&ctx, 3 * n);
+    xy.in[q] = in[q].buf.get();
+    xy.out[q] = tmp[q].get();
+    z.in[q] = tmp[q].get();
+    z.orig[q] = in[q].buf.get();
+    z.out[q] = out[q].buf.get();
+  }
+  {
+    Timed tm(ctx, "k_gauss_xy", fields * 24.0 * double(n));
+    if (!launch_gauss_xy_set(xy, d, t, s)) invalid("smoothing", "the batched x/y pass does not fit the grid");
+  }
+  {
+    Timed tm(ctx, "k_gauss_z", fields * (24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0)));
+    if (!launch_gauss_z_set(z, d, t, float(w1), float(w2), s)) invalid("smoothing", "the batched z pass does not fit the grid");
+  }
+  return out;
+}
+
+// The reports of the items of one batched inversion stage (coherent host memory).
+struct InvStage {
+  volatile InvertReport* reports = nullptr;
+  size_t count = 0;
+};
+
+// invert_deferred for several (field, initial) pairs on one grid: each initial buffer becomes its
+// inverse estimate (psi is updated in place), as invert_deferred returns it.
+InvStage invert_items(HipBackend& b, const std::vector<const HField*>& fields, std::vector<HField*>& psi,
+                      const InvertParams& p, InvertReport* report_dev, volatile InvertReport* report_host,
+                      std::vector<DevBuf>& keep) {
+  DeviceContext& ctx = b.ctx();
+  const Grid& g = fields[0]->grid;
+  const Dims d = dims_of(g);
+  if (d.n() >= (1ll << 31)) invalid("invert", "grids of 2^31 voxels or more are not supported");
+  float l[9];
+  put(l, g.index_from_offset);
+  const float3 isp = inv_spacing(g);
+  const int nb = inv_blocks(d, int(ctx.tuning().get("inv.max_blocks", uint64_t(d.n()), 2048)));
+  const size_t m = fields.size();
+  InvItems it{};
+  it.count = int(m);
+  // per item: the residual field, the state and the two partial buffers
+  DevBuf state(&ctx, m * (sizeof(InvertState) / sizeof(float)));
+  DevBuf partials(&ctx, m * 4 * size_t(nb));
+  for (size_t q = 0; q < m; ++q) {
+    keep.emplace_back(&ctx, 3 * g.num_voxels());
+    it.field[q] = fields[q]->buf.get();
+    it.psi[q] = psi[q]->buf.get();
+    it.res[q] = keep.back().get();
+    it.state[q] = reinterpret_cast<InvertState*>(state.get()) + q;
+    it.report[q] = report_dev + q;
+    it.partials[q] = partials.get() + q * 4 * size_t(nb);
+  }
+  hipStream_t s = ctx.stream();
+  const int max_it = int(p.max_iterations);
+  volatile uint32_t* mail = ctx.mailbox_host();
+  uint32_t* mail_dev = ctx.mailbox_device();
+  const uint32_t tag = ctx.next_tag();
+  {
+    Timed t(ctx, "k_inv_begin", 36.0 * double(d.n()) * double(m));
+    launch_inv_begin_items(it, d, l, isp, nb, max_it, s);
+  }
+  const auto all_stopped = [&] {
+    for (size_t q = 0; q < m; ++q)
+      if (mail[kInvItemWords + q] != tag) return false;
+    return true;
+  };
+  const auto started = [&]() -> int {
+    const uint32_t w = mail[1];
+    return (w >> 8) == (tag & 0xffffffu) ? int(w & 0xffu) : 0;
+  };
+  // as invert_deferred: at most one launch queued beyond the one the device has started
+  constexpr int kAhead = 1;
+  for (int step = 0; step < max_it; ++step) {
+    if (step > 0 && all_stopped()) break;
+    if (step >= kAhead) {
+      const auto t0 = clock_type::now();
+      for (unsigned spins = 1; started() < step - kAhead + 1 && !all_stopped(); ++spins)
+        if ((spins & 255) == 0 && clock_type::now() - t0 > std::chrono::milliseconds(50)) break;
+      if (all_stopped()) break;
+    }
+    Timed t(ctx, "k_inv_iter", 0.0);
+    launch_inv_iter_items(it, d, l, isp, nb, step, max_it, p.max_error_tolerance, p.mean_error_tolerance,
+                          p.enforce_boundary_condition ? 1 : 0, mail_dev, tag, s);
+  }
+  keep.push_back(std::move(state));
+  keep.push_back(std::move(partials));
+  return InvStage{report_host, m};
+}
+
+InvertStats stats_of(const volatile InvertReport& r) {
+  InvertStats st;
+  st.iterations = uint32_t(r.executed);
+  st.mean_error_norm = r.executed ? double(r.mean_error) : 0.0;
+  st.max_error_norm = r.executed ? double(r.max_error) : 0.0;
+  return st;
+}
+
+// One lockstep iteration of the subjects `act` (all on grid g). Fills it[q] for each.
+void iterate(HipBackend& b, const SynRecipe& recipe, const std::vector<double>& normalised,
+             const std::vector<AffineTransform>& initial,
+             const std::vector<std::vector<SmoothCache<HipBackend>::Pair>>& smooth, std::vector<Subject>& subj,
+             const std::vector<size_t>& act, const Grid& g, std::vector<SynIteration>& its) {
+  DeviceContext& ctx = b.ctx();
+  hipStream_t st = ctx.stream();
+  const size_t k = act.size(), nc = normalised.size();
+  const Dims d = dims_of(g);
+  const size_t n = g.num_voxels();
+  const double dn = double(n);
+
+  // 1. every channel of every subject resampled into the middle (syn.rs:895-911)
+  std::vector<std::vector<HImage>> fmid(k), mmid(k);
+  for (size_t c = 0; c < nc; ++c) {
+    WarpItems wf{}, wm{};
+    wf.count = wm.count = int(k);
+    double src_bytes = 0.0;
+    for (size_t q = 0; q < k; ++q) {
+      const Subject& u = subj[act[q]];
+      const HImage& F = *smooth[act[q]][c].fixed;
+      const HImage& M = *smooth[act[q]][c].moving;
+      fmid[q].push_back(HImage{g, DevBuf(&ctx, n), F.bound});
+      mmid[q].push_back(HImage{g, DevBuf(&ctx, n), M.bound});
+      wf.a[q] = warp_args(F.grid, *u.f2m_inv, nullptr, nullptr);
+      wf.a[q].src = F.buf.get();
+      wf.a[q].out = fmid[q][c].buf.get();
+      wm.a[q] = warp_args(M.grid, *u.m2m_inv, nullptr, &initial[act[q]]);
+      wm.a[q].src = M.buf.get();
+      wm.a[q].out = mmid[q][c].buf.get();
+      src_bytes += 4.0 * (std::min(dn, double(F.grid.num_voxels())) + std::min(dn, double(M.grid.num_voxels())));
+    }
+    Timed t(ctx, "k_warp", 32.0 * dn * double(k) + src_bytes);
+    launch_warp_items(wf, st);
+    launch_warp_items(wm, st);
+  }
+
+  // 2. the CC pair of every channel, both directions (syn.rs:337-420); block partials straight
+  // into coherent host memory
+  const int tile_z = cc_tile_for(d, int(ctx.tuning().get("cc.tile_z", n, b.cc_tile_z)),
+                                 int(ctx.tuning().get("cc.min_blocks", n, 512)));
+  const dim3 gr = cc_grid(d, tile_z);
+  const size_t blocks = size_t(gr.x) * gr.y * gr.z;
+  void* cc_dev = nullptr;
+  char* cc_host = static_cast<char*>(ctx.mapped(6, k * nc * blocks * (sizeof(float) + sizeof(int)), &cc_dev));
+  float* pcc_dev = static_cast<float*>(cc_dev);
+  int* pcount_dev = reinterpret_cast<int*>(pcc_dev + k * nc * blocks);
+  std::vector<std::vector<HField>> dfix(k), dmov(k);
+  for (size_t c = 0; c < nc; ++c) {
+    CcItems ci{};
+    ci.count = int(k);
+    for (size_t q = 0; q < k; ++q) {
+      dfix[q].push_back(HField{g, DevBuf(&ctx, 3 * n)});
+      dmov[q].push_back(HField{g, DevBuf(&ctx, 3 * n)});
+      CcArgs& a = ci.a[q];
+      a.fixed = fmid[q][c].buf.get();
+      a.moving = mmid[q][c].buf.get();
+      a.valid = nullptr;
+      a.d = d;
+      a.radius = int(recipe.cc_radius);
+      a.tile_z = tile_z;
+      a.scale_f = cc_split_scale(fmid[q][c].bound);
+      a.scale_m = cc_split_scale(mmid[q][c].bound);
+      a.step_f = 1.0f / a.scale_f;
+      a.step_m = 1.0f / a.scale_m;
+      a.bounds = nullptr;
+      a.probes = probes_of(g);
+      a.d_fixed = dfix[q][c].buf.get();
+      a.d_moving = dmov[q][c].buf.get();
+      a.partial_cc = pcc_dev + (q * nc + c) * blocks;
+      a.partial_count = pcount_dev + (q * nc + c) * blocks;
+    }
+    Timed t(ctx, "k_cc2", 32.0 * dn * double(k));
+    launch_cc_items(ci, st);
+  }
+  fmid.clear();
+  mmid.clear();
+
+  // the update of each side: the single channel's derivative, or the channels weighted on the
+  // device as the host weighting does (syn.rs:203-297)
+  std::vector<HField> grads;  // 2 q + side
+  uint32_t* bad_host = nullptr;
+  std::vector<DevBuf> scratch;
+  volatile float* maxima_host = nullptr;
+  void* small_dev = nullptr;
+  // coherent host words of this iteration: per subject 2 maxima, a bad flag, then the reports of
+  // the two inversion stages
+  const size_t small_bytes = k * (2 * sizeof(float) + sizeof(uint32_t)) + 4 * k * sizeof(InvertReport) + 64;
+  char* small_host = static_cast<char*>(ctx.mapped(7, small_bytes, &small_dev));
+  maxima_host = reinterpret_cast<volatile float*>(small_host);
+  bad_host = reinterpret_cast<uint32_t*>(small_host + 2 * k * sizeof(float));
+  const size_t report_off = ((2 * k * sizeof(float) + k * sizeof(uint32_t)) + 15) / 16 * 16;
+  auto* reports_host = reinterpret_cast<volatile InvertReport*>(small_host + report_off);
+  auto* reports_dev = reinterpret_cast<InvertReport*>(static_cast<char*>(small_dev) + report_off);
+  float* maxima_dev = static_cast<float*>(small_dev);
+  uint32_t* bad_dev = reinterpret_cast<uint32_t*>(static_cast<char*>(small_dev) + 2 * k * sizeof(float));
+  for (size_t q = 0; q < k; ++q) bad_host[q] = 0;
+  if (nc == 1) {
+    for (size_t q = 0; q < k; ++q) {
+      grads.push_back(std::move(dfix[q][0]));
+      grads.push_back(std::move(dmov[q][0]));
+    }
+  } else {
+    const int nb = norm_blocks(long(n));
+    DevBuf parts(&ctx, k * nc * 4 * size_t(nb));
+    const float3 isp = inv_spacing(g);
+    for (size_t c = 0; c < nc; ++c) {
+      NormItems it{};
+      it.count = int(k);
+      for (size_t q = 0; q < k; ++q) {
+        it.a[q] = dfix[q][c].buf.get();
+        it.b[q] = dmov[q][c].buf.get();
+        it.partial_max[q] = parts.get() + (q * nc + c) * 4 * size_t(nb);
+        it.partial_sq[q] = it.partial_max[q] + 2 * nb;
+      }
+      Timed t(ctx, "k_field_norms", 24.0 * dn * double(k));
+      launch_field_norms_items(it, long(n), isp, nb, st);
+    }
+    WeightItems w{};
+    w.channels = int(nc);
+    w.count = int(k);
+    for (size_t c = 0; c < nc; ++c) w.weight[c] = normalised[c];
+    for (size_t q = 0; q < k; ++q)
+      for (size_t c = 0; c < nc; ++c) w.partial_sq[q][c] = parts.get() + (q * nc + c) * 4 * size_t(nb) + 2 * nb;
+    DevBuf coef(&ctx, 2 * k * (kBatchChannels + 2));
+    launch_channel_weights(w, nb, coef.get(), bad_dev, st);
+    CombineItems ci{};
+    ci.channels = int(nc);
+    ci.count = int(2 * k);
+    for (size_t q = 0; q < k; ++q)
+      for (int side = 0; side < 2; ++side) {
+        grads.push_back(HField{g, DevBuf(&ctx, 3 * n)});
+        for (size_t c = 0; c < nc; ++c) ci.d[2 * q + side][c] = (side == 0 ? dfix[q][c] : dmov[q][c]).buf.get();
+        ci.out[2 * q + side] = grads.back().buf.get();
+      }
+    {
+      Timed t(ctx, "k_combine", (12.0 * double(nc) + 12.0) * dn * double(2 * k));
+      launch_combine_items(ci, coef.get(), long(3 * n), st);
+    }
+    scratch.push_back(std::move(parts));
+    scratch.push_back(std::move(coef));
+  }
+  dfix.clear();
+  dmov.clear();
+
+  // 3. smooth both updates of every subject
+  grads = smooth_fields(b, std::move(grads), recipe.update_field_variance);
+
+  // 4. each update scaled so its largest voxel step is the gradient step, applied in the
+  // composition (syn.rs:422-445)
+  {
+    const int nb = norm_blocks(long(n));
+    DevBuf parts(&ctx, k * 4 * size_t(nb));
+    NormItems it{};
+    it.count = int(k);
+    for (size_t q = 0; q < k; ++q) {
+      it.a[q] = grads[2 * q].buf.get();
+      it.b[q] = grads[2 * q + 1].buf.get();
+      it.partial_max[q] = parts.get() + q * 4 * size_t(nb);
+      it.partial_sq[q] = it.partial_max[q] + 2 * nb;
+    }
+    {
+      Timed t(ctx, "k_field_norms", 24.0 * dn * double(k));
+      launch_field_norms_items(it, long(n), inv_spacing(g), nb, st);
+    }
+    DevBuf factors(&ctx, 2 * k);
+    launch_step_factors_items(it, nb, recipe.gradient_step, factors.get(), maxima_dev, st);
+    // 5. compose the scaled updates into the total fields
+    std::vector<HField> composed;
+    FieldPtrs p{};
+    p.count = int(2 * k);
+    for (size_t q = 0; q < k; ++q) {
+      Subject& u = subj[act[q]];
+      for (int side = 0; side < 2; ++side) {
+        composed.push_back(HField{g, DevBuf(&ctx, 3 * n)});
+        p.in[2 * q + side] = grads[2 * q + side].buf.get();
+        p.orig[2 * q + side] = (side == 0 ? *u.f2m : *u.m2m).buf.get();
+        p.out[2 * q + side] = composed.back().buf.get();
+      }
+    }
+    float l[9];
+    put(l, g.index_from_offset);
+    {
+      Timed t(ctx, "k_compose", 36.0 * dn * double(2 * k));
+      launch_compose_set(p, d, l, factors.get(), st);
+    }
+    for (size_t q = 0; q < k; ++q) {
+      subj[act[q]].f2m.reset();
+      subj[act[q]].m2m.reset();
+    }
+    grads.clear();
+    scratch.push_back(std::move(parts));
+    scratch.push_back(std::move(factors));
+    composed = smooth_fields(b, std::move(composed), recipe.total_field_variance);
+
+    // 6. both inverses re-estimated: the new inverse from the old one, then the new field from it
+    const InvertParams inv_params;  // InvertParams::SYN
+    std::vector<DevBuf> keep;
+    std::vector<const HField*> fa;
+    std::vector<HField*> pa;
+    for (size_t q = 0; q < k; ++q) {
+      Subject& u = subj[act[q]];
+      fa.push_back(&composed[2 * q]);
+      pa.push_back(&*u.f2m_inv);
+      fa.push_back(&composed[2 * q + 1]);
+      pa.push_back(&*u.m2m_inv);
+    }
+    const InvStage sa = invert_items(b, fa, pa, inv_params, reports_dev, reports_host, keep);
+    std::vector<const HField*> fb;
+    std::vector<HField*> pb;
+    for (size_t q = 0; q < k; ++q) {
+      Subject& u = subj[act[q]];
+      fb.push_back(&*u.f2m_inv);
+      pb.push_back(&composed[2 * q]);
+      fb.push_back(&*u.m2m_inv);
+      pb.push_back(&composed[2 * q + 1]);
+    }
+    const InvStage sb = invert_items(b, fb, pb, inv_params, reports_dev + 2 * k, reports_host + 2 * k, keep);
+    for (size_t q = 0; q < k; ++q) {
+      subj[act[q]].f2m = std::move(composed[2 * q]);
+      subj[act[q]].m2m = std::move(composed[2 * q + 1]);
+    }
+    b.settle();
+    const bool profiling = ctx.profiling();
+    for (size_t q = 0; q < k; ++q) {
+      SynIteration& it_q = its[q];
+      const double fmax = maxima_host[2 * q], mmax = maxima_host[2 * q + 1];
+      if (!(std::isfinite(fmax) && std::isfinite(mmax))) invalid("syn", "the metric gradient is not finite");
+      // stats in the order of syn_fields: F inverse, F forward, M inverse, M forward
+      const volatile InvertReport* r[4] = {&sa.reports[2 * q], &sb.reports[2 * q], &sa.reports[2 * q + 1],
+                                           &sb.reports[2 * q + 1]};
+      for (int v = 0; v < 4; ++v) {
+        it_q.inversions[v] = stats_of(*r[v]);
+        if (profiling && r[v]->executed > 0)
+          ctx.add_bytes("k_inv_iter", dn * (60.0 * double(r[v]->executed - 1) + 36.0));
+        if (!(std::isfinite(it_q.inversions[v].mean_error_norm) && std::isfinite(it_q.inversions[v].max_error_norm)))
+          invalid("syn", "the displacement field is not finite");
+      }
+      if (nc > 1 && bad_host[q]) invalid("syn", "the metric gradient is not finite");
+      // the channels' values, each from its block partials in block order
+      const float* pcc = reinterpret_cast<const float*>(cc_host);
+      const int* pcount = reinterpret_cast<const int*>(cc_host + k * nc * blocks * sizeof(float));
+      double weighted[2] = {0.0, 0.0};
+      double first = 0.0;
+      for (size_t c = 0; c < nc; ++c) {
+        double total = 0.0;
+        long long count = 0;
+        for (size_t bl = 0; bl < blocks; ++bl) {
+          total += double(pcc[(q * nc + c) * blocks + bl]);
+          count += pcount[(q * nc + c) * blocks + bl];
+        }
+        if (count == 0) invalid("cc metric", "no virtual voxel maps inside both image buffers");
+        const double v = -total / double(count);
+        if (c == 0) first = v;
+        weighted[0] += nc == 1 ? v : v * normalised[c];
+        weighted[1] += nc == 1 ? v : v * normalised[c];
+      }
+      double scales[2] = {recipe.gradient_step, recipe.gradient_step};
+      if (fmax > 0.0) scales[0] /= fmax;
+      if (mmax > 0.0) scales[1] /= mmax;
+      it_q.energy = 0.5 * (first + first);
+      it_q.energy_weighted = 0.5 * (weighted[0] + weighted[1]);
+      it_q.fixed_update_scale = scales[0];
+      it_q.moving_update_scale = scales[1];
+    }
+  }
+}
+
+}  // namespace
+
+std::vector<SynFields> syn_fields_batch(HipBackend& b, const SynRecipe& recipe, const std::vector<AffineTransform>& initial,
+                                        const std::vector<SmoothCache<HipBackend>*>& caches, const volatile int32_t* cancel,
+                                        std::vector<std::optional<HField>>* forward,
+                                        std::vector<std::optional<HField>>* inverse) {
+  recipe.validate();
+  const size_t ns = caches.size();
+  if (ns == 0 || ns > size_t(kBatchItems))
+    invalid("batch syn", "a batch holds 1 to " + std::to_string(kBatchItems) + " subjects");
+  if (initial.size() != ns) invalid("batch syn", "one initial transform per subject is required");
+  const size_t nc = caches[0]->channels();
+  if (nc == 0) invalid("syn", "no channels");
+  if (nc != recipe.channel_weights.size()) invalid("syn", "the number of channels differs from the number of channel weights");
+  if (nc > size_t(kBatchChannels)) invalid("batch syn", "at most " + std::to_string(kBatchChannels) + " channels");
+  for (size_t c = 0; c < nc; ++c)
+    if (recipe.metric(c) != SynMetric::Cc) invalid("batch syn", "the batched loop takes CC channels only");
+  const Grid fixed_grid = caches[0]->fixed_grid(0);
+  for (size_t s = 0; s < ns; ++s) {
+    if (!caches[s] || caches[s]->channels() != nc) invalid("batch syn", "every subject needs the same channels");
+    for (size_t c = 0; c < nc; ++c)
+      if (!caches[s]->fixed_grid(c).approx_eq(fixed_grid, 1e-9))
+        invalid("batch syn", "the fixed images of the subjects differ in grid");
+  }
+  double wsum = 0.0;
+  for (double w : recipe.channel_weights) wsum += w;
+  std::vector<double> normalised;
+  for (double w : recipe.channel_weights) normalised.push_back(w / wsum);
+  std::vector<Subject> subj(ns);
+  std::vector<SynFields> out(ns);
+  for (size_t level = 0; level < recipe.levels(); ++level) {
+    check_cancel(cancel);
+    const auto t0 = clock_type::now();
+    const auto factors = shrink_factors_per_dimension(recipe.shrink_factors[level], fixed_grid.spacing);
+    const Grid grid = level_virtual_grid(fixed_grid, factors);
+    const double sigma = recipe.smoothing_sigmas[level];
+    std::vector<std::vector<SmoothCache<HipBackend>::Pair>> smooth(ns);
+    for (size_t s = 0; s < ns; ++s)
+      for (size_t c = 0; c < nc; ++c) smooth[s].push_back(caches[s]->get(c, sigma, recipe.sigma_units));
+    for (size_t s = 0; s < ns; ++s) {
+      Subject& u = subj[s];
+      if (!u.f2m) {
+        u.f2m = b.zero_field(grid);
+        u.f2m_inv = b.zero_field(grid);
+        u.m2m = b.zero_field(grid);
+        u.m2m_inv = b.zero_field(grid);
+      } else {
+        u.f2m = b.resample_field(*u.f2m, grid);
+        u.f2m_inv = b.resample_field(*u.f2m_inv, grid);
+        u.m2m = b.resample_field(*u.m2m, grid);
+        u.m2m_inv = b.resample_field(*u.m2m_inv, grid);
+      }
+      u.monitor.emplace(recipe.convergence_window);
+      u.trace = SynLevelTrace{};
+      u.trace.level = level;
+      u.trace.shrink_factors = factors;
+      u.trace.grid_size = grid.size;
+      u.current = 0;
+      u.active = recipe.iterations[level] > 0;
+    }
+    for (;;) {
+      std::vector<size_t> act;
+      for (size_t s = 0; s < ns; ++s)
+        if (subj[s].active) act.push_back(s);
+      if (act.empty()) break;
+      check_cancel(cancel);
+      const auto t_it = clock_type::now();
+      std::vector<SynIteration> its(act.size());
+      iterate(b, recipe, normalised, initial, smooth, subj, act, grid, its);
+      const double secs = std::chrono::duration<double>(clock_type::now() - t_it).count();
+      for (size_t q = 0; q < act.size(); ++q) {
+        Subject& u = subj[act[q]];
+        SynIteration& it = its[q];
+        ++u.current;
+        u.monitor->add_energy_value(it.energy);
+        it.convergence_value = u.monitor->convergence_value();
+        if (it.convergence_value < recipe.convergence_threshold) u.trace.converged = true;
+        it.seconds = secs;
+        u.trace.iterations.push_back(it);
+        if (u.trace.converged || u.current >= recipe.iterations[level]) u.active = false;
+      }
+    }
+    const double level_secs = std::chrono::duration<double>(clock_type::now() - t0).count();
+    for (size_t s = 0; s < ns; ++s) {
+      subj[s].trace.seconds = level_secs;
+      out[s].levels.push_back(std::move(subj[s].trace));
+    }
+  }
+  // syn.rs:969-980: SyN = Minv o F forward, Finv o M inverse
+  for (size_t s = 0; s < ns; ++s) {
+    Subject& u = subj[s];
+    if (forward) (*forward)[s] = b.compose(*u.m2m_inv, *u.f2m);
+    if (inverse) (*inverse)[s] = b.compose(*u.f2m_inv, *u.m2m);
+  }
+  return out;
+}
+
+std::vector<RegisterOutput> register_batch(HipBackend& b, HipLinearEngine& engine, const RegisterPlan& plan,
+                                           const std::vector<const HImage*>& fixed,
+                                           const std::vector<std::vector<const HImage*>>& moving,
+                                           const volatile int32_t* cancel, std::vector<std::optional<HField>>* forward,
+                                           std::vector<std::optional<HField>>* inverse) {
+  auto secs = [](clock_type::time_point a) { return std::chrono::duration<double>(clock_type::now() - a).count(); };
+  const size_t ns = moving.size(), nc = fixed.size();
+  if (ns == 0) invalid("register batch", "no subjects");
+  if (nc == 0) invalid("register", "fixed and moving must have the same, non-zero number of channels");
+  for (size_t s = 0; s < ns; ++s)
+    if (moving[s].size() != nc) invalid("register", "fixed and moving must have the same, non-zero number of channels");
+  if (plan.run_syn && plan.syn.channel_weights.size() != nc)
+    invalid("stage plan", "the number of channels differs from the number of channel weights");
+  for (size_t c = 0; c < nc; ++c) {
+    if (!fixed[c]) invalid("register", "missing image");
+    for (size_t s = 0; s < ns; ++s)
+      if (!moving[s][c]) invalid("register", "missing image");
+  }
+  if (forward) {
+    forward->clear();
+    forward->resize(ns);
+  }
+  if (inverse) {
+    inverse->clear();
+    inverse->resize(ns);
+  }
+  std::vector<RegisterOutput> out(ns);
+  // the template: preprocessed, its centre of mass and its smoothed levels taken once
+  auto t = clock_type::now();
+  std::vector<std::shared_ptr<const HImage>> tmpl;
+  for (size_t c = 0; c < nc; ++c)
+    tmpl.push_back(std::make_shared<const HImage>(b.preprocess_image(*fixed[c], plan.winsorize_lower, plan.winsorize_upper)));
+  const double tmpl_secs = secs(t);
+  const Vec3 fc = plan.initial_kind == InitialKind::CentreOfMass ? b.centre_of_gravity(*fixed[0]) : Vec3{0, 0, 0};
+  auto shared = std::make_shared<SmoothCache<HipBackend>::SharedFixed>();
+  std::vector<std::unique_ptr<SmoothCache<HipBackend>>> caches;
+  for (size_t s = 0; s < ns; ++s) {
+    RegisterOutput& o = out[s];
+    t = clock_type::now();
+    std::vector<SmoothCache<HipBackend>::Pair> resident;
+    for (size_t c = 0; c < nc; ++c)
+      resident.push_back(
+          {tmpl[c], std::make_shared<const HImage>(b.preprocess_image(*moving[s][c], plan.winsorize_lower, plan.winsorize_upper))});
+    o.seconds_preprocess = tmpl_secs + secs(t);
+    caches.push_back(std::make_unique<SmoothCache<HipBackend>>(b, std::move(resident)));
+    caches.back()->share_fixed(shared);
+    t = clock_type::now();
+    AffineTransform start = plan.initial_affine;
+    if (plan.initial_kind == InitialKind::CentreOfMass) {
+      const Vec3 mc = b.centre_of_gravity(*moving[s][0]);
+      start = AffineTransform{kIdentity3, vsub(mc, fc), fc};
+    } else if (plan.initial_kind == InitialKind::Identity) {
+      start = AffineTransform{};
+    }
+    if (plan.run_linear) {
+      o.linear = run_linear_chain(engine, plan.rigid, plan.affine, start, *caches.back(), cancel);
+      o.initial = o.linear.initial;
+      o.total = o.linear.total;
+    } else {
+      o.initial = o.total = start;
+    }
+    o.seconds_linear = secs(t);
+  }
+  if (plan.run_syn) {
+    b.release_idle_buffers();
+    for (size_t s0 = 0; s0 < ns; s0 += size_t(kBatchItems)) {
+      const size_t s1 = std::min(ns, s0 + size_t(kBatchItems));
+      std::vector<AffineTransform> init;
+      std::vector<SmoothCache<HipBackend>*> group;
+      for (size_t s = s0; s < s1; ++s) {
+        init.push_back(out[s].total);
+        group.push_back(caches[s].get());
+      }
+      std::vector<std::optional<HField>> fwd(s1 - s0), inv(s1 - s0);
+      t = clock_type::now();
+      std::vector<SynFields> syn = syn_fields_batch(b, plan.syn, init, group, cancel, forward ? &fwd : nullptr,
+                                                    inverse ? &inv : nullptr);
+      const double batch_secs = secs(t);
+      for (size_t s = s0; s < s1; ++s) {
+        out[s].syn = std::move(syn[s - s0]);
+        out[s].seconds_syn = batch_secs;
+        if (forward) (*forward)[s] = std::move(fwd[s - s0]);
+        if (inverse) (*inverse)[s] = std::move(inv[s - s0]);
+      }
+    }
+  }
+  for (auto& o : out) o.seconds_total = o.seconds_preprocess + o.seconds_linear + o.seconds_syn;
+  return out;
+}
+
+}  // namespace fw::hip

@@ -141,8 +141,10 @@ __device__ __forceinline__ void cc_column_step(CcColumn& col, const CcSplit& ea,
   col.c2ab += fma(Hae, lbe, lae * (Hbe + lbe)) - fma(Hal, lbl, lal * (Hbl + lbl));
 }
 
-template <int R, bool MASKED, bool DIAGONAL>
-__global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
+// The work of one CC block: the tile `tile()` names in the grid of `a` (x and y as the launch's,
+// z within the item's own grid).
+template <int R, bool MASKED, bool DIAGONAL, typename Tile>
+__device__ __forceinline__ void cc2_block(const CcArgs& a, Tile tile) {
   constexpr int kCcThreads = cc_threads<R>();
   constexpr int W = 2 * R + 1;
   constexpr int LX = kCcTX + 2 * R, LY = kCcTY + 2 * R;
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
 
   const Dims d = a.d;
   const int tid = threadIdx.x;
-  const uint3 bl = xcd_block();
+  const uint3 bl = tile();
   const int x0 = bl.x * kCcTX, y0 = bl.y * kCcTY, z0 = bl.z * a.tile_z;
   const int z1 = min(z0 + a.tile_z, d.nz);
   const long long plane = (long long)d.nx * d.ny;
@@ -365,16 +367,65 @@ __global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
   const float total = block_sum_f64(cc_sum);
   const float cnt = block_sum((float)counted);
   if (tid == 0) {
-    const int b = bl.x + gridDim.x * (bl.y + gridDim.y * bl.z);
+    const unsigned b = bl.x + gridDim.x * (bl.y + gridDim.y * bl.z);
     a.partial_cc[b] = total;
     a.partial_count[b] = (int)cnt;
   }
+}
+
+template <int R, bool MASKED, bool DIAGONAL>
+__global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
+  cc2_block<R, MASKED, DIAGONAL>(a, [] { return xcd_block(); });
+}
+
+// Several unmasked CC pairs on one grid: the grid's z blocks repeat once per item, and item q's
+// blocks are the blocks of its own launch (the same tiles, partials at the same indices of its
+// own partial buffers).
+template <int R, bool DIAGONAL>
+__global__ __launch_bounds__(cc_threads<R>()) void k_cc2_items(CcItems c) {
+  const unsigned gz = gridDim.z / (unsigned)c.count;
+  const unsigned q = xcd_block().z / gz;
+  cc2_block<R, false, DIAGONAL>(c.a[q], [q, gz] {
+    uint3 bl = xcd_block();
+    bl.z -= q * gz;
+    return bl;
+  });
 }
 
 template <int R, bool M>
 static void launch_cc_r(const CcArgs& a, hipStream_t s) {
   if (a.probes.diagonal) launch((k_cc2<R, M, true>), cc_grid(a.d, a.tile_z), dim3(cc_threads<R>()), 0, s, a);
   else launch((k_cc2<R, M, false>), cc_grid(a.d, a.tile_z), dim3(cc_threads<R>()), 0, s, a);
+}
+
+template <int R>
+static void launch_cc_items_r(const CcItems& c, hipStream_t s) {
+  const CcArgs& a = c.a[0];
+  dim3 g = cc_grid(a.d, a.tile_z);
+  g.z *= (unsigned)c.count;
+  if (a.probes.diagonal) launch((k_cc2_items<R, true>), g, dim3(cc_threads<R>()), 0, s, c);
+  else launch((k_cc2_items<R, false>), g, dim3(cc_threads<R>()), 0, s, c);
+}
+
+void launch_cc_items(const CcItems& c, hipStream_t s) {
+  if (c.count < 1 || c.count > kBatchItems) return;
+  switch (c.a[0].radius) {
+#define FW_CCI_CASE(R)              \
+  case R:                           \
+    launch_cc_items_r<R>(c, s);     \
+    break;
+    FW_CCI_CASE(1)
+    FW_CCI_CASE(2)
+    FW_CCI_CASE(3)
+    FW_CCI_CASE(4)
+    FW_CCI_CASE(5)
+    FW_CCI_CASE(6)
+    FW_CCI_CASE(7)
+    FW_CCI_CASE(8)
+#undef FW_CCI_CASE
+    default:
+      break;
+  }
 }
 
 dim3 cc_grid(const Dims& d, int tile_z) {
@@ -418,10 +469,11 @@ void launch_cc_pair(const CcArgs& a, hipStream_t s) {
 // Field norms (field_ops.rs:191-213): per block max |u/spacing| and sum of |u|^2 of up to two
 // fields; the host combines the block partials in block order.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_field_norms(const float* __restrict__ a, const float* __restrict__ b, long long n,
-                                                     float3 inv_sp, float* partial_max, float* partial_sq) {
+__device__ __forceinline__ void field_norms_block(const float* __restrict__ a, const float* __restrict__ b, long long n,
+                                                  float3 inv_sp, float* partial_max, float* partial_sq, unsigned blk,
+                                                  unsigned nblk) {
   float ma = 0.0f, mb = 0.0f, sa = 0.0f, ca = 0.0f, sb = 0.0f, cb = 0.0f;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = (long long)blk * blockDim.x + threadIdx.x; i < n; i += (long long)nblk * blockDim.x) {
     {
       const float x = a[i], y = a[i + n], z = a[i + 2 * n];
       const float px = x * inv_sp.x, py = y * inv_sp.y, pz = z * inv_sp.z;
@@ -446,11 +498,27 @@ __global__ __launch_bounds__(256) void k_field_norms(const float* __restrict__ a
   const float rmb = block_max(mb);
   const float rsb = block_sum(sb);
   if (threadIdx.x == 0) {
-    partial_max[blockIdx.x] = rma;
-    partial_sq[blockIdx.x] = rsa;
-    partial_max[gridDim.x + blockIdx.x] = rmb;
-    partial_sq[gridDim.x + blockIdx.x] = rsb;
+    partial_max[blk] = rma;
+    partial_sq[blk] = rsa;
+    partial_max[nblk + blk] = rmb;
+    partial_sq[nblk + blk] = rsb;
   }
+}
+
+__global__ __launch_bounds__(256) void k_field_norms(const float* __restrict__ a, const float* __restrict__ b, long long n,
+                                                     float3 inv_sp, float* partial_max, float* partial_sq) {
+  field_norms_block(a, b, n, inv_sp, partial_max, partial_sq, blockIdx.x, gridDim.x);
+}
+
+// Item q takes blocks [q * blocks, (q + 1) * blocks) and writes the partials of its own launch.
+__global__ __launch_bounds__(256) void k_field_norms_items(NormItems it, long long n, float3 inv_sp, int blocks) {
+  const unsigned q = blockIdx.x / (unsigned)blocks;
+  field_norms_block(it.a[q], it.b[q], n, inv_sp, it.partial_max[q], it.partial_sq[q], blockIdx.x - q * blocks, blocks);
+}
+
+void launch_field_norms_items(const NormItems& it, long long n, float3 inv_spacing, int blocks, hipStream_t s) {
+  if (it.count < 1 || it.count > kBatchItems) return;
+  launch(k_field_norms_items, dim3(blocks * it.count), dim3(256), 0, s, it, n, inv_spacing, blocks);
 }
 
 int norm_blocks(long long n) {
@@ -466,8 +534,8 @@ void launch_field_norms(const float* a, const float* b, long long n, float3 inv_
 // Update factors of the SyN loop (syn.rs:422-445): for each of `fields` fields, the NaN-propagating
 // maximum of its partial maxima and factor = step / max (step when the maximum is not positive),
 // divided in f64 and rounded once to f32, as the host computed it. One block.
-__global__ __launch_bounds__(256) void k_step_factors(const float* __restrict__ partial_max, int blocks, int fields,
-                                                      double step, float* factors, float* maxima) {
+__device__ __forceinline__ void step_factors_block(const float* __restrict__ partial_max, int blocks, int fields, double step,
+                                                   float* factors, float* maxima) {
   for (int f = 0; f < fields; ++f) {
     float m = 0.0f;
     for (int b = threadIdx.x; b < blocks; b += blockDim.x) m = nan_max(m, partial_max[f * blocks + b]);
@@ -477,6 +545,75 @@ __global__ __launch_bounds__(256) void k_step_factors(const float* __restrict__ 
       factors[f] = r > 0.0f ? (float)(step / (double)r) : (float)step;
     }
     __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_step_factors(const float* __restrict__ partial_max, int blocks, int fields,
+                                                      double step, float* factors, float* maxima) {
+  step_factors_block(partial_max, blocks, fields, step, factors, maxima);
+}
+
+// One block per item.
+__global__ __launch_bounds__(256) void k_step_factors_items(NormItems it, int blocks, double step, float* factors,
+                                                            float* maxima) {
+  const unsigned q = blockIdx.x;
+  step_factors_block(it.partial_max[q], blocks, 2, step, factors + 2 * q, maxima + 2 * q);
+}
+
+void launch_step_factors_items(const NormItems& it, int blocks, double step, float* factors, float* maxima, hipStream_t s) {
+  if (it.count < 1 || it.count > kBatchItems) return;
+  launch(k_step_factors_items, dim3(it.count), dim3(256), 0, s, it, blocks, step, factors, maxima);
+}
+
+// One thread per subject: the magnitudes and coefficients of both sides, in the order the host
+// weighting takes them (channels in order, each magnitude from its partials summed in block order).
+__global__ __launch_bounds__(64) void k_channel_weights(WeightItems w, int blocks, float* coef, uint32_t* bad) {
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= w.count) return;
+  uint32_t nonfinite = 0;
+  for (int side = 0; side < 2; ++side) {
+    float* out = coef + (size_t)(2 * q + side) * (kBatchChannels + 2);
+    double total = 0.0;
+    uint32_t use = 0;
+    for (int c = 0; c < w.channels; ++c) {
+      const float* sq = w.partial_sq[q][c] + side * blocks;
+      double s2 = 0.0;
+      for (int b = 0; b < blocks; ++b) s2 += (double)sq[b];
+      const double mag = sqrt(s2);
+      if (!isfinite(mag)) nonfinite = 1;
+      total += mag;
+      out[c] = 0.0f;
+      if (mag > (double)kEps64) {
+        out[c] = (float)(w.weight[c] / mag);
+        use |= 1u << c;
+      }
+    }
+    out[kBatchChannels] = (float)(total / (double)w.channels);
+    out[kBatchChannels + 1] = __uint_as_float(use);
+  }
+  bad[q] = nonfinite;
+}
+
+void launch_channel_weights(const WeightItems& w, int blocks, float* coef, uint32_t* bad, hipStream_t s) {
+  if (w.count < 1 || w.count > kBatchItems || w.channels < 1 || w.channels > kBatchChannels) return;
+  launch(k_channel_weights, dim3(1), dim3(64), 0, s, w, blocks, coef, bad);
+}
+
+// Item q takes blocks [q * per, (q + 1) * per): acc = 0, acc = fma(coef_c, d_c, acc) over the
+// used channels in order, then acc * scale, as zero_field, axpy per channel and scale compute it.
+__global__ __launch_bounds__(256) void k_combine_items(CombineItems c, const float* __restrict__ coef, long long count,
+                                                       int per) {
+  const unsigned q = blockIdx.x / (unsigned)per;
+  const unsigned blk = blockIdx.x - q * per;
+  const float* cq = coef + (size_t)q * (kBatchChannels + 2);
+  const uint32_t use = __float_as_uint(cq[kBatchChannels + 1]);
+  const float scale = cq[kBatchChannels];
+  float* out = c.out[q];
+  for (long long i = (long long)blk * blockDim.x + threadIdx.x; i < count; i += (long long)per * blockDim.x) {
+    float acc = 0.0f;
+    for (int ch = 0; ch < c.channels; ++ch)
+      if (use & (1u << ch)) acc = fmaf(cq[ch], c.d[q][ch][i], acc);
+    out[i] = acc * scale;
   }
 }
 
@@ -725,7 +862,7 @@ __global__ __launch_bounds__(256) void k_scale(float* __restrict__ x, long long 
 
 __global__ __launch_bounds__(256) void k_axpy(float* __restrict__ acc, const float* __restrict__ x, long long count, float a) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x)
-    acc[i] += a * x[i];
+    acc[i] = fmaf(a, x[i], acc[i]);
 }
 
 static int stream_blocks(long long count) {
@@ -747,6 +884,12 @@ void launch_scale(float* x, long long count, float s, hipStream_t st) {
 
 void launch_axpy(float* acc, const float* x, long long count, float a, hipStream_t st) {
   launch(k_axpy, dim3(stream_blocks(count)), dim3(256), 0, st, acc, x, count, a);
+}
+
+void launch_combine_items(const CombineItems& c, const float* coef, long long count, hipStream_t s) {
+  if (c.count < 1 || c.count > kBatchFields || c.channels < 1 || c.channels > kBatchChannels) return;
+  const int per = stream_blocks(count);
+  launch(k_combine_items, dim3(per * c.count), dim3(256), 0, s, c, coef, count, per);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -810,6 +953,25 @@ __global__ __launch_bounds__(256) void k_warp2(WarpArgs a0, WarpArgs a1) {
   const WarpArgs& a = second ? a1 : a0;
   if (i >= a.od.nx || j >= a.od.ny) return;
   warp_at(a, i, j, second ? (int)bl.z - a0.od.nz : (int)bl.z);
+}
+
+// Several warps onto one output grid: planes [q nz, (q + 1) nz) of the grid take item q.
+__global__ __launch_bounds__(256) void k_warp_items(WarpItems w) {
+  const uint3 bl = xcd_block();
+  const int nz = w.a[0].od.nz;
+  const int q = (int)bl.z / nz;
+  const WarpArgs& a = w.a[q];
+  const int i = bl.x * 64 + (threadIdx.x & 63);
+  const int j = bl.y * 4 + (threadIdx.x >> 6);
+  if (i >= a.od.nx || j >= a.od.ny) return;
+  warp_at(a, i, j, (int)bl.z - q * nz);
+}
+
+void launch_warp_items(const WarpItems& w, hipStream_t s) {
+  if (w.count < 1 || w.count > kBatchItems) return;
+  const Dims& od = w.a[0].od;
+  const dim3 grid((od.nx + 63) / 64, (od.ny + 3) / 4, od.nz * w.count);
+  launch(k_warp_items, grid, dim3(256), 0, s, w);
 }
 
 void launch_warp(const WarpArgs& a, hipStream_t s) {
@@ -896,22 +1058,19 @@ struct L9 {
   float v[9];
 };
 
-// Planes [0, nz) of the grid compose (u, w) into out; with a second triple, planes [nz, 2 nz)
-// compose (u1, w1) into out1. With `factors` (device, one per pair), u is the update before its
-// scaling: the interpolated update is multiplied by the pair's factor (the SyN loop's step / max,
-// so that the scaled update is never stored).
-__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u0, const float* __restrict__ w0, Dims d, L9 l,
-                                                 float* __restrict__ out0, const float* __restrict__ u1,
-                                                 const float* __restrict__ w1, float* __restrict__ out1,
-                                                 const float* __restrict__ factors) {
+// Planes [f nz, (f + 1) nz) of the grid compose (in[f], orig[f]) = (u, w) into out[f]. With
+// `factors` (device, one per field), u is the update before its scaling: the interpolated update
+// is multiplied by the field's factor (the SyN loop's step / max, so that the scaled update is
+// never stored).
+__global__ __launch_bounds__(256) void k_compose(FieldPtrs p, Dims d, L9 l, const float* __restrict__ factors) {
   const uint3 bl = xcd_block();
   const int i = bl.x * 64 + (threadIdx.x & 63);
   const int j = bl.y * 4 + (threadIdx.x >> 6);
-  const bool second = (int)bl.z >= d.nz;
-  const int k = second ? (int)bl.z - d.nz : (int)bl.z;
-  const float* u = second ? u1 : u0;
-  const float* w = second ? w1 : w0;
-  float* out = second ? out1 : out0;
+  const int f = (int)bl.z / d.nz;
+  const int k = (int)bl.z - f * d.nz;
+  const float* __restrict__ u = p.in[f];
+  const float* __restrict__ w = p.orig[f];
+  float* __restrict__ out = p.out[f];
   if (i >= d.nx || j >= d.ny) return;
   const long long g = lin(d, i, j, k), n = d.n();
   const float wx = w[g], wy = w[g + n], wz = w[g + 2 * n];
@@ -923,10 +1082,10 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u0, c
     float ux, uy, uz;
     tri3(u, d, cx, cy, cz, ux, uy, uz);
     if (factors) {
-      const float f = factors[second ? 1 : 0];
-      ux *= f;
-      uy *= f;
-      uz *= f;
+      const float fa = factors[f];
+      ux *= fa;
+      uy *= fa;
+      uz *= fa;
     }
     ox += ux;
     oy += uy;
@@ -937,20 +1096,34 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ u0, c
   out[g + 2 * n] = oz;
 }
 
-void launch_compose(const float* displacement, const float* warping, Dims d, const float l[9], float* out, hipStream_t s) {
+void launch_compose_set(const FieldPtrs& p, Dims d, const float l[9], const float* factors, hipStream_t s) {
+  if (p.count < 1 || p.count > kBatchFields) return;
   L9 lv;
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
-  launch(k_compose, grid, dim3(256), 0, s, displacement, warping, d, lv, out, (const float*)nullptr, (const float*)nullptr,
-         (float*)nullptr, (const float*)nullptr);
+  const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz * p.count);
+  launch(k_compose, grid, dim3(256), 0, s, p, d, lv, factors);
+}
+
+void launch_compose(const float* displacement, const float* warping, Dims d, const float l[9], float* out, hipStream_t s) {
+  FieldPtrs p{};
+  p.in[0] = displacement;
+  p.orig[0] = warping;
+  p.out[0] = out;
+  p.count = 1;
+  launch_compose_set(p, d, l, nullptr, s);
 }
 
 void launch_compose2(const float* u0, const float* w0, float* out0, const float* u1, const float* w1, float* out1, Dims d,
                      const float l[9], const float* factors, hipStream_t s) {
-  L9 lv;
-  for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
-  const dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, 2 * d.nz);
-  launch(k_compose, grid, dim3(256), 0, s, u0, w0, d, lv, out0, u1, w1, out1, factors);
+  FieldPtrs p{};
+  p.in[0] = u0;
+  p.orig[0] = w0;
+  p.out[0] = out0;
+  p.in[1] = u1;
+  p.orig[1] = w1;
+  p.out[1] = out1;
+  p.count = 2;
+  launch_compose_set(p, d, l, factors, s);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -966,21 +1139,19 @@ void launch_compose2(const float* u0, const float* w0, float* out0, const float*
 // ------------------------------------------------------------------------------------------
 constexpr int kG3TX = 64, kG3TY = 16, kG3MaxR = 8;
 
-// Grid z runs over component x plane; components 3..5 (a second field, when the grid has 6 nz
-// planes) read in1 and write out1.
-__global__ __launch_bounds__(256) void k_gauss_xy(const float* __restrict__ in, float* __restrict__ out, Dims d,
-                                                  GaussTaps3 t, const float* __restrict__ in1, float* __restrict__ out1) {
+// Grid z runs over field x component x plane: field f reads p.in[f] and writes p.out[f].
+__global__ __launch_bounds__(256) void k_gauss_xy(FieldPtrs p, Dims d, GaussTaps3 t) {
   __shared__ float tile[kG3TY + 2 * kG3MaxR][kG3TX + 2 * kG3MaxR];
   __shared__ float xs[kG3TY + 2 * kG3MaxR][kG3TX];
   const int rx = t.rx, ry = t.ry;
   const int LX = kG3TX + 2 * rx, LY = kG3TY + 2 * ry;
   const uint3 bl = xcd_block();
   const int cz = bl.z / d.nz, z = bl.z - cz * d.nz;
-  const int comp = cz % 3;
+  const int comp = cz % 3, f = cz / 3;
   const int x0 = bl.x * kG3TX, y0 = bl.y * kG3TY;
   const long long n = d.n();
-  const float* src = (cz < 3 ? in : in1) + comp * n + (long long)d.nx * d.ny * z;
-  float* dst = (cz < 3 ? out : out1) + comp * n + (long long)d.nx * d.ny * z;
+  const float* src = p.in[f] + comp * n + (long long)d.nx * d.ny * z;
+  float* dst = p.out[f] + comp * n + (long long)d.nx * d.ny * z;
   const int tid = threadIdx.x;
   for (int e = tid; e < LX * LY; e += 256) {
     const int ly = e / LX, lx = e - ly * LX;
@@ -1042,8 +1213,7 @@ __global__ __launch_bounds__(256) void k_gauss_xy(const float* __restrict__ in, 
 // reads its 4 + 2R inputs as float4, the loops have no run-time bounds, and the tap chains are
 // those of k_gauss_xy, so the results are the same bits.
 template <int R>
-__global__ __launch_bounds__(256) void k_gauss_xy_r(const float* __restrict__ in, float* __restrict__ out, Dims d,
-                                                    GaussTaps3 t, const float* __restrict__ in1, float* __restrict__ out1) {
+__global__ __launch_bounds__(256) void k_gauss_xy_r(FieldPtrs p, Dims d, GaussTaps3 t) {
   constexpr int LX = kG3TX + 2 * R, LY = kG3TY + 2 * R;
   constexpr int NV = (4 + 2 * R + 3) / 4;  // float4 reads per x run
   constexpr int LXP0 = (LX + 3) / 4 * 4;
@@ -1052,11 +1222,11 @@ __global__ __launch_bounds__(256) void k_gauss_xy_r(const float* __restrict__ in
   __shared__ __align__(16) float xs[LY][kG3TX];
   const uint3 bl = xcd_block();
   const int cz = bl.z / d.nz, z = bl.z - cz * d.nz;
-  const int comp = cz % 3;
+  const int comp = cz % 3, f = cz / 3;
   const int x0 = bl.x * kG3TX, y0 = bl.y * kG3TY;
   const long long n = d.n();
-  const float* src = (cz < 3 ? in : in1) + comp * n + (long long)d.nx * d.ny * z;
-  float* dst = (cz < 3 ? out : out1) + comp * n + (long long)d.nx * d.ny * z;
+  const float* src = p.in[f] + comp * n + (long long)d.nx * d.ny * z;
+  float* dst = p.out[f] + comp * n + (long long)d.nx * d.ny * z;
   const int tid = threadIdx.x;
   for (int ly = tid >> 6; ly < LY; ly += 4) {
     int gy = y0 - R + ly;
@@ -1119,12 +1289,10 @@ __global__ __launch_bounds__(256) void k_gauss_xy_r(const float* __restrict__ in
 // smoothed along x and y exactly as in k_gauss_xy_r; a thread keeps the x/y results of its four
 // outputs for the last 2R + 1 planes in registers and finishes the z pass, the blend and the zeroed
 // faces as k_gauss_z does, so the results are the bits of the two-pass form. The next plane's tile
-// is fetched into registers while the current one is smoothed. Grid z runs over z chunk x
-// component; with `fields` 2, components 3..5 read in1 and write out1.
+// is fetched into registers while the current one is smoothed. Grid z runs over z chunk x field x
+// component: field f reads p.in[f] (also the blend source) and writes p.out[f].
 template <int R>
-__global__ __launch_bounds__(256) void k_gauss3(const float* __restrict__ in, float* __restrict__ out, Dims d, GaussTaps3 t,
-                                                int zrun, float w1, float w2, int fields, const float* __restrict__ in1,
-                                                float* __restrict__ out1) {
+__global__ __launch_bounds__(256) void k_gauss3(FieldPtrs p, Dims d, GaussTaps3 t, int zrun, float w1, float w2) {
   constexpr int W = 2 * R + 1;
   constexpr int LX = kG3TX + 2 * R, LY = kG3TY + 2 * R;
   constexpr int NV = (4 + 2 * R + 3) / 4;
@@ -1134,11 +1302,12 @@ __global__ __launch_bounds__(256) void k_gauss3(const float* __restrict__ in, fl
   __shared__ __align__(16) float tile[LY][LXP];
   __shared__ __align__(16) float xs[LY][kG3TX];
   const uint3 bl = xcd_block();
+  const int fields = p.count;
   const int cz = bl.z % (3 * fields), chunk = bl.z / (3 * fields);
-  const int comp = cz % 3;
+  const int comp = cz % 3, f = cz / 3;
   const long long n = d.n(), plane = (long long)d.nx * d.ny;
-  const float* src = (cz < 3 ? in : in1) + comp * n;
-  float* dst = (cz < 3 ? out : out1) + comp * n;
+  const float* src = p.in[f] + comp * n;
+  float* dst = p.out[f] + comp * n;
   const int x0 = bl.x * kG3TX, y0 = bl.y * kG3TY;
   const int z0 = chunk * zrun, z1 = min(z0 + zrun, d.nz);
   const int tid = threadIdx.x;
@@ -1256,21 +1425,17 @@ static int g3_zrun(const Dims& d, int r, int fields) {
 }
 
 template <int RZ>
-// Grid z runs over z chunk x component; with `fields` 2, components 3..5 read in1 / orig1 and
-// write out1.
-__global__ __launch_bounds__(256) void k_gauss_z(const float* __restrict__ in, const float* __restrict__ orig,
-                                                 float* __restrict__ out, Dims d, GaussTaps3 t, int zrun, float w1,
-                                                 float w2, int fields, const float* __restrict__ in1,
-                                                 const float* __restrict__ orig1, float* __restrict__ out1) {
+// Grid z runs over z chunk x field x component: field f reads p.in[f] (the x/y pass's result) and
+// p.orig[f] (the blend source) and writes p.out[f].
+__global__ __launch_bounds__(256) void k_gauss_z(FieldPtrs p, Dims d, GaussTaps3 t, int zrun, float w1, float w2) {
   constexpr int W = 2 * RZ + 1;
   const uint3 bl = xcd_block();
+  const int fields = p.count;
   const int cz = bl.z % (3 * fields), chunk = bl.z / (3 * fields);
-  const int comp = cz % 3;
-  if (cz >= 3) {
-    in = in1;
-    orig = orig1;
-    out = out1;
-  }
+  const int comp = cz % 3, f = cz / 3;
+  const float* __restrict__ in = p.in[f];
+  const float* __restrict__ orig = p.orig[f];
+  float* __restrict__ out = p.out[f];
   const int x = bl.x * 64 + (threadIdx.x & 63);
   const int y = bl.y * 4 + (threadIdx.x >> 6);
   if (x >= d.nx || y >= d.ny) return;
@@ -1320,16 +1485,31 @@ bool gauss3_fits(const Dims& d, const GaussTaps3& t) {
   return t.rx <= kG3MaxR && t.ry <= kG3MaxR && t.rz <= kG3MaxR && t.rz >= 1 && 6LL * d.nz <= 65535;
 }
 
-void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s, const float* in1,
-                     float* tmp1) {
-  const int fields = in1 ? 2 : 1;
-  const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * fields * d.nz);
+static FieldPtrs field_ptrs(const float* in0, const float* orig0, float* out0, const float* in1, const float* orig1,
+                            float* out1) {
+  FieldPtrs p{};
+  p.in[0] = in0;
+  p.orig[0] = orig0;
+  p.out[0] = out0;
+  p.count = 1;
+  if (in1) {
+    p.in[1] = in1;
+    p.orig[1] = orig1;
+    p.out[1] = out1;
+    p.count = 2;
+  }
+  return p;
+}
+
+bool launch_gauss_xy_set(const FieldPtrs& p, Dims d, const GaussTaps3& t, hipStream_t s) {
+  if (p.count < 1 || p.count > kBatchFields || 3LL * p.count * d.nz > 65535) return false;
+  const dim3 gxy((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * p.count * d.nz);
   if (t.rx == t.ry) {
     switch (t.rx) {
-#define FW_GXY_CASE(R)                                                        \
-  case R:                                                                     \
-    launch(k_gauss_xy_r<R>, gxy, dim3(256), 0, s, in, tmp, d, t, in1, tmp1); \
-    return;
+#define FW_GXY_CASE(R)                                        \
+  case R:                                                     \
+    launch(k_gauss_xy_r<R>, gxy, dim3(256), 0, s, p, d, t);  \
+    return true;
       FW_GXY_CASE(1)
       FW_GXY_CASE(2)
       FW_GXY_CASE(3)
@@ -1343,21 +1523,27 @@ void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, h
         break;
     }
   }
-  launch(k_gauss_xy, gxy, dim3(256), 0, s, in, tmp, d, t, in1, tmp1);
+  launch(k_gauss_xy, gxy, dim3(256), 0, s, p, d, t);
+  return true;
+}
+
+void launch_gauss_xy(const float* in, float* tmp, Dims d, const GaussTaps3& t, hipStream_t s, const float* in1,
+                     float* tmp1) {
+  launch_gauss_xy_set(field_ptrs(in, nullptr, tmp, in1, nullptr, tmp1), d, t, s);
 }
 
 bool gauss3_pays(const Dims& d) { return d.n() >= 4000000; }
 
-bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s,
-                   const float* in1, float* out1) {
+bool launch_gauss3_set(const FieldPtrs& p, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s) {
   if (t.rx != t.ry || t.ry != t.rz) return false;
-  const int fields = in1 ? 2 : 1;
-  const int run = g3_zrun(d, t.rz, fields);
-  const dim3 g((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * fields * ((d.nz + run - 1) / run));
+  if (p.count < 1 || p.count > kBatchFields) return false;
+  const int run = g3_zrun(d, t.rz, p.count);
+  const dim3 g((d.nx + kG3TX - 1) / kG3TX, (d.ny + kG3TY - 1) / kG3TY, 3 * p.count * ((d.nz + run - 1) / run));
+  if (g.z > 65535) return false;
   switch (t.rz) {
-#define FW_G3_CASE(R)                                                                      \
-  case R:                                                                                  \
-    launch(k_gauss3<R>, g, dim3(256), 0, s, in, out, d, t, run, w1, w2, fields, in1, out1); \
+#define FW_G3_CASE(R)                                                  \
+  case R:                                                              \
+    launch(k_gauss3<R>, g, dim3(256), 0, s, p, d, t, run, w1, w2);    \
     return true;
     FW_G3_CASE(1)
     FW_G3_CASE(2)
@@ -1373,16 +1559,20 @@ bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, flo
   }
 }
 
-void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
-                    hipStream_t s, const float* tmp1, const float* in1, float* out1) {
+bool launch_gauss3(const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s,
+                   const float* in1, float* out1) {
+  return launch_gauss3_set(field_ptrs(in, nullptr, out, in1, nullptr, out1), d, t, w1, w2, s);
+}
+
+bool launch_gauss_z_set(const FieldPtrs& p, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s) {
   const int run = gz_zrun(d, t.rz);
-  const int fields = tmp1 ? 2 : 1;
-  const dim3 gz((d.nx + 63) / 64, (d.ny + 3) / 4, 3 * fields * ((d.nz + run - 1) / run));
+  const dim3 gz((d.nx + 63) / 64, (d.ny + 3) / 4, 3 * p.count * ((d.nz + run - 1) / run));
+  if (p.count < 1 || p.count > kBatchFields || gz.z > 65535) return false;
   switch (t.rz) {
-#define FW_GZ_CASE(R)                                                                              \
-  case R:                                                                                          \
-    launch(k_gauss_z<R>, gz, dim3(256), 0, s, tmp, in, out, d, t, run, w1, w2, fields, tmp1, in1, out1); \
-    break;
+#define FW_GZ_CASE(R)                                                  \
+  case R:                                                              \
+    launch(k_gauss_z<R>, gz, dim3(256), 0, s, p, d, t, run, w1, w2);  \
+    return true;
     FW_GZ_CASE(1)
     FW_GZ_CASE(2)
     FW_GZ_CASE(3)
@@ -1393,8 +1583,13 @@ void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const
     FW_GZ_CASE(8)
 #undef FW_GZ_CASE
     default:
-      break;
+      return false;
   }
+}
+
+void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const GaussTaps3& t, float w1, float w2,
+                    hipStream_t s, const float* tmp1, const float* in1, float* out1) {
+  launch_gauss_z_set(field_ptrs(tmp, in, out, tmp1, in1, out1), d, t, w1, w2, s);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1603,10 +1798,10 @@ struct InvRange {
   unsigned chunk;
   long long lo, hi;
 };
-__device__ __forceinline__ InvRange inv_range(long long n) {
-  const long long len = ((n + gridDim.x - 1) / gridDim.x + kInvThreads - 1) / kInvThreads * kInvThreads;
+__device__ __forceinline__ InvRange inv_range(long long n, unsigned blk, unsigned nblk) {
+  const long long len = ((n + nblk - 1) / nblk + kInvThreads - 1) / kInvThreads * kInvThreads;
   InvRange r;
-  r.chunk = xcd_slab(blockIdx.x, gridDim.x);
+  r.chunk = xcd_slab(blk, nblk);
   r.lo = (long long)r.chunk * len;
   r.hi = r.lo + len < n ? r.lo + len : n;
   return r;
@@ -1622,12 +1817,12 @@ __device__ __forceinline__ void inv_coords(const InvIndex& ix, long long g, int&
   k = (int)q;
 }
 
-__global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restrict__ field, const float* __restrict__ psi,
-                                                           float* __restrict__ res, Dims d, InvIndex ix, L9 l, float3 inv_sp,
-                                                           InvertState* state, InvertReport* report, float* partials,
-                                                           int max_iterations) {
+__device__ __forceinline__ void inv_begin_block(const float* __restrict__ field, const float* __restrict__ psi,
+                                                float* __restrict__ res, const Dims& d, const InvIndex& ix, const L9& l,
+                                                float3 inv_sp, InvertState* state, InvertReport* report, float* partials,
+                                                int max_iterations, unsigned blk, unsigned nblk) {
   const long long n = d.n();
-  const InvRange rg = inv_range(n);
+  const InvRange rg = inv_range(n, blk, nblk);
   float s = 0.0f, c = 0.0f, m = 0.0f;
   for (long long base = rg.lo + threadIdx.x; base < rg.hi; base += kInvThreads * kInvVoxels) {
 #pragma unroll
@@ -1647,8 +1842,8 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
       }
     }
   }
-  inv_write_partials(s, m, partials, gridDim.x, rg.chunk);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  inv_write_partials(s, m, partials, nblk, rg.chunk);
+  if (blk == 0 && threadIdx.x == 0) {
     state->mean_error = 0.0f;
     state->max_error = 0.0f;
     state->executed = 0;
@@ -1660,19 +1855,21 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restri
   }
 }
 
-__global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi,
-                                                          float* __restrict__ res, Dims d, InvIndex ix, L9 l, float3 inv_sp,
-                                                          InvertState* state, InvertReport* report, float* partials, int step,
-                                                          int max_iterations, float max_tol, float mean_tol, int enforce,
-                                                          uint32_t* mailbox, uint32_t tag) {
+// One block of iteration `step`; `stop_word` receives the tag once the iteration loop ends.
+__device__ __forceinline__ void inv_iter_block(const float* __restrict__ field, float* __restrict__ psi,
+                                               float* __restrict__ res, const Dims& d, const InvIndex& ix, const L9& l,
+                                               float3 inv_sp, InvertState* state, InvertReport* report, float* partials,
+                                               int step, int max_iterations, float max_tol, float mean_tol, int enforce,
+                                               uint32_t* mailbox, uint32_t* stop_word, uint32_t tag, unsigned blk,
+                                               unsigned nblk) {
   if (state->stop[step & 1]) {
     // once stopped, stay stopped: the next launch reads the other word
-    if (blockIdx.x == 0 && threadIdx.x == 0) state->stop[(step + 1) & 1] = 1;
+    if (blk == 0 && threadIdx.x == 0) state->stop[(step + 1) & 1] = 1;
     return;
   }
-  const int blocks = gridDim.x;
+  const int blocks = (int)nblk;
   // progress hint for the host: this inversion has started iteration `step`
-  if (blockIdx.x == 0 && threadIdx.x == 0)
+  if (blk == 0 && threadIdx.x == 0)
     __hip_atomic_store(mailbox + 1, ((tag & 0xffffffu) << 8) | (uint32_t)(step + 1), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
   const float* in = partials + (step & 1) * 2 * blocks;
@@ -1696,7 +1893,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
   const float mean = stats[0] / (float)n;
   const float mx = stats[1];
   const bool next = (step + 1) < max_iterations && mx > max_tol && mean > mean_tol;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     state->mean_error = mean;
     state->max_error = mx;
     state->executed = step + 1;
@@ -1705,11 +1902,11 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
     report->max_error = mx;
     report->executed = step + 1;
     // a hint for the host: later launches of this inversion would return at once
-    if (!next) __hip_atomic_store(mailbox, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (!next) __hip_atomic_store(stop_word, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   const float eps = step == 0 ? 0.75f : 0.5f;
   const float limit = eps * mx;
-  const InvRange rg = inv_range(n);
+  const InvRange rg = inv_range(n, blk, nblk);
   s = 0.0f;
   c = 0.0f;
   m = 0.0f;
@@ -1750,6 +1947,41 @@ __global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restric
   if (next) inv_write_partials(s, m, out, blocks, rg.chunk);
 }
 
+__global__ __launch_bounds__(kInvThreads) void k_inv_begin(const float* __restrict__ field, const float* __restrict__ psi,
+                                                           float* __restrict__ res, Dims d, InvIndex ix, L9 l, float3 inv_sp,
+                                                           InvertState* state, InvertReport* report, float* partials,
+                                                           int max_iterations) {
+  inv_begin_block(field, psi, res, d, ix, l, inv_sp, state, report, partials, max_iterations, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kInvThreads) void k_inv_iter(const float* __restrict__ field, float* __restrict__ psi,
+                                                          float* __restrict__ res, Dims d, InvIndex ix, L9 l, float3 inv_sp,
+                                                          InvertState* state, InvertReport* report, float* partials, int step,
+                                                          int max_iterations, float max_tol, float mean_tol, int enforce,
+                                                          uint32_t* mailbox, uint32_t tag) {
+  inv_iter_block(field, psi, res, d, ix, l, inv_sp, state, report, partials, step, max_iterations, max_tol, mean_tol, enforce,
+                 mailbox, mailbox, tag, blockIdx.x, gridDim.x);
+}
+
+// Several inversions on one grid: item q takes blocks [q * blocks, (q + 1) * blocks), the blocks
+// of its own launch, and stores its stop hint in mailbox[kInvItemWords + q].
+__global__ __launch_bounds__(kInvThreads) void k_inv_begin_items(InvItems it, Dims d, InvIndex ix, L9 l, float3 inv_sp,
+                                                                 int blocks, int max_iterations) {
+  const unsigned q = blockIdx.x / (unsigned)blocks;
+  inv_begin_block(it.field[q], it.psi[q], it.res[q], d, ix, l, inv_sp, it.state[q], it.report[q], it.partials[q],
+                  max_iterations, blockIdx.x - q * blocks, blocks);
+}
+
+__global__ __launch_bounds__(kInvThreads) void k_inv_iter_items(InvItems it, Dims d, InvIndex ix, L9 l, float3 inv_sp,
+                                                                int blocks, int step, int max_iterations, float max_tol,
+                                                                float mean_tol, int enforce, uint32_t* mailbox,
+                                                                uint32_t tag) {
+  const unsigned q = blockIdx.x / (unsigned)blocks;
+  inv_iter_block(it.field[q], it.psi[q], it.res[q], d, ix, l, inv_sp, it.state[q], it.report[q], it.partials[q], step,
+                 max_iterations, max_tol, mean_tol, enforce, mailbox, mailbox + kInvItemWords + q, tag,
+                 blockIdx.x - q * blocks, blocks);
+}
+
 int inv_blocks(const Dims& d, int cap) {
   const long long b = (d.n() + kInvThreads * kInvVoxels - 1) / (kInvThreads * kInvVoxels);
   const long long c = cap < 1 ? 1 : cap;
@@ -1763,6 +1995,25 @@ void launch_inv_begin(const float* field, const float* psi, float* residual, Dim
   for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
   launch(k_inv_begin, dim3(blocks), dim3(kInvThreads), 0, s, field, psi, residual, d, inv_index(d), lv, inv_spacing, state,
          report, partials, max_iterations);
+}
+
+void launch_inv_begin_items(const InvItems& it, Dims d, const float l[9], float3 inv_spacing, int blocks, int max_iterations,
+                            hipStream_t s) {
+  if (it.count < 1 || it.count > kBatchFields) return;
+  L9 lv;
+  for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
+  launch(k_inv_begin_items, dim3(blocks * it.count), dim3(kInvThreads), 0, s, it, d, inv_index(d), lv, inv_spacing, blocks,
+         max_iterations);
+}
+
+void launch_inv_iter_items(const InvItems& it, Dims d, const float l[9], float3 inv_spacing, int blocks, int step,
+                           int max_iterations, double max_tol, double mean_tol, int enforce_boundary, uint32_t* mailbox,
+                           uint32_t tag, hipStream_t s) {
+  if (it.count < 1 || it.count > kBatchFields) return;
+  L9 lv;
+  for (int q = 0; q < 9; ++q) lv.v[q] = l[q];
+  launch(k_inv_iter_items, dim3(blocks * it.count), dim3(kInvThreads), 0, s, it, d, inv_index(d), lv, inv_spacing, blocks,
+         step, max_iterations, (float)max_tol, (float)mean_tol, enforce_boundary, mailbox, tag);
 }
 
 void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, const float l[9], float3 inv_spacing,

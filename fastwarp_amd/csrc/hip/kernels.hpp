@@ -201,6 +201,83 @@ void launch_inv_iter(const float* field, float* psi, float* residual, Dims d, co
                      double max_tol, double mean_tol, int enforce_boundary, uint32_t* mailbox, uint32_t tag, hipStream_t s);
 int inv_blocks(const Dims& d, int cap = 2048);
 
+// --- batches: one launch over several items on one grid (the subjects of a batched SyN level).
+// Every item's results are the bits its own launch gives: the items only share the dispatch.
+constexpr int kBatchFields = 16;  // fields per smoothing, composition or inversion launch
+constexpr int kBatchItems = 8;    // warps, CC pairs, norm pairs or step pairs per launch
+constexpr int kBatchChannels = 4; // channels combined per update
+// Field pointers of a batched field pass: `in` is read, `orig` is the second input (the blend
+// source of a z pass, the warping field of a composition), `out` is written.
+struct FieldPtrs {
+  const float* in[kBatchFields];
+  const float* orig[kBatchFields];
+  float* out[kBatchFields];
+  int count;
+};
+struct WarpItems {
+  WarpArgs a[kBatchItems];
+  int count;
+};
+void launch_warp_items(const WarpItems& w, hipStream_t s);
+struct CcItems {
+  CcArgs a[kBatchItems];  // one grid (d, radius, tile_z, probes) for every item; no masks
+  int count;
+};
+void launch_cc_items(const CcItems& c, hipStream_t s);
+struct NormItems {
+  const float* a[kBatchItems];
+  const float* b[kBatchItems];
+  float* partial_max[kBatchItems];  // 2 * blocks per item, as launch_field_norms lays them out
+  float* partial_sq[kBatchItems];
+  int count;
+};
+void launch_field_norms_items(const NormItems& it, long long n, float3 inv_spacing, int blocks, hipStream_t s);
+// Per item: launch_step_factors of its two fields, factors[2 q + f] and maxima[2 q + f].
+void launch_step_factors_items(const NormItems& it, int blocks, double step, float* factors, float* maxima, hipStream_t s);
+// The channel weighting of a multi-metric update (syn.rs:203-297), per subject and side: the
+// magnitude |d_c| = sqrt(sum of its squares, the block partials added in order in f64), the
+// coefficient w_c / |d_c| (f64, rounded once) for the channels with |d_c| > eps, and the scale
+// (sum of the magnitudes) / channels. coef holds per (subject, side) kBatchChannels coefficients,
+// then the scale, then a use mask (as bits); bad[q] is set for a non-finite magnitude of subject q.
+struct WeightItems {
+  const float* partial_sq[kBatchItems][kBatchChannels];  // the channel's launch_field_norms partials
+  double weight[kBatchChannels];
+  int channels;
+  int count;
+};
+void launch_channel_weights(const WeightItems& w, int blocks, float* coef, uint32_t* bad, hipStream_t s);
+// out[q] = (fma chain over the used channels c of coef_c * d[q][c], from zero) * scale, item q
+// reading coef + q * (kBatchChannels + 2).
+struct CombineItems {
+  const float* d[kBatchFields][kBatchChannels];
+  float* out[kBatchFields];
+  int channels;
+  int count;
+};
+void launch_combine_items(const CombineItems& c, const float* coef, long long count, hipStream_t s);
+// The batched smoothing passes (false, and nothing launched, when the grid would not fit).
+bool launch_gauss_xy_set(const FieldPtrs& p, Dims d, const GaussTaps3& t, hipStream_t s);
+bool launch_gauss_z_set(const FieldPtrs& p, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s);
+bool launch_gauss3_set(const FieldPtrs& p, Dims d, const GaussTaps3& t, float w1, float w2, hipStream_t s);
+// out = w + u(x + w) with u = in, w = orig; with factors, u is scaled by factors[f] as it is read.
+void launch_compose_set(const FieldPtrs& p, Dims d, const float l[9], const float* factors, hipStream_t s);
+struct InvItems {
+  const float* field[kBatchFields];
+  float* psi[kBatchFields];
+  float* res[kBatchFields];
+  InvertState* state[kBatchFields];
+  InvertReport* report[kBatchFields];
+  float* partials[kBatchFields];  // 4 * blocks floats each
+  int count;
+};
+// The mailbox word where item q of a batched inversion stores the tag once it has stopped.
+constexpr int kInvItemWords = 8;
+void launch_inv_begin_items(const InvItems& it, Dims d, const float l[9], float3 inv_spacing, int blocks, int max_iterations,
+                            hipStream_t s);
+void launch_inv_iter_items(const InvItems& it, Dims d, const float l[9], float3 inv_spacing, int blocks, int step,
+                           int max_iterations, double max_tol, double mean_tol, int enforce_boundary, uint32_t* mailbox,
+                           uint32_t tag, hipStream_t s);
+
 // --- recursive Gaussian (Deriche, 4th order) along one axis, constant extension at both ends ---
 struct DericheCoefficients {
   double n[4], d[4], m[4], bn[4], bm[4];

@@ -78,6 +78,12 @@ class SmoothCache {
 
   B& backend() { return b_; }
   const Grid& fixed_grid(size_t channel) const { return grids_.at(channel); }
+
+  // Smoothed fixed images shared between the caches of one template's subjects (each cache holds
+  // the same unsmoothed fixed images): a level of the template is smoothed once, by whichever
+  // cache asks for it first, and the others take that image.
+  using SharedFixed = std::map<std::pair<size_t, double>, Ptr>;
+  void share_fixed(std::shared_ptr<SharedFixed> shared) { shared_ = std::move(shared); }
   size_t channels() const { return grids_.size(); }
 
   Pair get(size_t channel, double sigma, SigmaUnits units) {
@@ -92,7 +98,13 @@ class SmoothCache {
     } else {
       const Pair raw = get(channel, 0.0, units);
       HostTimer t("smooth_images");
-      p.fixed = std::make_shared<const ImageH>(b_.smooth_image(*raw.fixed, sigma, units));
+      if (shared_) {
+        Ptr& f = (*shared_)[{channel, key}];
+        if (!f) f = std::make_shared<const ImageH>(b_.smooth_image(*raw.fixed, sigma, units));
+        p.fixed = f;
+      } else {
+        p.fixed = std::make_shared<const ImageH>(b_.smooth_image(*raw.fixed, sigma, units));
+      }
       p.moving = std::make_shared<const ImageH>(b_.smooth_image(*raw.moving, sigma, units));
     }
     pairs_[{channel, key}] = p;
@@ -117,6 +129,7 @@ class SmoothCache {
   std::vector<Grid> grids_;
   std::map<std::pair<size_t, double>, Pair> pairs_;
   std::map<std::pair<size_t, double>, std::pair<double, double>> ranges_;
+  std::shared_ptr<SharedFixed> shared_;
 };
 
 // mi.rs:193-207 `fixed_sample_range`: count and range of the finite nearest-voxel values.

@@ -90,6 +90,32 @@ def register_resident(backend: HipBackend, plan: Plan, fixed: list[DeviceImage],
     return ResidentRegistration(res.affine, wrap(fwd) if want else None, wrap(inv) if want else None, res)
 
 
+def register_batch(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], moving: list[list[DeviceImage]],
+                   fields: bool = True, cancel: C.c_int32 | None = None) -> list[ResidentRegistration]:
+    """Several subjects (moving[s] holds subject s's channels) registered to one template (the
+    fixed images): the template's preprocessing, centre of mass and smoothed levels are shared, and
+    the SyN stages run batched on the device (fw_register_batch). Each subject's result is the
+    bits of its own register_resident run."""
+    h = _sig()
+    ns = len(moving)
+    fp = (C.c_void_p * len(fixed))(*[im.handle for im in fixed])
+    mp = (C.c_void_p * (ns * len(fixed)))(*[im.handle for subj in moving for im in subj])
+    want = fields and plan.run_syn
+    fwd, inv = (C.c_void_p * ns)(), (C.c_void_p * ns)()
+    res = (Result * ns)()
+    st = h.fw_register_batch(backend.ctx, C.byref(plan), fp, ns, mp, fwd if want else None, inv if want else None, res,
+                             C.byref(cancel) if cancel is not None else None)
+    if st != FW_OK:
+        raise RegError(st, h.fw_last_error(backend.ctx).decode())
+    wrap = lambda p: DeviceField(backend, C.c_void_p(p)) if p else None
+    return [ResidentRegistration(res[s].affine, wrap(fwd[s]) if want else None, wrap(inv[s]) if want else None, res[s])
+            for s in range(ns)]
+
+
+def last_batch_trace(subject: int) -> list[TraceStep]:
+    """last_trace for one subject of this thread's last register_batch call."""
+    return read_trace(lambda out, cap: lib().fw_last_batch_trace(subject, out, cap))
+
 
 def last_trace() -> list[TraceStep]:
     """Per-iteration records (CG values, SyN energies and inversions) of this thread's last

@@ -1,0 +1,66 @@
+"""Batched registration of several subjects against one template (fw_register_batch): the SyN
+iterations of all subjects run in lockstep, each step one launch over every subject, and the
+template's smoothed levels are built once. Each subject's result must be the bits of its own
+fw_register_resident run (crates/fastwarp-reg/src/syn.rs:776-986 per subject): the same affine,
+the same SyN iteration counts, energies and inversion step counts per iteration, and the same
+fields. Those single runs are themselves held to the CPU engine by tests/test_register.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from fastwarp_amd.phantom import make_pair
+from fastwarp_amd.register import (last_batch_trace, last_trace, plan_2mm, register_batch,
+                                   register_resident)
+
+pytestmark = pytest.mark.gpu
+
+
+def affine_array(a) -> np.ndarray:
+    return np.array(list(a.matrix) + list(a.translation) + list(a.center))
+
+
+def steps(trace):
+    return [(t.stage, t.level, t.iteration, t.value, tuple(t.inversion_steps), t.update_scale[0], t.update_scale[1])
+            for t in trace]
+
+
+def cohort(channels: int, subjects: int, shape=(96, 112, 96), spacing=2.0):
+    """One template (the fixed channels of the first phantom) and the moving channels of
+    `subjects` phantoms of other seeds."""
+    template = make_pair(shape, spacing, seed=11, channels=channels)
+    moving = [make_pair(shape, spacing, seed=20 + s, channels=channels).moving for s in range(subjects)]
+    return template.grid, template.fixed, moving
+
+
+@pytest.mark.parametrize("channels,subjects", [(1, 3), (2, 2)])
+def test_batched_subjects_give_the_bits_of_their_single_runs(backend, channels, subjects):
+    grid, fixed, moving = cohort(channels, subjects)
+    plan = plan_2mm(channels)
+    plan.set_linear([30, 15, 8], [4, 2, 1], [2.0, 1.0, 0.0])
+    up = lambda vols: [backend.upload_image(grid, v) for v in vols]
+    tmpl = up(fixed)
+    subj = [up(m) for m in moving]
+    batch = register_batch(backend, plan, tmpl, subj)
+    traces = [steps(last_batch_trace(s)) for s in range(subjects)]
+    for s in range(subjects):
+        alone = register_resident(backend, plan, tmpl, subj[s])
+        want = steps(last_trace())
+        got = batch[s]
+        label = f"subject {s} of {subjects}, {channels} channel(s)"
+        assert np.array_equal(affine_array(got.affine), affine_array(alone.affine)), label
+        assert list(got.result.syn_iterations)[: plan.syn_levels] == list(alone.result.syn_iterations)[: plan.syn_levels], label
+        assert traces[s] == want, label
+        for name in ("forward", "inverse"):
+            a = backend.download_field(getattr(got, name))
+            b = backend.download_field(getattr(alone, name))
+            assert np.array_equal(a, b), (label, name, float(np.max(np.abs(a - b))))
+
+
+def test_batch_without_subjects_is_an_error(backend):
+    from fastwarp_amd.backend import RegError
+
+    grid, fixed, _ = cohort(1, 0, shape=(24, 24, 24), spacing=4.0)
+    with pytest.raises(RegError, match="no subjects"):
+        register_batch(backend, plan_2mm(1), [backend.upload_image(grid, fixed[0])], [])
