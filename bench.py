@@ -189,6 +189,61 @@ def batched(backend, subjects: int) -> dict:
                     "with one launch per step over all subjects; fields left in HBM"}
 
 
+def fit_sizes(fits: int, seed: int = 3):
+    """The batched linear workload's pairs: grids drawn uniformly from 64^3 to 192 x 224 x 192,
+    isotropic spacing drawn from 1 to 2 mm, one phantom seed per pair, random affine only."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(fits):
+        shape = (int(rng.integers(64, 193)), int(rng.integers(64, 225)), int(rng.integers(64, 193)))
+        out.append((shape, round(float(rng.uniform(1.0, 2.0)), 3), 1000 + i))
+    return out
+
+
+def make_fit(args):
+    from fastwarp_amd.phantom import make_pair
+
+    shape, spacing, seed = args
+    p = make_pair(shape, spacing, seed=seed, affine_only=True)
+    g = p.grid
+    return (shape, tuple(g.spacing), tuple(g.origin), p.fixed[0].astype(np.float32), p.moving[0].astype(np.float32))
+
+
+def linear_fits(backend, data, workers: int) -> dict:
+    """The batched linear workload: len(data) independent rigid+affine Mattes MI fits of uneven
+    sizes (the cohort's linear recipe: 32 bins, 25 % regular sampling, shrink 8/4/2/1), raw volumes
+    resident in HBM, side by side on `workers` worker streams (fw_linear_batch), against the same
+    fits one at a time (fw_register_resident without SyN)."""
+    from fastwarp_amd._abi import Grid
+    from fastwarp_amd.register import cohort_plan, linear_batch, register_resident
+
+    plan = cohort_plan(1)
+    plan.run_syn = 0
+    fixed, moving = [], []
+    voxels = 0
+    for shape, spacing, origin, f, m in data:
+        g = Grid.make(shape, spacing, origin)
+        fixed.append(backend.upload_image(g, f))
+        moving.append(backend.upload_image(g, m))
+        voxels += int(np.prod(shape))
+    linear_batch(backend, plan, fixed[:workers], moving[:workers], workers=workers)  # warm the workers
+    t = time.perf_counter()
+    linear_batch(backend, plan, fixed, moving, workers=workers)
+    batch_wall = time.perf_counter() - t
+    t = time.perf_counter()
+    for f, m in zip(fixed, moving):
+        register_resident(backend, plan, [f], [m], fields=False)
+    plain_wall = time.perf_counter() - t
+    n = len(data)
+    log(f"linear fits: {n} fits ({voxels / n / 1e6:.2f} M voxels on average) in {batch_wall:.2f} s on {workers} workers; "
+        f"one at a time {plain_wall:.2f} s")
+    return {"value": round(3600.0 * n / batch_wall, 3), "unit": "fits/hour", "fits": n, "workers": workers,
+            "mean_voxels": int(voxels / n), "plain_per_hour": round(3600.0 * n / plain_wall, 3),
+            "vs_plain": round(plain_wall / batch_wall, 3),
+            "path": "fw_linear_batch: raw volumes resident in HBM, fits taken in turn by worker contexts with their own "
+                    "HIP streams"}
+
+
 def kernels(repeats: int) -> int:
     """Each hot-path op on the 1 mm pair at full resolution (182x218x182), `repeats` launches
     after one warm-up: one JSON line per kernel family with its mean launch time (HIP events on
@@ -264,6 +319,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=256, help="requests of the served stream")
     ap.add_argument("--load", type=float, default=0.8, help="arrival rate of the served stream as a share of saturation")
     ap.add_argument("--batch", type=int, default=8, help="subjects of the batched two-channel workload (0 skips it)")
+    ap.add_argument("--fits", type=int, default=64, help="fits of the batched linear workload (0 skips it)")
+    ap.add_argument("--workers", type=int, default=8, help="worker streams of the batched linear fits")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for {args.gpus}): registrations are independent and would shard across "
@@ -271,6 +328,16 @@ def main() -> int:
         return 2
     if args.kernels:
         return kernels(max(args.steps, 1) * 10)
+
+    fit_data = []
+    if args.fits > 0:
+        # generated in worker processes before anything touches the GPU
+        from concurrent.futures import ProcessPoolExecutor
+
+        t = time.perf_counter()
+        with ProcessPoolExecutor(max_workers=8) as pool:
+            fit_data = list(pool.map(make_fit, fit_sizes(args.fits)))
+        log(f"linear fits: {args.fits} phantom pairs ready in {time.perf_counter() - t:.1f} s")
 
     from fastwarp_amd.backend import HipBackend
     from fastwarp_amd.phantom import CONFIGS, make_pair
@@ -374,6 +441,14 @@ def main() -> int:
         "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "gbs": round(v[2] / max(v[1], 1e-9) / 1e6, 1)}
                     for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][1])},
     }
+    if fit_data:
+        backend.release_idle_buffers()
+        try:
+            line["linear_fits"] = linear_fits(backend, fit_data, args.workers)
+        except Exception as e:  # the plain line is still reported
+            log(f"batched linear fits failed: {e}")
+            line["linear_fits"] = {"error": str(e)}
+        fit_data = None
     if args.batch > 0:
         backend.release_idle_buffers()
         try:

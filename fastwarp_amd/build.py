@@ -21,7 +21,7 @@ SOURCES = [
     CSRC / "hip" / "backend.hip",
     CSRC / "hip" / "mi_backend.hip",
     CSRC / "hip" / "noalloc.hip",
-    CSRC / "hip" / "batch_syn.hip",
+    CSRC / "hip" / "batch.hip",
     CSRC / "capi.cpp",
     CSRC / "serve.cpp",
 ]

@@ -7,7 +7,7 @@
 #include <string>
 
 #include "hip/backend.hpp"
-#include "hip/batch_syn.hpp"
+#include "hip/batch.hpp"
 #include "host/capacity.hpp"
 #include "host/itk_mat.hpp"
 #include "plan.hpp"
@@ -19,8 +19,9 @@ using fw::hip::HipBackend;
 
 struct fw_ctx {
   HipBackend backend;
+  fw::hip::WorkerPool workers;  // the side-by-side linear fits of the batched entry points
   std::string error;
-  explicit fw_ctx(int device) : backend(device) {}
+  explicit fw_ctx(int device) : backend(device), workers(device) {}
 };
 struct fw_image {
   HImage h;
@@ -682,9 +683,50 @@ std::vector<std::vector<fw_trace_step>>& last_batch_traces() {
 }
 }  // namespace
 
+namespace {
+constexpr uint32_t kMaxWorkers = 16;
+std::vector<fw::hip::HipBackend*> take_workers(fw_ctx* ctx, uint32_t workers, size_t jobs) {
+  if (workers > kMaxWorkers) fw::invalid("workers", "at most " + std::to_string(kMaxWorkers) + " workers");
+  const size_t w = std::max<size_t>(1, std::min<size_t>(workers ? workers : 8, jobs));
+  return ctx->workers.take(w);
+}
+}  // namespace
+
+int fw_linear_batch(fw_ctx* ctx, const fw_plan* plan, uint32_t fits, const fw_image* const* fixed,
+                    const fw_image* const* moving, uint32_t workers, fw_result* results, const volatile int32_t* cancel) {
+  return guard(ctx, [&] {
+    need(plan, "plan");
+    if (!fixed || !moving) fw::invalid("register", "null image list");
+    if (fits == 0) fw::invalid("linear batch", "no fits");
+    fw::RegisterPlan p = fw::plan_from(plan);
+    p.run_syn = false;
+    std::vector<fw::hip::LinearFitJob> jobs(fits);
+    for (uint32_t i = 0; i < fits; ++i) {
+      if (!fixed[i] || !moving[i]) fw::invalid("register", "null image");
+      jobs[i].fixed_raw = &fixed[i]->h;
+      jobs[i].moving_raw = &moving[i]->h;
+    }
+    ctx->backend.ctx().sync();  // the images are complete before other streams read them
+    fw::hip::run_linear_fits(take_workers(ctx, workers, fits), p, jobs, cancel);
+    auto& traces = last_batch_traces();
+    traces.assign(fits, {});
+    for (uint32_t i = 0; i < fits; ++i) {
+      fw::RegisterOutput o;
+      o.linear = jobs[i].out;
+      o.initial = jobs[i].out.initial;
+      o.total = jobs[i].out.total;
+      o.seconds_preprocess = jobs[i].seconds_preprocess;
+      o.seconds_linear = jobs[i].seconds_linear;
+      o.seconds_total = o.seconds_preprocess + o.seconds_linear;
+      fw::result_to(o, results ? &results[i] : nullptr);
+      traces[i] = fw::last_trace();
+    }
+  });
+}
+
 int fw_register_batch(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, uint32_t subjects,
-                      const fw_image* const* moving, fw_field** forward, fw_field** inverse, fw_result* results,
-                      const volatile int32_t* cancel) {
+                      const fw_image* const* moving, uint32_t workers, fw_field** forward, fw_field** inverse,
+                      fw_result* results, const volatile int32_t* cancel) {
   return guard(ctx, [&] {
     need(plan, "plan");
     if (!fixed || !moving) fw::invalid("register", "null image list");
@@ -703,10 +745,10 @@ int fw_register_batch(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* f
         if (!im) fw::invalid("register", "null image");
         m[s].push_back(&im->h);
       }
-    fw::hip::HipLinearEngine engine(ctx->backend);
     std::vector<std::optional<HField>> fwd, inv;
-    const std::vector<fw::RegisterOutput> o = fw::hip::register_batch(
-        ctx->backend, engine, p, f, m, cancel, forward ? &fwd : nullptr, inverse ? &inv : nullptr);
+    const std::vector<fw::RegisterOutput> o =
+        fw::hip::register_batch(ctx->backend, take_workers(ctx, workers, subjects), p, f, m, cancel,
+                                forward ? &fwd : nullptr, inverse ? &inv : nullptr);
     auto& traces = last_batch_traces();
     traces.assign(subjects, {});
     for (size_t s = 0; s < subjects; ++s) {

@@ -1,8 +1,11 @@
 // The batched SyN loop: the iteration of syn.hpp (crates/fastwarp-reg/src/syn.rs:776-986) for
 // several subjects at once, each step one launch over every active subject.
-#include "batch_syn.hpp"
+#include "batch.hpp"
 
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <mutex>
 #include <thread>
 
 #include "geometry.hpp"
@@ -537,7 +540,73 @@ std::vector<SynFields> syn_fields_batch(HipBackend& b, const SynRecipe& recipe, 
   return out;
 }
 
-std::vector<RegisterOutput> register_batch(HipBackend& b, HipLinearEngine& engine, const RegisterPlan& plan,
+std::vector<HipBackend*> WorkerPool::take(size_t count) {
+  while (workers_.size() < count) workers_.push_back(std::make_unique<HipBackend>(device_));
+  std::vector<HipBackend*> out;
+  for (size_t w = 0; w < count; ++w) out.push_back(workers_[w].get());
+  return out;
+}
+
+void run_linear_fits(const std::vector<HipBackend*>& workers, const RegisterPlan& plan, std::vector<LinearFitJob>& jobs,
+                     const volatile int32_t* cancel) {
+  if (workers.empty()) invalid("linear fits", "no workers");
+  std::atomic<size_t> next{0};
+  std::atomic<bool> failed{false};
+  std::mutex error_mutex;
+  std::exception_ptr error;
+  auto work = [&](HipBackend& b) {
+    try {
+      check(hipSetDevice(b.ctx().device()), "hipSetDevice");
+      for (size_t i = next++; i < jobs.size() && !failed; i = next++) {
+        LinearFitJob& job = jobs[i];
+        auto t = clock_type::now();
+        auto fixed_pre = job.fixed_pre ? job.fixed_pre
+                                       : std::make_shared<const HImage>(
+                                             b.preprocess_image(*job.fixed_raw, plan.winsorize_lower, plan.winsorize_upper));
+        auto moving_pre = job.moving_pre ? job.moving_pre
+                                         : std::make_shared<const HImage>(
+                                               b.preprocess_image(*job.moving_raw, plan.winsorize_lower, plan.winsorize_upper));
+        job.seconds_preprocess = std::chrono::duration<double>(clock_type::now() - t).count();
+        t = clock_type::now();
+        SmoothCache<HipBackend> cache(b, std::vector<SmoothCache<HipBackend>::Pair>{{fixed_pre, moving_pre}});
+        if (job.shared) cache.share_fixed(job.shared);
+        AffineTransform start = plan.initial_affine;
+        if (plan.initial_kind == InitialKind::CentreOfMass) {
+          const Vec3 fc = job.fixed_centre ? *job.fixed_centre : b.centre_of_gravity(*job.fixed_raw);
+          const Vec3 mc = b.centre_of_gravity(*job.moving_raw);
+          start = AffineTransform{kIdentity3, vsub(mc, fc), fc};
+        } else if (plan.initial_kind == InitialKind::Identity) {
+          start = AffineTransform{};
+        }
+        if (plan.run_linear) {
+          HipLinearEngine engine(b);
+          job.out = run_linear_chain(engine, plan.rigid, plan.affine, start, cache, cancel);
+        } else {
+          job.out.initial = job.out.total = start;
+        }
+        b.ctx().sync();
+        job.seconds_linear = std::chrono::duration<double>(clock_type::now() - t).count();
+      }
+      b.ctx().sync();
+    } catch (...) {
+      failed = true;
+      std::lock_guard<std::mutex> lock(error_mutex);
+      if (!error) error = std::current_exception();
+      try {
+        b.ctx().sync();
+      } catch (...) {
+      }
+    }
+  };
+  const size_t nw = std::min(workers.size(), jobs.size());
+  std::vector<std::thread> threads;
+  for (size_t w = 1; w < nw; ++w) threads.emplace_back(work, std::ref(*workers[w]));
+  if (nw > 0) work(*workers[0]);
+  for (auto& th : threads) th.join();
+  if (error) std::rethrow_exception(error);
+}
+
+std::vector<RegisterOutput> register_batch(HipBackend& b, const std::vector<HipBackend*>& workers, const RegisterPlan& plan,
                                            const std::vector<const HImage*>& fixed,
                                            const std::vector<std::vector<const HImage*>>& moving,
                                            const volatile int32_t* cancel, std::vector<std::optional<HField>>* forward,
@@ -564,44 +633,60 @@ std::vector<RegisterOutput> register_batch(HipBackend& b, HipLinearEngine& engin
     inverse->resize(ns);
   }
   std::vector<RegisterOutput> out(ns);
-  // the template: preprocessed, its centre of mass and its smoothed levels taken once
+  // the template: preprocessed, its centre of mass taken and every level it is smoothed at built
+  // once (in the order a single run asks for them: rigid, affine, then SyN), on this context
   auto t = clock_type::now();
   std::vector<std::shared_ptr<const HImage>> tmpl;
   for (size_t c = 0; c < nc; ++c)
     tmpl.push_back(std::make_shared<const HImage>(b.preprocess_image(*fixed[c], plan.winsorize_lower, plan.winsorize_upper)));
-  const double tmpl_secs = secs(t);
-  const Vec3 fc = plan.initial_kind == InitialKind::CentreOfMass ? b.centre_of_gravity(*fixed[0]) : Vec3{0, 0, 0};
+  std::vector<std::vector<std::shared_ptr<const HImage>>> subj(ns);
+  for (size_t s = 0; s < ns; ++s)
+    for (size_t c = 0; c < nc; ++c)
+      subj[s].push_back(
+          std::make_shared<const HImage>(b.preprocess_image(*moving[s][c], plan.winsorize_lower, plan.winsorize_upper)));
+  const double pre_secs = secs(t);
   auto shared = std::make_shared<SmoothCache<HipBackend>::SharedFixed>();
-  std::vector<std::unique_ptr<SmoothCache<HipBackend>>> caches;
+  auto level = [&](size_t c, double sigma, SigmaUnits units) {
+    if (sigma <= 0.0 || shared->count({c, sigma})) return;
+    (*shared)[{c, sigma}] = std::make_shared<const HImage>(b.smooth_image(*tmpl[c], sigma, units));
+  };
+  if (plan.run_linear)
+    for (const LinearRecipe* r : {&plan.rigid, &plan.affine})
+      for (double sigma : r->smoothing_sigmas) level(0, sigma, r->sigma_units);
+  if (plan.run_syn)
+    for (double sigma : plan.syn.smoothing_sigmas)
+      for (size_t c = 0; c < nc; ++c) level(c, sigma, plan.syn.sigma_units);
+  std::optional<Vec3> fc;
+  if (plan.initial_kind == InitialKind::CentreOfMass) fc = b.centre_of_gravity(*fixed[0]);
+  b.ctx().sync();
+  // the subjects' linear chains, side by side on the workers
+  std::vector<LinearFitJob> jobs(ns);
+  for (size_t s = 0; s < ns; ++s) {
+    jobs[s].fixed_raw = fixed[0];
+    jobs[s].moving_raw = moving[s][0];
+    jobs[s].fixed_pre = tmpl[0];
+    jobs[s].moving_pre = subj[s][0];
+    jobs[s].shared = shared;
+    jobs[s].fixed_centre = fc;
+  }
+  run_linear_fits(workers, plan, jobs, cancel);
   for (size_t s = 0; s < ns; ++s) {
     RegisterOutput& o = out[s];
-    t = clock_type::now();
-    std::vector<SmoothCache<HipBackend>::Pair> resident;
-    for (size_t c = 0; c < nc; ++c)
-      resident.push_back(
-          {tmpl[c], std::make_shared<const HImage>(b.preprocess_image(*moving[s][c], plan.winsorize_lower, plan.winsorize_upper))});
-    o.seconds_preprocess = tmpl_secs + secs(t);
-    caches.push_back(std::make_unique<SmoothCache<HipBackend>>(b, std::move(resident)));
-    caches.back()->share_fixed(shared);
-    t = clock_type::now();
-    AffineTransform start = plan.initial_affine;
-    if (plan.initial_kind == InitialKind::CentreOfMass) {
-      const Vec3 mc = b.centre_of_gravity(*moving[s][0]);
-      start = AffineTransform{kIdentity3, vsub(mc, fc), fc};
-    } else if (plan.initial_kind == InitialKind::Identity) {
-      start = AffineTransform{};
-    }
-    if (plan.run_linear) {
-      o.linear = run_linear_chain(engine, plan.rigid, plan.affine, start, *caches.back(), cancel);
-      o.initial = o.linear.initial;
-      o.total = o.linear.total;
-    } else {
-      o.initial = o.total = start;
-    }
-    o.seconds_linear = secs(t);
+    o.seconds_preprocess = pre_secs;
+    o.linear = jobs[s].out;
+    o.initial = jobs[s].out.initial;
+    o.total = jobs[s].out.total;
+    o.seconds_linear = jobs[s].seconds_linear;
   }
   if (plan.run_syn) {
     b.release_idle_buffers();
+    std::vector<std::unique_ptr<SmoothCache<HipBackend>>> caches;
+    for (size_t s = 0; s < ns; ++s) {
+      std::vector<SmoothCache<HipBackend>::Pair> resident;
+      for (size_t c = 0; c < nc; ++c) resident.push_back({tmpl[c], subj[s][c]});
+      caches.push_back(std::make_unique<SmoothCache<HipBackend>>(b, std::move(resident)));
+      caches.back()->share_fixed(shared);
+    }
     for (size_t s0 = 0; s0 < ns; s0 += size_t(kBatchItems)) {
       const size_t s1 = std::min(ns, s0 + size_t(kBatchItems));
       std::vector<AffineTransform> init;

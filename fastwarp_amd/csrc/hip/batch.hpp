@@ -8,6 +8,7 @@
 // decisions (convergence, inversion stops).
 #pragma once
 
+#include <memory>
 #include <optional>
 #include <vector>
 
@@ -25,10 +26,43 @@ std::vector<SynFields> syn_fields_batch(HipBackend& b, const SynRecipe& recipe, 
                                         std::vector<std::optional<HField>>* forward,
                                         std::vector<std::optional<HField>>* inverse);
 
+// Worker contexts of one device (each its own HIP stream and buffer pool), kept between calls, for
+// linear fits that run side by side.
+class WorkerPool {
+ public:
+  explicit WorkerPool(int device) : device_(device) {}
+  // The first `count` workers, created as needed.
+  std::vector<HipBackend*> take(size_t count);
+
+ private:
+  int device_;
+  std::vector<std::unique_ptr<HipBackend>> workers_;
+};
+
+// One linear chain (centre of mass, Rigid, Affine on the first channel) of a fit: the raw images
+// (device, finished) and, where a template is shared, its preprocessed image, its complete map of
+// smoothed levels and its centre of mass.
+struct LinearFitJob {
+  const HImage* fixed_raw = nullptr;
+  const HImage* moving_raw = nullptr;
+  std::shared_ptr<const HImage> fixed_pre, moving_pre;
+  std::shared_ptr<SmoothCache<HipBackend>::SharedFixed> shared;
+  std::optional<Vec3> fixed_centre;
+  LinearChainResult out;
+  double seconds_preprocess = 0.0, seconds_linear = 0.0;
+};
+
+// The linear chains of `jobs` as register_resident runs them, spread over the workers: a worker
+// takes the next job when it finishes one, so the fits' kernels and host round trips overlap on the
+// device. Each job's result is the bits of its own run. The first error is raised once every
+// worker has stopped.
+void run_linear_fits(const std::vector<HipBackend*>& workers, const RegisterPlan& plan, std::vector<LinearFitJob>& jobs,
+                     const volatile int32_t* cancel);
+
 // register_resident for every subject against one template: the template is preprocessed, its
-// centre of mass taken and its pyramid smoothed once; each subject's linear chain runs as in
-// register_resident; the SyN stages run batched, kBatchItems subjects at a time.
-std::vector<RegisterOutput> register_batch(HipBackend& b, HipLinearEngine& engine, const RegisterPlan& plan,
+// centre of mass taken and its pyramid smoothed once; the subjects' linear chains run side by side
+// on `workers`; the SyN stages run batched, kBatchItems subjects at a time.
+std::vector<RegisterOutput> register_batch(HipBackend& b, const std::vector<HipBackend*>& workers, const RegisterPlan& plan,
                                            const std::vector<const HImage*>& fixed,
                                            const std::vector<std::vector<const HImage*>>& moving,
                                            const volatile int32_t* cancel, std::vector<std::optional<HField>>* forward,

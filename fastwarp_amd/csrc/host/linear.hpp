@@ -81,7 +81,8 @@ class SmoothCache {
 
   // Smoothed fixed images shared between the caches of one template's subjects (each cache holds
   // the same unsmoothed fixed images): a level of the template is smoothed once, by whichever
-  // cache asks for it first, and the others take that image.
+  // cache asks for it first, and the others take that image. Caches used from several threads
+  // need the map complete beforehand (they only read it then).
   using SharedFixed = std::map<std::pair<size_t, double>, Ptr>;
   void share_fixed(std::shared_ptr<SharedFixed> shared) { shared_ = std::move(shared); }
   size_t channels() const { return grids_.size(); }
@@ -99,9 +100,14 @@ class SmoothCache {
       const Pair raw = get(channel, 0.0, units);
       HostTimer t("smooth_images");
       if (shared_) {
-        Ptr& f = (*shared_)[{channel, key}];
-        if (!f) f = std::make_shared<const ImageH>(b_.smooth_image(*raw.fixed, sigma, units));
-        p.fixed = f;
+        // several threads may read a complete map; an entry is only added by a single user
+        const auto hit = shared_->find({channel, key});
+        if (hit != shared_->end() && hit->second) {
+          p.fixed = hit->second;
+        } else {
+          p.fixed = std::make_shared<const ImageH>(b_.smooth_image(*raw.fixed, sigma, units));
+          (*shared_)[{channel, key}] = p.fixed;
+        }
       } else {
         p.fixed = std::make_shared<const ImageH>(b_.smooth_image(*raw.fixed, sigma, units));
       }

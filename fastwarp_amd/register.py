@@ -91,20 +91,21 @@ def register_resident(backend: HipBackend, plan: Plan, fixed: list[DeviceImage],
 
 
 def register_batch(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], moving: list[list[DeviceImage]],
-                   fields: bool = True, cancel: C.c_int32 | None = None) -> list[ResidentRegistration]:
+                   fields: bool = True, workers: int = 0, cancel: C.c_int32 | None = None) -> list[ResidentRegistration]:
     """Several subjects (moving[s] holds subject s's channels) registered to one template (the
-    fixed images): the template's preprocessing, centre of mass and smoothed levels are shared, and
-    the SyN stages run batched on the device (fw_register_batch). Each subject's result is the
-    bits of its own register_resident run."""
+    fixed images): the template's preprocessing, centre of mass and smoothed levels are shared, the
+    linear chains run side by side on `workers` worker streams (0: 8), and the SyN stages run
+    batched on the device (fw_register_batch). Each subject's result is the bits of its own
+    register_resident run."""
     h = _sig()
     ns = len(moving)
     fp = (C.c_void_p * len(fixed))(*[im.handle for im in fixed])
-    mp = (C.c_void_p * (ns * len(fixed)))(*[im.handle for subj in moving for im in subj])
+    mp = (C.c_void_p * max(ns * len(fixed), 1))(*[im.handle for subj in moving for im in subj])
     want = fields and plan.run_syn
-    fwd, inv = (C.c_void_p * ns)(), (C.c_void_p * ns)()
-    res = (Result * ns)()
-    st = h.fw_register_batch(backend.ctx, C.byref(plan), fp, ns, mp, fwd if want else None, inv if want else None, res,
-                             C.byref(cancel) if cancel is not None else None)
+    fwd, inv = (C.c_void_p * max(ns, 1))(), (C.c_void_p * max(ns, 1))()
+    res = (Result * max(ns, 1))()
+    st = h.fw_register_batch(backend.ctx, C.byref(plan), fp, ns, mp, workers, fwd if want else None, inv if want else None,
+                             res, C.byref(cancel) if cancel is not None else None)
     if st != FW_OK:
         raise RegError(st, h.fw_last_error(backend.ctx).decode())
     wrap = lambda p: DeviceField(backend, C.c_void_p(p)) if p else None
@@ -112,8 +113,26 @@ def register_batch(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], mo
             for s in range(ns)]
 
 
+def linear_batch(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], moving: list[DeviceImage], workers: int = 0,
+                 cancel: C.c_int32 | None = None) -> list[Result]:
+    """Independent linear fits (centre of mass, Rigid, Affine) of fixed[i] / moving[i] (raw images
+    on the device, each pair its own grids), side by side on `workers` worker streams (0: 8)
+    (fw_linear_batch). Each fit's result is the bits of its own register_resident run without SyN."""
+    h = _sig()
+    n = len(fixed)
+    if len(moving) != n:
+        raise ValueError("one moving image per fixed image")
+    fp = (C.c_void_p * max(n, 1))(*[im.handle for im in fixed])
+    mp = (C.c_void_p * max(n, 1))(*[im.handle for im in moving])
+    res = (Result * max(n, 1))()
+    st = h.fw_linear_batch(backend.ctx, C.byref(plan), n, fp, mp, workers, res, C.byref(cancel) if cancel is not None else None)
+    if st != FW_OK:
+        raise RegError(st, h.fw_last_error(backend.ctx).decode())
+    return [res[i] for i in range(n)]
+
+
 def last_batch_trace(subject: int) -> list[TraceStep]:
-    """last_trace for one subject of this thread's last register_batch call."""
+    """last_trace for one subject (or fit) of this thread's last register_batch or linear_batch call."""
     return read_trace(lambda out, cap: lib().fw_last_batch_trace(subject, out, cap))
 
 

@@ -366,20 +366,30 @@ int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, con
 int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
                          fw_field** forward, fw_field** inverse, fw_result* result, const volatile int32_t* cancel);
 
+/* Many independent linear fits (centre of mass, Rigid, Affine Mattes MI on the fixed[i] /
+ * moving[i] pair, each its own grids; fw_register_resident with run_syn 0): the reference runs
+ * them one instance each (crates/fastwarp-cli/src/engine.rs:1016-1068). Here `workers` (1..16; 0
+ * for 8) worker contexts of the device, each its own HIP stream and pool, take the fits in turn,
+ * so their kernels and host round trips overlap on the device. results (may be NULL) holds one
+ * entry per fit, each the bits of the fit's own run; an error of one fit fails the call. */
+int fw_linear_batch(fw_ctx* ctx, const fw_plan* plan, uint32_t fits, const fw_image* const* fixed,
+                    const fw_image* const* moving, uint32_t workers, fw_result* results, const volatile int32_t* cancel);
+
 /* Several subjects registered to one template (the fixed images, `channels` of them, shared by
  * every subject): the reference runs a cohort as one registration per subject
  * (crates/fastwarp-cli/src/engine.rs:1016-1068, python/fastwarp/_api.py:1104-1137). Here the
- * template is preprocessed, its centre of mass taken and its smoothed levels built once; each
- * subject's linear chain runs as in fw_register_resident; then the subjects' SyN stages run in
- * lockstep, 8 at a time, each step of an iteration one launch over all of them (CC channels
- * only). moving holds subjects x channels handles (subject-major); forward / inverse (may be NULL)
- * and results (may be NULL) hold one entry per subject. Every subject's result is the bits of its
- * own fw_register_resident run; an error of one subject fails the call. */
+ * template is preprocessed, its centre of mass taken and its smoothed levels built once; the
+ * subjects' linear chains run side by side on `workers` worker contexts (as fw_linear_batch); then
+ * the subjects' SyN stages run in lockstep, 8 at a time, each step of an iteration one launch over
+ * all of them (CC channels only). moving holds subjects x channels handles (subject-major);
+ * forward / inverse (may be NULL) and results (may be NULL) hold one entry per subject. Every
+ * subject's result is the bits of its own fw_register_resident run; an error of one subject fails
+ * the call. */
 int fw_register_batch(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, uint32_t subjects,
-                      const fw_image* const* moving, fw_field** forward, fw_field** inverse, fw_result* results,
-                      const volatile int32_t* cancel);
-/* fw_last_trace for one subject of the calling thread's last fw_register_batch (-1: no such
- * subject). */
+                      const fw_image* const* moving, uint32_t workers, fw_field** forward, fw_field** inverse,
+                      fw_result* results, const volatile int32_t* cancel);
+/* fw_last_trace for one subject (or fit) of the calling thread's last fw_register_batch or
+ * fw_linear_batch (-1: no such subject). */
 int32_t fw_last_batch_trace(uint32_t subject, fw_trace_step* out, int32_t capacity);
 
 /* ---- Output files (crates/fastwarp-reg/src/transform/itk_mat.rs) ----

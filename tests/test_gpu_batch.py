@@ -1,9 +1,11 @@
 """Batched registration of several subjects against one template (fw_register_batch): the SyN
 iterations of all subjects run in lockstep, each step one launch over every subject, and the
-template's smoothed levels are built once. Each subject's result must be the bits of its own
-fw_register_resident run (crates/fastwarp-reg/src/syn.rs:776-986 per subject): the same affine,
-the same SyN iteration counts, energies and inversion step counts per iteration, and the same
-fields. Those single runs are themselves held to the CPU engine by tests/test_register.py.
+template's smoothed levels are built once; and many independent linear fits of uneven grids side
+by side on worker streams (fw_linear_batch). Each subject's or fit's result must be the bits of its
+own fw_register_resident run (crates/fastwarp-reg/src/syn.rs:776-986 and linear.rs:1148-1212 per
+subject): the same affine, the same CG values and SyN energies, iteration and inversion step counts
+per iteration, and the same fields. Those single runs are themselves held to the CPU engine by
+tests/test_register.py.
 """
 from __future__ import annotations
 
@@ -11,7 +13,7 @@ import numpy as np
 import pytest
 
 from fastwarp_amd.phantom import make_pair
-from fastwarp_amd.register import (last_batch_trace, last_trace, plan_2mm, register_batch,
+from fastwarp_amd.register import (last_batch_trace, last_trace, linear_batch, plan_2mm, register_batch,
                                    register_resident)
 
 pytestmark = pytest.mark.gpu
@@ -42,7 +44,7 @@ def test_batched_subjects_give_the_bits_of_their_single_runs(backend, channels, 
     up = lambda vols: [backend.upload_image(grid, v) for v in vols]
     tmpl = up(fixed)
     subj = [up(m) for m in moving]
-    batch = register_batch(backend, plan, tmpl, subj)
+    batch = register_batch(backend, plan, tmpl, subj, workers=2)
     traces = [steps(last_batch_trace(s)) for s in range(subjects)]
     for s in range(subjects):
         alone = register_resident(backend, plan, tmpl, subj[s])
@@ -64,3 +66,26 @@ def test_batch_without_subjects_is_an_error(backend):
     grid, fixed, _ = cohort(1, 0, shape=(24, 24, 24), spacing=4.0)
     with pytest.raises(RegError, match="no subjects"):
         register_batch(backend, plan_2mm(1), [backend.upload_image(grid, fixed[0])], [])
+
+
+def test_linear_fits_side_by_side_give_the_bits_of_their_single_runs(backend):
+    """Six rigid+affine fits of uneven grids and spacings (the batched linear workload's generator,
+    smaller), on three worker streams, against each fit run alone."""
+    shapes = [((40, 44, 36), 4.0), ((64, 64, 64), 2.0), ((56, 72, 48), 3.0), ((48, 40, 52), 2.5), ((72, 80, 64), 2.0),
+              ((44, 48, 40), 3.5)]
+    plan = plan_2mm(1)
+    plan.set_linear([30, 15, 8], [4, 2, 1], [2.0, 1.0, 0.0])
+    plan.run_syn = 0
+    fixed, moving = [], []
+    for i, (shape, spacing) in enumerate(shapes):
+        pair = make_pair(shape, spacing, seed=40 + i, affine_only=True)
+        fixed.append(backend.upload_image(pair.grid, pair.fixed[0]))
+        moving.append(backend.upload_image(pair.grid, pair.moving[0]))
+    results = linear_batch(backend, plan, fixed, moving, workers=3)
+    traces = [steps(last_batch_trace(i)) for i in range(len(shapes))]
+    for i in range(len(shapes)):
+        alone = register_resident(backend, plan, [fixed[i]], [moving[i]], fields=False)
+        label = f"fit {i}, grid {shapes[i]}"
+        assert np.array_equal(affine_array(results[i].affine), affine_array(alone.affine)), label
+        assert np.array_equal(np.array(results[i].linear_iterations), np.array(alone.result.linear_iterations)), label
+        assert traces[i] == steps(last_trace()), label
