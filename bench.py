@@ -138,7 +138,8 @@ def served(slots: int, requests: int, load: float) -> dict:
 
 def batched(backend, subjects: int) -> dict:
     """The batched multichannel workload: `subjects` two-channel subjects (T1-like and T2-like
-    phantom channels, 1 mm, 182 x 218 x 182) registered to one shared template resident in HBM,
+    phantom channels, 1 mm, 182 x 218 x 182; template seed 4000, subjects 4001 on) registered to
+    one shared template resident in HBM,
     full normalisations (centre of mass, Rigid+Affine MI, SyN CC radius 4 on both channels with
     equal weights, 100/70/50/20) through fw_register_batch, against the same subjects registered
     one at a time (fw_register_resident)."""
@@ -146,13 +147,13 @@ def batched(backend, subjects: int) -> dict:
     from fastwarp_amd.register import cohort_plan, register_batch, register_resident
 
     t = time.perf_counter()
-    tmpl_pair = make_pair((182, 218, 182), 1.0, seed=100, disp_max_mm=6.0, channels=2)
+    tmpl_pair = make_pair((182, 218, 182), 1.0, seed=4000, disp_max_mm=6.0, channels=2)
     grid = tmpl_pair.grid
     tmpl = [backend.upload_image(grid, v) for v in tmpl_pair.fixed]
     del tmpl_pair
     subj = []
     for s in range(subjects):
-        p = make_pair((182, 218, 182), 1.0, seed=101 + s, disp_max_mm=6.0, channels=2)
+        p = make_pair((182, 218, 182), 1.0, seed=4001 + s, disp_max_mm=6.0, channels=2)
         subj.append([backend.upload_image(grid, v) for v in p.moving])
     log(f"batched: template and {subjects} two-channel subjects ready in {time.perf_counter() - t:.1f} s")
     plan = cohort_plan(2)
@@ -190,13 +191,14 @@ def batched(backend, subjects: int) -> dict:
 
 
 def fit_sizes(fits: int, seed: int = 3):
-    """The batched linear workload's pairs: grids drawn uniformly from 64^3 to 192 x 224 x 192,
-    isotropic spacing drawn from 1 to 2 mm, one phantom seed per pair, random affine only."""
+    """The batched linear workload's pairs: grids drawn uniformly per axis from 64^3 to
+    192 x 224 x 192, isotropic spacing drawn from {1, 1.5, 2} mm, phantom seed 1000 + i for pair i,
+    random affine only."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(fits):
         shape = (int(rng.integers(64, 193)), int(rng.integers(64, 225)), int(rng.integers(64, 193)))
-        out.append((shape, round(float(rng.uniform(1.0, 2.0)), 3), 1000 + i))
+        out.append((shape, float(rng.choice([1.0, 1.5, 2.0])), 1000 + i))
     return out
 
 
