@@ -191,6 +191,7 @@ class Plan(C.Structure):
         ("run_syn", C.c_int32),
         ("initial_kind", C.c_int32),
         ("initial_affine", Affine),
+        ("syn_channel_metrics", C.c_int32 * 8),
     ]
 
     def set_linear(self, iterations, shrink, sigmas):

@@ -110,6 +110,8 @@ _SIGNATURES = [
                                     C.POINTER(_P), C.POINTER(_P), C.POINTER(Result), C.POINTER(C.c_int32)]),
     ("fw_linear_batch", C.c_int, [_P, C.POINTER(Plan), C.c_uint32, C.POINTER(_P), C.POINTER(_P), C.c_uint32,
                                   C.POINTER(Result), C.POINTER(C.c_int32)]),
+    ("fw_syn_stage", C.c_int, [_P, C.POINTER(Plan), C.POINTER(_P), C.POINTER(_P), C.POINTER(Affine), _P, _P, _P, _P, _PP,
+                               _PP, C.POINTER(Result), C.POINTER(C.c_int32)]),
     ("fw_last_batch_trace", C.c_int32, [C.c_uint32, C.POINTER(TraceStep), C.c_int32]),
     ("fw_resample_isotropic", C.c_int, [_P, _P, C.c_double, _PP]),
     ("fw_write_affine_mat", C.c_int, [C.c_char_p, C.POINTER(Affine)]),

@@ -675,6 +675,33 @@ int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const
   });
 }
 
+int fw_syn_stage(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
+                 const fw_affine* initial, const fw_image* fixed_mask, const fw_image* moving_mask, const fw_field* prior,
+                 const fw_field* prior_inverse, fw_field** forward, fw_field** inverse, fw_result* result,
+                 const volatile int32_t* cancel) {
+  return guard(ctx, [&] {
+    need(plan, "plan");
+    if (!fixed || !moving) fw::invalid("syn stage", "null image list");
+    std::vector<const HImage*> f, m;
+    for (int32_t c = 0; c < plan->channels; ++c) {
+      if (!fixed[c] || !moving[c]) fw::invalid("syn stage", "null image");
+      f.push_back(&fixed[c]->h);
+      m.push_back(&moving[c]->h);
+    }
+    fw::SynStageInputs<HipBackend> extra;
+    extra.fixed_mask = fixed_mask ? &fixed_mask->h : nullptr;
+    extra.moving_mask = moving_mask ? &moving_mask->h : nullptr;
+    extra.prior = prior ? &prior->h : nullptr;
+    extra.prior_inverse = prior_inverse ? &prior_inverse->h : nullptr;
+    std::optional<HField> fwd, inv;
+    const fw::RegisterOutput o = fw::run_syn_stage(ctx->backend, plan, f, m, initial, extra, cancel, fw::FieldOutputs{},
+                                                   forward ? &fwd : nullptr, inverse ? &inv : nullptr);
+    fw::result_to(o, result);
+    if (forward) *forward = fwd ? new fw_field{std::move(*fwd)} : nullptr;
+    if (inverse) *inverse = inv ? new fw_field{std::move(*inv)} : nullptr;
+  });
+}
+
 namespace {
 // The per-subject records of the calling thread's last fw_register_batch.
 std::vector<std::vector<fw_trace_step>>& last_batch_traces() {

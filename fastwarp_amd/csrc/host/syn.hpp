@@ -82,6 +82,19 @@ struct FieldOutputs {
   double* inverse = nullptr;
 };
 
+// What a SyN stage may start from beyond its linear transform (syn.rs:160-180 `SynStart::prior`,
+// syn.rs:473-580 `StageMasks`): binary masks of the fixed and of the moving images (each on its own
+// grid, non-zero inside; null for none) and the fields of the SyN stages before it (forward and
+// inverse, on the grid of this stage's last level), which enter every moving-side warp and are
+// collapsed with this stage's fields at the end.
+template <typename B>
+struct SynStageInputs {
+  const typename B::ImageH* fixed_mask = nullptr;
+  const typename B::ImageH* moving_mask = nullptr;
+  const typename B::FieldH* prior = nullptr;
+  const typename B::FieldH* prior_inverse = nullptr;
+};
+
 namespace detail {
 
 // Backends that run both sides of a step in single launches (warp_pair, compose_pair,
@@ -150,7 +163,7 @@ template <typename B>
 SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initial, SmoothCache<B>& cache,
                      const volatile int32_t* cancel, FieldOutputs outputs = {},
                      std::optional<typename B::FieldH>* forward_device = nullptr,
-                     std::optional<typename B::FieldH>* inverse_device = nullptr) {
+                     std::optional<typename B::FieldH>* inverse_device = nullptr, const SynStageInputs<B>& extra = {}) {
   using FieldH = typename B::FieldH;
   using ImageH = typename B::ImageH;
   recipe.validate();
@@ -166,6 +179,21 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
   for (double w : recipe.channel_weights) normalised.push_back(w / wsum);
   const size_t nc = cache.channels();
   const InvertParams inv_params;  // InvertParams::SYN
+  const typename B::FieldH* prior = extra.prior;
+  if (bool(extra.prior) != bool(extra.prior_inverse))
+    invalid("syn", "the earlier SyN stages need both their forward and their inverse field");
+  if (prior) {
+    // syn.rs:804-817
+    const Grid last = level_virtual_grid(
+        fixed_grid, shrink_factors_per_dimension(recipe.shrink_factors[recipe.levels() - 1], fixed_grid.spacing));
+    if (!(b.field_grid(*extra.prior) == last) || !(b.field_grid(*extra.prior_inverse) == last))
+      invalid("syn",
+              "the field of the earlier SyN stages is not on the grid of this stage's last level: every SyN stage must "
+              "end with the same shrink factor");
+  }
+  const bool masks = extra.fixed_mask || extra.moving_mask;
+  bool uses_cc = false, uses_mattes = false;
+  for (size_t c = 0; c < nc; ++c) (recipe.metric(c) == SynMetric::Cc ? uses_cc : uses_mattes) = true;
   std::optional<FieldH> f2m, f2m_inv, m2m, m2m_inv;
   SynFields out;
   for (size_t level = 0; level < recipe.levels(); ++level) {
@@ -201,22 +229,45 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
       // The iteration is issued without waiting on the device: the metric values, the update
       // maxima and the inversion statistics are read back once, at the settle below, where the
       // reference's finiteness checks run on them (same errors, after the device work).
+      // the masks of this iteration on the level grid (syn.rs:505-580 `resolve`): for CC the
+      // voxels inside both, for Mattes each mask on its own
+      std::optional<ImageH> valid, fixed_in, moving_in;
+      if (masks) {
+        if (uses_cc) {
+          std::optional<ImageH> f;
+          if (extra.fixed_mask) f = b.warp_mask_through_field(*extra.fixed_mask, *f2m_inv, nullptr, nullptr, nullptr);
+          if (extra.moving_mask)
+            valid = b.warp_mask_through_field(*extra.moving_mask, *m2m_inv, prior, &initial, f ? &*f : nullptr);
+          else
+            valid = std::move(f);
+        }
+        if (uses_mattes) {
+          if (extra.fixed_mask) fixed_in = b.warp_mask_through_field(*extra.fixed_mask, *f2m_inv, nullptr, nullptr, nullptr);
+          if (extra.moving_mask)
+            moving_in = b.warp_mask_through_field(*extra.moving_mask, *m2m_inv, prior, &initial, nullptr);
+        }
+      }
       // 1. resample every channel into the middle (syn.rs:895-911)
       detail::Side<B> fs, ms;
       const bool single = nc == 1;
       std::vector<Pending<std::array<double, 2>>> later(nc);
       for (size_t c = 0; c < nc; ++c) {
-        auto mids = detail::warp_both(b, *smooth[c].fixed, *f2m_inv, *smooth[c].moving, *m2m_inv, initial);
+        std::pair<ImageH, ImageH> mids =
+            prior ? std::pair<ImageH, ImageH>{b.warp_through_field(*smooth[c].fixed, *f2m_inv, nullptr, nullptr),
+                                              b.warp_through_field(*smooth[c].moving, *m2m_inv, prior, &initial)}
+                  : detail::warp_both(b, *smooth[c].fixed, *f2m_inv, *smooth[c].moving, *m2m_inv, initial);
         ImageH fmid = std::move(mids.first), mmid = std::move(mids.second);
         // 2. the metric both ways (syn.rs:337-420 `side_updates`); several channels need the
         // derivative norms for their weights at once, a single one needs its values only later
         MetricPair<FieldH> pair;
-        if (recipe.metric(c) == SynMetric::Cc && single) {
+        if (recipe.metric(c) == SynMetric::Cc && single && !valid) {
           pair = b.cc_pair_deferred(fmid, mmid, recipe.cc_radius);
           later[c] = std::move(pair.later);
         } else {
-          pair = recipe.metric(c) == SynMetric::Cc ? b.cc_pair(fmid, mmid, nullptr, recipe.cc_radius, !single)
-                                                   : b.mi_pair(fmid, mmid, nullptr, nullptr, recipe.mi_bins, !single);
+          pair = recipe.metric(c) == SynMetric::Cc
+                     ? b.cc_pair(fmid, mmid, valid ? &*valid : nullptr, recipe.cc_radius, !single)
+                     : b.mi_pair(fmid, mmid, fixed_in ? &*fixed_in : nullptr, moving_in ? &*moving_in : nullptr,
+                                 recipe.mi_bins, !single);
           later[c] = Pending<std::array<double, 2>>(std::array<double, 2>{pair.values[0], pair.values[1]});
         }
         fs.add(b, normalised[c], 0.0, std::move(pair.fields[0]), single ? nullptr : &pair.norms[0]);
@@ -296,14 +347,18 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
     trace.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     out.levels.push_back(std::move(trace));
   }
-  // syn.rs:969-980: SyN = Minv o F forward, Finv o M inverse
+  // syn.rs:969-980: SyN = Minv o F forward, Finv o M inverse; after earlier stages, collapsed with
+  // theirs (syn.rs:746-774): forward earlier(x + later(x)) + later(x), inverse
+  // later_inverse(x + earlier_inverse(x)) + earlier_inverse(x)
   if (outputs.forward || forward_device) {
     FieldH f = b.compose(*m2m_inv, *f2m);
+    if (prior) f = b.compose(*prior, f);
     if (outputs.forward) b.download_field_to(f, outputs.forward);
     if (forward_device) *forward_device = std::move(f);
   }
   if (outputs.inverse || inverse_device) {
     FieldH f = b.compose(*f2m_inv, *m2m);
+    if (prior) f = b.compose(f, *extra.prior_inverse);
     if (outputs.inverse) b.download_field_to(f, outputs.inverse);
     if (inverse_device) *inverse_device = std::move(f);
   }

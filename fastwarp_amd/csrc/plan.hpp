@@ -68,6 +68,16 @@ inline RegisterPlan plan_from(const fw_plan* p) {
   s.total_field_variance = p->total_variance;
   s.cc_radius = size_t(p->cc_radius);
   s.channel_weights.assign(p->channel_weights, p->channel_weights + p->channels);
+  s.channel_metrics.clear();
+  for (int32_t c = 0; c < p->channels; ++c) {
+    const int32_t m = p->syn_channel_metrics[c];
+    if (m != 0 && m != 1) invalid("stage plan", "a SyN channel metric is 0 (CC) or 1 (Mattes)");
+    if (m == 1) s.channel_metrics.assign(size_t(p->channels), SynMetric::Cc);
+  }
+  if (!s.channel_metrics.empty())
+    for (int32_t c = 0; c < p->channels; ++c)
+      s.channel_metrics[c] = p->syn_channel_metrics[c] == 1 ? SynMetric::Mattes : SynMetric::Cc;
+  s.mi_bins = size_t(p->mi_bins);
   s.convergence_threshold = p->syn_convergence_threshold;
   s.convergence_window = size_t(p->syn_convergence_window);
   r.winsorize_lower = float(p->winsorize_lower);
@@ -99,6 +109,46 @@ inline RegisterPlan plan_from(const fw_plan* p) {
     default: invalid("stage plan", "initial_kind must be 0 (centre of mass), 1 (identity) or 2 (given affine)");
   }
   return r;
+}
+
+inline AffineTransform affine_of(const fw_affine* a) {
+  AffineTransform t;
+  if (!a) return t;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) t.matrix[i][j] = a->matrix[3 * i + j];
+    t.translation[i] = a->translation[i];
+    t.center[i] = a->center[i];
+  }
+  return t;
+}
+
+// One SyN stage (fw_syn_stage, and the oracle's counterpart) on the backend's images, taken as
+// they are: the plan's SyN recipe from `initial`, with the stage's masks and earlier fields.
+template <typename B>
+RegisterOutput run_syn_stage(B& b, const fw_plan* plan, const std::vector<const typename B::ImageH*>& fixed,
+                             const std::vector<const typename B::ImageH*>& moving, const fw_affine* initial,
+                             const SynStageInputs<B>& extra, const volatile int32_t* cancel, FieldOutputs outputs = {},
+                             std::optional<typename B::FieldH>* forward_device = nullptr,
+                             std::optional<typename B::FieldH>* inverse_device = nullptr) {
+  using ImageH = typename B::ImageH;
+  if (!plan) invalid("syn stage", "null plan");
+  const RegisterPlan p = plan_from(plan);
+  if (fixed.size() != size_t(plan->channels) || moving.size() != size_t(plan->channels))
+    invalid("syn", "the number of channels differs from the number of channel weights");
+  std::vector<typename SmoothCache<B>::Pair> resident;
+  for (size_t c = 0; c < fixed.size(); ++c) {
+    if (!fixed[c] || !moving[c]) invalid("syn stage", "missing image");
+    // the caller's images, not owned
+    resident.push_back({std::shared_ptr<const ImageH>(fixed[c], [](const ImageH*) {}),
+                        std::shared_ptr<const ImageH>(moving[c], [](const ImageH*) {})});
+  }
+  SmoothCache<B> cache(b, std::move(resident));
+  RegisterOutput o;
+  o.initial = o.total = affine_of(initial);
+  const auto t = std::chrono::steady_clock::now();
+  o.syn = syn_fields(b, p.syn, o.total, cache, cancel, outputs, forward_device, inverse_device, extra);
+  o.seconds_syn = o.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+  return o;
 }
 
 // The per-iteration records of the calling thread's last registration (fw_last_trace).

@@ -113,6 +113,28 @@ def register_batch(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], mo
             for s in range(ns)]
 
 
+def syn_stage(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], moving: list[DeviceImage],
+              initial: Affine | None = None, fixed_mask: DeviceImage | None = None, moving_mask: DeviceImage | None = None,
+              prior: DeviceField | None = None, prior_inverse: DeviceField | None = None,
+              cancel: C.c_int32 | None = None) -> ResidentRegistration:
+    """One SyN stage (the plan's SyN recipe) on images already on the device, taken as they are,
+    from `initial`, with optional masks and the fields of earlier SyN stages, which are collapsed
+    with this stage's (fw_syn_stage). The fields stay on the device."""
+    h = _sig()
+    fp = (C.c_void_p * len(fixed))(*[im.handle for im in fixed])
+    mp = (C.c_void_p * len(moving))(*[im.handle for im in moving])
+    fwd, inv = C.c_void_p(), C.c_void_p()
+    res = Result()
+    handle = lambda x: x.handle if x is not None else None
+    st = h.fw_syn_stage(backend.ctx, C.byref(plan), fp, mp, C.byref(initial) if initial is not None else None,
+                        handle(fixed_mask), handle(moving_mask), handle(prior), handle(prior_inverse), C.byref(fwd),
+                        C.byref(inv), C.byref(res), C.byref(cancel) if cancel is not None else None)
+    if st != FW_OK:
+        raise RegError(st, h.fw_last_error(backend.ctx).decode())
+    wrap = lambda p: DeviceField(backend, p) if p.value else None
+    return ResidentRegistration(res.affine, wrap(fwd), wrap(inv), res)
+
+
 def linear_batch(backend: HipBackend, plan: Plan, fixed: list[DeviceImage], moving: list[DeviceImage], workers: int = 0,
                  cancel: C.c_int32 | None = None) -> list[Result]:
     """Independent linear fits (centre of mass, Rigid, Affine) of fixed[i] / moving[i] (raw images

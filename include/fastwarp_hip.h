@@ -267,6 +267,9 @@ typedef struct {
    * moving points; finite and invertible, the first linear stage is centred at its centre). */
   int32_t initial_kind;
   fw_affine initial_affine;
+  /* The SyN metric of each channel (recipe.rs SynRecipe::channel_metrics): 0 CC (radius
+   * cc_radius), 1 Mattes MI (mi_bins bins, dense); all 0 by default. */
+  int32_t syn_channel_metrics[8];
 } fw_plan;
 
 /* Summary of one registration: the collapsed affine, per-phase seconds and per-level counts. */
@@ -365,6 +368,21 @@ int fw_register(fw_ctx* ctx, const fw_plan* plan, const fw_grid* fixed_grid, con
  * resident in HBM (and the throughput measurement of the device path). */
 int fw_register_resident(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
                          fw_field** forward, fw_field** inverse, fw_result* result, const volatile int32_t* cancel);
+
+/* One SyN stage on images already on the device (crates/fastwarp-reg/src/syn.rs run_syn_from /
+ * syn_fields with SynStart and the recipe's masks): the plan's SyN recipe (syn_* fields, channels,
+ * channel weights and metrics) from `initial` (fixed points to moving points), the images taken
+ * as they are (no preprocessing). fixed_mask / moving_mask (may be NULL; each on its own grid,
+ * non-zero inside) limit the metrics as ITK resamples them onto the level grid every iteration
+ * (syn.rs:505-580); prior / prior_inverse (both or neither) are the fields of the SyN stages
+ * before this one, on the grid of this stage's last level: they enter every moving-side warp and
+ * are collapsed with this stage's fields (syn.rs:746-774), so *forward / *inverse (new handles,
+ * may be NULL) are the whole chain's. result (may be NULL) gets the per-level counts and
+ * energies; fw_last_trace the iterations. */
+int fw_syn_stage(fw_ctx* ctx, const fw_plan* plan, const fw_image* const* fixed, const fw_image* const* moving,
+                 const fw_affine* initial, const fw_image* fixed_mask, const fw_image* moving_mask, const fw_field* prior,
+                 const fw_field* prior_inverse, fw_field** forward, fw_field** inverse, fw_result* result,
+                 const volatile int32_t* cancel);
 
 /* Many independent linear fits (centre of mass, Rigid, Affine Mattes MI on the fixed[i] /
  * moving[i] pair, each its own grids; fw_register_resident with run_syn 0): the reference runs

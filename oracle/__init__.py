@@ -243,6 +243,39 @@ def register(plan, fixed_grid: Grid, fixed, moving_grid: Grid, moving, fields: b
     return res, fwd, inv
 
 
+def syn_stage(plan, fixed_grid: Grid, fixed, moving_grid: Grid, moving, initial: Affine | None = None,
+              fixed_mask=None, moving_mask=None, prior=None):
+    """One SyN stage on the f64 CPU engine (fw_syn_stage's counterpart): masks as (grid, data)
+    pairs, the earlier stages' fields as (grid, forward (n, 3), inverse (n, 3)). Returns (Result,
+    forward, inverse) on the fixed grid."""
+    from fastwarp_amd._abi import Result
+    from fastwarp_amd.register import channel_array
+
+    h = lib()
+    fa, fp = channel_array(fixed)
+    ma, mp = channel_array(moving)
+    n = fixed_grid.n
+    fwd, inv = np.empty((n, 3)), np.empty((n, 3))
+    res = Result()
+    keep = []
+
+    def arr(x):
+        a = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        keep.append(a)
+        return dptr(a)
+
+    fm_grid, fm = (C.byref(fixed_mask[0]), arr(fixed_mask[1])) if fixed_mask is not None else (None, None)
+    mm_grid, mm = (C.byref(moving_mask[0]), arr(moving_mask[1])) if moving_mask is not None else (None, None)
+    if prior is not None:
+        p_grid, pw, pi = C.byref(prior[0]), arr(prior[1]), arr(prior[2])
+    else:
+        p_grid, pw, pi = None, None, None
+    _check(h.fwo_syn_stage(C.byref(plan), C.byref(fixed_grid), fp, C.byref(moving_grid), mp,
+                           C.byref(initial) if initial is not None else None, fm_grid, fm, mm_grid, mm, p_grid, pw, pi,
+                           dptr(fwd), dptr(inv), C.byref(res)))
+    return res, fwd, inv
+
+
 def last_trace():
     """Per-iteration records of this thread's last `register` call (fastwarp_amd._abi.TraceStep)."""
     from fastwarp_amd._abi import read_trace

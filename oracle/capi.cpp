@@ -322,6 +322,52 @@ extern "C" int fwo_register(const fw_plan* plan, const fw_grid* fixed_grid, cons
   });
 }
 
+// fw_syn_stage on the CPU engine: f64 host images (each channel's fixed on fixed_grid, moving on
+// moving_grid), masks on their own grids (data may be null), earlier fields (3n interleaved on
+// prior_grid, may be null), forward / inverse 3n on the fixed grid (may be null).
+extern "C" int fwo_syn_stage(const fw_plan* plan, const fw_grid* fixed_grid, const double* const* fixed,
+                             const fw_grid* moving_grid, const double* const* moving, const fw_affine* initial,
+                             const fw_grid* fixed_mask_grid, const double* fixed_mask, const fw_grid* moving_mask_grid,
+                             const double* moving_mask, const fw_grid* prior_grid, const double* prior,
+                             const double* prior_inverse, double* forward, double* inverse, fw_result* result) {
+  return guard([&] {
+    if (!plan) invalid("syn stage", "null plan");
+    const Grid fg = grid_of(fixed_grid), mg = grid_of(moving_grid);
+    std::vector<Image> f, m;
+    for (int32_t c = 0; c < plan->channels; ++c) {
+      f.emplace_back(fg, std::vector<double>(fixed[c], fixed[c] + fg.num_voxels()));
+      m.emplace_back(mg, std::vector<double>(moving[c], moving[c] + mg.num_voxels()));
+    }
+    std::vector<const Image*> fp, mp;
+    for (size_t c = 0; c < f.size(); ++c) {
+      fp.push_back(&f[c]);
+      mp.push_back(&m[c]);
+    }
+    auto image = [](const fw_grid* g, const double* d) -> std::optional<Image> {
+      if (!d) return std::nullopt;
+      const Grid gg = grid_of(g);
+      return Image(gg, std::vector<double>(d, d + gg.num_voxels()));
+    };
+    auto field = [](const fw_grid* g, const double* d) -> std::optional<VectorField> {
+      if (!d) return std::nullopt;
+      const Grid gg = grid_of(g);
+      std::vector<Vec3> v(gg.num_voxels());
+      for (size_t i = 0; i < v.size(); ++i) v[i] = {d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+      return VectorField(gg, std::move(v));
+    };
+    const std::optional<Image> fm = image(fixed_mask_grid, fixed_mask), mm = image(moving_mask_grid, moving_mask);
+    const std::optional<VectorField> pw = field(prior_grid, prior), pi = field(prior_grid, prior_inverse);
+    SynStageInputs<CpuBackend> extra;
+    extra.fixed_mask = fm ? &*fm : nullptr;
+    extra.moving_mask = mm ? &*mm : nullptr;
+    extra.prior = pw ? &*pw : nullptr;
+    extra.prior_inverse = pi ? &*pi : nullptr;
+    CpuBackend backend;
+    const RegisterOutput o = run_syn_stage(backend, plan, fp, mp, initial, extra, nullptr, FieldOutputs{forward, inverse});
+    result_to(o, result);
+  });
+}
+
 // The next fwo_register calls replay these inversion step counts in call order (n < 0: off).
 extern "C" void fwo_lockstep(const int32_t* steps, int64_t n) {
   g_script_on = n >= 0;

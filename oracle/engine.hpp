@@ -61,6 +61,10 @@ class CpuBackend {
   Image warp_through_field(const Image& src, const VectorField& f, const VectorField* pre, const AffineTransform* a) {
     return oracle::warp_through_field(src, f, pre, a);
   }
+  Image warp_mask_through_field(const Image& mask, const VectorField& f, const VectorField* pre, const AffineTransform* a,
+                                const Image* product) {
+    return oracle::warp_mask_through_field(mask, f, pre, a, product);
+  }
   MetricPair<VectorField> cc_pair(const Image& f, const Image& m, const Image* valid, size_t radius, bool norms) {
     MetricPair<VectorField> out;
     PointMap id;
