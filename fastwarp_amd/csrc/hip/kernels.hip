@@ -141,241 +141,11 @@ __device__ __forceinline__ void cc_column_step(CcColumn& col, const CcSplit& ea,
   col.c2ab += fma(Hae, lbe, lae * (Hbe + lbe)) - fma(Hal, lbl, lal * (Hbl + lbl));
 }
 
-// The work of one CC block: the tile `tile()` names in the grid of `a` (x and y as the launch's,
-// z within the item's own grid).
-template <int R, bool MASKED, bool DIAGONAL, typename Tile>
-__device__ __forceinline__ void cc2_block(const CcArgs& a, Tile tile) {
-  constexpr int kCcThreads = cc_threads<R>();
-  constexpr int W = 2 * R + 1;
-  constexpr int LX = kCcTX + 2 * R, LY = kCcTY + 2 * R;
-  constexpr int NC = LX * LY;
-  constexpr int KC = (NC + kCcThreads - 1) / kCcThreads;  // columns per thread
-  constexpr int RUNS = LY * (kCcTX / 2);                   // x pass: pairs of outputs per plane
-  constexpr int RPT = (RUNS + kCcThreads - 1) / kCcThreads;
-  // z sums of the current plane by column, then (in place) the x sums by output column
-  __shared__ double za[LY][LX], zb[LY][LX], zaa[LY][LX], zbb[LY][LX], zab[LY][LX];
-  __shared__ int zv[MASKED ? LY : 1][MASKED ? LX : 1];
-
-  const Dims d = a.d;
-  const int tid = threadIdx.x;
-  const uint3 bl = tile();
-  const int x0 = bl.x * kCcTX, y0 = bl.y * kCcTY, z0 = bl.z * a.tile_z;
-  const int z1 = min(z0 + a.tile_z, d.nz);
-  const long long plane = (long long)d.nx * d.ny;
-  const long long n3 = d.n();
-  float sa_ = a.scale_f, sb_ = a.scale_m, qa = a.step_f, qb = a.step_m;
-  if (a.bounds) {
-    sa_ = cc_split_scale_dev(a.bounds[0]);
-    sb_ = cc_split_scale_dev(a.bounds[1]);
-    qa = 1.0f / sa_;
-    qb = 1.0f / sb_;
-  }
-  const double qa2 = (double)qa * qa, qb2 = (double)qb * qb, qab = (double)qa * qb;
-
-  // the columns this thread owns: plane offset, or -1 outside the image
-  long long off[KC];
-  CcColumn col[KC];
-  int sv[KC];
-#pragma unroll
-  for (int k = 0; k < KC; ++k) {
-    const int c = tid + kCcThreads * k;
-    off[k] = -1;
-    if (c < NC) {
-      const int gx = x0 - R + c % LX, gy = y0 - R + c / LX;
-      if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) off[k] = (long long)gx + (long long)d.nx * gy;
-    }
-    col[k] = CcColumn{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    sv[k] = 0;
-  }
-  // the split values of one voxel (zero outside the image, the mask, or the z range)
-  auto load = [&](long long o, int z, CcSplit& va, CcSplit& vb, int& vv) {
-    va = vb = CcSplit{0.0f, 0.0f};
-    vv = 0;
-    if (o < 0 || z < 0 || z >= d.nz) return;
-    const long long g = o + plane * z;
-    if (MASKED && a.valid[g] == 0.0f) return;
-    va = cc_split(a.fixed[g], sa_, qa);
-    vb = cc_split(a.moving[g], sb_, qb);
-    vv = 1;
-  };
-  // warm-up: the window of plane z0
-  const CcSplit none{0.0f, 0.0f};
-  for (int z = z0 - R; z <= z0 + R; ++z) {
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      CcSplit va, vb;
-      int vv;
-      load(off[k], z, va, vb, vv);
-      cc_column_step(col[k], va, vb, none, none, qa, qb);
-      sv[k] += vv;
-    }
-  }
-
-  const int tx = tid & 63, ty = tid >> 6;  // y pass and output: column tx, row ty (ty < 8)
-  double cc_sum = 0.0;
-  int counted = 0;
-  for (int z = z0; z < z1; ++z) {
-    // the values entering and leaving the window for the next plane, fetched early
-    CcSplit ea[KC], eb[KC], la[KC], lb[KC];
-    int ev[KC], lv[KC];
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      load(off[k], z + R + 1, ea[k], eb[k], ev[k]);
-      load(off[k], z - R, la[k], lb[k], lv[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      const int c = tid + kCcThreads * k;
-      if (c < NC) {
-        const int ly = c / LX, lx = c % LX;
-        const CcColumn& s = col[k];
-        za[ly][lx] = fma(s.e1a, (double)qa, s.c1a);
-        zb[ly][lx] = fma(s.e1b, (double)qb, s.c1b);
-        zaa[ly][lx] = fma(s.e2a, qa2, s.c2a);
-        zbb[ly][lx] = fma(s.e2b, qb2, s.c2b);
-        zab[ly][lx] = fma(s.e2ab, qab, s.c2ab);
-        if constexpr (MASKED) zv[ly][lx] = sv[k];
-      }
-    }
-    __syncthreads();
-    // x pass: pairs of outputs along x share their middle 2r columns (summed directly, no running
-    // sum); held in registers until every pair has read its inputs, then written over the z sums
-    double xs[RPT][2][5];
-    int xv[RPT][2];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int e = tid + kCcThreads * q;
-      if (RUNS % kCcThreads == 0 || e < RUNS) {
-        const int ly = e / (kCcTX / 2), lx = (e % (kCcTX / 2)) * 2;
-        double m1 = 0.0, m2 = 0.0, m11 = 0.0, m22 = 0.0, m12 = 0.0;
-        int m3 = 0;
-#pragma unroll
-        for (int t = 1; t < W; ++t) {
-          m1 += za[ly][lx + t];
-          m2 += zb[ly][lx + t];
-          m11 += zaa[ly][lx + t];
-          m22 += zbb[ly][lx + t];
-          m12 += zab[ly][lx + t];
-          if constexpr (MASKED) m3 += zv[ly][lx + t];
-        }
-        xs[q][0][0] = za[ly][lx] + m1;
-        xs[q][0][1] = zb[ly][lx] + m2;
-        xs[q][0][2] = zaa[ly][lx] + m11;
-        xs[q][0][3] = zbb[ly][lx] + m22;
-        xs[q][0][4] = zab[ly][lx] + m12;
-        xs[q][1][0] = m1 + za[ly][lx + W];
-        xs[q][1][1] = m2 + zb[ly][lx + W];
-        xs[q][1][2] = m11 + zaa[ly][lx + W];
-        xs[q][1][3] = m22 + zbb[ly][lx + W];
-        xs[q][1][4] = m12 + zab[ly][lx + W];
-        if constexpr (MASKED) {
-          xv[q][0] = zv[ly][lx] + m3;
-          xv[q][1] = m3 + zv[ly][lx + W];
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int e = tid + kCcThreads * q;
-      if (RUNS % kCcThreads == 0 || e < RUNS) {
-        const int ly = e / (kCcTX / 2), lx = (e % (kCcTX / 2)) * 2;
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          za[ly][lx + r] = xs[q][r][0];
-          zb[ly][lx + r] = xs[q][r][1];
-          zaa[ly][lx + r] = xs[q][r][2];
-          zbb[ly][lx + r] = xs[q][r][3];
-          zab[ly][lx + r] = xs[q][r][4];
-          if constexpr (MASKED) zv[ly][lx + r] = xv[q][r];
-        }
-      }
-    }
-    __syncthreads();
-    // y pass: row ty of column tx
-    if (ty < kCcTY) {
-      double wa = 0.0, wb = 0.0, waa = 0.0, wbb = 0.0, wab = 0.0;
-      int wv = 0;
-#pragma unroll 3
-      for (int t = 0; t < W; ++t) {
-        wa += za[ty + t][tx];
-        wb += zb[ty + t][tx];
-        waa += zaa[ty + t][tx];
-        wbb += zbb[ty + t][tx];
-        wab += zab[ty + t][tx];
-        if constexpr (MASKED) wv += zv[ty + t][tx];
-      }
-      const int x = x0 + tx, y = y0 + ty;
-      if (x < d.nx && y < d.ny) {
-        const long long g = (long long)x + (long long)d.nx * y + plane * z;
-        float dfx = 0.0f, dfy = 0.0f, dfz = 0.0f, dmx = 0.0f, dmy = 0.0f, dmz = 0.0f;
-        const bool centre_valid = !MASKED || a.valid[g] != 0.0f;
-        if (centre_valid) {
-          // the window's voxel count (at most (2R + 1)^3)
-          const int n = MASKED ? wv : window_count(x, d.nx, R) * window_count(y, d.ny, R) * window_count(z, d.nz, R);
-          const double count = (double)n;
-          const double fmean = wa / count, mmean = wb / count;
-          const double s_ff = waa - fmean * wa - fmean * wa + count * fmean * fmean;
-          const double s_mm = wbb - mmean * wb - mmean * wb + count * mmean * mmean;
-          const double s_fm = wab - mmean * wa - fmean * wb + count * mmean * fmean;
-          const double product = s_ff * s_mm;
-          double cc = 1.0;
-          if (fabs(product) > (double)kEps64) cc = s_fm * s_fm / product;
-          cc_sum += cc;
-          counted += 1;
-          if (s_ff > (double)kEps64 && s_mm > (double)kEps64) {
-            const double fi = (double)a.fixed[g] - fmean;
-            const double mi = (double)a.moving[g] - mmean;
-            const double k1 = 2.0 * s_fm / product;
-            const double cf = k1 * (fi - s_fm / s_mm * mi);
-            const double cm = k1 * (mi - s_fm / s_ff * fi);
-            float gm[3], gf[3];
-            if constexpr (DIAGONAL) {
-              axis_gradient(a.moving, d, a.probes, x, y, z, gm);
-              axis_gradient(a.fixed, d, a.probes, x, y, z, gf);
-            } else {
-#pragma unroll 1
-              for (int q = 0; q < 3; ++q) {
-                gm[q] = central_difference(a.moving, d, a.probes, q, x, y, z);
-                gf[q] = central_difference(a.fixed, d, a.probes, q, x, y, z);
-              }
-            }
-            dfx = (float)(cf * (double)gm[0]);
-            dfy = (float)(cf * (double)gm[1]);
-            dfz = (float)(cf * (double)gm[2]);
-            dmx = (float)(cm * (double)gf[0]);
-            dmy = (float)(cm * (double)gf[1]);
-            dmz = (float)(cm * (double)gf[2]);
-          }
-        }
-        a.d_fixed[g] = dfx;
-        a.d_fixed[g + n3] = dfy;
-        a.d_fixed[g + 2 * n3] = dfz;
-        a.d_moving[g] = dmx;
-        a.d_moving[g + n3] = dmy;
-        a.d_moving[g + 2 * n3] = dmz;
-      }
-    }
-    // slide every owned column's window to the next plane
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      cc_column_step(col[k], ea[k], eb[k], la[k], lb[k], qa, qb);
-      sv[k] += ev[k] - lv[k];
-    }
-    __syncthreads();
-  }
-  const float total = block_sum_f64(cc_sum);
-  const float cnt = block_sum((float)counted);
-  if (tid == 0) {
-    const unsigned b = bl.x + gridDim.x * (bl.y + gridDim.y * bl.z);
-    a.partial_cc[b] = total;
-    a.partial_count[b] = (int)cnt;
-  }
-}
-
 template <int R, bool MASKED, bool DIAGONAL>
 __global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
-  cc2_block<R, MASKED, DIAGONAL>(a, [] { return xcd_block(); });
+#define FW_CC2_TILE xcd_block()
+#include "cc2_body.inc"
+#undef FW_CC2_TILE
 }
 
 // Several unmasked CC pairs on one grid: the grid's z blocks repeat once per item, and item q's
@@ -383,13 +153,13 @@ __global__ __launch_bounds__(cc_threads<R>()) void k_cc2(CcArgs a) {
 // own partial buffers).
 template <int R, bool DIAGONAL>
 __global__ __launch_bounds__(cc_threads<R>()) void k_cc2_items(CcItems c) {
+  constexpr bool MASKED = false;
   const unsigned gz = gridDim.z / (unsigned)c.count;
   const unsigned q = xcd_block().z / gz;
-  cc2_block<R, false, DIAGONAL>(c.a[q], [q, gz] {
-    uint3 bl = xcd_block();
-    bl.z -= q * gz;
-    return bl;
-  });
+  const CcArgs& a = c.a[q];
+#define FW_CC2_TILE make_uint3(xcd_block().x, xcd_block().y, xcd_block().z - q * gz)
+#include "cc2_body.inc"
+#undef FW_CC2_TILE
 }
 
 template <int R, bool M>
