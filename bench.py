@@ -171,6 +171,12 @@ def batched(backend, subjects: int) -> dict:
     syn_s = rs[0].result.seconds_syn
     linear_s = sum(r.result.seconds_linear for r in rs)
     free(rs)
+    # the SyN stage's kernels (on this context; the linear chains run on the workers' own streams),
+    # every launch timed, in one more run
+    backend.set_profiling(True)
+    free(register_batch(backend, plan, tmpl, subj))
+    prof = {k: v for k, v in backend.take_profile().items() if not k.startswith("host:")}
+    backend.set_profiling(False)
     t = time.perf_counter()
     plain_syn = 0.0
     for s in range(subjects):
@@ -186,8 +192,10 @@ def batched(backend, subjects: int) -> dict:
             "syn_s": {"batched": round(syn_s, 3), "one_at_a_time": round(plain_syn, 3),
                       "vs_one_at_a_time": round(plain_syn / max(syn_s, 1e-9), 3)},
             "linear_s": round(linear_s, 3),
-            "path": "fw_register_batch: shared template resident in HBM, linear chains per subject, SyN in lockstep "
-                    "with one launch per step over all subjects; fields left in HBM"}
+            "syn_kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "gbs": round(v[2] / max(v[1], 1e-9) / 1e6, 1)}
+                            for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+            "path": "fw_register_batch: shared template resident in HBM, linear chains side by side on worker streams, "
+                    "SyN in lockstep with one launch per step over all subjects; fields left in HBM"}
 
 
 def fit_sizes(fits: int, seed: int = 3):
