@@ -88,7 +88,7 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
   if (plan.run_syn) {
     backend.release_idle_buffers();
     t = clock::now();
-    out.syn = syn_fields(backend, plan.syn, out.total, cache, cancel, outputs);
+    out.syn = syn_stage_of(backend, plan.syn, out.total, cache, cancel, outputs, nullptr, nullptr, SynStageInputs<B>{});
     out.seconds_syn = secs(t);
   }
   out.seconds_total = secs(t_all);
@@ -141,7 +141,8 @@ RegisterOutput register_resident(B& backend, E& engine, const RegisterPlan& plan
   if (plan.run_syn) {
     backend.release_idle_buffers();
     t = clock::now();
-    out.syn = syn_fields(backend, plan.syn, out.total, cache, cancel, FieldOutputs{}, forward_device, inverse_device);
+    out.syn = syn_stage_of(backend, plan.syn, out.total, cache, cancel, FieldOutputs{}, forward_device, inverse_device,
+                           SynStageInputs<B>{});
     out.seconds_syn = secs(t);
   }
   out.seconds_total = secs(t_all);

@@ -365,4 +365,14 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
   return out;
 }
 
+// The SyN stage a registration runs on backend B: the loop above. An engine may provide its own
+// stage for its backend type (an overload found by argument-dependent lookup).
+template <typename B>
+SynFields syn_stage_of(B& b, const SynRecipe& recipe, const AffineTransform& initial, SmoothCache<B>& cache,
+                       const volatile int32_t* cancel, FieldOutputs outputs,
+                       std::optional<typename B::FieldH>* forward_device, std::optional<typename B::FieldH>* inverse_device,
+                       const SynStageInputs<B>& extra) {
+  return syn_fields(b, recipe, initial, cache, cancel, outputs, forward_device, inverse_device, extra);
+}
+
 }  // namespace fw

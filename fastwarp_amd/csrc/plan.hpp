@@ -146,7 +146,7 @@ RegisterOutput run_syn_stage(B& b, const fw_plan* plan, const std::vector<const 
   RegisterOutput o;
   o.initial = o.total = affine_of(initial);
   const auto t = std::chrono::steady_clock::now();
-  o.syn = syn_fields(b, p.syn, o.total, cache, cancel, outputs, forward_device, inverse_device, extra);
+  o.syn = syn_stage_of(b, p.syn, o.total, cache, cancel, outputs, forward_device, inverse_device, extra);
   o.seconds_syn = o.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
   return o;
 }

@@ -276,6 +276,12 @@ def syn_stage(plan, fixed_grid: Grid, fixed, moving_grid: Grid, moving, initial:
     return res, fwd, inv
 
 
+def generic_syn(on: bool):
+    """Run the CPU engine's SyN stages through the product's generic loop (True) or the
+    oracle's own restatement of syn.rs (False, the default)."""
+    lib().fwo_generic_syn(int(on))
+
+
 def last_trace():
     """Per-iteration records of this thread's last `register` call (fastwarp_amd._abi.TraceStep)."""
     from fastwarp_amd._abi import read_trace

@@ -7,6 +7,8 @@ run_syn_from with SynStart::prior and the recipe's masks; fw_syn_stage):
 * a stage with no iterations returns the earlier stages' fields exactly (the collapse of
   syn.rs:746-774 composes them with zero fields), and earlier fields off the last level's grid are
   refused (syn.rs:804-817);
+* the CPU engine's own SyN loop (oracle/syn.hpp) and the product's generic loop over the same ops
+  give the same bits;
 * the HIP path against the CPU engine with masks and mixed metrics, and over two stages, to the
   accuracy bounds of whole registrations (fields mean 0.1, p99 0.5 voxel; energies 1e-4).
 """
@@ -106,6 +108,32 @@ def test_earlier_fields_off_the_last_level_grid_are_refused():
     zero = np.zeros((coarse.n, 3))
     with pytest.raises(RuntimeError, match="same shrink factor"):
         oracle.syn_stage(stage_plan([CC]), GRID, [fixed[0]], GRID, [moving[0]], prior=(coarse, zero, zero))
+
+
+def test_the_cpu_engines_own_syn_loop_gives_the_bits_of_the_generic_loop():
+    """The CPU engine runs its own restatement of syn.rs (oracle/syn.hpp); the product's generic
+    SyN loop (which defers read-backs and pairs operations for the device) over the same f64 ops
+    must give the same bits: one CC channel, CC + Mattes with half masks, and a second stage
+    after a first."""
+    fixed, moving = channels()
+    half = (GRID, half_mask())
+    _, f1, i1 = oracle.syn_stage(stage_plan([CC]), GRID, [fixed[0]], GRID, [moving[0]])
+    cases = [
+        (stage_plan([CC]), [fixed[0]], [moving[0]], {}),
+        (stage_plan([CC, MATTES]), fixed, moving, {"fixed_mask": half, "moving_mask": half}),
+        (stage_plan([CC]), [fixed[0]], [moving[0]], {"prior": (GRID, f1, i1)}),
+    ]
+    for k, (plan, f, m, kw) in enumerate(cases):
+        own = oracle.syn_stage(plan, GRID, f, GRID, m, **kw)
+        own_trace = [(t.value, tuple(t.inversion_steps)) for t in oracle.last_trace()]
+        oracle.generic_syn(True)
+        try:
+            generic = oracle.syn_stage(plan, GRID, f, GRID, m, **kw)
+            generic_trace = [(t.value, tuple(t.inversion_steps)) for t in oracle.last_trace()]
+        finally:
+            oracle.generic_syn(False)
+        assert own_trace == generic_trace, k
+        assert np.array_equal(own[1], generic[1]) and np.array_equal(own[2], generic[2]), k
 
 
 # ---------------------------------------------------------------------------- the HIP path --
