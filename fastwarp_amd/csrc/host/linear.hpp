@@ -428,6 +428,14 @@ LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, co
   return out;
 }
 
+// The linear chain a registration runs with engine E: the chain above. An engine may provide its
+// own chain for its type (an overload found by argument-dependent lookup).
+template <typename Engine, typename B>
+LinearChainResult linear_chain_of(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine,
+                                  const AffineTransform& initial, SmoothCache<B>& cache, const volatile int32_t* cancel) {
+  return run_linear_chain(engine, rigid, affine, initial, cache, cancel);
+}
+
 // The same from the raw first channel: the centre-of-mass initialisation first.
 template <typename Engine, typename B, typename V>
 LinearChainResult run_linear_chain(Engine& engine, const LinearRecipe& rigid, const LinearRecipe& affine, const V& fixed_raw,

@@ -78,7 +78,7 @@ RegisterOutput register_full(B& backend, E& engine, const RegisterPlan& plan, co
                                 : plan.initial_kind == InitialKind::Identity   ? AffineTransform{}
                                                                                 : plan.initial_affine;
   if (plan.run_linear) {
-    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, start, cache, cancel);
+    out.linear = linear_chain_of(engine, plan.rigid, plan.affine, start, cache, cancel);
     out.initial = out.linear.initial;
     out.total = out.linear.total;
   } else {
@@ -131,7 +131,7 @@ RegisterOutput register_resident(B& backend, E& engine, const RegisterPlan& plan
     start = AffineTransform{};
   }
   if (plan.run_linear) {
-    out.linear = run_linear_chain(engine, plan.rigid, plan.affine, start, cache, cancel);
+    out.linear = linear_chain_of(engine, plan.rigid, plan.affine, start, cache, cancel);
     out.initial = out.linear.initial;
     out.total = out.linear.total;
   } else {

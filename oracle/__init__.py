@@ -282,6 +282,12 @@ def generic_syn(on: bool):
     lib().fwo_generic_syn(int(on))
 
 
+def generic_linear(on: bool):
+    """Run the CPU engine's linear chains through the product's generic driver (True) or the
+    oracle's own restatement of linear.rs and optimizer.rs (False, the default)."""
+    lib().fwo_generic_linear(int(on))
+
+
 def last_trace():
     """Per-iteration records of this thread's last `register` call (fastwarp_amd._abi.TraceStep)."""
     from fastwarp_amd._abi import read_trace

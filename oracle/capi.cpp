@@ -8,6 +8,7 @@
 #include "../include/fastwarp_hip.h"
 #include "../fastwarp_amd/csrc/plan.hpp"
 #include "engine.hpp"
+#include "linear.hpp"
 #include "syn.hpp"
 #include "oracle.hpp"
 
@@ -371,6 +372,8 @@ extern "C" int fwo_syn_stage(const fw_plan* plan, const fw_grid* fixed_grid, con
 
 // 1: the CPU engine's SyN stages run the product's generic loop over the oracle's ops; 0: its own.
 extern "C" void fwo_generic_syn(int on) { use_generic_syn() = on != 0; }
+// The same for the linear chain.
+extern "C" void fwo_generic_linear(int on) { use_generic_linear() = on != 0; }
 
 // The next fwo_register calls replay these inversion step counts in call order (n < 0: off).
 extern "C" void fwo_lockstep(const int32_t* steps, int64_t n) {

@@ -60,6 +60,31 @@ def test_batched_subjects_give_the_bits_of_their_single_runs(backend, channels, 
             assert np.array_equal(a, b), (label, name, float(np.max(np.abs(a - b))))
 
 
+def test_batched_two_channel_subjects_at_1mm_give_the_bits_of_their_single_runs(backend):
+    """The batched multichannel workload's shape (1 mm, 182 x 218 x 182, two channels, CC radius 4
+    on both, the cohort pyramid 8/4/2/1) with two subjects and a shortened plan: every subject's
+    affine, trace and fields are the bits of its own run."""
+    from fastwarp_amd.register import cohort_plan
+
+    grid, fixed, moving = cohort(2, 2, shape=(182, 218, 182), spacing=1.0)
+    plan = cohort_plan(2)
+    plan.set_linear([20, 10, 5, 2], [8, 4, 2, 1], [3.0, 2.0, 1.0, 0.0])
+    plan.set_syn([10, 6, 3, 2], [8, 4, 2, 1], [3.0, 2.0, 1.0, 0.0])
+    up = lambda vols: [backend.upload_image(grid, v) for v in vols]
+    tmpl = up(fixed)
+    subj = [up(m) for m in moving]
+    batch = register_batch(backend, plan, tmpl, subj)
+    traces = [steps(last_batch_trace(s)) for s in range(2)]
+    for s in range(2):
+        alone = register_resident(backend, plan, tmpl, subj[s])
+        assert np.array_equal(affine_array(batch[s].affine), affine_array(alone.affine)), s
+        assert traces[s] == steps(last_trace()), s
+        for name in ("forward", "inverse"):
+            a = backend.download_field(getattr(batch[s], name))
+            b = backend.download_field(getattr(alone, name))
+            assert np.array_equal(a, b), (s, name, float(np.max(np.abs(a - b))))
+
+
 def test_batch_without_subjects_is_an_error(backend):
     from fastwarp_amd.backend import RegError
 

@@ -117,6 +117,33 @@ def test_hip_syn_matches_the_cpu_engine(backend, channels):
         assert abs(g - w) <= 1e-4 * abs(w), (report, energies, iters)
 
 
+def test_the_cpu_engines_own_linear_chain_gives_the_bits_of_the_generic_chain():
+    """The CPU engine runs its own restatement of linear.rs and optimizer.rs (oracle/linear.hpp:
+    no prefetch, no level reuse, the plain golden section); the product's generic chain over the
+    same f64 ops must give the same bits: the affine, every level's CG values and counts. Three
+    pairs: two shifted phantoms and one with a level skipped for want of samples."""
+    cases = [(make_pair((40, 44, 36), 4.0, seed=3, affine_only=True), small_plan()),
+             (make_pair((48, 40, 44), 3.0, seed=7, affine_only=True), small_plan())]
+    p = cohort_plan(1)
+    p.set_linear([30, 15, 8], [8, 2, 1], [2.0, 1.0, 0.0])
+    cases.append((make_pair((24, 24, 24), 4.0, seed=1, affine_only=True), p))
+    for k, (pair, plan) in enumerate(cases):
+        plan.run_syn = 0
+        runs = []
+        for generic in (False, True):
+            oracle.generic_linear(generic)
+            try:
+                res, _, _ = oracle.register(plan, pair.grid, pair.fixed, pair.grid, pair.moving, fields=False)
+                runs.append((res, [(t.stage, t.level, t.iteration, t.value) for t in oracle.last_trace()]))
+            finally:
+                oracle.generic_linear(False)
+        (own, own_trace), (gen, gen_trace) = runs
+        assert own_trace and own_trace == gen_trace, k
+        assert [list(x) for x in own.linear_iterations] == [list(x) for x in gen.linear_iterations], k
+        for a, b in zip(affine_matrix(own.affine), affine_matrix(gen.affine)):
+            assert np.array_equal(a, b), k
+
+
 def test_a_given_initial_transform_starts_the_chain_where_the_centre_of_mass_would():
     """stages.rs:74-86, 319-325: the chain starts from the centre of mass, the identity or a given
     affine; the centre-of-mass affine given explicitly reproduces the default run bit for bit,
