@@ -17,6 +17,8 @@ using clock_type = std::chrono::steady_clock;
 
 struct Subject {
   std::optional<HField> f2m, f2m_inv, m2m, m2m_inv;
+  // the buffers the next compositions write (they swap with f2m / m2m every iteration)
+  std::optional<HField> alt_f, alt_m;
   std::optional<WindowConvergenceMonitor> monitor;
   SynLevelTrace trace;
   uint32_t current = 0;
@@ -175,20 +177,166 @@ InvertStats stats_of(const volatile InvertReport& r) {
   return st;
 }
 
-// One lockstep iteration of the subjects `act` (all on grid g). Fills it[q] for each.
-void iterate(HipBackend& b, const SynRecipe& recipe, const std::vector<double>& normalised,
-             const std::vector<AffineTransform>& initial,
-             const std::vector<std::vector<SmoothCache<HipBackend>::Pair>>& smooth, std::vector<Subject>& subj,
-             const std::vector<size_t>& act, const Grid& g, std::vector<SynIteration>& its) {
+// The buffers of one level's iterations, sized for the `k` subjects active when the level starts
+// and kept for the whole level (the q-th active subject uses the q-th slots, so the smaller active
+// sets of later iterations fit), so that an iteration's launches can be captured into a HIP graph
+// once and replayed: every pointer they take stays fixed. The graphs are keyed by the active set
+// and the buffers its launches use (the compositions' buffers alternate between two per side).
+struct LevelBuffers {
+  size_t k = 0, nc = 0;
+  Grid grid;
+  std::vector<std::vector<HImage>> fmid, mmid;  // [k][nc]
+  std::vector<std::vector<HField>> dfix, dmov;  // [k][nc]
+  std::vector<HField> grads;                    // [2k] the channels combined (several channels)
+  std::vector<HField> smoothed;                 // [2k]
+  std::vector<DevBuf> tmp;                      // [2k] the x/y pass of the two-pass smoothing
+  DevBuf norm_parts, coef, step_parts, factors;
+  int nb = 0, tile_z = 0;
+  size_t blocks = 0;
+  char* cc_host = nullptr;
+  float* pcc_dev = nullptr;
+  int* pcount_dev = nullptr;
+  char* small_host = nullptr;
+  char* small_dev = nullptr;
+  std::vector<std::pair<std::vector<const void*>, hipGraphExec_t>> graphs;
+
+  LevelBuffers() = default;
+  LevelBuffers(const LevelBuffers&) = delete;
+  LevelBuffers& operator=(const LevelBuffers&) = delete;
+  ~LevelBuffers() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  }
+};
+
+// Offsets in the iteration's coherent host words: per subject 2 maxima and a bad flag, then the
+// reports of the two inversion stages (4 per subject).
+size_t report_offset(size_t k) { return ((2 * k * sizeof(float) + k * sizeof(uint32_t)) + 15) / 16 * 16; }
+
+std::unique_ptr<LevelBuffers> make_level_buffers(HipBackend& b, const SynRecipe& recipe, size_t k, size_t nc, const Grid& g) {
+  DeviceContext& ctx = b.ctx();
+  auto L = std::make_unique<LevelBuffers>();
+  L->k = k;
+  L->nc = nc;
+  L->grid = g;
+  const size_t n = g.num_voxels();
+  const Dims d = dims_of(g);
+  L->fmid.resize(k);
+  L->mmid.resize(k);
+  L->dfix.resize(k);
+  L->dmov.resize(k);
+  for (size_t q = 0; q < k; ++q)
+    for (size_t c = 0; c < nc; ++c) {
+      L->fmid[q].push_back(HImage{g, DevBuf(&ctx, n), 0.0f});
+      L->mmid[q].push_back(HImage{g, DevBuf(&ctx, n), 0.0f});
+      L->dfix[q].push_back(HField{g, DevBuf(&ctx, 3 * n)});
+      L->dmov[q].push_back(HField{g, DevBuf(&ctx, 3 * n)});
+    }
+  for (size_t q = 0; q < 2 * k; ++q) {
+    if (nc > 1) L->grads.push_back(HField{g, DevBuf(&ctx, 3 * n)});
+    if (recipe.update_field_variance > 0.0) {
+      L->smoothed.push_back(HField{g, DevBuf(&ctx, 3 * n)});
+      L->tmp.emplace_back(&ctx, 3 * n);
+    }
+  }
+  L->nb = norm_blocks(long(n));
+  if (nc > 1) {
+    L->norm_parts = DevBuf(&ctx, k * nc * 4 * size_t(L->nb));
+    L->coef = DevBuf(&ctx, 2 * k * (kBatchChannels + 2));
+  }
+  L->step_parts = DevBuf(&ctx, k * 4 * size_t(L->nb));
+  L->factors = DevBuf(&ctx, 2 * k);
+  L->tile_z = cc_tile_for(d, int(ctx.tuning().get("cc.tile_z", n, b.cc_tile_z)), int(ctx.tuning().get("cc.min_blocks", n, 512)));
+  const dim3 gr = cc_grid(d, L->tile_z);
+  L->blocks = size_t(gr.x) * gr.y * gr.z;
+  // coherent host memory for the whole level (later, smaller active sets fit in it)
+  void* cc_dev = nullptr;
+  L->cc_host = static_cast<char*>(ctx.mapped(6, k * nc * L->blocks * (sizeof(float) + sizeof(int)), &cc_dev));
+  L->pcc_dev = static_cast<float*>(cc_dev);
+  L->pcount_dev = reinterpret_cast<int*>(L->pcc_dev + k * nc * L->blocks);
+  void* small_dev = nullptr;
+  L->small_host = static_cast<char*>(ctx.mapped(7, report_offset(k) + 4 * k * sizeof(InvertReport) + 64, &small_dev));
+  L->small_dev = static_cast<char*>(small_dev);
+  return L;
+}
+
+// The taps of the update smoothing on grid g (as smooth_field_pair takes them) and whether the
+// batched kernels take m fields of that grid at once.
+bool batch_taps(DeviceContext& ctx, const Grid& g, size_t m, double variance, GaussTaps3& t) {
+  int* radii[3] = {&t.rx, &t.ry, &t.rz};
+  float* dst[3] = {t.x, t.y, t.z};
+  for (int q = 0; q < 3; ++q) {
+    const std::vector<double>& taps = ctx.host_taps(variance, g.size[q]);
+    if (taps.size() > 17) return false;
+    *radii[q] = int(taps.size() / 2);
+    for (size_t k = 0; k < taps.size(); ++k) dst[q][k] = float(taps[k]);
+  }
+  const Dims d = dims_of(g);
+  return gauss3_fits(d, t) && m <= size_t(kBatchFields) && 3ull * m * size_t(d.nz) <= 65535;
+}
+
+// smooth_fields into the level's buffers: the m updates `in` smoothed into L.smoothed with the
+// same taps, pass shape and kernels (so the same bits), allocating nothing. The caller has checked
+// batch_taps.
+void smooth_into(DeviceContext& ctx, LevelBuffers& L, const std::vector<const float*>& in, double variance) {
+  const Grid& g = L.grid;
+  const Dims d = dims_of(g);
+  const size_t n = g.num_voxels(), m = in.size();
+  const double w1 = variance < 0.5 ? 1.0 - variance / 0.5 : 1.0;
+  const double w2 = 1.0 - w1;
+  GaussTaps3 t{};
+  if (!batch_taps(ctx, g, m, variance, t)) invalid("smoothing", "the batched smoothing does not fit the grid");
+  hipStream_t s = ctx.stream();
+  const double fields = double(m);
+  FieldPtrs p{};
+  p.count = int(m);
+  for (size_t q = 0; q < m; ++q) {
+    p.in[q] = in[q];
+    p.orig[q] = in[q];
+    p.out[q] = L.smoothed[q].buf.get();
+  }
+  if (t.rx == t.ry && t.ry == t.rz && ctx.tuning().get("gauss.one_pass", uint64_t(d.n()), gauss3_pays(d) ? 1 : 0)) {
+    Timed tm(ctx, "k_gauss3", fields * (24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0)));
+    if (launch_gauss3_set(p, d, t, float(w1), float(w2), s)) return;
+  }
+  FieldPtrs xy{}, z{};
+  xy.count = z.count = int(m);
+  for (size_t q = 0; q < m; ++q) {
+    xy.in[q] = in[q];
+    xy.out[q] = L.tmp[q].get();
+    z.in[q] = L.tmp[q].get();
+    z.orig[q] = in[q];
+    z.out[q] = L.smoothed[q].buf.get();
+  }
+  {
+    Timed tm(ctx, "k_gauss_xy", fields * 24.0 * double(n));
+    if (!launch_gauss_xy_set(xy, d, t, s)) invalid("smoothing", "the batched x/y pass does not fit the grid");
+  }
+  Timed tm(ctx, "k_gauss_z", fields * (24.0 * double(n) + (w2 != 0.0 ? 12.0 * double(n) : 0.0)));
+  if (!launch_gauss_z_set(z, d, t, float(w1), float(w2), s)) invalid("smoothing", "the batched z pass does not fit the grid");
+}
+
+// Whether the updates of m fields on grid g can be smoothed by smooth_into.
+bool smooth_fits(DeviceContext& ctx, const Grid& g, size_t m, double variance) {
+  GaussTaps3 t{};
+  return variance <= 0.0 || batch_taps(ctx, g, m, variance, t);
+}
+
+// The launches of one iteration before the inversions: warps, CC pairs, channel weighting, update
+// smoothing, step factors, compositions into the subjects' alternate buffers. Where the update
+// smoothing fits the batched kernels they read and write only the level buffers, the subjects'
+// fields and the level's coherent host words, and allocate nothing, so they can be captured.
+void issue_updates(HipBackend& b, const SynRecipe& recipe, const std::vector<double>& normalised,
+                   const std::vector<AffineTransform>& initial,
+                   const std::vector<std::vector<SmoothCache<HipBackend>::Pair>>& smooth, std::vector<Subject>& subj,
+                   const std::vector<size_t>& act, LevelBuffers& L) {
   DeviceContext& ctx = b.ctx();
   hipStream_t st = ctx.stream();
+  const Grid& g = L.grid;
   const size_t k = act.size(), nc = normalised.size();
   const Dims d = dims_of(g);
   const size_t n = g.num_voxels();
   const double dn = double(n);
-
   // 1. every channel of every subject resampled into the middle (syn.rs:895-911)
-  std::vector<std::vector<HImage>> fmid(k), mmid(k);
   for (size_t c = 0; c < nc; ++c) {
     WarpItems wf{}, wm{};
     wf.count = wm.count = int(k);
@@ -197,97 +345,66 @@ void iterate(HipBackend& b, const SynRecipe& recipe, const std::vector<double>& 
       const Subject& u = subj[act[q]];
       const HImage& F = *smooth[act[q]][c].fixed;
       const HImage& M = *smooth[act[q]][c].moving;
-      fmid[q].push_back(HImage{g, DevBuf(&ctx, n), F.bound});
-      mmid[q].push_back(HImage{g, DevBuf(&ctx, n), M.bound});
+      L.fmid[q][c].bound = F.bound;
+      L.mmid[q][c].bound = M.bound;
       wf.a[q] = warp_args(F.grid, *u.f2m_inv, nullptr, nullptr);
       wf.a[q].src = F.buf.get();
-      wf.a[q].out = fmid[q][c].buf.get();
+      wf.a[q].out = L.fmid[q][c].buf.get();
       wm.a[q] = warp_args(M.grid, *u.m2m_inv, nullptr, &initial[act[q]]);
       wm.a[q].src = M.buf.get();
-      wm.a[q].out = mmid[q][c].buf.get();
+      wm.a[q].out = L.mmid[q][c].buf.get();
       src_bytes += 4.0 * (std::min(dn, double(F.grid.num_voxels())) + std::min(dn, double(M.grid.num_voxels())));
     }
     Timed t(ctx, "k_warp", 32.0 * dn * double(k) + src_bytes);
     launch_warp_items(wf, st);
     launch_warp_items(wm, st);
   }
-
   // 2. the CC pair of every channel, both directions (syn.rs:337-420); block partials straight
   // into coherent host memory
-  const int tile_z = cc_tile_for(d, int(ctx.tuning().get("cc.tile_z", n, b.cc_tile_z)),
-                                 int(ctx.tuning().get("cc.min_blocks", n, 512)));
-  const dim3 gr = cc_grid(d, tile_z);
-  const size_t blocks = size_t(gr.x) * gr.y * gr.z;
-  void* cc_dev = nullptr;
-  char* cc_host = static_cast<char*>(ctx.mapped(6, k * nc * blocks * (sizeof(float) + sizeof(int)), &cc_dev));
-  float* pcc_dev = static_cast<float*>(cc_dev);
-  int* pcount_dev = reinterpret_cast<int*>(pcc_dev + k * nc * blocks);
-  std::vector<std::vector<HField>> dfix(k), dmov(k);
   for (size_t c = 0; c < nc; ++c) {
     CcItems ci{};
     ci.count = int(k);
     for (size_t q = 0; q < k; ++q) {
-      dfix[q].push_back(HField{g, DevBuf(&ctx, 3 * n)});
-      dmov[q].push_back(HField{g, DevBuf(&ctx, 3 * n)});
       CcArgs& a = ci.a[q];
-      a.fixed = fmid[q][c].buf.get();
-      a.moving = mmid[q][c].buf.get();
+      a.fixed = L.fmid[q][c].buf.get();
+      a.moving = L.mmid[q][c].buf.get();
       a.valid = nullptr;
       a.d = d;
       a.radius = int(recipe.cc_radius);
-      a.tile_z = tile_z;
-      a.scale_f = cc_split_scale(fmid[q][c].bound);
-      a.scale_m = cc_split_scale(mmid[q][c].bound);
+      a.tile_z = L.tile_z;
+      a.scale_f = cc_split_scale(L.fmid[q][c].bound);
+      a.scale_m = cc_split_scale(L.mmid[q][c].bound);
       a.step_f = 1.0f / a.scale_f;
       a.step_m = 1.0f / a.scale_m;
       a.bounds = nullptr;
       a.probes = probes_of(g);
-      a.d_fixed = dfix[q][c].buf.get();
-      a.d_moving = dmov[q][c].buf.get();
-      a.partial_cc = pcc_dev + (q * nc + c) * blocks;
-      a.partial_count = pcount_dev + (q * nc + c) * blocks;
+      a.d_fixed = L.dfix[q][c].buf.get();
+      a.d_moving = L.dmov[q][c].buf.get();
+      a.partial_cc = L.pcc_dev + (q * nc + c) * L.blocks;
+      a.partial_count = L.pcount_dev + (q * nc + c) * L.blocks;
     }
     Timed t(ctx, "k_cc2", 32.0 * dn * double(k));
     launch_cc_items(ci, st);
   }
-  fmid.clear();
-  mmid.clear();
-
   // the update of each side: the single channel's derivative, or the channels weighted on the
   // device as the host weighting does (syn.rs:203-297)
-  std::vector<HField> grads;  // 2 q + side
-  uint32_t* bad_host = nullptr;
-  std::vector<DevBuf> scratch;
-  volatile float* maxima_host = nullptr;
-  void* small_dev = nullptr;
-  // coherent host words of this iteration: per subject 2 maxima, a bad flag, then the reports of
-  // the two inversion stages
-  const size_t small_bytes = k * (2 * sizeof(float) + sizeof(uint32_t)) + 4 * k * sizeof(InvertReport) + 64;
-  char* small_host = static_cast<char*>(ctx.mapped(7, small_bytes, &small_dev));
-  maxima_host = reinterpret_cast<volatile float*>(small_host);
-  bad_host = reinterpret_cast<uint32_t*>(small_host + 2 * k * sizeof(float));
-  const size_t report_off = ((2 * k * sizeof(float) + k * sizeof(uint32_t)) + 15) / 16 * 16;
-  auto* reports_host = reinterpret_cast<volatile InvertReport*>(small_host + report_off);
-  auto* reports_dev = reinterpret_cast<InvertReport*>(static_cast<char*>(small_dev) + report_off);
-  float* maxima_dev = static_cast<float*>(small_dev);
-  uint32_t* bad_dev = reinterpret_cast<uint32_t*>(static_cast<char*>(small_dev) + 2 * k * sizeof(float));
-  for (size_t q = 0; q < k; ++q) bad_host[q] = 0;
+  std::vector<const float*> upd(2 * k);
+  uint32_t* bad_dev = reinterpret_cast<uint32_t*>(L.small_dev + 2 * k * sizeof(float));
   if (nc == 1) {
     for (size_t q = 0; q < k; ++q) {
-      grads.push_back(std::move(dfix[q][0]));
-      grads.push_back(std::move(dmov[q][0]));
+      upd[2 * q] = L.dfix[q][0].buf.get();
+      upd[2 * q + 1] = L.dmov[q][0].buf.get();
     }
   } else {
-    const int nb = norm_blocks(long(n));
-    DevBuf parts(&ctx, k * nc * 4 * size_t(nb));
+    const int nb = L.nb;
     const float3 isp = inv_spacing(g);
     for (size_t c = 0; c < nc; ++c) {
       NormItems it{};
       it.count = int(k);
       for (size_t q = 0; q < k; ++q) {
-        it.a[q] = dfix[q][c].buf.get();
-        it.b[q] = dmov[q][c].buf.get();
-        it.partial_max[q] = parts.get() + (q * nc + c) * 4 * size_t(nb);
+        it.a[q] = L.dfix[q][c].buf.get();
+        it.b[q] = L.dmov[q][c].buf.get();
+        it.partial_max[q] = L.norm_parts.get() + (q * nc + c) * 4 * size_t(nb);
         it.partial_sq[q] = it.partial_max[q] + 2 * nb;
       }
       Timed t(ctx, "k_field_norms", 24.0 * dn * double(k));
@@ -298,148 +415,201 @@ void iterate(HipBackend& b, const SynRecipe& recipe, const std::vector<double>& 
     w.count = int(k);
     for (size_t c = 0; c < nc; ++c) w.weight[c] = normalised[c];
     for (size_t q = 0; q < k; ++q)
-      for (size_t c = 0; c < nc; ++c) w.partial_sq[q][c] = parts.get() + (q * nc + c) * 4 * size_t(nb) + 2 * nb;
-    DevBuf coef(&ctx, 2 * k * (kBatchChannels + 2));
-    launch_channel_weights(w, nb, coef.get(), bad_dev, st);
+      for (size_t c = 0; c < nc; ++c) w.partial_sq[q][c] = L.norm_parts.get() + (q * nc + c) * 4 * size_t(nb) + 2 * nb;
+    launch_channel_weights(w, nb, L.coef.get(), bad_dev, st);
     CombineItems ci{};
     ci.channels = int(nc);
     ci.count = int(2 * k);
     for (size_t q = 0; q < k; ++q)
       for (int side = 0; side < 2; ++side) {
-        grads.push_back(HField{g, DevBuf(&ctx, 3 * n)});
-        for (size_t c = 0; c < nc; ++c) ci.d[2 * q + side][c] = (side == 0 ? dfix[q][c] : dmov[q][c]).buf.get();
-        ci.out[2 * q + side] = grads.back().buf.get();
+        for (size_t c = 0; c < nc; ++c) ci.d[2 * q + side][c] = (side == 0 ? L.dfix[q][c] : L.dmov[q][c]).buf.get();
+        ci.out[2 * q + side] = L.grads[2 * q + side].buf.get();
+        upd[2 * q + side] = L.grads[2 * q + side].buf.get();
       }
-    {
-      Timed t(ctx, "k_combine", (12.0 * double(nc) + 12.0) * dn * double(2 * k));
-      launch_combine_items(ci, coef.get(), long(3 * n), st);
-    }
-    scratch.push_back(std::move(parts));
-    scratch.push_back(std::move(coef));
+    Timed t(ctx, "k_combine", (12.0 * double(nc) + 12.0) * dn * double(2 * k));
+    launch_combine_items(ci, L.coef.get(), long(3 * n), st);
   }
-  dfix.clear();
-  dmov.clear();
-
-  // 3. smooth both updates of every subject
-  grads = smooth_fields(b, std::move(grads), recipe.update_field_variance);
-
+  // 3. smooth both updates of every subject: in the level's buffers, or (a grid the batched
+  // kernels do not take; never captured) copies through the pair form
+  std::vector<HField> held;
+  if (recipe.update_field_variance > 0.0) {
+    if (smooth_fits(ctx, g, 2 * k, recipe.update_field_variance)) {
+      smooth_into(ctx, L, upd, recipe.update_field_variance);
+      for (size_t q = 0; q < 2 * k; ++q) upd[q] = L.smoothed[q].buf.get();
+    } else {
+      std::vector<HField> copies;
+      for (size_t q = 0; q < 2 * k; ++q) {
+        copies.push_back(HField{g, DevBuf(&ctx, 3 * n)});
+        check(hipMemcpyAsync(copies.back().buf.get(), upd[q], 3 * n * sizeof(float), hipMemcpyDeviceToDevice, st),
+              "copy update");
+      }
+      held = smooth_fields(b, std::move(copies), recipe.update_field_variance);
+      for (size_t q = 0; q < 2 * k; ++q) upd[q] = held[q].buf.get();
+    }
+  }
   // 4. each update scaled so its largest voxel step is the gradient step, applied in the
-  // composition (syn.rs:422-445)
+  // composition (syn.rs:422-445); 5. composed into the subject's alternate buffers
+  const int nb = L.nb;
+  NormItems it{};
+  it.count = int(k);
+  for (size_t q = 0; q < k; ++q) {
+    it.a[q] = upd[2 * q];
+    it.b[q] = upd[2 * q + 1];
+    it.partial_max[q] = L.step_parts.get() + q * 4 * size_t(nb);
+    it.partial_sq[q] = it.partial_max[q] + 2 * nb;
+  }
   {
-    const int nb = norm_blocks(long(n));
-    DevBuf parts(&ctx, k * 4 * size_t(nb));
-    NormItems it{};
-    it.count = int(k);
-    for (size_t q = 0; q < k; ++q) {
-      it.a[q] = grads[2 * q].buf.get();
-      it.b[q] = grads[2 * q + 1].buf.get();
-      it.partial_max[q] = parts.get() + q * 4 * size_t(nb);
-      it.partial_sq[q] = it.partial_max[q] + 2 * nb;
+    Timed t(ctx, "k_field_norms", 24.0 * dn * double(k));
+    launch_field_norms_items(it, long(n), inv_spacing(g), nb, st);
+  }
+  launch_step_factors_items(it, nb, recipe.gradient_step, L.factors.get(), reinterpret_cast<float*>(L.small_dev), st);
+  FieldPtrs p{};
+  p.count = int(2 * k);
+  for (size_t q = 0; q < k; ++q) {
+    Subject& u = subj[act[q]];
+    for (int side = 0; side < 2; ++side) {
+      p.in[2 * q + side] = upd[2 * q + side];
+      p.orig[2 * q + side] = (side == 0 ? *u.f2m : *u.m2m).buf.get();
+      p.out[2 * q + side] = (side == 0 ? *u.alt_f : *u.alt_m).buf.get();
     }
-    {
-      Timed t(ctx, "k_field_norms", 24.0 * dn * double(k));
-      launch_field_norms_items(it, long(n), inv_spacing(g), nb, st);
-    }
-    DevBuf factors(&ctx, 2 * k);
-    launch_step_factors_items(it, nb, recipe.gradient_step, factors.get(), maxima_dev, st);
-    // 5. compose the scaled updates into the total fields
-    std::vector<HField> composed;
-    FieldPtrs p{};
-    p.count = int(2 * k);
-    for (size_t q = 0; q < k; ++q) {
-      Subject& u = subj[act[q]];
-      for (int side = 0; side < 2; ++side) {
-        composed.push_back(HField{g, DevBuf(&ctx, 3 * n)});
-        p.in[2 * q + side] = grads[2 * q + side].buf.get();
-        p.orig[2 * q + side] = (side == 0 ? *u.f2m : *u.m2m).buf.get();
-        p.out[2 * q + side] = composed.back().buf.get();
-      }
-    }
-    float l[9];
-    put(l, g.index_from_offset);
-    {
-      Timed t(ctx, "k_compose", 36.0 * dn * double(2 * k));
-      launch_compose_set(p, d, l, factors.get(), st);
-    }
-    for (size_t q = 0; q < k; ++q) {
-      subj[act[q]].f2m.reset();
-      subj[act[q]].m2m.reset();
-    }
-    grads.clear();
-    scratch.push_back(std::move(parts));
-    scratch.push_back(std::move(factors));
-    composed = smooth_fields(b, std::move(composed), recipe.total_field_variance);
+  }
+  float l[9];
+  put(l, g.index_from_offset);
+  Timed t(ctx, "k_compose", 36.0 * dn * double(2 * k));
+  launch_compose_set(p, d, l, L.factors.get(), st);
+}
 
-    // 6. both inverses re-estimated: the new inverse from the old one, then the new field from it
-    const InvertParams inv_params;  // InvertParams::SYN
-    std::vector<DevBuf> keep;
-    std::vector<const HField*> fa;
-    std::vector<HField*> pa;
+// One lockstep iteration of the subjects `act` on the level buffers L. Fills its[q] for each.
+void iterate(HipBackend& b, const SynRecipe& recipe, const std::vector<double>& normalised,
+             const std::vector<AffineTransform>& initial,
+             const std::vector<std::vector<SmoothCache<HipBackend>::Pair>>& smooth, std::vector<Subject>& subj,
+             const std::vector<size_t>& act, LevelBuffers& L, std::vector<SynIteration>& its) {
+  DeviceContext& ctx = b.ctx();
+  hipStream_t st = ctx.stream();
+  const Grid& g = L.grid;
+  const size_t k = act.size(), nc = normalised.size();
+  const size_t n = g.num_voxels();
+  const double dn = double(n);
+  auto* bad_host = reinterpret_cast<uint32_t*>(L.small_host + 2 * k * sizeof(float));
+  for (size_t q = 0; q < k; ++q) bad_host[q] = 0;
+  // The updates: issued launch by launch, or (tuning knob batch.graphs = 1) replayed from a HIP
+  // graph captured for this active set and these buffers. Replays give the same bits; on gfx950
+  // with ROCm 7 they measured no faster at any level of the 1 mm pyramid (the host queues the
+  // launches well ahead of the device), so direct launches are the default. Never while profiling
+  // (the timing events belong to single launches) or when the total field is smoothed (that
+  // smoothing allocates).
+  const bool graphs = !ctx.profiling() && recipe.total_field_variance <= 0.0 &&
+                      ctx.tuning().get("batch.graphs", uint64_t(n), 0) != 0 &&
+                      smooth_fits(ctx, g, 2 * k, recipe.update_field_variance);
+  if (graphs) {
+    std::vector<const void*> key;
+    for (size_t q : act) {
+      const Subject& u = subj[q];
+      key.push_back(reinterpret_cast<const void*>(q));
+      for (const std::optional<HField>* f : {&u.f2m, &u.m2m, &u.f2m_inv, &u.m2m_inv, &u.alt_f, &u.alt_m})
+        key.push_back((*f)->buf.get());
+      for (const auto& pair : smooth[q]) {
+        key.push_back(pair.fixed->buf.get());
+        key.push_back(pair.moving->buf.get());
+      }
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : L.graphs)
+      if (e.first == key) exec = e.second;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        issue_updates(b, recipe, normalised, initial, smooth, subj, act, L);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      check(e, "hipGraphInstantiate");
+      L.graphs.emplace_back(std::move(key), exec);
+    }
+    check(hipGraphLaunch(exec, st), "hipGraphLaunch");
+  } else {
+    issue_updates(b, recipe, normalised, initial, smooth, subj, act, L);
+  }
+  if (recipe.total_field_variance > 0.0)
     for (size_t q = 0; q < k; ++q) {
       Subject& u = subj[act[q]];
-      fa.push_back(&composed[2 * q]);
-      pa.push_back(&*u.f2m_inv);
-      fa.push_back(&composed[2 * q + 1]);
-      pa.push_back(&*u.m2m_inv);
+      u.alt_f = b.smooth_field(std::move(*u.alt_f), recipe.total_field_variance);
+      u.alt_m = b.smooth_field(std::move(*u.alt_m), recipe.total_field_variance);
     }
-    const InvStage sa = invert_items(b, fa, pa, inv_params, reports_dev, reports_host, keep);
-    std::vector<const HField*> fb;
-    std::vector<HField*> pb;
-    for (size_t q = 0; q < k; ++q) {
-      Subject& u = subj[act[q]];
-      fb.push_back(&*u.f2m_inv);
-      pb.push_back(&composed[2 * q]);
-      fb.push_back(&*u.m2m_inv);
-      pb.push_back(&composed[2 * q + 1]);
+  // 6. both inverses re-estimated: the new inverse from the old one, then the new field from it
+  // (the composition, in its buffer)
+  auto* reports_host = reinterpret_cast<volatile InvertReport*>(L.small_host + report_offset(L.k));
+  auto* reports_dev = reinterpret_cast<InvertReport*>(L.small_dev + report_offset(L.k));
+  const InvertParams inv_params;  // InvertParams::SYN
+  std::vector<DevBuf> keep;
+  std::vector<const HField*> fa, fb;
+  std::vector<HField*> pa, pb;
+  for (size_t q = 0; q < k; ++q) {
+    Subject& u = subj[act[q]];
+    fa.push_back(&*u.alt_f);
+    pa.push_back(&*u.f2m_inv);
+    fa.push_back(&*u.alt_m);
+    pa.push_back(&*u.m2m_inv);
+    fb.push_back(&*u.f2m_inv);
+    pb.push_back(&*u.alt_f);
+    fb.push_back(&*u.m2m_inv);
+    pb.push_back(&*u.alt_m);
+  }
+  const InvStage sa = invert_items(b, fa, pa, inv_params, reports_dev, reports_host, keep);
+  const InvStage sb = invert_items(b, fb, pb, inv_params, reports_dev + 2 * k, reports_host + 2 * k, keep);
+  b.settle();
+  for (size_t q = 0; q < k; ++q) {
+    Subject& u = subj[act[q]];
+    std::swap(u.f2m, u.alt_f);
+    std::swap(u.m2m, u.alt_m);
+  }
+  const volatile float* maxima_host = reinterpret_cast<const volatile float*>(L.small_host);
+  const bool profiling = ctx.profiling();
+  for (size_t q = 0; q < k; ++q) {
+    SynIteration& it_q = its[q];
+    const double fmax = maxima_host[2 * q], mmax = maxima_host[2 * q + 1];
+    if (!(std::isfinite(fmax) && std::isfinite(mmax))) invalid("syn", "the metric gradient is not finite");
+    // stats in the order of syn_fields: F inverse, F forward, M inverse, M forward
+    const volatile InvertReport* r[4] = {&sa.reports[2 * q], &sb.reports[2 * q], &sa.reports[2 * q + 1],
+                                         &sb.reports[2 * q + 1]};
+    for (int v = 0; v < 4; ++v) {
+      it_q.inversions[v] = stats_of(*r[v]);
+      if (profiling && r[v]->executed > 0) ctx.add_bytes("k_inv_iter", dn * (60.0 * double(r[v]->executed - 1) + 36.0));
+      if (!(std::isfinite(it_q.inversions[v].mean_error_norm) && std::isfinite(it_q.inversions[v].max_error_norm)))
+        invalid("syn", "the displacement field is not finite");
     }
-    const InvStage sb = invert_items(b, fb, pb, inv_params, reports_dev + 2 * k, reports_host + 2 * k, keep);
-    for (size_t q = 0; q < k; ++q) {
-      subj[act[q]].f2m = std::move(composed[2 * q]);
-      subj[act[q]].m2m = std::move(composed[2 * q + 1]);
-    }
-    b.settle();
-    const bool profiling = ctx.profiling();
-    for (size_t q = 0; q < k; ++q) {
-      SynIteration& it_q = its[q];
-      const double fmax = maxima_host[2 * q], mmax = maxima_host[2 * q + 1];
-      if (!(std::isfinite(fmax) && std::isfinite(mmax))) invalid("syn", "the metric gradient is not finite");
-      // stats in the order of syn_fields: F inverse, F forward, M inverse, M forward
-      const volatile InvertReport* r[4] = {&sa.reports[2 * q], &sb.reports[2 * q], &sa.reports[2 * q + 1],
-                                           &sb.reports[2 * q + 1]};
-      for (int v = 0; v < 4; ++v) {
-        it_q.inversions[v] = stats_of(*r[v]);
-        if (profiling && r[v]->executed > 0)
-          ctx.add_bytes("k_inv_iter", dn * (60.0 * double(r[v]->executed - 1) + 36.0));
-        if (!(std::isfinite(it_q.inversions[v].mean_error_norm) && std::isfinite(it_q.inversions[v].max_error_norm)))
-          invalid("syn", "the displacement field is not finite");
+    if (nc > 1 && bad_host[q]) invalid("syn", "the metric gradient is not finite");
+    // the channels' values, each from its block partials in block order
+    const float* pcc = reinterpret_cast<const float*>(L.cc_host);
+    const int* pcount = reinterpret_cast<const int*>(L.cc_host + L.k * nc * L.blocks * sizeof(float));
+    double weighted[2] = {0.0, 0.0};
+    double first = 0.0;
+    for (size_t c = 0; c < nc; ++c) {
+      double total = 0.0;
+      long long count = 0;
+      for (size_t bl = 0; bl < L.blocks; ++bl) {
+        total += double(pcc[(q * nc + c) * L.blocks + bl]);
+        count += pcount[(q * nc + c) * L.blocks + bl];
       }
-      if (nc > 1 && bad_host[q]) invalid("syn", "the metric gradient is not finite");
-      // the channels' values, each from its block partials in block order
-      const float* pcc = reinterpret_cast<const float*>(cc_host);
-      const int* pcount = reinterpret_cast<const int*>(cc_host + k * nc * blocks * sizeof(float));
-      double weighted[2] = {0.0, 0.0};
-      double first = 0.0;
-      for (size_t c = 0; c < nc; ++c) {
-        double total = 0.0;
-        long long count = 0;
-        for (size_t bl = 0; bl < blocks; ++bl) {
-          total += double(pcc[(q * nc + c) * blocks + bl]);
-          count += pcount[(q * nc + c) * blocks + bl];
-        }
-        if (count == 0) invalid("cc metric", "no virtual voxel maps inside both image buffers");
-        const double v = -total / double(count);
-        if (c == 0) first = v;
-        weighted[0] += nc == 1 ? v : v * normalised[c];
-        weighted[1] += nc == 1 ? v : v * normalised[c];
-      }
-      double scales[2] = {recipe.gradient_step, recipe.gradient_step};
-      if (fmax > 0.0) scales[0] /= fmax;
-      if (mmax > 0.0) scales[1] /= mmax;
-      it_q.energy = 0.5 * (first + first);
-      it_q.energy_weighted = 0.5 * (weighted[0] + weighted[1]);
-      it_q.fixed_update_scale = scales[0];
-      it_q.moving_update_scale = scales[1];
+      if (count == 0) invalid("cc metric", "no virtual voxel maps inside both image buffers");
+      const double v = -total / double(count);
+      if (c == 0) first = v;
+      weighted[0] += nc == 1 ? v : v * normalised[c];
+      weighted[1] += nc == 1 ? v : v * normalised[c];
     }
+    double scales[2] = {recipe.gradient_step, recipe.gradient_step};
+    if (fmax > 0.0) scales[0] /= fmax;
+    if (mmax > 0.0) scales[1] /= mmax;
+    it_q.energy = 0.5 * (first + first);
+    it_q.energy_weighted = 0.5 * (weighted[0] + weighted[1]);
+    it_q.fixed_update_scale = scales[0];
+    it_q.moving_update_scale = scales[1];
   }
 }
 
@@ -502,7 +672,12 @@ std::vector<SynFields> syn_fields_batch(HipBackend& b, const SynRecipe& recipe, 
       u.trace.grid_size = grid.size;
       u.current = 0;
       u.active = recipe.iterations[level] > 0;
+      if (u.active) {
+        u.alt_f = HField{grid, DevBuf(&b.ctx(), 3 * grid.num_voxels())};
+        u.alt_m = HField{grid, DevBuf(&b.ctx(), 3 * grid.num_voxels())};
+      }
     }
+    std::unique_ptr<LevelBuffers> bufs;
     for (;;) {
       std::vector<size_t> act;
       for (size_t s = 0; s < ns; ++s)
@@ -511,7 +686,8 @@ std::vector<SynFields> syn_fields_batch(HipBackend& b, const SynRecipe& recipe, 
       check_cancel(cancel);
       const auto t_it = clock_type::now();
       std::vector<SynIteration> its(act.size());
-      iterate(b, recipe, normalised, initial, smooth, subj, act, grid, its);
+      if (!bufs) bufs = make_level_buffers(b, recipe, act.size(), nc, grid);
+      iterate(b, recipe, normalised, initial, smooth, subj, act, *bufs, its);
       const double secs = std::chrono::duration<double>(clock_type::now() - t_it).count();
       for (size_t q = 0; q < act.size(); ++q) {
         Subject& u = subj[act[q]];
@@ -525,8 +701,11 @@ std::vector<SynFields> syn_fields_batch(HipBackend& b, const SynRecipe& recipe, 
         if (u.trace.converged || u.current >= recipe.iterations[level]) u.active = false;
       }
     }
+    bufs.reset();
     const double level_secs = std::chrono::duration<double>(clock_type::now() - t0).count();
     for (size_t s = 0; s < ns; ++s) {
+      subj[s].alt_f.reset();
+      subj[s].alt_m.reset();
       subj[s].trace.seconds = level_secs;
       out[s].levels.push_back(std::move(subj[s].trace));
     }

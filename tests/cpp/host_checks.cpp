@@ -155,6 +155,7 @@ static void tuning() {
   CHECK(t.get("cc.tile_z", 1024, 16) == 16 && t.size() == 1, "removed");
   CHECK(TuningTable::valid("mi.threads", 512) && !TuningTable::valid("mi.threads", 384), "mi threads");
   CHECK(TuningTable::valid("gauss.one_pass", 1) && !TuningTable::valid("gauss.one_pass", 2), "one pass");
+  CHECK(TuningTable::valid("batch.graphs", 0) && !TuningTable::valid("batch.graphs", 2), "graphs");
   CHECK(!TuningTable::valid("cc.tile_z", 0) && !TuningTable::valid("nonsense", 1), "unknown knob");
   const char* path = "/tmp/fw_tuning_check.json";
   {

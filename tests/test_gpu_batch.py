@@ -114,3 +114,32 @@ def test_linear_fits_side_by_side_give_the_bits_of_their_single_runs(backend):
         assert np.array_equal(affine_array(results[i].affine), affine_array(alone.affine)), label
         assert np.array_equal(np.array(results[i].linear_iterations), np.array(alone.result.linear_iterations)), label
         assert traces[i] == steps(last_trace()), label
+
+
+def test_graph_replays_give_the_bits_of_direct_launches(backend):
+    """The batched SyN iterations replay a HIP graph of each iteration's update launches (captured
+    once per active set and buffer parity); with graphs switched off (the tuning knob
+    batch.graphs = 0) the same launches are issued one by one. Three subjects (whose convergence
+    may shrink the active set within a level): the same traces and fields."""
+    grid, fixed, moving = cohort(1, 3)
+    plan = plan_2mm(1)
+    plan.set_linear([30, 15, 8], [4, 2, 1], [2.0, 1.0, 0.0])
+    up = lambda vols: [backend.upload_image(grid, v) for v in vols]
+    tmpl = up(fixed)
+    subj = [up(m) for m in moving]
+    runs = []
+    for graphs in (1, 0):
+        backend.set_tuning("batch.graphs", graphs)
+        try:
+            batch = register_batch(backend, plan, tmpl, subj)
+        finally:
+            backend.set_tuning("batch.graphs", -1)
+        runs.append(([steps(last_batch_trace(s)) for s in range(3)],
+                     [[backend.download_field(getattr(r, name)) for name in ("forward", "inverse")] for r in batch]))
+    (t_graph, f_graph), (t_direct, f_direct) = runs
+    # SyN iterations per level of each subject (stage 2), for the messages
+    iterations = [[sum(1 for t in tr if t[0] == 2 and t[1] == lv) for lv in range(plan.syn_levels)] for tr in t_graph]
+    for s in range(3):
+        assert t_graph[s] == t_direct[s], (s, iterations)
+        for a, b in zip(f_graph[s], f_direct[s]):
+            assert np.array_equal(a, b), (s, iterations)
