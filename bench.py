@@ -172,7 +172,8 @@ def batched(backend, subjects: int) -> dict:
     linear_s = sum(r.result.seconds_linear for r in rs)
     free(rs)
     # the SyN stage's kernels (on this context; the linear chains run on the workers' own streams),
-    # every launch timed, in one more run
+    # every launch timed, in one more run (no profiling focus left from an earlier measurement)
+    backend.set_profiling_focus(None)
     backend.set_profiling(True)
     free(register_batch(backend, plan, tmpl, subj))
     prof = {k: v for k, v in backend.take_profile().items() if not k.startswith("host:")}
