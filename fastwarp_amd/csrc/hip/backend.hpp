@@ -110,7 +110,7 @@ class DeviceContext : public HostSpanSink {
     void* p = nullptr;
     size_t bytes = 0;
   };
-  Scratch scratch_[8];
+  Scratch scratch_[10];
   void* pinned_ = nullptr;
   struct StageChunk {
     char* p = nullptr;
@@ -364,9 +364,8 @@ class MiLevel {
 
  private:
   void batch_values(const LinearMovingTransform* ts, size_t k, std::vector<double>& out);
-  double histogram(const LinearMovingTransform& t, MiSampledArgs& args, std::vector<double>* ratio,
-                   std::vector<double>* joint = nullptr, std::vector<double>* fixed_marginal = nullptr,
-                   uint64_t* valid_out = nullptr);
+  int pass_args(const LinearMovingTransform& t, MiSampledArgs& args, size_t histograms);
+  volatile MiFinishOut* launch_finish(const unsigned long long* slots, int blocks, size_t k, float* ratio, uint32_t& tag);
   DeviceContext* ctx_;
   const HImage* moving_;
   DevBuf samples_;

@@ -71,8 +71,8 @@ MiLevel::MiLevel(HipBackend& backend, const HImage& moving, const HostVec<Vec3>&
   ctx_->sync();
 }
 
-double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std::vector<double>* ratio,
-                          std::vector<double>* joint, std::vector<double>* fixed_marginal, uint64_t* valid_out) {
+// The arguments of one histogram pass of transform t; returns the pass's block count.
+int MiLevel::pass_args(const LinearMovingTransform& t, MiSampledArgs& a, size_t histograms) {
   const Grid& mg = moving_->grid;
   const AffineTransform T = t.collapsed();
   const Mat3 A = mat3_mul(mg.index_from_offset, T.matrix);
@@ -88,12 +88,51 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
   a.moving_min = float(binning_.moving_min);
   a.moving_max = float(binning_.moving_max);
   a.bins = int(binning_.bins);
-  const size_t bins = binning_.bins;
-  const size_t cells = bins * bins + bins + 1;
+  const size_t cells = binning_.bins * binning_.bins + binning_.bins + 1;
   const int blocks = mi_sampled_blocks(int(count_), int(ctx_->tuning().get("mi.max_blocks", count_, 512)));
   a.threads = int(ctx_->tuning().get("mi.threads", count_, 0));
-  a.hist = static_cast<unsigned long long*>(ctx_->scratch(0, cells * sizeof(unsigned long long)));
-  a.hist_slots = static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * cells * sizeof(unsigned long long)));
+  a.hist_slots =
+      static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * histograms * cells * sizeof(unsigned long long)));
+  return blocks;
+}
+
+// Adds the slots of `k` histograms into device memory and finishes them there (k_mi_finish);
+// the outputs are in coherent host memory once done words [0, k) hold `tag`.
+volatile MiFinishOut* MiLevel::launch_finish(const unsigned long long* slots, int blocks, size_t k, float* ratio,
+                                             uint32_t& tag) {
+  const size_t bins = binning_.bins;
+  const size_t all = k * (bins * bins + bins + 1);
+  auto* dev_hist = static_cast<unsigned long long*>(ctx_->scratch(8, all * sizeof(unsigned long long)));
+  hipStream_t s = ctx_->stream();
+  {
+    Timed tm(*ctx_, "k_mi_sum_slots", 8.0 * double(all) * double(blocks + 1));
+    launch_mi_sum_slots(slots, blocks, int(all), dev_hist, ctx_->done_words_device(), ctx_->next_tag(), s);
+  }
+  void* dev_out = nullptr;
+  auto* out = static_cast<volatile MiFinishOut*>(ctx_->mapped(5, k * sizeof(MiFinishOut), &dev_out));
+  tag = ctx_->next_tag();
+  Timed tm(*ctx_, "k_mi_finish", 8.0 * double(all) + (ratio ? 4.0 * double(bins * bins) : 0.0));
+  launch_mi_finish(dev_hist, int(k), int(bins), binning_.moving_bin_size, static_cast<MiFinishOut*>(dev_out), ratio,
+                   ctx_->done_words_device(), tag, s);
+  return out;
+}
+
+namespace {
+
+double finished_value(const volatile MiFinishOut& o) {
+  if (o.status == 1) invalid("mattes", "all samples map outside the moving image buffer");
+  if (o.status == 2) invalid("mattes", "joint PDF summed to zero");
+  return o.value;
+}
+
+}  // namespace
+
+void MiLevel::histogram_cells(const LinearMovingTransform& t, std::vector<double>& joint, std::vector<double>& fixed_marginal,
+                              uint64_t& valid) {
+  MiSampledArgs a;
+  const int blocks = pass_args(t, a, 1);
+  const size_t bins = binning_.bins;
+  const size_t cells = bins * bins + bins + 1;
   hipStream_t s = ctx_->stream();
   {
     Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
@@ -109,28 +148,22 @@ double MiLevel::histogram(const LinearMovingTransform& t, MiSampledArgs& a, std:
                         ctx_->done_words_device(), tag, s);
   }
   ctx_->wait_done(mi_sum_slots_blocks(int(cells)), tag);
-  const uint64_t valid = host[cells - 1];
-  if (joint) *joint = joint_from(host, bins);
-  if (fixed_marginal) *fixed_marginal = marginal_from(host, bins);
-  if (valid_out) {
-    *valid_out = valid;
-    return 0.0;
-  }
-  HistogramFinish fin =
-      finish_histogram(bins, binning_.moving_bin_size, valid, joint_from(host, bins), marginal_from(host, bins), ratio != nullptr);
-  if (ratio) *ratio = std::move(fin.ratio);
-  return fin.value;
-}
-
-void MiLevel::histogram_cells(const LinearMovingTransform& t, std::vector<double>& joint, std::vector<double>& fixed_marginal,
-                              uint64_t& valid) {
-  MiSampledArgs a;
-  histogram(t, a, nullptr, &joint, &fixed_marginal, &valid);
+  valid = host[cells - 1];
+  joint = joint_from(host, bins);
+  fixed_marginal = marginal_from(host, bins);
 }
 
 double MiLevel::value(const LinearMovingTransform& t) {
   MiSampledArgs a;
-  return histogram(t, a, nullptr);
+  const int blocks = pass_args(t, a, 1);
+  {
+    Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
+    launch_mi_sampled_hist(a, blocks, ctx_->stream());
+  }
+  uint32_t tag = 0;
+  volatile MiFinishOut* out = launch_finish(a.hist_slots, blocks, 1, nullptr, tag);
+  ctx_->wait_done(1, tag);
+  return finished_value(out[0]);
 }
 
 std::vector<double> MiLevel::values(const std::vector<LinearMovingTransform>& ts) {
@@ -146,10 +179,11 @@ std::vector<double> MiLevel::values(const std::vector<LinearMovingTransform>& ts
   return out;
 }
 
-// Up to kMiMaxBatch transforms in one histogram pass and one slot sum.
+// Up to kMiMaxBatch transforms in one histogram pass, one slot sum and one finish.
 void MiLevel::batch_values(const LinearMovingTransform* ts, size_t k, std::vector<double>& out) {
   const Grid& mg = moving_->grid;
   MiSampledArgs a{};
+  const int blocks = pass_args(ts[0], a, k);
   MiBatch batch{};
   for (size_t q = 0; q < k; ++q) {
     const AffineTransform T = ts[q].collapsed();
@@ -158,51 +192,30 @@ void MiLevel::batch_values(const LinearMovingTransform* ts, size_t k, std::vecto
     put(batch.a[q], A);
     put(batch.b[q], b);
   }
-  a.samples = reinterpret_cast<const float4*>(samples_.get());
-  a.count = int(count_);
-  a.moving = moving_->buf.get();
-  a.md = dims_of(mg);
-  a.inv_bin_size = float(1.0 / binning_.moving_bin_size);
-  a.moving_min = float(binning_.moving_min);
-  a.moving_max = float(binning_.moving_max);
-  a.bins = int(binning_.bins);
-  const size_t bins = binning_.bins;
-  const size_t cells = bins * bins + bins + 1;
-  const size_t all = k * cells;
-  const int blocks = mi_sampled_blocks(int(count_), int(ctx_->tuning().get("mi.max_blocks", count_, 512)));
-  a.threads = int(ctx_->tuning().get("mi.threads", count_, 0));
-  a.hist_slots = static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * all * sizeof(unsigned long long)));
-  hipStream_t s = ctx_->stream();
   {
     Timed tm(*ctx_, "k_mi_sampled_hist_batch", (16.0 + 4.0 * double(k)) * double(count_));
-    launch_mi_sampled_hist_batch(a, batch, int(k), blocks, s);
+    launch_mi_sampled_hist_batch(a, batch, int(k), blocks, ctx_->stream());
   }
-  void* dev_hist = nullptr;
-  auto* host = static_cast<unsigned long long*>(ctx_->mapped(0, all * sizeof(unsigned long long), &dev_hist));
-  const uint32_t tag = ctx_->next_tag();
-  {
-    Timed tm(*ctx_, "k_mi_sum_slots", 8.0 * double(all) * double(blocks + 1));
-    launch_mi_sum_slots(a.hist_slots, blocks, int(all), static_cast<unsigned long long*>(dev_hist), ctx_->done_words_device(),
-                        tag, s);
-  }
-  ctx_->wait_done(mi_sum_slots_blocks(int(all)), tag);
-  for (size_t q = 0; q < k; ++q) {
-    const unsigned long long* h = host + q * cells;
-    out.push_back(finish_histogram(bins, binning_.moving_bin_size, h[cells - 1], joint_from(h, bins), marginal_from(h, bins), false)
-                      .value);
-  }
+  uint32_t tag = 0;
+  volatile MiFinishOut* fin = launch_finish(a.hist_slots, blocks, k, nullptr, tag);
+  ctx_->wait_done(int(k), tag);
+  for (size_t q = 0; q < k; ++q) out.push_back(finished_value(fin[q]));
 }
 
 std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const LinearMovingTransform& t) {
   MiSampledArgs a;
-  std::vector<double> ratio;
-  const double value = histogram(t, a, &ratio);
+  const int hist_blocks = pass_args(t, a, 1);
   const size_t bins = binning_.bins;
-  auto* rf = static_cast<float*>(ctx_->pinned(ratio.size() * sizeof(float)));
-  for (size_t i = 0; i < ratio.size(); ++i) rf[i] = float(ratio[i]);
-  auto* dev_ratio = static_cast<float*>(ctx_->scratch(1, ratio.size() * sizeof(float)));
   hipStream_t s = ctx_->stream();
-  check(hipMemcpyAsync(dev_ratio, rf, ratio.size() * sizeof(float), hipMemcpyHostToDevice, s), "upload ratio");
+  {
+    Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
+    launch_mi_sampled_hist(a, hist_blocks, s);
+  }
+  // the finish leaves the ratio weights on the device, where the derivative pass reads them: one
+  // wait for the value and the derivative together
+  auto* dev_ratio = static_cast<float*>(ctx_->scratch(1, bins * bins * sizeof(float)));
+  uint32_t finish_tag = 0;
+  volatile MiFinishOut* fin = launch_finish(a.hist_slots, hist_blocks, 1, dev_ratio, finish_tag);
   a.ratio = dev_ratio;
   a.probes = probes_of(moving_->grid);
   put(a.outer, t.outer.matrix);
@@ -225,7 +238,7 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
     launch_mi_sampled_deriv(a, blocks, s);
   }
   // the blocks' partials are added on the device, straight into coherent host memory; the
-  // completion word says when the sums are there
+  // completion word says when the sums (and, before them, the finish) are there
   void* dev_sum = nullptr;
   auto* host = static_cast<double*>(ctx_->mapped(1, 12 * sizeof(double), &dev_sum));
   const uint32_t tag = ctx_->next_tag();
@@ -234,8 +247,9 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
     launch_mi_deriv_finish(a.partial, blocks, int(np), static_cast<double*>(dev_sum), ctx_->done_words_device(), tag, s);
   }
   ctx_->wait_done(1, tag);
+  (void)finish_tag;
+  const double value = finished_value(fin[0]);
   std::vector<double> d(host, host + np);
-  (void)bins;
   return {value, d};
 }
 

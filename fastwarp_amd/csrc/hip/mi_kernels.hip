@@ -335,6 +335,99 @@ void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells,
   launch(k_mi_sum_slots, dim3(mi_sum_slots_blocks(cells)), dim3(1024), 0, s, slots, nslots, cells, hist, done, tag);
 }
 
+// The Mattes finish (mi.rs:702-769) of histogram `blockIdx.x` on the device: hist holds the
+// histograms one after another, each bins*bins joint cells (2^-32 fixed point), the bins fixed
+// marginal counts and the valid count. The joint total is the exact integer sum of the cells,
+// rounded once; the cells and marginals are scaled by its inverse; the moving marginal adds each
+// column in row order; the value adds jv (log(jv / p_M) - log(p_F)) over the cells whose three
+// terms exceed the f64 epsilon (per-thread sums, then the waves' sums in order). ratio (block 0,
+// if given) receives n_factor log(jv / p_M) per cell in f32, zero elsewhere, as the derivative
+// pass reads it. out[b] gets the value, the valid count and a status (1: no valid sample, 2: the
+// joint cells sum to zero; the ratio is then all zero); then tag goes to done[b] with a
+// system-scope release.
+__global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __restrict__ hist, int bins,
+                                                    double moving_bin_size, MiFinishOut* out, float* __restrict__ ratio,
+                                                    uint32_t* done, uint32_t tag) {
+  __shared__ double jn[kMiMaxBins * kMiMaxBins];
+  __shared__ double mm[kMiMaxBins], fm[kMiMaxBins];
+  __shared__ unsigned long long wsum_u[16];
+  __shared__ double wsum_d[16];
+  __shared__ int status_s;
+  const double eps = 2.220446049250313e-16;
+  const int cells = bins * bins;
+  const int b = blockIdx.x;
+  const unsigned long long* h = hist + (size_t)b * (cells + bins + 1);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = (int)(blockDim.x + 63) >> 6;
+  const unsigned long long valid = h[cells + bins];
+  float* rb = b == 0 ? ratio : nullptr;
+  // the exact total of the joint cells
+  unsigned long long t = 0ull;
+  for (int c = tid; c < cells; c += blockDim.x) t += h[c];
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  if (lane == 0) wsum_u[w] = t;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long all = 0ull;
+    for (int q = 0; q < nw; ++q) all += wsum_u[q];
+    const double sum = (double)all / kMiFixedPoint;
+    status_s = valid == 0ull ? 1 : (sum < eps ? 2 : 0);
+    wsum_d[0] = status_s == 0 ? 1.0 / sum : 0.0;
+  }
+  __syncthreads();
+  const int status = status_s;
+  const double norm = wsum_d[0];
+  if (status != 0) {
+    if (rb)
+      for (int c = tid; c < cells; c += blockDim.x) rb[c] = 0.0f;
+    if (tid == 0) {
+      out[b].value = 0.0;
+      out[b].valid = valid;
+      out[b].status = status;
+      __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return;
+  }
+  for (int c = tid; c < cells; c += blockDim.x) jn[c] = ((double)h[c] / kMiFixedPoint) * norm;
+  if (tid < bins) fm[tid] = (double)h[cells + tid] * norm;
+  __syncthreads();
+  if (tid < bins) {
+    double s = 0.0;
+    for (int r = 0; r < bins; ++r) s += jn[r * bins + tid];
+    mm[tid] = s;
+  }
+  __syncthreads();
+  const double n_factor = -1.0 / (moving_bin_size * (double)valid);
+  double total = 0.0;
+  for (int c = tid; c < cells; c += blockDim.x) {
+    const int f = c / bins, m = c - f * bins;
+    const double jv = jn[c];
+    float r = 0.0f;
+    if (fm[f] > eps && mm[m] > eps && jv > eps) {
+      const double pr = log(jv / mm[m]);
+      total += jv * (pr - log(fm[f]));
+      r = (float)(n_factor * pr);
+    }
+    if (rb) rb[c] = r;
+  }
+  for (int off = 32; off > 0; off >>= 1) total += __shfl_down(total, off, 64);
+  __syncthreads();
+  if (lane == 0) wsum_d[w] = total;
+  __syncthreads();
+  if (tid == 0) {
+    double all = 0.0;
+    for (int q = 0; q < nw; ++q) all += wsum_d[q];
+    out[b].value = -all;
+    out[b].valid = valid;
+    out[b].status = 0;
+    __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+void launch_mi_finish(const unsigned long long* hist, int count, int bins, double moving_bin_size, MiFinishOut* out,
+                      float* ratio, uint32_t* done, uint32_t tag, hipStream_t s) {
+  launch(k_mi_finish, dim3(count), dim3(1024), 0, s, hist, bins, moving_bin_size, out, ratio, done, tag);
+}
+
 // Per sample: sum_k B3'(k - term) * ratio[f, k] * (grad M . dT/dtheta_p) for every parameter
 // (mi.rs:405-433 with the ratio weights of :752-757), accumulated in FP64.
 __global__ __launch_bounds__(256) void k_mi_sampled_deriv(MiSampledArgs a) {

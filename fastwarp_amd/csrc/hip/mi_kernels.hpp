@@ -54,6 +54,19 @@ void launch_mi_sampled_hist_batch(const MiSampledArgs& a, const MiBatch& batch, 
 int mi_sum_slots_blocks(int cells);
 void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, uint32_t* done,
                          uint32_t tag, hipStream_t s);
+// The Mattes finish of `count` histograms laid out as k_mi_sum_slots writes them (each bins*bins
+// joint cells, bins fixed marginal counts, the valid count), on the device: out[k] receives the
+// value of histogram k, its valid count and a status (0 ok, 1 no valid sample, 2 the joint cells
+// sum to zero); `ratio` (if not null) the derivative pass's f32 weights of histogram 0; then done[k]
+// = tag (system-scope release), k < count.
+struct MiFinishOut {
+  double value;
+  unsigned long long valid;
+  int32_t status;
+  int32_t pad;
+};
+void launch_mi_finish(const unsigned long long* hist, int count, int bins, double moving_bin_size, MiFinishOut* out,
+                      float* ratio, uint32_t* done, uint32_t tag, hipStream_t s);
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
 int mi_sampled_blocks(int count, int cap = 512);
 int mi_sampled_threads(int count);

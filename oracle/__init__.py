@@ -170,6 +170,17 @@ def mi_sampled(binning: MiBinning, moving_grid: Grid, moving, points, fixed_bin,
     return {"value": value.value, "derivative": d, "num_valid": nv.value, "joint_pdf": pdf}
 
 
+def mi_finish(bins: int, moving_bin_size: float, num_valid: int, joint, fixed_marginal) -> tuple[float, np.ndarray]:
+    """finish_histogram (metric/mi.rs:702-769) on given cells: (value, ratio weights)."""
+    j = np.ascontiguousarray(joint, dtype=np.float64).reshape(-1)
+    f = np.ascontiguousarray(fixed_marginal, dtype=np.float64).reshape(-1)
+    value = C.c_double()
+    ratio = np.zeros(bins * bins)
+    _check(lib().fwo_mi_finish(C.c_int64(bins), C.c_double(moving_bin_size), C.c_int64(num_valid), dptr(j), dptr(f),
+                               C.byref(value), dptr(ratio)))
+    return value.value, ratio
+
+
 def gaussian_taps(variance: float, max_error: float = 0.001, max_width: int = 1000) -> np.ndarray:
     out = np.zeros(4096)
     n = C.c_int64()

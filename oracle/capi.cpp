@@ -214,6 +214,19 @@ int fwo_mi_dense(const fw_grid* fixed_grid, const double* fixed, const fw_grid* 
 }
 
 // Sampled (linear-stage) Mattes MI with prepared fixed bins.
+// mi.rs:702-769 on given cells: the value and (if ratio is not null) the ratio weights.
+int fwo_mi_finish(int64_t bins, double moving_bin_size, int64_t num_valid, const double* joint, const double* fixed_marginal,
+                  double* value, double* ratio) {
+  return guard([&] {
+    const size_t nb = size_t(bins);
+    HistogramResult r = fw::oracle::finish_histogram(nb, moving_bin_size, size_t(num_valid), std::vector<double>(joint, joint + nb * nb),
+                                         std::vector<double>(fixed_marginal, fixed_marginal + nb), ratio != nullptr);
+    *value = r.value;
+    if (ratio)
+      for (size_t i = 0; i < nb * nb; ++i) ratio[i] = r.ratio[i];
+  });
+}
+
 int fwo_mi_sampled(const fw_mi_binning* b, const fw_grid* moving_grid, const double* moving, const double* points,
                    const uint32_t* fixed_bin, int64_t m, const fw_linear_transform* t, int want_derivative, double* value,
                    double* derivative, int64_t* num_valid, double* joint_pdf) {

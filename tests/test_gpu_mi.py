@@ -90,6 +90,29 @@ def test_batched_values_equal_single_values(backend, bins):
         level.values([ts[0], away, ts[1]])
 
 
+@pytest.mark.parametrize("bins", [32, 12, 64])
+def test_device_finish_matches_the_f64_finish_of_the_same_cells(backend, bins):
+    """The linear stage finishes its histograms on the device (k_mi_finish). On the integer cells
+    of the same pass, the CPU engine's finish (mi.rs:702-769, f64) gives the same value to the
+    rounding of another summation order (1e-12 relative); the value with the derivative, whose pass
+    reads the device's ratio weights, is the same value."""
+    g = cases.grid((40, 36, 30), (1.2, 1.0, 1.4), (-20.0, -15.0, -18.0), oblique=0.3)
+    f = smooth_vol(g, (0, 0, 0))
+    m = smooth_vol(g, (1.1, -0.8, 0.6), 0.3)
+    rng = np.random.default_rng(5)
+    pts = np.array([g.index_to_point(p) for p in rng.uniform([4, 4, 4], np.array(g.shape_xyz) - 5, size=(12000, 3))])
+    b = MiBinning.from_ranges(bins, (f.min(), f.max()), (m.min(), m.max()))
+    level = MiLevel(backend, backend.upload_image(g, m), pts, fixed_bins(g, f, pts, b), b)
+    centre = tuple(g.index_to_point(np.array(g.shape_xyz) / 2))
+    for q in range(4):
+        t = LinearTransform.make(0, np.concatenate([rng.normal(0, 0.04, 3), rng.normal(0, 1.5, 3)]), centre)
+        joint, marginal, valid = level.histogram(t, bins)
+        want, _ = oracle.mi_finish(bins, b.moving_bin_size, valid, joint, marginal)
+        got = level.value(t)
+        assert abs(got - want) <= 1e-12 * abs(want), (q, got, want)
+        assert level.value_and_derivative(t)[0] == got, q
+
+
 def test_sampled_mi_is_deterministic_and_errors_without_overlap(backend):
     g = cases.grid((24, 24, 24))
     f = smooth_vol(g, (0, 0, 0))
