@@ -365,7 +365,9 @@ class MiLevel {
  private:
   void batch_values(const LinearMovingTransform* ts, size_t k, std::vector<double>& out);
   int pass_args(const LinearMovingTransform& t, MiSampledArgs& args, size_t histograms);
-  volatile MiFinishOut* launch_finish(const unsigned long long* slots, int blocks, size_t k, float* ratio, uint32_t& tag);
+  void launch_finish(const unsigned long long* slots, int blocks, size_t k, float* ratio, MiFinishOut* out, bool signal,
+                     uint32_t tag);
+  volatile MiFinishOut* finish_slots(size_t k, MiFinishOut** dev);
   DeviceContext* ctx_;
   const HImage* moving_;
   DevBuf samples_;

@@ -96,25 +96,29 @@ int MiLevel::pass_args(const LinearMovingTransform& t, MiSampledArgs& a, size_t 
   return blocks;
 }
 
-// Adds the slots of `k` histograms into device memory and finishes them there (k_mi_finish);
-// the outputs are in coherent host memory once done words [0, k) hold `tag`.
-volatile MiFinishOut* MiLevel::launch_finish(const unsigned long long* slots, int blocks, size_t k, float* ratio,
-                                             uint32_t& tag) {
+// Adds the slots of `k` histograms into device memory and finishes them there (k_mi_finish)
+// into `out` (a device view); with `signal` the finish then stores `tag` into done words [0, k).
+void MiLevel::launch_finish(const unsigned long long* slots, int blocks, size_t k, float* ratio, MiFinishOut* out,
+                            bool signal, uint32_t tag) {
   const size_t bins = binning_.bins;
   const size_t all = k * (bins * bins + bins + 1);
   auto* dev_hist = static_cast<unsigned long long*>(ctx_->scratch(8, all * sizeof(unsigned long long)));
   hipStream_t s = ctx_->stream();
   {
     Timed tm(*ctx_, "k_mi_sum_slots", 8.0 * double(all) * double(blocks + 1));
-    launch_mi_sum_slots(slots, blocks, int(all), dev_hist, ctx_->done_words_device(), ctx_->next_tag(), s);
+    launch_mi_sum_slots(slots, blocks, int(all), dev_hist, nullptr, 0, s);
   }
-  void* dev_out = nullptr;
-  auto* out = static_cast<volatile MiFinishOut*>(ctx_->mapped(5, k * sizeof(MiFinishOut), &dev_out));
-  tag = ctx_->next_tag();
   Timed tm(*ctx_, "k_mi_finish", 8.0 * double(all) + (ratio ? 4.0 * double(bins * bins) : 0.0));
-  launch_mi_finish(dev_hist, int(k), int(bins), binning_.moving_bin_size, static_cast<MiFinishOut*>(dev_out), ratio,
-                   ctx_->done_words_device(), tag, s);
-  return out;
+  launch_mi_finish(dev_hist, int(k), int(bins), binning_.moving_bin_size, out, ratio,
+                   signal ? ctx_->done_words_device() : nullptr, tag, s);
+}
+
+// The finish of `k` histograms in coherent host memory: host view, with the device view in *dev.
+volatile MiFinishOut* MiLevel::finish_slots(size_t k, MiFinishOut** dev) {
+  void* d = nullptr;
+  auto* host = static_cast<volatile MiFinishOut*>(ctx_->mapped(5, k * sizeof(MiFinishOut), &d));
+  *dev = static_cast<MiFinishOut*>(d);
+  return host;
 }
 
 namespace {
@@ -160,8 +164,10 @@ double MiLevel::value(const LinearMovingTransform& t) {
     Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
     launch_mi_sampled_hist(a, blocks, ctx_->stream());
   }
-  uint32_t tag = 0;
-  volatile MiFinishOut* out = launch_finish(a.hist_slots, blocks, 1, nullptr, tag);
+  MiFinishOut* dev = nullptr;
+  volatile MiFinishOut* out = finish_slots(1, &dev);
+  const uint32_t tag = ctx_->next_tag();
+  launch_finish(a.hist_slots, blocks, 1, nullptr, dev, true, tag);
   ctx_->wait_done(1, tag);
   return finished_value(out[0]);
 }
@@ -196,8 +202,10 @@ void MiLevel::batch_values(const LinearMovingTransform* ts, size_t k, std::vecto
     Timed tm(*ctx_, "k_mi_sampled_hist_batch", (16.0 + 4.0 * double(k)) * double(count_));
     launch_mi_sampled_hist_batch(a, batch, int(k), blocks, ctx_->stream());
   }
-  uint32_t tag = 0;
-  volatile MiFinishOut* fin = launch_finish(a.hist_slots, blocks, k, nullptr, tag);
+  MiFinishOut* dev = nullptr;
+  volatile MiFinishOut* fin = finish_slots(k, &dev);
+  const uint32_t tag = ctx_->next_tag();
+  launch_finish(a.hist_slots, blocks, k, nullptr, dev, true, tag);
   ctx_->wait_done(int(k), tag);
   for (size_t q = 0; q < k; ++q) out.push_back(finished_value(fin[q]));
 }
@@ -211,11 +219,12 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
     Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
     launch_mi_sampled_hist(a, hist_blocks, s);
   }
-  // the finish leaves the ratio weights on the device, where the derivative pass reads them: one
-  // wait for the value and the derivative together
+  // the finish leaves the ratio weights and its result on the device, where the derivative pass
+  // reads the weights and its last kernel passes the result to the host with the derivative: one
+  // wait for both
   auto* dev_ratio = static_cast<float*>(ctx_->scratch(1, bins * bins * sizeof(float)));
-  uint32_t finish_tag = 0;
-  volatile MiFinishOut* fin = launch_finish(a.hist_slots, hist_blocks, 1, dev_ratio, finish_tag);
+  auto* dev_fin = static_cast<MiFinishOut*>(ctx_->scratch(9, sizeof(MiFinishOut)));
+  launch_finish(a.hist_slots, hist_blocks, 1, dev_ratio, dev_fin, false, 0);
   a.ratio = dev_ratio;
   a.probes = probes_of(moving_->grid);
   put(a.outer, t.outer.matrix);
@@ -241,13 +250,15 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
   // completion word says when the sums (and, before them, the finish) are there
   void* dev_sum = nullptr;
   auto* host = static_cast<double*>(ctx_->mapped(1, 12 * sizeof(double), &dev_sum));
+  MiFinishOut* fin_dev = nullptr;
+  volatile MiFinishOut* fin = finish_slots(1, &fin_dev);
   const uint32_t tag = ctx_->next_tag();
   {
     Timed tm(*ctx_, "k_mi_deriv_finish", 8.0 * 12.0 * double(blocks));
-    launch_mi_deriv_finish(a.partial, blocks, int(np), static_cast<double*>(dev_sum), ctx_->done_words_device(), tag, s);
+    launch_mi_deriv_finish(a.partial, blocks, int(np), static_cast<double*>(dev_sum), ctx_->done_words_device(), tag, s,
+                           dev_fin, fin_dev);
   }
   ctx_->wait_done(1, tag);
-  (void)finish_tag;
   const double value = finished_value(fin[0]);
   std::vector<double> d(host, host + np);
   return {value, d};

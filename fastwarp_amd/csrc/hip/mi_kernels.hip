@@ -325,7 +325,9 @@ __global__ __launch_bounds__(1024) void k_mi_sum_slots(const unsigned long long*
     hist[c] = t;
   }
   // the wave that wrote this block's cells releases them to the host with its completion word
-  if (threadIdx.x == 0) __hip_atomic_store(done + blockIdx.x, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  // (none when the sums stay on the device for a later kernel: the system-scope release would
+  // write back this XCD's L2)
+  if (done && threadIdx.x == 0) __hip_atomic_store(done + blockIdx.x, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 int mi_sum_slots_blocks(int cells) { return (cells + 63) / 64; }
@@ -339,17 +341,18 @@ void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells,
 // histograms one after another, each bins*bins joint cells (2^-32 fixed point), the bins fixed
 // marginal counts and the valid count. The joint total is the exact integer sum of the cells,
 // rounded once; the cells and marginals are scaled by its inverse; the moving marginal adds each
-// column in row order; the value adds jv (log(jv / p_M) - log(p_F)) over the cells whose three
+// column in row order (one thread per column, beside the fixed marginal's logs, one per row); the
+// value adds jv (log(jv / p_M) - log(p_F)) over the cells whose three
 // terms exceed the f64 epsilon (per-thread sums, then the waves' sums in order). ratio (block 0,
 // if given) receives n_factor log(jv / p_M) per cell in f32, zero elsewhere, as the derivative
 // pass reads it. out[b] gets the value, the valid count and a status (1: no valid sample, 2: the
-// joint cells sum to zero; the ratio is then all zero); then tag goes to done[b] with a
-// system-scope release.
+// joint cells sum to zero; the ratio is then all zero); then, if done is given, tag goes to
+// done[b] with a system-scope release.
 __global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __restrict__ hist, int bins,
                                                     double moving_bin_size, MiFinishOut* out, float* __restrict__ ratio,
                                                     uint32_t* done, uint32_t tag) {
   __shared__ double jn[kMiMaxBins * kMiMaxBins];
-  __shared__ double mm[kMiMaxBins], fm[kMiMaxBins];
+  __shared__ double mm[kMiMaxBins], fm[kMiMaxBins], lf[kMiMaxBins];
   __shared__ unsigned long long wsum_u[16];
   __shared__ double wsum_d[16];
   __shared__ int status_s;
@@ -383,17 +386,22 @@ __global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __
       out[b].value = 0.0;
       out[b].valid = valid;
       out[b].status = status;
-      __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (done) __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     return;
   }
   for (int c = tid; c < cells; c += blockDim.x) jn[c] = ((double)h[c] / kMiFixedPoint) * norm;
   if (tid < bins) fm[tid] = (double)h[cells + tid] * norm;
   __syncthreads();
+  // the fixed marginal's logs on the first wave, the moving marginal on the second
   if (tid < bins) {
+    lf[tid] = fm[tid] > eps ? log(fm[tid]) : 0.0;
+  } else if (tid >= 64 && tid < 64 + bins) {
+    const int m = tid - 64;
     double s = 0.0;
-    for (int r = 0; r < bins; ++r) s += jn[r * bins + tid];
-    mm[tid] = s;
+#pragma unroll 8
+    for (int r = 0; r < bins; ++r) s += jn[r * bins + m];
+    mm[m] = s;
   }
   __syncthreads();
   const double n_factor = -1.0 / (moving_bin_size * (double)valid);
@@ -404,7 +412,7 @@ __global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __
     float r = 0.0f;
     if (fm[f] > eps && mm[m] > eps && jv > eps) {
       const double pr = log(jv / mm[m]);
-      total += jv * (pr - log(fm[f]));
+      total += jv * (pr - lf[f]);
       r = (float)(n_factor * pr);
     }
     if (rb) rb[c] = r;
@@ -419,7 +427,7 @@ __global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __
     out[b].value = -all;
     out[b].valid = valid;
     out[b].status = 0;
-    __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (done) __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -513,7 +521,8 @@ void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s) 
 // consecutive slices of the blocks, each lane adds its blocks' rows (all loads in flight at
 // once), a butterfly adds each wave's lanes, and the 16 wave sums are added in order.
 __global__ __launch_bounds__(1024) void k_mi_deriv_finish(const double* __restrict__ partial, int blocks, int np,
-                                                          double* __restrict__ out, uint32_t* done, uint32_t tag) {
+                                                          double* __restrict__ out, uint32_t* done, uint32_t tag,
+                                                          const MiFinishOut* finish, MiFinishOut* finish_out) {
   __shared__ double wsum[16][12];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int per = (blocks + 15) / 16;
@@ -542,13 +551,21 @@ __global__ __launch_bounds__(1024) void k_mi_deriv_finish(const double* __restri
     for (int q = 0; q < 16; ++q) t += wsum[q][threadIdx.x];
     out[threadIdx.x] = t;
   }
-  // wave 0 wrote the sums; it releases them to the host with the completion word
-  if (threadIdx.x == 0) __hip_atomic_store(done, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  // wave 0 wrote the sums (and copies the value pass's finish beside them); it releases them to
+  // the host with the completion word
+  if (threadIdx.x == 0) {
+    if (finish) {
+      finish_out->value = finish->value;
+      finish_out->valid = finish->valid;
+      finish_out->status = finish->status;
+    }
+    __hip_atomic_store(done, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, uint32_t* done, uint32_t tag,
-                            hipStream_t s) {
-  launch(k_mi_deriv_finish, dim3(1), dim3(1024), 0, s, partial, blocks, np, out, done, tag);
+                            hipStream_t s, const MiFinishOut* finish, MiFinishOut* finish_out) {
+  launch(k_mi_deriv_finish, dim3(1), dim3(1024), 0, s, partial, blocks, np, out, done, tag, finish, finish_out);
 }
 
 // ------------------------------------------------------------------------------ dense pair ---

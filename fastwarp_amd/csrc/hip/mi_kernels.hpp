@@ -49,16 +49,16 @@ struct MiBatch {
 int mi_batch_capacity(int bins);
 void launch_mi_sampled_hist_batch(const MiSampledArgs& a, const MiBatch& batch, int count, int blocks, hipStream_t s);
 // Adds `nslots` per-block histograms of `cells` u64 entries into `hist`.
-// hist[c] = the slots' cell c added up (hist in coherent host memory); block b then stores `tag` into
-// done[b] with a system-scope release, b < mi_sum_slots_blocks(cells).
+// hist[c] = the slots' cell c added up; block b then stores `tag` into done[b] with a system-scope
+// release, b < mi_sum_slots_blocks(cells), unless done is null (sums left for a later kernel).
 int mi_sum_slots_blocks(int cells);
 void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells, unsigned long long* hist, uint32_t* done,
                          uint32_t tag, hipStream_t s);
 // The Mattes finish of `count` histograms laid out as k_mi_sum_slots writes them (each bins*bins
 // joint cells, bins fixed marginal counts, the valid count), on the device: out[k] receives the
 // value of histogram k, its valid count and a status (0 ok, 1 no valid sample, 2 the joint cells
-// sum to zero); `ratio` (if not null) the derivative pass's f32 weights of histogram 0; then done[k]
-// = tag (system-scope release), k < count.
+// sum to zero); `ratio` (if not null) the derivative pass's f32 weights of histogram 0; then, if
+// done is not null, done[k] = tag (system-scope release), k < count.
 struct MiFinishOut {
   double value;
   unsigned long long valid;
@@ -72,10 +72,12 @@ int mi_sampled_blocks(int count, int cap = 512);
 int mi_sampled_threads(int count);
 constexpr int kMiDerivMaxBlocks = 2048;
 int mi_deriv_blocks(int count, int cap = kMiDerivMaxBlocks);
-// out[k] (k < np) = the blocks' derivative partials added in a fixed order; then done[0] = tag
-// (system-scope release).
+// out[k] (k < np) = the blocks' derivative partials added in a fixed order; *finish_out = *finish
+// if finish is given (a finish left in device memory, passed to the host with the sums); then
+// done[0] = tag (system-scope release).
+struct MiFinishOut;
 void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, uint32_t* done, uint32_t tag,
-                            hipStream_t s);
+                            hipStream_t s, const MiFinishOut* finish = nullptr, MiFinishOut* finish_out = nullptr);
 
 // Dense pair on one grid (mi.rs:578-676 in both directions, wgpu_backend/mi/dense.rs:241-487).
 struct MiDenseArgs {
