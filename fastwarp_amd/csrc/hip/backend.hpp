@@ -368,6 +368,8 @@ class MiLevel {
   void launch_finish(const unsigned long long* slots, int blocks, size_t k, float* ratio, MiFinishOut* out, bool signal,
                      uint32_t tag);
   volatile MiFinishOut* finish_slots(size_t k, MiFinishOut** dev);
+  void evaluate(const MiSampledArgs& a, const MiBatch& batch, size_t k, int blocks, float* ratio, MiFinishOut* out, bool signal,
+                uint32_t tag);
   DeviceContext* ctx_;
   const HImage* moving_;
   DevBuf samples_;

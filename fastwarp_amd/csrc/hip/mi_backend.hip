@@ -113,6 +113,30 @@ void MiLevel::launch_finish(const unsigned long long* slots, int blocks, size_t 
                    signal ? ctx_->done_words_device() : nullptr, tag, s);
 }
 
+// One evaluation of k transforms (k = 1: the pass arguments in `a`; else `batch`), finished into
+// `out` (device view): the histogram pass, then the slot sum and the finish, in one launch when
+// the pass has at most kMiFusedSlots slots (tuning knob mi.fused_slots) and kMiFusedBins bins.
+void MiLevel::evaluate(const MiSampledArgs& a, const MiBatch& batch, size_t k, int blocks, float* ratio, MiFinishOut* out,
+                       bool signal, uint32_t tag) {
+  hipStream_t s = ctx_->stream();
+  if (k == 1) {
+    Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
+    launch_mi_sampled_hist(a, blocks, s);
+  } else {
+    Timed tm(*ctx_, "k_mi_sampled_hist_batch", (16.0 + 4.0 * double(k)) * double(count_));
+    launch_mi_sampled_hist_batch(a, batch, int(k), blocks, s);
+  }
+  const size_t bins = binning_.bins;
+  if (blocks <= int(ctx_->tuning().get("mi.fused_slots", count_, kMiFusedSlots)) && bins <= size_t(kMiFusedBins)) {
+    const double cells = double(k * (bins * bins + bins + 1));
+    Timed tm(*ctx_, "k_mi_sum_finish", 8.0 * cells * double(blocks) + (ratio ? 4.0 * double(bins * bins) : 0.0));
+    launch_mi_sum_finish(a.hist_slots, blocks, int(k), int(bins), binning_.moving_bin_size, out, ratio,
+                         signal ? ctx_->done_words_device() : nullptr, tag, s);
+    return;
+  }
+  launch_finish(a.hist_slots, blocks, k, ratio, out, signal, tag);
+}
+
 // The finish of `k` histograms in coherent host memory: host view, with the device view in *dev.
 volatile MiFinishOut* MiLevel::finish_slots(size_t k, MiFinishOut** dev) {
   void* d = nullptr;
@@ -160,14 +184,10 @@ void MiLevel::histogram_cells(const LinearMovingTransform& t, std::vector<double
 double MiLevel::value(const LinearMovingTransform& t) {
   MiSampledArgs a;
   const int blocks = pass_args(t, a, 1);
-  {
-    Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
-    launch_mi_sampled_hist(a, blocks, ctx_->stream());
-  }
   MiFinishOut* dev = nullptr;
   volatile MiFinishOut* out = finish_slots(1, &dev);
   const uint32_t tag = ctx_->next_tag();
-  launch_finish(a.hist_slots, blocks, 1, nullptr, dev, true, tag);
+  evaluate(a, MiBatch{}, 1, blocks, nullptr, dev, true, tag);
   ctx_->wait_done(1, tag);
   return finished_value(out[0]);
 }
@@ -198,14 +218,10 @@ void MiLevel::batch_values(const LinearMovingTransform* ts, size_t k, std::vecto
     put(batch.a[q], A);
     put(batch.b[q], b);
   }
-  {
-    Timed tm(*ctx_, "k_mi_sampled_hist_batch", (16.0 + 4.0 * double(k)) * double(count_));
-    launch_mi_sampled_hist_batch(a, batch, int(k), blocks, ctx_->stream());
-  }
   MiFinishOut* dev = nullptr;
   volatile MiFinishOut* fin = finish_slots(k, &dev);
   const uint32_t tag = ctx_->next_tag();
-  launch_finish(a.hist_slots, blocks, k, nullptr, dev, true, tag);
+  evaluate(a, batch, k, blocks, nullptr, dev, true, tag);
   ctx_->wait_done(int(k), tag);
   for (size_t q = 0; q < k; ++q) out.push_back(finished_value(fin[q]));
 }
@@ -215,16 +231,12 @@ std::pair<double, std::vector<double>> MiLevel::value_and_derivative(const Linea
   const int hist_blocks = pass_args(t, a, 1);
   const size_t bins = binning_.bins;
   hipStream_t s = ctx_->stream();
-  {
-    Timed tm(*ctx_, "k_mi_sampled_hist", 20.0 * double(count_));
-    launch_mi_sampled_hist(a, hist_blocks, s);
-  }
   // the finish leaves the ratio weights and its result on the device, where the derivative pass
   // reads the weights and its last kernel passes the result to the host with the derivative: one
   // wait for both
   auto* dev_ratio = static_cast<float*>(ctx_->scratch(1, bins * bins * sizeof(float)));
   auto* dev_fin = static_cast<MiFinishOut*>(ctx_->scratch(9, sizeof(MiFinishOut)));
-  launch_finish(a.hist_slots, hist_blocks, 1, dev_ratio, dev_fin, false, 0);
+  evaluate(a, MiBatch{}, 1, hist_blocks, dev_ratio, dev_fin, false, 0);
   a.ratio = dev_ratio;
   a.probes = probes_of(moving_->grid);
   put(a.outer, t.outer.matrix);

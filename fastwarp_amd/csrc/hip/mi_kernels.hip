@@ -348,21 +348,21 @@ void launch_mi_sum_slots(const unsigned long long* slots, int nslots, int cells,
 // pass reads it. out[b] gets the value, the valid count and a status (1: no valid sample, 2: the
 // joint cells sum to zero; the ratio is then all zero); then, if done is given, tag goes to
 // done[b] with a system-scope release.
-__global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __restrict__ hist, int bins,
-                                                    double moving_bin_size, MiFinishOut* out, float* __restrict__ ratio,
-                                                    uint32_t* done, uint32_t tag) {
-  __shared__ double jn[kMiMaxBins * kMiMaxBins];
-  __shared__ double mm[kMiMaxBins], fm[kMiMaxBins], lf[kMiMaxBins];
+// The finish of one histogram h (global or LDS) of at most MAXB bins by the whole block (every
+// thread calls it).
+template <int MAXB>
+__device__ void finish_block(const unsigned long long* h, int bins, double moving_bin_size, MiFinishOut* out, float* rb,
+                             uint32_t* done, uint32_t tag) {
+  __shared__ double jn[MAXB * MAXB];
+  __shared__ double mm[MAXB], fm[MAXB], lf[MAXB];
   __shared__ unsigned long long wsum_u[16];
   __shared__ double wsum_d[16];
   __shared__ int status_s;
   const double eps = 2.220446049250313e-16;
   const int cells = bins * bins;
-  const int b = blockIdx.x;
-  const unsigned long long* h = hist + (size_t)b * (cells + bins + 1);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = (int)(blockDim.x + 63) >> 6;
+  __syncthreads();  // the scratch of an earlier call in this block is free
   const unsigned long long valid = h[cells + bins];
-  float* rb = b == 0 ? ratio : nullptr;
   // the exact total of the joint cells
   unsigned long long t = 0ull;
   for (int c = tid; c < cells; c += blockDim.x) t += h[c];
@@ -383,10 +383,10 @@ __global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __
     if (rb)
       for (int c = tid; c < cells; c += blockDim.x) rb[c] = 0.0f;
     if (tid == 0) {
-      out[b].value = 0.0;
-      out[b].valid = valid;
-      out[b].status = status;
-      if (done) __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      out->value = 0.0;
+      out->valid = valid;
+      out->status = status;
+      if (done) __hip_atomic_store(done, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     return;
   }
@@ -424,16 +424,57 @@ __global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __
   if (tid == 0) {
     double all = 0.0;
     for (int q = 0; q < nw; ++q) all += wsum_d[q];
-    out[b].value = -all;
-    out[b].valid = valid;
-    out[b].status = 0;
-    if (done) __hip_atomic_store(done + b, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    out->value = -all;
+    out->valid = valid;
+    out->status = 0;
+    if (done) __hip_atomic_store(done, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+}
+
+__global__ __launch_bounds__(1024) void k_mi_finish(const unsigned long long* __restrict__ hist, int bins,
+                                                    double moving_bin_size, MiFinishOut* out, float* __restrict__ ratio,
+                                                    uint32_t* done, uint32_t tag) {
+  const int b = blockIdx.x;
+  finish_block<kMiMaxBins>(hist + (size_t)b * (bins * bins + bins + 1), bins, moving_bin_size, out + b,
+                           b == 0 ? ratio : nullptr, done ? done + b : nullptr, tag);
 }
 
 void launch_mi_finish(const unsigned long long* hist, int count, int bins, double moving_bin_size, MiFinishOut* out,
                       float* ratio, uint32_t* done, uint32_t tag, hipStream_t s) {
   launch(k_mi_finish, dim3(count), dim3(1024), 0, s, hist, bins, moving_bin_size, out, ratio, done, tag);
+}
+
+// The slot sum and the finish in one launch, for passes of few slots and at most kMiFusedBins
+// bins: block k adds histogram k's cells over the slots (slot q holds `per_slot` entries, the
+// histograms one after another) into LDS, as k_mi_sum_slots does (integer sums, so the same
+// cells), and finishes them.
+__global__ __launch_bounds__(1024) void k_mi_sum_finish(const unsigned long long* __restrict__ slots, int nslots,
+                                                        int per_slot, int bins, double moving_bin_size, MiFinishOut* out,
+                                                        float* __restrict__ ratio, uint32_t* done, uint32_t tag) {
+  __shared__ unsigned long long hs[kMiFusedBins * kMiFusedBins + kMiFusedBins + 1];
+  const int C = bins * bins + bins + 1;
+  const int b = blockIdx.x;
+  const unsigned long long* src = slots + (size_t)b * C;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    unsigned long long t0 = 0ull, t1 = 0ull, t2 = 0ull, t3 = 0ull;
+    int q = 0;
+    for (; q + 3 < nslots; q += 4) {
+      t0 += src[(size_t)q * per_slot + c];
+      t1 += src[(size_t)(q + 1) * per_slot + c];
+      t2 += src[(size_t)(q + 2) * per_slot + c];
+      t3 += src[(size_t)(q + 3) * per_slot + c];
+    }
+    for (; q < nslots; ++q) t0 += src[(size_t)q * per_slot + c];
+    hs[c] = (t0 + t1) + (t2 + t3);
+  }
+  finish_block<kMiFusedBins>(hs, bins, moving_bin_size, out + b, b == 0 ? ratio : nullptr, done ? done + b : nullptr, tag);
+}
+
+void launch_mi_sum_finish(const unsigned long long* slots, int nslots, int count, int bins, double moving_bin_size,
+                          MiFinishOut* out, float* ratio, uint32_t* done, uint32_t tag, hipStream_t s) {
+  const int per_slot = count * (bins * bins + bins + 1);
+  launch(k_mi_sum_finish, dim3(count), dim3(1024), 0, s, slots, nslots, per_slot, bins, moving_bin_size, out, ratio, done,
+         tag);
 }
 
 // Per sample: sum_k B3'(k - term) * ratio[f, k] * (grad M . dT/dtheta_p) for every parameter

@@ -67,6 +67,14 @@ struct MiFinishOut {
 };
 void launch_mi_finish(const unsigned long long* hist, int count, int bins, double moving_bin_size, MiFinishOut* out,
                       float* ratio, uint32_t* done, uint32_t tag, hipStream_t s);
+// launch_mi_sum_slots and launch_mi_finish in one launch, for `count` histograms of at most
+// kMiFusedBins bins in `nslots` slots (slot q: the count histograms one after another): each block
+// adds its histogram's slots in LDS and finishes it. For passes of few slots, where the two
+// launches cost more than the sums.
+constexpr int kMiFusedBins = 32;
+constexpr int kMiFusedSlots = 32;
+void launch_mi_sum_finish(const unsigned long long* slots, int nslots, int count, int bins, double moving_bin_size,
+                          MiFinishOut* out, float* ratio, uint32_t* done, uint32_t tag, hipStream_t s);
 void launch_mi_sampled_deriv(const MiSampledArgs& a, int blocks, hipStream_t s);
 int mi_sampled_blocks(int count, int cap = 512);
 int mi_sampled_threads(int count);
@@ -75,7 +83,6 @@ int mi_deriv_blocks(int count, int cap = kMiDerivMaxBlocks);
 // out[k] (k < np) = the blocks' derivative partials added in a fixed order; *finish_out = *finish
 // if finish is given (a finish left in device memory, passed to the host with the sums); then
 // done[0] = tag (system-scope release).
-struct MiFinishOut;
 void launch_mi_deriv_finish(const double* partial, int blocks, int np, double* out, uint32_t* done, uint32_t tag,
                             hipStream_t s, const MiFinishOut* finish = nullptr, MiFinishOut* finish_out = nullptr);
 

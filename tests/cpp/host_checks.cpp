@@ -156,6 +156,7 @@ static void tuning() {
   CHECK(TuningTable::valid("mi.threads", 512) && !TuningTable::valid("mi.threads", 384), "mi threads");
   CHECK(TuningTable::valid("gauss.one_pass", 1) && !TuningTable::valid("gauss.one_pass", 2), "one pass");
   CHECK(TuningTable::valid("batch.graphs", 0) && !TuningTable::valid("batch.graphs", 2), "graphs");
+  CHECK(TuningTable::valid("mi.fused_slots", 0) && !TuningTable::valid("mi.fused_slots", -1), "fused slots");
   CHECK(!TuningTable::valid("cc.tile_z", 0) && !TuningTable::valid("nonsense", 1), "unknown knob");
   const char* path = "/tmp/fw_tuning_check.json";
   {

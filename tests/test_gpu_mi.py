@@ -113,6 +113,34 @@ def test_device_finish_matches_the_f64_finish_of_the_same_cells(backend, bins):
         assert level.value_and_derivative(t)[0] == got, q
 
 
+@pytest.mark.parametrize("bins", [32, 12])
+def test_fused_slot_sum_and_finish_give_the_bits_of_the_two_launches(backend, bins):
+    """Passes of few blocks add their slots and finish them in one launch (k_mi_sum_finish); the
+    cells are the same integers as k_mi_sum_slots adds, so values (single and batched) and
+    derivatives are bit-identical with the fusion switched off (tuning knob mi.fused_slots = 0)."""
+    g = cases.grid((40, 36, 30), (1.2, 1.0, 1.4), (-20.0, -15.0, -18.0), oblique=0.3)
+    f = smooth_vol(g, (0, 0, 0))
+    m = smooth_vol(g, (1.1, -0.8, 0.6), 0.3)
+    rng = np.random.default_rng(9)
+    pts = np.array([g.index_to_point(p) for p in rng.uniform([4, 4, 4], np.array(g.shape_xyz) - 5, size=(5000, 3))])
+    b = MiBinning.from_ranges(bins, (f.min(), f.max()), (m.min(), m.max()))
+    level = MiLevel(backend, backend.upload_image(g, m), pts, fixed_bins(g, f, pts, b), b)
+    centre = tuple(g.index_to_point(np.array(g.shape_xyz) / 2))
+    ts = [LinearTransform.make(0, np.concatenate([rng.normal(0, 0.04, 3), rng.normal(0, 1.5, 3)]), centre) for _ in range(5)]
+    ts.append(LinearTransform.make(1, np.array([1.01, 0.02, 0.0, -0.01, 0.98, 0.01, 0.0, 0.02, 1.0, 0.4, -0.2, 0.3]), centre))
+    runs = []
+    for fused in (64, 0):
+        backend.set_tuning("mi.fused_slots", fused)
+        try:
+            runs.append(([level.value(t) for t in ts], list(level.values(ts[:4])), [level.value_and_derivative(t) for t in ts]))
+        finally:
+            backend.set_tuning("mi.fused_slots", -1)
+    (v1, b1, d1), (v0, b0, d0) = runs
+    assert v1 == v0 and b1 == b0
+    for (x, dx), (y, dy) in zip(d1, d0):
+        assert x == y and np.array_equal(dx, dy)
+
+
 def test_sampled_mi_is_deterministic_and_errors_without_overlap(backend):
     g = cases.grid((24, 24, 24))
     f = smooth_vol(g, (0, 0, 0))
