@@ -2,7 +2,7 @@
 class of a workload's pyramid levels, and writes the winners as the table every context loads
 (fastwarp_amd/tuning/gfx950.json).
 
-    python -m fastwarp_amd.autotune [--out fastwarp_amd/tuning/gfx950.json] [--repeats 20]
+    python -m fastwarp_amd.autotune [--out fastwarp_amd/tuning/gfx950.json] [--repeats 50]
 
 A size class is floor(log2(size)) of a level grid's voxels (CC, smoothing, inversion) or of its
 linear-stage samples (MI). The workload is the cohort pyramid (shrink 8/4/2/1) of the 1 mm grid and
@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import sys
 import time
 from pathlib import Path
@@ -35,7 +36,7 @@ CANDIDATES = {
     "cc": [("cc.tile_z", t, "cc.min_blocks", m) for t in (8, 16, 32, 64) for m in (256, 512)],
     "gauss": [("gauss.one_pass", v) for v in (0, 1)],
     "inv": [("inv.max_blocks", v) for v in (512, 1024, 2048, 4096, 8192)],
-    "mi": [("mi.threads", t, "mi.max_blocks", b) for t in (256, 512, 1024) for b in (256, 512, 1024, 2048)],
+    "mi": [("mi.threads", t, "mi.max_blocks", b) for t in (256, 512, 1024) for b in (16, 32, 256, 512, 1024, 2048)],
     "mi_deriv": [("mi.deriv_blocks", v) for v in (256, 512, 1024, 2048)],
 }
 
@@ -91,15 +92,12 @@ def tune(b: HipBackend, name: str, cls: int, op, families, repeats: int, table: 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(OUT))
-    ap.add_argument("--repeats", type=int, default=20)
+    ap.add_argument("--repeats", type=int, default=50)
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    # start from the library's defaults: the context loads an empty table instead of the shipped one
+    os.environ["FASTWARP_TUNING"] = os.devnull
     b = HipBackend(0)
-    # start from the library's defaults: drop whatever table the context loaded
-    for name, cands in CANDIDATES.items():
-        for cand in cands:
-            for q in range(0, len(cand), 2):
-                b.set_tuning(cand[q], -1)
     table: dict = {}
     t0 = time.perf_counter()
     grids = {}
@@ -134,7 +132,8 @@ def main(argv=None) -> int:
         centre = tuple(np.array(list(lg.origin)) + (size - 1) / 2 * np.array(list(lg.spacing)))
         t = LinearTransform.make(0, [0.01, -0.02, 0.015, 0.5, -0.3, 0.2], centre)
         scls = size_class(idx.size)
-        won = tune(b, "mi", scls, lambda: level.value(t), ["k_mi_sampled_hist", "k_mi_sum_slots"], args.repeats, table, log)
+        won = tune(b, "mi", scls, lambda: level.value(t), ["k_mi_sampled_hist", "k_mi_sum_slots", "k_mi_finish", "k_mi_sum_finish"],
+                   args.repeats, table, log)
         apply(b, won, scls, True)
         tune(b, "mi_deriv", scls, lambda: level.value_and_derivative(t), ["k_mi_sampled_deriv", "k_mi_deriv_finish"],
              args.repeats, table, log)
