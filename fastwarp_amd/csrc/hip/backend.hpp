@@ -325,6 +325,14 @@ class HipBackend {
   std::pair<HField, HField> compose_pair_scaled(const HField& u0, const HField& w0, const HField& u1, const HField& w1,
                                                 const DevBuf& factors);
   std::pair<HField, Pending<InvertStats>> invert_deferred(const HField& field, HField initial, const InvertParams& p);
+  // invert_deferred of two fields on one grid in the same launches (each item as alone, so the
+  // same bits): {first estimate, second estimate, their statistics}.
+  struct InvertedPair {
+    HField first, second;
+    Pending<InvertStats> first_stats, second_stats;
+  };
+  InvertedPair invert_pair_deferred(const HField& f0, HField initial0, const HField& f1, HField initial1,
+                                    const InvertParams& p);
   void settle() { ctx_->settle(); }
   HField resample_field(const HField& field, const Grid& grid);
 

@@ -615,6 +615,34 @@ void iterate(HipBackend& b, const SynRecipe& recipe, const std::vector<double>& 
 
 }  // namespace
 
+HipBackend::InvertedPair HipBackend::invert_pair_deferred(const HField& f0, HField initial0, const HField& f1,
+                                                          HField initial1, const InvertParams& p) {
+  if (!(f0.grid == f1.grid) || !(initial0.grid == f0.grid) || !(initial1.grid == f1.grid)) {
+    auto a = invert_deferred(f0, std::move(initial0), p);
+    auto b = invert_deferred(f1, std::move(initial1), p);
+    return {std::move(a.first), std::move(b.first), std::move(a.second), std::move(b.second)};
+  }
+  // two adjacent slots of the inversion reports' ring (read when the statistics are asked for)
+  constexpr size_t kReports = 64;
+  void* report_dev = nullptr;
+  auto* reports = static_cast<volatile InvertReport*>(ctx_->mapped(2, kReports * sizeof(InvertReport), &report_dev));
+  const size_t slot = 2 * (size_t(ctx_->next_tag()) % (kReports / 2));
+  std::vector<const HField*> fields{&f0, &f1};
+  std::vector<HField*> psi{&initial0, &initial1};
+  std::vector<DevBuf> keep;
+  const InvStage st = invert_items(*this, fields, psi, p, static_cast<InvertReport*>(report_dev) + slot, reports + slot, keep);
+  DeviceContext* ctx = ctx_.get();
+  const double voxels = double(f0.grid.num_voxels());
+  const bool profiling = ctx_->profiling();
+  auto stats = [ctx, voxels, profiling](volatile InvertReport* r) {
+    return Pending<InvertStats>(std::function<InvertStats()>([r, ctx, voxels, profiling] {
+      if (profiling && r->executed > 0) ctx->add_bytes("k_inv_iter", voxels * (60.0 * double(r->executed - 1) + 36.0));
+      return stats_of(*r);
+    }));
+  };
+  return {std::move(initial0), std::move(initial1), stats(st.reports), stats(st.reports + 1)};
+}
+
 std::vector<SynFields> syn_fields_batch(HipBackend& b, const SynRecipe& recipe, const std::vector<AffineTransform>& initial,
                                         const std::vector<SmoothCache<HipBackend>*>& caches, const volatile int32_t* cancel,
                                         std::vector<std::optional<HField>>* forward,

@@ -127,6 +127,12 @@ struct has_scaled_compose : std::false_type {};
 template <typename B>
 struct has_scaled_compose<B, std::void_t<decltype(&B::step_factors), decltype(&B::compose_pair_scaled)>> : std::true_type {};
 
+// Backends that invert the two sides' fields in the same launches (invert_pair_deferred).
+template <typename B, typename = void>
+struct has_invert_pair : std::false_type {};
+template <typename B>
+struct has_invert_pair<B, std::void_t<decltype(&B::invert_pair_deferred)>> : std::true_type {};
+
 template <typename B>
 std::pair<typename B::FieldH, typename B::FieldH> smooth_both(B& b, typename B::FieldH f0, typename B::FieldH f1, double variance) {
   if constexpr (has_pair_ops<B>::value) return b.smooth_field_pair(std::move(f0), std::move(f1), variance);
@@ -305,14 +311,28 @@ SynFields syn_fields(B& b, const SynRecipe& recipe, const AffineTransform& initi
       fields[1]->reset();
       auto totals = detail::smooth_both(b, std::move(composed.first), std::move(composed.second), recipe.total_field_variance);
       FieldH tmps[2] = {std::move(totals.first), std::move(totals.second)};
-      for (int s = 0; s < 2; ++s) {
-        FieldH tmp = std::move(tmps[s]);
-        auto inv = b.invert_deferred(tmp, std::move(**inverses[s]), inv_params);
-        auto fwd = b.invert_deferred(inv.first, std::move(tmp), inv_params);
-        *inverses[s] = std::move(inv.first);
-        *fields[s] = std::move(fwd.first);
-        stats[2 * s] = std::move(inv.second);
-        stats[2 * s + 1] = std::move(fwd.second);
+      if constexpr (detail::has_invert_pair<B>::value) {
+        // both sides at once: the new inverses from the old ones, then the new fields from them
+        auto inv = b.invert_pair_deferred(tmps[0], std::move(**inverses[0]), tmps[1], std::move(**inverses[1]), inv_params);
+        auto fwd = b.invert_pair_deferred(inv.first, std::move(tmps[0]), inv.second, std::move(tmps[1]), inv_params);
+        *inverses[0] = std::move(inv.first);
+        *inverses[1] = std::move(inv.second);
+        *fields[0] = std::move(fwd.first);
+        *fields[1] = std::move(fwd.second);
+        stats[0] = std::move(inv.first_stats);
+        stats[1] = std::move(fwd.first_stats);
+        stats[2] = std::move(inv.second_stats);
+        stats[3] = std::move(fwd.second_stats);
+      } else {
+        for (int s = 0; s < 2; ++s) {
+          FieldH tmp = std::move(tmps[s]);
+          auto inv = b.invert_deferred(tmp, std::move(**inverses[s]), inv_params);
+          auto fwd = b.invert_deferred(inv.first, std::move(tmp), inv_params);
+          *inverses[s] = std::move(inv.first);
+          *fields[s] = std::move(fwd.first);
+          stats[2 * s] = std::move(inv.second);
+          stats[2 * s + 1] = std::move(fwd.second);
+        }
       }
       b.settle();
       const auto [fmax, mmax] = maxima.get();
