@@ -799,7 +799,7 @@ HImage HipBackend::smooth_image_mm(const HImage& src, const Vec3& sigma_mm) {
     }
     // one read and one write per voxel (the anti-causal pass re-reads its line from cache)
     Timed t(*ctx_, "k_deriche_axis", 8.0 * double(n));
-    launch_deriche_axis(in, bufs[q], d, axes[q], c, s);
+    launch_deriche_axis(in, bufs[q], d, axes[q], c, s, ctx_->tuning().get("deriche.lds", uint64_t(n), 1) != 0);
     in = bufs[q];
   }
   return out;

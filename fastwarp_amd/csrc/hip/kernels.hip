@@ -1369,31 +1369,10 @@ void launch_gauss_z(const float* tmp, const float* in, float* out, Dims d, const
 // adjacent in x for the y and z axes (coalesced); x lines stay in the same cache lines for 16
 // consecutive voxels.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ in, float* __restrict__ out, Dims d,
-                                                      int axis, DericheCoefficients c) {
-  const long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long plane = (long long)d.nx * d.ny;
-  long long lines, base, stride;
-  int len;
-  if (axis == 0) {
-    lines = (long long)d.ny * d.nz;
-    base = line * d.nx;
-    stride = 1;
-    len = d.nx;
-  } else if (axis == 1) {
-    lines = (long long)d.nx * d.nz;
-    base = (line % d.nx) + (line / d.nx) * plane;
-    stride = d.nx;
-    len = d.ny;
-  } else {
-    lines = plane;
-    base = line;
-    stride = plane;
-    len = d.nz;
-  }
-  if (line >= lines) return;
-  const float* p = in + base;
-  float* q = out + base;
+// The recursions of one line: input p, output q, element i at offset i * stride (global memory,
+// or a line staged in LDS).
+__device__ __forceinline__ void deriche_line(const float* __restrict__ p, float* __restrict__ q, long long stride, int len,
+                                             const DericheCoefficients& c) {
   // The recursions run in f64 (the vector FP64 rate of gfx950 is that of FP32): in f32 their
   // rounding grows with sigma as the poles approach 1. The causal part is stored in f32 and the
   // anti-causal part added to it, so the result carries two f32 roundings.
@@ -1445,8 +1424,69 @@ __global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ 
   }
 }
 
-void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s) {
+__global__ __launch_bounds__(256) void k_deriche_axis(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                      int axis, DericheCoefficients c) {
+  const long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long plane = (long long)d.nx * d.ny;
+  long long lines, base, stride;
+  int len;
+  if (axis == 0) {
+    lines = (long long)d.ny * d.nz;
+    base = line * d.nx;
+    stride = 1;
+    len = d.nx;
+  } else if (axis == 1) {
+    lines = (long long)d.nx * d.nz;
+    base = (line % d.nx) + (line / d.nx) * plane;
+    stride = d.nx;
+    len = d.ny;
+  } else {
+    lines = plane;
+    base = line;
+    stride = plane;
+    len = d.nz;
+  }
+  if (line >= lines) return;
+  deriche_line(in + base, out + base, stride, len, c);
+}
+
+// Lines along x (contiguous in memory, so one thread per line reads nothing coalesced): a block
+// of one wave stages `per` consecutive lines in LDS with coalesced loads, runs one line's
+// recursions per thread on LDS, and stores the outputs coalesced. The same recursions as
+// k_deriche_axis, so the same bits.
+constexpr int kDericheLdsBytes = 48 << 10;
+__global__ __launch_bounds__(64) void k_deriche_x(const float* __restrict__ in, float* __restrict__ out, Dims d, int per,
+                                                  DericheCoefficients c) {
+  extern __shared__ float tile[];
+  const int nx = d.nx, row = nx + 1;
+  const long long lines = (long long)d.ny * d.nz;
+  const long long first = (long long)blockIdx.x * per;
+  const int nl = (int)(lines - first < per ? lines - first : per);
+  float* tin = tile;
+  float* tout = tile + per * row;
+  const float* src = in + first * nx;
+  for (int l = 0; l < nl; ++l)
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) tin[l * row + x] = src[(long long)l * nx + x];
+  __syncthreads();
+  if ((int)threadIdx.x < nl) deriche_line(tin + threadIdx.x * row, tout + threadIdx.x * row, 1, nx, c);
+  __syncthreads();
+  float* dst = out + first * nx;
+  for (int l = 0; l < nl; ++l)
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) dst[(long long)l * nx + x] = tout[l * row + x];
+}
+
+void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s,
+                         bool lds) {
   const long long lines = axis == 0 ? (long long)d.ny * d.nz : axis == 1 ? (long long)d.nx * d.nz : (long long)d.nx * d.ny;
+  if (axis == 0 && lds) {
+    const int per = (int)(kDericheLdsBytes / (2 * sizeof(float) * (size_t)(d.nx + 1)));
+    if (per >= 4) {
+      const int p = per > 64 ? 64 : per;
+      const size_t bytes = 2 * sizeof(float) * (size_t)p * (size_t)(d.nx + 1);
+      launch(k_deriche_x, dim3((unsigned)((lines + p - 1) / p)), dim3(64), (uint32_t)bytes, s, in, out, d, p, c);
+      return;
+    }
+  }
   launch(k_deriche_axis, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, in, out, d, axis, c);
 }
 

@@ -283,7 +283,10 @@ struct DericheCoefficients {
   double n[4], d[4], m[4], bn[4], bm[4];
 };
 // `in` and `out` must differ; every line needs at least 4 voxels.
-void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s);
+// With `lds`, lines along x go through LDS in tiles (the same bits; lines too long for the tile
+// budget take the direct form).
+void launch_deriche_axis(const float* in, float* out, Dims d, int axis, const DericheCoefficients& c, hipStream_t s,
+                         bool lds = true);
 
 
 }  // namespace fw::hip

@@ -52,6 +52,7 @@ class TuningTable {
     if (knob == "mi.max_blocks") return v >= 1 && v <= 65535;
     if (knob == "mi.deriv_blocks") return v >= 1 && v <= 65535;
     if (knob == "gauss.one_pass") return v == 0 || v == 1;
+    if (knob == "deriche.lds") return v == 0 || v == 1;
     if (knob == "mi.fused_slots") return v >= 0 && v <= 65535;
     if (knob == "batch.graphs") return v == 0 || v == 1;
     return false;

@@ -104,7 +104,8 @@ int fw_release_idle(fw_ctx* ctx);
  * grid's voxels or the level's samples; a negative value removes the entry. Knobs: cc.tile_z
  * (1..256), cc.min_blocks, inv.max_blocks, mi.threads (256, 512, 1024), mi.max_blocks,
  * mi.deriv_blocks, mi.fused_slots (passes of at most this many blocks add their slots and finish in
- * one launch; 0 never), gauss.one_pass (0, 1), batch.graphs (0, 1: the batched SyN replays each
+ * one launch; 0 never), gauss.one_pass (0, 1), deriche.lds (0, 1: x lines of the level smoothing
+ * staged in LDS; default 1), batch.graphs (0, 1: the batched SyN replays each
  * iteration's update launches from a HIP graph; default 0). A context starts from the table the
  * project's autotuner wrote (fastwarp_amd/tuning/gfx950.json, or the file FASTWARP_TUNING names);
  * results change at most by rounding between choices (batch.graphs not at all). */

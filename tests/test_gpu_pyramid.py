@@ -37,6 +37,23 @@ def test_smooth_image_matches_the_f64_filter(backend, name, sigma_vox):
     assert cases.rel_err(got, want) <= 5e-7, (name, sigma_vox)
 
 
+@pytest.mark.parametrize("shape", [(11, 9, 8), (182, 40, 9), (67, 71, 45), (1600, 5, 4)])
+def test_lines_staged_in_lds_give_the_bits_of_the_direct_form(backend, shape):
+    """The x lines of the recursive Gaussian go through LDS tiles (a partial last tile on most of
+    these shapes; the 1600-voxel lines exceed the tile budget and take the direct form): the same
+    bits as the direct form (tuning knob deriche.lds = 0)."""
+    g = cases.grid(shape, (1.0, 1.2, 0.9), (-5.0, 3.0, 1.0))
+    img = cases.image(g, 7)
+    out = []
+    for lds in (1, 0):
+        backend.set_tuning("deriche.lds", lds)
+        try:
+            out.append(backend.download_image(backend.smooth_image(backend.upload_image(g, img), (2.0, 1.5, 1.0))))
+        finally:
+            backend.set_tuning("deriche.lds", -1)
+    assert np.array_equal(out[0], out[1])
+
+
 def test_smooth_image_keeps_constants_and_copies_at_zero(backend):
     g = GRIDS["medium"]
     c = np.full(g.n, 0.625)
