@@ -1,6 +1,6 @@
 // Multi-resolution pyramid of antsRegistration (crates/fastwarp-reg/src/pyramid.rs): ANTs shrink
-// factors per axis, ITK ShrinkImageFilter virtual grids, and ITK's recursive Gaussian (Deriche,
-// fourth order, constant edge extension) of the full-resolution images, in f64 on the host.
+// factors per axis, ITK ShrinkImageFilter virtual grids, and the coefficients of ITK's recursive
+// Gaussian (Deriche, fourth order, constant edge extension), whose recursions run on the device.
 #pragma once
 
 #include "grid.hpp"
@@ -60,128 +60,71 @@ inline Grid shrink_grid(const Grid& g, const std::array<uint32_t, 3>& factors) {
 
 inline Grid level_virtual_grid(const Grid& fixed, const std::array<uint32_t, 3>& factors) { return shrink_grid(fixed, factors); }
 
-// pyramid.rs:184-305: Deriche's fourth-order recursive approximation of a zero-order Gaussian.
+// Coefficients of ITK's zero-order recursive Gaussian (pyramid.rs:184-244, ITK
+// RecursiveGaussianImageFilter): Deriche's fourth-order fit of the Gaussian (R. Deriche,
+// "Recursively implementing the Gaussian and its derivatives", INRIA RR-1893, 1993),
+//   g(t) ~ sum_r (a_r cos(w_r t) + b_r sin(w_r t)) exp(l_r t),  t >= 0 in units of sigma.
+// Sampled every 1/sd (sd = sigma in voxels), each term is the impulse response of a two-pole
+// section with numerator a + e (b sin - a cos) z^-1 and denominator 1 - 2 e cos z^-1 + e^2 z^-2
+// (e = exp(l / sd), angles w / sd). The causal filter N/D is their sum over the common denominator
+// D = D1 D2; the anti-causal filter is its mirror image without the k = 0 tap, so its numerator
+// is M(z) = N(1/z) - n0 D(1/z) read as a polynomial in z. Both numerators are scaled so that the
+// two filters together have unit DC gain. The lines are extended by their end values, so the
+// outputs before a recursion starts are the steady state of that value: v N(1)/D(1) and
+// v M(1)/D(1), which enter the first four steps as v * bn_k and v * bm_k.
 struct Deriche {
-  double n[4], d[4], m[4], bn[4], bm[4];
+  double n[4];   // causal numerator, taps x[i], x[i-1], x[i-2], x[i-3]
+  double d[4];   // denominator without its leading 1, on y[i-1] .. y[i-4] (both directions)
+  double m[4];   // anti-causal numerator, taps x[i+1] .. x[i+4]
+  double bn[4];  // d_k times the causal steady-state gain
+  double bm[4];  // d_k times the anti-causal steady-state gain
 
   static Deriche zero_order(double sigma, double spacing) {
     if (spacing < 1e-8) invalid("smoothing", "spacing is too small");
     const double sd = sigma / spacing;
-    const double a1 = 1.3530, b1 = 1.8151, w1 = 0.6681, l1 = -1.3932;
-    const double a2 = -0.3531, b2 = 0.0902, w2 = 2.0787, l2 = -1.3732;
-    const double sin1 = std::sin(w1 / sd), sin2 = std::sin(w2 / sd);
-    const double cos1 = std::cos(w1 / sd), cos2 = std::cos(w2 / sd);
-    const double e1 = std::exp(l1 / sd), e2 = std::exp(l2 / sd);
+    // Deriche's fitted constants (a, b, w, l) of the two terms, as ITK uses them
+    static constexpr double kFit[2][4] = {{1.3530, 1.8151, 0.6681, -1.3932}, {-0.3531, 0.0902, 2.0787, -1.3732}};
+    double num[2][2], den[2][3], num_dc[2], den_dc[2];
+    for (int r = 0; r < 2; ++r) {
+      const double a = kFit[r][0], b = kFit[r][1], w = kFit[r][2] / sd, e = std::exp(kFit[r][3] / sd);
+      const double c = std::cos(w), s = std::sin(w);
+      num[r][0] = a;
+      num[r][1] = e * (b * s - a * c);
+      den[r][0] = 1.0;
+      den[r][1] = -2.0 * e * c;
+      den[r][2] = e * e;
+      // the sections at z = 1, without the cancellation in 1 - 2 e cos w + e^2 as the poles
+      // approach 1 (large sigma): 1 - cos w = 2 sin^2(w / 2) and 1 - e = -expm1(l / sd)
+      const double one_minus_e = -std::expm1(kFit[r][3] / sd);
+      const double h = std::sin(0.5 * w);
+      den_dc[r] = one_minus_e * one_minus_e + 4.0 * e * h * h;
+      num_dc[r] = a * one_minus_e + e * (2.0 * a * h * h + b * s);
+    }
+    // D = D1 D2 and N = N1 D2 + N2 D1 as polynomials in z^-1
+    double D[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, N[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) D[i + j] += den[0][i] * den[1][j];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 3; ++j) N[i + j] += num[0][i] * den[1][j] + num[1][i] * den[0][j];
+    // N(1) / D(1): the causal filter's DC gain before scaling
+    const double causal_gain = (num_dc[0] * den_dc[1] + num_dc[1] * den_dc[0]) / (den_dc[0] * den_dc[1]);
+    // DC gain of causal plus anti-causal, the k = 0 tap counted once
+    const double gain = 2.0 * causal_gain - N[0];
     Deriche k{};
-    const double d4 = e1 * e1 * e2 * e2;
-    double d3 = -2.0 * cos1 * e1 * e2 * e2;
-    d3 += -2.0 * cos2 * e2 * e1 * e1;
-    double d2 = 4.0 * cos2 * cos1 * e1 * e2;
-    d2 += e1 * e1 + e2 * e2;
-    const double d1 = -2.0 * (e2 * cos2 + e1 * cos1);
-    const double sdd = 1.0 + d1 + d2 + d3 + d4;
-    double n0 = a1 + a2;
-    double n1 = e2 * (b2 * sin2 - (a2 + 2.0 * a1) * cos2);
-    n1 += e1 * (b1 * sin1 - (a1 + 2.0 * a2) * cos1);
-    double n2 = (a1 + a2) * cos2 * cos1;
-    n2 -= b1 * cos2 * sin1 + b2 * cos1 * sin2;
-    n2 *= 2.0 * e1 * e2;
-    n2 += a2 * e1 * e1 + a1 * e2 * e2;
-    double n3 = e2 * e1 * e1 * (b2 * sin2 - a2 * cos2);
-    n3 += e1 * e2 * e2 * (b1 * sin1 - a1 * cos1);
-    const double sn0 = n0 + n1 + n2 + n3;
-    const double alpha0 = 2.0 * sn0 / sdd - n0;
-    n0 /= alpha0;
-    n1 /= alpha0;
-    n2 /= alpha0;
-    n3 /= alpha0;
-    const double m1 = n1 - d1 * n0, m2 = n2 - d2 * n0, m3 = n3 - d3 * n0, m4 = -d4 * n0;
-    const double sn = n0 + n1 + n2 + n3;
-    const double sm = m1 + m2 + m3 + m4;
-    const double dd[4] = {d1, d2, d3, d4};
-    const double nn[4] = {n0, n1, n2, n3};
-    const double mm[4] = {m1, m2, m3, m4};
     for (int q = 0; q < 4; ++q) {
-      k.n[q] = nn[q];
-      k.d[q] = dd[q];
-      k.m[q] = mm[q];
-      k.bn[q] = dd[q] * sn / sdd;
-      k.bm[q] = dd[q] * sm / sdd;
+      k.n[q] = N[q] / gain;
+      k.d[q] = D[q + 1];
+    }
+    for (int q = 0; q < 4; ++q) k.m[q] = (q < 3 ? k.n[q + 1] : 0.0) - k.d[q] * k.n[0];
+    // steady states of a constant unit input: N(1)/D(1) scaled, and M(1)/D(1) = that minus n0
+    const double causal_dc = causal_gain / gain;
+    const double anti_dc = causal_dc - k.n[0];
+    for (int q = 0; q < 4; ++q) {
+      k.bn[q] = k.d[q] * causal_dc;
+      k.bm[q] = k.d[q] * anti_dc;
     }
     return k;
   }
-
-  // pyramid.rs:246-304: causal and anti-causal passes with constant extension; line length >= 4.
-  void filter_line(const double* data, size_t ln, double* causal, double* anti, double* out) const {
-    const double v1 = data[0];
-    causal[0] = v1 * n[0] + v1 * n[1] + v1 * n[2] + v1 * n[3];
-    causal[1] = data[1] * n[0] + v1 * n[1] + v1 * n[2] + v1 * n[3];
-    causal[2] = data[2] * n[0] + data[1] * n[1] + v1 * n[2] + v1 * n[3];
-    causal[3] = data[3] * n[0] + data[2] * n[1] + data[1] * n[2] + v1 * n[3];
-    causal[0] -= v1 * bn[0] + v1 * bn[1] + v1 * bn[2] + v1 * bn[3];
-    causal[1] -= causal[0] * d[0] + v1 * bn[1] + v1 * bn[2] + v1 * bn[3];
-    causal[2] -= causal[1] * d[0] + causal[0] * d[1] + v1 * bn[2] + v1 * bn[3];
-    causal[3] -= causal[2] * d[0] + causal[1] * d[1] + causal[0] * d[2] + v1 * bn[3];
-    for (size_t i = 4; i < ln; ++i) {
-      causal[i] = data[i] * n[0] + data[i - 1] * n[1] + data[i - 2] * n[2] + data[i - 3] * n[3];
-      causal[i] -= causal[i - 1] * d[0] + causal[i - 2] * d[1] + causal[i - 3] * d[2] + causal[i - 4] * d[3];
-    }
-    const double v2 = data[ln - 1];
-    anti[ln - 1] = v2 * m[0] + v2 * m[1] + v2 * m[2] + v2 * m[3];
-    anti[ln - 2] = data[ln - 1] * m[0] + v2 * m[1] + v2 * m[2] + v2 * m[3];
-    anti[ln - 3] = data[ln - 2] * m[0] + data[ln - 1] * m[1] + v2 * m[2] + v2 * m[3];
-    anti[ln - 4] = data[ln - 3] * m[0] + data[ln - 2] * m[1] + data[ln - 1] * m[2] + v2 * m[3];
-    anti[ln - 1] -= v2 * bm[0] + v2 * bm[1] + v2 * bm[2] + v2 * bm[3];
-    anti[ln - 2] -= anti[ln - 1] * d[0] + v2 * bm[1] + v2 * bm[2] + v2 * bm[3];
-    anti[ln - 3] -= anti[ln - 2] * d[0] + anti[ln - 1] * d[1] + v2 * bm[2] + v2 * bm[3];
-    anti[ln - 4] -= anti[ln - 3] * d[0] + anti[ln - 2] * d[1] + anti[ln - 1] * d[2] + v2 * bm[3];
-    for (size_t i = ln - 4; i >= 1; --i) {
-      anti[i - 1] = data[i] * m[0] + data[i + 1] * m[1] + data[i + 2] * m[2] + data[i + 3] * m[3];
-      anti[i - 1] -= anti[i] * d[0] + anti[i + 1] * d[1] + anti[i + 2] * d[2] + anti[i + 3] * d[3];
-    }
-    for (size_t i = 0; i < ln; ++i) out[i] = causal[i] + anti[i];
-  }
 };
-
-// Filters every line of `data` along `axis` in place, lines in parallel (pyramid.rs:366-419).
-inline void deriche_axis(HostVec<double>& data, const std::array<size_t, 3>& size, int axis, const Deriche& k) {
-  const size_t nx = size[0], ny = size[1], nz = size[2];
-  const size_t ln = size[axis];
-  const size_t stride = axis == 0 ? 1 : axis == 1 ? nx : nx * ny;
-  const size_t lines = data.size() / ln;
-  parallel_chunks(lines, 64, [&](size_t, size_t b, size_t e) {
-    std::vector<double> line(ln), causal(ln), anti(ln), out(ln);
-    for (size_t l = b; l < e; ++l) {
-      size_t base;
-      if (axis == 0) base = l * nx;
-      else if (axis == 1) base = (l / nx) * nx * ny + (l % nx);
-      else base = l;  // l < nx*ny
-      for (size_t t = 0; t < ln; ++t) line[t] = data[base + t * stride];
-      k.filter_line(line.data(), ln, causal.data(), anti.data(), out.data());
-      for (size_t t = 0; t < ln; ++t) data[base + t * stride] = out[t];
-    }
-  });
-  (void)nz;
-}
-
-// pyramid.rs:429-437: axes z, x, y with per-axis sigma in mm; every axis needs 4 voxels.
-inline Image smooth_recursive_gaussian(const Image& vol, const Vec3& sigma_mm) {
-  HostVec<double> out = vol.data;
-  for (int axis : {2, 0, 1}) {
-    if (vol.grid.size[axis] < 4) invalid("smoothing", "every axis needs at least 4 voxels");
-    if (!(sigma_mm[axis] > 0.0)) invalid("smoothing", "sigma must be greater than zero");
-    deriche_axis(out, vol.grid.size, axis, Deriche::zero_order(sigma_mm[axis], vol.grid.spacing[axis]));
-  }
-  return Image(vol.grid, std::move(out));
-}
-
-// pyramid.rs:441-465 `sigma_mm_per_axis`, `smooth_for_level`: a copy for sigma <= 0.
-inline Image smooth_for_level(const Image& vol, double sigma, SigmaUnits units) {
-  HostTimer host_timer("deriche");
-  if (sigma <= 0.0) return vol;
-  const Vec3& s = vol.grid.spacing;
-  const Vec3 mm = units == SigmaUnits::Voxels ? Vec3{sigma * s[0], sigma * s[1], sigma * s[2]} : Vec3{sigma, sigma, sigma};
-  return smooth_recursive_gaussian(vol, mm);
-}
 
 }  // namespace fw

@@ -75,7 +75,7 @@ const char* fwo_last_error() { return g_error.c_str(); }
 int fwo_smooth_image(const fw_grid* grid, const double* data, const double sigma_mm[3], double* out) {
   return guard([&] {
     Image v = image_from(grid, data);
-    Image r = fw::smooth_recursive_gaussian(v, {sigma_mm[0], sigma_mm[1], sigma_mm[2]});
+    Image r = fw::oracle::smooth_recursive_gaussian(v, {sigma_mm[0], sigma_mm[1], sigma_mm[2]});
     std::copy(r.data.begin(), r.data.end(), out);
   });
 }

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../fastwarp_amd/csrc/host/grid.hpp"
+#include "../fastwarp_amd/csrc/host/pyramid.hpp"
 #include "../fastwarp_amd/csrc/host/transform.hpp"
 
 namespace fw::oracle {
@@ -950,6 +951,116 @@ inline Image resample_isotropic(const Image& v, double mm) {
         data[i + out[0] * (j + out[1] * k)] = double(float(acc));
       }
   return Image(og, std::move(data));
+}
+
+// ------------------------------------------------ level smoothing (pyramid.rs:184-465) --
+
+// pyramid.rs:184-244: the zero-order coefficients of ITK's RecursiveGaussianImageFilter, as the
+// reference forms them from Deriche's constants (n: causal numerator, d: denominator without its
+// leading 1, m: anti-causal numerator, bn/bm: the boundary terms of constant extension).
+struct DericheCoefs {
+  double n[4], d[4], m[4], bn[4], bm[4];
+};
+
+inline DericheCoefs deriche_coefficients(double sigma, double spacing) {
+  if (spacing < 1e-8) invalid("smoothing", "spacing is too small");
+  const double sg = sigma / spacing;
+  const double a1 = 1.3530, b1 = 1.8151, w1 = 0.6681, l1 = -1.3932;
+  const double a2 = -0.3531, b2 = 0.0902, w2 = 2.0787, l2 = -1.3732;
+  const double s1 = std::sin(w1 / sg), s2 = std::sin(w2 / sg);
+  const double c1 = std::cos(w1 / sg), c2 = std::cos(w2 / sg);
+  const double e1 = std::exp(l1 / sg), e2 = std::exp(l2 / sg);
+  const double d1 = -2.0 * (e2 * c2 + e1 * c1);
+  const double d2 = 4.0 * c2 * c1 * e1 * e2 + (e1 * e1 + e2 * e2);
+  const double d3 = -2.0 * c1 * e1 * e2 * e2 + -2.0 * c2 * e2 * e1 * e1;
+  const double d4 = e1 * e1 * e2 * e2;
+  const double sd = 1.0 + d1 + d2 + d3 + d4;
+  const double n0 = a1 + a2;
+  const double n1 = e2 * (b2 * s2 - (a2 + 2.0 * a1) * c2) + e1 * (b1 * s1 - (a1 + 2.0 * a2) * c1);
+  const double n2 = 2.0 * e1 * e2 * ((a1 + a2) * c2 * c1 - (b1 * c2 * s1 + b2 * c1 * s2)) + (a2 * e1 * e1 + a1 * e2 * e2);
+  const double n3 = e2 * e1 * e1 * (b2 * s2 - a2 * c2) + e1 * e2 * e2 * (b1 * s1 - a1 * c1);
+  const double alpha = 2.0 * (n0 + n1 + n2 + n3) / sd - n0;
+  DericheCoefs k{};
+  const double nn[4] = {n0 / alpha, n1 / alpha, n2 / alpha, n3 / alpha};
+  const double dd[4] = {d1, d2, d3, d4};
+  double sn = 0.0, sm = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    k.n[q] = nn[q];
+    k.d[q] = dd[q];
+    k.m[q] = (q < 3 ? nn[q + 1] : 0.0) - dd[q] * nn[0];
+    sn += k.n[q];
+    sm += k.m[q];
+  }
+  for (int q = 0; q < 4; ++q) {
+    k.bn[q] = dd[q] * sn / sd;
+    k.bm[q] = dd[q] * sm / sd;
+  }
+  return k;
+}
+
+// pyramid.rs:246-304: the causal recursion y[i] = sum_q n_q x[i-q] - sum_q d_q y[i-1-q] and the
+// anti-causal one a[i] = sum_q m_q x[i+1+q] - sum_q d_q a[i+1+q], added. The input is extended by
+// its end values; the outputs before each recursion starts are replaced by v * bn_q (v * bm_q).
+inline void deriche_filter_line(const DericheCoefs& k, const double* x, size_t len, double* y, double* a, double* out) {
+  const long long L = (long long)len;
+  const double v1 = x[0], v2 = x[len - 1];
+  const auto at = [&](long long i) { return i < 0 ? v1 : (i >= L ? v2 : x[i]); };
+  for (long long i = 0; i < L; ++i) {
+    double fwd = 0.0, back = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      fwd += k.n[q] * at(i - q);
+      back += i - 1 - q >= 0 ? k.d[q] * y[i - 1 - q] : v1 * k.bn[q];
+    }
+    y[i] = fwd - back;
+  }
+  for (long long i = L - 1; i >= 0; --i) {
+    double fwd = 0.0, back = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      fwd += k.m[q] * at(i + 1 + q);
+      back += i + 1 + q < L ? k.d[q] * a[i + 1 + q] : v2 * k.bm[q];
+    }
+    a[i] = fwd - back;
+  }
+  for (size_t i = 0; i < len; ++i) out[i] = y[i] + a[i];
+}
+
+// pyramid.rs:366-419: every line of `data` along `axis`, in place.
+inline void deriche_axis(HostVec<double>& data, const std::array<size_t, 3>& size, int axis, const DericheCoefs& k) {
+  const size_t nx = size[0], ny = size[1];
+  const size_t len = size[axis];
+  const size_t stride = axis == 0 ? 1 : (axis == 1 ? nx : nx * ny);
+  const size_t lines = data.size() / len;
+  parallel_chunks(lines, 64, [&](size_t, size_t b, size_t e) {
+    std::vector<double> line(len), y(len), a(len), out(len);
+    for (size_t l = b; l < e; ++l) {
+      // the first voxel of line l: lines along x are rows, along y a row index within each plane,
+      // along z a voxel of the first plane
+      const size_t first = axis == 0 ? l * nx : (axis == 1 ? (l / nx) * nx * ny + l % nx : l);
+      for (size_t t = 0; t < len; ++t) line[t] = data[first + t * stride];
+      deriche_filter_line(k, line.data(), len, y.data(), a.data(), out.data());
+      for (size_t t = 0; t < len; ++t) data[first + t * stride] = out[t];
+    }
+  });
+}
+
+// pyramid.rs:429-437: axes z, x, y, each with its own sigma in mm; every axis needs 4 voxels.
+inline Image smooth_recursive_gaussian(const Image& vol, const Vec3& sigma_mm) {
+  HostVec<double> out = vol.data;
+  for (int axis : {2, 0, 1}) {
+    if (vol.grid.size[axis] < 4) invalid("smoothing", "every axis needs at least 4 voxels");
+    if (!(sigma_mm[axis] > 0.0)) invalid("smoothing", "sigma must be greater than zero");
+    deriche_axis(out, vol.grid.size, axis, deriche_coefficients(sigma_mm[axis], vol.grid.spacing[axis]));
+  }
+  return Image(vol.grid, std::move(out));
+}
+
+// pyramid.rs:441-465 `sigma_mm_per_axis`, `smooth_for_level`: sigma in voxels scales by each
+// axis's spacing; sigma <= 0 returns the image unchanged.
+inline Image smooth_for_level(const Image& vol, double sigma, SigmaUnits units) {
+  if (sigma <= 0.0) return vol;
+  const Vec3& s = vol.grid.spacing;
+  const Vec3 mm = units == SigmaUnits::Voxels ? Vec3{sigma * s[0], sigma * s[1], sigma * s[2]} : Vec3{sigma, sigma, sigma};
+  return smooth_recursive_gaussian(vol, mm);
 }
 
 }  // namespace fw::oracle

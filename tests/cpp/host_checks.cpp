@@ -93,13 +93,15 @@ static void shrink() {
   ok("shrink");
 }
 
-// pyramid.rs:544-584: unit DC gain, the requested variance, a preserved constant.
+// pyramid.rs:544-584: unit DC gain, the requested variance, a preserved constant (the oracle's
+// line filter); the product's coefficients, derived as polynomial products, against the oracle's
+// restatement of the reference's closed forms.
 static void deriche() {
   const Grid g({81, 4, 4}, {2.0, 1.0, 1.0}, {0, 0, 0}, kIdentity3);
   std::vector<double> d(g.num_voxels(), 0.0);
   d[g.linear_index(40, 1, 1)] = 1.0;
   HostVec<double> line(d.begin(), d.end());
-  deriche_axis(line, g.size, 0, Deriche::zero_order(6.0, 2.0));
+  fw::oracle::deriche_axis(line, g.size, 0, fw::oracle::deriche_coefficients(6.0, 2.0));
   double sum = 0, var = 0;
   for (size_t i = 0; i < 81; ++i) {
     const double w = line[g.linear_index(i, 1, 1)];
@@ -121,8 +123,36 @@ static void deriche() {
           "deriche symmetry");
   const Grid c({4, 4, 4}, {1, 1, 1}, {0, 0, 0}, kIdentity3);
   Image k(c, std::vector<double>(64, 7.0));
-  Image s = smooth_recursive_gaussian(k, {1.5, 1.5, 1.5});
+  Image s = fw::oracle::smooth_recursive_gaussian(k, {1.5, 1.5, 1.5});
   for (double v : s.data) CHECK(std::fabs(v - 7.0) < 1e-9, "deriche constant");
+  // coefficients over sigma 0.3 .. 40 voxels. The reference's closed forms lose digits as the
+  // poles approach 1: D(1) = 1 + d1 + d2 + d3 + d4 cancels terms up to cond = (1 + sum |d_k|) / |D(1)|
+  // times larger, so the bound is 1e-14 of the largest coefficient times cond (cond is 1.0 at
+  // sigma 0.3 voxels, 2.7e6 at 40). A dropped term, a wrong sign or a swapped pair of the fit
+  // constants moves some coefficient by more than 1e-4 of the largest.
+  for (double sd : {0.3, 0.5, 0.8, 1.0, 1.7, 2.5, 4.0, 6.5, 10.0, 17.0, 40.0}) {
+    for (double spacing : {0.7, 1.0, 2.0}) {
+      const Deriche p = Deriche::zero_order(sd * spacing, spacing);
+      const fw::oracle::DericheCoefs o = fw::oracle::deriche_coefficients(sd * spacing, spacing);
+      const double* pv[5] = {p.n, p.d, p.m, p.bn, p.bm};
+      const double* ov[5] = {o.n, o.d, o.m, o.bn, o.bm};
+      double scale = 0.0, abs_den = 1.0, den = 1.0;
+      for (int q = 0; q < 4; ++q) {
+        abs_den += std::fabs(o.d[q]);
+        den += o.d[q];
+      }
+      const double cond = abs_den / std::fabs(den);
+      for (int f = 0; f < 5; ++f)
+        for (int q = 0; q < 4; ++q) scale = std::max(scale, std::fabs(ov[f][q]));
+      for (int f = 0; f < 5; ++f)
+        for (int q = 0; q < 4; ++q)
+          CHECK(std::fabs(pv[f][q] - ov[f][q]) <= 1e-14 * cond * scale, "deriche coefficients");
+      // unit DC gain of the two recursions together, from the product's own coefficients
+      const double dc = (p.n[0] + p.n[1] + p.n[2] + p.n[3] + p.m[0] + p.m[1] + p.m[2] + p.m[3]) /
+                        (1.0 + p.d[0] + p.d[1] + p.d[2] + p.d[3]);
+      CHECK(std::fabs(dc - 1.0) <= 1e-14 * cond, "deriche coefficients dc gain");
+    }
+  }
   ok("deriche");
 }
 
