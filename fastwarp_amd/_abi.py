@@ -74,6 +74,21 @@ class Affine(C.Structure):
         return m @ (np.asarray(p, dtype=np.float64) - c) + c + np.array(list(self.translation))
 
 
+FW_CHAIN_AFFINE = 0
+FW_CHAIN_FIELD = 1
+
+
+class ChainElement(C.Structure):
+    """fw_chain_element: one element of an ITK transform chain (an affine or a displacement field)."""
+
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("affine", Affine),
+        ("grid", Grid),
+        ("values", C.POINTER(C.c_double)),
+    ]
+
+
 class InvertParams(C.Structure):
     _fields_ = [
         ("max_iterations", C.c_uint32),

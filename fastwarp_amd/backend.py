@@ -14,8 +14,8 @@ from pathlib import Path
 
 import numpy as np
 
-from ._abi import (Affine, Capacity, Grid, InvertParams, InvertStats, LinearTransform, MiBinning, Plan, Result, ServedTiming,
-                   TraceStep, aos, dptr, read_trace, u32ptr)
+from ._abi import (FW_CHAIN_AFFINE, FW_CHAIN_FIELD, Affine, Capacity, ChainElement, Grid, InvertParams, InvertStats,
+                   LinearTransform, MiBinning, Plan, Result, ServedTiming, TraceStep, aos, dptr, read_trace, u32ptr)
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfastwarp_hip.so"
 
@@ -116,6 +116,9 @@ _SIGNATURES = [
     ("fw_resample_isotropic", C.c_int, [_P, _P, C.c_double, _PP]),
     ("fw_write_affine_mat", C.c_int, [C.c_char_p, C.POINTER(Affine)]),
     ("fw_read_affine_mat", C.c_int, [C.c_char_p, C.POINTER(Affine)]),
+    ("fw_write_composite_h5", C.c_int, [C.c_char_p, C.POINTER(ChainElement), C.c_uint32]),
+    ("fw_read_composite_h5", C.c_int, [C.c_char_p, C.POINTER(ChainElement), C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("fw_write_composite_pair", C.c_int, [C.c_char_p, C.POINTER(Affine), C.POINTER(Grid), _D, _D]),
     ("fw_server_create", C.c_int, [C.c_int32, C.c_uint32, C.c_uint64, _PP]),
     ("fw_server_destroy", None, [_P]),
     ("fw_server_last_error", C.c_char_p, []),
@@ -143,6 +146,83 @@ def read_affine_mat(path) -> Affine:
     if lib().fw_read_affine_mat(str(path).encode(), C.byref(out)) != FW_OK:
         raise RegError(FW_ERR_INVALID, lib().fw_last_error(None).decode())
     return out
+
+
+def _field_array(grid: Grid, values) -> np.ndarray:
+    n = int(grid.size[0]) * int(grid.size[1]) * int(grid.size[2])
+    a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+    if a.size != 3 * n:
+        raise ValueError(f"a field on this grid needs {3 * n} values, got {a.size}")
+    return a
+
+
+def write_composite_h5(path, chain) -> None:
+    """An ITK HDF5 transform file (itk_h5.rs:115-199). `chain` lists, in queue order, Affine
+    objects and (Grid, values) pairs, values being 3 per voxel ((x, y, z) displacements, x fastest)."""
+    elems = (ChainElement * max(1, len(chain)))()
+    keep = []
+    for i, item in enumerate(chain):
+        if isinstance(item, Affine):
+            elems[i].kind = FW_CHAIN_AFFINE
+            elems[i].affine = item
+        else:
+            grid, values = item
+            a = _field_array(grid, values)
+            keep.append(a)
+            elems[i].kind = FW_CHAIN_FIELD
+            elems[i].grid = grid
+            elems[i].values = a.ctypes.data_as(C.POINTER(C.c_double))
+    if lib().fw_write_composite_h5(str(path).encode(), elems, len(chain)) != FW_OK:
+        raise RegError(FW_ERR_INVALID, lib().fw_last_error(None).decode())
+
+
+def read_composite_h5(path) -> list:
+    """The chain of an ITK HDF5 transform file in queue order (itk_h5.rs:54-110): Affine objects
+    and (Grid, values) pairs, values of shape (voxels, 3)."""
+    count = C.c_uint32()
+    name = str(path).encode()
+
+    def call(elems, capacity):
+        if lib().fw_read_composite_h5(name, elems, capacity, C.byref(count)) != FW_OK:
+            raise RegError(FW_ERR_INVALID, lib().fw_last_error(None).decode())
+
+    call(None, 0)
+    n = count.value
+    elems = (ChainElement * n)()
+    call(elems, n)
+    buffers = {}
+    for i in range(n):
+        if elems[i].kind == FW_CHAIN_FIELD:
+            g = elems[i].grid
+            buffers[i] = np.empty((int(g.size[0]) * int(g.size[1]) * int(g.size[2]), 3), dtype=np.float64)
+            elems[i].values = buffers[i].ctypes.data_as(C.POINTER(C.c_double))
+    if buffers:
+        call(elems, n)
+    out = []
+    for i in range(n):
+        if elems[i].kind == FW_CHAIN_FIELD:
+            g = Grid()
+            C.memmove(C.byref(g), C.byref(elems[i].grid), C.sizeof(Grid))
+            out.append((g, buffers[i]))
+        else:
+            a = Affine()
+            C.memmove(C.byref(a), C.byref(elems[i].affine), C.sizeof(Affine))
+            out.append(a)
+    return out
+
+
+def write_composite_pair(prefix, affine: Affine, grid: Grid | None = None, field=None, inverse_field=None) -> None:
+    """<prefix>Composite.h5 = [affine, field] and <prefix>InverseComposite.h5 = [inverse_field,
+    affine^-1], or [affine] / [affine^-1] without a grid (ants_io.rs:100-141)."""
+    if grid is None:
+        rc = lib().fw_write_composite_pair(str(prefix).encode(), C.byref(affine), None, None, None)
+    else:
+        f = _field_array(grid, field)
+        b = _field_array(grid, inverse_field)
+        rc = lib().fw_write_composite_pair(str(prefix).encode(), C.byref(affine), C.byref(grid),
+                                           f.ctypes.data_as(_D), b.ctypes.data_as(_D))
+    if rc != FW_OK:
+        raise RegError(FW_ERR_INVALID, lib().fw_last_error(None).decode())
 
 
 def plan_capacity(plan: Plan, fixed_grid: Grid, moving_grid: Grid | None = None) -> Capacity:

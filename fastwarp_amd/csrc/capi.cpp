@@ -9,6 +9,7 @@
 #include "hip/backend.hpp"
 #include "hip/batch.hpp"
 #include "host/capacity.hpp"
+#include "host/itk_h5.hpp"
 #include "host/itk_mat.hpp"
 #include "plan.hpp"
 
@@ -645,6 +646,98 @@ int fw_read_affine_mat(const char* path, fw_affine* out) {
     g_create_error = e.what();
     return FW_ERR_INVALID;
   }
+}
+
+}  // extern "C"
+
+namespace {
+
+void affine_to(const fw::AffineTransform& t, fw_affine* out) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) out->matrix[3 * r + c] = t.matrix[r][c];
+    out->translation[r] = t.translation[r];
+    out->center[r] = t.center[r];
+  }
+}
+
+std::vector<float> field_values(const Grid& g, const double* v) {
+  need(v, "field values");
+  std::vector<float> out(3 * g.num_voxels());
+  for (size_t i = 0; i < out.size(); ++i) out[i] = float(v[i]);
+  return out;
+}
+
+template <typename F>
+int host_guard(F&& f) {
+  try {
+    f();
+    return FW_OK;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return FW_ERR_INVALID;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fw_write_composite_h5(const char* path, const fw_chain_element* elements, uint32_t count) {
+  return host_guard([&] {
+    need(path, "path");
+    if (count == 0) fw::invalid("ITK HDF5 transform", "cannot write an empty chain");
+    need(elements, "elements");
+    std::vector<fw::ChainElement> chain(count);
+    for (uint32_t i = 0; i < count; ++i) {
+      const fw_chain_element& e = elements[i];
+      if (e.kind == FW_CHAIN_AFFINE) {
+        chain[i].affine = affine_from(&e.affine);
+      } else if (e.kind == FW_CHAIN_FIELD) {
+        chain[i].is_field = true;
+        chain[i].grid = grid_from(&e.grid);
+        chain[i].field = field_values(chain[i].grid, e.values);
+      } else {
+        fw::invalid("ITK HDF5 transform", "unknown chain element kind " + std::to_string(e.kind));
+      }
+    }
+    fw::write_composite_h5(path, chain);
+  });
+}
+
+int fw_read_composite_h5(const char* path, fw_chain_element* elements, uint32_t capacity, uint32_t* count) {
+  return host_guard([&] {
+    need(path, "path");
+    need(count, "count");
+    if (capacity > 0) need(elements, "elements");
+    const std::vector<fw::ChainElement> chain = fw::read_composite_h5(path);
+    *count = uint32_t(chain.size());
+    for (size_t i = 0; i < chain.size() && i < capacity; ++i) {
+      fw_chain_element& out = elements[i];
+      const fw::ChainElement& e = chain[i];
+      out.kind = e.is_field ? FW_CHAIN_FIELD : FW_CHAIN_AFFINE;
+      if (!e.is_field) {
+        affine_to(e.affine, &out.affine);
+        continue;
+      }
+      grid_to(e.grid, &out.grid);
+      if (out.values)
+        for (size_t k = 0; k < e.field.size(); ++k) out.values[k] = double(e.field[k]);
+    }
+  });
+}
+
+int fw_write_composite_pair(const char* prefix, const fw_affine* affine, const fw_grid* grid, const double* field,
+                            const double* inverse_field) {
+  return host_guard([&] {
+    need(prefix, "prefix");
+    need(affine, "affine");
+    if (!grid) {
+      fw::write_composite_pair(prefix, affine_from(affine), nullptr, {}, {});
+      return;
+    }
+    const Grid g = grid_from(grid);
+    fw::write_composite_pair(prefix, affine_from(affine), &g, field_values(g, field), field_values(g, inverse_field));
+  });
 }
 
 void fw_plan_cohort(fw_plan* plan, int32_t channels) {

@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._abi import Affine, Grid, Plan, Result, TraceStep, dptr, read_trace
-from .backend import FW_OK, DeviceField, DeviceImage, HipBackend, RegError, lib
+from .backend import FW_OK, DeviceField, DeviceImage, HipBackend, RegError, lib, write_composite_pair
 
 
 def cohort_plan(channels: int = 1) -> Plan:
@@ -34,6 +34,17 @@ class Registration:
     forward: np.ndarray | None  # (n, 3) on the fixed grid
     inverse: np.ndarray | None
     result: Result
+
+
+def write_composite(reg: Registration, fixed_grid: Grid, prefix) -> None:
+    """A registration's transforms as antsRegistration writes them with --write-composite-transform
+    (ants_io.rs:100-141): <prefix>Composite.h5 = [affine, forward field] and
+    <prefix>InverseComposite.h5 = [inverse field, affine^-1]; [affine] / [affine^-1] when the plan
+    had no SyN stage or the fields were not downloaded."""
+    if reg.forward is None or reg.inverse is None:
+        write_composite_pair(prefix, reg.affine)
+    else:
+        write_composite_pair(prefix, reg.affine, fixed_grid, reg.forward, reg.inverse)
 
 
 def _sig():

@@ -423,6 +423,38 @@ int32_t fw_last_batch_trace(uint32_t subject, fw_trace_step* out, int32_t capaci
 int fw_write_affine_mat(const char* path, const fw_affine* affine);
 int fw_read_affine_mat(const char* path, fw_affine* out);
 
+/* ITK HDF5 transform files, Composite.h5 and InverseComposite.h5 as antsRegistration writes them
+ * (crates/fastwarp-reg/src/transform/itk_h5.rs:115-199 format_h5 / write_h5_file / read_h5_file,
+ * on the HDF5 subset of transform/hdf5.rs): group TransformGroup/0 is the composite, members
+ * 1, 2, ... the chain in queue order. An element is an affine (AffineTransform_double_3_3) or a
+ * displacement field (DisplacementFieldTransform_double_3_3: its grid, and 3 values per voxel,
+ * (x, y, z) displacements in mm with x fastest, as fw_field_download gives them; held and read as
+ * f32 and written as f64, as the reference holds fields). Host only, no device; on failure
+ * fw_last_error(NULL) gives the text. */
+enum { FW_CHAIN_AFFINE = 0, FW_CHAIN_FIELD = 1 };
+typedef struct {
+  int32_t kind;     /* FW_CHAIN_AFFINE or FW_CHAIN_FIELD */
+  fw_affine affine; /* an affine element */
+  fw_grid grid;     /* a field element's grid */
+  double* values;   /* a field element's 3 * voxels values: read when writing, filled when reading
+                       (NULL: not filled) */
+} fw_chain_element;
+/* write_h5_file (itk_h5.rs:189-199): `count` >= 1 elements in queue order. */
+int fw_write_composite_h5(const char* path, const fw_chain_element* elements, uint32_t count);
+/* read_h5_file (itk_h5.rs:54-110): members in numeric order; the composite's own group gives no
+ * element; affine, MatrixOffsetTransformBase, Rigid3D, Euler3D and translation classes become
+ * affines (itk_params.rs:58-102). *count receives the chain's length; the first `capacity`
+ * elements are filled, and a field element's values are copied where its `values` pointer (set
+ * by the caller, 3 * voxels doubles) is not NULL: call once with capacity 0 for the length, once
+ * to learn the grids, and once with the buffers. */
+int fw_read_composite_h5(const char* path, fw_chain_element* elements, uint32_t capacity, uint32_t* count);
+/* write_composite_pair with CompositePair::new / ::linear (ants_io.rs:100-141):
+ * <prefix>Composite.h5 = [affine, field] and <prefix>InverseComposite.h5 = [inverse_field,
+ * affine^-1]; with grid NULL (a linear-only registration) [affine] and [affine^-1]. The prefix is
+ * used as given (a prefix ending in '/' names a folder). A singular affine is refused. */
+int fw_write_composite_pair(const char* prefix, const fw_affine* affine, const fw_grid* grid, const double* field,
+                            const double* inverse_field);
+
 /* ---- Serving: continuous batching of whole registrations on one device ------------------------
  * The reference runs several registrations on one GPU as independent device instances, as many
  * as its capacity model fits at once (crates/fastwarp-cli/src/engine.rs:1015-1068

@@ -1,6 +1,7 @@
 // Checks of the product's host code (the C++ drivers' building blocks) against the reference's
 // published values and closed forms. Built and run by tests/test_host.py; prints "ok <name>" per
 // passing check and exits non-zero on the first failure.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -8,12 +9,18 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <random>
+#include <sstream>
 #include <thread>
+
+#include <unistd.h>
 
 #include "../../fastwarp_amd/csrc/hip/fastdiv.hpp"
 #include "../../fastwarp_amd/csrc/host/admission.hpp"
 #include "../../fastwarp_amd/csrc/host/gauss_taps.hpp"
+#include "../../fastwarp_amd/csrc/host/hdf5.hpp"
+#include "../../fastwarp_amd/csrc/host/itk_h5.hpp"
 #include "../../fastwarp_amd/csrc/host/linear.hpp"
 #include "../../fastwarp_amd/csrc/host/mi_host.hpp"
 #include "../../fastwarp_amd/csrc/host/preprocess.hpp"
@@ -91,6 +98,152 @@ static void shrink() {
   const Grid one = shrink_grid(Grid({7, 6, 5}, {1, 2, 3}, {1, 2, 3}, kIdentity3), {100, 100, 100});
   CHECK(one.size[0] == 1 && one.size[1] == 1 && one.size[2] == 1, "collapse to one voxel");
   ok("shrink");
+}
+
+// The HDF5 container (hdf5.rs:1465-1560 cases): lookup3's published answers; the ITK file written
+// by libhdf5 1.12.1 (tests/golden/itk/ctest.h5) read with every checksum verified; a tree of
+// groups, strings and f64 values written and read back bit for bit, with wide size fields for a
+// long name and a large dataset; damaged files refused.
+static void hdf5_container() {
+  const std::string four = "Four score and seven years ago";
+  CHECK(h5::lookup3(reinterpret_cast<const uint8_t*>(""), 0) == 0xdeadbeefu, "lookup3 empty");
+  CHECK(h5::lookup3(reinterpret_cast<const uint8_t*>(four.data()), four.size()) == 0x17770551u, "lookup3 four score");
+  std::string here = __FILE__;
+  const std::string golden = here.substr(0, here.rfind('/')) + "/../golden/itk/ctest.h5";
+  const h5::File f = h5::File::open(golden);
+  const std::vector<std::string> root = f.members("");
+  CHECK(std::find(root.begin(), root.end(), "TransformGroup") != root.end(), "ctest root group");
+  CHECK(f.read_string("ITKVersion").rfind("5", 0) == 0, "ctest ITK version");
+  CHECK((f.members("TransformGroup") == std::vector<std::string>{"0", "1", "2"}), "ctest members");
+  CHECK(f.read_string("TransformGroup/0/TransformType") == "CompositeTransform_double_3_3", "ctest composite");
+  CHECK(f.read_string("TransformGroup/1/TransformType") == "AffineTransform_double_3_3", "ctest affine");
+  CHECK(f.read_doubles("TransformGroup/1/TransformParameters").size() == 12, "ctest affine parameters");
+  CHECK(f.read_doubles("TransformGroup/2/TransformFixedParameters").size() == 18, "ctest field fixed parameters");
+  CHECK(!f.exists("TransformGroup/3"), "ctest has three members");
+  CHECK(f.is_group("TransformGroup/1") && !f.is_group("TransformGroup/1/TransformType"), "ctest groups");
+
+  const std::vector<double> vals = {0.1, -0.0, std::numeric_limits<double>::denorm_min(), 1e300, -1234.5678901234567};
+  std::vector<float> widen = {1.5f, -0.0f, std::numeric_limits<float>::denorm_min(), 3.0e38f};
+  std::vector<double> big(5000);
+  for (size_t i = 0; i < big.size(); ++i) big[i] = double(i) * 0.25 - 7.0;
+  const std::string long_name(300, 'x');
+  std::ostringstream out;
+  h5::write(out, {h5::Node::string("Name", "hello"),
+                  h5::Node::group("G", {h5::Node::values("v", vals), h5::Node::values("empty", {}),
+                                        h5::Node::string("s", "world wide"),
+                                        h5::Node::widened("w", widen.data(), widen.size())}),
+                  h5::Node::values(long_name, big)});
+  const std::string bytes = out.str();
+  const h5::File g(std::vector<uint8_t>(bytes.begin(), bytes.end()));
+  CHECK((g.members("") == std::vector<std::string>{"Name", "G", long_name}), "written members");
+  CHECK(g.read_string("Name") == "hello" && g.read_string("G/s") == "world wide", "written strings");
+  const std::vector<double> back = g.read_doubles("G/v");
+  bool same = back.size() == vals.size();
+  for (size_t i = 0; same && i < vals.size(); ++i) same = std::memcmp(&back[i], &vals[i], 8) == 0;
+  CHECK(same, "written f64 values bit for bit");
+  const std::vector<double> wb = g.read_doubles("G/w");
+  same = wb.size() == widen.size();
+  for (size_t i = 0; same && i < widen.size(); ++i) {
+    const double d = double(widen[i]);
+    same = std::memcmp(&wb[i], &d, 8) == 0;
+  }
+  CHECK(same, "widened f32 values");
+  CHECK(g.read_doubles("G/empty").empty() && g.read_doubles(long_name) == big, "empty and large datasets");
+  bool refused = false;
+  try {
+    g.read_doubles("Name");
+  } catch (const Error&) {
+    refused = true;
+  }
+  CHECK(refused, "a string is not a float dataset");
+  // a flipped byte in any object header or in the superblock fails its checksum
+  const auto fails = [](std::vector<uint8_t> b) {
+    try {
+      const h5::File x(std::move(b));
+      x.members("");
+      x.members("TransformGroup");
+      for (const char* m : {"0", "1", "2"}) x.read_string(std::string("TransformGroup/") + m + "/TransformType");
+      return false;
+    } catch (const Error&) {
+      return true;
+    }
+  };
+  std::ifstream in(golden, std::ios::binary);
+  const std::vector<uint8_t> ctest((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  CHECK(!fails(ctest), "ctest unchanged reads");
+  for (size_t at : {size_t(20), size_t(60)}) {
+    std::vector<uint8_t> bad = ctest;
+    bad[at] ^= 0x40;
+    CHECK(fails(bad), "a flipped byte is refused");
+  }
+  CHECK(fails(std::vector<uint8_t>(ctest.begin(), ctest.begin() + 40)), "a truncated file is refused");
+  CHECK(fails(std::vector<uint8_t>(10, 0)), "not an HDF5 file");
+  ok("hdf5_container");
+}
+
+// itk_h5.rs:205-260: the ITK file is [affine, field] as its values say; a chain written by the
+// product reads back with the same bits; an unknown class and a non-numeric member are refused.
+static void itk_composite() {
+  std::string here = __FILE__;
+  const std::string golden = here.substr(0, here.rfind('/')) + "/../golden/itk/ctest.h5";
+  const std::vector<ChainElement> c = read_composite_h5(golden);
+  CHECK(c.size() == 2 && !c[0].is_field && c[1].is_field, "ctest chain");
+  CHECK(c[0].affine.matrix[0][0] == 2.0 && c[0].affine.matrix[1][1] == 1.0 && c[0].affine.matrix[2][2] == 1.0 &&
+            c[0].affine.matrix[0][1] == 0.0 && c[0].affine.translation[0] == 0.0 && c[0].affine.center[2] == 0.0,
+        "ctest affine values");
+  CHECK(c[1].grid.size[0] == 6 && c[1].grid.size[1] == 5 && c[1].grid.size[2] == 4 && c[1].field.size() == 360,
+        "ctest field grid");
+  bool unit = true;
+  for (size_t v = 0; v < 120; ++v) unit = unit && c[1].field[3 * v] == 1.0f && c[1].field[3 * v + 1] == 0.0f && c[1].field[3 * v + 2] == 0.0f;
+  CHECK(unit, "ctest field values");
+
+  const std::string tmp = "/tmp/fw_host_checks_" + std::to_string(::getpid()) + "_Composite.h5";
+  std::vector<ChainElement> chain(2);
+  chain[0].affine.matrix = {{{1.01, 0.02, 0.0}, {-0.02, 0.99, 0.01}, {0.0, 0.0, 1.0}}};
+  chain[0].affine.translation = {1.0, -2.0, 0.5};
+  chain[0].affine.center = {3.0, 4.0, 5.0};
+  chain[1].is_field = true;
+  chain[1].grid = Grid({5, 6, 7}, {2.0, 2.5, 3.0}, {-4.0, -5.0, -6.0}, {{{0.0, -1.0, 0.0}, {1.0, 0.0, 0.0}, {0.0, 0.0, 1.0}}});
+  chain[1].field.resize(3 * 5 * 6 * 7);
+  for (size_t i = 0; i < chain[1].field.size(); ++i) chain[1].field[i] = float(std::sin(0.37 * double(i)) * 3.0);
+  write_composite_h5(tmp, chain);
+  const std::vector<ChainElement> back = read_composite_h5(tmp);
+  CHECK(back.size() == 2 && !back[0].is_field && back[1].is_field, "written chain kinds");
+  CHECK(back[0].affine.parameters() == chain[0].affine.parameters() && back[0].affine.center == chain[0].affine.center,
+        "written affine bits");
+  CHECK(back[1].grid.size == chain[1].grid.size && back[1].grid.spacing == chain[1].grid.spacing &&
+            back[1].grid.origin == chain[1].grid.origin && back[1].grid.direction == chain[1].grid.direction,
+        "written field grid");
+  CHECK(back[1].field == chain[1].field, "written field bits");
+  const h5::File w = h5::File::open(tmp);
+  CHECK((w.members("") == std::vector<std::string>{"HDFVersion", "ITKVersion", "OSName", "OSVersion", "TransformGroup"}),
+        "written root layout");
+  CHECK(w.read_string("TransformGroup/0/TransformType") == "CompositeTransform_double_3_3" &&
+            w.read_string("TransformGroup/2/TransformType") == "DisplacementFieldTransform_double_3_3",
+        "written type names");
+  const std::vector<double> fixed = w.read_doubles("TransformGroup/2/TransformFixedParameters");
+  CHECK(fixed.size() == 18 && fixed[0] == 5.0 && fixed[3] == -4.0 && fixed[6] == 2.0 && fixed[9 + 1] == -1.0,
+        "field fixed parameters: size, origin, spacing, direction row-major");
+  // an unknown class and a non-numeric member
+  const auto refused = [&](std::vector<h5::Node> members) {
+    h5::write_file(tmp, {h5::Node::group("TransformGroup", std::move(members))});
+    try {
+      read_composite_h5(tmp);
+      return false;
+    } catch (const Error&) {
+      return true;
+    }
+  };
+  CHECK(refused({h5::Node::group("1", {h5::Node::string("TransformType", "BSplineTransform_double_3_3"),
+                                       h5::Node::values("TransformFixedParameters", {0.0}),
+                                       h5::Node::values("TransformParameters", {0.0})})}),
+        "unknown class refused");
+  CHECK(refused({h5::Node::group("x", {h5::Node::string("TransformType", "CompositeTransform_double_3_3")})}),
+        "non-numeric member refused");
+  CHECK(refused({h5::Node::group("0", {h5::Node::string("TransformType", "CompositeTransform_double_3_3")})}),
+        "a file without a transform refused");
+  std::remove(tmp.c_str());
+  ok("itk_composite");
 }
 
 // pyramid.rs:544-584: unit DC gain, the requested variance, a preserved constant (the oracle's
@@ -838,6 +991,8 @@ int main() {
   regular_sampling();
   shrink();
   deriche();
+  hdf5_container();
+  itk_composite();
   convergence();
   wait_timeout();
   tuning();

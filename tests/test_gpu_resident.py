@@ -13,9 +13,9 @@ import numpy as np
 import pytest
 
 import oracle
-from fastwarp_amd.backend import RegError
+from fastwarp_amd.backend import RegError, read_composite_h5
 from fastwarp_amd.phantom import make_pair
-from fastwarp_amd.register import cohort_plan, register, register_resident
+from fastwarp_amd.register import cohort_plan, register, register_resident, write_composite
 from tests import cases
 
 pytestmark = pytest.mark.gpu
@@ -65,13 +65,21 @@ def test_device_centre_of_gravity_matches_host(backend, name):
     assert np.max(np.abs(got - want)) <= 1e-4, (got, want)
 
 
-def test_resident_registration_matches_the_host_path(backend):
+def test_resident_registration_matches_the_host_path(backend, tmp_path):
     pair = make_pair((48, 52, 44), 2.0, seed=9, disp_max_mm=3.0)
     fixed, moving = [f32(pair.fixed[0])], [f32(pair.moving[0])]
     plan = cohort_plan(1)
     plan.set_linear([30, 15, 8], [4, 2, 1], [2.0, 1.0, 0.0])
     plan.set_syn([10, 6, 4], [4, 2, 1], [2.0, 1.0, 0.0])
     host = register(backend, plan, pair.grid, fixed, pair.grid, moving)
+    # the ANTs composite files of the result hold its affine and fields bit for bit
+    write_composite(host, pair.grid, tmp_path / "sub_")
+    forward = read_composite_h5(tmp_path / "sub_Composite.h5")
+    inverse = read_composite_h5(tmp_path / "sub_InverseComposite.h5")
+    assert list(forward[0].matrix) == list(host.affine.matrix)
+    assert list(forward[0].translation) == list(host.affine.translation)
+    assert np.array_equal(forward[1][1], host.forward) and np.array_equal(inverse[0][1], host.inverse)
+    assert tuple(forward[1][0].size) == tuple(pair.grid.size) and len(inverse) == 2
     imgs_f = [backend.upload_image(pair.grid, fixed[0])]
     imgs_m = [backend.upload_image(pair.grid, moving[0])]
     dev = register_resident(backend, plan, imgs_f, imgs_m)
