@@ -646,6 +646,7 @@ class File {
     if (lclass == 0) {
       const uint64_t size = uint(layout->at + 2, 2);
       if (size < bytes) invalid(kWhat, "compact dataset shorter than its extent");
+      if (4 + size > layout->size) invalid(kWhat, "compact data overruns its message");
       ds.data = layout->at + 4;
     } else if (lclass == 1) {
       const uint64_t at = uint(layout->at + 2, off_), size = uint(layout->at + 2 + off_, len_);
