@@ -195,3 +195,22 @@ def test_linear_only_pair_and_refusals(tmp_path):
     (tmp_path / "notes.h5").write_bytes(b"not an hdf5 file at all, just text" * 4)
     with pytest.raises(RegError, match="signature"):
         read_composite_h5(tmp_path / "notes.h5")
+
+
+def test_a_1mm_field_pair_streams_through_the_writer(tmp_path):
+    """The 1 mm grid of the plain workload (182 x 218 x 182, 7.2 M voxels): each file holds 174 MB
+    of f64 parameters, widened from the field's f32 values in blocks as they are written; both
+    files read back with every value, and the metadata's lengths take 8-byte fields."""
+    g = Grid.make((182, 218, 182), (1.0, 1.0, 1.0), (-90.0, -126.0, -72.0))
+    n = 182 * 218 * 182
+    rng = np.random.default_rng(17)
+    fwd = rng.normal(0.0, 2.0, (n, 3)).astype(np.float32).astype(np.float64)
+    inv = -fwd
+    write_composite_pair(tmp_path / "big_", sample(), g, fwd, inv)
+    size = (tmp_path / "big_Composite.h5").stat().st_size
+    assert size >= 3 * n * 8 and size < 3 * n * 8 + (1 << 16)
+    forward = read_composite_h5(tmp_path / "big_Composite.h5")
+    assert np.array_equal(forward[1][1], fwd)
+    del forward
+    inverse = read_composite_h5(tmp_path / "big_InverseComposite.h5")
+    assert np.array_equal(inverse[0][1], inv)
