@@ -16,13 +16,17 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iterator>
 #include <string>
 #include <utility>
 #include <vector>
+
+#include <unistd.h>
 
 #include "common.hpp"
 
@@ -390,12 +394,25 @@ inline void write(std::ostream& out, const std::vector<Node>& root) {
   w.layout.write(out);
 }
 
+// Written to a temporary file beside `path` and renamed over it once complete, so a failed write
+// leaves no partial file (the reference's fs_write.rs `write_complete`); the temporary file is
+// removed on failure.
 inline void write_file(const std::string& path, const std::vector<Node>& root) {
-  std::ofstream f(path, std::ios::binary | std::ios::trunc);
-  if (!f) invalid("io", "cannot open " + path + " for writing");
-  write(f, root);
-  f.flush();
-  if (!f) invalid("io", "cannot write " + path);
+  static std::atomic<unsigned> serial{0};
+  const std::string tmp = path + ".partial-" + std::to_string(::getpid()) + "-" + std::to_string(serial++);
+  try {
+    {
+      std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+      if (!f) invalid("io", "cannot open " + tmp + " for writing");
+      write(f, root);
+      f.flush();
+      if (!f) invalid("io", "cannot write " + tmp);
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) invalid("io", "cannot rename " + tmp + " to " + path);
+  } catch (...) {
+    std::remove(tmp.c_str());
+    throw;
+  }
 }
 
 // ------------------------------------------------------------------------------- reading --

@@ -214,3 +214,16 @@ def test_a_1mm_field_pair_streams_through_the_writer(tmp_path):
     del forward
     inverse = read_composite_h5(tmp_path / "big_InverseComposite.h5")
     assert np.array_equal(inverse[0][1], inv)
+
+
+def test_composite_files_are_written_whole_or_not_at_all(tmp_path):
+    # fs_write.rs write_complete: a temporary file beside the target, renamed once complete
+    a = sample()
+    with pytest.raises(RegError, match="cannot open"):
+        write_composite_h5(tmp_path / "missing" / "x.h5", [a])
+    assert not (tmp_path / "missing").exists()
+    path = tmp_path / "c.h5"
+    path.write_bytes(b"old")
+    write_composite_h5(path, [a, (field_grid(), field_values(6))])
+    assert len(read_composite_h5(path)) == 2
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["c.h5"]
