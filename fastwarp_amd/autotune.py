@@ -38,6 +38,7 @@ CANDIDATES = {
     "inv": [("inv.max_blocks", v) for v in (512, 1024, 2048, 4096, 8192)],
     "mi": [("mi.threads", t, "mi.max_blocks", b) for t in (256, 512, 1024) for b in (16, 32, 256, 512, 1024, 2048)],
     "mi_deriv": [("mi.deriv_blocks", v) for v in (256, 512, 1024, 2048)],
+    "mi_batch": [("mi.batch_blocks", v) for v in (16, 32, 64, 128, 256, 512)],
 }
 
 
@@ -137,6 +138,10 @@ def main(argv=None) -> int:
         apply(b, won, scls, True)
         tune(b, "mi_deriv", scls, lambda: level.value_and_derivative(t), ["k_mi_sampled_deriv", "k_mi_deriv_finish"],
              args.repeats, table, log)
+        # the line search's speculative probes: four transforms in one pass
+        probes = [LinearTransform.make(0, [0.01 * q, -0.02, 0.015, 0.5, -0.3 * q, 0.2], centre) for q in range(4)]
+        tune(b, "mi_batch", scls, lambda: level.values(probes),
+             ["k_mi_sampled_hist_batch", "k_mi_sum_slots", "k_mi_finish", "k_mi_sum_finish"], args.repeats, table, log)
     out = {"_command": "python -m fastwarp_amd.autotune", "_device": b.name(),
            "_seconds": round(time.perf_counter() - t0, 1), **dict(sorted(table.items()))}
     Path(args.out).parent.mkdir(parents=True, exist_ok=True)

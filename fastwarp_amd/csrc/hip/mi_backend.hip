@@ -89,7 +89,10 @@ int MiLevel::pass_args(const LinearMovingTransform& t, MiSampledArgs& a, size_t 
   a.moving_max = float(binning_.moving_max);
   a.bins = int(binning_.bins);
   const size_t cells = binning_.bins * binning_.bins + binning_.bins + 1;
-  const int blocks = mi_sampled_blocks(int(count_), int(ctx_->tuning().get("mi.max_blocks", count_, 512)));
+  // a batched pass bins every sample under each transform, so its blocks (and their slots) have a
+  // cap of their own (tuning knob mi.batch_blocks; the single pass's cap by default)
+  const uint64_t cap = ctx_->tuning().get("mi.max_blocks", count_, 512);
+  const int blocks = mi_sampled_blocks(int(count_), int(histograms > 1 ? ctx_->tuning().get("mi.batch_blocks", count_, cap) : cap));
   a.threads = int(ctx_->tuning().get("mi.threads", count_, 0));
   a.hist_slots =
       static_cast<unsigned long long*>(ctx_->scratch(6, size_t(blocks) * histograms * cells * sizeof(unsigned long long)));

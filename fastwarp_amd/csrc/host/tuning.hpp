@@ -50,6 +50,7 @@ class TuningTable {
     if (knob == "inv.max_blocks") return v >= 1 && v <= 65535;
     if (knob == "mi.threads") return v == 0 || v == 256 || v == 512 || v == 1024;
     if (knob == "mi.max_blocks") return v >= 1 && v <= 65535;
+    if (knob == "mi.batch_blocks") return v >= 1 && v <= 65535;
     if (knob == "mi.deriv_blocks") return v >= 1 && v <= 65535;
     if (knob == "gauss.one_pass") return v == 0 || v == 1;
     if (knob == "deriche.lds") return v == 0 || v == 1;

@@ -103,6 +103,7 @@ int fw_release_idle(fw_ctx* ctx);
  * have fixed workgroup shapes): key "<knob>" or "<knob>@<class>", class = floor(log2(size)) of the
  * grid's voxels or the level's samples; a negative value removes the entry. Knobs: cc.tile_z
  * (1..256), cc.min_blocks, inv.max_blocks, mi.threads (256, 512, 1024), mi.max_blocks,
+ * mi.batch_blocks (the cap of the passes that bin several transforms; mi.max_blocks's by default),
  * mi.deriv_blocks, mi.fused_slots (passes of at most this many blocks add their slots and finish in
  * one launch; 0 never), gauss.one_pass (0, 1), deriche.lds (0, 1: x lines of the level smoothing
  * staged in LDS; default 1), batch.graphs (0, 1: the batched SyN replays each

@@ -340,6 +340,7 @@ static void tuning() {
   CHECK(TuningTable::valid("gauss.one_pass", 1) && !TuningTable::valid("gauss.one_pass", 2), "one pass");
   CHECK(TuningTable::valid("batch.graphs", 0) && !TuningTable::valid("batch.graphs", 2), "graphs");
   CHECK(TuningTable::valid("mi.fused_slots", 0) && !TuningTable::valid("mi.fused_slots", -1), "fused slots");
+  CHECK(TuningTable::valid("mi.batch_blocks", 32) && !TuningTable::valid("mi.batch_blocks", 0), "batch blocks");
   CHECK(TuningTable::valid("deriche.lds", 1) && !TuningTable::valid("deriche.lds", 2), "deriche lds");
   CHECK(!TuningTable::valid("cc.tile_z", 0) && !TuningTable::valid("nonsense", 1), "unknown knob");
   const char* path = "/tmp/fw_tuning_check.json";
