@@ -79,6 +79,15 @@ def test_mi_launch_shapes_give_the_same_histogram_cells(tuned):
         knobs(tuned, cls, cand, False)
         assert v == v0, cand
         assert np.max(np.abs(np.asarray(d) - np.asarray(d0))) <= 1e-9 * np.max(np.abs(d0)), cand
+    # the batched passes' block cap: the same integer cells, so the same values as single passes
+    centre = tuple(g.index_to_point(np.array(g.shape_xyz) / 2))
+    probes = [LinearTransform.make(0, [0.02 * q, -0.01, 0.03, 0.8, -0.5 * q, 0.4], centre) for q in range(4)]
+    single = [level.value(p) for p in probes]
+    for cand in CANDIDATES["mi_batch"]:
+        knobs(tuned, cls, cand)
+        batched = list(level.values(probes))
+        knobs(tuned, cls, cand, False)
+        assert batched == single, cand
 
 
 def test_inversion_blocks_and_gaussian_passes_change_results_only_by_rounding(tuned):
