@@ -326,7 +326,7 @@ def main() -> int:
                     help="the plain registration the CPU engine runs for the baseline (the 2 mm one by default)")
     ap.add_argument("--config", default="1mm", choices=["1mm", "2mm"])
     ap.add_argument("--kernels", action="store_true", help="time each hot-path kernel at 1 mm instead")
-    ap.add_argument("--slots", type=int, default=4, help="slots of the served stream (0 skips it)")
+    ap.add_argument("--slots", type=int, default=8, help="slots of the served stream (0 skips it)")
     ap.add_argument("--requests", type=int, default=256, help="requests of the served stream")
     ap.add_argument("--load", type=float, default=0.8, help="arrival rate of the served stream as a share of saturation")
     ap.add_argument("--batch", type=int, default=8, help="subjects of the batched two-channel workload (0 skips it)")
