@@ -157,7 +157,7 @@ def _field_array(grid: Grid, values) -> np.ndarray:
 
 
 def write_composite_h5(path, chain) -> None:
-    """An ITK HDF5 transform file (itk_h5.rs:115-199). `chain` lists, in queue order, Affine
+    """An ITK HDF5 transform file (itk_h5.rs:114-199). `chain` lists, in queue order, Affine
     objects and (Grid, values) pairs, values being 3 per voxel ((x, y, z) displacements, x fastest)."""
     elems = (ChainElement * max(1, len(chain)))()
     keep = []

@@ -130,7 +130,7 @@ inline std::string os_version() {
 
 }  // namespace itk_h5
 
-// itk_h5.rs:115-186 `format_h5` / `write_h5_file`: group 0 is the composite, groups 1.. the
+// itk_h5.rs:114-199 `format_h5` / `write_h5_file`: group 0 is the composite, groups 1.. the
 // chain in queue order. An empty chain is refused.
 inline void write_composite_h5(const std::string& path, const std::vector<ChainElement>& chain) {
   using h5::Node;
@@ -158,8 +158,8 @@ inline void write_composite_h5(const std::string& path, const std::vector<ChainE
                         Node::group(itk_h5::kGroup, std::move(groups))});
 }
 
-// itk_h5.rs:54-105 `read_chain`: the members of TransformGroup in numeric order (a non-numeric
-// name is an error); a displacement field keeps its values in f32; a file without a transform
+// itk_h5.rs:54-110 `read_chain` / `read_h5_file`: the members of TransformGroup in numeric
+// order (a non-numeric name is an error); a displacement field keeps its values in f32; a file without a transform
 // is an error.
 inline std::vector<ChainElement> read_composite_h5(const std::string& path) {
   const h5::File f = h5::File::open(path);

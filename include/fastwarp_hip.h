@@ -425,7 +425,7 @@ int fw_write_affine_mat(const char* path, const fw_affine* affine);
 int fw_read_affine_mat(const char* path, fw_affine* out);
 
 /* ITK HDF5 transform files, Composite.h5 and InverseComposite.h5 as antsRegistration writes them
- * (crates/fastwarp-reg/src/transform/itk_h5.rs:115-199 format_h5 / write_h5_file / read_h5_file,
+ * (crates/fastwarp-reg/src/transform/itk_h5.rs:54-199 read_h5_file / format_h5 / write_h5_file,
  * on the HDF5 subset of transform/hdf5.rs): group TransformGroup/0 is the composite, members
  * 1, 2, ... the chain in queue order. An element is an affine (AffineTransform_double_3_3) or a
  * displacement field (DisplacementFieldTransform_double_3_3: its grid, and 3 values per voxel,
@@ -440,7 +440,7 @@ typedef struct {
   double* values;   /* a field element's 3 * voxels values: read when writing, filled when reading
                        (NULL: not filled) */
 } fw_chain_element;
-/* write_h5_file (itk_h5.rs:189-199): `count` >= 1 elements in queue order. */
+/* write_h5_file (itk_h5.rs:114-199): `count` >= 1 elements in queue order. */
 int fw_write_composite_h5(const char* path, const fw_chain_element* elements, uint32_t count);
 /* read_h5_file (itk_h5.rs:54-110): members in numeric order; the composite's own group gives no
  * element; affine, MatrixOffsetTransformBase, Rigid3D, Euler3D and translation classes become
