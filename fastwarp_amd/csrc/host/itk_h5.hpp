@@ -159,8 +159,8 @@ inline void write_composite_h5(const std::string& path, const std::vector<ChainE
 }
 
 // itk_h5.rs:54-110 `read_chain` / `read_h5_file`: the members of TransformGroup in numeric
-// order (a non-numeric name is an error); a displacement field keeps its values in f32; a file without a transform
-// is an error.
+// order (a non-numeric name is an error); a displacement field keeps its values in f32; a file
+// without a transform is an error.
 inline std::vector<ChainElement> read_composite_h5(const std::string& path) {
   const h5::File f = h5::File::open(path);
   std::vector<std::pair<unsigned long long, std::string>> names;
