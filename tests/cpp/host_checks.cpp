@@ -234,6 +234,35 @@ static void itk_composite() {
       return true;
     }
   };
+  // itk_params.rs:58-102: Euler3D (ITK's Rz Rx Ry, or Rz Ry Rx with the ZYX flag), translation and
+  // float forms become affines
+  {
+    const std::vector<double> euler = {0.1, -0.2, 0.3, 1.0, 2.0, 3.0};
+    h5::write_file(tmp, {h5::Node::group(
+                       "TransformGroup",
+                       {h5::Node::group("0", {h5::Node::string("TransformType", "CompositeTransform_double_3_3")}),
+                        h5::Node::group("1", {h5::Node::string("TransformType", "Euler3DTransform_double_3_3"),
+                                              h5::Node::values("TransformFixedParameters", {4.0, 5.0, 6.0}),
+                                              h5::Node::values("TransformParameters", euler)}),
+                        h5::Node::group("2", {h5::Node::string("TransformType", "TranslationTransform_float_3_3"),
+                                              h5::Node::values("TransformFixedParameters", {}),
+                                              h5::Node::values("TransformParameters", {0.5, -1.5, 2.5})}),
+                        h5::Node::group("10", {h5::Node::string("TransformType", "Euler3DTransform_double_3_3"),
+                                               h5::Node::values("TransformFixedParameters", {4.0, 5.0, 6.0, 1.0}),
+                                               h5::Node::values("TransformParameters", euler)})})});
+    const std::vector<ChainElement> e = read_composite_h5(tmp);
+    CHECK(e.size() == 3 && !e[0].is_field && !e[1].is_field && !e[2].is_field, "euler and translation records");
+    // Rz Rx Ry at (0.1, -0.2, 0.3): the rotation's (0, 2) entry is cz sy + sz sx cy
+    const double cx = std::cos(0.1), sx = std::sin(0.1), cy = std::cos(-0.2), sy = std::sin(-0.2);
+    const double cz = std::cos(0.3), sz = std::sin(0.3);
+    CHECK(std::fabs(e[0].affine.matrix[0][2] - (cz * sy + sz * sx * cy)) < 1e-15 && e[0].affine.center[2] == 6.0 &&
+              e[0].affine.translation[1] == 2.0,
+          "euler3d matrix, centre and translation");
+    CHECK(e[1].affine.translation[0] == 0.5 && e[1].affine.translation[2] == 2.5 && e[1].affine.matrix[1][1] == 1.0,
+          "translation record");
+    // member "10" sorts after "2" (numeric order); with the ZYX flag the matrix is Rz Ry Rx: (0, 2) = cz sy cx + sz sx
+    CHECK(std::fabs(e[2].affine.matrix[0][2] - (cz * sy * cx + sz * sx)) < 1e-15, "euler3d zyx and numeric member order");
+  }
   CHECK(refused({h5::Node::group("1", {h5::Node::string("TransformType", "BSplineTransform_double_3_3"),
                                        h5::Node::values("TransformFixedParameters", {0.0}),
                                        h5::Node::values("TransformParameters", {0.0})})}),
