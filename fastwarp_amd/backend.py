@@ -119,6 +119,7 @@ _SIGNATURES = [
     ("fw_write_composite_h5", C.c_int, [C.c_char_p, C.POINTER(ChainElement), C.c_uint32]),
     ("fw_read_composite_h5", C.c_int, [C.c_char_p, C.POINTER(ChainElement), C.c_uint32, C.POINTER(C.c_uint32)]),
     ("fw_write_composite_pair", C.c_int, [C.c_char_p, C.POINTER(Affine), C.POINTER(Grid), _D, _D]),
+    ("fw_write_composite_fields", C.c_int, [_P, C.c_char_p, C.POINTER(Affine), _P, _P]),
     ("fw_server_create", C.c_int, [C.c_int32, C.c_uint32, C.c_uint64, _PP]),
     ("fw_server_destroy", None, [_P]),
     ("fw_server_last_error", C.c_char_p, []),
@@ -418,6 +419,12 @@ class HipBackend:
 
     def field_grid(self, field: DeviceField) -> Grid:
         return field.grid
+
+    def write_composite_fields(self, prefix, affine: Affine, forward: DeviceField, inverse: DeviceField) -> None:
+        """<prefix>Composite.h5 / <prefix>InverseComposite.h5 from fields resident on this device
+        (fw_write_composite_fields; ants_io.rs:100-141)."""
+        self._check(lib().fw_write_composite_fields(self.ctx, str(prefix).encode(), C.byref(affine), forward.handle,
+                                                    inverse.handle))
 
     def copy_field(self, field: DeviceField) -> DeviceField:
         return self._fld(lib().fw_field_copy, field.handle)

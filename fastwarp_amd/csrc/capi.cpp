@@ -740,6 +740,25 @@ int fw_write_composite_pair(const char* prefix, const fw_affine* affine, const f
   });
 }
 
+int fw_write_composite_fields(fw_ctx* ctx, const char* prefix, const fw_affine* affine, const fw_field* forward,
+                              const fw_field* inverse) {
+  return guard(ctx, [&] {
+    need(prefix, "prefix");
+    need(affine, "affine");
+    need(forward, "forward field");
+    need(inverse, "inverse field");
+    if (!(forward->h.grid == inverse->h.grid)) fw::invalid("ITK HDF5 transform", "the two fields are on different grids");
+    const auto as_f32 = [&](const fw_field* f) {
+      const fw::VectorField v = ctx->backend.download_field(f->h);
+      std::vector<float> out(3 * v.data.size());
+      for (size_t i = 0; i < v.data.size(); ++i)
+        for (int c = 0; c < 3; ++c) out[3 * i + size_t(c)] = float(v.data[i][c]);
+      return out;
+    };
+    fw::write_composite_pair(prefix, affine_from(affine), &forward->h.grid, as_f32(forward), as_f32(inverse));
+  });
+}
+
 void fw_plan_cohort(fw_plan* plan, int32_t channels) {
   if (plan) fw::cohort_plan(plan, channels);
 }

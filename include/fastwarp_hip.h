@@ -455,6 +455,10 @@ int fw_read_composite_h5(const char* path, fw_chain_element* elements, uint32_t 
  * used as given (a prefix ending in '/' names a folder). A singular affine is refused. */
 int fw_write_composite_pair(const char* prefix, const fw_affine* affine, const fw_grid* grid, const double* field,
                             const double* inverse_field);
+/* fw_write_composite_pair for fields resident on the device (fw_register_resident's outputs): both
+ * are downloaded in f32 and written; neither handle is consumed. */
+int fw_write_composite_fields(fw_ctx* ctx, const char* prefix, const fw_affine* affine, const fw_field* forward,
+                              const fw_field* inverse);
 
 /* ---- Serving: continuous batching of whole registrations on one device ------------------------
  * The reference runs several registrations on one GPU as independent device instances, as many

@@ -93,6 +93,13 @@ def test_resident_registration_matches_the_host_path(backend, tmp_path):
     for name, want, got in (("forward", host.forward, dev.forward), ("inverse", host.inverse, dev.inverse)):
         d = np.linalg.norm(backend.download_field(got) - want, axis=1) / spacing
         assert d.mean() <= 0.1 and np.percentile(d, 99) <= 0.5, (name, d.mean(), np.percentile(d, 99))
+    # the ANTs files of the resident result, written from the device fields, hold their f32 values
+    backend.write_composite_fields(tmp_path / "dev_", dev.affine, dev.forward, dev.inverse)
+    forward = read_composite_h5(tmp_path / "dev_Composite.h5")
+    inverse = read_composite_h5(tmp_path / "dev_InverseComposite.h5")
+    assert list(forward[0].matrix) == list(dev.affine.matrix)
+    assert np.array_equal(forward[1][1], backend.download_field(dev.forward))
+    assert np.array_equal(inverse[0][1], backend.download_field(dev.inverse))
     # the input handles are not consumed: a second run gives the same result
     again = register_resident(backend, plan, imgs_f, imgs_m, fields=False)
     assert np.array_equal(np.array(list(again.affine.matrix)), m_d)
