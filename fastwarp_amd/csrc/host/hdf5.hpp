@@ -270,13 +270,13 @@ class Writer {
           put(raw, bits, 8);
         }
         layout.bytes(std::move(raw));
-        return dataset(dataspace_1d(node.doubles.size()), f64_type(), data, 8 * node.doubles.size());
+        return dataset(dataspace_1d(node.doubles.size()), f64_type(), data, 8 * node.doubles.size(), 0x0a);
       }
       case Node::Kind::Floats: {
         layout.align8();
         const uint64_t data = layout.size();
         layout.widened(node.floats, node.count);
-        return dataset(dataspace_1d(node.count), f64_type(), data, 8 * uint64_t(node.count));
+        return dataset(dataspace_1d(node.count), f64_type(), data, 8 * uint64_t(node.count), 0x0a);
       }
       case Node::Kind::Text: {
         // one element: its length, the heap collection and the string's object index
@@ -287,7 +287,7 @@ class Writer {
         put(raw, heap_, 8);
         put(raw, ++next_string_, 4);
         layout.bytes(std::move(raw));
-        return dataset(dataspace_1d(1), vlen_string_type(), data, 16);
+        return dataset(dataspace_1d(1), vlen_string_type(), data, 16, 0x02);
       }
     }
     return 0;
@@ -321,13 +321,15 @@ class Writer {
   }
 
  private:
-  uint64_t dataset(std::vector<uint8_t> space, std::vector<uint8_t> type, uint64_t data, uint64_t size) {
+  // The messages and flags libhdf5 writes for a contiguous dataset: dataspace, datatype (constant),
+  // fill value (version 3, constant: space allocated late; a float's fill value written if set,
+  // a string's when allocated) and layout.
+  uint64_t dataset(std::vector<uint8_t> space, std::vector<uint8_t> type, uint64_t data, uint64_t size, uint8_t fill) {
     layout.align8();
     const uint64_t at = layout.size();
-    // fill value (version 3): space allocated late, fill value written if set, none defined
-    layout.bytes(object_header({{0x01, 1, std::move(space)},
+    layout.bytes(object_header({{0x01, 0, std::move(space)},
                                 {0x03, 1, std::move(type)},
-                                {0x05, 1, {3, 0x0a}},
+                                {0x05, 1, {3, fill}},
                                 {0x08, 0, contiguous(data, size)}}));
     return at;
   }
@@ -442,6 +444,15 @@ class File {
   std::vector<std::string> members(const std::string& path) const {
     std::vector<std::string> out;
     for (const auto& l : links(resolve(path))) out.push_back(l.first);
+    return out;
+  }
+
+  // The header messages of the object at `path` in storage order (type and body; continuation
+  // blocks followed), for checks of the format itself.
+  std::vector<std::pair<uint8_t, std::vector<uint8_t>>> header_messages(const std::string& path) const {
+    std::vector<std::pair<uint8_t, std::vector<uint8_t>>> out;
+    for (const Msg& m : messages(resolve(path)))
+      out.emplace_back(m.type, std::vector<uint8_t>(d_.begin() + std::ptrdiff_t(m.at), d_.begin() + std::ptrdiff_t(m.at + m.size)));
     return out;
   }
 

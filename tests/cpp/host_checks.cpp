@@ -221,6 +221,24 @@ static void itk_composite() {
   CHECK(w.read_string("TransformGroup/0/TransformType") == "CompositeTransform_double_3_3" &&
             w.read_string("TransformGroup/2/TransformType") == "DisplacementFieldTransform_double_3_3",
         "written type names");
+  // the writer's dataset messages are the bytes libhdf5 1.12.1 wrote into the fixture for the same
+  // kinds of dataset: dataspace, datatype (IEEE f64; variable-length string) and fill value bodies
+  // equal, layout version 3 contiguous; for a string, a 12-value and an 18-value f64 dataset
+  const h5::File f = h5::File::open(golden);
+  const auto body = [](const h5::File& file, const std::string& path, uint8_t type) {
+    for (const auto& m : file.header_messages(path))
+      if (m.first == type) return m.second;
+    return std::vector<uint8_t>{};
+  };
+  for (const char* ds : {"TransformGroup/1/TransformType", "TransformGroup/1/TransformParameters",
+                         "TransformGroup/2/TransformFixedParameters"}) {
+    for (uint8_t type : {uint8_t(0x01), uint8_t(0x03), uint8_t(0x05)})
+      CHECK(!body(f, ds, type).empty() && body(f, ds, type) == body(w, ds, type), "message bodies as libhdf5 writes them");
+    const std::vector<uint8_t> lf = body(f, ds, 0x08), lw = body(w, ds, 0x08);
+    CHECK(lf.size() == 18 && lw.size() == 18 && lf[0] == 3 && lw[0] == 3 && lf[1] == 1 && lw[1] == 1 &&
+              std::equal(lf.begin() + 10, lf.end(), lw.begin() + 10),
+          "contiguous layout of the same size");
+  }
   const std::vector<double> fixed = w.read_doubles("TransformGroup/2/TransformFixedParameters");
   CHECK(fixed.size() == 18 && fixed[0] == 5.0 && fixed[3] == -4.0 && fixed[6] == 2.0 && fixed[9 + 1] == -1.0,
         "field fixed parameters: size, origin, spacing, direction row-major");
